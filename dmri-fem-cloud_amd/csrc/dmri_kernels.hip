@@ -1,0 +1,1032 @@
+// gfx950 kernels of libdmri: device set-up (incidence, DOF map, facets, CSR pattern,
+// element colouring, closed-form P1 assembly) and the FP64 theta-scheme / BiCGStab solve.
+//
+// Wave64 throughout.  Every reduction is a fixed-shape tree (wave butterfly, then the
+// waves of a block in index order, then the blocks of a grid in index order by the last
+// block to finish), so results are bitwise reproducible run to run.
+#include "dmri_internal.hpp"
+
+namespace dmri {
+
+namespace {
+
+// --------------------------------------------------------------------------------------
+// small helpers
+// --------------------------------------------------------------------------------------
+__device__ __forceinline__ double2 c_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 c_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 c_mul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 c_div(double2 a, double2 b) {
+    double d = b.x * b.x + b.y * b.y;
+    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+// y = a + i*s*j  for complex accumulators a = sum(va x), j = sum(vj x)
+__device__ __forceinline__ double2 c_axpj(double2 a, double s, double2 j) {
+    return make_double2(a.x - s * j.y, a.y + s * j.x);
+}
+
+template <int NV>
+__device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
+    }
+}
+
+// Block reduction; the result is valid in thread 0.  Fixed order.
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV]) {
+    __shared__ double sh[NV][kBlock / 64];
+    wave_reduce<NV>(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[k][i];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Grid reduction by the last block to arrive.  Returns true in the last block; the
+// total is then valid in its thread 0.  Block partials are summed in block order.
+template <int NV>
+__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *part, unsigned *counter) {
+    __shared__ int s_last;
+    block_reduce<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            __hip_atomic_store(&part[(size_t)blockIdx.x * NV + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            acc[k] += __hip_atomic_load(&part[(size_t)b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_reduce<NV>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = acc[k];
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
+}
+
+inline unsigned grid_for(int64_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
+
+// --------------------------------------------------------------------------------------
+// exclusive scan (int32, non-negative counts)
+// --------------------------------------------------------------------------------------
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__global__ void __launch_bounds__(kScanBlock) scan_tile_kernel(const int *in, int *out, int *tile_sums, int64_t n) {
+    __shared__ int s[kScanBlock];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int vals[kScanItems];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        int64_t i = base + k;
+        vals[k] = i < n ? in[i] : 0;
+        sum += vals[k];
+    }
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        int t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int run = s[threadIdx.x] - sum;
+    if (threadIdx.x == kScanBlock - 1) tile_sums[blockIdx.x] = s[threadIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        int64_t i = base + k;
+        if (i < n) out[i] = run;
+        run += vals[k];
+    }
+}
+
+__global__ void __launch_bounds__(kScanBlock) scan_add_kernel(int *out, const int *tile_offs, int64_t n) {
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    const int off = tile_offs[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        int64_t i = base + k;
+        if (i < n) out[i] += off;
+    }
+}
+
+__global__ void set_total_kernel(int *out, int64_t n, const int *in, const int *tile_offs, int ntiles) {
+    // out[n] = total = exclusive[n-1] + in[n-1]
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = n > 0 ? out[n - 1] + in[n - 1] : 0;
+}
+
+// --------------------------------------------------------------------------------------
+// incidence lists (vertex -> cells, vertex -> interface facets)
+// --------------------------------------------------------------------------------------
+template <int NVE>
+__global__ void count_incidence_kernel(const int *ev, int64_t ne, int *cnt) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+#pragma unroll
+    for (int a = 0; a < NVE; ++a) atomicAdd(&cnt[ev[NVE * e + a]], 1);
+}
+
+template <int NVE>
+__global__ void fill_incidence_kernel(const int *ev, int64_t ne, const int *ptr, int *cursor, int *list) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+#pragma unroll
+    for (int a = 0; a < NVE; ++a) {
+        int v = ev[NVE * e + a];
+        int pos = atomicAdd(&cursor[v], 1);
+        list[ptr[v] + pos] = (int)e;
+    }
+}
+
+// Sort each list (insertion sort; lists are short) so that the lists, and everything
+// derived from their order, are deterministic.
+__global__ void sort_lists_kernel(const int *ptr, int *list, int64_t n) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    int b = ptr[v], e = ptr[v + 1];
+    for (int i = b + 1; i < e; ++i) {
+        int x = list[i];
+        int j = i - 1;
+        while (j >= b && list[j] > x) {
+            list[j + 1] = list[j];
+            --j;
+        }
+        list[j + 1] = x;
+    }
+}
+
+// --------------------------------------------------------------------------------------
+// DOF map: live (compartment, vertex) pairs numbered compartment-major
+// --------------------------------------------------------------------------------------
+__global__ void mark_live_kernel(const int *tets, const int *phase, int64_t nc, int64_t nv, int *live) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc) return;
+    int c = phase ? phase[t] : 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) live[(int64_t)c * nv + tets[4 * t + a]] = 1;
+}
+
+__global__ void dof_from_scan_kernel(const int *live, const int *scan, int64_t n, int *dof_of, int64_t nv,
+                                     int *dof_vert, int *dof_comp) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (live[i]) {
+        int d = scan[i];
+        dof_of[i] = d;
+        dof_vert[d] = (int)(i % nv);
+        dof_comp[d] = (int)(i / nv);
+    } else {
+        dof_of[i] = -1;
+    }
+}
+
+// --------------------------------------------------------------------------------------
+// faces: neighbour across each face, interface (phase 0 | phase 1) and boundary facets
+// --------------------------------------------------------------------------------------
+__device__ __forceinline__ void face_verts(const int *T, int o, int &f0, int &f1, int &f2) {
+    int v[3], k = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        if (a != o) v[k++] = T[a];
+    // ascending
+    int a0 = min(v[0], min(v[1], v[2]));
+    int a2 = max(v[0], max(v[1], v[2]));
+    int a1 = v[0] + v[1] + v[2] - a0 - a2;
+    f0 = a0;
+    f1 = a1;
+    f2 = a2;
+}
+
+__global__ void face_classify_kernel(const int *tets, const int *phase, int64_t nc, const int *vptr,
+                                     const int *vlist, int *iface_flag, int *bnd_flag, int *nbr) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 4 * nc) return;
+    int64_t t = idx >> 2;
+    int o = (int)(idx & 3);
+    int T[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) T[a] = tets[4 * t + a];
+    int f0, f1, f2;
+    face_verts(T, o, f0, f1, f2);
+    // scan the shortest of the three incidence lists
+    int best = f0, bl = vptr[f0 + 1] - vptr[f0];
+    int l1 = vptr[f1 + 1] - vptr[f1], l2 = vptr[f2 + 1] - vptr[f2];
+    if (l1 < bl) { best = f1; bl = l1; }
+    if (l2 < bl) { best = f2; bl = l2; }
+    int found = -1;
+    for (int p = vptr[best]; p < vptr[best + 1]; ++p) {
+        int u = vlist[p];
+        if (u == (int)t) continue;
+        int hit = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            int w = tets[4 * (int64_t)u + a];
+            hit += (w == f0) + (w == f1) + (w == f2);
+        }
+        if (hit == 3) { found = u; break; }
+    }
+    nbr[idx] = found;
+    int pt = phase ? phase[t] : 0;
+    iface_flag[idx] = (found >= 0 && phase && pt == 0 && phase[found] == 1) ? 1 : 0;
+    bnd_flag[idx] = found < 0 ? 1 : 0;
+}
+
+__global__ void face_compact_kernel(const int *tets, int64_t nc, const int *nbr, const int *iface_flag,
+                                    const int *iface_scan, const int *bnd_flag, const int *bnd_scan, int *iface_v,
+                                    int *iface_c, int *bnd_v, int *bnd_c) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 4 * nc) return;
+    int64_t t = idx >> 2;
+    int o = (int)(idx & 3);
+    if (!iface_flag[idx] && !bnd_flag[idx]) return;
+    int T[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) T[a] = tets[4 * t + a];
+    int f0, f1, f2;
+    face_verts(T, o, f0, f1, f2);
+    if (iface_flag[idx]) {
+        int j = iface_scan[idx];
+        iface_v[3 * j] = f0;
+        iface_v[3 * j + 1] = f1;
+        iface_v[3 * j + 2] = f2;
+        iface_c[2 * j] = (int)t;
+        iface_c[2 * j + 1] = nbr[idx];
+    }
+    if (bnd_flag[idx]) {
+        int j = bnd_scan[idx];
+        bnd_v[3 * j] = f0;
+        bnd_v[3 * j + 1] = f1;
+        bnd_v[3 * j + 2] = f2;
+        bnd_c[j] = (int)t;
+    }
+}
+
+// --------------------------------------------------------------------------------------
+// CSR pattern: one wave per DOF row.  The candidate columns of row (c, v) are the DOFs
+// of every cell of phase c touching v and of every interface facet touching v (both
+// compartments).  Sorted unique set, computed by ranking in LDS (no atomics), so the
+// result equals the brute-force set union exactly.
+// --------------------------------------------------------------------------------------
+constexpr int kPatCap = 2048;
+
+__global__ void __launch_bounds__(64) pattern_kernel(int pass, const int *dof_vert, const int *dof_comp,
+                                                     const int *tets, const int *phase, int64_t nv,
+                                                     const int *vptr, const int *vlist, const int *fptr,
+                                                     const int *flist, const int *iface_v, const int *dof_of,
+                                                     int *row_count, const int *rowptr, int *col, int *overflow) {
+    __shared__ int cand[kPatCap];
+    __shared__ unsigned char first[kPatCap];
+    const int d = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int v = dof_vert[d], c = dof_comp[d];
+    const int c0 = vptr[v], ncell = vptr[v + 1] - c0;
+    const int f0 = fptr ? fptr[v] : 0, nface = fptr ? fptr[v + 1] - f0 : 0;
+    const int m = 4 * ncell + 6 * nface;
+    if (m > kPatCap) {
+        if (lane == 0) atomicOr(overflow, 1);
+        return;
+    }
+    for (int i = lane; i < ncell; i += 64) {
+        int t = vlist[c0 + i];
+        bool ok = (phase ? phase[t] : 0) == c;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) cand[4 * i + a] = ok ? dof_of[(int64_t)c * nv + tets[4 * (int64_t)t + a]] : -1;
+    }
+    for (int i = lane; i < nface; i += 64) {
+        int f = flist[f0 + i];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            int w = iface_v[3 * (int64_t)f + a];
+            cand[4 * ncell + 6 * i + a] = dof_of[w];
+            cand[4 * ncell + 6 * i + 3 + a] = dof_of[nv + w];
+        }
+    }
+    __syncthreads();
+    for (int i = lane; i < m; i += 64) {
+        int ci = cand[i];
+        unsigned char fl = ci >= 0;
+        for (int j = 0; j < i && fl; ++j)
+            if (cand[j] == ci) fl = 0;
+        first[i] = fl;
+    }
+    __syncthreads();
+    if (pass == 0) {
+        int cnt = 0;
+        for (int i = lane; i < m; i += 64) cnt += first[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+        if (lane == 0) row_count[d] = cnt;
+    } else {
+        const int base = rowptr[d];
+        for (int i = lane; i < m; i += 64) {
+            if (!first[i]) continue;
+            int ci = cand[i], rank = 0;
+            for (int j = 0; j < m; ++j) rank += (first[j] && cand[j] < ci);
+            col[base + rank] = ci;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------------------
+// Jones-Plassmann colouring of elements (cells or facets) that share a vertex.
+// An uncoloured element takes colour k when it beats every neighbour that is uncoloured
+// or coloured k in this round.  Priorities are a fixed hash, so the colouring is
+// deterministic.
+// --------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned hash32(unsigned x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ bool beats(int a, int b) {
+    unsigned ha = hash32((unsigned)a), hb = hash32((unsigned)b);
+    return ha > hb || (ha == hb && a > b);
+}
+
+__global__ void jp_round_kernel(int k, int64_t ne, int nvert, const int *ev, const int *vptr, const int *vlist,
+                                int *color, int *remaining) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool left = false;
+    if (e < ne && color[e] < 0) {
+        bool win = true;
+        for (int a = 0; a < nvert && win; ++a) {
+            int v = ev[(int64_t)nvert * e + a];
+            for (int p = vptr[v]; p < vptr[v + 1]; ++p) {
+                int u = vlist[p];
+                if (u == (int)e) continue;
+                int cu = color[u];
+                if ((cu < 0 || cu == k) && beats(u, (int)e)) { win = false; break; }
+            }
+        }
+        if (win) color[e] = k;
+        else left = true;
+    }
+    unsigned long long b = __ballot(left);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(remaining, __popcll(b));
+}
+
+__global__ void color_hist_kernel(const int *color, int64_t ne, int *hist) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < ne) atomicAdd(&hist[color[e]], 1);
+}
+
+__global__ void color_scatter_kernel(const int *color, int64_t ne, const int *offs, int *cursor, int *perm) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    int c = color[e];
+    int pos = atomicAdd(&cursor[c], 1);
+    perm[offs[c] + pos] = (int)e;
+}
+
+// structure-of-arrays copy of the cells in colour order (coalesced vertex-index reads)
+__global__ void gather_cells_kernel(const int *perm, int64_t ne, const int *tets, const int *phase, int *ct,
+                                    int *cph) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne) return;
+    int t = perm[i];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) ct[(int64_t)a * ne + i] = tets[4 * (int64_t)t + a];
+    cph[i] = phase ? phase[t] : 0;
+}
+
+// --------------------------------------------------------------------------------------
+// closed-form P1 element assembly (one thread per element of one colour; no two elements
+// of a colour share a vertex, so the read-modify-write of the CSR values is race free and
+// every entry receives its contributions in colour order).
+// --------------------------------------------------------------------------------------
+__device__ __forceinline__ int csr_find(const int *rowptr, const int *col, int r, int c) {
+    int lo = rowptr[r], hi = rowptr[r + 1] - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        int cm = col[mid];
+        if (cm == c) return mid;
+        if (cm < c) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return -1;
+}
+
+__global__ void assemble_cells_kernel(int64_t start, int64_t count, int64_t ne, const int *perm, const int *ct,
+                                      const int *cph, const double *xyz, int64_t nv, const int *dof_of,
+                                      const int *rowptr, const int *col, const double *Dc, const double *Dcell,
+                                      const double *iT2c, const double *iT2cell, double *mv, double *kv, double *rv,
+                                      double *jxv, double *jyv, double *jzv, int *miss) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    const int64_t i = start + g;
+    const int t = perm[i];
+    const int c = cph[i];
+    int vid[4];
+    double X[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        vid[a] = ct[(int64_t)a * ne + i];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) X[a][q] = xyz[3 * (int64_t)vid[a] + q];
+    }
+    double e[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) e[a][q] = X[a + 1][q] - X[0][q];
+    const double c23[3] = {e[1][1] * e[2][2] - e[1][2] * e[2][1], e[1][2] * e[2][0] - e[1][0] * e[2][2],
+                           e[1][0] * e[2][1] - e[1][1] * e[2][0]};
+    const double c31[3] = {e[2][1] * e[0][2] - e[2][2] * e[0][1], e[2][2] * e[0][0] - e[2][0] * e[0][2],
+                           e[2][0] * e[0][1] - e[2][1] * e[0][0]};
+    const double c12[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                           e[0][0] * e[1][1] - e[0][1] * e[1][0]};
+    const double det = e[0][0] * c23[0] + e[0][1] * c23[1] + e[0][2] * c23[2];
+    double G[4][3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        G[1][q] = c23[q] / det;
+        G[2][q] = c31[q] / det;
+        G[3][q] = c12[q] / det;
+        G[0][q] = -(G[1][q] + G[2][q] + G[3][q]);
+    }
+    const double vol = fabs(det) / 6.0;
+    double D[9];
+    if (Dcell) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) D[q] = Dcell[9 * (int64_t)t + q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) D[q] = 0.0;
+        D[0] = D[4] = D[8] = Dc[c];
+    }
+    const double it2 = iT2cell ? iT2cell[t] : iT2c[c];
+    double S[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) S[q] += X[a][q];
+    int d[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) d[a] = dof_of[(int64_t)c * nv + vid[a]];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double DG[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) DG[q] = D[3 * q] * G[a][0] + D[3 * q + 1] * G[a][1] + D[3 * q + 2] * G[a][2];
+        for (int b = 0; b < 4; ++b) {
+            int pos = csr_find(rowptr, col, d[a], d[b]);
+            if (pos < 0) {
+                atomicOr(miss, 1);
+                continue;
+            }
+            const double dab = (a == b) ? 2.0 : 1.0;
+            const double mab = vol * dab / 20.0;
+            mv[pos] += mab;
+            rv[pos] += it2 * mab;
+            kv[pos] += vol * (G[b][0] * DG[0] + G[b][1] * DG[1] + G[b][2] * DG[2]);
+            const double w = vol * dab / 120.0;
+            jxv[pos] += w * (S[0] + X[a][0] + X[b][0]);
+            jyv[pos] += w * (S[1] + X[a][1] + X[b][1]);
+            jzv[pos] += w * (S[2] + X[a][2] + X[b][2]);
+        }
+    }
+}
+
+__global__ void assemble_facets_kernel(int64_t start, int64_t count, const int *perm, const int *iface_v,
+                                       const double *xyz, int64_t nv, const int *dof_of, const int *rowptr,
+                                       const int *col, double kappa, const double *kappa_f, double *gi, int *miss) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    const int f = perm[start + g];
+    double X[3][3];
+    int d0[3], d1[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int v = iface_v[3 * (int64_t)f + a];
+        d0[a] = dof_of[v];
+        d1[a] = dof_of[nv + v];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) X[a][q] = xyz[3 * (int64_t)v + q];
+    }
+    double e1[3], e2[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        e1[q] = X[1][q] - X[0][q];
+        e2[q] = X[2][q] - X[0][q];
+    }
+    const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                 cz = e1[0] * e2[1] - e1[1] * e2[0];
+    const double area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+    const double kap = kappa_f ? kappa_f[f] : kappa;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            const double val = kap * area * ((a == b) ? 2.0 : 1.0) / 12.0;
+            int p00 = csr_find(rowptr, col, d0[a], d0[b]);
+            int p11 = csr_find(rowptr, col, d1[a], d1[b]);
+            int p01 = csr_find(rowptr, col, d0[a], d1[b]);
+            int p10 = csr_find(rowptr, col, d1[a], d0[b]);
+            if (p00 < 0 || p11 < 0 || p01 < 0 || p10 < 0) {
+                atomicOr(miss, 1);
+                continue;
+            }
+            gi[p00] += val;
+            gi[p11] += val;
+            gi[p01] -= val;
+            gi[p10] -= val;
+        }
+}
+
+__global__ void diag_weights_kernel(int64_t n, const int *rowptr, const int *col, const double *m, int *diag,
+                                    double *w, int *missing) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int p = csr_find(rowptr, col, (int)i, (int)i);
+    diag[i] = p;
+    if (p < 0) atomicOr(missing, 1);
+    double s = 0.0;
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) s += m[e];
+    w[i] = s;
+}
+
+__global__ void __launch_bounds__(kBlock) sum_kernel(const double *x, int64_t n, double *part, unsigned *counter,
+                                                     double *out) {
+    double v[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[0] += x[i];
+    if (grid_reduce<1>(v, part, counter) && threadIdx.x == 0) out[0] = v[0];
+}
+
+// --------------------------------------------------------------------------------------
+// Solve-time preparation.  Jacobi preconditioning is applied as the symmetric scaling
+// S = diag(a)^(-1/2) of the whole system (a = M/dt + theta*L): the solver works on
+// S A S, whose real diagonal is 1, with x = S^-1 u.
+// --------------------------------------------------------------------------------------
+__global__ void prep_scale_kernel(int64_t n, const int *diag, const double *m, const double *k, const double *r,
+                                  const double *gi, double dt, double theta, double *sc) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int p = diag[i];
+    double a = m[p] / dt + theta * (k[p] + r[p] + gi[p]);
+    sc[i] = 1.0 / sqrt(a);
+}
+
+__global__ void prep_values_kernel(int64_t n, const int *rowptr, const int *col, const double *sc, const double *m,
+                                   const double *k, const double *r, const double *gi, const double *jx,
+                                   const double *jy, const double *jz, double gx, double gy, double gz, double dt,
+                                   double theta, double2 *aj, double *bre) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double si = sc[i];
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+        const double s = si * sc[col[e]];
+        const double l = k[e] + r[e] + gi[e];
+        const double mk = m[e] / dt;
+        const double jg = gx * jx[e] + gy * jy[e] + gz * jz[e];
+        aj[e] = make_double2(s * (mk + theta * l), s * jg);
+        bre[e] = s * (mk - (1.0 - theta) * l);
+    }
+}
+
+__global__ void init_x_kernel(int64_t n, const double *sc, double2 *x) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = make_double2(1.0 / sc[i], 0.0);
+}
+
+__global__ void unscale_kernel(int64_t n, const double *sc, const double2 *x, double2 *u) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] = make_double2(sc[i] * x[i].x, sc[i] * x[i].y);
+}
+
+// --------------------------------------------------------------------------------------
+// Solver kernels.  kRowGroup lanes per row; lane l of a group handles entries l, l+G, ...
+// of the row and the group is reduced with a butterfly.
+// --------------------------------------------------------------------------------------
+__device__ __forceinline__ void group_reduce2(double2 &a) {
+#pragma unroll
+    for (int off = kRowGroup / 2; off > 0; off >>= 1) {
+        a.x += __shfl_xor(a.x, off, 64);
+        a.y += __shfl_xor(a.y, off, 64);
+    }
+}
+
+// rhs + initial residual: b = (Bre + i sB J) x, r = b - (Are + i sA J) x; r^ = r.
+__global__ void __launch_bounds__(kBlock) rhs_residual_kernel(int64_t n, const int *rowptr, const int *col,
+                                                              const double2 *aj, const double *bre, double sA,
+                                                              double sB, const double2 *x, double2 *r, double2 *rh,
+                                                              double rtol, double atol, int maxit, double *part,
+                                                              KrylovState *st) {
+    const int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + (threadIdx.x / kRowGroup);
+    const int l = threadIdx.x % kRowGroup;
+    double red[2] = {0.0, 0.0};
+    if (row < n) {
+        double2 sa = make_double2(0, 0), sj = make_double2(0, 0), sb = make_double2(0, 0);
+        for (int e = rowptr[row] + l; e < rowptr[row + 1]; e += kRowGroup) {
+            const double2 v = aj[e];
+            const double bv = bre[e];
+            const double2 xv = x[col[e]];
+            sa.x += v.x * xv.x; sa.y += v.x * xv.y;
+            sj.x += v.y * xv.x; sj.y += v.y * xv.y;
+            sb.x += bv * xv.x;  sb.y += bv * xv.y;
+        }
+        group_reduce2(sa);
+        group_reduce2(sj);
+        group_reduce2(sb);
+        if (l == 0) {
+            const double2 b = c_axpj(sb, sB, sj);
+            const double2 ax = c_axpj(sa, sA, sj);
+            const double2 rr = c_sub(b, ax);
+            r[row] = rr;
+            rh[row] = rr;
+            red[0] = b.x * b.x + b.y * b.y;
+            red[1] = rr.x * rr.x + rr.y * rr.y;
+        }
+    }
+    if (grid_reduce<2>(red, part, st->counter + 0) && threadIdx.x == 0) {
+        st->bnorm2 = red[0];
+        st->rnorm2 = red[1];
+        st->rho = make_double2(red[1], 0.0);
+        st->rho_old = make_double2(1.0, 0.0);
+        st->alpha = make_double2(1.0, 0.0);
+        st->omega = make_double2(1.0, 0.0);
+        st->beta = make_double2(0.0, 0.0);
+        const double t1 = rtol * rtol * red[0], t2 = atol * atol;
+        st->tol2 = t1 > t2 ? t1 : t2;
+        st->iter = 0;
+        st->maxit = maxit;
+        st->breakdown = 0;
+        st->done = red[1] <= st->tol2 ? 1 : 0;
+    }
+}
+
+// p = r (first iteration) or p = r + beta (p - omega v)
+__global__ void __launch_bounds__(kBlock) bicg_p_kernel(int64_t n, const double2 *r, double2 *p, const double2 *v,
+                                                        const KrylovState *st) {
+    if (st->done) return;
+    const bool first = st->iter == 0;
+    const double2 beta = st->beta, omega = st->omega;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 ri = r[i];
+        if (first) {
+            p[i] = ri;
+        } else {
+            const double2 w = c_sub(p[i], c_mul(omega, v[i]));
+            p[i] = c_add(ri, c_mul(beta, w));
+        }
+    }
+}
+
+// v = A p ; alpha = rho / <r^, v>
+__global__ void __launch_bounds__(kBlock) spmv_v_kernel(int64_t n, const int *rowptr, const int *col,
+                                                        const double2 *aj, double sA, const double2 *p, double2 *v,
+                                                        const double2 *rh, double *part, KrylovState *st) {
+    if (st->done) return;
+    const int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + (threadIdx.x / kRowGroup);
+    const int l = threadIdx.x % kRowGroup;
+    double red[2] = {0.0, 0.0};
+    if (row < n) {
+        double2 sa = make_double2(0, 0), sj = make_double2(0, 0);
+        for (int e = rowptr[row] + l; e < rowptr[row + 1]; e += kRowGroup) {
+            const double2 a = aj[e];
+            const double2 xv = p[col[e]];
+            sa.x += a.x * xv.x; sa.y += a.x * xv.y;
+            sj.x += a.y * xv.x; sj.y += a.y * xv.y;
+        }
+        group_reduce2(sa);
+        group_reduce2(sj);
+        if (l == 0) {
+            const double2 y = c_axpj(sa, sA, sj);
+            v[row] = y;
+            const double2 h = rh[row];
+            red[0] = h.x * y.x + h.y * y.y;   // conj(h) * y
+            red[1] = h.x * y.y - h.y * y.x;
+        }
+    }
+    if (grid_reduce<2>(red, part, st->counter + 1) && threadIdx.x == 0) {
+        const double2 rv = make_double2(red[0], red[1]);
+        if (rv.x == 0.0 && rv.y == 0.0) {
+            st->breakdown = 1;
+            st->done = 1;
+        } else {
+            st->alpha = c_div(st->rho, rv);
+        }
+    }
+}
+
+// s = r - alpha v (on the fly), t = A s ; omega = <t, s> / <t, t>
+__global__ void __launch_bounds__(kBlock) spmv_t_kernel(int64_t n, const int *rowptr, const int *col,
+                                                        const double2 *aj, double sA, const double2 *r,
+                                                        const double2 *v, double2 *t, double *part,
+                                                        KrylovState *st) {
+    if (st->done) return;
+    const double2 alpha = st->alpha;
+    const int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + (threadIdx.x / kRowGroup);
+    const int l = threadIdx.x % kRowGroup;
+    double red[4] = {0.0, 0.0, 0.0, 0.0};
+    if (row < n) {
+        double2 sa = make_double2(0, 0), sj = make_double2(0, 0);
+        for (int e = rowptr[row] + l; e < rowptr[row + 1]; e += kRowGroup) {
+            const double2 a = aj[e];
+            const int c = col[e];
+            const double2 xv = c_sub(r[c], c_mul(alpha, v[c]));
+            sa.x += a.x * xv.x; sa.y += a.x * xv.y;
+            sj.x += a.y * xv.x; sj.y += a.y * xv.y;
+        }
+        group_reduce2(sa);
+        group_reduce2(sj);
+        if (l == 0) {
+            const double2 y = c_axpj(sa, sA, sj);
+            t[row] = y;
+            const double2 s = c_sub(r[row], c_mul(alpha, v[row]));
+            red[0] = y.x * s.x + y.y * s.y;  // conj(t) * s
+            red[1] = y.x * s.y - y.y * s.x;
+            red[2] = y.x * y.x + y.y * y.y;
+            red[3] = s.x * s.x + s.y * s.y;
+        }
+    }
+    if (grid_reduce<4>(red, part, st->counter + 2) && threadIdx.x == 0) {
+        st->snorm2 = red[3];
+        if (red[3] <= st->tol2 || red[2] == 0.0) {
+            st->omega = make_double2(0.0, 0.0);  // x += alpha p, r = s
+        } else {
+            st->omega = make_double2(red[0] / red[2], red[1] / red[2]);
+        }
+    }
+}
+
+// x += alpha p + omega s ; r = s - omega t ; rho = <r^, r> ; beta for the next iteration
+__global__ void __launch_bounds__(kBlock) bicg_update_kernel(int64_t n, double2 *x, const double2 *p, double2 *r,
+                                                             const double2 *v, const double2 *t, const double2 *rh,
+                                                             double *part, KrylovState *st) {
+    if (st->done) return;
+    const double2 alpha = st->alpha, omega = st->omega;
+    double red[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 s = c_sub(r[i], c_mul(alpha, v[i]));
+        x[i] = c_add(x[i], c_add(c_mul(alpha, p[i]), c_mul(omega, s)));
+        const double2 rn = c_sub(s, c_mul(omega, t[i]));
+        r[i] = rn;
+        const double2 h = rh[i];
+        red[0] += h.x * rn.x + h.y * rn.y;
+        red[1] += h.x * rn.y - h.y * rn.x;
+        red[2] += rn.x * rn.x + rn.y * rn.y;
+    }
+    if (grid_reduce<3>(red, part, st->counter + 3) && threadIdx.x == 0) {
+        const double2 rho = make_double2(red[0], red[1]);
+        st->rho_old = st->rho;
+        st->rho = rho;
+        st->rnorm2 = red[2];
+        st->iter += 1;
+        const bool conv = red[2] <= st->tol2;
+        if (conv) {
+            st->done = 1;
+        } else if ((omega.x == 0.0 && omega.y == 0.0) || (rho.x == 0.0 && rho.y == 0.0)) {
+            st->breakdown = 1;
+            st->done = 1;
+        } else {
+            st->beta = c_mul(c_div(rho, st->rho_old), c_div(alpha, omega));
+            if (st->iter >= st->maxit) st->done = 1;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) signal_kernel(int64_t n, const double *w, const double *sc,
+                                                        const double2 *x, double *part, unsigned *counter,
+                                                        double *out) {
+    double red[2] = {0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double ws = w[i] * sc[i];
+        const double2 xi = x[i];
+        red[0] += ws * xi.x;
+        red[1] += ws * xi.y;
+    }
+    if (grid_reduce<2>(red, part, counter) && threadIdx.x == 0) {
+        out[0] = red[0];
+        out[1] = red[1];
+    }
+}
+
+}  // namespace
+
+// ======================================================================================
+// launchers
+// ======================================================================================
+static inline void check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(-2, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+void exclusive_scan(const int *d_in, int *d_out, int64_t n, hipStream_t s) {
+    if (n <= 0) {
+        DMRI_HIP(hipMemsetAsync(d_out, 0, sizeof(int), s));
+        return;
+    }
+    const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    int *tile_sums = nullptr, *tile_offs = nullptr;
+    DMRI_HIP(hipMallocAsync((void **)&tile_sums, sizeof(int) * ntiles, s));
+    DMRI_HIP(hipMallocAsync((void **)&tile_offs, sizeof(int) * (ntiles + 1), s));
+    scan_tile_kernel<<<(unsigned)ntiles, kScanBlock, 0, s>>>(d_in, d_out, tile_sums, n);
+    check_launch();
+    if (ntiles > 1) {
+        exclusive_scan(tile_sums, tile_offs, ntiles, s);
+        scan_add_kernel<<<(unsigned)ntiles, kScanBlock, 0, s>>>(d_out, tile_offs, n);
+        check_launch();
+    }
+    set_total_kernel<<<1, 1, 0, s>>>(d_out, n, d_in, tile_offs, (int)ntiles);
+    check_launch();
+    DMRI_HIP(hipFreeAsync(tile_sums, s));
+    DMRI_HIP(hipFreeAsync(tile_offs, s));
+}
+
+void launch_count_incidence(const int *tets, int64_t nc, int *cnt, hipStream_t s) {
+    if (nc <= 0) return;
+    count_incidence_kernel<4><<<grid_for(nc, 256), 256, 0, s>>>(tets, nc, cnt);
+    check_launch();
+}
+void launch_fill_incidence(const int *tets, int64_t nc, const int *ptr, int *cursor, int *list, hipStream_t s) {
+    if (nc <= 0) return;
+    fill_incidence_kernel<4><<<grid_for(nc, 256), 256, 0, s>>>(tets, nc, ptr, cursor, list);
+    check_launch();
+}
+void launch_count_incidence3(const int *fv, int64_t nf, int *cnt, hipStream_t s) {
+    if (nf <= 0) return;
+    count_incidence_kernel<3><<<grid_for(nf, 256), 256, 0, s>>>(fv, nf, cnt);
+    check_launch();
+}
+void launch_fill_incidence3(const int *fv, int64_t nf, const int *ptr, int *cursor, int *list, hipStream_t s) {
+    if (nf <= 0) return;
+    fill_incidence_kernel<3><<<grid_for(nf, 256), 256, 0, s>>>(fv, nf, ptr, cursor, list);
+    check_launch();
+}
+void launch_sort_lists(const int *ptr, int *list, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    sort_lists_kernel<<<grid_for(n, 256), 256, 0, s>>>(ptr, list, n);
+    check_launch();
+}
+void launch_mark_live(const int *tets, const int *phase, int64_t nc, int64_t nv, int *live, hipStream_t s) {
+    mark_live_kernel<<<grid_for(nc, 256), 256, 0, s>>>(tets, phase, nc, nv, live);
+    check_launch();
+}
+void launch_dof_from_scan(const int *live, const int *scan, int64_t n, int *dof_of, int64_t nv, int *dof_vert,
+                          int *dof_comp, hipStream_t s) {
+    dof_from_scan_kernel<<<grid_for(n, 256), 256, 0, s>>>(live, scan, n, dof_of, nv, dof_vert, dof_comp);
+    check_launch();
+}
+void launch_face_classify(const int *tets, const int *phase, int64_t nc, const int *vptr, const int *vlist,
+                          int *iface_flag, int *bnd_flag, int *nbr, hipStream_t s) {
+    face_classify_kernel<<<grid_for(4 * nc, 256), 256, 0, s>>>(tets, phase, nc, vptr, vlist, iface_flag, bnd_flag,
+                                                                nbr);
+    check_launch();
+}
+void launch_face_compact(const int *tets, int64_t nc, const int *nbr, const int *iface_flag, const int *iface_scan,
+                         const int *bnd_flag, const int *bnd_scan, int *iface_v, int *iface_c, int *bnd_v,
+                         int *bnd_c, hipStream_t s) {
+    face_compact_kernel<<<grid_for(4 * nc, 256), 256, 0, s>>>(tets, nc, nbr, iface_flag, iface_scan, bnd_flag,
+                                                               bnd_scan, iface_v, iface_c, bnd_v, bnd_c);
+    check_launch();
+}
+void launch_pattern(int pass, int64_t ndof, int64_t nv, const int *dof_vert, const int *dof_comp, const int *tets,
+                    const int *phase, const int *vptr, const int *vlist, const int *fptr, const int *flist,
+                    const int *iface_v, const int *dof_of, int *row_count, const int *rowptr, int *col,
+                    int *overflow, hipStream_t s) {
+    if (ndof <= 0) return;
+    pattern_kernel<<<(unsigned)ndof, 64, 0, s>>>(pass, dof_vert, dof_comp, tets, phase, nv, vptr, vlist, fptr, flist,
+                                                 iface_v, dof_of, row_count, rowptr, col, overflow);
+    check_launch();
+}
+void launch_jp_round(int k, int64_t ne, int nvert, const int *ev, const int *vptr, const int *vlist, int *color,
+                     int *remaining, hipStream_t s) {
+    jp_round_kernel<<<grid_for(ne, 256), 256, 0, s>>>(k, ne, nvert, ev, vptr, vlist, color, remaining);
+    check_launch();
+}
+void launch_color_hist(const int *color, int64_t ne, int *hist, hipStream_t s) {
+    color_hist_kernel<<<grid_for(ne, 256), 256, 0, s>>>(color, ne, hist);
+    check_launch();
+}
+void launch_color_scatter(const int *color, int64_t ne, const int *offs, int *cursor, int *perm, hipStream_t s) {
+    color_scatter_kernel<<<grid_for(ne, 256), 256, 0, s>>>(color, ne, offs, cursor, perm);
+    check_launch();
+}
+void launch_gather_cells(const int *perm, int64_t ne, const int *tets, const int *phase, int *ct, int *cph,
+                         hipStream_t s) {
+    gather_cells_kernel<<<grid_for(ne, 256), 256, 0, s>>>(perm, ne, tets, phase, ct, cph);
+    check_launch();
+}
+void launch_assemble_cells(int64_t start, int64_t count, int64_t ne, const int *perm, const int *ct, const int *cph,
+                           const double *xyz, int64_t nv, const int *dof_of, const int *rowptr, const int *col,
+                           const double *Dc, const double *Dcell, const double *iT2c, const double *iT2cell,
+                           double *m, double *k, double *r, double *jx, double *jy, double *jz, int *miss,
+                           hipStream_t s) {
+    if (count <= 0) return;
+    assemble_cells_kernel<<<grid_for(count, 128), 128, 0, s>>>(start, count, ne, perm, ct, cph, xyz, nv,
+                                                               dof_of, rowptr, col, Dc, Dcell, iT2c, iT2cell, m, k, r,
+                                                               jx, jy, jz, miss);
+    check_launch();
+}
+void launch_assemble_facets(int64_t start, int64_t count, const int *perm, const int *iface_v, const double *xyz,
+                            int64_t nv, const int *dof_of, const int *rowptr, const int *col, double kappa,
+                            const double *kappa_f, double *gi, int *miss, hipStream_t s) {
+    if (count <= 0) return;
+    assemble_facets_kernel<<<grid_for(count, 128), 128, 0, s>>>(start, count, perm, iface_v, xyz, nv, dof_of, rowptr,
+                                                                col, kappa, kappa_f, gi, miss);
+    check_launch();
+}
+void launch_diag_and_weights(int64_t n, const int *rowptr, const int *col, const double *m, int *diag, double *w,
+                             int *missing_diag, hipStream_t s) {
+    diag_weights_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rowptr, col, m, diag, w, missing_diag);
+    check_launch();
+}
+void launch_sum(const double *x, int64_t n, double *part, unsigned *counter, double *out, hipStream_t s) {
+    unsigned g = grid_for(n, kBlock);
+    if (g > (unsigned)kVecBlocks) g = kVecBlocks;
+    if (g == 0) g = 1;
+    sum_kernel<<<g, kBlock, 0, s>>>(x, n, part, counter, out);
+    check_launch();
+}
+
+void launch_prep_scale(int64_t n, const int *diag, const double *m, const double *k, const double *r,
+                       const double *gi, double dt, double theta, double *sc, hipStream_t s) {
+    prep_scale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, diag, m, k, r, gi, dt, theta, sc);
+    check_launch();
+}
+void launch_prep_values(int64_t n, const int *rowptr, const int *col, const double *sc, const double *m,
+                        const double *k, const double *r, const double *gi, const double *jx, const double *jy,
+                        const double *jz, double gx, double gy, double gz, double dt, double theta, double2 *aj,
+                        double *bre, hipStream_t s) {
+    prep_values_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rowptr, col, sc, m, k, r, gi, jx, jy, jz, gx, gy, gz, dt,
+                                                         theta, aj, bre);
+    check_launch();
+}
+void launch_init_x(int64_t n, const double *sc, double2 *x, hipStream_t s) {
+    init_x_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, sc, x);
+    check_launch();
+}
+void launch_unscale(int64_t n, const double *sc, const double2 *x, double2 *u, hipStream_t s) {
+    unscale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, sc, x, u);
+    check_launch();
+}
+void launch_rhs_residual(int64_t n, const int *rowptr, const int *col, const double2 *aj, const double *bre,
+                         double sA, double sB, const double2 *x, double2 *r, double2 *rh, double rtol, double atol,
+                         int maxit, double *part, KrylovState *st, hipStream_t s) {
+    rhs_residual_kernel<<<grid_for(n, kRowsPerBlock), kBlock, 0, s>>>(n, rowptr, col, aj, bre, sA, sB, x, r, rh, rtol,
+                                                                      atol, maxit, part, st);
+    check_launch();
+}
+static inline unsigned vec_grid(int64_t n) {
+    unsigned g = grid_for(n, kBlock);
+    if (g > (unsigned)kVecBlocks) g = kVecBlocks;
+    return g == 0 ? 1 : g;
+}
+void launch_bicg_p(int64_t n, const double2 *r, double2 *p, const double2 *v, const KrylovState *st,
+                   hipStream_t s) {
+    bicg_p_kernel<<<vec_grid(n), kBlock, 0, s>>>(n, r, p, v, st);
+    check_launch();
+}
+void launch_spmv_v(int64_t n, const int *rowptr, const int *col, const double2 *aj, double sA, const double2 *p,
+                   double2 *v, const double2 *rh, double *part, KrylovState *st, hipStream_t s) {
+    spmv_v_kernel<<<grid_for(n, kRowsPerBlock), kBlock, 0, s>>>(n, rowptr, col, aj, sA, p, v, rh, part, st);
+    check_launch();
+}
+void launch_spmv_t(int64_t n, const int *rowptr, const int *col, const double2 *aj, double sA, const double2 *r,
+                   const double2 *v, double2 *t, double *part, KrylovState *st, hipStream_t s) {
+    spmv_t_kernel<<<grid_for(n, kRowsPerBlock), kBlock, 0, s>>>(n, rowptr, col, aj, sA, r, v, t, part, st);
+    check_launch();
+}
+void launch_bicg_update(int64_t n, double2 *x, const double2 *p, double2 *r, const double2 *v, const double2 *t,
+                        const double2 *rh, double *part, KrylovState *st, hipStream_t s) {
+    bicg_update_kernel<<<vec_grid(n), kBlock, 0, s>>>(n, x, p, r, v, t, rh, part, st);
+    check_launch();
+}
+void launch_signal(int64_t n, const double *w, const double *sc, const double2 *x, double *part, unsigned *counter,
+                   double *out, hipStream_t s) {
+    signal_kernel<<<vec_grid(n), kBlock, 0, s>>>(n, w, sc, x, part, counter, out);
+    check_launch();
+}
+
+}  // namespace dmri

@@ -1,0 +1,205 @@
+"""ctypes binding of libdmri (include/dmri.h).
+
+This is the only route from Python to the solver: there is no CPU fallback.  If
+libdmri.so is missing the import of this module raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("DMRI_LIB", os.path.join(_PKG, "lib", "libdmri.so"))
+
+DMRI_MAT = {"m": 0, "k": 1, "r": 2, "gi": 3, "jx": 4, "jy": 5, "jz": 6}
+ERRORS = {-1: "ARG", -2: "HIP", -3: "STATE", -4: "NOCONV", -5: "MESH", -6: "ALLOC", -7: "LIMIT"}
+
+
+class DmriError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libdmri error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class _Coeffs(ctypes.Structure):
+    _fields_ = [("D", ctypes.c_double * 2), ("D_cell", ctypes.POINTER(ctypes.c_double)),
+                ("invT2", ctypes.c_double * 2), ("invT2_cell", ctypes.POINTER(ctypes.c_double)),
+                ("kappa", ctypes.c_double), ("kappa_facet", ctypes.POINTER(ctypes.c_double))]
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [("nv", ctypes.c_int64), ("nc", ctypes.c_int64), ("ncomp", ctypes.c_int64), ("ndof", ctypes.c_int64),
+                ("nnz", ctypes.c_int64), ("n_iface", ctypes.c_int64), ("n_bfacet", ctypes.c_int64),
+                ("ncolors_cells", ctypes.c_int32), ("ncolors_iface", ctypes.c_int32), ("volume", ctypes.c_double),
+                ("hmin", ctypes.c_double), ("hmax", ctypes.c_double), ("setup_ms", ctypes.c_double)]
+
+
+class _Seq(ctypes.Structure):
+    _fields_ = [("nsteps", ctypes.c_int32), ("f_implicit", ctypes.POINTER(ctypes.c_double)),
+                ("f_explicit", ctypes.POINTER(ctypes.c_double)), ("dt", ctypes.c_double),
+                ("theta", ctypes.c_double), ("q", ctypes.c_double), ("gdir", ctypes.c_double * 3)]
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxit", ctypes.c_int32),
+                ("profile", ctypes.c_int32)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("iters_total", ctypes.c_int64), ("iters_max", ctypes.c_int32), ("restarts", ctypes.c_int32),
+                ("relres_max", ctypes.c_double), ("solve_ms", ctypes.c_double), ("spmv_ms", ctypes.c_double),
+                ("spmv_launches", ctypes.c_int64), ("spmv_bytes", ctypes.c_double)]
+
+
+EXPORTED = ["dmri_create", "dmri_destroy", "dmri_last_error", "dmri_version", "dmri_mesh_set", "dmri_coeffs_set",
+            "dmri_assemble", "dmri_get_info", "dmri_export_pattern", "dmri_export_values", "dmri_export_facets",
+            "dmri_solve", "dmri_get_solution"]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libdmri.so not found at {LIB_PATH}: build it with "
+                          f"`python dmri-fem-cloud_amd/build_native.py` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp, i64, i32p, dp = ctypes.c_void_p, ctypes.c_int64, P(ctypes.c_int32), P(ctypes.c_double)
+    L.dmri_create.argtypes = [P(vp), ctypes.c_int]
+    L.dmri_destroy.argtypes = [vp]
+    L.dmri_last_error.restype = ctypes.c_char_p
+    L.dmri_version.restype = ctypes.c_char_p
+    L.dmri_mesh_set.argtypes = [vp, i64, dp, i64, i32p, i32p]
+    L.dmri_coeffs_set.argtypes = [vp, P(_Coeffs)]
+    L.dmri_assemble.argtypes = [vp]
+    L.dmri_get_info.argtypes = [vp, P(_Info)]
+    L.dmri_export_pattern.argtypes = [vp, P(ctypes.c_int64), i32p, i32p]
+    L.dmri_export_values.argtypes = [vp, ctypes.c_int, dp]
+    L.dmri_export_facets.argtypes = [vp, i32p]
+    L.dmri_solve.argtypes = [vp, P(_Seq), P(_Opts), dp, P(_Stats)]
+    L.dmri_get_solution.argtypes = [vp, dp]
+    for name in EXPORTED:
+        if name not in ("dmri_last_error", "dmri_version"):
+            getattr(L, name).restype = ctypes.c_int
+    return L
+
+
+LIB = _load()
+
+
+def _ptr(a, ct):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def _check(rc):
+    if rc != 0:
+        raise DmriError(rc, LIB.dmri_last_error().decode())
+
+
+def version() -> str:
+    return LIB.dmri_version().decode()
+
+
+class Context:
+    """One solver context on one GPU (see include/dmri.h)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        _check(LIB.dmri_create(ctypes.byref(self._h), int(device)))
+
+    def close(self):
+        if self._h:
+            LIB.dmri_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- set-up -------------------------------------------------------------------
+    def set_mesh(self, xyz, tets, phase=None):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        tets = np.ascontiguousarray(tets, dtype=np.int32)
+        ph = None if phase is None else np.ascontiguousarray(phase, dtype=np.int32)
+        _check(LIB.dmri_mesh_set(self._h, xyz.shape[0], _ptr(xyz, ctypes.c_double), tets.shape[0],
+                                 _ptr(tets, ctypes.c_int32), _ptr(ph, ctypes.c_int32)))
+
+    def set_coeffs(self, D=3e-3, invT2=1e-16, kappa=0.0, D_cell=None, invT2_cell=None, kappa_facet=None):
+        c = _Coeffs()
+        Dv = np.broadcast_to(np.asarray(D, dtype=np.float64), (2,))
+        Tv = np.broadcast_to(np.asarray(invT2, dtype=np.float64), (2,))
+        c.D[0], c.D[1] = float(Dv[0]), float(Dv[1])
+        c.invT2[0], c.invT2[1] = float(Tv[0]), float(Tv[1])
+        c.kappa = float(kappa)
+        self._keep = []
+        for name, arr in (("D_cell", D_cell), ("invT2_cell", invT2_cell), ("kappa_facet", kappa_facet)):
+            if arr is not None:
+                a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+                self._keep.append(a)
+                setattr(c, name, _ptr(a, ctypes.c_double))
+        _check(LIB.dmri_coeffs_set(self._h, ctypes.byref(c)))
+
+    def assemble(self):
+        _check(LIB.dmri_assemble(self._h))
+
+    def info(self) -> dict:
+        i = _Info()
+        _check(LIB.dmri_get_info(self._h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in _Info._fields_}
+
+    # -- parity hooks ---------------------------------------------------------------
+    def export_pattern(self):
+        inf = self.info()
+        rowptr = np.zeros(inf["ndof"] + 1, dtype=np.int64)
+        col = np.zeros(inf["nnz"], dtype=np.int32)
+        dof_of = np.zeros(inf["ncomp"] * inf["nv"], dtype=np.int32)
+        _check(LIB.dmri_export_pattern(self._h, _ptr(rowptr, ctypes.c_int64), _ptr(col, ctypes.c_int32),
+                                       _ptr(dof_of, ctypes.c_int32)))
+        return rowptr, col, dof_of
+
+    def export_values(self, which: str) -> np.ndarray:
+        out = np.zeros(self.info()["nnz"])
+        _check(LIB.dmri_export_values(self._h, DMRI_MAT[which], _ptr(out, ctypes.c_double)))
+        return out
+
+    def export_facets(self) -> np.ndarray:
+        n = self.info()["n_iface"]
+        out = np.zeros((max(n, 1), 3), dtype=np.int32)
+        _check(LIB.dmri_export_facets(self._h, _ptr(out, ctypes.c_int32)))
+        return out[:n]
+
+    # -- solve --------------------------------------------------------------------------
+    def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
+              profile=False):
+        fA = np.ascontiguousarray(f_implicit, dtype=np.float64)
+        fB = np.ascontiguousarray(f_explicit, dtype=np.float64)
+        if fA.shape != fB.shape:
+            raise ValueError("f_implicit and f_explicit differ in length")
+        s = _Seq()
+        s.nsteps = fA.shape[0]
+        s.f_implicit = _ptr(fA, ctypes.c_double)
+        s.f_explicit = _ptr(fB, ctypes.c_double)
+        s.dt, s.theta, s.q = float(dt), float(theta), float(q)
+        for i in range(3):
+            s.gdir[i] = float(gdir[i])
+        o = _Opts()
+        o.rtol, o.atol, o.maxit, o.profile = float(rtol), float(atol), int(maxit), 1 if profile else 0
+        sig = np.zeros(2)
+        st = _Stats()
+        _check(LIB.dmri_solve(self._h, ctypes.byref(s), ctypes.byref(o), _ptr(sig, ctypes.c_double),
+                              ctypes.byref(st)))
+        return complex(sig[0], sig[1]), {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    def solution(self) -> np.ndarray:
+        n = self.info()["ndof"]
+        u = np.zeros(2 * n)
+        _check(LIB.dmri_get_solution(self._h, _ptr(u, ctypes.c_double)))
+        return u.view(np.complex128)

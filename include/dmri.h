@@ -1,0 +1,153 @@
+/*
+ * dmri.h -- C ABI of libdmri, the MI355X (gfx950) Bloch-Torrey FEM solver.
+ *
+ * Plain C: pointers, sizes and status codes.  Every function returns DMRI_OK (0) or a
+ * negative DMRI_ERR_* code; dmri_last_error() returns the message of the calling
+ * thread's last failure.  No exception crosses this boundary.
+ *
+ * Ownership: the caller owns every host array and the library copies what it needs;
+ * the context owns its device buffers.  Threading: one context per host thread at a
+ * time (a context is not re-entrant); separate contexts may run concurrently.
+ * All arithmetic is FP64.  Units follow the reference: um, us, D in um^2/us (== mm^2/s),
+ * kappa in um/us (== m/s), b in us/um^2 (== s/mm^2), q in 1/(um us).
+ *
+ * The reference interfaces each entry point replaces are cited per function
+ * (paths relative to the van-dang/DMRI-FEM-Cloud repository).
+ */
+#ifndef DMRI_H
+#define DMRI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMRI_OK 0
+#define DMRI_ERR_ARG (-1)     /* invalid argument */
+#define DMRI_ERR_HIP (-2)     /* HIP runtime failure */
+#define DMRI_ERR_STATE (-3)   /* call out of order (e.g. solve before assemble) */
+#define DMRI_ERR_NOCONV (-4)  /* Krylov solver did not converge within maxit */
+#define DMRI_ERR_MESH (-5)    /* mesh / phase / facet data inconsistent */
+#define DMRI_ERR_ALLOC (-6)   /* device or host allocation failed */
+#define DMRI_ERR_LIMIT (-7)   /* a size exceeds a documented limit */
+
+typedef struct dmri_ctx dmri_ctx;
+
+/* Value arrays produced by dmri_assemble (all on the DOF sparsity pattern). */
+enum dmri_matrix {
+    DMRI_MAT_M = 0,  /* mass  int phi_i phi_j                       (MassMatrix1c/2c, DmriFemLib.py:240-254) */
+    DMRI_MAT_K = 1,  /* stiffness int grad phi_i . D grad phi_j     (FuncF_wBC, DmriFemLib.py:43-44) */
+    DMRI_MAT_R = 2,  /* relaxation int (1/T2) phi_i phi_j           (FuncF_wBC, DmriFemLib.py:43-44) */
+    DMRI_MAT_I = 3,  /* interface kappa int_Gamma (u0-u1)(v0-v1)    (icondition_wBC, DmriFemLib.py:47-50) */
+    DMRI_MAT_JX = 4, /* int x phi_i phi_j  (GdotX, DmriFemLib.py:33-39; Jg = gx Jx + gy Jy + gz Jz) */
+    DMRI_MAT_JY = 5,
+    DMRI_MAT_JZ = 6
+};
+
+/* Create a context on HIP device `device`.
+ * Replaces the process set-up of the drivers: MPI_Init + DOLFIN init
+ * (comri/one-comp/hpc-fenics-cpp/main.cpp:117-127) and `from dolfin import *`
+ * (GCloudDmriSolver.py:47-49). */
+int dmri_create(dmri_ctx **out, int device);
+int dmri_destroy(dmri_ctx *ctx);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *dmri_last_error(void);
+
+/* Library version string. */
+const char *dmri_version(void);
+
+/* Mesh: nv vertices (xyz, nv*3, um), nc tetrahedra (tets, nc*4, 0-based), optional
+ * cell phase in {0,1} (NULL = one compartment).
+ * Replaces Mesh(fmesh) / HDF5File.read(mesh) and the phase function
+ * (comri/two-comp/hpc-fenics-cpp/main.cpp:670, MarkPhase 242-337; GCloudDmriSolver.py:150-152,
+ * 173-177; CreatePhaseFunc, DmriFemLib.py:754-798). */
+int dmri_mesh_set(dmri_ctx *ctx, int64_t nv, const double *xyz, int64_t nc, const int32_t *tets,
+                  const int32_t *cell_phase);
+
+/* Coefficients.  Replaces MyDomain.D / D0 / ImposeDiffusionTensor, MyDomain.kappa and
+ * MRI_parameters.T2 (DmriFemLib.py:583-616,807; GCloudDmriSolver.py:157-171,204-215) and the
+ * comri kcoeff/kappa coefficients (comri/two-comp/hpc-fenics-cpp/main.cpp:741). */
+typedef struct {
+    double D[2];                /* scalar diffusivity per compartment */
+    const double *D_cell;       /* optional nc*9 row-major tensor per cell (overrides D) */
+    double invT2[2];            /* 1/T2 per compartment (reference default T2 = 1e16) */
+    const double *invT2_cell;   /* optional nc values (overrides invT2) */
+    double kappa;               /* interface permeability */
+    const double *kappa_facet;  /* optional, one value per interface facet in the order of
+                                   dmri_export_facets (overrides kappa) */
+} dmri_coeffs;
+int dmri_coeffs_set(dmri_ctx *ctx, const dmri_coeffs *coeffs);
+
+/* Build on the device: DOF map, interface facets, CSR pattern, element colouring and the
+ * value arrays M, K, R, I, Jx, Jy, Jz.
+ * Replaces DOLFIN's sparsity builder + UFC tabulate_dofs / tabulate_tensor
+ * (comri/one-comp/hpc-fenics-cpp/ufc/Bloch_Torrey_NoTime3D.cpp:3869-3880,4110-4353;
+ * comri/two-comp/hpc-fenics-cpp/ufc/Bloch_Torrey_NoTime3D.cpp:6699-7050,7421-8890) as driven by
+ * assembler.assemble (main.cpp:261-281, two-comp 829-849) and MassMatrix / assemble(F)
+ * (DmriFemLib.py:245,252,904). */
+int dmri_assemble(dmri_ctx *ctx);
+
+typedef struct {
+    int64_t nv, nc, ncomp, ndof, nnz, n_iface, n_bfacet;
+    int32_t ncolors_cells, ncolors_iface;
+    double volume;        /* sum of M (|Omega|; whole_vol of PostProcessing, DmriFemLib.py:923) */
+    double hmin, hmax;    /* min/max greatest vertex distance of a cell (mymesh.hmin/hmax) */
+    double setup_ms;      /* device time of dmri_assemble */
+} dmri_info;
+int dmri_get_info(dmri_ctx *ctx, dmri_info *info);
+
+/* Parity hooks: copy the device results to host buffers sized from dmri_get_info.
+ * rowptr: ndof+1, col: nnz, dof_of: ncomp*nv (-1 = dead DOF), val: nnz, fv: 3*n_iface. */
+int dmri_export_pattern(dmri_ctx *ctx, int64_t *rowptr, int32_t *col, int32_t *dof_of);
+int dmri_export_values(dmri_ctx *ctx, int which, double *val);
+int dmri_export_facets(dmri_ctx *ctx, int32_t *fv);
+
+/* Time sequence for one signal.
+ * f_implicit[n] multiplies J on the implicit side of step n, f_explicit[n] on the explicit
+ * side.  GCloud: f(t_n) and f(t_{n-1}) with t_{-1}=0 (MRI_simulation.solve, DmriFemLib.py:897-910);
+ * comri: both f(t_n) (main.cpp:300-318). */
+typedef struct {
+    int32_t nsteps;
+    const double *f_implicit;
+    const double *f_explicit;
+    double dt;        /* k (MRI_simulation.k, -k) */
+    double theta;     /* 0.5 (MRI_simulation.theta, DmriFemLib.py:863) */
+    double q;         /* gamma*G (MRI_parameters.qvalue; comri gnorm) */
+    double gdir[3];   /* gradient direction (normalised inside) */
+} dmri_seq;
+
+typedef struct {
+    double rtol;       /* ||r|| <= max(rtol ||b||, atol); reference 1e-9 (GCloudDmriSolver.py:220) */
+    double atol;
+    int32_t maxit;     /* per time step */
+    int32_t profile;   /* 1: time the SpMV launches with HIP events (dmri_stats.spmv_ms) */
+} dmri_solver_opts;
+
+typedef struct {
+    int64_t iters_total;    /* Krylov iterations over all steps */
+    int32_t iters_max;      /* max iterations of one step */
+    int32_t restarts;       /* BiCGStab breakdown restarts */
+    double relres_max;      /* max final ||r||/||b|| over the steps */
+    double solve_ms;        /* wall time of the time loop (device synchronised) */
+    double spmv_ms;         /* summed device time of the timed SpMV launches (profile=1) */
+    int64_t spmv_launches;  /* number of timed SpMV launches */
+    double spmv_bytes;      /* algorithmic bytes of one SpMV launch */
+} dmri_stats;
+
+/* Run the theta scheme from u0 = 1 (real part, every live DOF) and return the signal
+ * sig_out[0] = sum_i w_i Re u_i / |Omega|, sig_out[1] = the same for Im u (w = M 1).
+ * Replaces MRI_simulation.solve + PostProcessing (DmriFemLib.py:878-988) and the comri
+ * time loop + Comp_Sig3D (main.cpp:296-333, two-comp 854-900). */
+int dmri_solve(dmri_ctx *ctx, const dmri_seq *seq, const dmri_solver_opts *opts, double sig_out[2],
+               dmri_stats *stats);
+
+/* Copy u^N of the last dmri_solve to host (2*ndof doubles, interleaved re/im, DOF order). */
+int dmri_get_solution(dmri_ctx *ctx, double *u);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMRI_H */

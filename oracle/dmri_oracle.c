@@ -1,0 +1,512 @@
+/*
+ * dmri_oracle.c -- CPU FP64 restatement of the reference's Bloch-Torrey FEM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the smoke()
+ * entry and the cpu_baseline leg of bench.py may load it.  The product path
+ * (libdmri.so, HIP kernels) never calls, links or falls back to it.
+ *
+ * What it restates (reference files, van-dang/DMRI-FEM-Cloud):
+ *   - DOF layout: mixed P1 space, 2 (1-compartment) or 4 (2-compartment) real
+ *     components per vertex; DmriFemLib.py:453-486, comri/two-comp/hpc-fenics-cpp/
+ *     ufc/Bloch_Torrey_NoTime3D.cpp:6050-6397 (dof = comp*Nv + vertex).  Here the
+ *     real/imag pair is one complex unknown and dead compartment DOFs (vertices no
+ *     cell of that compartment touches; M.ident_zeros() at DmriFemLib.py:246) are
+ *     dropped, which leaves the live solution unchanged.
+ *   - Sparsity: union of element cliques (DOLFIN's sparsity builder; the reference
+ *     relies on third-party code here, so the definition is the brute-force set union).
+ *   - Element matrices (closed forms of the P1 integrals in the forms of
+ *     DmriFemLib.py:41-50,79-93,125-145,240-254 and comri/one-comp/hpc-fenics-cpp/ufc/
+ *     Bloch_Torrey_NoTime3D.ufl): mass, stiffness with scalar or DG0 tensor D, T2
+ *     relaxation, gradient moments Jx/Jy/Jz, interface facet mass on phase jumps.
+ *   - Theta time stepping (DmriFemLib.py:878-915): A_n u^{n+1} = B_n u^n with
+ *     A_n = M/k + theta*L + i*theta*q*fA[n]*Jg, B_n = M/k - (1-theta)*L - i*(1-theta)*q*fB[n]*Jg,
+ *     L = K + R + I, solved by BiCGStab with Jacobi (GCloudDmriSolver.py:219-222,
+ *     comri/one-comp/hpc-fenics-cpp/main.cpp:232).
+ *
+ * Every parallel loop uses a static partition and every reduction sums fixed
+ * chunks in a fixed order, so results do not depend on the thread count.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+#define ORC_CHUNK 4096
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------------- */
+/* DOF map.  dof_of[c*nv + v] = DOF index of (compartment c, vertex v) or -1.   */
+/* Numbering is compartment-major, increasing vertex index inside a compartment */
+/* (the DOLFIN-HPC serial layout, Bloch_Torrey_NoTime3D.cpp:6050-6397).        */
+/* ------------------------------------------------------------------------- */
+int64_t orc_dof_map(int64_t nv, int64_t nc, const int32_t *tets, const int32_t *phase,
+                    int ncomp, int32_t *dof_of) {
+    memset(dof_of, 0, sizeof(int32_t) * (size_t)(ncomp * nv));
+    for (int64_t t = 0; t < nc; ++t) {
+        int c = phase ? phase[t] : 0;
+        if (c < 0 || c >= ncomp) return -1;
+        for (int a = 0; a < 4; ++a) {
+            int32_t v = tets[4 * t + a];
+            if (v < 0 || v >= nv) return -2;
+            dof_of[c * nv + v] = 1;
+        }
+    }
+    int64_t d = 0;
+    for (int c = 0; c < ncomp; ++c)
+        for (int64_t v = 0; v < nv; ++v)
+            dof_of[c * nv + v] = dof_of[c * nv + v] ? (int32_t)(d++) : -1;
+    return d;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Faces.  Interface facets Gamma are the faces shared by two cells of         */
+/* different phase (abs(jump(phase))*dS, DmriFemLib.py:112,139).  Boundary     */
+/* facets are faces owned by exactly one cell.                                 */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int32_t v[3];
+    int32_t cell;
+} face_rec;
+
+static int face_cmp(const void *pa, const void *pb) {
+    const face_rec *a = (const face_rec *)pa, *b = (const face_rec *)pb;
+    for (int i = 0; i < 3; ++i) {
+        if (a->v[i] != b->v[i]) return a->v[i] < b->v[i] ? -1 : 1;
+    }
+    return a->cell < b->cell ? -1 : (a->cell > b->cell);
+}
+
+static void sort3(int32_t *v) {
+    int32_t t;
+    if (v[0] > v[1]) { t = v[0]; v[0] = v[1]; v[1] = t; }
+    if (v[1] > v[2]) { t = v[1]; v[1] = v[2]; v[2] = t; }
+    if (v[0] > v[1]) { t = v[0]; v[0] = v[1]; v[1] = t; }
+}
+
+static face_rec *collect_faces(int64_t nc, const int32_t *tets) {
+    face_rec *f = (face_rec *)malloc(sizeof(face_rec) * (size_t)(4 * nc));
+    if (!f) return NULL;
+    for (int64_t t = 0; t < nc; ++t) {
+        const int32_t *T = tets + 4 * t;
+        for (int o = 0; o < 4; ++o) { /* face opposite local vertex o */
+            face_rec *r = f + 4 * t + o;
+            int k = 0;
+            for (int a = 0; a < 4; ++a)
+                if (a != o) r->v[k++] = T[a];
+            sort3(r->v);
+            r->cell = (int32_t)t;
+        }
+    }
+    qsort(f, (size_t)(4 * nc), sizeof(face_rec), face_cmp);
+    return f;
+}
+
+/* Interface facets sorted by vertex triple.  fv[3*i..] vertices (ascending),
+ * fc[2*i] = cell on phase 0, fc[2*i+1] = cell on phase 1.  Returns the count
+ * (writes at most cap records; call with cap=0 to size).                     */
+int64_t orc_interface_facets(int64_t nc, const int32_t *tets, const int32_t *phase,
+                             int32_t *fv, int32_t *fc, int64_t cap) {
+    if (!phase) return 0;
+    face_rec *f = collect_faces(nc, tets);
+    if (!f) return -1;
+    int64_t n = 0;
+    for (int64_t i = 0; i + 1 < 4 * nc; ++i) {
+        if (f[i].v[0] == f[i + 1].v[0] && f[i].v[1] == f[i + 1].v[1] && f[i].v[2] == f[i + 1].v[2]) {
+            int32_t c0 = f[i].cell, c1 = f[i + 1].cell;
+            if (phase[c0] != phase[c1]) {
+                if (n < cap) {
+                    memcpy(fv + 3 * n, f[i].v, sizeof(int32_t) * 3);
+                    fc[2 * n] = phase[c0] == 0 ? c0 : c1;
+                    fc[2 * n + 1] = phase[c0] == 0 ? c1 : c0;
+                }
+                ++n;
+            }
+            ++i;
+        }
+    }
+    free(f);
+    return n;
+}
+
+/* Boundary (exterior) facets sorted by vertex triple; fc[i] = owning cell.    */
+int64_t orc_boundary_facets(int64_t nc, const int32_t *tets, int32_t *fv, int32_t *fc, int64_t cap) {
+    face_rec *f = collect_faces(nc, tets);
+    if (!f) return -1;
+    int64_t n = 0, m = 4 * nc;
+    for (int64_t i = 0; i < m;) {
+        int64_t j = i + 1;
+        while (j < m && f[j].v[0] == f[i].v[0] && f[j].v[1] == f[i].v[1] && f[j].v[2] == f[i].v[2]) ++j;
+        if (j - i == 1) {
+            if (n < cap) {
+                memcpy(fv + 3 * n, f[i].v, sizeof(int32_t) * 3);
+                fc[n] = f[i].cell;
+            }
+            ++n;
+        }
+        i = j;
+    }
+    free(f);
+    return n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Sparsity pattern: set union of the DOF cliques of every cell (4 DOFs in its  */
+/* own compartment) and every interface facet (3 vertices x 2 compartments).   */
+/* Keys row*ndof+col are radix-sorted and made unique.                          */
+/* ------------------------------------------------------------------------- */
+static void radix_sort_u64(uint64_t *a, uint64_t *tmp, size_t n, int nbits) {
+    size_t cnt[65536];
+    for (int shift = 0; shift < nbits; shift += 16) {
+        memset(cnt, 0, sizeof(cnt));
+        for (size_t i = 0; i < n; ++i) cnt[(a[i] >> shift) & 0xFFFF]++;
+        size_t s = 0;
+        for (int d = 0; d < 65536; ++d) { size_t c = cnt[d]; cnt[d] = s; s += c; }
+        for (size_t i = 0; i < n; ++i) tmp[cnt[(a[i] >> shift) & 0xFFFF]++] = a[i];
+        memcpy(a, tmp, n * sizeof(uint64_t));
+    }
+}
+
+int64_t orc_pattern(int64_t ndof, int64_t nv, int64_t nc, const int32_t *tets, const int32_t *phase,
+                    const int32_t *dof_of, int64_t nf, const int32_t *fv,
+                    int64_t *rowptr, int32_t *col) {
+    size_t npairs = (size_t)(16 * nc + 36 * nf);
+    uint64_t *keys = (uint64_t *)malloc(npairs * sizeof(uint64_t));
+    uint64_t *tmp = (uint64_t *)malloc(npairs * sizeof(uint64_t));
+    if (!keys || !tmp) { free(keys); free(tmp); return -1; }
+    size_t p = 0;
+    for (int64_t t = 0; t < nc; ++t) {
+        int c = phase ? phase[t] : 0;
+        int32_t d[4];
+        for (int a = 0; a < 4; ++a) d[a] = dof_of[c * nv + tets[4 * t + a]];
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) keys[p++] = (uint64_t)d[a] * (uint64_t)ndof + (uint64_t)d[b];
+    }
+    for (int64_t f = 0; f < nf; ++f) {
+        int32_t d[6];
+        for (int a = 0; a < 3; ++a) {
+            d[a] = dof_of[0 * nv + fv[3 * f + a]];
+            d[3 + a] = dof_of[1 * nv + fv[3 * f + a]];
+        }
+        for (int a = 0; a < 6; ++a)
+            for (int b = 0; b < 6; ++b) keys[p++] = (uint64_t)d[a] * (uint64_t)ndof + (uint64_t)d[b];
+    }
+    int nbits = 0;
+    uint64_t maxkey = (uint64_t)ndof * (uint64_t)ndof;
+    while (nbits < 64 && (maxkey >> nbits)) nbits += 16;
+    radix_sort_u64(keys, tmp, p, nbits);
+    free(tmp);
+    for (int64_t r = 0; r <= ndof; ++r) rowptr[r] = 0;
+    int64_t nnz = 0;
+    for (size_t i = 0; i < p; ++i) {
+        if (i > 0 && keys[i] == keys[i - 1]) continue;
+        int64_t r = (int64_t)(keys[i] / (uint64_t)ndof);
+        if (col) col[nnz] = (int32_t)(keys[i] % (uint64_t)ndof);
+        rowptr[r + 1]++;
+        ++nnz;
+    }
+    for (int64_t r = 0; r < ndof; ++r) rowptr[r + 1] += rowptr[r];
+    free(keys);
+    return nnz;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Closed-form P1 element matrices.                                            */
+/*   int_T l_a l_b       = |T|(1+d_ab)/20                                      */
+/*   int_T l_a l_b l_c   = |T|/20, /60, /120 for 3, 2, 0 equal indices          */
+/*   => Jx_ab = |T|/120 (1+d_ab) (sum_c x_c + x_a + x_b)                       */
+/*   K_ab = |T| grad(l_a) . D grad(l_b)                                        */
+/*   int_F l_a l_b       = |F|(1+d_ab)/12                                      */
+/* ------------------------------------------------------------------------- */
+static double tet_geom(const double X[4][3], double G[4][3]) {
+    double e[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) e[i][k] = X[i + 1][k] - X[0][k];
+    double c23[3] = {e[1][1] * e[2][2] - e[1][2] * e[2][1], e[1][2] * e[2][0] - e[1][0] * e[2][2],
+                     e[1][0] * e[2][1] - e[1][1] * e[2][0]};
+    double c31[3] = {e[2][1] * e[0][2] - e[2][2] * e[0][1], e[2][2] * e[0][0] - e[2][0] * e[0][2],
+                     e[2][0] * e[0][1] - e[2][1] * e[0][0]};
+    double c12[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                     e[0][0] * e[1][1] - e[0][1] * e[1][0]};
+    double det = e[0][0] * c23[0] + e[0][1] * c23[1] + e[0][2] * c23[2];
+    for (int k = 0; k < 3; ++k) {
+        G[1][k] = c23[k] / det;
+        G[2][k] = c31[k] / det;
+        G[3][k] = c12[k] / det;
+        G[0][k] = -(G[1][k] + G[2][k] + G[3][k]);
+    }
+    return fabs(det) / 6.0;
+}
+
+static int64_t csr_find(const int64_t *rowptr, const int32_t *col, int64_t r, int32_t c) {
+    int64_t lo = rowptr[r], hi = rowptr[r + 1] - 1;
+    while (lo <= hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (col[mid] == c) return mid;
+        if (col[mid] < c) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+/* Assemble the time-independent matrices on the DOF pattern.
+ *   m  : mass                      (MassMatrix1c/2c, DmriFemLib.py:240-254)
+ *   k  : stiffness, D per cell tensor (Dcell, row-major 3x3) or per-compartment scalar Dc
+ *   r  : T2 relaxation, invT2 per cell or per compartment (FuncF_wBC, DmriFemLib.py:43-44)
+ *   gi : permeable interface kappa*int_F (u0-u1)(v0-v1) (icondition_wBC, DmriFemLib.py:47-50)
+ *   jx,jy,jz : int_T x_d phi_i phi_j  (GdotX, DmriFemLib.py:33-39; Jg = g.(Jx,Jy,Jz))
+ * kappa_f (per interface facet) overrides kappa when non-NULL.
+ * Returns 0, or a negative code when an entry falls outside the pattern.     */
+int orc_assemble(int64_t nv, const double *xyz, int64_t nc, const int32_t *tets, const int32_t *phase,
+                 const int32_t *dof_of, const double *Dc, const double *Dcell, const double *invT2c,
+                 const double *invT2cell, double kappa, const double *kappa_f, int64_t nf, const int32_t *fv,
+                 const int64_t *rowptr, const int32_t *col, int64_t nnz, double *m, double *k, double *r,
+                 double *gi, double *jx, double *jy, double *jz) {
+    double *outs[7] = {m, k, r, gi, jx, jy, jz};
+    for (int o = 0; o < 7; ++o) memset(outs[o], 0, sizeof(double) * (size_t)nnz);
+    for (int64_t t = 0; t < nc; ++t) {
+        int c = phase ? phase[t] : 0;
+        double X[4][3], G[4][3];
+        int32_t d[4];
+        for (int a = 0; a < 4; ++a) {
+            int32_t v = tets[4 * t + a];
+            d[a] = dof_of[c * nv + v];
+            for (int q = 0; q < 3; ++q) X[a][q] = xyz[3 * v + q];
+        }
+        double vol = tet_geom(X, G);
+        double D[9];
+        if (Dcell) memcpy(D, Dcell + 9 * t, sizeof(D));
+        else { memset(D, 0, sizeof(D)); D[0] = D[4] = D[8] = Dc[c]; }
+        double it2 = invT2cell ? invT2cell[t] : invT2c[c];
+        double S[3] = {0, 0, 0};
+        for (int a = 0; a < 4; ++a)
+            for (int q = 0; q < 3; ++q) S[q] += X[a][q];
+        for (int a = 0; a < 4; ++a) {
+            double DG[3];
+            for (int q = 0; q < 3; ++q) DG[q] = D[3 * q] * G[a][0] + D[3 * q + 1] * G[a][1] + D[3 * q + 2] * G[a][2];
+            for (int b = 0; b < 4; ++b) {
+                int64_t pos = csr_find(rowptr, col, d[a], d[b]);
+                if (pos < 0) return -3;
+                double dab = (a == b) ? 2.0 : 1.0;
+                double mab = vol * dab / 20.0;
+                m[pos] += mab;
+                r[pos] += it2 * mab;
+                k[pos] += vol * (G[b][0] * DG[0] + G[b][1] * DG[1] + G[b][2] * DG[2]);
+                double w = vol * dab / 120.0;
+                jx[pos] += w * (S[0] + X[a][0] + X[b][0]);
+                jy[pos] += w * (S[1] + X[a][1] + X[b][1]);
+                jz[pos] += w * (S[2] + X[a][2] + X[b][2]);
+            }
+        }
+    }
+    for (int64_t f = 0; f < nf; ++f) {
+        double X[3][3];
+        int32_t d0[3], d1[3];
+        for (int a = 0; a < 3; ++a) {
+            int32_t v = fv[3 * f + a];
+            d0[a] = dof_of[v];
+            d1[a] = dof_of[nv + v];
+            for (int q = 0; q < 3; ++q) X[a][q] = xyz[3 * v + q];
+        }
+        double e1[3], e2[3];
+        for (int q = 0; q < 3; ++q) { e1[q] = X[1][q] - X[0][q]; e2[q] = X[2][q] - X[0][q]; }
+        double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+        double area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+        double kap = kappa_f ? kappa_f[f] : kappa;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                double val = kap * area * ((a == b) ? 2.0 : 1.0) / 12.0;
+                int64_t p00 = csr_find(rowptr, col, d0[a], d0[b]);
+                int64_t p11 = csr_find(rowptr, col, d1[a], d1[b]);
+                int64_t p01 = csr_find(rowptr, col, d0[a], d1[b]);
+                int64_t p10 = csr_find(rowptr, col, d1[a], d0[b]);
+                if (p00 < 0 || p11 < 0 || p01 < 0 || p10 < 0) return -4;
+                gi[p00] += val;
+                gi[p11] += val;
+                gi[p01] -= val;
+                gi[p10] -= val;
+            }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Linear algebra helpers (complex vectors, real CSR value arrays).            */
+/* ------------------------------------------------------------------------- */
+
+/* y = (va + i*s*vj) x */
+static void spmv_c(int64_t n, const int64_t *rowptr, const int32_t *col, const double *va, const double *vj,
+                   double s, const cplx *x, cplx *y) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        cplx accA = 0, accJ = 0;
+        for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+            cplx xv = x[col[p]];
+            accA += va[p] * xv;
+            accJ += vj[p] * xv;
+        }
+        y[i] = accA + I * s * accJ;
+    }
+}
+
+/* deterministic x^H y */
+static cplx dotc(int64_t n, const cplx *x, const cplx *y) {
+    int64_t nch = (n + ORC_CHUNK - 1) / ORC_CHUNK;
+    cplx *part = (cplx *)malloc(sizeof(cplx) * (size_t)(nch > 0 ? nch : 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < nch; ++c) {
+        cplx s = 0;
+        int64_t e = (c + 1) * ORC_CHUNK < n ? (c + 1) * ORC_CHUNK : n;
+        for (int64_t i = c * ORC_CHUNK; i < e; ++i) s += conj(x[i]) * y[i];
+        part[c] = s;
+    }
+    cplx s = 0;
+    for (int64_t c = 0; c < nch; ++c) s += part[c];
+    free(part);
+    return s;
+}
+
+static double nrm2(int64_t n, const cplx *x) { return sqrt(creal(dotc(n, x, x))); }
+
+/* ------------------------------------------------------------------------- */
+/* Theta-scheme time stepping (MRI_simulation.solve, DmriFemLib.py:878-915;    */
+/* comri/one-comp/hpc-fenics-cpp/main.cpp:296-328).                            */
+/*   a  = m/dt + theta*l                      (time independent)               */
+/*   A_n = a + i*theta*q*fA[n]*jg                                              */
+/*   b_n = (m/dt - (1-theta)*l) u^n - i*(1-theta)*q*fB[n]*jg u^n               */
+/* Krylov: BiCGStab, Jacobi preconditioner diag(a) -- the diagonal of the real  */
+/* 2x2-block operator PETSc sees (the J coupling is off the block diagonal).    */
+/* Stop when ||r|| <= max(rtol*||b||, atol).  u (2n doubles, interleaved      */
+/* re/im) is the initial condition on entry and u^N on exit.                   */
+/* stats[0] = total iterations, stats[1] = max iterations in one step,         */
+/* stats[2] = max final relative residual, stats[3] = breakdown restarts.      */
+/* Returns 0, or -1 when a step did not converge within maxit.                 */
+/* ------------------------------------------------------------------------- */
+int orc_theta_solve(int64_t n, const int64_t *rowptr, const int32_t *col, const double *m, const double *l,
+                    const double *jg, double dt, double theta, double q, int nsteps, const double *fA,
+                    const double *fB, double rtol, double atol, int maxit, double *u_io, double *stats) {
+    cplx *u = (cplx *)u_io;
+    int64_t nnz = rowptr[n];
+    double *a = (double *)malloc(sizeof(double) * (size_t)nnz);
+    double *bre = (double *)malloc(sizeof(double) * (size_t)nnz);
+    double *dinv = (double *)malloc(sizeof(double) * (size_t)n);
+    cplx *b = (cplx *)malloc(sizeof(cplx) * (size_t)n), *r = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *rh = (cplx *)malloc(sizeof(cplx) * (size_t)n), *p = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *v = (cplx *)malloc(sizeof(cplx) * (size_t)n), *s = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *t = (cplx *)malloc(sizeof(cplx) * (size_t)n), *ph = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *sh = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    int rc = 0;
+    stats[0] = stats[1] = stats[2] = stats[3] = 0;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < nnz; ++e) {
+        a[e] = m[e] / dt + theta * l[e];
+        bre[e] = m[e] / dt - (1.0 - theta) * l[e];
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        double d = 0;
+        for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e)
+            if (col[e] == i) d = a[e];
+        dinv[i] = d != 0 ? 1.0 / d : 1.0;
+    }
+    for (int step = 0; step < nsteps; ++step) {
+        double sA = theta * q * fA[step];
+        double sB = -(1.0 - theta) * q * fB[step];
+        spmv_c(n, rowptr, col, bre, jg, sB, u, b);
+        double bn = nrm2(n, b);
+        double tol = rtol * bn > atol ? rtol * bn : atol;
+        /* warm start x0 = u^n (the previous solution, DOLFIN reuses the vector) */
+        spmv_c(n, rowptr, col, a, jg, sA, u, t);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) { r[i] = b[i] - t[i]; rh[i] = r[i]; p[i] = 0; v[i] = 0; }
+        double rn = nrm2(n, r);
+        cplx rho_old = 1, alpha = 1, omega = 1;
+        int it = 0;
+        while (rn > tol) {
+            if (it >= maxit) { rc = -1; break; }
+            cplx rho = dotc(n, rh, r);
+            if (cabs(rho) == 0.0 || cabs(omega) == 0.0) { /* breakdown: restart with r^ = r */
+                stats[3] += 1;
+#pragma omp parallel for schedule(static)
+                for (int64_t i = 0; i < n; ++i) { rh[i] = r[i]; p[i] = 0; v[i] = 0; }
+                rho = dotc(n, rh, r);
+                rho_old = 1; alpha = 1; omega = 1;
+            }
+            cplx beta = (rho / rho_old) * (alpha / omega);
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) {
+                p[i] = r[i] + beta * (p[i] - omega * v[i]);
+                ph[i] = dinv[i] * p[i];
+            }
+            spmv_c(n, rowptr, col, a, jg, sA, ph, v);
+            cplx rv = dotc(n, rh, v);
+            alpha = rho / rv;
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) {
+                s[i] = r[i] - alpha * v[i];
+                sh[i] = dinv[i] * s[i];
+            }
+            spmv_c(n, rowptr, col, a, jg, sA, sh, t);
+            cplx ts = dotc(n, t, s);
+            double tt = creal(dotc(n, t, t));
+            omega = tt > 0 ? ts / tt : 0;
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) {
+                u[i] += alpha * ph[i] + omega * sh[i];
+                r[i] = s[i] - omega * t[i];
+            }
+            rho_old = rho;
+            rn = nrm2(n, r);
+            ++it;
+        }
+        stats[0] += it;
+        if (it > stats[1]) stats[1] = it;
+        double rel = bn > 0 ? rn / bn : rn;
+        if (rel > stats[2]) stats[2] = rel;
+        if (rc) break;
+    }
+    free(a); free(bre); free(dinv); free(b); free(r); free(rh); free(p); free(v); free(s); free(t);
+    free(ph); free(sh);
+    return rc;
+}
+
+/* y = (va + i*s*vj) x, exported for tests (x, y interleaved complex). */
+void orc_spmv(int64_t n, const int64_t *rowptr, const int32_t *col, const double *va, const double *vj, double s,
+              const double *x, double *y) {
+    spmv_c(n, rowptr, col, va, vj, s, (const cplx *)x, (cplx *)y);
+}
+
+/* Signal: sum_i w_i Re(u_i) with w = M 1 (PostProcessing, DmriFemLib.py:921-931,971;
+ * Comp_Sig3D.ufl).  Fixed-chunk summation.  out[0] = sum w Re u, out[1] = sum w Im u. */
+void orc_signal(int64_t n, const double *w, const double *u_io, double *out) {
+    const cplx *u = (const cplx *)u_io;
+    int64_t nch = (n + ORC_CHUNK - 1) / ORC_CHUNK;
+    double re = 0, im = 0;
+    for (int64_t c = 0; c < nch; ++c) {
+        double sr = 0, si = 0;
+        int64_t e = (c + 1) * ORC_CHUNK < n ? (c + 1) * ORC_CHUNK : n;
+        for (int64_t i = c * ORC_CHUNK; i < e; ++i) { sr += w[i] * creal(u[i]); si += w[i] * cimag(u[i]); }
+        re += sr;
+        im += si;
+    }
+    out[0] = re;
+    out[1] = im;
+}

@@ -1,0 +1,262 @@
+"""Python face of the CPU oracle (liboracle.so built from dmri_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and by the
+``cpu_baseline`` leg of bench.py, always as the checker or the timed CPU baseline, never
+as part of the product path.
+
+Besides the ctypes bindings this module restates the reference's time-sequence logic:
+
+* PGSE profile as the sympy Piecewise of GCloudDmriSolver.py:188-193 (exclusive at s=T)
+  and as the comri ``FT`` (comri/one-comp/hpc-fenics-cpp/main.cpp:87-92, inclusive at
+  t=Delta+delta);
+* the GCloud loop ``while t < T + k`` with t accumulated by ``+=`` and the right-hand side
+  lagged one step (DmriFemLib.py:886-911; first step uses f(0) on both sides);
+* the comri loop ``while (t < T + dt)`` with the same f on both sides (main.cpp:296-318);
+* q = sqrt(b / int_0^T F(s)^2 ds) with int F^2 = delta^2 (Delta - delta/3) for PGSE
+  (DmriFemLib.py:826-842; main.cpp:187-191).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+GAMMA_RATIO = 2.675e8  # DmriFemLib.py:680-686
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        i64, i32p, dp, i64p = ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)
+        L.orc_dof_map.restype = i64
+        L.orc_dof_map.argtypes = [i64, i64, i32p, i32p, ctypes.c_int, i32p]
+        L.orc_interface_facets.restype = i64
+        L.orc_interface_facets.argtypes = [i64, i32p, i32p, i32p, i32p, i64]
+        L.orc_boundary_facets.restype = i64
+        L.orc_boundary_facets.argtypes = [i64, i32p, i32p, i32p, i64]
+        L.orc_pattern.restype = i64
+        L.orc_pattern.argtypes = [i64, i64, i64, i32p, i32p, i32p, i64, i32p, i64p, i32p]
+        L.orc_assemble.restype = ctypes.c_int
+        L.orc_assemble.argtypes = [i64, dp, i64, i32p, i32p, i32p, dp, dp, dp, dp, ctypes.c_double, dp, i64, i32p,
+                                   i64p, i32p, i64, dp, dp, dp, dp, dp, dp, dp]
+        L.orc_theta_solve.restype = ctypes.c_int
+        L.orc_theta_solve.argtypes = [i64, i64p, i32p, dp, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, dp]
+        L.orc_spmv.restype = None
+        L.orc_spmv.argtypes = [i64, i64p, i32p, dp, dp, ctypes.c_double, dp, dp]
+        L.orc_signal.restype = None
+        L.orc_signal.argtypes = [i64, dp, dp, dp]
+        L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, ct):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+# ------------------------------------------------------------------------------------
+# time sequences
+# ------------------------------------------------------------------------------------
+def pgse_piecewise(delta, Delta):
+    """sympy Piecewise of GCloudDmriSolver.py:188-193 evaluated in floats."""
+    T = Delta + delta
+
+    def f(s):
+        if s < delta:
+            return 1.0
+        if s < Delta:
+            return 0.0
+        if s < T:
+            return -1.0
+        return 0.0
+
+    return f
+
+
+def pgse_comri(delta, Delta):
+    """comri FT (main.cpp:87-92)."""
+
+    def f(t):
+        return 1.0 * (0 <= t < delta) - 1.0 * (Delta <= t <= Delta + delta)
+
+    return f
+
+
+def q_from_b_pgse(b, delta, Delta):
+    return math.sqrt(b) / math.sqrt(delta * delta * (Delta - delta / 3.0))
+
+
+def gcloud_sequence(f, T, k):
+    """(fA, fB) of MRI_simulation.solve: loop while t < T + k, t += k, RHS lagged
+    (tp starts at 0), DmriFemLib.py:886-911."""
+    fA, fB = [], []
+    t, tp = 0.0, 0.0
+    while t < T + k:
+        fA.append(f(t))
+        fB.append(f(tp))
+        tp = t
+        t += k
+    return np.asarray(fA), np.asarray(fB)
+
+
+def lagged_sequence_until(f, T, k):
+    """The inline loop of ConvergenceTest.ipynb cell 10: while t < T, lagged RHS
+    starting from f(0)."""
+    fA, fB = [], []
+    t = 0.0
+    fprev = f(t)
+    while t < T:
+        ft = f(t)
+        fA.append(ft)
+        fB.append(fprev)
+        fprev = ft
+        t += k
+    return np.asarray(fA), np.asarray(fB)
+
+
+def comri_sequence(f, T, dt, until_T=False):
+    """comri loops: same f on both sides; while (t < T + dt) (one-/two-comp main.cpp:296,854)
+    or while (t < T) (multilayer main.cpp:301)."""
+    fA = []
+    t = 0.0
+    end = T if until_T else T + dt
+    while t < end:
+        fA.append(f(t))
+        t += dt
+    fA = np.asarray(fA)
+    return fA, fA.copy()
+
+
+# ------------------------------------------------------------------------------------
+# assembled system
+# ------------------------------------------------------------------------------------
+class System:
+    """Oracle-assembled matrices on the DOF pattern of a mesh.
+
+    D: scalar or per-compartment pair; invT2 likewise; kappa scalar (interface)."""
+
+    def __init__(self, xyz, tets, phase=None, D=3e-3, invT2=1e-16, kappa=0.0, Dcell=None, invT2cell=None,
+                 kappa_facet=None):
+        L = lib()
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.tets = np.ascontiguousarray(tets, dtype=np.int32)
+        self.phase = None if phase is None else np.ascontiguousarray(phase, dtype=np.int32)
+        self.ncomp = 1 if self.phase is None else 2
+        nv, nc = self.xyz.shape[0], self.tets.shape[0]
+        self.nv, self.nc = nv, nc
+        self.dof_of = np.zeros(self.ncomp * nv, dtype=np.int32)
+        ndof = L.orc_dof_map(nv, nc, _p(self.tets, ctypes.c_int32), _p(self.phase, ctypes.c_int32), self.ncomp,
+                             _p(self.dof_of, ctypes.c_int32))
+        if ndof < 0:
+            raise ValueError("bad mesh / phase")
+        self.ndof = int(ndof)
+        if self.phase is not None:
+            nf = L.orc_interface_facets(nc, _p(self.tets, ctypes.c_int32), _p(self.phase, ctypes.c_int32), None,
+                                        None, 0)
+            self.fv = np.zeros((max(nf, 1), 3), dtype=np.int32)
+            self.fc = np.zeros((max(nf, 1), 2), dtype=np.int32)
+            L.orc_interface_facets(nc, _p(self.tets, ctypes.c_int32), _p(self.phase, ctypes.c_int32),
+                                   _p(self.fv, ctypes.c_int32), _p(self.fc, ctypes.c_int32), nf)
+            self.fv, self.fc = self.fv[:nf], self.fc[:nf]
+        else:
+            self.fv = np.zeros((0, 3), dtype=np.int32)
+            self.fc = np.zeros((0, 2), dtype=np.int32)
+        nf = self.fv.shape[0]
+        self.rowptr = np.zeros(self.ndof + 1, dtype=np.int64)
+        fvp = _p(self.fv, ctypes.c_int32) if nf else None
+        nnz = L.orc_pattern(self.ndof, nv, nc, _p(self.tets, ctypes.c_int32), _p(self.phase, ctypes.c_int32),
+                            _p(self.dof_of, ctypes.c_int32), nf, fvp, _p(self.rowptr, ctypes.c_int64), None)
+        if nnz < 0:
+            raise MemoryError("oracle pattern")
+        self.col = np.zeros(nnz, dtype=np.int32)
+        L.orc_pattern(self.ndof, nv, nc, _p(self.tets, ctypes.c_int32), _p(self.phase, ctypes.c_int32),
+                      _p(self.dof_of, ctypes.c_int32), nf, fvp, _p(self.rowptr, ctypes.c_int64),
+                      _p(self.col, ctypes.c_int32))
+        self.nnz = int(nnz)
+        Dc = np.ascontiguousarray(np.broadcast_to(np.asarray(D, dtype=np.float64), (2,)))
+        iT = np.ascontiguousarray(np.broadcast_to(np.asarray(invT2, dtype=np.float64), (2,)))
+        Dcell = None if Dcell is None else np.ascontiguousarray(Dcell, dtype=np.float64).reshape(nc, 9)
+        invT2cell = None if invT2cell is None else np.ascontiguousarray(invT2cell, dtype=np.float64)
+        kappa_facet = None if kappa_facet is None else np.ascontiguousarray(kappa_facet, dtype=np.float64)
+        self.mats = {k: np.zeros(nnz) for k in ("m", "k", "r", "gi", "jx", "jy", "jz")}
+        M = self.mats
+        rc = L.orc_assemble(nv, _p(self.xyz, ctypes.c_double), nc, _p(self.tets, ctypes.c_int32),
+                            _p(self.phase, ctypes.c_int32), _p(self.dof_of, ctypes.c_int32), _p(Dc, ctypes.c_double),
+                            _p(Dcell, ctypes.c_double), _p(iT, ctypes.c_double), _p(invT2cell, ctypes.c_double),
+                            float(kappa), _p(kappa_facet, ctypes.c_double), nf, fvp,
+                            _p(self.rowptr, ctypes.c_int64), _p(self.col, ctypes.c_int32), nnz,
+                            *[_p(M[k], ctypes.c_double) for k in ("m", "k", "r", "gi", "jx", "jy", "jz")])
+        if rc != 0:
+            raise RuntimeError(f"oracle assembly failed ({rc})")
+        # signal weights w = M 1 (row sums of the mass matrix on live DOFs)
+        self.w = np.add.reduceat(M["m"], self.rowptr[:-1]) if nnz else np.zeros(self.ndof)
+        self.volume = float(self.w.sum())
+
+    # -- helpers ------------------------------------------------------------------
+    def l(self):
+        return self.mats["k"] + self.mats["r"] + self.mats["gi"]
+
+    def jg(self, g):
+        g = np.asarray(g, dtype=np.float64)
+        g = g / np.linalg.norm(g)
+        return g[0] * self.mats["jx"] + g[1] * self.mats["jy"] + g[2] * self.mats["jz"]
+
+    def dense(self, name_or_vals):
+        vals = self.mats[name_or_vals] if isinstance(name_or_vals, str) else name_or_vals
+        A = np.zeros((self.ndof, self.ndof))
+        rows = np.repeat(np.arange(self.ndof), np.diff(self.rowptr))
+        A[rows, self.col] = vals
+        return A
+
+    def initial_condition(self):
+        u = np.zeros(self.ndof, dtype=np.complex128)
+        u[:] = 1.0
+        return u
+
+    def solve(self, gdir, q, dt, fA, fB, theta=0.5, rtol=1e-13, atol=0.0, maxit=100000, u0=None, l=None):
+        L = lib()
+        fA = np.ascontiguousarray(fA, dtype=np.float64)
+        fB = np.ascontiguousarray(fB, dtype=np.float64)
+        u = self.initial_condition() if u0 is None else np.array(u0, dtype=np.complex128)
+        uv = u.view(np.float64)
+        lv = np.ascontiguousarray(self.l() if l is None else l)
+        jg = np.ascontiguousarray(self.jg(gdir))
+        stats = np.zeros(4)
+        rc = L.orc_theta_solve(self.ndof, _p(self.rowptr, ctypes.c_int64), _p(self.col, ctypes.c_int32),
+                               _p(self.mats["m"], ctypes.c_double), _p(lv, ctypes.c_double), _p(jg, ctypes.c_double),
+                               float(dt), float(theta), float(q), len(fA), _p(fA, ctypes.c_double),
+                               _p(fB, ctypes.c_double), float(rtol), float(atol), int(maxit), _p(uv, ctypes.c_double),
+                               _p(stats, ctypes.c_double))
+        if rc != 0:
+            raise RuntimeError("oracle Krylov solve did not converge")
+        sig = np.zeros(2)
+        L.orc_signal(self.ndof, _p(self.w, ctypes.c_double), _p(uv, ctypes.c_double), _p(sig, ctypes.c_double))
+        return sig[0] / self.volume, u, {"iters": stats[0], "max_iters": stats[1], "max_relres": stats[2],
+                                         "restarts": stats[3], "signal_raw": sig.copy()}
