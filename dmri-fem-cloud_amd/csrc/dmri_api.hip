@@ -60,6 +60,22 @@ T read_scalar(const T *d, hipStream_t s) {
 
 constexpr int kEventPool = 512;
 
+struct SolverWork {
+    DevArray<double2> x, r, rh, p, v, t, b;
+    DevArray<double> sc, vals, bre, part, tot, coef, sig, spart;
+    DevArray<KrylovState> st;
+    DevArray<int> all_done, mask;
+    KrylovState *h_st = nullptr;
+    int *h_flag = nullptr;
+    SolverWork() = default;
+    SolverWork(const SolverWork &) = delete;
+    SolverWork &operator=(const SolverWork &) = delete;
+    ~SolverWork() {
+        if (h_st) (void)hipHostFree(h_st);
+        if (h_flag) (void)hipHostFree(h_flag);
+    }
+};
+
 }  // namespace
 
 struct dmri_ctx {
@@ -88,17 +104,14 @@ struct dmri_ctx {
     DevArray<double> val[7];
     DevArray<double> w;
     // solver workspace
-    DevArray<double2> aj, x, r, rh, p, v, t;
-    DevArray<double> bre, sc, part, dsig;
-    DevArray<KrylovState> st;
-    DevArray<unsigned> counters;
-    KrylovState *h_st = nullptr;
+    SolverWork work;
+    DevArray<double> part, dsig;
+    int last_S = 1;
     bool solved = false;
     hipEvent_t ev0[kEventPool], ev1[kEventPool];
     bool events_made = false;
 
     ~dmri_ctx() {
-        if (h_st) (void)hipHostFree(h_st);
         if (events_made)
             for (int i = 0; i < kEventPool; ++i) {
                 (void)hipEventDestroy(ev0[i]);
@@ -321,11 +334,9 @@ void assemble_impl(dmri_ctx *c) {
         launch_diag_and_weights(c->ndof, c->rowptr.p, c->col.p, c->val[DMRI_MAT_M].p, c->diag.p, c->w.p, missing.p,
                                 s);
         require(read_scalar(missing.p, s) == 0, DMRI_ERR_MESH, "missing diagonal entry");
-        c->part.alloc(std::max<size_t>(c->part.n, 4 * (size_t)kVecBlocks + 16));
-        c->counters.alloc(8);
-        c->counters.zero(s);
-        c->dsig.alloc(4);
-        launch_sum(c->w.p, c->ndof, c->part.p, c->counters.p, c->dsig.p, s);
+        c->part.alloc((size_t)kMaxGrid * kNV);
+        c->dsig.alloc(kNV);
+        launch_sum(c->w.p, c->ndof, c->part.p, c->dsig.p, s);
         c->volume = read_scalar(c->dsig.p, s);
     }
     DMRI_HIP(hipEventRecord(e1, s));
@@ -339,30 +350,43 @@ void assemble_impl(dmri_ctx *c) {
     c->solved = false;
 }
 
-void ensure_solver_workspace(dmri_ctx *c) {
+// Lanes per row for S signals when the caller does not choose: wide rows for one signal,
+// one lane per (row, signal) once the signals alone fill a quarter wave.
+int auto_lanes(int S) {
+    if (S >= 8) return 1;
+    if (S >= 4) return 2;
+    if (S >= 2) return 4;
+    return 8;
+}
+
+void ensure_group_workspace(dmri_ctx *c, const SolveGeom &g, int nsteps, bool per_gdir) {
     hipStream_t s = c->stream;
-    const size_t n = c->ndof;
-    if (c->x.n != n) {
-        c->x.alloc(n);
-        c->r.alloc(n);
-        c->rh.alloc(n);
-        c->p.alloc(n);
-        c->v.alloc(n);
-        c->t.alloc(n);
-        c->sc.alloc(n);
+    const size_t nS = (size_t)c->ndof * g.S;
+    SolverWork &w = c->work;
+    if (w.x.n != nS) {
+        w.x.alloc(nS);
+        w.r.alloc(nS);
+        w.rh.alloc(nS);
+        w.p.alloc(nS);
+        w.v.alloc(nS);
+        w.t.alloc(nS);
+        w.b.alloc(nS);
     }
-    if (c->aj.n != (size_t)c->nnz) {
-        c->aj.alloc(c->nnz);
-        c->bre.alloc(c->nnz);
-    }
-    const size_t nblk = (n + kRowsPerBlock - 1) / kRowsPerBlock;
-    const size_t need = 4 * std::max<size_t>(nblk, kVecBlocks) + 16;
-    if (c->part.n < need) c->part.alloc(need);
-    if (!c->st.p) {
-        c->st.alloc(1);
-        c->st.zero(s);
-    }
-    if (!c->h_st) DMRI_HIP(hipHostMalloc((void **)&c->h_st, sizeof(KrylovState), hipHostMallocDefault));
+    if (w.sc.n != (size_t)c->ndof) w.sc.alloc(c->ndof);
+    const size_t nval = (size_t)c->nnz * (per_gdir ? 4 : 2);
+    if (w.vals.n != nval) w.vals.alloc(nval);
+    if (w.bre.n != (size_t)c->nnz) w.bre.alloc(c->nnz);
+    const size_t np = (size_t)std::max(g.grid, g.vgrid) * g.S * kNV;
+    if (w.part.n < 3 * np) w.part.alloc(3 * np);
+    if (w.tot.n < (size_t)3 * g.S * kNV) w.tot.alloc(3 * (size_t)g.S * kNV);
+    if (w.coef.n < (size_t)2 * nsteps * g.S * 3) w.coef.alloc(std::max<size_t>(1, (size_t)2 * nsteps * g.S * 3));
+    if (w.st.n < (size_t)g.S) w.st.alloc(kMaxSignals);
+    if (!w.all_done.p) w.all_done.alloc(1);
+    if (!w.mask.p) w.mask.alloc(kMaxSignals);
+    if (w.sig.n < (size_t)kMaxSignals * kNV) w.sig.alloc(kMaxSignals * kNV);
+    if (w.spart.n < (size_t)kMaxGrid * kMaxSignals * kNV) w.spart.alloc((size_t)kMaxGrid * kMaxSignals * kNV);
+    if (!w.h_st) DMRI_HIP(hipHostMalloc((void **)&w.h_st, sizeof(KrylovState) * kMaxSignals, hipHostMallocDefault));
+    if (!w.h_flag) DMRI_HIP(hipHostMalloc((void **)&w.h_flag, sizeof(int) * 4, hipHostMallocDefault));
     if (!c->events_made) {
         for (int i = 0; i < kEventPool; ++i) {
             DMRI_HIP(hipEventCreate(&c->ev0[i]));
@@ -370,72 +394,110 @@ void ensure_solver_workspace(dmri_ctx *c) {
         }
         c->events_made = true;
     }
+    (void)s;
 }
 
-void solve_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts, double *sig, dmri_stats *stats) {
-    require(c->assembled, DMRI_ERR_STATE, "dmri_solve: call dmri_assemble first");
-    require(seq != nullptr && opts != nullptr && sig != nullptr, DMRI_ERR_ARG, "dmri_solve: null argument");
-    require(seq->nsteps >= 0, DMRI_ERR_ARG, "nsteps < 0");
-    require(seq->nsteps == 0 || (seq->f_implicit && seq->f_explicit), DMRI_ERR_ARG, "null time profile");
-    require(seq->dt > 0 && std::isfinite(seq->dt), DMRI_ERR_ARG, "dt must be > 0");
-    require(seq->theta >= 0.0 && seq->theta <= 1.0, DMRI_ERR_ARG, "theta must lie in [0, 1]");
-    require(std::isfinite(seq->q), DMRI_ERR_ARG, "q is not finite");
-    const double gn = std::sqrt(seq->gdir[0] * seq->gdir[0] + seq->gdir[1] * seq->gdir[1] + seq->gdir[2] * seq->gdir[2]);
-    // |g| = 0 is rejected by the reference (set_gradient_dir, DmriFemLib.py:813-823)
-    require(gn > 1e-10, DMRI_ERR_ARG, "|g|=0: gradient direction must be non-zero");
-    require(opts->maxit > 0, DMRI_ERR_ARG, "maxit must be > 0");
-    require(opts->rtol >= 0 && opts->atol >= 0 && (opts->rtol > 0 || opts->atol > 0), DMRI_ERR_ARG,
-            "need rtol > 0 or atol > 0");
-    for (int i = 0; i < seq->nsteps; ++i)
-        require(std::isfinite(seq->f_implicit[i]) && std::isfinite(seq->f_explicit[i]), DMRI_ERR_ARG,
-                "time profile has a non-finite value");
-    set_device(c);
-    ensure_solver_workspace(c);
+// One launch group of S <= 64 signals sharing the time profile.  q[s], g[3*s] (unit).
+void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_gdir, const dmri_seq *seq,
+                 const dmri_solver_opts *opts, double *sig, dmri_stats &st) {
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
-    const double g[3] = {seq->gdir[0] / gn, seq->gdir[1] / gn, seq->gdir[2] / gn};
-    const double theta = seq->theta, dt = seq->dt, q = seq->q;
+    const int lanes = opts->lanes_per_row > 0 ? opts->lanes_per_row : auto_lanes(S);
+    const SolveGeom geo = make_geom(n, lanes, S);
+    const int N = seq->nsteps;
+    ensure_group_workspace(c, geo, N, per_gdir);
+    SolverWork &w = c->work;
     auto &V = c->val;
+    const double theta = seq->theta, dt = seq->dt;
 
     launch_prep_scale(n, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, dt, theta,
-                      c->sc.p, s);
-    launch_prep_values(n, c->rowptr.p, c->col.p, c->sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
-                       V[DMRI_MAT_I].p, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, g[0], g[1], g[2], dt,
-                       theta, c->aj.p, c->bre.p, s);
-    launch_init_x(n, c->sc.p, c->x.p, s);
-    DMRI_HIP(hipStreamSynchronize(s));
+                      w.sc.p, s);
+    launch_prep_values(n, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
+                       V[DMRI_MAT_I].p, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, g[0], g[1],
+                       g[2], dt, theta, w.vals.p, w.bre.p, s);
+    // per-step imaginary coefficients: implicit side theta q f_A(n) (g), explicit side -(1-theta) q f_B(n) (g)
+    std::vector<double> coef((size_t)2 * std::max(N, 1) * S * 3, 0.0);
+    double *cA = coef.data(), *cB = coef.data() + (size_t)std::max(N, 1) * S * 3;
+    for (int k = 0; k < N; ++k)
+        for (int j = 0; j < S; ++j) {
+            const double a = theta * q[j] * seq->f_implicit[k];
+            const double b = -(1.0 - theta) * q[j] * seq->f_explicit[k];
+            for (int d = 0; d < 3; ++d) {
+                const double gd = per_gdir ? g[3 * j + d] : (d == 0 ? 1.0 : 0.0);
+                cA[((size_t)k * S + j) * 3 + d] = a * gd;
+                cB[((size_t)k * S + j) * 3 + d] = b * gd;
+            }
+        }
+    DMRI_HIP(hipMemcpyAsync(w.coef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
+    launch_init_x(n, S, w.sc.p, w.x.p, s);
 
-    dmri_stats st{};
-    st.spmv_bytes = (double)c->nnz * 20.0 + 4.0 * (double)(n + 1) + 48.0 * (double)n;
-    const auto t0 = std::chrono::steady_clock::now();
-    int chunk = 8;
+    // Finalize kernels reduce the partials unless one signal's few partials can be summed by
+    // every consumer block directly (mixed first/later iterations need per-signal counts).
+    const bool use_fin = S > 1 || std::max(geo.grid, geo.vgrid) > 512;
+    const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
+    SolveArgs a{};
+    a.n = n;
+    a.rowptr = c->rowptr.p;
+    a.col = c->col.p;
+    a.vals = w.vals.p;
+    a.bre = w.bre.p;
+    a.per_gdir = per_gdir;
+    a.x = w.x.p;
+    a.r = w.r.p;
+    a.rh = w.rh.p;
+    a.p = w.p.p;
+    a.v = w.v.p;
+    a.t = w.t.p;
+    a.b = w.b.p;
+    a.partU = w.part.p;
+    a.partV = w.part.p + np;
+    a.partT = w.part.p + 2 * np;
+    a.totU = use_fin ? w.tot.p : nullptr;
+    a.totV = use_fin ? w.tot.p + S * kNV : nullptr;
+    a.totT = use_fin ? w.tot.p + 2 * S * kNV : nullptr;
+    a.st = w.st.p;
+    a.all_done = w.all_done.p;
+    a.rtol = opts->rtol;
+    a.atol = opts->atol;
+    a.maxit = opts->maxit;
+    st.spmv_bytes = (double)c->nnz * (4.0 + 8.0 * (per_gdir ? 4 : 2)) + 4.0 * (double)(n + 1) +
+                    48.0 * (double)n * S;
     const bool prof = opts->profile != 0;
-    for (int step = 0; step < seq->nsteps; ++step) {
-        const double sA = theta * q * seq->f_implicit[step];
-        const double sB = -(1.0 - theta) * q * seq->f_explicit[step];
-        int step_iters = 0, restarts = 0;
-        launch_rhs_residual(n, c->rowptr.p, c->col.p, c->aj.p, c->bre.p, sA, sB, c->x.p, c->r.p, c->rh.p, opts->rtol,
-                            opts->atol, opts->maxit, c->part.p, c->st.p, s);
-        int iter_before = 0;
+    int chunk = 8;
+    std::vector<KrylovState> zero_st(S);
+    for (int step = 0; step < N; ++step) {
+        a.coefA = w.coef.p + (size_t)step * S * 3;
+        a.coefB = w.coef.p + (size_t)std::max(N, 1) * S * 3 + (size_t)step * S * 3;
+        std::memset(zero_st.data(), 0, sizeof(KrylovState) * S);
+        DMRI_HIP(hipMemcpyAsync(w.st.p, zero_st.data(), sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
+        DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
+        launch_rhs_residual(geo, a, false, nullptr, s);
+        int it = 0, it_before = 0, restarts = 0;
         for (;;) {
             const int nl = std::min(chunk, kEventPool / 2);
-            for (int k = 0; k < nl; ++k) {
-                launch_bicg_p(n, c->r.p, c->p.p, c->v.p, c->st.p, s);
+            for (int k = 0; k < nl; ++k, ++it) {
+                launch_bicg_p(geo, a, it, s);
                 if (prof) DMRI_HIP(hipEventRecord(c->ev0[2 * k], s));
-                launch_spmv_v(n, c->rowptr.p, c->col.p, c->aj.p, sA, c->p.p, c->v.p, c->rh.p, c->part.p, c->st.p, s);
+                launch_spmv_v(geo, a, it, s);
                 if (prof) {
                     DMRI_HIP(hipEventRecord(c->ev1[2 * k], s));
                     DMRI_HIP(hipEventRecord(c->ev0[2 * k + 1], s));
                 }
-                launch_spmv_t(n, c->rowptr.p, c->col.p, c->aj.p, sA, c->r.p, c->v.p, c->t.p, c->part.p, c->st.p, s);
+                launch_spmv_t(geo, a, it, s);
                 if (prof) DMRI_HIP(hipEventRecord(c->ev1[2 * k + 1], s));
-                launch_bicg_update(n, c->x.p, c->p.p, c->r.p, c->v.p, c->t.p, c->rh.p, c->part.p, c->st.p, s);
+                launch_bicg_update(geo, a, it, s);
             }
-            DMRI_HIP(hipMemcpyAsync(c->h_st, c->st.p, sizeof(KrylovState), hipMemcpyDeviceToHost, s));
+            DMRI_HIP(hipMemcpyAsync(w.h_flag, w.all_done.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            DMRI_HIP(hipMemcpyAsync(w.h_st, w.st.p, sizeof(KrylovState) * S, hipMemcpyDeviceToHost, s));
             DMRI_HIP(hipStreamSynchronize(s));
-            const KrylovState &h = *c->h_st;
+            const bool all = w.h_flag[0] != 0;
+            int it_last = it;
+            if (all) {
+                it_last = 0;
+                for (int j = 0; j < S; ++j) it_last = std::max(it_last, w.h_st[j].iters);
+            }
             if (prof) {
-                const int real = std::min(nl, h.iter - iter_before);
+                const int real = std::max(0, std::min(nl, it_last - it_before));
                 for (int k = 0; k < real; ++k)
                     for (int j = 0; j < 2; ++j) {
                         float ms = 0;
@@ -444,46 +506,105 @@ void solve_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts, 
                         st.spmv_launches += 1;
                     }
             }
-            iter_before = h.iter;
-            if (h.done) {
-                if (h.breakdown && !(h.rnorm2 <= h.tol2)) {
-                    ++restarts;
-                    require(restarts <= 20, DMRI_ERR_NOCONV, "BiCGStab: repeated breakdown at step " +
-                                                                 std::to_string(step));
-                    step_iters += h.iter;
-                    launch_rhs_residual(n, c->rowptr.p, c->col.p, c->aj.p, c->bre.p, sA, sB, c->x.p, c->r.p, c->rh.p,
-                                        opts->rtol, opts->atol, opts->maxit, c->part.p, c->st.p, s);
-                    iter_before = 0;
+            it_before = it;
+            if (!all) {
+                chunk = std::max(2, chunk / 4);
+                continue;
+            }
+            // every signal stopped: converged, broke down, or hit maxit
+            bool any_restart = false;
+            std::vector<int> mask(S, 0);
+            for (int j = 0; j < S; ++j) {
+                KrylovState &h = w.h_st[j];
+                if (h.conv) continue;
+                if (h.breakdown) {
+                    mask[j] = 1;
+                    any_restart = true;
+                    h.done = 0;
+                    h.breakdown = 0;
+                    h.it0 = it;
                     continue;
                 }
-                if (!(h.rnorm2 <= h.tol2))
-                    throw Error(DMRI_ERR_NOCONV, "BiCGStab did not converge in " + std::to_string(opts->maxit) +
-                                                     " iterations at step " + std::to_string(step) +
-                                                     " (relres " + std::to_string(std::sqrt(h.rnorm2 / h.bnorm2)) +
-                                                     ")");
-                break;
+                throw Error(DMRI_ERR_NOCONV, "BiCGStab did not converge in " + std::to_string(opts->maxit) +
+                                                 " iterations at step " + std::to_string(step) + " (signal " +
+                                                 std::to_string(j) + ", relres " +
+                                                 std::to_string(std::sqrt(h.rnorm2 / std::max(h.bnorm2, 1e-300))) +
+                                                 ")");
             }
-            if (h.iter >= opts->maxit)
-                throw Error(DMRI_ERR_NOCONV, "BiCGStab did not converge at step " + std::to_string(step));
-            chunk = std::max(2, chunk / 4);
+            if (!any_restart) break;
+            ++restarts;
+            require(restarts <= 20, DMRI_ERR_NOCONV, "BiCGStab: repeated breakdown at step " + std::to_string(step));
+            DMRI_HIP(hipMemcpyAsync(w.st.p, w.h_st, sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
+            DMRI_HIP(hipMemcpyAsync(w.mask.p, mask.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
+            DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
+            launch_rhs_residual(geo, a, true, w.mask.p, s);
+            DMRI_HIP(hipStreamSynchronize(s));
         }
-        const KrylovState &h = *c->h_st;
-        step_iters += h.iter;
-        st.iters_total += step_iters;
-        st.iters_max = std::max(st.iters_max, step_iters);
+        int step_max = 0;
+        for (int j = 0; j < S; ++j) {
+            const KrylovState &h = w.h_st[j];
+            st.iters_total += h.iters;
+            step_max = std::max(step_max, h.iters);
+            const double rel = h.bnorm2 > 0 ? std::sqrt(h.rnorm2 / h.bnorm2) : std::sqrt(h.rnorm2);
+            st.relres_max = std::max(st.relres_max, rel);
+        }
+        st.iters_max = std::max(st.iters_max, step_max);
         st.restarts += restarts;
-        const double rel = h.bnorm2 > 0 ? std::sqrt(h.rnorm2 / h.bnorm2) : std::sqrt(h.rnorm2);
-        st.relres_max = std::max(st.relres_max, rel);
-        chunk = std::max(1, h.iter);
+        chunk = std::max(1, step_max);
     }
-    launch_signal(n, c->w.p, c->sc.p, c->x.p, c->part.p, c->counters.p + 1, c->dsig.p, s);
-    double hs[2];
-    DMRI_HIP(hipMemcpyAsync(hs, c->dsig.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    launch_signal(n, S, c->w.p, w.sc.p, w.x.p, w.spart.p, w.sig.p, s);
+    std::vector<double> hs((size_t)S * kNV);
+    DMRI_HIP(hipMemcpyAsync(hs.data(), w.sig.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
     DMRI_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < S; ++j) {
+        sig[2 * j] = hs[(size_t)j * kNV] / c->volume;
+        sig[2 * j + 1] = hs[(size_t)j * kNV + 1] / c->volume;
+    }
+    c->last_S = S;
+}
+
+void validate_seq(const dmri_seq *seq, const dmri_solver_opts *opts) {
+    require(seq != nullptr && opts != nullptr, DMRI_ERR_ARG, "null argument");
+    require(seq->nsteps >= 0, DMRI_ERR_ARG, "nsteps < 0");
+    require(seq->nsteps == 0 || (seq->f_implicit && seq->f_explicit), DMRI_ERR_ARG, "null time profile");
+    require(seq->dt > 0 && std::isfinite(seq->dt), DMRI_ERR_ARG, "dt must be > 0");
+    require(seq->theta >= 0.0 && seq->theta <= 1.0, DMRI_ERR_ARG, "theta must lie in [0, 1]");
+    require(opts->maxit > 0, DMRI_ERR_ARG, "maxit must be > 0");
+    require(opts->rtol >= 0 && opts->atol >= 0 && (opts->rtol > 0 || opts->atol > 0), DMRI_ERR_ARG,
+            "need rtol > 0 or atol > 0");
+    require(opts->lanes_per_row >= 0, DMRI_ERR_ARG, "lanes_per_row must be >= 0");
+    for (int i = 0; i < seq->nsteps; ++i)
+        require(std::isfinite(seq->f_implicit[i]) && std::isfinite(seq->f_explicit[i]), DMRI_ERR_ARG,
+                "time profile has a non-finite value");
+}
+
+// nsig signals with per-signal q and gradient direction, in launch groups of <= 64.
+void solve_batched_impl(dmri_ctx *c, int nsig, const double *q, const double *gdir, const dmri_seq *seq,
+                        const dmri_solver_opts *opts, double *sig, dmri_stats *stats) {
+    require(c->assembled, DMRI_ERR_STATE, "solve: call dmri_assemble first");
+    require(nsig >= 1 && q && gdir && sig, DMRI_ERR_ARG, "solve: need nsig >= 1 and non-null q, gdir, sig_out");
+    validate_seq(seq, opts);
+    std::vector<double> g(3 * (size_t)nsig);
+    for (int j = 0; j < nsig; ++j) {
+        require(std::isfinite(q[j]), DMRI_ERR_ARG, "q is not finite");
+        const double *d = gdir + 3 * (size_t)j;
+        const double gn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        // |g| = 0 is rejected by the reference (set_gradient_dir, DmriFemLib.py:813-823)
+        require(gn > 1e-10 && std::isfinite(gn), DMRI_ERR_ARG, "|g|=0: gradient direction must be non-zero");
+        for (int k = 0; k < 3; ++k) g[3 * (size_t)j + k] = d[k] / gn;
+    }
+    set_device(c);
+    dmri_stats st{};
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int j0 = 0; j0 < nsig; j0 += kMaxSignals) {
+        const int S = std::min(kMaxSignals, nsig - j0);
+        bool same = true;
+        for (int j = 1; j < S && same; ++j)
+            for (int k = 0; k < 3; ++k) same = same && g[3 * (size_t)(j0 + j) + k] == g[3 * (size_t)j0 + k];
+        solve_group(c, S, q + j0, g.data() + 3 * (size_t)j0, !same, seq, opts, sig + 2 * (size_t)j0, st);
+    }
     const auto t1 = std::chrono::steady_clock::now();
     st.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    sig[0] = hs[0] / c->volume;
-    sig[1] = hs[1] / c->volume;
     if (stats) *stats = st;
     c->solved = true;
 }
@@ -678,19 +799,28 @@ int dmri_export_facets(dmri_ctx *c, int32_t *fv) {
 int dmri_solve(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts, double sig_out[2],
                dmri_stats *stats) {
     return guarded([&] {
-        require(c != nullptr, DMRI_ERR_ARG, "null context");
-        solve_impl(c, seq, opts, sig_out, stats);
+        require(c != nullptr && seq != nullptr, DMRI_ERR_ARG, "null argument");
+        solve_batched_impl(c, 1, &seq->q, seq->gdir, seq, opts, sig_out, stats);
     });
 }
 
-int dmri_get_solution(dmri_ctx *c, double *u) {
+int dmri_solve_batched(dmri_ctx *c, int32_t nsig, const double *q, const double *gdir, const dmri_seq *seq,
+                       const dmri_solver_opts *opts, double *sig_out, dmri_stats *stats) {
+    return guarded([&] {
+        require(c != nullptr, DMRI_ERR_ARG, "null context");
+        solve_batched_impl(c, nsig, q, gdir, seq, opts, sig_out, stats);
+    });
+}
+
+int dmri_get_solution(dmri_ctx *c, int32_t sig, double *u) {
     return guarded([&] {
         require(c != nullptr && u != nullptr, DMRI_ERR_ARG, "null argument");
         require(c->solved, DMRI_ERR_STATE, "no solution (call dmri_solve first)");
+        require(sig >= 0 && sig < c->last_S, DMRI_ERR_ARG, "signal index outside the last launch group");
         set_device(c);
         DevArray<double2> tmp;
         tmp.alloc(c->ndof);
-        launch_unscale(c->ndof, c->sc.p, c->x.p, tmp.p, c->stream);
+        launch_unscale(c->ndof, c->last_S, sig, c->work.sc.p, c->work.x.p, tmp.p, c->stream);
         DMRI_HIP(hipMemcpyAsync(u, tmp.p, sizeof(double2) * c->ndof, hipMemcpyDeviceToHost, c->stream));
         DMRI_HIP(hipStreamSynchronize(c->stream));
     });

@@ -24,21 +24,33 @@ struct Error : std::runtime_error {
                                         std::to_string(__LINE__));                                       \
     } while (0)
 
-// Device state of one Krylov solve, written by the last block of each reducing kernel
-// and read by the kernels that follow (kernel boundaries order the accesses).
+// Per-signal BiCGStab state.  Scalars of iteration i live in slot i&1: a kernel only
+// writes the slot of its own iteration and reads slots written by earlier kernels, so no
+// block ever reads a value another block of the same launch writes.
 struct KrylovState {
-    double2 rho, rho_old, alpha, omega, beta;
-    double bnorm2, rnorm2, snorm2, tol2;
-    int iter, done, breakdown, maxit;
-    double ts_re, ts_im, tt, pad0;
-    unsigned int counter[8];
+    double2 rho[2], alpha[2], omega[2];
+    double tol2, bnorm2, rnorm2, pad;
+    int done, conv, breakdown, iters;
+    int it0, restarts, pad1, pad2;
 };
 
-constexpr int kBlock = 256;          // threads per block of every solver kernel
-constexpr int kRowGroup = 8;         // lanes per matrix row in the CSR SpMV kernels
-constexpr int kRowsPerBlock = kBlock / kRowGroup;
-constexpr int kVecBlocks = 1024;     // grid of the vector-update kernels (grid-stride)
-constexpr int kMaxPartials = 1 << 20;
+constexpr int kThreads = 256;   // target threads per block of every solver kernel
+constexpr int kMaxGrid = 512;      // blocks of the vector kernels (grid-stride)
+constexpr int kMaxGridSpmv = 2048; // blocks of the SpMV kernels: 8 blocks of 4 waves on each of 256 CUs
+constexpr int kMaxSignals = 64; // signals per batched launch
+constexpr int kNV = 4;          // doubles per (block, signal) partial slot
+
+// Launch geometry of the solver kernels for a (G lanes per row, S signals) mapping.
+struct SolveGeom {
+    int G = 8, S = 1;
+    int rpb = 32;           // rows per block in the SpMV kernels
+    int spmv_threads = 256; // rpb * G * S
+    int vec_rpb = 256;      // rows per block in the vector kernels
+    int vec_threads = 256;  // vec_rpb * S
+    int grid = 1;           // blocks of the SpMV-type kernels
+    int vgrid = 1;          // blocks of the vector kernels
+};
+SolveGeom make_geom(int64_t n, int G, int S);
 
 // ---------------------------------------------------------------- setup launchers
 void exclusive_scan(const int *d_in, int *d_out, int64_t n, hipStream_t s);  // d_out has n+1 entries
@@ -76,29 +88,43 @@ void launch_assemble_facets(int64_t start, int64_t count, const int *perm, const
                             const double *kappa_f, double *gi, int *miss, hipStream_t s);
 void launch_diag_and_weights(int64_t n, const int *rowptr, const int *col, const double *m, int *diag, double *w,
                              int *missing_diag, hipStream_t s);
-void launch_sum(const double *x, int64_t n, double *part, unsigned *counter, double *out, hipStream_t s);
+// deterministic sum of x[0..n) into out[0] (two kernels, no atomics); part >= kMaxGrid doubles
+void launch_sum(const double *x, int64_t n, double *part, double *out, hipStream_t s);
 
 // ---------------------------------------------------------------- solve launchers
 void launch_prep_scale(int64_t n, const int *diag, const double *m, const double *k, const double *r,
                        const double *gi, double dt, double theta, double *sc, hipStream_t s);
+// per-gdir == false: val2[e] = {a, g.J} ; per-gdir == true: val4[e] = {a, Jx, Jy, Jz}; all scaled by sc_i sc_j
 void launch_prep_values(int64_t n, const int *rowptr, const int *col, const double *sc, const double *m,
                         const double *k, const double *r, const double *gi, const double *jx, const double *jy,
-                        const double *jz, double gx, double gy, double gz, double dt, double theta, double2 *aj,
-                        double *bre, hipStream_t s);
-void launch_init_x(int64_t n, const double *sc, double2 *x, hipStream_t s);
-void launch_rhs_residual(int64_t n, const int *rowptr, const int *col, const double2 *aj, const double *bre,
-                         double sA, double sB, const double2 *x, double2 *r, double2 *rh, double rtol, double atol,
-                         int maxit, double *part, KrylovState *st, hipStream_t s);
-void launch_bicg_p(int64_t n, const double2 *r, double2 *p, const double2 *v, const KrylovState *st,
+                        const double *jz, bool per_gdir, double gx, double gy, double gz, double dt, double theta,
+                        double *vals, double *bre, hipStream_t s);
+void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s);
+
+struct SolveArgs {
+    int64_t n;
+    const int *rowptr, *col;
+    const double *vals;    // 2 or 4 doubles per nnz
+    const double *bre;     // explicit-side real part per nnz
+    bool per_gdir;
+    const double *coefA;   // [S*3] implicit-side imaginary coefficients of this step
+    const double *coefB;   // [S*3] explicit-side coefficients of this step
+    double2 *x, *r, *rh, *p, *v, *t, *b;
+    double *partU, *partV, *partT;  // [grid][S][kNV] each
+    double *totU, *totV, *totT;     // [S][kNV] finalize outputs, or null: consumers reduce the partials
+    KrylovState *st;
+    int *all_done;
+    double rtol, atol;
+    int maxit;
+};
+void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, bool restart, const int *mask, hipStream_t s);
+void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
+void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
+void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
+void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
+// per-signal sum_i w_i sc_i x[i*S+s] into out[2*s..2*s+1]
+void launch_signal(int64_t n, int S, const double *w, const double *sc, const double2 *x, double *part, double *out,
                    hipStream_t s);
-void launch_spmv_v(int64_t n, const int *rowptr, const int *col, const double2 *aj, double sA, const double2 *p,
-                   double2 *v, const double2 *rh, double *part, KrylovState *st, hipStream_t s);
-void launch_spmv_t(int64_t n, const int *rowptr, const int *col, const double2 *aj, double sA, const double2 *r,
-                   const double2 *v, double2 *t, double *part, KrylovState *st, hipStream_t s);
-void launch_bicg_update(int64_t n, double2 *x, const double2 *p, double2 *r, const double2 *v, const double2 *t,
-                        const double2 *rh, double *part, KrylovState *st, hipStream_t s);
-void launch_signal(int64_t n, const double *w, const double *sc, const double2 *x, double *part,
-                   unsigned *counter, double *out, hipStream_t s);
-void launch_unscale(int64_t n, const double *sc, const double2 *x, double2 *u, hipStream_t s);
+void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *x, double2 *u, hipStream_t s);
 
 }  // namespace dmri

@@ -4,6 +4,8 @@
 // Wave64 throughout.  Every reduction is a fixed-shape tree (wave butterfly, then the
 // waves of a block in index order, then the blocks of a grid in index order by the last
 // block to finish), so results are bitwise reproducible run to run.
+#include <algorithm>
+
 #include "dmri_internal.hpp"
 
 namespace dmri {
@@ -27,6 +29,16 @@ __device__ __forceinline__ double2 c_axpj(double2 a, double s, double2 j) {
     return make_double2(a.x - s * j.y, a.y + s * j.x);
 }
 
+
+// --------------------------------------------------------------------------------------
+// deterministic reductions without atomics or device-scope fences
+//
+// A producing kernel writes one partial per (block, signal) with plain stores; the next
+// kernel (a kernel boundary orders the accesses) sums the partials in block order, either
+// in every block's prologue (few partials) or through a finalize kernel with one block per
+// signal (many partials).  Every sum has a fixed shape, so results are bitwise
+// reproducible.
+// --------------------------------------------------------------------------------------
 template <int NV>
 __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
 #pragma unroll
@@ -39,7 +51,7 @@ __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
 // Block reduction; the result is valid in thread 0.  Fixed order.
 template <int NV>
 __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
-    __shared__ double sh[NV][kBlock / 64];
+    __shared__ double sh[NV][kThreads / 64];
     wave_reduce<NV>(v);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     if (l == 0) {
@@ -48,47 +60,99 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             double s = 0.0;
-            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[k][i];
+            for (int i = 0; i < nw; ++i) s += sh[k][i];
             v[k] = s;
         }
     }
     __syncthreads();
 }
 
-// Grid reduction by the last block to arrive.  Returns true in the last block; the
-// total is then valid in its thread 0.  Block partials are summed in block order.
-template <int NV>
-__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *part, unsigned *counter) {
-    __shared__ int s_last;
-    block_reduce<NV>(v);
-    if (threadIdx.x == 0) {
+// Per-signal block partials.  slot = (row in block)*S + signal; every slot in [0, nslots)
+// is written by exactly one thread (writer == true).
+__device__ __forceinline__ void block_signal_partials(const double (&v)[kNV], bool writer, int slot, int nslots,
+                                                      int S, double *parts) {
+    __shared__ double buf[kNV][kThreads];
+    if (writer) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
-            __hip_atomic_store(&part[(size_t)blockIdx.x * NV + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (prev == gridDim.x - 1);
+        for (int k = 0; k < kNV; ++k) buf[k][slot] = v[k];
     }
     __syncthreads();
-    if (!s_last) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    double acc[NV];
+    if ((int)threadIdx.x < S) {
+        const int s = threadIdx.x;
+        double acc[kNV] = {0.0, 0.0, 0.0, 0.0};
+        for (int j = s; j < nslots; j += S) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+            for (int k = 0; k < kNV; ++k) acc[k] += buf[k][j];
+        }
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
-            acc[k] += __hip_atomic_load(&part[(size_t)b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < kNV; ++k) parts[((size_t)blockIdx.x * S + s) * kNV + k] = acc[k];
     }
-    block_reduce<NV>(acc);
+}
+
+// Totals of every signal into tot[s][k]: from the finalize output when given, else by
+// summing the nparts block partials in block order.
+__device__ __forceinline__ void signal_totals(const double *parts, const double *ftot, int nparts, int S,
+                                              double (*tot)[kNV]) {
+    if (ftot) {
+        for (int i = threadIdx.x; i < S * kNV; i += blockDim.x) tot[i / kNV][i % kNV] = ftot[i];
+        __syncthreads();
+        return;
+    }
+    __shared__ double buf[kNV][kThreads];
+    const int J = blockDim.x / S;
+    const int j = threadIdx.x / S, s = threadIdx.x % S;
+    if (j < J) {
+        double acc[kNV] = {0.0, 0.0, 0.0, 0.0};
+        for (int b = j; b < nparts; b += J) {
+#pragma unroll
+            for (int k = 0; k < kNV; ++k) acc[k] += parts[((size_t)b * S + s) * kNV + k];
+        }
+#pragma unroll
+        for (int k = 0; k < kNV; ++k) buf[k][j * S + s] = acc[k];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < S) {
+        double acc[kNV] = {0.0, 0.0, 0.0, 0.0};
+        for (int jj = 0; jj < J; ++jj) {
+#pragma unroll
+            for (int k = 0; k < kNV; ++k) acc[k] += buf[k][jj * S + threadIdx.x];
+        }
+#pragma unroll
+        for (int k = 0; k < kNV; ++k) tot[threadIdx.x][k] = acc[k];
+    }
+    __syncthreads();
+}
+
+// One block per signal: tot[s][k] = sum over blocks of parts[b][s][k], in block order.
+__global__ void __launch_bounds__(kThreads) finalize_kernel(const double *parts, int nparts, int S, double *tot) {
+    const int s = blockIdx.x;
+    double acc[kNV] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < kNV; ++k) acc[k] += parts[((size_t)b * S + s) * kNV + k];
+    }
+    block_reduce<kNV>(acc);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] = acc[k];
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < kNV; ++k) tot[s * kNV + k] = acc[k];
     }
-    return true;
+}
+
+__global__ void __launch_bounds__(kThreads) sum_part_kernel(const double *x, int64_t n, double *parts) {
+    double v[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[0] += x[i];
+    block_reduce<1>(v);
+    if (threadIdx.x == 0) {
+        parts[(size_t)blockIdx.x * kNV + 0] = v[0];
+        parts[(size_t)blockIdx.x * kNV + 1] = 0.0;
+        parts[(size_t)blockIdx.x * kNV + 2] = 0.0;
+        parts[(size_t)blockIdx.x * kNV + 3] = 0.0;
+    }
 }
 
 inline unsigned grid_for(int64_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
@@ -575,18 +639,11 @@ __global__ void diag_weights_kernel(int64_t n, const int *rowptr, const int *col
     w[i] = s;
 }
 
-__global__ void __launch_bounds__(kBlock) sum_kernel(const double *x, int64_t n, double *part, unsigned *counter,
-                                                     double *out) {
-    double v[1] = {0.0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        v[0] += x[i];
-    if (grid_reduce<1>(v, part, counter) && threadIdx.x == 0) out[0] = v[0];
-}
-
 // --------------------------------------------------------------------------------------
 // Solve-time preparation.  Jacobi preconditioning is applied as the symmetric scaling
 // S = diag(a)^(-1/2) of the whole system (a = M/dt + theta*L): the solver works on
-// S A S, whose real diagonal is 1, with x = S^-1 u.
+// S A S, whose real diagonal is 1, with x = S^-1 u.  (PETSc's Jacobi on the reference's
+// real 2x2-block operator uses the same diagonal: the J coupling is off the block diagonal.)
 // --------------------------------------------------------------------------------------
 __global__ void prep_scale_kernel(int64_t n, const int *diag, const double *m, const double *k, const double *r,
                                   const double *gi, double dt, double theta, double *sc) {
@@ -599,8 +656,8 @@ __global__ void prep_scale_kernel(int64_t n, const int *diag, const double *m, c
 
 __global__ void prep_values_kernel(int64_t n, const int *rowptr, const int *col, const double *sc, const double *m,
                                    const double *k, const double *r, const double *gi, const double *jx,
-                                   const double *jy, const double *jz, double gx, double gy, double gz, double dt,
-                                   double theta, double2 *aj, double *bre) {
+                                   const double *jy, const double *jz, int per_gdir, double gx, double gy,
+                                   double gz, double dt, double theta, double *vals, double *bre) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double si = sc[i];
@@ -608,228 +665,421 @@ __global__ void prep_values_kernel(int64_t n, const int *rowptr, const int *col,
         const double s = si * sc[col[e]];
         const double l = k[e] + r[e] + gi[e];
         const double mk = m[e] / dt;
-        const double jg = gx * jx[e] + gy * jy[e] + gz * jz[e];
-        aj[e] = make_double2(s * (mk + theta * l), s * jg);
         bre[e] = s * (mk - (1.0 - theta) * l);
+        if (per_gdir) {
+            vals[4 * (int64_t)e + 0] = s * (mk + theta * l);
+            vals[4 * (int64_t)e + 1] = s * jx[e];
+            vals[4 * (int64_t)e + 2] = s * jy[e];
+            vals[4 * (int64_t)e + 3] = s * jz[e];
+        } else {
+            vals[2 * (int64_t)e + 0] = s * (mk + theta * l);
+            vals[2 * (int64_t)e + 1] = s * (gx * jx[e] + gy * jy[e] + gz * jz[e]);
+        }
     }
 }
 
-__global__ void init_x_kernel(int64_t n, const double *sc, double2 *x) {
+__global__ void init_x_kernel(int64_t n, int S, const double *sc, double2 *x) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = make_double2(1.0 / sc[i], 0.0);
+    if (i < n * S) x[i] = make_double2(1.0 / sc[i / S], 0.0);
 }
 
-__global__ void unscale_kernel(int64_t n, const double *sc, const double2 *x, double2 *u) {
+__global__ void unscale_kernel(int64_t n, int S, int sig, const double *sc, const double2 *x, double2 *u) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) u[i] = make_double2(sc[i] * x[i].x, sc[i] * x[i].y);
+    if (i < n) {
+        const double2 xi = x[i * S + sig];
+        u[i] = make_double2(sc[i] * xi.x, sc[i] * xi.y);
+    }
 }
 
 // --------------------------------------------------------------------------------------
-// Solver kernels.  kRowGroup lanes per row; lane l of a group handles entries l, l+G, ...
-// of the row and the group is reduced with a butterfly.
+// Solver kernels for S signals sharing one pattern.  SpMV mapping: thread =
+// (row in block, signal s, lane g of G); the G lanes of a (row, signal) split the row's
+// entries and are reduced with a butterfly.  Vectors are stored x[row*S + s] so that the
+// signals of one gathered column are contiguous.  Vector kernels: thread = (row, s).
+//
+// y_s = (Are + i (c_s . J)) x_s with Are = S(M/dt + theta L)S and J = S(g.J)S (one
+// direction, PG = false) or J = S(Jx, Jy, Jz)S with per-signal c_s = theta q_s f_n g_s
+// (PG = true).
 // --------------------------------------------------------------------------------------
-__device__ __forceinline__ void group_reduce2(double2 &a) {
+template <bool PG>
+struct Acc {
+    double2 a;
+    double2 j[PG ? 3 : 1];
+};
+
+template <int G, bool PG>
+__device__ __forceinline__ double2 imag_coef_apply(const Acc<PG> &acc, const double *coef, int s) {
+    double2 cj;
+    if (PG) {
+        const double cx = coef[3 * s], cy = coef[3 * s + 1], cz = coef[3 * s + 2];
+        cj = make_double2(cx * acc.j[0].x + cy * acc.j[1].x + cz * acc.j[2].x,
+                          cx * acc.j[0].y + cy * acc.j[1].y + cz * acc.j[2].y);
+    } else {
+        const double c = coef[3 * s];
+        cj = make_double2(c * acc.j[0].x, c * acc.j[0].y);
+    }
+    // a + i*cj
+    return make_double2(acc.a.x - cj.y, acc.a.y + cj.x);
+}
+
+template <int G>
+__device__ __forceinline__ void group_reduce(double2 &v) {
 #pragma unroll
-    for (int off = kRowGroup / 2; off > 0; off >>= 1) {
-        a.x += __shfl_xor(a.x, off, 64);
-        a.y += __shfl_xor(a.y, off, 64);
+    for (int off = G / 2; off > 0; off >>= 1) {
+        v.x += __shfl_xor(v.x, off, 64);
+        v.y += __shfl_xor(v.y, off, 64);
     }
 }
 
-// rhs + initial residual: b = (Bre + i sB J) x, r = b - (Are + i sA J) x; r^ = r.
-__global__ void __launch_bounds__(kBlock) rhs_residual_kernel(int64_t n, const int *rowptr, const int *col,
-                                                              const double2 *aj, const double *bre, double sA,
-                                                              double sB, const double2 *x, double2 *r, double2 *rh,
-                                                              double rtol, double atol, int maxit, double *part,
-                                                              KrylovState *st) {
-    const int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + (threadIdx.x / kRowGroup);
-    const int l = threadIdx.x % kRowGroup;
-    double red[2] = {0.0, 0.0};
-    if (row < n) {
-        double2 sa = make_double2(0, 0), sj = make_double2(0, 0), sb = make_double2(0, 0);
-        for (int e = rowptr[row] + l; e < rowptr[row + 1]; e += kRowGroup) {
-            const double2 v = aj[e];
-            const double bv = bre[e];
-            const double2 xv = x[col[e]];
-            sa.x += v.x * xv.x; sa.y += v.x * xv.y;
-            sj.x += v.y * xv.x; sj.y += v.y * xv.y;
-            sb.x += bv * xv.x;  sb.y += bv * xv.y;
-        }
-        group_reduce2(sa);
-        group_reduce2(sj);
-        group_reduce2(sb);
-        if (l == 0) {
-            const double2 b = c_axpj(sb, sB, sj);
-            const double2 ax = c_axpj(sa, sA, sj);
-            const double2 rr = c_sub(b, ax);
-            r[row] = rr;
-            rh[row] = rr;
-            red[0] = b.x * b.x + b.y * b.y;
-            red[1] = rr.x * rr.x + rr.y * rr.y;
-        }
-    }
-    if (grid_reduce<2>(red, part, st->counter + 0) && threadIdx.x == 0) {
-        st->bnorm2 = red[0];
-        st->rnorm2 = red[1];
-        st->rho = make_double2(red[1], 0.0);
-        st->rho_old = make_double2(1.0, 0.0);
-        st->alpha = make_double2(1.0, 0.0);
-        st->omega = make_double2(1.0, 0.0);
-        st->beta = make_double2(0.0, 0.0);
-        const double t1 = rtol * rtol * red[0], t2 = atol * atol;
-        st->tol2 = t1 > t2 ? t1 : t2;
-        st->iter = 0;
-        st->maxit = maxit;
-        st->breakdown = 0;
-        st->done = red[1] <= st->tol2 ? 1 : 0;
+template <int G, bool PG>
+__device__ __forceinline__ void group_reduce_acc(Acc<PG> &acc) {
+    group_reduce<G>(acc.a);
+#pragma unroll
+    for (int d = 0; d < (PG ? 3 : 1); ++d) group_reduce<G>(acc.j[d]);
+}
+
+// Accumulate row entries g, g+G, ... of x = X(col) (X(c) returns the gathered value).
+template <bool PG>
+__device__ __forceinline__ void acc_entry(const double *vals, int e, double2 xv, Acc<PG> &acc) {
+    if (PG) {
+        const double4 v = *reinterpret_cast<const double4 *>(vals + 4 * (int64_t)e);
+        acc.a.x += v.x * xv.x;
+        acc.a.y += v.x * xv.y;
+        acc.j[0].x += v.y * xv.x;
+        acc.j[0].y += v.y * xv.y;
+        acc.j[1].x += v.z * xv.x;
+        acc.j[1].y += v.z * xv.y;
+        acc.j[2].x += v.w * xv.x;
+        acc.j[2].y += v.w * xv.y;
+    } else {
+        const double2 v = *reinterpret_cast<const double2 *>(vals + 2 * (int64_t)e);
+        acc.a.x += v.x * xv.x;
+        acc.a.y += v.x * xv.y;
+        acc.j[0].x += v.y * xv.x;
+        acc.j[0].y += v.y * xv.y;
     }
 }
 
-// p = r (first iteration) or p = r + beta (p - omega v)
-__global__ void __launch_bounds__(kBlock) bicg_p_kernel(int64_t n, const double2 *r, double2 *p, const double2 *v,
-                                                        const KrylovState *st) {
-    if (st->done) return;
-    const bool first = st->iter == 0;
-    const double2 beta = st->beta, omega = st->omega;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 ri = r[i];
+// Accumulate row entries g, g+G, ... of x = X(col) (X(c) returns the gathered value).
+// Two entries per step, so each lane keeps two independent gathers in flight.
+template <int G, bool PG, class Gather>
+__device__ __forceinline__ void row_accumulate(const int *rowptr, const int *col, const double *vals, int64_t row,
+                                               int g, Gather X, Acc<PG> &acc) {
+    acc.a = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < (PG ? 3 : 1); ++d) acc.j[d] = make_double2(0.0, 0.0);
+    const int e1 = rowptr[row + 1];
+    int e = rowptr[row] + g;
+    for (; e + G < e1; e += 2 * G) {
+        const int c0 = col[e], c1 = col[e + G];
+        const double2 x0 = X(c0), x1 = X(c1);
+        acc_entry<PG>(vals, e, x0, acc);
+        acc_entry<PG>(vals, e + G, x1, acc);
+    }
+    if (e < e1) acc_entry<PG>(vals, e, X(col[e]), acc);
+}
+
+struct Lane {
+    int lrow, s, g;
+};
+template <int G>
+__device__ __forceinline__ Lane spmv_lane(int S) {
+    const int gs = G * S;
+    const int rem = threadIdx.x % gs;
+    return Lane{(int)threadIdx.x / gs, rem / G, rem % G};
+}
+
+// b = (Bre + i cB.J) x  (or the stored b on a restart);  r = b - (Are + i cA.J) x;  r^ = r.
+// partials U: [|r|^2, 0, |r|^2, |b|^2]
+template <int G, bool PG>
+__global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int S, int rpb, int restart,
+                                                                const int *mask) {
+    const Lane L = spmv_lane<G>(S);
+    const bool sel = !restart || mask[L.s];
+    const int S_ = S;
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t row = (int64_t)blockIdx.x * rpb + L.lrow; row < a.n; row += (int64_t)gridDim.x * rpb) {
+        if (!sel) continue;
+        const double2 *x = a.x;
+        const int s = L.s;
+        Acc<PG> acc;
+        row_accumulate<G, PG>(a.rowptr, a.col, a.vals, row, L.g, [&](int c) { return x[(int64_t)c * S_ + s]; },
+                              acc);
+        double2 sb = make_double2(0.0, 0.0);
+        if (!restart) {
+            const int e1 = a.rowptr[row + 1];
+            for (int e = a.rowptr[row] + L.g; e < e1; e += G) {
+                const double bv = a.bre[e];
+                const double2 xv = x[(int64_t)a.col[e] * S_ + s];
+                sb.x += bv * xv.x;
+                sb.y += bv * xv.y;
+            }
+            group_reduce<G>(sb);
+        }
+        group_reduce_acc<G, PG>(acc);
+        if (L.g == 0) {
+            const int64_t idx = row * S_ + s;
+            double2 b;
+            if (restart) {
+                b = a.b[idx];
+            } else {
+                Acc<PG> bacc = acc;
+                bacc.a = sb;
+                b = imag_coef_apply<G, PG>(bacc, a.coefB, s);
+                a.b[idx] = b;
+            }
+            const double2 ax = imag_coef_apply<G, PG>(acc, a.coefA, s);
+            const double2 r = make_double2(b.x - ax.x, b.y - ax.y);
+            a.r[idx] = r;
+            a.rh[idx] = r;
+            const double rr = r.x * r.x + r.y * r.y;
+            red[0] += rr;
+            red[2] += rr;
+            red[3] += b.x * b.x + b.y * b.y;
+        }
+    }
+    block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partU);
+}
+
+// Iteration head: totals of U give rho_i = <r^, r_i> and |r_i|^2; convergence, breakdown
+// and iteration-limit decisions; p = r (first iteration) or p = r + beta (p - omega v).
+// The partials of U come from the residual kernel (SpMV grid) on a signal's first iteration
+// and from the update kernel (vector grid) afterwards; without a finalize pass S is 1.
+__global__ void __launch_bounds__(kThreads) bicg_p_kernel(SolveArgs a, int S, int vrpb, int nparts_rhs,
+                                                          int nparts_upd, int it) {
+    if (*a.all_done) return;
+    __shared__ double tot[kMaxSignals][kNV];
+    __shared__ double2 sbeta[kMaxSignals], somega[kMaxSignals];
+    __shared__ int sact[kMaxSignals], sfirst[kMaxSignals];
+    const int nparts = (it - a.st[0].it0) == 0 ? nparts_rhs : nparts_upd;
+    signal_totals(a.partU, a.totU, nparts, S, tot);
+    if ((int)threadIdx.x < S) {
+        const int s = threadIdx.x;
+        KrylovState &st = a.st[s];
+        int act = !st.done;
+        const int local = it - st.it0;
+        sfirst[s] = local == 0;
+        if (act) {
+            const double2 rho = make_double2(tot[s][0], tot[s][1]);
+            const double rn2 = tot[s][2];
+            double tol2;
+            if (local == 0) {
+                const double t1 = a.rtol * a.rtol * tot[s][3], t2 = a.atol * a.atol;
+                tol2 = t1 > t2 ? t1 : t2;
+            } else {
+                tol2 = st.tol2;
+            }
+            const bool conv = rn2 <= tol2;
+            const double2 om_prev = local > 0 ? st.omega[(local - 1) & 1] : make_double2(1.0, 0.0);
+            const bool brk = !conv && local > 0 &&
+                             ((rho.x == 0.0 && rho.y == 0.0) || (om_prev.x == 0.0 && om_prev.y == 0.0));
+            const bool lim = !conv && local >= a.maxit;
+            const bool newly = conv || brk || lim;
+            if (blockIdx.x == 0) {
+                if (local == 0) {
+                    st.tol2 = tol2;
+                    st.bnorm2 = tot[s][3];
+                }
+                st.rnorm2 = rn2;
+                st.iters = it;
+                if (newly) {
+                    st.done = 1;
+                    st.conv = conv ? 1 : 0;
+                    st.breakdown = brk ? 1 : 0;
+                } else {
+                    st.rho[local & 1] = rho;
+                }
+            }
+            if (newly) {
+                act = 0;
+            } else if (local > 0) {
+                const double2 rho_prev = st.rho[(local - 1) & 1];
+                const double2 al_prev = st.alpha[(local - 1) & 1];
+                sbeta[s] = c_mul(c_div(rho, rho_prev), c_div(al_prev, om_prev));
+                somega[s] = om_prev;
+            }
+        }
+        sact[s] = act;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int all = 1;
+        for (int s = 0; s < S; ++s)
+            if (sact[s]) all = 0;
+        if (all) *a.all_done = 1;
+    }
+    const int64_t total = a.n * S;
+    const int s = threadIdx.x % S;
+    if ((int)threadIdx.x >= vrpb * S || !sact[s]) return;
+    const bool first = sfirst[s];
+    const double2 beta = sbeta[s], omega = somega[s];
+    for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total; i += (int64_t)gridDim.x * vrpb * S) {
+        const double2 ri = a.r[i];
         if (first) {
-            p[i] = ri;
+            a.p[i] = ri;
         } else {
-            const double2 w = c_sub(p[i], c_mul(omega, v[i]));
-            p[i] = c_add(ri, c_mul(beta, w));
+            const double2 w = c_sub(a.p[i], c_mul(omega, a.v[i]));
+            a.p[i] = c_add(ri, c_mul(beta, w));
         }
     }
 }
 
-// v = A p ; alpha = rho / <r^, v>
-__global__ void __launch_bounds__(kBlock) spmv_v_kernel(int64_t n, const int *rowptr, const int *col,
-                                                        const double2 *aj, double sA, const double2 *p, double2 *v,
-                                                        const double2 *rh, double *part, KrylovState *st) {
-    if (st->done) return;
-    const int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + (threadIdx.x / kRowGroup);
-    const int l = threadIdx.x % kRowGroup;
-    double red[2] = {0.0, 0.0};
-    if (row < n) {
-        double2 sa = make_double2(0, 0), sj = make_double2(0, 0);
-        for (int e = rowptr[row] + l; e < rowptr[row + 1]; e += kRowGroup) {
-            const double2 a = aj[e];
-            const double2 xv = p[col[e]];
-            sa.x += a.x * xv.x; sa.y += a.x * xv.y;
-            sj.x += a.y * xv.x; sj.y += a.y * xv.y;
-        }
-        group_reduce2(sa);
-        group_reduce2(sj);
-        if (l == 0) {
-            const double2 y = c_axpj(sa, sA, sj);
-            v[row] = y;
-            const double2 h = rh[row];
-            red[0] = h.x * y.x + h.y * y.y;   // conj(h) * y
-            red[1] = h.x * y.y - h.y * y.x;
-        }
-    }
-    if (grid_reduce<2>(red, part, st->counter + 1) && threadIdx.x == 0) {
-        const double2 rv = make_double2(red[0], red[1]);
-        if (rv.x == 0.0 && rv.y == 0.0) {
-            st->breakdown = 1;
-            st->done = 1;
-        } else {
-            st->alpha = c_div(st->rho, rv);
+// v = A p ; partials V: [<r^, v>.re, <r^, v>.im, 0, 0]
+template <int G, bool PG>
+__global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, int rpb, int it) {
+    if (*a.all_done) return;
+    const Lane L = spmv_lane<G>(S);
+    const bool act = !a.st[L.s].done;
+    const int S_ = S;
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t row = (int64_t)blockIdx.x * rpb + L.lrow; row < a.n; row += (int64_t)gridDim.x * rpb) {
+        if (!act) continue;
+        const double2 *p = a.p;
+        const int s = L.s;
+        Acc<PG> acc;
+        row_accumulate<G, PG>(a.rowptr, a.col, a.vals, row, L.g, [&](int c) { return p[(int64_t)c * S_ + s]; },
+                              acc);
+        group_reduce_acc<G, PG>(acc);
+        if (L.g == 0) {
+            const int64_t idx = row * S_ + s;
+            const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
+            a.v[idx] = y;
+            const double2 h = a.rh[idx];
+            red[0] += h.x * y.x + h.y * y.y;
+            red[1] += h.x * y.y - h.y * y.x;
         }
     }
+    block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
 }
 
-// s = r - alpha v (on the fly), t = A s ; omega = <t, s> / <t, t>
-__global__ void __launch_bounds__(kBlock) spmv_t_kernel(int64_t n, const int *rowptr, const int *col,
-                                                        const double2 *aj, double sA, const double2 *r,
-                                                        const double2 *v, double2 *t, double *part,
-                                                        KrylovState *st) {
-    if (st->done) return;
-    const double2 alpha = st->alpha;
-    const int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + (threadIdx.x / kRowGroup);
-    const int l = threadIdx.x % kRowGroup;
-    double red[4] = {0.0, 0.0, 0.0, 0.0};
-    if (row < n) {
-        double2 sa = make_double2(0, 0), sj = make_double2(0, 0);
-        for (int e = rowptr[row] + l; e < rowptr[row + 1]; e += kRowGroup) {
-            const double2 a = aj[e];
-            const int c = col[e];
-            const double2 xv = c_sub(r[c], c_mul(alpha, v[c]));
-            sa.x += a.x * xv.x; sa.y += a.x * xv.y;
-            sj.x += a.y * xv.x; sj.y += a.y * xv.y;
+// alpha = rho / <r^, v> ; t = A (r - alpha v) with s = r - alpha v formed on the fly;
+// partials T: [<t, s>.re, <t, s>.im, |t|^2, |s|^2]
+template <int G, bool PG>
+__global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, int rpb, int nparts, int it) {
+    if (*a.all_done) return;
+    __shared__ double tot[kMaxSignals][kNV];
+    __shared__ double2 salpha[kMaxSignals];
+    __shared__ int sact[kMaxSignals];
+    signal_totals(a.partV, a.totV, nparts, S, tot);
+    if ((int)threadIdx.x < S) {
+        const int s = threadIdx.x;
+        KrylovState &st = a.st[s];
+        int act = !st.done;
+        const int local = it - st.it0;
+        if (act) {
+            const double2 rv = make_double2(tot[s][0], tot[s][1]);
+            if (rv.x == 0.0 && rv.y == 0.0) {
+                if (blockIdx.x == 0) {
+                    st.done = 1;
+                    st.breakdown = 1;
+                    st.conv = 0;
+                    st.iters = it;
+                }
+                act = 0;
+            } else {
+                const double2 al = c_div(st.rho[local & 1], rv);
+                salpha[s] = al;
+                if (blockIdx.x == 0) st.alpha[local & 1] = al;
+            }
         }
-        group_reduce2(sa);
-        group_reduce2(sj);
-        if (l == 0) {
-            const double2 y = c_axpj(sa, sA, sj);
-            t[row] = y;
-            const double2 s = c_sub(r[row], c_mul(alpha, v[row]));
-            red[0] = y.x * s.x + y.y * s.y;  // conj(t) * s
-            red[1] = y.x * s.y - y.y * s.x;
-            red[2] = y.x * y.x + y.y * y.y;
-            red[3] = s.x * s.x + s.y * s.y;
+        sact[s] = act;
+    }
+    __syncthreads();
+    const Lane L = spmv_lane<G>(S);
+    const bool act = sact[L.s];
+    const double2 alpha = salpha[L.s];
+    const int S_ = S;
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t row = (int64_t)blockIdx.x * rpb + L.lrow; row < a.n; row += (int64_t)gridDim.x * rpb) {
+        if (!act) continue;
+        const double2 *r = a.r, *v = a.v;
+        const int s = L.s;
+        Acc<PG> acc;
+        row_accumulate<G, PG>(
+            a.rowptr, a.col, a.vals, row, L.g,
+            [&](int c) {
+                const int64_t j = (int64_t)c * S_ + s;
+                return c_sub(r[j], c_mul(alpha, v[j]));
+            },
+            acc);
+        group_reduce_acc<G, PG>(acc);
+        if (L.g == 0) {
+            const int64_t idx = row * S_ + s;
+            const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
+            a.t[idx] = y;
+            const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
+            red[0] += y.x * sv.x + y.y * sv.y;
+            red[1] += y.x * sv.y - y.y * sv.x;
+            red[2] += y.x * y.x + y.y * y.y;
+            red[3] += sv.x * sv.x + sv.y * sv.y;
         }
     }
-    if (grid_reduce<4>(red, part, st->counter + 2) && threadIdx.x == 0) {
-        st->snorm2 = red[3];
-        if (red[3] <= st->tol2 || red[2] == 0.0) {
-            st->omega = make_double2(0.0, 0.0);  // x += alpha p, r = s
-        } else {
-            st->omega = make_double2(red[0] / red[2], red[1] / red[2]);
-        }
-    }
+    block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
 }
 
-// x += alpha p + omega s ; r = s - omega t ; rho = <r^, r> ; beta for the next iteration
-__global__ void __launch_bounds__(kBlock) bicg_update_kernel(int64_t n, double2 *x, const double2 *p, double2 *r,
-                                                             const double2 *v, const double2 *t, const double2 *rh,
-                                                             double *part, KrylovState *st) {
-    if (st->done) return;
-    const double2 alpha = st->alpha, omega = st->omega;
-    double red[3] = {0.0, 0.0, 0.0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 s = c_sub(r[i], c_mul(alpha, v[i]));
-        x[i] = c_add(x[i], c_add(c_mul(alpha, p[i]), c_mul(omega, s)));
-        const double2 rn = c_sub(s, c_mul(omega, t[i]));
-        r[i] = rn;
-        const double2 h = rh[i];
-        red[0] += h.x * rn.x + h.y * rn.y;
-        red[1] += h.x * rn.y - h.y * rn.x;
-        red[2] += rn.x * rn.x + rn.y * rn.y;
+// omega = <t, s> / <t, t> (0 when |s| already meets the tolerance);
+// x += alpha p + omega s ; r = s - omega t ; partials U: [<r^, r>.re, .im, |r|^2, 0]
+__global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int S, int vrpb, int nparts, int it) {
+    if (*a.all_done) return;
+    __shared__ double tot[kMaxSignals][kNV];
+    __shared__ double2 salpha[kMaxSignals], somega[kMaxSignals];
+    __shared__ int sact[kMaxSignals];
+    signal_totals(a.partT, a.totT, nparts, S, tot);
+    if ((int)threadIdx.x < S) {
+        const int s = threadIdx.x;
+        KrylovState &st = a.st[s];
+        const int act = !st.done;
+        const int local = it - st.it0;
+        if (act) {
+            const double2 ts = make_double2(tot[s][0], tot[s][1]);
+            const double tt = tot[s][2], s2 = tot[s][3];
+            double2 om = make_double2(0.0, 0.0);
+            if (!(s2 <= st.tol2) && tt != 0.0) om = make_double2(ts.x / tt, ts.y / tt);
+            somega[s] = om;
+            salpha[s] = st.alpha[local & 1];
+            if (blockIdx.x == 0) st.omega[local & 1] = om;
+        }
+        sact[s] = act;
     }
-    if (grid_reduce<3>(red, part, st->counter + 3) && threadIdx.x == 0) {
-        const double2 rho = make_double2(red[0], red[1]);
-        st->rho_old = st->rho;
-        st->rho = rho;
-        st->rnorm2 = red[2];
-        st->iter += 1;
-        const bool conv = red[2] <= st->tol2;
-        if (conv) {
-            st->done = 1;
-        } else if ((omega.x == 0.0 && omega.y == 0.0) || (rho.x == 0.0 && rho.y == 0.0)) {
-            st->breakdown = 1;
-            st->done = 1;
-        } else {
-            st->beta = c_mul(c_div(rho, st->rho_old), c_div(alpha, omega));
-            if (st->iter >= st->maxit) st->done = 1;
+    __syncthreads();
+    const int64_t total = a.n * S;
+    const int s = threadIdx.x % S;
+    const bool valid = (int)threadIdx.x < vrpb * S;
+    const bool act = valid && sact[s];
+    const double2 alpha = salpha[s], omega = somega[s];
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    if (act) {
+        for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total;
+             i += (int64_t)gridDim.x * vrpb * S) {
+            const double2 sv = c_sub(a.r[i], c_mul(alpha, a.v[i]));
+            a.x[i] = c_add(a.x[i], c_add(c_mul(alpha, a.p[i]), c_mul(omega, sv)));
+            const double2 rn = c_sub(sv, c_mul(omega, a.t[i]));
+            a.r[i] = rn;
+            const double2 h = a.rh[i];
+            red[0] += h.x * rn.x + h.y * rn.y;
+            red[1] += h.x * rn.y - h.y * rn.x;
+            red[2] += rn.x * rn.x + rn.y * rn.y;
         }
     }
+    block_signal_partials(red, valid, threadIdx.x, vrpb * S, S, a.partU);
 }
 
-__global__ void __launch_bounds__(kBlock) signal_kernel(int64_t n, const double *w, const double *sc,
-                                                        const double2 *x, double *part, unsigned *counter,
-                                                        double *out) {
-    double red[2] = {0.0, 0.0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double ws = w[i] * sc[i];
-        const double2 xi = x[i];
-        red[0] += ws * xi.x;
-        red[1] += ws * xi.y;
+// per-signal partials of sum_i w_i sc_i x[i*S+s]
+__global__ void __launch_bounds__(kThreads) signal_part_kernel(int64_t n, int S, int vrpb, const double *w,
+                                                               const double *sc, const double2 *x, double *parts) {
+    const int64_t total = n * S;
+    const bool valid = (int)threadIdx.x < vrpb * S;
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    if (valid) {
+        for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total;
+             i += (int64_t)gridDim.x * vrpb * S) {
+            const int64_t row = i / S;
+            const double ws = w[row] * sc[row];
+            red[0] += ws * x[i].x;
+            red[1] += ws * x[i].y;
+        }
     }
-    if (grid_reduce<2>(red, part, counter) && threadIdx.x == 0) {
-        out[0] = red[0];
-        out[1] = red[1];
-    }
+    block_signal_partials(red, valid, threadIdx.x, vrpb * S, S, parts);
 }
 
 }  // namespace
@@ -962,11 +1212,32 @@ void launch_diag_and_weights(int64_t n, const int *rowptr, const int *col, const
     diag_weights_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rowptr, col, m, diag, w, missing_diag);
     check_launch();
 }
-void launch_sum(const double *x, int64_t n, double *part, unsigned *counter, double *out, hipStream_t s) {
-    unsigned g = grid_for(n, kBlock);
-    if (g > (unsigned)kVecBlocks) g = kVecBlocks;
+
+SolveGeom make_geom(int64_t n, int G, int S) {
+    if (S < 1 || S > kMaxSignals) throw Error(-1, "signals per launch must lie in [1, 64]");
+    if (G < 1 || G > 32 || (G & (G - 1)) || G * S > 64)
+        throw Error(-1, "lanes per row must be a power of two with lanes*signals <= 64");
+    SolveGeom g;
+    g.G = G;
+    g.S = S;
+    g.rpb = std::max(1, kThreads / (G * S));
+    g.spmv_threads = g.rpb * G * S;
+    g.vec_rpb = std::max(1, kThreads / S);
+    g.vec_threads = g.vec_rpb * S;
+    int64_t need = (n + g.rpb - 1) / g.rpb;
+    g.grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, kMaxGridSpmv));
+    int64_t vneed = (n + g.vec_rpb - 1) / g.vec_rpb;
+    g.vgrid = (int)std::max<int64_t>(1, std::min<int64_t>(vneed, kMaxGrid));
+    return g;
+}
+
+void launch_sum(const double *x, int64_t n, double *part, double *out, hipStream_t s) {
+    unsigned g = grid_for(n, kThreads);
+    if (g > (unsigned)kMaxGrid) g = kMaxGrid;
     if (g == 0) g = 1;
-    sum_kernel<<<g, kBlock, 0, s>>>(x, n, part, counter, out);
+    sum_part_kernel<<<g, kThreads, 0, s>>>(x, n, part);
+    check_launch();
+    finalize_kernel<<<1, kThreads, 0, s>>>(part, (int)g, 1, out);
     check_launch();
 }
 
@@ -977,55 +1248,88 @@ void launch_prep_scale(int64_t n, const int *diag, const double *m, const double
 }
 void launch_prep_values(int64_t n, const int *rowptr, const int *col, const double *sc, const double *m,
                         const double *k, const double *r, const double *gi, const double *jx, const double *jy,
-                        const double *jz, double gx, double gy, double gz, double dt, double theta, double2 *aj,
-                        double *bre, hipStream_t s) {
-    prep_values_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rowptr, col, sc, m, k, r, gi, jx, jy, jz, gx, gy, gz, dt,
-                                                         theta, aj, bre);
+                        const double *jz, bool per_gdir, double gx, double gy, double gz, double dt, double theta,
+                        double *vals, double *bre, hipStream_t s) {
+    prep_values_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rowptr, col, sc, m, k, r, gi, jx, jy, jz, per_gdir ? 1 : 0,
+                                                         gx, gy, gz, dt, theta, vals, bre);
     check_launch();
 }
-void launch_init_x(int64_t n, const double *sc, double2 *x, hipStream_t s) {
-    init_x_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, sc, x);
+void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s) {
+    init_x_kernel<<<grid_for(n * S, 256), 256, 0, s>>>(n, S, sc, x);
     check_launch();
 }
-void launch_unscale(int64_t n, const double *sc, const double2 *x, double2 *u, hipStream_t s) {
-    unscale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, sc, x, u);
+void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *x, double2 *u, hipStream_t s) {
+    unscale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, S, sig, sc, x, u);
     check_launch();
 }
-void launch_rhs_residual(int64_t n, const int *rowptr, const int *col, const double2 *aj, const double *bre,
-                         double sA, double sB, const double2 *x, double2 *r, double2 *rh, double rtol, double atol,
-                         int maxit, double *part, KrylovState *st, hipStream_t s) {
-    rhs_residual_kernel<<<grid_for(n, kRowsPerBlock), kBlock, 0, s>>>(n, rowptr, col, aj, bre, sA, sB, x, r, rh, rtol,
-                                                                      atol, maxit, part, st);
+
+// dispatch on the lanes-per-row template parameter and the gradient mode
+#define DMRI_DISPATCH_G(G_, PG_, CALL)                                                      \
+    do {                                                                                    \
+        switch (G_) {                                                                       \
+            case 1: { constexpr int GG = 1; if (PG_) { constexpr bool PP = true; CALL; }   \
+                      else { constexpr bool PP = false; CALL; } } break;                   \
+            case 2: { constexpr int GG = 2; if (PG_) { constexpr bool PP = true; CALL; }   \
+                      else { constexpr bool PP = false; CALL; } } break;                   \
+            case 4: { constexpr int GG = 4; if (PG_) { constexpr bool PP = true; CALL; }   \
+                      else { constexpr bool PP = false; CALL; } } break;                   \
+            case 8: { constexpr int GG = 8; if (PG_) { constexpr bool PP = true; CALL; }   \
+                      else { constexpr bool PP = false; CALL; } } break;                   \
+            case 16: { constexpr int GG = 16; if (PG_) { constexpr bool PP = true; CALL; } \
+                       else { constexpr bool PP = false; CALL; } } break;                  \
+            case 32: { constexpr int GG = 32; if (PG_) { constexpr bool PP = true; CALL; } \
+                       else { constexpr bool PP = false; CALL; } } break;                  \
+            default: throw Error(-1, "unsupported lanes per row");                         \
+        }                                                                                   \
+    } while (0)
+
+void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, bool restart, const int *mask, hipStream_t s) {
+    DMRI_DISPATCH_G(g.G, a.per_gdir,
+                    (rhs_residual_kernel<GG, PP><<<g.grid, g.spmv_threads, 0, s>>>(a, g.S, g.rpb, restart ? 1 : 0,
+                                                                                   mask)));
+    check_launch();
+    if (a.totU) {
+        finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partU, g.grid, g.S, a.totU);
+        check_launch();
+    }
+}
+void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s) {
+    bicg_p_kernel<<<g.vgrid, g.vec_threads, 0, s>>>(a, g.S, g.vec_rpb, g.grid, g.vgrid, it);
     check_launch();
 }
-static inline unsigned vec_grid(int64_t n) {
-    unsigned g = grid_for(n, kBlock);
-    if (g > (unsigned)kVecBlocks) g = kVecBlocks;
-    return g == 0 ? 1 : g;
+void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s) {
+    DMRI_DISPATCH_G(g.G, a.per_gdir, (spmv_v_kernel<GG, PP><<<g.grid, g.spmv_threads, 0, s>>>(a, g.S, g.rpb, it)));
+    check_launch();
+    if (a.totV) {
+        finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partV, g.grid, g.S, a.totV);
+        check_launch();
+    }
 }
-void launch_bicg_p(int64_t n, const double2 *r, double2 *p, const double2 *v, const KrylovState *st,
+void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s) {
+    DMRI_DISPATCH_G(g.G, a.per_gdir,
+                    (spmv_t_kernel<GG, PP><<<g.grid, g.spmv_threads, 0, s>>>(a, g.S, g.rpb, g.grid, it)));
+    check_launch();
+    if (a.totT) {
+        finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partT, g.grid, g.S, a.totT);
+        check_launch();
+    }
+}
+void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s) {
+    bicg_update_kernel<<<g.vgrid, g.vec_threads, 0, s>>>(a, g.S, g.vec_rpb, g.grid, it);
+    check_launch();
+    if (a.totU) {
+        finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partU, g.vgrid, g.S, a.totU);
+        check_launch();
+    }
+}
+void launch_signal(int64_t n, int S, const double *w, const double *sc, const double2 *x, double *part, double *out,
                    hipStream_t s) {
-    bicg_p_kernel<<<vec_grid(n), kBlock, 0, s>>>(n, r, p, v, st);
+    const int vrpb = std::max(1, kThreads / S);
+    int64_t need = (n + vrpb - 1) / vrpb;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, kMaxGrid));
+    signal_part_kernel<<<grid, vrpb * S, 0, s>>>(n, S, vrpb, w, sc, x, part);
     check_launch();
-}
-void launch_spmv_v(int64_t n, const int *rowptr, const int *col, const double2 *aj, double sA, const double2 *p,
-                   double2 *v, const double2 *rh, double *part, KrylovState *st, hipStream_t s) {
-    spmv_v_kernel<<<grid_for(n, kRowsPerBlock), kBlock, 0, s>>>(n, rowptr, col, aj, sA, p, v, rh, part, st);
-    check_launch();
-}
-void launch_spmv_t(int64_t n, const int *rowptr, const int *col, const double2 *aj, double sA, const double2 *r,
-                   const double2 *v, double2 *t, double *part, KrylovState *st, hipStream_t s) {
-    spmv_t_kernel<<<grid_for(n, kRowsPerBlock), kBlock, 0, s>>>(n, rowptr, col, aj, sA, r, v, t, part, st);
-    check_launch();
-}
-void launch_bicg_update(int64_t n, double2 *x, const double2 *p, double2 *r, const double2 *v, const double2 *t,
-                        const double2 *rh, double *part, KrylovState *st, hipStream_t s) {
-    bicg_update_kernel<<<vec_grid(n), kBlock, 0, s>>>(n, x, p, r, v, t, rh, part, st);
-    check_launch();
-}
-void launch_signal(int64_t n, const double *w, const double *sc, const double2 *x, double *part, unsigned *counter,
-                   double *out, hipStream_t s) {
-    signal_kernel<<<vec_grid(n), kBlock, 0, s>>>(n, w, sc, x, part, counter, out);
+    finalize_kernel<<<S, kThreads, 0, s>>>(part, grid, S, out);
     check_launch();
 }
 

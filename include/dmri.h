@@ -119,10 +119,13 @@ typedef struct {
 } dmri_seq;
 
 typedef struct {
-    double rtol;       /* ||r|| <= max(rtol ||b||, atol); reference 1e-9 (GCloudDmriSolver.py:220) */
+    double rtol;           /* ||r|| <= max(rtol ||b||, atol); reference 1e-9 (GCloudDmriSolver.py:220) */
     double atol;
-    int32_t maxit;     /* per time step */
-    int32_t profile;   /* 1: time the SpMV launches with HIP events (dmri_stats.spmv_ms) */
+    int32_t maxit;         /* per time step */
+    int32_t profile;       /* 1: time the SpMV launches with HIP events (dmri_stats.spmv_ms) */
+    int32_t lanes_per_row; /* SpMV lanes per (row, signal): 0 = library choice, else 1..32, power of two,
+                              lanes * min(nsig, 64) <= 64 (see dmri_tune) */
+    int32_t reserved;
 } dmri_solver_opts;
 
 typedef struct {
@@ -143,8 +146,19 @@ typedef struct {
 int dmri_solve(dmri_ctx *ctx, const dmri_seq *seq, const dmri_solver_opts *opts, double sig_out[2],
                dmri_stats *stats);
 
-/* Copy u^N of the last dmri_solve to host (2*ndof doubles, interleaved re/im, DOF order). */
-int dmri_get_solution(dmri_ctx *ctx, double *u);
+/* Batched solve: nsig signals with their own q[s] and gdir[3*s], sharing the mesh, the
+ * coefficients and the time profile of seq (seq->q and seq->gdir are ignored); signals go
+ * to the device in launch groups of up to 64 that advance together through every theta step
+ * and Krylov iteration on the shared sparsity pattern.  sig_out has 2*nsig doubles.
+ * No reference counterpart for the batching itself: the reference runs one signal per
+ * process (GCloudDmriSolver.py:232) and loops over b in its notebooks; each batched
+ * signal equals the one dmri_solve returns for that (q, gdir). */
+int dmri_solve_batched(dmri_ctx *ctx, int32_t nsig, const double *q, const double *gdir, const dmri_seq *seq,
+                       const dmri_solver_opts *opts, double *sig_out, dmri_stats *stats);
+
+/* Copy u^N of signal `sig` of the last launch group to host (2*ndof doubles, interleaved
+ * re/im, DOF order; the DOF of (compartment c, vertex v) is dof_of[c*nv+v]). */
+int dmri_get_solution(dmri_ctx *ctx, int32_t sig, double *u);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,58 @@ def test_signal_matches_oracle(name, b, g):
     assert abs(s_gpu.real - s_ref) <= 1e-9 * abs(s_ref)
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
+def test_lanes_per_row_do_not_change_signal(lanes):
+    mesh, kw = MESHES["cyl2"]
+    delta, Delta, k = 10600.0, 43100.0, 400.0
+    fA, fB = _pgse_gcloud(delta, Delta, k)
+    q = O.q_from_b_pgse(2000.0, delta, Delta)
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    s_ref, _ = ctx.solve(fA, fB, k, q, (1, 0, 0), rtol=1e-13, lanes=8)
+    s, _ = ctx.solve(fA, fB, k, q, (1, 0, 0), rtol=1e-13, lanes=lanes)
+    assert abs(s - s_ref) <= 1e-11 * abs(s_ref)
+
+
+@pytest.mark.parametrize("mode", ["shared_gdir", "per_gdir"])
+def test_batched_equals_single_and_oracle(mode):
+    mesh, kw = MESHES["cyl2"]
+    print(f"[batched] {mode}", flush=True)
+    delta, Delta, k = 10600.0, 43100.0, 400.0
+    fA, fB = _pgse_gcloud(delta, Delta, k)
+    bvals = np.array([0.0, 500.0, 1000.0, 2000.0, 4000.0, 3000.0, 1500.0])
+    qs = np.array([O.q_from_b_pgse(b, delta, Delta) for b in bvals])
+    if mode == "shared_gdir":
+        gd = np.tile([0.0, 1.0, 0.0], (len(qs), 1))
+    else:
+        rng = np.random.default_rng(3)
+        gd = rng.normal(size=(len(qs), 3))
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    sig, st = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13)
+    S = O.System(mesh.xyz, mesh.tets, mesh.phase, D=kw["D"], kappa=kw["kappa"])
+    for j in range(len(qs)):
+        s1, _ = ctx.solve(fA, fB, k, qs[j], gd[j], rtol=1e-13)
+        assert abs(sig[j] - s1) <= 1e-11 * max(abs(s1), 1e-12), (j, sig[j], s1)
+        if j in (0, 3, 4):
+            s_ref, _, _ = S.solve(gd[j], qs[j], k, fA, fB, rtol=1e-13)
+            assert abs(sig[j].real - s_ref) <= 1e-9 * abs(s_ref)
+    assert abs(sig[0].real - 1.0) <= 1e-11
+
+
+def test_batched_more_than_one_launch_group():
+    mesh, kw = MESHES["box4"]
+    delta, Delta, k = 1000.0, 10000.0, 200.0
+    fA, fB = O.gcloud_sequence(O.pgse_piecewise(delta, Delta), delta + Delta, k)
+    rng = np.random.default_rng(5)
+    nsig = 70
+    gd = rng.normal(size=(nsig, 3))
+    qs = np.sqrt(rng.uniform(0, 3000, nsig)) / np.sqrt(delta * delta * (Delta - delta / 3))
+    ctx = _gpu_system(mesh, 2e-3, 0.0)
+    sig, _ = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13)
+    for j in (0, 40, 63, 64, 69):
+        s1, _ = ctx.solve(fA, fB, k, qs[j], gd[j], rtol=1e-13)
+        assert abs(sig[j] - s1) <= 1e-11 * abs(s1)
+
+
 def test_boxmesh_golden_gpu():
     """ConvergenceTest.ipynb:423 prints 'Normalized signal: 8.440078e-01' (hmin=hmax=5.413e-01,
     i.e. BoxMesh n=16 on [-2.5,2.5]^3; D=2e-3, delta=1000, Delta=10000, b=1000, gdir z, k=10,
