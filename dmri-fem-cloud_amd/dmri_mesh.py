@@ -330,7 +330,7 @@ def write_dolfin_xml(mesh: TetMesh, path: str) -> None:
         fh.write('<?xml version="1.0"?>\n<dolfin xmlns:dolfin="http://fenicsproject.org">\n')
         fh.write('  <mesh celltype="tetrahedron" dim="3">\n')
         fh.write(f'    <vertices size="{mesh.nv}">\n')
-        for i, (x, y, z) in enumerate(mesh.xyz):
+        for i, (x, y, z) in enumerate(mesh.xyz.tolist()):
             fh.write(f'      <vertex index="{i}" x="{x!r}" y="{y!r}" z="{z!r}" />\n')
         fh.write('    </vertices>\n')
         fh.write(f'    <cells size="{mesh.nc}">\n')

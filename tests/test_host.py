@@ -1,0 +1,135 @@
+"""Host-side logic and the C ABI surface -- CPU only (no GPU calls)."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import dmri_mesh as dm
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "dmri-fem-cloud_amd", "lib", "libdmri.so")
+HEADER = os.path.join(ROOT, "include", "dmri.h")
+
+
+def _declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(dmri_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_function():
+    if not os.path.exists(LIB):
+        pytest.skip("libdmri.so not built (run __graft_entry__.build())")
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (dmri_[a-z_]+)\b", out))
+    declared = _declared_functions()
+    assert declared, "no declarations parsed"
+    missing = [f for f in declared if f not in exported]
+    assert not missing, missing
+
+
+def test_ctypes_binding_loads_and_knows_every_declaration():
+    if not os.path.exists(LIB):
+        pytest.skip("libdmri.so not built")
+    import dmri_native as N
+
+    assert set(_declared_functions()) == set(N.EXPORTED)
+    assert "gfx950" in N.version()
+
+
+def _faces(tets):
+    f = np.concatenate([np.sort(tets[:, [1, 2, 3]], 1), np.sort(tets[:, [0, 2, 3]], 1),
+                        np.sort(tets[:, [0, 1, 3]], 1), np.sort(tets[:, [0, 1, 2]], 1)])
+    _, counts = np.unique(f, axis=0, return_counts=True)
+    return f, counts
+
+
+@pytest.mark.parametrize("maker", ["box", "cyl", "cyl2", "cylbox"])
+def test_generated_meshes_are_conforming_and_positive(maker):
+    if maker == "box":
+        m = dm.box_mesh((0, 0, 0), (2, 3, 4), 3, 4, 5)
+        vol = 24.0
+    elif maker == "cyl":
+        m = dm.cylinder_mesh(5.0, 10.0, 4000, seed=1000)
+        vol = None
+    elif maker == "cyl2":
+        m = dm.cylinder_mesh(8.0, 6.0, 5000, seed=1001, interface_radii=(4.0,))
+        vol = None
+    else:
+        m = dm.cylinders_box_mesh(20.0, 20.0, 4.0, ncyl=6, rmin=1.5, rmax=3.0, target_nv=12000, seed=1002)
+        vol = 20.0 * 20.0 * 4.0
+    v = dm.cell_volumes(m.xyz, m.tets)
+    assert v.min() > 1e-6 * v.max()
+    _, counts = _faces(m.tets)
+    assert counts.max() <= 2  # no face shared by three cells
+    if vol is not None:
+        assert abs(v.sum() - vol) <= 1e-9 * vol
+    if m.phase is not None:
+        assert set(np.unique(m.phase)) == {0, 1}
+
+
+def test_cylinder_volume_and_interface_radius():
+    m = dm.cylinder_mesh(8.0, 6.0, 20000, seed=1001, interface_radii=(4.0,))
+    v = dm.cell_volumes(m.xyz, m.tets)
+    inner = v[m.phase == 1].sum()
+    assert abs(v.sum() / (np.pi * 64 * 6) - 1) < 5e-3
+    assert abs(inner / (np.pi * 16 * 6) - 1) < 2e-2
+
+
+def test_meshes_are_seed_deterministic():
+    a = dm.cylinder_mesh(5.0, 10.0, 3000, seed=42)
+    b = dm.cylinder_mesh(5.0, 10.0, 3000, seed=42)
+    c = dm.cylinder_mesh(5.0, 10.0, 3000, seed=43)
+    assert np.array_equal(a.xyz, b.xyz) and np.array_equal(a.tets, b.tets)
+    assert not np.array_equal(a.xyz, c.xyz)
+
+
+def test_dolfin_xml_round_trip(tmp_path):
+    m = dm.box_mesh((0, 0, 0), (1, 1, 1), 2, 2, 2)
+    p = tmp_path / "m.xml"
+    dm.write_dolfin_xml(m, str(p))
+    r = dm.read_dolfin_xml(str(p))
+    assert np.array_equal(r.tets, m.tets) and np.allclose(r.xyz, m.xyz, rtol=0, atol=0)
+
+
+def test_dmrifemlib_parameters_match_closed_forms():
+    if not os.path.exists(LIB):
+        pytest.skip("libdmri.so not built")
+    import DmriFemLib as L
+
+    mesh = L.BoxMesh(L.Point(-2.5, -2.5, -2.5), L.Point(2.5, 2.5, 2.5), 4, 4, 4)
+    p = L.MRI_parameters()
+    p.bvalue = 1000
+    p.delta, p.Delta = 10600, 43100
+    p.set_gradient_dir(mesh, 2, 0, 0)
+    p.T = p.Delta + p.delta
+    p.fs_sym = L.sp.Piecewise((1., p.s < p.delta), (0., p.s < p.Delta), (-1., p.s < p.T), (0., True))
+    p.Apply()
+    assert abs(p.qvalue - O.q_from_b_pgse(1000.0, 10600.0, 43100.0)) <= 1e-12 * p.qvalue
+    assert abs(p.gdir.x() - 1.0) < 1e-15
+    simu = L.MRI_simulation()
+    simu.k = 200
+    fA, fB = simu._sequence(p)
+    rA, rB = O.gcloud_sequence(O.pgse_piecewise(10600.0, 43100.0), 53700.0, 200.0)
+    assert np.array_equal(fA, rA) and np.array_equal(fB, rB)
+    dom = L.MyDomain(mesh, p)
+    assert f"{dom.hmin:.3e}" == f"{np.sqrt(3) * 5 / 4:.3e}"
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, cwd=ROOT, timeout=120)
+    assert r.returncode != 0 and "one GPU" in r.stdout
+
+
+def test_bench_sequence_matches_oracle():
+    sys.path.insert(0, ROOT)
+    import bench
+
+    a = bench.pgse_sequence(10600.0, 43100.0, 200.0)
+    b = O.gcloud_sequence(O.pgse_piecewise(10600.0, 43100.0), 53700.0, 200.0)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])

@@ -1,0 +1,292 @@
+"""Pins the CPU oracle (oracle/dmri_oracle.c) -- CPU only.
+
+Each check is independent of the oracle's own formulas:
+  * element matrices against tensor-product Gauss quadrature on collapsed coordinates
+    (degree far above the P1 integrands) -- a dropped term, wrong factor or swapped index fails;
+  * sparsity against a pure-Python set union;
+  * assembled matrices against identities fixed by the mathematics (partition of unity,
+    constant null space of K and of the interface coupling, exact P1 reproduction of x);
+  * the theta scheme + BiCGStab against a dense direct solve of the same discrete scheme;
+  * b = 0 and T2-only closed forms; the reference notebook's printed BoxMesh result.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import dmri_mesh as dm
+import oracle as O
+
+
+# ----------------------------------------------------------------------------------------
+# independent quadrature
+# ----------------------------------------------------------------------------------------
+def _gl01(n):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def tet_quadrature(X, n=7):
+    """Points (physical) and weights on tetrahedron X (4x3) via the Duffy map of the cube."""
+    u, wu = _gl01(n)
+    U, V, W = np.meshgrid(u, u, u, indexing="ij")
+    WU, WV, WW = np.meshgrid(wu, wu, wu, indexing="ij")
+    xi1 = U
+    xi2 = V * (1 - U)
+    xi3 = W * (1 - U) * (1 - V)
+    jac = (1 - U) ** 2 * (1 - V)
+    w = (WU * WV * WW * jac).ravel()
+    xi = np.stack([xi1.ravel(), xi2.ravel(), xi3.ravel()], axis=1)
+    E = (X[1:] - X[0]).T  # columns e1 e2 e3
+    det = np.linalg.det(E)
+    pts = X[0] + xi @ E.T
+    lam = np.concatenate([1 - xi.sum(axis=1, keepdims=True), xi], axis=1)
+    return pts, w * abs(det), lam, E
+
+
+def tri_quadrature(X, n=7):
+    u, wu = _gl01(n)
+    U, V = np.meshgrid(u, u, indexing="ij")
+    WU, WV = np.meshgrid(wu, wu, indexing="ij")
+    xi1, xi2 = U.ravel(), (V * (1 - U)).ravel()
+    w = (WU * WV * (1 - U)).ravel()
+    area2 = np.linalg.norm(np.cross(X[1] - X[0], X[2] - X[0]))
+    lam = np.stack([1 - xi1 - xi2, xi1, xi2], axis=1)
+    return w * area2, lam
+
+
+def _dense_block(S, which, dofs):
+    A = S.dense(which)
+    return A[np.ix_(dofs, dofs)]
+
+
+def test_single_tet_element_matrices_match_quadrature():
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-3, 3, size=(4, 3))
+    if np.linalg.det((X[1:] - X[0]).T) < 0:
+        X[[1, 2]] = X[[2, 1]]
+    A = rng.normal(size=(3, 3))
+    Dt = A @ A.T * 1e-3 + np.eye(3) * 1e-3
+    S = O.System(X, np.array([[0, 1, 2, 3]]), None, D=0.0, invT2=0.37, Dcell=Dt.reshape(1, 9))
+    pts, w, lam, E = tet_quadrature(X)
+    grads = np.zeros((4, 3))
+    Einv = np.linalg.inv(E)  # rows: grad of lambda_1..3
+    grads[1:] = Einv
+    grads[0] = -Einv.sum(axis=0)
+    vol = w.sum()
+    M = np.einsum("q,qa,qb->ab", w, lam, lam)
+    K = vol * grads @ Dt @ grads.T
+    J = [np.einsum("q,q,qa,qb->ab", w, pts[:, d], lam, lam) for d in range(3)]
+    got = {k: S.dense(k) for k in ("m", "k", "r", "jx", "jy", "jz")}
+    scale = lambda a: np.abs(a).max()
+    assert np.abs(got["m"] - M).max() <= 1e-13 * scale(M)
+    assert np.abs(got["r"] - 0.37 * M).max() <= 1e-13 * scale(M)
+    assert np.abs(got["k"] - K).max() <= 1e-12 * scale(K)
+    for d, k in enumerate(("jx", "jy", "jz")):
+        assert np.abs(got[k] - J[d]).max() <= 1e-12 * scale(J[d]), k
+
+
+def test_interface_facet_mass_matches_quadrature():
+    # two tetrahedra sharing face (1,2,3), different phases
+    X = np.array([[0.0, 0.0, 0.0], [1.3, 0.1, 0.2], [0.2, 1.1, -0.1], [0.1, 0.3, 0.9], [1.2, 1.0, 1.1]])
+    tets = np.array([[0, 1, 2, 3], [4, 1, 2, 3]])
+    S = O.System(X, tets, np.array([0, 1]), D=1e-3, kappa=2.5e-5)
+    assert S.fv.shape == (1, 3) and tuple(S.fv[0]) == (1, 2, 3)
+    w, lam = tri_quadrature(X[[1, 2, 3]])
+    Fm = 2.5e-5 * np.einsum("q,qa,qb->ab", w, lam, lam)
+    gi = S.dense("gi")
+    d0 = [S.dof_of[v] for v in (1, 2, 3)]
+    d1 = [S.dof_of[S.nv + v] for v in (1, 2, 3)]
+    assert np.abs(gi[np.ix_(d0, d0)] - Fm).max() <= 1e-13 * Fm.max()
+    assert np.abs(gi[np.ix_(d1, d1)] - Fm).max() <= 1e-13 * Fm.max()
+    assert np.abs(gi[np.ix_(d0, d1)] + Fm).max() <= 1e-13 * Fm.max()
+    assert np.abs(gi[np.ix_(d1, d0)] + Fm).max() <= 1e-13 * Fm.max()
+    # vertex 0 only touches the phase-0 cell, vertex 4 only the phase-1 cell: dead DOFs dropped
+    assert S.dof_of[S.nv + 0] == -1 and S.dof_of[4] == -1
+    assert S.ndof == 8
+
+
+def _tiny_two_comp():
+    m = dm.box_mesh((0, 0, 0), (3, 3, 3), 3, 3, 3)
+    c = m.xyz[m.tets].mean(axis=1)
+    phase = ((np.abs(c - 1.5) < 0.6).all(axis=1)).astype(np.int32)
+    return m, phase
+
+
+def _python_pattern(tets, phase, dof_of, nv, fv):
+    rows = {}
+    for t, T in enumerate(tets):
+        c = 0 if phase is None else phase[t]
+        d = [dof_of[c * nv + v] for v in T]
+        for a in d:
+            rows.setdefault(a, set()).update(d)
+    for F in fv:
+        d = [dof_of[v] for v in F] + [dof_of[nv + v] for v in F]
+        for a in d:
+            rows.setdefault(a, set()).update(d)
+    return rows
+
+
+@pytest.mark.parametrize("two", [False, True])
+def test_pattern_equals_python_set_union(two):
+    if two:
+        m, phase = _tiny_two_comp()
+    else:
+        m, phase = dm.cylinder_mesh(2.0, 2.0, 300, seed=3), None
+    S = O.System(m.xyz, m.tets, phase)
+    rows = _python_pattern(m.tets, phase, S.dof_of, S.nv, S.fv)
+    assert len(rows) == S.ndof
+    for r in range(S.ndof):
+        assert list(S.col[S.rowptr[r]:S.rowptr[r + 1]]) == sorted(rows[r])
+    if two:
+        assert S.fv.shape[0] > 0
+
+
+def test_assembled_identities_two_compartments():
+    m, phase = _tiny_two_comp()
+    S = O.System(m.xyz, m.tets, phase, D=(2e-3, 1e-3), kappa=1e-5)
+    M = S.dense("m")
+    assert abs(M.sum() - 27.0) <= 1e-12 * 27.0 and abs(S.volume - 27.0) <= 1e-12
+    for k in ("m", "k", "r", "gi", "jx", "jy", "jz"):
+        A = S.dense(k)
+        assert np.abs(A - A.T).max() <= 1e-15 * max(np.abs(A).max(), 1e-300), k
+    one = np.ones(S.ndof)
+    assert np.abs(S.dense("k") @ one).max() <= 1e-15 * np.abs(S.dense("k")).max() * 30
+    assert np.abs(S.dense("gi") @ one).max() <= 1e-18
+    # x is reproduced exactly by P1: Jx 1 = M x_dof
+    xyz_dof = np.zeros((S.ndof, 3))
+    for c in range(2):
+        for v in range(S.nv):
+            d = S.dof_of[c * S.nv + v]
+            if d >= 0:
+                xyz_dof[d] = m.xyz[v]
+    for d, k in enumerate(("jx", "jy", "jz")):
+        assert np.abs(S.dense(k) @ one - M @ xyz_dof[:, d]).max() <= 1e-13 * np.abs(M).max() * 3
+    # compartment volumes: cells of phase 1 are the central unit cube
+    n0 = int((S.dof_of[:S.nv] >= 0).sum())
+    assert abs(S.w[n0:].sum() - 1.0) <= 1e-13 and abs(S.w[:n0].sum() - 26.0) <= 1e-12
+
+
+def _dense_theta(S, g, q, dt, fA, fB, theta, invT2=None):
+    M, L = S.dense("m"), S.dense(S.l())
+    Jg = S.dense(S.jg(g))
+    u = np.ones(S.ndof, dtype=complex)
+    for a, b in zip(fA, fB):
+        A = M / dt + theta * L + 1j * theta * q * a * Jg
+        B = M / dt - (1 - theta) * L - 1j * (1 - theta) * q * b * Jg
+        u = np.linalg.solve(A, B @ u)
+    return (S.w @ u.real) / S.volume, u
+
+
+@pytest.mark.parametrize("two", [False, True])
+def test_theta_bicgstab_matches_dense_direct_solve(two):
+    if two:
+        m, phase = _tiny_two_comp()
+        S = O.System(m.xyz, m.tets, phase, D=(3e-3, 1.5e-3), kappa=2e-4, invT2=(1 / 5e4, 1 / 8e4))
+    else:
+        m = dm.cylinder_mesh(2.0, 3.0, 400, seed=5)
+        S = O.System(m.xyz, m.tets, None, D=3e-3, invT2=1 / 6e4)
+    delta, Delta, k = 3000.0, 9000.0, 300.0
+    fA, fB = O.gcloud_sequence(O.pgse_piecewise(delta, Delta), delta + Delta, k)
+    q = O.q_from_b_pgse(3000.0, delta, Delta)
+    g = (0.3, -1.0, 0.5)
+    s_dense, u_dense = _dense_theta(S, g, q, k, fA, fB, 0.5)
+    s_orc, u_orc, st = S.solve(g, q, k, fA, fB, rtol=1e-14)
+    assert abs(s_orc - s_dense) <= 1e-11 * abs(s_dense)
+    assert np.abs(u_orc - u_dense).max() <= 1e-10 * np.abs(u_dense).max()
+    # theta = 1 (backward Euler) through the same path
+    s_d1, _ = _dense_theta(S, g, q, k, fA, fB, 1.0)
+    s_o1, _, _ = S.solve(g, q, k, fA, fB, theta=1.0, rtol=1e-14)
+    assert abs(s_o1 - s_d1) <= 1e-11 * abs(s_d1)
+
+
+def test_zero_gradient_and_T2_closed_form():
+    m, phase = _tiny_two_comp()
+    T2, k, theta = 8e4, 200.0, 0.5
+    S = O.System(m.xyz, m.tets, phase, D=3e-3, kappa=1e-5, invT2=1.0 / T2)
+    fA, fB = O.gcloud_sequence(O.pgse_piecewise(10600.0, 43100.0), 53700.0, k)
+    assert len(fA) == 270
+    s, u, _ = S.solve((1, 0, 0), 0.0, k, fA, fB, rtol=1e-15)
+    r = (1 / k - (1 - theta) / T2) / (1 / k + theta / T2)
+    assert abs(s - r ** len(fA)) <= 1e-12
+    # default T2 = 1e16 of MRI_parameters (DmriFemLib.py:807) still enters through 1/T2
+    S0 = O.System(m.xyz, m.tets, phase, D=3e-3, kappa=1e-5)
+    s0, _, _ = S0.solve((1, 0, 0), 0.0, k, fA, fB, rtol=1e-15)
+    # (the 1e-16 relative decay per step sits at the FP64 rounding level, so only its size is checked)
+    r0 = (1 / k - (1 - theta) / 1e16) / (1 / k + theta / 1e16)
+    assert abs(s0 - r0 ** len(fA)) <= 1e-12 and abs(s0 - 1.0) < 1e-11
+
+
+def test_sequences_and_q_match_reference_prints():
+    # ExplicitImplementation.ipynb:472 prints "q: 1.500e-05" for b=1000, delta=10600, Delta=43100
+    q = O.q_from_b_pgse(1000.0, 10600.0, 43100.0)
+    assert f"{q:.3e}" == "1.500e-05"
+    assert abs(q - 1.4997863706e-5) <= 1e-15
+    # ConvergenceTest.ipynb:423 prints q 3.216e-04 and g 1.202 for delta=1000, Delta=10000, b=1000
+    q2 = O.q_from_b_pgse(1000.0, 1000.0, 10000.0)
+    assert f"{q2:.6e}" == "3.216338e-04" and f"{q2 / O.GAMMA_RATIO * 1e12:.3f}" == "1.202"
+    # profile endpoints: sympy Piecewise exclusive at T, comri FT inclusive at Delta+delta
+    f_py, f_c = O.pgse_piecewise(1000.0, 10000.0), O.pgse_comri(1000.0, 10000.0)
+    assert f_py(11000.0) == 0.0 and f_c(11000.0) == -1.0
+    assert f_py(1000.0) == 0.0 and f_c(1000.0) == 0.0 and f_py(0.0) == 1.0
+    fA, fB = O.gcloud_sequence(f_py, 11000.0, 10.0)
+    assert len(fA) == 1101 and fB[0] == fA[0] == 1.0 and np.all(fB[1:] == fA[:-1])
+    fA2, fB2 = O.comri_sequence(f_c, 11000.0, 10.0)
+    assert np.array_equal(fA2, fB2) and len(fA2) == 1101
+    fA3, _ = O.comri_sequence(f_c, 11000.0, 10.0, until_T=True)
+    assert len(fA3) == 1100
+
+
+def test_free_diffusion_large_box_approaches_exp_minus_bD():
+    m = dm.box_mesh((-30, -30, -30), (30, 30, 30), 12, 12, 12)  # h = 5 um, diffusion length ~5 um
+    S = O.System(m.xyz, m.tets, None, D=2e-3)
+    delta, Delta, k = 1000.0, 5000.0, 50.0
+    fA, fB = O.gcloud_sequence(O.pgse_piecewise(delta, Delta), delta + Delta, k)
+    s, _, _ = S.solve((1, 0, 0), O.q_from_b_pgse(300.0, delta, Delta), k, fA, fB, rtol=1e-13)
+    assert abs(s - math.exp(-300.0 * 2e-3)) < 1.5e-2  # discretisation error of the coarse mesh
+
+
+@pytest.mark.slow
+def test_boxmesh_golden_reference_notebook():
+    """ConvergenceTest.ipynb:423: 'Normalized signal: 8.440078e-01, ... hmin: 5.413e-01, hmax:
+    5.413e-01, whole_vol: 125.000' for BoxMesh [-2.5,2.5]^3 (n=16: hmin = sqrt(3)*5/16), D=2e-3,
+    delta=1000, Delta=10000, b=1000, gdir z, k=10, loop while t < T with lagged f."""
+    m = dm.box_mesh((-2.5,) * 3, (2.5,) * 3, 16, 16, 16)
+    hmin, hmax = m.hminmax()
+    assert f"{hmin:.3e}" == "5.413e-01" and f"{hmax:.3e}" == "5.413e-01"
+    S = O.System(m.xyz, m.tets, None, D=2e-3)
+    assert f"{S.volume:.3f}" == "125.000"
+    fA, fB = O.lagged_sequence_until(O.pgse_piecewise(1000.0, 10000.0), 11000.0, 10.0)
+    s, _, _ = S.solve((0, 0, 1), O.q_from_b_pgse(1000.0, 1000.0, 10000.0), 10.0, fA, fB, rtol=1e-14)
+    assert f"{s:.6e}" == "8.440078e-01"
+
+
+def test_oracle_is_thread_count_independent():
+    m = dm.cylinder_mesh(3.0, 3.0, 1500, seed=9)
+    S = O.System(m.xyz, m.tets, None)
+    fA, fB = O.gcloud_sequence(O.pgse_piecewise(3000.0, 9000.0), 12000.0, 400.0)
+    q = O.q_from_b_pgse(2000.0, 3000.0, 9000.0)
+    n0 = O.num_threads()
+    try:
+        O.set_threads(1)
+        a, _, _ = S.solve((1, 1, 0), q, 400.0, fA, fB)
+        O.set_threads(max(2, n0))
+        b, _, _ = S.solve((1, 1, 0), q, 400.0, fA, fB)
+    finally:
+        O.set_threads(n0)
+    assert a == b
+
+
+def test_planted_sign_error_is_detected():
+    """A J sign flip in the scheme (a plausible mistake) is caught by the dense check."""
+    m = dm.cylinder_mesh(2.0, 3.0, 400, seed=5)
+    S = O.System(m.xyz, m.tets, None, D=3e-3)
+    fA, fB = O.gcloud_sequence(O.pgse_piecewise(3000.0, 9000.0), 12000.0, 300.0)
+    q = O.q_from_b_pgse(3000.0, 3000.0, 9000.0)
+    s_dense, _ = _dense_theta(S, (1, 0, 0), q, 300.0, fA, fB, 0.5)
+    # planted mistake: explicit side with the implicit sign
+    s_bad, _ = _dense_theta(S, (1, 0, 0), q, 300.0, fA, -np.asarray(fB), 0.5)
+    s_orc, _, _ = S.solve((1, 0, 0), q, 300.0, fA, fB, rtol=1e-14)
+    assert abs(s_orc - s_dense) <= 1e-11 * abs(s_dense)
+    assert abs(s_orc - s_bad) > 1e-6
