@@ -64,6 +64,7 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-12)
     ap.add_argument("--lanes", type=int, default=0, help="SpMV lanes per (row, signal); 0 = library choice")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-signal-at-a-time comparison")
+    ap.add_argument("--guess", type=int, default=0, help="Krylov initial guess: 0 extrapolated, 1 previous step")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -91,7 +92,8 @@ def main():
 
     def one_step(profile=False):
         ctx.assemble()
-        sig, st = ctx.solve_batched(fA, fB, k, qs, gdir, rtol=args.rtol, profile=profile, lanes=args.lanes)
+        sig, st = ctx.solve_batched(fA, fB, k, qs, gdir, rtol=args.rtol, profile=profile, lanes=args.lanes,
+                                    guess=args.guess)
         return list(sig), [st]
 
     for _ in range(args.warmup):

@@ -61,7 +61,7 @@ T read_scalar(const T *d, hipStream_t s) {
 constexpr int kEventPool = 512;
 
 struct SolverWork {
-    DevArray<double2> x, r, rh, p, v, t, b;
+    DevArray<double2> x, r, rh, p, v, t, b, xprev;
     DevArray<double> sc, vals, bre, part, tot, coef, sig, spart;
     DevArray<KrylovState> st;
     DevArray<int> all_done, mask;
@@ -107,6 +107,7 @@ struct dmri_ctx {
     SolverWork work;
     DevArray<double> part, dsig;
     int last_S = 1;
+    std::vector<std::pair<long long, int>> tuned;  // (S*2 + per_gdir) -> lanes per row
     bool solved = false;
     hipEvent_t ev0[kEventPool], ev1[kEventPool];
     bool events_made = false;
@@ -371,6 +372,7 @@ void ensure_group_workspace(dmri_ctx *c, const SolveGeom &g, int nsteps, bool pe
         w.v.alloc(nS);
         w.t.alloc(nS);
         w.b.alloc(nS);
+        w.xprev.alloc(nS);
     }
     if (w.sc.n != (size_t)c->ndof) w.sc.alloc(c->ndof);
     const size_t nval = (size_t)c->nnz * (per_gdir ? 4 : 2);
@@ -402,7 +404,10 @@ void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_
                  const dmri_solver_opts *opts, double *sig, dmri_stats &st) {
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
-    const int lanes = opts->lanes_per_row > 0 ? opts->lanes_per_row : auto_lanes(S);
+    int lanes = opts->lanes_per_row > 0 ? opts->lanes_per_row : auto_lanes(S);
+    if (opts->lanes_per_row <= 0)
+        for (const auto &t : c->tuned)
+            if (t.first == (long long)S * 2 + (per_gdir ? 1 : 0)) lanes = t.second;
     const SolveGeom geo = make_geom(n, lanes, S);
     const int N = seq->nsteps;
     ensure_group_workspace(c, geo, N, per_gdir);
@@ -442,7 +447,10 @@ void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_
     a.vals = w.vals.p;
     a.bre = w.bre.p;
     a.per_gdir = per_gdir;
-    a.x = w.x.p;
+    double2 *xc = w.x.p, *xp = w.xprev.p;
+    const bool extrap = opts->initial_guess == 0;
+    a.x = xc;
+    a.xprev = xp;
     a.r = w.r.p;
     a.rh = w.rh.p;
     a.p = w.p.p;
@@ -471,7 +479,18 @@ void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_
         std::memset(zero_st.data(), 0, sizeof(KrylovState) * S);
         DMRI_HIP(hipMemcpyAsync(w.st.p, zero_st.data(), sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
         DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
-        launch_rhs_residual(geo, a, false, nullptr, s);
+        if (extrap && step > 0) {
+            // b from u^n; xp <- 2 u^n - u^(n-1); swap so that the guess is iterated on
+            launch_rhs_residual(geo, a, 2, nullptr, s);
+            std::swap(xc, xp);
+            a.x = xc;
+            a.xprev = xp;
+            launch_rhs_residual(geo, a, 1, nullptr, s);
+        } else {
+            launch_rhs_residual(geo, a, 0, nullptr, s);
+            if (extrap)
+                DMRI_HIP(hipMemcpyAsync(xp, xc, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        }
         int it = 0, it_before = 0, restarts = 0;
         for (;;) {
             const int nl = std::min(chunk, kEventPool / 2);
@@ -537,7 +556,7 @@ void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_
             DMRI_HIP(hipMemcpyAsync(w.st.p, w.h_st, sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
             DMRI_HIP(hipMemcpyAsync(w.mask.p, mask.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
             DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
-            launch_rhs_residual(geo, a, true, w.mask.p, s);
+            launch_rhs_residual(geo, a, 1, w.mask.p, s);
             DMRI_HIP(hipStreamSynchronize(s));
         }
         int step_max = 0;
@@ -552,6 +571,8 @@ void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_
         st.restarts += restarts;
         chunk = std::max(1, step_max);
     }
+    if (xc != w.x.p)
+        DMRI_HIP(hipMemcpyAsync(w.x.p, xc, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
     launch_signal(n, S, c->w.p, w.sc.p, w.x.p, w.spart.p, w.sig.p, s);
     std::vector<double> hs((size_t)S * kNV);
     DMRI_HIP(hipMemcpyAsync(hs.data(), w.sig.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
@@ -561,6 +582,89 @@ void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_
         sig[2 * j + 1] = hs[(size_t)j * kNV + 1] / c->volume;
     }
     c->last_S = S;
+}
+
+// Time the SpMV kernel of every lanes-per-row candidate for S signals on the assembled pattern
+// and remember the fastest for later solves of this context.
+void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best, double *us) {
+    require(c->assembled, DMRI_ERR_STATE, "dmri_tune: call dmri_assemble first");
+    require(S >= 1 && S <= kMaxSignals, DMRI_ERR_ARG, "dmri_tune: nsig must lie in [1, 64]");
+    set_device(c);
+    hipStream_t s = c->stream;
+    const int64_t n = c->ndof;
+    const int cand[6] = {1, 2, 4, 8, 16, 32};
+    SolveGeom g0 = make_geom(n, 1, S);
+    ensure_group_workspace(c, g0, 1, per_gdir);
+    SolverWork &w = c->work;
+    auto &V = c->val;
+    const double gx = 1.0;
+    launch_prep_scale(n, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, 1.0, 0.5,
+                      w.sc.p, s);
+    launch_prep_values(n, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
+                       V[DMRI_MAT_I].p, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, gx, 0.0, 0.0,
+                       1.0, 0.5, w.vals.p, w.bre.p, s);
+    launch_init_x(n, S, w.sc.p, w.p.p, s);
+    std::vector<double> coef((size_t)S * 3, 1e-4);
+    DMRI_HIP(hipMemcpyAsync(w.coef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
+    std::vector<KrylovState> zero(S);
+    std::memset(zero.data(), 0, sizeof(KrylovState) * S);
+    DMRI_HIP(hipMemcpyAsync(w.st.p, zero.data(), sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
+    DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
+    double best_us = 1e300;
+    *best = auto_lanes(S);
+    for (int i = 0; i < 6; ++i) {
+        us[i] = -1.0;
+        const int G = cand[i];
+        if (G * S > 64) continue;
+        const SolveGeom geo = make_geom(n, G, S);
+        const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
+        if (w.part.n < 3 * np) w.part.alloc(3 * np);
+        SolveArgs a{};
+        a.n = n;
+        a.rowptr = c->rowptr.p;
+        a.col = c->col.p;
+        a.vals = w.vals.p;
+        a.bre = w.bre.p;
+        a.per_gdir = per_gdir;
+        a.coefA = w.coef.p;
+        a.coefB = w.coef.p;
+        a.x = w.x.p;
+        a.r = w.r.p;
+        a.rh = w.p.p;
+        a.p = w.p.p;
+        a.v = w.v.p;
+        a.t = w.t.p;
+        a.b = w.b.p;
+        a.xprev = w.xprev.p;
+        a.partU = w.part.p;
+        a.partV = w.part.p + np;
+        a.partT = w.part.p + 2 * np;
+        a.st = w.st.p;
+        a.all_done = w.all_done.p;
+        a.rtol = 1e-12;
+        a.maxit = 1;
+        launch_spmv_v(geo, a, 0, s);  // warm-up
+        const int reps = 5;
+        DMRI_HIP(hipEventRecord(c->ev0[0], s));
+        for (int k = 0; k < reps; ++k) launch_spmv_v(geo, a, 0, s);
+        DMRI_HIP(hipEventRecord(c->ev1[0], s));
+        DMRI_HIP(hipEventSynchronize(c->ev1[0]));
+        float ms = 0;
+        DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[0], c->ev1[0]));
+        us[i] = 1e3 * ms / reps;
+        if (us[i] < best_us) {
+            best_us = us[i];
+            *best = G;
+        }
+    }
+    const long long key = (long long)S * 2 + (per_gdir ? 1 : 0);
+    bool found = false;
+    for (auto &t : c->tuned)
+        if (t.first == key) {
+            t.second = *best;
+            found = true;
+        }
+    if (!found) c->tuned.emplace_back(key, *best);
 }
 
 void validate_seq(const dmri_seq *seq, const dmri_solver_opts *opts) {
@@ -809,6 +913,31 @@ int dmri_solve_batched(dmri_ctx *c, int32_t nsig, const double *q, const double 
     return guarded([&] {
         require(c != nullptr, DMRI_ERR_ARG, "null context");
         solve_batched_impl(c, nsig, q, gdir, seq, opts, sig_out, stats);
+    });
+}
+
+int dmri_tune(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t *best_lanes, double *us_per_launch) {
+    return guarded([&] {
+        require(c != nullptr && best_lanes != nullptr && us_per_launch != nullptr, DMRI_ERR_ARG, "null argument");
+        int best = 0;
+        tune_impl(c, nsig, per_gdir != 0, &best, us_per_launch);
+        *best_lanes = best;
+    });
+}
+
+int dmri_set_tuning(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t lanes) {
+    return guarded([&] {
+        require(c != nullptr, DMRI_ERR_ARG, "null context");
+        require(nsig >= 1 && nsig <= kMaxSignals, DMRI_ERR_ARG, "nsig must lie in [1, 64]");
+        require(lanes >= 1 && lanes <= 32 && !(lanes & (lanes - 1)) && lanes * nsig <= 64, DMRI_ERR_ARG,
+                "lanes must be a power of two in [1, 32] with lanes * nsig <= 64");
+        const long long key = (long long)nsig * 2 + (per_gdir ? 1 : 0);
+        for (auto &t : c->tuned)
+            if (t.first == key) {
+                t.second = lanes;
+                return;
+            }
+        c->tuned.emplace_back(key, lanes);
     });
 }
 

@@ -109,7 +109,7 @@ struct SolveArgs {
     bool per_gdir;
     const double *coefA;   // [S*3] implicit-side imaginary coefficients of this step
     const double *coefB;   // [S*3] explicit-side coefficients of this step
-    double2 *x, *r, *rh, *p, *v, *t, *b;
+    double2 *x, *r, *rh, *p, *v, *t, *b, *xprev;
     double *partU, *partV, *partT;  // [grid][S][kNV] each
     double *totU, *totV, *totT;     // [S][kNV] finalize outputs, or null: consumers reduce the partials
     KrylovState *st;
@@ -117,7 +117,9 @@ struct SolveArgs {
     double rtol, atol;
     int maxit;
 };
-void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, bool restart, const int *mask, hipStream_t s);
+// mode 0: b and r from x; 1: r from the stored b (masked signals, nullptr = all); 2: b only and
+// xprev <- 2 x - xprev
+void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const int *mask, hipStream_t s);
 void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
 void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
 void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);

@@ -789,13 +789,16 @@ __device__ __forceinline__ Lane spmv_lane(int S) {
     return Lane{(int)threadIdx.x / gs, rem / G, rem % G};
 }
 
-// b = (Bre + i cB.J) x  (or the stored b on a restart);  r = b - (Are + i cA.J) x;  r^ = r.
-// partials U: [|r|^2, 0, |r|^2, |b|^2]
+// mode 0: b = (Bre + i cB.J) x, r = b - (Are + i cA.J) x, r^ = r          (start of a step)
+// mode 1: r = b - (Are + i cA.J) x with the stored b, for the masked signals (restart, or
+//         after an extrapolated initial guess; mask == nullptr selects every signal)
+// mode 2: b = (Bre + i cB.J) x only, and xprev <- 2 x - xprev (the extrapolated guess)
+// partials U (modes 0, 1): [|r|^2, 0, |r|^2, |b|^2]
 template <int G, bool PG>
-__global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int S, int rpb, int restart,
+__global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int S, int rpb, int mode,
                                                                 const int *mask) {
     const Lane L = spmv_lane<G>(S);
-    const bool sel = !restart || mask[L.s];
+    const bool sel = mode != 1 || mask == nullptr || mask[L.s];
     const int S_ = S;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     for (int64_t row = (int64_t)blockIdx.x * rpb + L.lrow; row < a.n; row += (int64_t)gridDim.x * rpb) {
@@ -803,42 +806,66 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
         const double2 *x = a.x;
         const int s = L.s;
         Acc<PG> acc;
-        row_accumulate<G, PG>(a.rowptr, a.col, a.vals, row, L.g, [&](int c) { return x[(int64_t)c * S_ + s]; },
-                              acc);
+        if (mode != 2) {
+            row_accumulate<G, PG>(a.rowptr, a.col, a.vals, row, L.g,
+                                  [&](int c) { return x[(int64_t)c * S_ + s]; }, acc);
+            group_reduce_acc<G, PG>(acc);
+        }
         double2 sb = make_double2(0.0, 0.0);
-        if (!restart) {
+        Acc<PG> jacc;
+        if (mode != 1) {
+            jacc.a = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int d = 0; d < (PG ? 3 : 1); ++d) jacc.j[d] = make_double2(0.0, 0.0);
             const int e1 = a.rowptr[row + 1];
             for (int e = a.rowptr[row] + L.g; e < e1; e += G) {
                 const double bv = a.bre[e];
                 const double2 xv = x[(int64_t)a.col[e] * S_ + s];
                 sb.x += bv * xv.x;
                 sb.y += bv * xv.y;
+                if (mode == 2) {
+                    // J part of the explicit side (mode 0 reuses the implicit accumulation)
+                    if (PG) {
+                        const double4 v = *reinterpret_cast<const double4 *>(a.vals + 4 * (int64_t)e);
+                        jacc.j[0].x += v.y * xv.x; jacc.j[0].y += v.y * xv.y;
+                        jacc.j[1].x += v.z * xv.x; jacc.j[1].y += v.z * xv.y;
+                        jacc.j[2].x += v.w * xv.x; jacc.j[2].y += v.w * xv.y;
+                    } else {
+                        const double jv = a.vals[2 * (int64_t)e + 1];
+                        jacc.j[0].x += jv * xv.x; jacc.j[0].y += jv * xv.y;
+                    }
+                }
             }
             group_reduce<G>(sb);
+            if (mode == 2) group_reduce_acc<G, PG>(jacc);
         }
-        group_reduce_acc<G, PG>(acc);
         if (L.g == 0) {
             const int64_t idx = row * S_ + s;
             double2 b;
-            if (restart) {
+            if (mode == 1) {
                 b = a.b[idx];
             } else {
-                Acc<PG> bacc = acc;
+                Acc<PG> bacc = mode == 0 ? acc : jacc;
                 bacc.a = sb;
                 b = imag_coef_apply<G, PG>(bacc, a.coefB, s);
                 a.b[idx] = b;
             }
-            const double2 ax = imag_coef_apply<G, PG>(acc, a.coefA, s);
-            const double2 r = make_double2(b.x - ax.x, b.y - ax.y);
-            a.r[idx] = r;
-            a.rh[idx] = r;
-            const double rr = r.x * r.x + r.y * r.y;
-            red[0] += rr;
-            red[2] += rr;
-            red[3] += b.x * b.x + b.y * b.y;
+            if (mode == 2) {
+                const double2 xi = x[idx], xp = a.xprev[idx];
+                a.xprev[idx] = make_double2(2.0 * xi.x - xp.x, 2.0 * xi.y - xp.y);
+            } else {
+                const double2 ax = imag_coef_apply<G, PG>(acc, a.coefA, s);
+                const double2 r = make_double2(b.x - ax.x, b.y - ax.y);
+                a.r[idx] = r;
+                a.rh[idx] = r;
+                const double rr = r.x * r.x + r.y * r.y;
+                red[0] += rr;
+                red[2] += rr;
+                red[3] += b.x * b.x + b.y * b.y;
+            }
         }
     }
-    block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partU);
+    if (mode != 2) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partU);
 }
 
 // Iteration head: totals of U give rho_i = <r^, r_i> and |r_i|^2; convergence, breakdown
@@ -1283,12 +1310,11 @@ void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *
         }                                                                                   \
     } while (0)
 
-void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, bool restart, const int *mask, hipStream_t s) {
+void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const int *mask, hipStream_t s) {
     DMRI_DISPATCH_G(g.G, a.per_gdir,
-                    (rhs_residual_kernel<GG, PP><<<g.grid, g.spmv_threads, 0, s>>>(a, g.S, g.rpb, restart ? 1 : 0,
-                                                                                   mask)));
+                    (rhs_residual_kernel<GG, PP><<<g.grid, g.spmv_threads, 0, s>>>(a, g.S, g.rpb, mode, mask)));
     check_launch();
-    if (a.totU) {
+    if (a.totU && mode != 2) {
         finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partU, g.grid, g.S, a.totU);
         check_launch();
     }

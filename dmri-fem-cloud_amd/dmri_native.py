@@ -44,7 +44,7 @@ class _Seq(ctypes.Structure):
 
 class _Opts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxit", ctypes.c_int32),
-                ("profile", ctypes.c_int32), ("lanes_per_row", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("profile", ctypes.c_int32), ("lanes_per_row", ctypes.c_int32), ("initial_guess", ctypes.c_int32)]
 
 
 class _Stats(ctypes.Structure):
@@ -55,7 +55,7 @@ class _Stats(ctypes.Structure):
 
 EXPORTED = ["dmri_create", "dmri_destroy", "dmri_last_error", "dmri_version", "dmri_mesh_set", "dmri_coeffs_set",
             "dmri_assemble", "dmri_get_info", "dmri_export_pattern", "dmri_export_values", "dmri_export_facets",
-            "dmri_solve", "dmri_solve_batched", "dmri_get_solution"]
+            "dmri_solve", "dmri_solve_batched", "dmri_tune", "dmri_set_tuning", "dmri_get_solution"]
 
 
 def _load():
@@ -79,6 +79,8 @@ def _load():
     L.dmri_solve.argtypes = [vp, P(_Seq), P(_Opts), dp, P(_Stats)]
     L.dmri_solve_batched.argtypes = [vp, ctypes.c_int32, dp, dp, P(_Seq), P(_Opts), dp, P(_Stats)]
     L.dmri_get_solution.argtypes = [vp, ctypes.c_int32, dp]
+    L.dmri_tune.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, i32p, dp]
+    L.dmri_set_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     for name in EXPORTED:
         if name not in ("dmri_last_error", "dmri_version"):
             getattr(L, name).restype = ctypes.c_int
@@ -179,7 +181,7 @@ class Context:
 
     # -- solve --------------------------------------------------------------------------
     @staticmethod
-    def _seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes):
+    def _seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes, guess=0):
         fA = np.ascontiguousarray(f_implicit, dtype=np.float64)
         fB = np.ascontiguousarray(f_explicit, dtype=np.float64)
         if fA.shape != fB.shape:
@@ -194,11 +196,13 @@ class Context:
         o = _Opts()
         o.rtol, o.atol, o.maxit, o.profile = float(rtol), float(atol), int(maxit), 1 if profile else 0
         o.lanes_per_row = int(lanes)
+        o.initial_guess = int(guess)
         return s, o, (fA, fB)
 
     def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-              profile=False, lanes=0):
-        s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes)
+              profile=False, lanes=0, guess=0):
+        s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes,
+                                    guess)
         sig = np.zeros(2)
         st = _Stats()
         _check(LIB.dmri_solve(self._h, ctypes.byref(s), ctypes.byref(o), _ptr(sig, ctypes.c_double),
@@ -206,19 +210,29 @@ class Context:
         return complex(sig[0], sig[1]), {k: getattr(st, k) for k, _ in _Stats._fields_}
 
     def solve_batched(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-                      profile=False, lanes=0):
+                      profile=False, lanes=0, guess=0):
         """Signals for every (q[s], gdir[s]) on the shared pattern; returns complex array (nsig,)."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(gdirs, dtype=np.float64).reshape(-1, 3),
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
-                                    lanes)
+                                    lanes, guess)
         sig = np.zeros(2 * qs.shape[0])
         st = _Stats()
         _check(LIB.dmri_solve_batched(self._h, qs.shape[0], _ptr(qs, ctypes.c_double), _ptr(g, ctypes.c_double),
                                       ctypes.byref(s), ctypes.byref(o), _ptr(sig, ctypes.c_double),
                                       ctypes.byref(st)))
         return sig.view(np.complex128), {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    def tune(self, nsig: int, per_gdir: bool = False):
+        """Time every lanes-per-row candidate; returns (best, {lanes: us per SpMV launch})."""
+        best = ctypes.c_int32()
+        us = np.zeros(6)
+        _check(LIB.dmri_tune(self._h, int(nsig), 1 if per_gdir else 0, ctypes.byref(best), _ptr(us, ctypes.c_double)))
+        return best.value, {l: t for l, t in zip((1, 2, 4, 8, 16, 32), us) if t >= 0}
+
+    def set_tuning(self, nsig: int, per_gdir: bool, lanes: int):
+        _check(LIB.dmri_set_tuning(self._h, int(nsig), 1 if per_gdir else 0, int(lanes)))
 
     def solution(self, sig: int = 0) -> np.ndarray:
         n = self.info()["ndof"]

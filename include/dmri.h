@@ -123,9 +123,9 @@ typedef struct {
     double atol;
     int32_t maxit;         /* per time step */
     int32_t profile;       /* 1: time the SpMV launches with HIP events (dmri_stats.spmv_ms) */
-    int32_t lanes_per_row; /* SpMV lanes per (row, signal): 0 = library choice, else 1..32, power of two,
-                              lanes * min(nsig, 64) <= 64 (see dmri_tune) */
-    int32_t reserved;
+    int32_t lanes_per_row; /* SpMV lanes per (row, signal): 0 = tuned value (dmri_tune / dmri_set_tuning)
+                              or the library default; else 1..32, power of two, lanes * min(nsig, 64) <= 64 */
+    int32_t initial_guess; /* Krylov start of step n: 0 = 2u^n - u^(n-1) (extrapolated), 1 = u^n */
 } dmri_solver_opts;
 
 typedef struct {
@@ -155,6 +155,14 @@ int dmri_solve(dmri_ctx *ctx, const dmri_seq *seq, const dmri_solver_opts *opts,
  * signal equals the one dmri_solve returns for that (q, gdir). */
 int dmri_solve_batched(dmri_ctx *ctx, int32_t nsig, const double *q, const double *gdir, const dmri_seq *seq,
                        const dmri_solver_opts *opts, double *sig_out, dmri_stats *stats);
+
+/* Autotuner: time the SpMV kernel for every lanes-per-row candidate {1,2,4,8,16,32} with
+ * lanes * nsig <= 64 on the assembled pattern (us_per_launch[6], -1 = not valid) and keep the
+ * fastest for later solves of this context with lanes_per_row = 0.  dmri_set_tuning installs a
+ * stored choice (e.g. from a tuning database) without timing.  No reference counterpart (the
+ * reference has no GPU kernels); results do not depend on the choice beyond rounding. */
+int dmri_tune(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t *best_lanes, double *us_per_launch);
+int dmri_set_tuning(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t lanes);
 
 /* Copy u^N of signal `sig` of the last launch group to host (2*ndof doubles, interleaved
  * re/im, DOF order; the DOF of (compartment c, vertex v) is dof_of[c*nv+v]). */

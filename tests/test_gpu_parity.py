@@ -186,3 +186,29 @@ def test_bad_arguments_raise():
     with pytest.raises(N.DmriError) as e:
         c2.set_mesh(mesh.xyz, bad)
     assert e.value.code == -5
+
+
+def test_autotuner_picks_valid_lanes_and_results_agree():
+    mesh, kw = MESHES["cyl2"]
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    for nsig, pg in ((1, False), (5, False), (16, True)):
+        best, times = ctx.tune(nsig, pg)
+        assert best in times and best * nsig <= 64
+        assert all(t > 0 for t in times.values())
+    fA, fB = _pgse_gcloud(10600.0, 43100.0, 400.0)
+    q = O.q_from_b_pgse(1000.0, 10600.0, 43100.0)
+    s_tuned, _ = ctx.solve(fA, fB, 400.0, q, (0, 1, 0), rtol=1e-13)
+    ctx.set_tuning(1, False, 4)
+    s_set, _ = ctx.solve(fA, fB, 400.0, q, (0, 1, 0), rtol=1e-13)
+    assert abs(s_tuned - s_set) <= 1e-11 * abs(s_set)
+
+
+def test_initial_guess_modes_agree():
+    mesh, kw = MESHES["cyl2"]
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    fA, fB = _pgse_gcloud(10600.0, 43100.0, 200.0)
+    q = O.q_from_b_pgse(2000.0, 10600.0, 43100.0)
+    s0, st0 = ctx.solve(fA, fB, 200.0, q, (0, 1, 0), rtol=1e-13, guess=0)
+    s1, st1 = ctx.solve(fA, fB, 200.0, q, (0, 1, 0), rtol=1e-13, guess=1)
+    print(f"  iterations: extrapolated {st0['iters_total']} previous {st1['iters_total']}", flush=True)
+    assert abs(s0 - s1) <= 1e-10 * abs(s1)
