@@ -65,6 +65,9 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, help="SpMV lanes per (row, signal); 0 = library choice")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-signal-at-a-time comparison")
     ap.add_argument("--guess", type=int, default=0, help="Krylov initial guess: 0 extrapolated, 1 previous step")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="counter runs only: truncate to the first K theta steps (result line not comparable)")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel reference")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -80,6 +83,8 @@ def main():
     bvals = [0.0, 500.0, 1000.0, 2000.0, 4000.0]
     gdir = (0.0, 1.0, 0.0)
     fA, fB = pgse_sequence(delta, Delta, k)
+    if args.profile_steps > 0:
+        fA, fB = fA[:args.profile_steps], fB[:args.profile_steps]
 
     t0 = time.time()
     mesh = dm.cylinder_mesh(8.0, 20.0, args.nv, seed=1001, interface_radii=(4.0,))
@@ -96,6 +101,7 @@ def main():
                                     guess=args.guess)
         return list(sig), [st]
 
+    copy_gbs = ctx.copy_bandwidth(args.copy_bytes, 10) if args.copy_bytes > 0 else None
     for _ in range(args.warmup):
         one_step()
     t1 = time.perf_counter()
@@ -175,7 +181,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "spmv_v/spmv_t (batched complex value-split CSR SpMV, 5 signals, fused reductions)",
-                     "avg_launch_us": avg_us, "bytes_per_launch": spmv_bytes, "launches": spmv_n},
+                     "avg_launch_us": avg_us, "bytes_per_launch": spmv_bytes, "launches": spmv_n,
+                     "copy_kernel_gbs": copy_gbs,
+                     "frac_of_copy": (achieved / copy_gbs) if copy_gbs else None},
         "cpu_baseline": cpu,
         "plain_one_signal_at_a_time": plain,
         "parity": {"b0_signal_abs_err": b0_err, "prefix_rel_err_vs_cpu": prefix_parity,

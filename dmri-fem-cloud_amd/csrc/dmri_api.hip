@@ -941,6 +941,32 @@ int dmri_set_tuning(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t lanes) 
     });
 }
 
+int dmri_copy_bandwidth(dmri_ctx *c, int64_t bytes, int32_t reps, double *gbps) {
+    return guarded([&] {
+        require(c != nullptr && gbps != nullptr, DMRI_ERR_ARG, "null argument");
+        require(bytes >= 16 && reps >= 1, DMRI_ERR_ARG, "need bytes >= 16 and reps >= 1");
+        set_device(c);
+        const int64_t n = bytes / 16;
+        DevArray<double2> src, dst;
+        src.alloc(n);
+        dst.alloc(n);
+        src.zero(c->stream);
+        hipEvent_t e0, e1;
+        DMRI_HIP(hipEventCreate(&e0));
+        DMRI_HIP(hipEventCreate(&e1));
+        launch_copy(src.p, dst.p, n, c->stream);
+        DMRI_HIP(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < reps; ++i) launch_copy(src.p, dst.p, n, c->stream);
+        DMRI_HIP(hipEventRecord(e1, c->stream));
+        DMRI_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        DMRI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *gbps = 2.0 * 16.0 * (double)n * reps / (ms * 1e-3) / 1e9;
+    });
+}
+
 int dmri_get_solution(dmri_ctx *c, int32_t sig, double *u) {
     return guarded([&] {
         require(c != nullptr && u != nullptr, DMRI_ERR_ARG, "null argument");

@@ -128,5 +128,6 @@ void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStrea
 void launch_signal(int64_t n, int S, const double *w, const double *sc, const double2 *x, double *part, double *out,
                    hipStream_t s);
 void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *x, double2 *u, hipStream_t s);
+void launch_copy(const double2 *src, double2 *dst, int64_t n, hipStream_t s);
 
 }  // namespace dmri

@@ -683,6 +683,14 @@ __global__ void init_x_kernel(int64_t n, int S, const double *sc, double2 *x) {
     if (i < n * S) x[i] = make_double2(1.0 / sc[i / S], 0.0);
 }
 
+// Plain streaming copy (16 B per lane per access); the bandwidth reference for the roofline
+// and the calibration kernel for the HBM byte counters.
+__global__ void __launch_bounds__(kThreads) copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                        int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void unscale_kernel(int64_t n, int S, int sig, const double *sc, const double2 *x, double2 *u) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -779,6 +787,24 @@ __device__ __forceinline__ void row_accumulate(const int *rowptr, const int *col
     if (e < e1) acc_entry<PG>(vals, e, X(col[e]), acc);
 }
 
+// Rows of this block: a contiguous chunk, with the chunks of consecutive virtual blocks on
+// the same XCD.  Workgroups are dealt to the 8 XCDs round-robin, so block b runs on XCD b % 8;
+// virtual block (b % 8) * (grid / 8) + b / 8 gives every XCD one contiguous eighth of the rows,
+// and the columns a row gathers are mostly in that XCD's L2.  The grid is a multiple of 8.
+constexpr int kXcds = 8;
+struct RowRange {
+    int64_t r0, r1;
+};
+__device__ __forceinline__ RowRange block_rows(int64_t n) {
+    const int g = gridDim.x;
+    const int per = g / kXcds;
+    const int vb = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+    const int64_t chunk = (n + g - 1) / g;
+    const int64_t r0 = (int64_t)vb * chunk;
+    const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
+    return RowRange{r0 < n ? r0 : n, r1};
+}
+
 struct Lane {
     int lrow, s, g;
 };
@@ -801,7 +827,8 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
     const bool sel = mode != 1 || mask == nullptr || mask[L.s];
     const int S_ = S;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t row = (int64_t)blockIdx.x * rpb + L.lrow; row < a.n; row += (int64_t)gridDim.x * rpb) {
+    const RowRange R = block_rows(a.n);
+    for (int64_t row = R.r0 + L.lrow; row < R.r1; row += rpb) {
         if (!sel) continue;
         const double2 *x = a.x;
         const int s = L.s;
@@ -959,7 +986,8 @@ __global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, in
     const bool act = !a.st[L.s].done;
     const int S_ = S;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t row = (int64_t)blockIdx.x * rpb + L.lrow; row < a.n; row += (int64_t)gridDim.x * rpb) {
+    const RowRange R = block_rows(a.n);
+    for (int64_t row = R.r0 + L.lrow; row < R.r1; row += rpb) {
         if (!act) continue;
         const double2 *p = a.p;
         const int s = L.s;
@@ -1017,7 +1045,8 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
     const double2 alpha = salpha[L.s];
     const int S_ = S;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t row = (int64_t)blockIdx.x * rpb + L.lrow; row < a.n; row += (int64_t)gridDim.x * rpb) {
+    const RowRange R = block_rows(a.n);
+    for (int64_t row = R.r0 + L.lrow; row < R.r1; row += rpb) {
         if (!act) continue;
         const double2 *r = a.r, *v = a.v;
         const int s = L.s;
@@ -1252,7 +1281,8 @@ SolveGeom make_geom(int64_t n, int G, int S) {
     g.vec_rpb = std::max(1, kThreads / S);
     g.vec_threads = g.vec_rpb * S;
     int64_t need = (n + g.rpb - 1) / g.rpb;
-    g.grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, kMaxGridSpmv));
+    need = (need + 7) / 8 * 8;  // XCD-contiguous row chunks need a multiple of 8 blocks
+    g.grid = (int)std::max<int64_t>(8, std::min<int64_t>(need, kMaxGridSpmv));
     int64_t vneed = (n + g.vec_rpb - 1) / g.vec_rpb;
     g.vgrid = (int)std::max<int64_t>(1, std::min<int64_t>(vneed, kMaxGrid));
     return g;
@@ -1285,6 +1315,11 @@ void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s
     init_x_kernel<<<grid_for(n * S, 256), 256, 0, s>>>(n, S, sc, x);
     check_launch();
 }
+void launch_copy(const double2 *src, double2 *dst, int64_t n, hipStream_t s) {
+    copy_kernel<<<4096, kThreads, 0, s>>>(src, dst, n);
+    check_launch();
+}
+
 void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *x, double2 *u, hipStream_t s) {
     unscale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, S, sig, sc, x, u);
     check_launch();

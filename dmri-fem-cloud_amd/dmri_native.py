@@ -55,7 +55,8 @@ class _Stats(ctypes.Structure):
 
 EXPORTED = ["dmri_create", "dmri_destroy", "dmri_last_error", "dmri_version", "dmri_mesh_set", "dmri_coeffs_set",
             "dmri_assemble", "dmri_get_info", "dmri_export_pattern", "dmri_export_values", "dmri_export_facets",
-            "dmri_solve", "dmri_solve_batched", "dmri_tune", "dmri_set_tuning", "dmri_get_solution"]
+            "dmri_solve", "dmri_solve_batched", "dmri_tune", "dmri_set_tuning", "dmri_copy_bandwidth",
+            "dmri_get_solution"]
 
 
 def _load():
@@ -81,6 +82,7 @@ def _load():
     L.dmri_get_solution.argtypes = [vp, ctypes.c_int32, dp]
     L.dmri_tune.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, i32p, dp]
     L.dmri_set_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.dmri_copy_bandwidth.argtypes = [vp, i64, ctypes.c_int32, dp]
     for name in EXPORTED:
         if name not in ("dmri_last_error", "dmri_version"):
             getattr(L, name).restype = ctypes.c_int
@@ -233,6 +235,12 @@ class Context:
 
     def set_tuning(self, nsig: int, per_gdir: bool, lanes: int):
         _check(LIB.dmri_set_tuning(self._h, int(nsig), 1 if per_gdir else 0, int(lanes)))
+
+    def copy_bandwidth(self, nbytes: int, reps: int = 10) -> float:
+        """GB/s of the library's plain copy kernel (read + write bytes)."""
+        g = ctypes.c_double()
+        _check(LIB.dmri_copy_bandwidth(self._h, int(nbytes), int(reps), ctypes.byref(g)))
+        return g.value
 
     def solution(self, sig: int = 0) -> np.ndarray:
         n = self.info()["ndof"]

@@ -164,6 +164,11 @@ int dmri_solve_batched(dmri_ctx *ctx, int32_t nsig, const double *q, const doubl
 int dmri_tune(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t *best_lanes, double *us_per_launch);
 int dmri_set_tuning(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t lanes);
 
+/* Bandwidth of a plain device copy kernel over `bytes` (read + write counted), averaged over
+ * `reps` launches: the measured reference beside the nominal HBM peak, and the calibration
+ * kernel for the HBM byte counters.  No reference counterpart. */
+int dmri_copy_bandwidth(dmri_ctx *ctx, int64_t bytes, int32_t reps, double *gbps);
+
 /* Copy u^N of signal `sig` of the last launch group to host (2*ndof doubles, interleaved
  * re/im, DOF order; the DOF of (compartment c, vertex v) is dof_of[c*nv+v]). */
 int dmri_get_solution(dmri_ctx *ctx, int32_t sig, double *u);
