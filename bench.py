@@ -62,7 +62,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60, help="theta steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rtol", type=float, default=1e-12)
-    ap.add_argument("--lanes", type=int, default=0, help="SpMV lanes per (row, signal); 0 = library choice")
+    ap.add_argument("--lanes", type=int, default=0, help="SpMV lanes per row; 0 = tuned")
+    ap.add_argument("--mapping", type=int, default=0, help="SpMV mapping 1 spread, 2 staged, 3 registers; 0 = tuned")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-signal-at-a-time comparison")
     ap.add_argument("--guess", type=int, default=0, help="Krylov initial guess: 0 extrapolated, 1 previous step")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -98,10 +100,14 @@ def main():
     def one_step(profile=False):
         ctx.assemble()
         sig, st = ctx.solve_batched(fA, fB, k, qs, gdir, rtol=args.rtol, profile=profile, lanes=args.lanes,
-                                    guess=args.guess)
+                                    guess=args.guess, mapping=args.mapping)
         return list(sig), [st]
 
     copy_gbs = ctx.copy_bandwidth(args.copy_bytes, 10) if args.copy_bytes > 0 else None
+    tuned = None
+    if not args.no_tune and args.lanes == 0 and args.mapping == 0:
+        ctx.assemble()
+        tuned = ctx.tune(len(bvals), False)[0]
     for _ in range(args.warmup):
         one_step()
     t1 = time.perf_counter()
@@ -173,6 +179,7 @@ def main():
                         "b in {0,500,1000,2000,4000}; step = device assembly + the 5 signals in one batched launch group",
             "nv": info["nv"], "nc": info["nc"], "ndof": info["ndof"], "nnz": info["nnz"],
             "n_iface": info["n_iface"], "theta_steps": len(fA), "b_values": bvals, "rtol": args.rtol,
+            "spmv_mapping_lanes": tuned,
         },
         "dof_steps_per_s": dof_steps / elapsed,
         "krylov_iters_per_theta_step": iters / (len(fA) * nsig),

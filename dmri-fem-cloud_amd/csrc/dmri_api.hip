@@ -351,13 +351,24 @@ void assemble_impl(dmri_ctx *c) {
     c->solved = false;
 }
 
-// Lanes per row for S signals when the caller does not choose: wide rows for one signal,
-// one lane per (row, signal) once the signals alone fill a quarter wave.
+// SpMV mapping and lanes per row when neither the caller nor the tuner chose: vector CSR for
+// one signal, signals in registers up to 8 signals, one lane per (row, signal) beyond.
+void default_mapping(int S, int &mapping, int &lanes) {
+    if (S == 1) {
+        mapping = kMapSL;
+        lanes = 8;
+    } else if (S <= kMaxSRSignals) {
+        mapping = kMapSR;
+        lanes = 4;
+    } else {
+        mapping = kMapSL;
+        lanes = 1;
+    }
+}
 int auto_lanes(int S) {
-    if (S >= 8) return 1;
-    if (S >= 4) return 2;
-    if (S >= 2) return 4;
-    return 8;
+    int m, l;
+    default_mapping(S, m, l);
+    return l;
 }
 
 void ensure_group_workspace(dmri_ctx *c, const SolveGeom &g, int nsteps, bool per_gdir) {
@@ -404,11 +415,19 @@ void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_
                  const dmri_solver_opts *opts, double *sig, dmri_stats &st) {
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
-    int lanes = opts->lanes_per_row > 0 ? opts->lanes_per_row : auto_lanes(S);
-    if (opts->lanes_per_row <= 0)
+    int mapping, lanes;
+    default_mapping(S, mapping, lanes);
+    if (opts->lanes_per_row <= 0 && opts->mapping <= 0) {
         for (const auto &t : c->tuned)
-            if (t.first == (long long)S * 2 + (per_gdir ? 1 : 0)) lanes = t.second;
-    const SolveGeom geo = make_geom(n, lanes, S);
+            if (t.first == (long long)S * 2 + (per_gdir ? 1 : 0)) {
+                mapping = t.second / 64;
+                lanes = t.second % 64;
+            }
+    } else {
+        if (opts->mapping > 0) mapping = opts->mapping;
+        if (opts->lanes_per_row > 0) lanes = opts->lanes_per_row;
+    }
+    const SolveGeom geo = make_geom(n, lanes, S, mapping);
     const int N = seq->nsteps;
     ensure_group_workspace(c, geo, N, per_gdir);
     SolverWork &w = c->work;
@@ -586,85 +605,98 @@ void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_
 
 // Time the SpMV kernel of every lanes-per-row candidate for S signals on the assembled pattern
 // and remember the fastest for later solves of this context.
-void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best, double *us) {
+// Time spmv_v + spmv_t for every (mapping, lanes per row) candidate for S signals on the
+// assembled pattern and remember the fastest for later solves of this context.
+// us[3*6]: [mapping-1][log2 lanes] microseconds per (spmv_v + spmv_t), -1 where not valid.
+void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_mapping, double *us) {
     require(c->assembled, DMRI_ERR_STATE, "dmri_tune: call dmri_assemble first");
     require(S >= 1 && S <= kMaxSignals, DMRI_ERR_ARG, "dmri_tune: nsig must lie in [1, 64]");
     set_device(c);
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
     const int cand[6] = {1, 2, 4, 8, 16, 32};
-    SolveGeom g0 = make_geom(n, 1, S);
+    SolveGeom g0 = make_geom(n, 1, S, kMapSL);
     ensure_group_workspace(c, g0, 1, per_gdir);
     SolverWork &w = c->work;
     auto &V = c->val;
-    const double gx = 1.0;
     launch_prep_scale(n, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, 1.0, 0.5,
                       w.sc.p, s);
     launch_prep_values(n, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
-                       V[DMRI_MAT_I].p, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, gx, 0.0, 0.0,
+                       V[DMRI_MAT_I].p, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, 1.0, 0.0, 0.0,
                        1.0, 0.5, w.vals.p, w.bre.p, s);
     launch_init_x(n, S, w.sc.p, w.p.p, s);
+    launch_init_x(n, S, w.sc.p, w.r.p, s);
+    launch_init_x(n, S, w.sc.p, w.v.p, s);
     std::vector<double> coef((size_t)S * 3, 1e-4);
     DMRI_HIP(hipMemcpyAsync(w.coef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
-    std::vector<KrylovState> zero(S);
-    std::memset(zero.data(), 0, sizeof(KrylovState) * S);
-    DMRI_HIP(hipMemcpyAsync(w.st.p, zero.data(), sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
-    DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
+    std::vector<KrylovState> st0(S);
+    std::memset(st0.data(), 0, sizeof(KrylovState) * S);
+    for (auto &x : st0) x.rho[0] = make_double2(1.0, 0.0);
     double best_us = 1e300;
-    *best = auto_lanes(S);
-    for (int i = 0; i < 6; ++i) {
-        us[i] = -1.0;
-        const int G = cand[i];
-        if (G * S > 64) continue;
-        const SolveGeom geo = make_geom(n, G, S);
-        const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
-        if (w.part.n < 3 * np) w.part.alloc(3 * np);
-        SolveArgs a{};
-        a.n = n;
-        a.rowptr = c->rowptr.p;
-        a.col = c->col.p;
-        a.vals = w.vals.p;
-        a.bre = w.bre.p;
-        a.per_gdir = per_gdir;
-        a.coefA = w.coef.p;
-        a.coefB = w.coef.p;
-        a.x = w.x.p;
-        a.r = w.r.p;
-        a.rh = w.p.p;
-        a.p = w.p.p;
-        a.v = w.v.p;
-        a.t = w.t.p;
-        a.b = w.b.p;
-        a.xprev = w.xprev.p;
-        a.partU = w.part.p;
-        a.partV = w.part.p + np;
-        a.partT = w.part.p + 2 * np;
-        a.st = w.st.p;
-        a.all_done = w.all_done.p;
-        a.rtol = 1e-12;
-        a.maxit = 1;
-        launch_spmv_v(geo, a, 0, s);  // warm-up
-        const int reps = 5;
-        DMRI_HIP(hipEventRecord(c->ev0[0], s));
-        for (int k = 0; k < reps; ++k) launch_spmv_v(geo, a, 0, s);
-        DMRI_HIP(hipEventRecord(c->ev1[0], s));
-        DMRI_HIP(hipEventSynchronize(c->ev1[0]));
-        float ms = 0;
-        DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[0], c->ev1[0]));
-        us[i] = 1e3 * ms / reps;
-        if (us[i] < best_us) {
-            best_us = us[i];
-            *best = G;
+    default_mapping(S, *best_mapping, *best_lanes);
+    for (int m = kMapSL; m <= kMapSR; ++m)
+        for (int i = 0; i < 6; ++i) {
+            const int G = cand[i];
+            double &slot = us[(m - 1) * 6 + i];
+            slot = -1.0;
+            if (m == kMapSR ? S > kMaxSRSignals : G * S > 64) continue;
+            const SolveGeom geo = make_geom(n, G, S, m);
+            const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
+            if (w.part.n < 3 * np) w.part.alloc(3 * np);
+            SolveArgs a{};
+            a.n = n;
+            a.rowptr = c->rowptr.p;
+            a.col = c->col.p;
+            a.vals = w.vals.p;
+            a.bre = w.bre.p;
+            a.per_gdir = per_gdir;
+            a.coefA = w.coef.p;
+            a.coefB = w.coef.p;
+            a.x = w.x.p;
+            a.r = w.r.p;
+            a.rh = w.p.p;
+            a.p = w.p.p;
+            a.v = w.v.p;
+            a.t = w.t.p;
+            a.b = w.b.p;
+            a.xprev = w.xprev.p;
+            a.partU = w.part.p;
+            a.partV = w.part.p + np;
+            a.partT = w.part.p + 2 * np;
+            a.st = w.st.p;
+            a.all_done = w.all_done.p;
+            a.rtol = 1e-12;
+            a.maxit = 1;
+            const int reps = 4;
+            float total = 0.0f;
+            for (int rep = -1; rep < reps; ++rep) {  // rep -1 warms up
+                DMRI_HIP(hipMemcpyAsync(w.st.p, st0.data(), sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
+                DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
+                DMRI_HIP(hipEventRecord(c->ev0[0], s));
+                launch_spmv_v(geo, a, 0, s);
+                launch_spmv_t(geo, a, 0, s);
+                DMRI_HIP(hipEventRecord(c->ev1[0], s));
+                DMRI_HIP(hipEventSynchronize(c->ev1[0]));
+                float ms = 0;
+                DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[0], c->ev1[0]));
+                if (rep >= 0) total += ms;
+            }
+            slot = 1e3 * total / reps;
+            if (slot < best_us) {
+                best_us = slot;
+                *best_lanes = G;
+                *best_mapping = m;
+            }
         }
-    }
     const long long key = (long long)S * 2 + (per_gdir ? 1 : 0);
+    const int val = *best_mapping * 64 + *best_lanes;
     bool found = false;
     for (auto &t : c->tuned)
         if (t.first == key) {
-            t.second = *best;
+            t.second = val;
             found = true;
         }
-    if (!found) c->tuned.emplace_back(key, *best);
+    if (!found) c->tuned.emplace_back(key, val);
 }
 
 void validate_seq(const dmri_seq *seq, const dmri_solver_opts *opts) {
@@ -916,28 +948,30 @@ int dmri_solve_batched(dmri_ctx *c, int32_t nsig, const double *q, const double 
     });
 }
 
-int dmri_tune(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t *best_lanes, double *us_per_launch) {
+int dmri_tune(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t *best_lanes, int32_t *best_mapping,
+              double *us) {
     return guarded([&] {
-        require(c != nullptr && best_lanes != nullptr && us_per_launch != nullptr, DMRI_ERR_ARG, "null argument");
-        int best = 0;
-        tune_impl(c, nsig, per_gdir != 0, &best, us_per_launch);
-        *best_lanes = best;
+        require(c != nullptr && best_lanes != nullptr && best_mapping != nullptr && us != nullptr, DMRI_ERR_ARG,
+                "null argument");
+        int bl = 0, bm = 0;
+        tune_impl(c, nsig, per_gdir != 0, &bl, &bm, us);
+        *best_lanes = bl;
+        *best_mapping = bm;
     });
 }
 
-int dmri_set_tuning(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t lanes) {
+int dmri_set_tuning(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t lanes, int32_t mapping) {
     return guarded([&] {
         require(c != nullptr, DMRI_ERR_ARG, "null context");
         require(nsig >= 1 && nsig <= kMaxSignals, DMRI_ERR_ARG, "nsig must lie in [1, 64]");
-        require(lanes >= 1 && lanes <= 32 && !(lanes & (lanes - 1)) && lanes * nsig <= 64, DMRI_ERR_ARG,
-                "lanes must be a power of two in [1, 32] with lanes * nsig <= 64");
+        (void)make_geom(c->ndof > 0 ? c->ndof : 1, lanes, nsig, mapping);  // validates the combination
         const long long key = (long long)nsig * 2 + (per_gdir ? 1 : 0);
         for (auto &t : c->tuned)
             if (t.first == key) {
-                t.second = lanes;
+                t.second = mapping * 64 + lanes;
                 return;
             }
-        c->tuned.emplace_back(key, lanes);
+        c->tuned.emplace_back(key, mapping * 64 + lanes);
     });
 }
 

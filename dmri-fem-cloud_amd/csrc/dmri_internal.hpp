@@ -39,18 +39,32 @@ constexpr int kMaxGrid = 512;      // blocks of the vector kernels (grid-stride)
 constexpr int kMaxGridSpmv = 2048; // blocks of the SpMV kernels: 8 blocks of 4 waves on each of 256 CUs
 constexpr int kMaxSignals = 64; // signals per batched launch
 constexpr int kNV = 4;          // doubles per (block, signal) partial slot
+constexpr int kStageNnzPerRow = 24; // LDS budget of a staged SpMV tile, entries per row
+constexpr int kStageBatch = 8;      // independent loads per thread per batch when staging a tile
 
-// Launch geometry of the solver kernels for a (G lanes per row, S signals) mapping.
+// SpMV mappings: signals spread over lanes (entries from global memory, or staged in LDS per
+// tile of rows), or all signals of a lane's entries kept in registers (S <= 8).
+constexpr int kMapSL = 1;
+constexpr int kMapStaged = 2;
+constexpr int kMapSR = 3;
+constexpr int kMaxSRSignals = 8;
+
+// Launch geometry of the solver kernels for a (mapping, G lanes per row, S signals) choice.
 struct SolveGeom {
-    int G = 8, S = 1;
-    int rpb = 32;           // rows per block in the SpMV kernels
-    int spmv_threads = 256; // rpb * G * S
-    int vec_rpb = 256;      // rows per block in the vector kernels
-    int vec_threads = 256;  // vec_rpb * S
-    int grid = 1;           // blocks of the SpMV-type kernels
-    int vgrid = 1;          // blocks of the vector kernels
+    int G = 8, S = 1, mapping = kMapSL;
+    int rpb = 32;            // rows per block pass in the spread-signal SpMV kernels
+    int spmv_threads = 256;  // rpb * G * S
+    int G_rhs = 8;           // lanes per row of the residual kernel (always spread-signal)
+    int rhs_rpb = 32, rhs_threads = 256;
+    int sr_rpb = 32, sr_threads = 256;  // signals-in-registers kernels: rows per pass, threads
+    int vec_rpb = 256;       // rows per block in the vector kernels
+    int vec_threads = 256;   // vec_rpb * S
+    int grid = 8;            // blocks of every SpMV-type kernel (a multiple of 8)
+    int vgrid = 1;           // blocks of the vector kernels
 };
-SolveGeom make_geom(int64_t n, int G, int S);
+SolveGeom make_geom(int64_t n, int G, int S, int mapping = kMapSL);
+int stage_rows(const SolveGeom &g, bool per_gdir);
+size_t stage_lds_bytes(int TR, bool per_gdir);
 
 // ---------------------------------------------------------------- setup launchers
 void exclusive_scan(const int *d_in, int *d_out, int64_t n, hipStream_t s);  // d_out has n+1 entries

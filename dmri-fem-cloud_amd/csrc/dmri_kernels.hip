@@ -787,6 +787,27 @@ __device__ __forceinline__ void row_accumulate(const int *rowptr, const int *col
     if (e < e1) acc_entry<PG>(vals, e, X(col[e]), acc);
 }
 
+// Same accumulation with the row's column indices and values already staged in LDS;
+// four entries per step keep four independent gathers in flight per lane.
+template <int G, bool PG, class Gather>
+__device__ __forceinline__ void lds_row_accumulate(const int *srp, const int *scol, const double *svals, int lr,
+                                                   int g, Gather X, Acc<PG> &acc) {
+    acc.a = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < (PG ? 3 : 1); ++d) acc.j[d] = make_double2(0.0, 0.0);
+    const int e1 = srp[lr + 1];
+    int e = srp[lr] + g;
+    for (; e + 3 * G < e1; e += 4 * G) {
+        const int c0 = scol[e], c1 = scol[e + G], c2 = scol[e + 2 * G], c3 = scol[e + 3 * G];
+        const double2 x0 = X(c0), x1 = X(c1), x2 = X(c2), x3 = X(c3);
+        acc_entry<PG>(svals, e, x0, acc);
+        acc_entry<PG>(svals, e + G, x1, acc);
+        acc_entry<PG>(svals, e + 2 * G, x2, acc);
+        acc_entry<PG>(svals, e + 3 * G, x3, acc);
+    }
+    for (; e < e1; e += G) acc_entry<PG>(svals, e, X(scol[e]), acc);
+}
+
 // Rows of this block: a contiguous chunk, with the chunks of consecutive virtual blocks on
 // the same XCD.  Workgroups are dealt to the 8 XCDs round-robin, so block b runs on XCD b % 8;
 // virtual block (b % 8) * (grid / 8) + b / 8 gives every XCD one contiguous eighth of the rows,
@@ -813,6 +834,82 @@ __device__ __forceinline__ Lane spmv_lane(int S) {
     const int gs = G * S;
     const int rem = threadIdx.x % gs;
     return Lane{(int)threadIdx.x / gs, rem / G, rem % G};
+}
+
+// LDS-staged SpMV over the block's rows.  A tile of TR consecutive rows is staged with
+// coalesced loads (row offsets, column indices, values), then every (row, signal) lane group
+// reads its entries from LDS and gathers x from global memory.  A tile whose entries exceed the
+// LDS budget (TR * kStageNnzPerRow) is processed straight from global memory instead.
+// emit(row, acc) receives the group-reduced accumulators (acted on by lane g == 0).
+template <int G, bool PG, class Gather, class Emit>
+__device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR, const Lane &L, bool act, Gather X,
+                                            Emit emit) {
+    constexpr int VW = PG ? 4 : 2;
+    if (TR <= 0) {  // direct: entries straight from global memory
+        const RowRange R = block_rows(a.n);
+        if (!act) return;
+        for (int64_t row = R.r0 + L.lrow; row < R.r1; row += rpb) {
+            Acc<PG> acc;
+            row_accumulate<G, PG>(a.rowptr, a.col, a.vals, row, L.g, X, acc);
+            group_reduce_acc<G, PG>(acc);
+            emit(row, acc);
+        }
+        return;
+    }
+    const int ECAP = TR * kStageNnzPerRow;
+    extern __shared__ double4 dmri_stage_lds[];
+    double *svals = reinterpret_cast<double *>(dmri_stage_lds);
+    int *scol = reinterpret_cast<int *>(svals + (size_t)ECAP * VW);
+    int *srp = scol + ECAP;
+    const RowRange R = block_rows(a.n);
+    for (int64_t t0 = R.r0; t0 < R.r1; t0 += TR) {
+        const int nr = (int)((R.r1 - t0) < TR ? (R.r1 - t0) : TR);
+        const int E0 = a.rowptr[t0];
+        const int ne = a.rowptr[t0 + nr] - E0;
+        const bool fits = ne <= ECAP;
+        __syncthreads();
+        if (fits) {
+            // Copy through registers in batches of kStageBatch independent loads per thread, so
+            // a thread waits on memory once per batch rather than once per element.
+            const double2 *gv = reinterpret_cast<const double2 *>(a.vals + (int64_t)E0 * VW);
+            double2 *lv = reinterpret_cast<double2 *>(svals);
+            const int nt = blockDim.x;
+            const int rp = threadIdx.x <= nr ? a.rowptr[t0 + threadIdx.x] : 0;
+            for (int base = 0; base < ne; base += kStageBatch * nt) {
+                int cv[kStageBatch];
+                double2 vv[kStageBatch][VW / 2];
+#pragma unroll
+                for (int u = 0; u < kStageBatch; ++u) {
+                    const int i = base + u * nt + threadIdx.x;
+                    const bool ok = i < ne;
+                    cv[u] = ok ? a.col[E0 + i] : 0;
+#pragma unroll
+                    for (int h = 0; h < VW / 2; ++h)
+                        vv[u][h] = ok ? gv[(int64_t)i * (VW / 2) + h] : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int u = 0; u < kStageBatch; ++u) {
+                    const int i = base + u * nt + threadIdx.x;
+                    if (i < ne) {
+                        scol[i] = cv[u];
+#pragma unroll
+                        for (int h = 0; h < VW / 2; ++h) lv[i * (VW / 2) + h] = vv[u][h];
+                    }
+                }
+            }
+            if ((int)threadIdx.x <= nr) srp[threadIdx.x] = rp - E0;
+            for (int i = threadIdx.x + nt; i <= nr; i += nt) srp[i] = a.rowptr[t0 + i] - E0;
+        }
+        __syncthreads();
+        if (!act) continue;
+        for (int lr = L.lrow; lr < nr; lr += rpb) {
+            Acc<PG> acc;
+            if (fits) lds_row_accumulate<G, PG>(srp, scol, svals, lr, L.g, X, acc);
+            else row_accumulate<G, PG>(a.rowptr, a.col, a.vals, t0 + lr, L.g, X, acc);
+            group_reduce_acc<G, PG>(acc);
+            emit(t0 + lr, acc);
+        }
+    }
 }
 
 // mode 0: b = (Bre + i cB.J) x, r = b - (Are + i cA.J) x, r^ = r          (start of a step)
@@ -980,37 +1077,32 @@ __global__ void __launch_bounds__(kThreads) bicg_p_kernel(SolveArgs a, int S, in
 
 // v = A p ; partials V: [<r^, v>.re, <r^, v>.im, 0, 0]
 template <int G, bool PG>
-__global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, int rpb, int it) {
+__global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
     if (*a.all_done) return;
     const Lane L = spmv_lane<G>(S);
     const bool act = !a.st[L.s].done;
     const int S_ = S;
+    const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
-    const RowRange R = block_rows(a.n);
-    for (int64_t row = R.r0 + L.lrow; row < R.r1; row += rpb) {
-        if (!act) continue;
-        const double2 *p = a.p;
-        const int s = L.s;
-        Acc<PG> acc;
-        row_accumulate<G, PG>(a.rowptr, a.col, a.vals, row, L.g, [&](int c) { return p[(int64_t)c * S_ + s]; },
-                              acc);
-        group_reduce_acc<G, PG>(acc);
-        if (L.g == 0) {
+    const double2 *p = a.p;
+    staged_spmv<G, PG>(
+        a, rpb, TR, L, act, [&](int c) { return p[(int64_t)c * S_ + s]; },
+        [&](int64_t row, const Acc<PG> &acc) {
+            if (L.g != 0) return;
             const int64_t idx = row * S_ + s;
             const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
             a.v[idx] = y;
             const double2 h = a.rh[idx];
             red[0] += h.x * y.x + h.y * y.y;
             red[1] += h.x * y.y - h.y * y.x;
-        }
-    }
+        });
     block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
 }
 
 // alpha = rho / <r^, v> ; t = A (r - alpha v) with s = r - alpha v formed on the fly;
 // partials T: [<t, s>.re, <t, s>.im, |t|^2, |s|^2]
 template <int G, bool PG>
-__global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, int rpb, int nparts, int it) {
+__global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, int rpb, int TR, int nparts, int it) {
     if (*a.all_done) return;
     __shared__ double tot[kMaxSignals][kNV];
     __shared__ double2 salpha[kMaxSignals];
@@ -1044,22 +1136,17 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
     const bool act = sact[L.s];
     const double2 alpha = salpha[L.s];
     const int S_ = S;
+    const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
-    const RowRange R = block_rows(a.n);
-    for (int64_t row = R.r0 + L.lrow; row < R.r1; row += rpb) {
-        if (!act) continue;
-        const double2 *r = a.r, *v = a.v;
-        const int s = L.s;
-        Acc<PG> acc;
-        row_accumulate<G, PG>(
-            a.rowptr, a.col, a.vals, row, L.g,
-            [&](int c) {
-                const int64_t j = (int64_t)c * S_ + s;
-                return c_sub(r[j], c_mul(alpha, v[j]));
-            },
-            acc);
-        group_reduce_acc<G, PG>(acc);
-        if (L.g == 0) {
+    const double2 *r = a.r, *v = a.v;
+    staged_spmv<G, PG>(
+        a, rpb, TR, L, act,
+        [&](int c) {
+            const int64_t j = (int64_t)c * S_ + s;
+            return c_sub(r[j], c_mul(alpha, v[j]));
+        },
+        [&](int64_t row, const Acc<PG> &acc) {
+            if (L.g != 0) return;
             const int64_t idx = row * S_ + s;
             const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
             a.t[idx] = y;
@@ -1068,9 +1155,147 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
             red[1] += y.x * sv.y - y.y * sv.x;
             red[2] += y.x * y.x + y.y * y.y;
             red[3] += sv.x * sv.x + sv.y * sv.y;
+        });
+    block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
+}
+
+// ---- signals in registers: each lane takes entries g, g+G, ... of a row for all S <= 8
+// signals, so column indices and values are loaded once per entry instead of once per signal.
+constexpr int kMaxSR = 8;
+
+template <int NVS>
+__device__ __forceinline__ void block_partials_sr(double (&red)[kMaxSR][NVS], int S, double *parts) {
+    for (int s = 0; s < S; ++s) {
+        double v[NVS];
+#pragma unroll
+        for (int k = 0; k < NVS; ++k) v[k] = red[s][k];
+        block_reduce<NVS>(v);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < kNV; ++k) parts[((size_t)blockIdx.x * S + s) * kNV + k] = k < NVS ? v[k] : 0.0;
         }
     }
-    block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
+}
+
+template <int G, bool PG, class Gather>
+__device__ __forceinline__ void sr_row(const SolveArgs &a, int64_t row, int g, int S, Gather X,
+                                       Acc<PG> (&acc)[kMaxSR]) {
+#pragma unroll
+    for (int s = 0; s < kMaxSR; ++s) {
+        acc[s].a = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int d = 0; d < (PG ? 3 : 1); ++d) acc[s].j[d] = make_double2(0.0, 0.0);
+    }
+    const int e1 = a.rowptr[row + 1];
+    for (int e = a.rowptr[row] + g; e < e1; e += G) {
+        const int c = a.col[e];
+#pragma unroll
+        for (int s = 0; s < kMaxSR; ++s)
+            if (s < S) acc_entry<PG>(a.vals, e, X(c, s), acc[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < kMaxSR; ++s)
+        if (s < S) group_reduce_acc<G, PG>(acc[s]);
+}
+
+template <int G, bool PG>
+__global__ void __launch_bounds__(kThreads) spmv_v_sr_kernel(SolveArgs a, int S, int rpb, int it) {
+    if (*a.all_done) return;
+    const int lrow = threadIdx.x / G, g = threadIdx.x % G;
+    bool act[kMaxSR];
+    double red[kMaxSR][2];
+#pragma unroll
+    for (int s = 0; s < kMaxSR; ++s) {
+        act[s] = s < S && !a.st[s].done;
+        red[s][0] = red[s][1] = 0.0;
+    }
+    const double2 *p = a.p;
+    const RowRange R = block_rows(a.n);
+    for (int64_t row = R.r0 + lrow; row < R.r1; row += rpb) {
+        Acc<PG> acc[kMaxSR];
+        sr_row<G, PG>(a, row, g, S, [&](int c, int s) { return p[(int64_t)c * S + s]; }, acc);
+        if (g != 0) continue;
+#pragma unroll
+        for (int s = 0; s < kMaxSR; ++s) {
+            if (!act[s]) continue;
+            const int64_t idx = row * S + s;
+            const double2 y = imag_coef_apply<G, PG>(acc[s], a.coefA, s);
+            a.v[idx] = y;
+            const double2 h = a.rh[idx];
+            red[s][0] += h.x * y.x + h.y * y.y;
+            red[s][1] += h.x * y.y - h.y * y.x;
+        }
+    }
+    block_partials_sr<2>(red, S, a.partV);
+}
+
+template <int G, bool PG>
+__global__ void __launch_bounds__(kThreads) spmv_t_sr_kernel(SolveArgs a, int S, int rpb, int nparts, int it) {
+    if (*a.all_done) return;
+    __shared__ double tot[kMaxSignals][kNV];
+    __shared__ double2 salpha[kMaxSignals];
+    __shared__ int sact[kMaxSignals];
+    signal_totals(a.partV, a.totV, nparts, S, tot);
+    if ((int)threadIdx.x < S) {
+        const int s = threadIdx.x;
+        KrylovState &st = a.st[s];
+        int actv = !st.done;
+        const int local = it - st.it0;
+        if (actv) {
+            const double2 rv = make_double2(tot[s][0], tot[s][1]);
+            if (rv.x == 0.0 && rv.y == 0.0) {
+                if (blockIdx.x == 0) {
+                    st.done = 1;
+                    st.breakdown = 1;
+                    st.conv = 0;
+                    st.iters = it;
+                }
+                actv = 0;
+            } else {
+                const double2 al = c_div(st.rho[local & 1], rv);
+                salpha[s] = al;
+                if (blockIdx.x == 0) st.alpha[local & 1] = al;
+            }
+        }
+        sact[s] = actv;
+    }
+    __syncthreads();
+    const int lrow = threadIdx.x / G, g = threadIdx.x % G;
+    bool act[kMaxSR];
+    double2 alpha[kMaxSR];
+    double red[kMaxSR][4];
+#pragma unroll
+    for (int s = 0; s < kMaxSR; ++s) {
+        act[s] = s < S && sact[s];
+        alpha[s] = s < S ? salpha[s] : make_double2(0.0, 0.0);
+        red[s][0] = red[s][1] = red[s][2] = red[s][3] = 0.0;
+    }
+    const double2 *r = a.r, *v = a.v;
+    const RowRange R = block_rows(a.n);
+    for (int64_t row = R.r0 + lrow; row < R.r1; row += rpb) {
+        Acc<PG> acc[kMaxSR];
+        sr_row<G, PG>(
+            a, row, g, S,
+            [&](int c, int s) {
+                const int64_t j = (int64_t)c * S + s;
+                return c_sub(r[j], c_mul(alpha[s], v[j]));
+            },
+            acc);
+        if (g != 0) continue;
+#pragma unroll
+        for (int s = 0; s < kMaxSR; ++s) {
+            if (!act[s]) continue;
+            const int64_t idx = row * S + s;
+            const double2 y = imag_coef_apply<G, PG>(acc[s], a.coefA, s);
+            a.t[idx] = y;
+            const double2 sv = c_sub(r[idx], c_mul(alpha[s], v[idx]));
+            red[s][0] += y.x * sv.x + y.y * sv.y;
+            red[s][1] += y.x * sv.y - y.y * sv.x;
+            red[s][2] += y.x * y.x + y.y * y.y;
+            red[s][3] += sv.x * sv.x + sv.y * sv.y;
+        }
+    }
+    block_partials_sr<4>(red, S, a.partT);
 }
 
 // omega = <t, s> / <t, t> (0 when |s| already meets the tolerance);
@@ -1269,23 +1494,45 @@ void launch_diag_and_weights(int64_t n, const int *rowptr, const int *col, const
     check_launch();
 }
 
-SolveGeom make_geom(int64_t n, int G, int S) {
+SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
     if (S < 1 || S > kMaxSignals) throw Error(-1, "signals per launch must lie in [1, 64]");
-    if (G < 1 || G > 32 || (G & (G - 1)) || G * S > 64)
-        throw Error(-1, "lanes per row must be a power of two with lanes*signals <= 64");
+    if (G < 1 || G > 32 || (G & (G - 1))) throw Error(-1, "lanes per row must be a power of two in [1, 32]");
+    if (mapping != kMapSL && mapping != kMapStaged && mapping != kMapSR) throw Error(-1, "unknown SpMV mapping");
+    if (mapping == kMapSR && S > kMaxSRSignals)
+        throw Error(-1, "signals-in-registers mapping needs at most 8 signals per launch group");
+    if (mapping != kMapSR && G * S > 64) throw Error(-1, "lanes per row times signals must not exceed 64");
     SolveGeom g;
     g.G = G;
     g.S = S;
-    g.rpb = std::max(1, kThreads / (G * S));
-    g.spmv_threads = g.rpb * G * S;
+    g.mapping = mapping;
+    // the residual kernel always spreads signals over lanes
+    g.G_rhs = mapping == kMapSR ? std::max(1, std::min(G, 64 / S)) : G;
+    while (g.G_rhs & (g.G_rhs - 1)) g.G_rhs &= g.G_rhs - 1;
+    g.rhs_rpb = std::max(1, kThreads / (g.G_rhs * S));
+    g.rhs_threads = g.rhs_rpb * g.G_rhs * S;
+    g.rpb = mapping == kMapSR ? g.rhs_rpb : std::max(1, kThreads / (G * S));
+    g.spmv_threads = mapping == kMapSR ? g.rhs_threads : g.rpb * G * S;
+    g.sr_rpb = std::max(1, kThreads / G);
+    g.sr_threads = g.sr_rpb * G;
     g.vec_rpb = std::max(1, kThreads / S);
     g.vec_threads = g.vec_rpb * S;
-    int64_t need = (n + g.rpb - 1) / g.rpb;
+    int64_t need = (n + g.rhs_rpb - 1) / g.rhs_rpb;
     need = (need + 7) / 8 * 8;  // XCD-contiguous row chunks need a multiple of 8 blocks
     g.grid = (int)std::max<int64_t>(8, std::min<int64_t>(need, kMaxGridSpmv));
     int64_t vneed = (n + g.vec_rpb - 1) / g.vec_rpb;
     g.vgrid = (int)std::max<int64_t>(1, std::min<int64_t>(vneed, kMaxGrid));
     return g;
+}
+
+// rows staged per tile: a multiple of the rows one pass of the block covers, about 64 rows
+// (32 with per-signal directions, whose entries carry four values)
+int stage_rows(const SolveGeom &g, bool per_gdir) {
+    const int target = per_gdir ? 32 : 64;
+    return g.rpb * std::max(1, target / g.rpb);
+}
+size_t stage_lds_bytes(int TR, bool per_gdir) {
+    const size_t ecap = (size_t)TR * kStageNnzPerRow;
+    return ecap * (8 * (per_gdir ? 4 : 2) + 4) + 4 * (size_t)(TR + 1) + 32;
 }
 
 void launch_sum(const double *x, int64_t n, double *part, double *out, hipStream_t s) {
@@ -1346,8 +1593,8 @@ void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *
     } while (0)
 
 void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const int *mask, hipStream_t s) {
-    DMRI_DISPATCH_G(g.G, a.per_gdir,
-                    (rhs_residual_kernel<GG, PP><<<g.grid, g.spmv_threads, 0, s>>>(a, g.S, g.rpb, mode, mask)));
+    DMRI_DISPATCH_G(g.G_rhs, a.per_gdir,
+                    (rhs_residual_kernel<GG, PP><<<g.grid, g.rhs_threads, 0, s>>>(a, g.S, g.rhs_rpb, mode, mask)));
     check_launch();
     if (a.totU && mode != 2) {
         finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partU, g.grid, g.S, a.totU);
@@ -1359,7 +1606,15 @@ void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
     check_launch();
 }
 void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s) {
-    DMRI_DISPATCH_G(g.G, a.per_gdir, (spmv_v_kernel<GG, PP><<<g.grid, g.spmv_threads, 0, s>>>(a, g.S, g.rpb, it)));
+    if (g.mapping == kMapSR) {
+        DMRI_DISPATCH_G(g.G, a.per_gdir,
+                        (spmv_v_sr_kernel<GG, PP><<<g.grid, g.sr_threads, 0, s>>>(a, g.S, g.sr_rpb, it)));
+    } else {
+        const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
+        const size_t lds = TR > 0 ? stage_lds_bytes(TR, a.per_gdir) : 0;
+        DMRI_DISPATCH_G(g.G, a.per_gdir,
+                        (spmv_v_kernel<GG, PP><<<g.grid, g.spmv_threads, lds, s>>>(a, g.S, g.rpb, TR, it)));
+    }
     check_launch();
     if (a.totV) {
         finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partV, g.grid, g.S, a.totV);
@@ -1367,8 +1622,15 @@ void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
     }
 }
 void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s) {
-    DMRI_DISPATCH_G(g.G, a.per_gdir,
-                    (spmv_t_kernel<GG, PP><<<g.grid, g.spmv_threads, 0, s>>>(a, g.S, g.rpb, g.grid, it)));
+    if (g.mapping == kMapSR) {
+        DMRI_DISPATCH_G(g.G, a.per_gdir,
+                        (spmv_t_sr_kernel<GG, PP><<<g.grid, g.sr_threads, 0, s>>>(a, g.S, g.sr_rpb, g.grid, it)));
+    } else {
+        const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
+        const size_t lds = TR > 0 ? stage_lds_bytes(TR, a.per_gdir) : 0;
+        DMRI_DISPATCH_G(g.G, a.per_gdir,
+                        (spmv_t_kernel<GG, PP><<<g.grid, g.spmv_threads, lds, s>>>(a, g.S, g.rpb, TR, g.grid, it)));
+    }
     check_launch();
     if (a.totT) {
         finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partT, g.grid, g.S, a.totT);

@@ -44,7 +44,8 @@ class _Seq(ctypes.Structure):
 
 class _Opts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxit", ctypes.c_int32),
-                ("profile", ctypes.c_int32), ("lanes_per_row", ctypes.c_int32), ("initial_guess", ctypes.c_int32)]
+                ("profile", ctypes.c_int32), ("lanes_per_row", ctypes.c_int32), ("initial_guess", ctypes.c_int32),
+                ("mapping", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class _Stats(ctypes.Structure):
@@ -80,8 +81,8 @@ def _load():
     L.dmri_solve.argtypes = [vp, P(_Seq), P(_Opts), dp, P(_Stats)]
     L.dmri_solve_batched.argtypes = [vp, ctypes.c_int32, dp, dp, P(_Seq), P(_Opts), dp, P(_Stats)]
     L.dmri_get_solution.argtypes = [vp, ctypes.c_int32, dp]
-    L.dmri_tune.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, i32p, dp]
-    L.dmri_set_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.dmri_tune.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, i32p, i32p, dp]
+    L.dmri_set_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     L.dmri_copy_bandwidth.argtypes = [vp, i64, ctypes.c_int32, dp]
     for name in EXPORTED:
         if name not in ("dmri_last_error", "dmri_version"):
@@ -183,7 +184,8 @@ class Context:
 
     # -- solve --------------------------------------------------------------------------
     @staticmethod
-    def _seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes, guess=0):
+    def _seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes, guess=0,
+                  mapping=0):
         fA = np.ascontiguousarray(f_implicit, dtype=np.float64)
         fB = np.ascontiguousarray(f_explicit, dtype=np.float64)
         if fA.shape != fB.shape:
@@ -199,12 +201,13 @@ class Context:
         o.rtol, o.atol, o.maxit, o.profile = float(rtol), float(atol), int(maxit), 1 if profile else 0
         o.lanes_per_row = int(lanes)
         o.initial_guess = int(guess)
+        o.mapping = int(mapping)
         return s, o, (fA, fB)
 
     def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-              profile=False, lanes=0, guess=0):
+              profile=False, lanes=0, guess=0, mapping=0):
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes,
-                                    guess)
+                                    guess, mapping)
         sig = np.zeros(2)
         st = _Stats()
         _check(LIB.dmri_solve(self._h, ctypes.byref(s), ctypes.byref(o), _ptr(sig, ctypes.c_double),
@@ -212,13 +215,13 @@ class Context:
         return complex(sig[0], sig[1]), {k: getattr(st, k) for k, _ in _Stats._fields_}
 
     def solve_batched(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-                      profile=False, lanes=0, guess=0):
+                      profile=False, lanes=0, guess=0, mapping=0):
         """Signals for every (q[s], gdir[s]) on the shared pattern; returns complex array (nsig,)."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(gdirs, dtype=np.float64).reshape(-1, 3),
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
-                                    lanes, guess)
+                                    lanes, guess, mapping)
         sig = np.zeros(2 * qs.shape[0])
         st = _Stats()
         _check(LIB.dmri_solve_batched(self._h, qs.shape[0], _ptr(qs, ctypes.c_double), _ptr(g, ctypes.c_double),
@@ -226,15 +229,20 @@ class Context:
                                       ctypes.byref(st)))
         return sig.view(np.complex128), {k: getattr(st, k) for k, _ in _Stats._fields_}
 
-    def tune(self, nsig: int, per_gdir: bool = False):
-        """Time every lanes-per-row candidate; returns (best, {lanes: us per SpMV launch})."""
-        best = ctypes.c_int32()
-        us = np.zeros(6)
-        _check(LIB.dmri_tune(self._h, int(nsig), 1 if per_gdir else 0, ctypes.byref(best), _ptr(us, ctypes.c_double)))
-        return best.value, {l: t for l, t in zip((1, 2, 4, 8, 16, 32), us) if t >= 0}
+    MAPPINGS = {1: "spread", 2: "spread-staged", 3: "registers"}
 
-    def set_tuning(self, nsig: int, per_gdir: bool, lanes: int):
-        _check(LIB.dmri_set_tuning(self._h, int(nsig), 1 if per_gdir else 0, int(lanes)))
+    def tune(self, nsig: int, per_gdir: bool = False):
+        """Time every (mapping, lanes) candidate; returns ((mapping, lanes), {(mapping, lanes): us})."""
+        bl, bm = ctypes.c_int32(), ctypes.c_int32()
+        us = np.zeros(18)
+        _check(LIB.dmri_tune(self._h, int(nsig), 1 if per_gdir else 0, ctypes.byref(bl), ctypes.byref(bm),
+                             _ptr(us, ctypes.c_double)))
+        times = {(m, l): us[(m - 1) * 6 + i] for m in (1, 2, 3) for i, l in enumerate((1, 2, 4, 8, 16, 32))
+                 if us[(m - 1) * 6 + i] >= 0}
+        return (bm.value, bl.value), times
+
+    def set_tuning(self, nsig: int, per_gdir: bool, lanes: int, mapping: int):
+        _check(LIB.dmri_set_tuning(self._h, int(nsig), 1 if per_gdir else 0, int(lanes), int(mapping)))
 
     def copy_bandwidth(self, nbytes: int, reps: int = 10) -> float:
         """GB/s of the library's plain copy kernel (read + write bytes)."""

@@ -123,9 +123,13 @@ typedef struct {
     double atol;
     int32_t maxit;         /* per time step */
     int32_t profile;       /* 1: time the SpMV launches with HIP events (dmri_stats.spmv_ms) */
-    int32_t lanes_per_row; /* SpMV lanes per (row, signal): 0 = tuned value (dmri_tune / dmri_set_tuning)
-                              or the library default; else 1..32, power of two, lanes * min(nsig, 64) <= 64 */
+    int32_t lanes_per_row; /* SpMV lanes per row: 0 = tuned value (dmri_tune / dmri_set_tuning) or the
+                              library default; else a power of two in 1..32 */
     int32_t initial_guess; /* Krylov start of step n: 0 = 2u^n - u^(n-1) (extrapolated), 1 = u^n */
+    int32_t mapping;       /* SpMV mapping: 0 = tuned/default, 1 = signals spread over lanes
+                              (lanes * signals <= 64), 2 = the same with rows staged in LDS,
+                              3 = signals kept in registers (at most 8 signals per launch group) */
+    int32_t reserved;
 } dmri_solver_opts;
 
 typedef struct {
@@ -156,13 +160,15 @@ int dmri_solve(dmri_ctx *ctx, const dmri_seq *seq, const dmri_solver_opts *opts,
 int dmri_solve_batched(dmri_ctx *ctx, int32_t nsig, const double *q, const double *gdir, const dmri_seq *seq,
                        const dmri_solver_opts *opts, double *sig_out, dmri_stats *stats);
 
-/* Autotuner: time the SpMV kernel for every lanes-per-row candidate {1,2,4,8,16,32} with
- * lanes * nsig <= 64 on the assembled pattern (us_per_launch[6], -1 = not valid) and keep the
- * fastest for later solves of this context with lanes_per_row = 0.  dmri_set_tuning installs a
- * stored choice (e.g. from a tuning database) without timing.  No reference counterpart (the
- * reference has no GPU kernels); results do not depend on the choice beyond rounding. */
-int dmri_tune(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t *best_lanes, double *us_per_launch);
-int dmri_set_tuning(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t lanes);
+/* Autotuner: time the two SpMV kernels of a Krylov iteration for every valid (mapping 1..3,
+ * lanes per row 1..32) on the assembled pattern (us[18] = [mapping-1][log2 lanes] microseconds,
+ * -1 = not valid) and keep the fastest for later solves of this context whose options leave
+ * mapping and lanes_per_row at 0.  dmri_set_tuning installs a stored choice (e.g. from a tuning
+ * database) without timing.  No reference counterpart (the reference has no GPU kernels);
+ * results do not depend on the choice beyond rounding. */
+int dmri_tune(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t *best_lanes, int32_t *best_mapping,
+              double *us);
+int dmri_set_tuning(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t lanes, int32_t mapping);
 
 /* Bandwidth of a plain device copy kernel over `bytes` (read + write counted), averaged over
  * `reps` launches: the measured reference beside the nominal HBM peak, and the calibration

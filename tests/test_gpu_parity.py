@@ -188,19 +188,34 @@ def test_bad_arguments_raise():
     assert e.value.code == -5
 
 
-def test_autotuner_picks_valid_lanes_and_results_agree():
+def test_autotuner_picks_valid_choice_and_results_agree():
     mesh, kw = MESHES["cyl2"]
     ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
     for nsig, pg in ((1, False), (5, False), (16, True)):
         best, times = ctx.tune(nsig, pg)
-        assert best in times and best * nsig <= 64
-        assert all(t > 0 for t in times.values())
+        assert best in times and all(t > 0 for t in times.values())
+        m, l = best
+        assert (m == 3 and nsig <= 8) or (m in (1, 2) and l * nsig <= 64)
     fA, fB = _pgse_gcloud(10600.0, 43100.0, 400.0)
     q = O.q_from_b_pgse(1000.0, 10600.0, 43100.0)
     s_tuned, _ = ctx.solve(fA, fB, 400.0, q, (0, 1, 0), rtol=1e-13)
-    ctx.set_tuning(1, False, 4)
+    ctx.set_tuning(1, False, 4, 2)
     s_set, _ = ctx.solve(fA, fB, 400.0, q, (0, 1, 0), rtol=1e-13)
     assert abs(s_tuned - s_set) <= 1e-11 * abs(s_set)
+
+
+@pytest.mark.parametrize("mapping,lanes", [(1, 1), (1, 4), (2, 1), (2, 4), (3, 1), (3, 2), (3, 8)])
+@pytest.mark.parametrize("pg", [False, True])
+def test_spmv_mappings_agree_batched(mapping, lanes, pg):
+    mesh, kw = MESHES["cyl2"]
+    delta, Delta, k = 10600.0, 43100.0, 400.0
+    fA, fB = _pgse_gcloud(delta, Delta, k)
+    qs = np.array([O.q_from_b_pgse(b, delta, Delta) for b in (0.0, 1000.0, 2500.0, 4000.0, 500.0)])
+    gd = np.random.default_rng(7).normal(size=(5, 3)) if pg else np.tile([1.0, 0.0, 0.0], (5, 1))
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    ref, _ = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13, lanes=2, mapping=1)
+    got, _ = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13, lanes=lanes, mapping=mapping)
+    assert np.all(np.abs(got - ref) <= 1e-11 * np.abs(ref))
 
 
 def test_initial_guess_modes_agree():
