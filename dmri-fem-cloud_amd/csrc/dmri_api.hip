@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -411,105 +412,121 @@ void ensure_group_workspace(dmri_ctx *c, const SolveGeom &g, int nsteps, bool pe
 }
 
 // One launch group of S <= 64 signals sharing the time profile.  q[s], g[3*s] (unit).
-void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_gdir, const dmri_seq *seq,
-                 const dmri_solver_opts *opts, double *sig, dmri_stats &st) {
-    hipStream_t s = c->stream;
-    const int64_t n = c->ndof;
-    int mapping, lanes;
-    default_mapping(S, mapping, lanes);
-    if (opts->lanes_per_row <= 0 && opts->mapping <= 0) {
-        for (const auto &t : c->tuned)
-            if (t.first == (long long)S * 2 + (per_gdir ? 1 : 0)) {
-                mapping = t.second / 64;
-                lanes = t.second % 64;
-            }
-    } else {
-        if (opts->mapping > 0) mapping = opts->mapping;
-        if (opts->lanes_per_row > 0) lanes = opts->lanes_per_row;
-    }
-    const SolveGeom geo = make_geom(n, lanes, S, mapping);
-    const int N = seq->nsteps;
-    ensure_group_workspace(c, geo, N, per_gdir);
-    SolverWork &w = c->work;
-    auto &V = c->val;
-    const double theta = seq->theta, dt = seq->dt;
-
-    launch_prep_scale(n, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, dt, theta,
-                      w.sc.p, s);
-    launch_prep_values(n, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
-                       V[DMRI_MAT_I].p, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, g[0], g[1],
-                       g[2], dt, theta, w.vals.p, w.bre.p, s);
-    // per-step imaginary coefficients: implicit side theta q f_A(n) (g), explicit side -(1-theta) q f_B(n) (g)
-    std::vector<double> coef((size_t)2 * std::max(N, 1) * S * 3, 0.0);
-    double *cA = coef.data(), *cB = coef.data() + (size_t)std::max(N, 1) * S * 3;
-    for (int k = 0; k < N; ++k)
-        for (int j = 0; j < S; ++j) {
-            const double a = theta * q[j] * seq->f_implicit[k];
-            const double b = -(1.0 - theta) * q[j] * seq->f_explicit[k];
-            for (int d = 0; d < 3; ++d) {
-                const double gd = per_gdir ? g[3 * j + d] : (d == 0 ? 1.0 : 0.0);
-                cA[((size_t)k * S + j) * 3 + d] = a * gd;
-                cB[((size_t)k * S + j) * 3 + d] = b * gd;
-            }
-        }
-    DMRI_HIP(hipMemcpyAsync(w.coef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
-    launch_init_x(n, S, w.sc.p, w.x.p, s);
-
-    // Finalize kernels reduce the partials unless one signal's few partials can be summed by
-    // every consumer block directly (mixed first/later iterations need per-signal counts).
-    const bool use_fin = S > 1 || std::max(geo.grid, geo.vgrid) > 512;
-    const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
+// Theta-scheme stepping engine for one launch group of S <= 64 signals sharing the time profile.
+struct Stepper {
+    dmri_ctx *c = nullptr;
+    hipStream_t s = nullptr;
+    int S = 1, N = 0;
+    int64_t n = 0;
+    bool per_gdir = false, extrap = true, prof = false;
+    SolveGeom geo;
     SolveArgs a{};
-    a.n = n;
-    a.rowptr = c->rowptr.p;
-    a.col = c->col.p;
-    a.vals = w.vals.p;
-    a.bre = w.bre.p;
-    a.per_gdir = per_gdir;
-    double2 *xc = w.x.p, *xp = w.xprev.p;
-    const bool extrap = opts->initial_guess == 0;
-    a.x = xc;
-    a.xprev = xp;
-    a.r = w.r.p;
-    a.rh = w.rh.p;
-    a.p = w.p.p;
-    a.v = w.v.p;
-    a.t = w.t.p;
-    a.b = w.b.p;
-    a.partU = w.part.p;
-    a.partV = w.part.p + np;
-    a.partT = w.part.p + 2 * np;
-    a.totU = use_fin ? w.tot.p : nullptr;
-    a.totV = use_fin ? w.tot.p + S * kNV : nullptr;
-    a.totT = use_fin ? w.tot.p + 2 * S * kNV : nullptr;
-    a.st = w.st.p;
-    a.all_done = w.all_done.p;
-    a.rtol = opts->rtol;
-    a.atol = opts->atol;
-    a.maxit = opts->maxit;
-    st.spmv_bytes = (double)c->nnz * (4.0 + 8.0 * (per_gdir ? 4 : 2)) + 4.0 * (double)(n + 1) +
-                    48.0 * (double)n * S;
-    const bool prof = opts->profile != 0;
+    const dmri_solver_opts *opts = nullptr;
+    dmri_stats *st = nullptr;
+    double2 *xc = nullptr, *xp = nullptr;
     int chunk = 8;
-    std::vector<KrylovState> zero_st(S);
-    for (int step = 0; step < N; ++step) {
+    std::vector<KrylovState> zero_st;
+
+    // q[s], g[3*s] (unit).  Scales the system, uploads the per-step coefficients, sets u0 = 1.
+    void setup(dmri_ctx *ctx, int S_, const double *q, const double *g, bool pg, const dmri_seq *seq,
+               const dmri_solver_opts *o, dmri_stats *stats) {
+        c = ctx;
+        s = c->stream;
+        S = S_;
+        n = c->ndof;
+        per_gdir = pg;
+        opts = o;
+        st = stats;
+        N = seq->nsteps;
+        int mapping, lanes;
+        default_mapping(S, mapping, lanes);
+        if (opts->lanes_per_row <= 0 && opts->mapping <= 0) {
+            for (const auto &t : c->tuned)
+                if (t.first == (long long)S * 2 + (per_gdir ? 1 : 0)) {
+                    mapping = t.second / 64;
+                    lanes = t.second % 64;
+                }
+        } else {
+            if (opts->mapping > 0) mapping = opts->mapping;
+            if (opts->lanes_per_row > 0) lanes = opts->lanes_per_row;
+        }
+        geo = make_geom(n, lanes, S, mapping);
+        ensure_group_workspace(c, geo, N, per_gdir);
+        SolverWork &w = c->work;
+        auto &V = c->val;
+        const double theta = seq->theta, dt = seq->dt;
+        launch_prep_scale(n, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, dt,
+                          theta, w.sc.p, s);
+        launch_prep_values(n, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
+                           V[DMRI_MAT_I].p, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, g[0],
+                           g[1], g[2], dt, theta, w.vals.p, w.bre.p, s);
+        // per-step imaginary coefficients: implicit side theta q f_A(n) (g), explicit -(1-theta) q f_B(n) (g)
+        std::vector<double> coef((size_t)2 * std::max(N, 1) * S * 3, 0.0);
+        double *cA = coef.data(), *cB = coef.data() + (size_t)std::max(N, 1) * S * 3;
+        for (int k = 0; k < N; ++k)
+            for (int j = 0; j < S; ++j) {
+                const double av = theta * q[j] * seq->f_implicit[k];
+                const double bv = -(1.0 - theta) * q[j] * seq->f_explicit[k];
+                for (int d = 0; d < 3; ++d) {
+                    const double gd = per_gdir ? g[3 * j + d] : (d == 0 ? 1.0 : 0.0);
+                    cA[((size_t)k * S + j) * 3 + d] = av * gd;
+                    cB[((size_t)k * S + j) * 3 + d] = bv * gd;
+                }
+            }
+        DMRI_HIP(hipMemcpyAsync(w.coef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
+        launch_init_x(n, S, w.sc.p, w.x.p, s);
+        // Finalize kernels reduce the partials unless one signal's few partials can be summed by
+        // every consumer block directly (mixed first/later iterations need per-signal counts).
+        const bool use_fin = S > 1 || std::max(geo.grid, geo.vgrid) > 512;
+        const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
+        a = SolveArgs{};
+        a.n = n;
+        a.rowptr = c->rowptr.p;
+        a.col = c->col.p;
+        a.vals = w.vals.p;
+        a.bre = w.bre.p;
+        a.per_gdir = per_gdir;
+        xc = w.x.p;
+        xp = w.xprev.p;
+        extrap = opts->initial_guess == 0;
+        a.x = xc;
+        a.xprev = xp;
+        a.r = w.r.p;
+        a.rh = w.rh.p;
+        a.p = w.p.p;
+        a.v = w.v.p;
+        a.t = w.t.p;
+        a.b = w.b.p;
+        a.partU = w.part.p;
+        a.partV = w.part.p + np;
+        a.partT = w.part.p + 2 * np;
+        a.totU = use_fin ? w.tot.p : nullptr;
+        a.totV = use_fin ? w.tot.p + S * kNV : nullptr;
+        a.totT = use_fin ? w.tot.p + 2 * S * kNV : nullptr;
+        a.st = w.st.p;
+        a.all_done = w.all_done.p;
+        a.rtol = opts->rtol;
+        a.atol = opts->atol;
+        a.maxit = opts->maxit;
+        st->spmv_bytes = (double)c->nnz * (4.0 + 8.0 * (per_gdir ? 4 : 2)) + 4.0 * (double)(n + 1) +
+                         48.0 * (double)n * S;
+        prof = opts->profile != 0;
+        chunk = 8;
+        zero_st.assign(S, KrylovState{});
+    }
+
+    void begin_step(int step) {
+        SolverWork &w = c->work;
         a.coefA = w.coef.p + (size_t)step * S * 3;
         a.coefB = w.coef.p + (size_t)std::max(N, 1) * S * 3 + (size_t)step * S * 3;
         std::memset(zero_st.data(), 0, sizeof(KrylovState) * S);
         DMRI_HIP(hipMemcpyAsync(w.st.p, zero_st.data(), sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
         DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
-        if (extrap && step > 0) {
-            // b from u^n; xp <- 2 u^n - u^(n-1); swap so that the guess is iterated on
-            launch_rhs_residual(geo, a, 2, nullptr, s);
-            std::swap(xc, xp);
-            a.x = xc;
-            a.xprev = xp;
-            launch_rhs_residual(geo, a, 1, nullptr, s);
-        } else {
-            launch_rhs_residual(geo, a, 0, nullptr, s);
-            if (extrap)
-                DMRI_HIP(hipMemcpyAsync(xp, xc, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
-        }
+    }
+
+    // BiCGStab iterations on a.x with the residual already formed; restarts on breakdown.
+    void krylov(int step) {
+        SolverWork &w = c->work;
         int it = 0, it_before = 0, restarts = 0;
         for (;;) {
             const int nl = std::min(chunk, kEventPool / 2);
@@ -540,8 +557,8 @@ void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_
                     for (int j = 0; j < 2; ++j) {
                         float ms = 0;
                         DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[2 * k + j], c->ev1[2 * k + j]));
-                        st.spmv_ms += ms;
-                        st.spmv_launches += 1;
+                        st->spmv_ms += ms;
+                        st->spmv_launches += 1;
                     }
             }
             it_before = it;
@@ -581,26 +598,197 @@ void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_
         int step_max = 0;
         for (int j = 0; j < S; ++j) {
             const KrylovState &h = w.h_st[j];
-            st.iters_total += h.iters;
+            st->iters_total += h.iters;
             step_max = std::max(step_max, h.iters);
             const double rel = h.bnorm2 > 0 ? std::sqrt(h.rnorm2 / h.bnorm2) : std::sqrt(h.rnorm2);
-            st.relres_max = std::max(st.relres_max, rel);
+            st->relres_max = std::max(st->relres_max, rel);
         }
-        st.iters_max = std::max(st.iters_max, step_max);
-        st.restarts += restarts;
+        st->iters_max = std::max(st->iters_max, step_max);
+        st->restarts += restarts;
         chunk = std::max(1, step_max);
     }
-    if (xc != w.x.p)
-        DMRI_HIP(hipMemcpyAsync(w.x.p, xc, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
-    launch_signal(n, S, c->w.p, w.sc.p, w.x.p, w.spart.p, w.sig.p, s);
-    std::vector<double> hs((size_t)S * kNV);
-    DMRI_HIP(hipMemcpyAsync(hs.data(), w.sig.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
-    DMRI_HIP(hipStreamSynchronize(s));
-    for (int j = 0; j < S; ++j) {
-        sig[2 * j] = hs[(size_t)j * kNV] / c->volume;
-        sig[2 * j + 1] = hs[(size_t)j * kNV + 1] / c->volume;
+
+    // One theta step of the forward problem on xc (xp holds the previous state).
+    void step(int k) {
+        begin_step(k);
+        a.x = xc;
+        a.xprev = xp;
+        if (extrap && k > 0) {
+            // b from u^n; xp <- 2 u^n - u^(n-1); swap so that the guess is iterated on
+            launch_rhs_residual(geo, a, 2, nullptr, s);
+            std::swap(xc, xp);
+            a.x = xc;
+            a.xprev = xp;
+            launch_rhs_residual(geo, a, 1, nullptr, s);
+        } else {
+            launch_rhs_residual(geo, a, 0, nullptr, s);
+            if (extrap)
+                DMRI_HIP(hipMemcpyAsync(xp, xc, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        }
+        krylov(k);
     }
-    c->last_S = S;
+
+    // Solve A_k y = rhs (complex-symmetric A_k) starting from y; one signal.
+    void solve_given(int k, const double2 *rhs, double2 *y) {
+        begin_step(k);
+        DMRI_HIP(hipMemcpyAsync(c->work.b.p, rhs, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        a.x = y;
+        launch_rhs_residual(geo, a, 1, nullptr, s);
+        krylov(k);
+        a.x = xc;
+    }
+
+    // out = B_k y
+    void apply_B(int k, double2 *y, double2 *out) {
+        begin_step(k);
+        a.x = y;
+        launch_rhs_residual(geo, a, 3, nullptr, s);
+        DMRI_HIP(hipMemcpyAsync(out, c->work.b.p, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        a.x = xc;
+    }
+
+    void signal(double *sig) {
+        SolverWork &w = c->work;
+        if (xc != w.x.p)
+            DMRI_HIP(hipMemcpyAsync(w.x.p, xc, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        launch_signal(n, S, c->w.p, w.sc.p, w.x.p, w.spart.p, w.sig.p, s);
+        std::vector<double> hs((size_t)S * kNV);
+        DMRI_HIP(hipMemcpyAsync(hs.data(), w.sig.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
+        DMRI_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < S; ++j) {
+            sig[2 * j] = hs[(size_t)j * kNV] / c->volume;
+            sig[2 * j + 1] = hs[(size_t)j * kNV + 1] / c->volume;
+        }
+        c->last_S = S;
+    }
+};
+
+void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_gdir, const dmri_seq *seq,
+                 const dmri_solver_opts *opts, double *sig, dmri_stats &st) {
+    Stepper run;
+    run.setup(c, S, q, g, per_gdir, seq, opts, &st);
+    for (int k = 0; k < run.N; ++k) run.step(k);
+    run.signal(sig);
+}
+
+// Discrete adjoint of the theta scheme for one signal (see the C ABI comment of dmri_adjoint).
+void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts, const dmri_adjoint_opts *aopts,
+                  double *sig, dmri_grads *grads, dmri_stats *stats) {
+    hipStream_t s = c->stream;
+    const double gn = std::sqrt(seq->gdir[0] * seq->gdir[0] + seq->gdir[1] * seq->gdir[1] + seq->gdir[2] * seq->gdir[2]);
+    require(gn > 1e-10 && std::isfinite(gn), DMRI_ERR_ARG, "|g|=0: gradient direction must be non-zero");
+    require(std::isfinite(seq->q), DMRI_ERR_ARG, "q is not finite");
+    const double g[3] = {seq->gdir[0] / gn, seq->gdir[1] / gn, seq->gdir[2] / gn};
+    dmri_stats st{};
+    Stepper run;
+    const auto t0 = std::chrono::steady_clock::now();
+    run.setup(c, 1, &seq->q, g, false, seq, opts, &st);
+    const int N = run.N;
+    const int64_t n = run.n;
+    int K = aopts && aopts->checkpoint_stride > 0 ? aopts->checkpoint_stride
+                                                  : (int)std::ceil(std::sqrt((double)std::max(N, 1)));
+    K = std::max(1, std::min(K, std::max(N, 1)));
+    const bool keep_all = K >= N;
+    const int nck = (N + K - 1) / K;
+    // checkpoints (x^k, x^(k-1)) at k = 0, K, 2K, ...; a segment holds x^k0 .. x^k1
+    DevArray<double2> ck, seg, mu, lam, lam2;
+    ck.alloc((size_t)std::max(nck, 1) * 2 * n);
+    seg.alloc((size_t)(std::min(K, std::max(N, 1)) + 1) * n);
+    mu.alloc(n);
+    lam.alloc(n);
+    lam2.alloc(n);
+    const size_t vb = sizeof(double2) * (size_t)n;
+    auto segp = [&](int j) { return seg.p + (size_t)j * n; };
+    // forward sweep
+    for (int k = 0; k < N; ++k) {
+        if (k % K == 0) {
+            const int j = k / K;
+            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(2 * j) * n, run.xc, vb, hipMemcpyDeviceToDevice, s));
+            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(2 * j + 1) * n, k > 0 ? run.xp : run.xc, vb,
+                                    hipMemcpyDeviceToDevice, s));
+        }
+        if (keep_all && k == 0) DMRI_HIP(hipMemcpyAsync(segp(0), run.xc, vb, hipMemcpyDeviceToDevice, s));
+        run.step(k);
+        if (keep_all) DMRI_HIP(hipMemcpyAsync(segp(k + 1), run.xc, vb, hipMemcpyDeviceToDevice, s));
+    }
+    run.signal(sig);
+    const auto t1 = std::chrono::steady_clock::now();
+    // backward sweep
+    SolverWork &w = c->work;
+    const double theta = seq->theta;
+    launch_adj_init_lambda(n, c->w.p, w.sc.p, 1.0 / c->volume, lam.p, s);
+    DMRI_HIP(hipMemsetAsync(mu.p, 0, vb, s));
+    DevArray<double> parts;
+    parts.alloc((size_t)kMaxGrid * 14);
+    std::vector<double> hp((size_t)kMaxGrid * 14);
+    double gK[2] = {0, 0}, gM[2] = {0, 0}, gI = 0, gq = 0;
+    std::vector<double> dfa(std::max(N, 1), 0.0), dfb(std::max(N, 1), 0.0);
+    int recomputed = 0;
+    for (int j = nck - 1; j >= 0; --j) {
+        const int k0 = j * K, k1 = std::min(N, k0 + K);
+        if (!keep_all) {
+            DMRI_HIP(hipMemcpyAsync(run.xc, ck.p + (size_t)(2 * j) * n, vb, hipMemcpyDeviceToDevice, s));
+            DMRI_HIP(hipMemcpyAsync(run.xp, ck.p + (size_t)(2 * j + 1) * n, vb, hipMemcpyDeviceToDevice, s));
+            DMRI_HIP(hipMemcpyAsync(segp(0), run.xc, vb, hipMemcpyDeviceToDevice, s));
+            for (int k = k0; k < k1; ++k) {
+                run.step(k);
+                DMRI_HIP(hipMemcpyAsync(segp(k - k0 + 1), run.xc, vb, hipMemcpyDeviceToDevice, s));
+                ++recomputed;
+            }
+        }
+        for (int k = k1 - 1; k >= k0; --k) {
+            const int base = keep_all ? 0 : k0;
+            run.solve_given(k, lam.p, mu.p);  // A_k mu = lambda^(k+1), warm start from mu_(k+1)
+            const int np = launch_adj_contract(n, c->rowptr.p, c->col.p, w.sc.p,
+                                               c->ncomp == 2 ? c->dof_comp.p : nullptr, c->val[DMRI_MAT_M].p,
+                                               c->val[DMRI_MAT_K].p, c->val[DMRI_MAT_I].p, w.vals.p, mu.p,
+                                               segp(k - base), segp(k - base + 1), theta, parts.p, s);
+            run.apply_B(k, mu.p, lam2.p);  // lambda^k = B_k mu_k
+            DMRI_HIP(hipMemcpyAsync(hp.data(), parts.p, sizeof(double) * 14 * np, hipMemcpyDeviceToHost, s));
+            DMRI_HIP(hipStreamSynchronize(s));
+            std::swap(lam.p, lam2.p);
+            double t[14] = {0};
+            for (int b = 0; b < np; ++b)
+                for (int q = 0; q < 14; ++q) t[q] += hp[(size_t)b * 14 + q];
+            // dS/dp = Re sum_k mu_k^T (dB_k x^k - dA_k x^(k+1))
+            gK[0] -= t[0];
+            gK[1] -= t[2];
+            gM[0] -= t[4];
+            gM[1] -= t[6];
+            gI -= t[8];
+            const double ia = theta * t[13];          // Im(mu^T Jg x^(k+1)) * theta
+            const double ib = (1.0 - theta) * t[11];  // Im(mu^T Jg x^k) * (1-theta)
+            dfa[k] = seq->q * ia;
+            dfb[k] = seq->q * ib;
+            gq += seq->f_implicit[k] * ia + seq->f_explicit[k] * ib;
+        }
+    }
+    const auto t2 = std::chrono::steady_clock::now();
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    for (int cc = 0; cc < 2; ++cc) {
+        if (cc >= c->ncomp) {
+            grads->dS_dD[cc] = 0.0;
+            grads->dS_dinvT2[cc] = 0.0;
+            continue;
+        }
+        // K = D_c Khat_c for a scalar D (a multiplier of the tensor field otherwise)
+        const double scale = c->has_Dcell ? 1.0 : c->D[cc];
+        grads->dS_dD[cc] = scale > 0 ? gK[cc] / scale : nan;
+        grads->dS_dinvT2[cc] = c->has_iT2cell ? nan : gM[cc];
+    }
+    const double kscale = c->has_kappa_f ? 1.0 : c->kappa;
+    grads->dS_dkappa = c->ncomp == 2 ? (kscale > 0 ? gI / kscale : nan) : 0.0;
+    grads->dS_dq = gq;
+    if (grads->dS_df_implicit)
+        for (int k = 0; k < N; ++k) grads->dS_df_implicit[k] = dfa[k];
+    if (grads->dS_df_explicit)
+        for (int k = 0; k < N; ++k) grads->dS_df_explicit[k] = dfb[k];
+    grads->ms_forward = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    grads->ms_adjoint = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    grads->checkpoints = keep_all ? 0 : nck;
+    grads->recomputed_steps = recomputed;
+    if (stats) *stats = st;
+    c->solved = false;  // work.x no longer holds u^N of a plain solve
 }
 
 // Time the SpMV kernel of every lanes-per-row candidate for S signals on the assembled pattern
@@ -998,6 +1186,17 @@ int dmri_copy_bandwidth(dmri_ctx *c, int64_t bytes, int32_t reps, double *gbps) 
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         *gbps = 2.0 * 16.0 * (double)n * reps / (ms * 1e-3) / 1e9;
+    });
+}
+
+int dmri_adjoint(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts, const dmri_adjoint_opts *aopts,
+                 double sig_out[2], dmri_grads *grads, dmri_stats *stats) {
+    return guarded([&] {
+        require(c != nullptr && grads != nullptr && sig_out != nullptr, DMRI_ERR_ARG, "null argument");
+        require(c->assembled, DMRI_ERR_STATE, "dmri_adjoint: call dmri_assemble first");
+        validate_seq(seq, opts);
+        set_device(c);
+        adjoint_impl(c, seq, opts, aopts, sig_out, grads, stats);
     });
 }
 

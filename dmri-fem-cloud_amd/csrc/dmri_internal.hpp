@@ -132,7 +132,7 @@ struct SolveArgs {
     int maxit;
 };
 // mode 0: b and r from x; 1: r from the stored b (masked signals, nullptr = all); 2: b only and
-// xprev <- 2 x - xprev
+// xprev <- 2 x - xprev; 3: b only
 void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const int *mask, hipStream_t s);
 void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
 void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
@@ -143,5 +143,11 @@ void launch_signal(int64_t n, int S, const double *w, const double *sc, const do
                    hipStream_t s);
 void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *x, double2 *u, hipStream_t s);
 void launch_copy(const double2 *src, double2 *dst, int64_t n, hipStream_t s);
+// adjoint contraction of one step; returns the number of block partials (14 doubles each)
+int launch_adj_contract(int64_t n, const int *rowptr, const int *col, const double *sc, const int *dof_comp,
+                        const double *mm, const double *kk, const double *gi, const double *vals, const double2 *mu,
+                        const double2 *xn, const double2 *xn1, double theta, double *parts, hipStream_t s);
+void launch_adj_init_lambda(int64_t n, const double *w, const double *sc, double inv_vol, double2 *lam,
+                            hipStream_t s);
 
 }  // namespace dmri

@@ -916,6 +916,7 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
 // mode 1: r = b - (Are + i cA.J) x with the stored b, for the masked signals (restart, or
 //         after an extrapolated initial guess; mask == nullptr selects every signal)
 // mode 2: b = (Bre + i cB.J) x only, and xprev <- 2 x - xprev (the extrapolated guess)
+// mode 3: b = (Bre + i cB.J) x only (the adjoint's lambda^n = B_n mu_n)
 // partials U (modes 0, 1): [|r|^2, 0, |r|^2, |b|^2]
 template <int G, bool PG>
 __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int S, int rpb, int mode,
@@ -947,7 +948,7 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
                 const double2 xv = x[(int64_t)a.col[e] * S_ + s];
                 sb.x += bv * xv.x;
                 sb.y += bv * xv.y;
-                if (mode == 2) {
+                if (mode >= 2) {
                     // J part of the explicit side (mode 0 reuses the implicit accumulation)
                     if (PG) {
                         const double4 v = *reinterpret_cast<const double4 *>(a.vals + 4 * (int64_t)e);
@@ -961,7 +962,7 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
                 }
             }
             group_reduce<G>(sb);
-            if (mode == 2) group_reduce_acc<G, PG>(jacc);
+            if (mode >= 2) group_reduce_acc<G, PG>(jacc);
         }
         if (L.g == 0) {
             const int64_t idx = row * S_ + s;
@@ -977,6 +978,7 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
             if (mode == 2) {
                 const double2 xi = x[idx], xp = a.xprev[idx];
                 a.xprev[idx] = make_double2(2.0 * xi.x - xp.x, 2.0 * xi.y - xp.y);
+            } else if (mode == 3) {
             } else {
                 const double2 ax = imag_coef_apply<G, PG>(acc, a.coefA, s);
                 const double2 r = make_double2(b.x - ax.x, b.y - ax.y);
@@ -989,7 +991,7 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
             }
         }
     }
-    if (mode != 2) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partU);
+    if (mode < 2) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partU);
 }
 
 // Iteration head: totals of U give rho_i = <r^, r_i> and |r_i|^2; convergence, breakdown
@@ -1345,6 +1347,66 @@ __global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int 
     block_signal_partials(red, valid, threadIdx.x, vrpb * S, S, a.partU);
 }
 
+// Adjoint contraction of one theta step (one signal).  With the scaled states x_n, x_n1, the
+// scaled adjoint mu and y = (1-theta) x_n + theta x_n1, every row i adds (complex products,
+// no conjugation; s_ij = sc_i sc_j):
+//   k[0..3]  mu_i sum_j s_ij K_ij y_j  into compartment 0 / 1 of row i
+//   k[4..7]  mu_i sum_j s_ij M_ij y_j  into compartment 0 / 1
+//   k[8,9]   mu_i sum_j s_ij I_ij y_j
+//   k[10,11] mu_i sum_j Jg_ij x_n,j    (Jg: the scaled g.J of vals[2e+1])
+//   k[12,13] mu_i sum_j Jg_ij x_n1,j
+// Block partials go to parts[block * 14 + k] and are summed on the host in block order.
+__global__ void __launch_bounds__(kThreads) adj_contract_kernel(int64_t n, const int *rowptr, const int *col,
+                                                                const double *sc, const int *dof_comp,
+                                                                const double *mm, const double *kk,
+                                                                const double *gi, const double *vals,
+                                                                const double2 *mu, const double2 *xn,
+                                                                const double2 *xn1, double theta, double *parts) {
+    double red[14];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) red[k] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double si = sc[i];
+        double2 aK = make_double2(0, 0), aM = make_double2(0, 0), aI = make_double2(0, 0);
+        double2 aJ0 = make_double2(0, 0), aJ1 = make_double2(0, 0);
+        for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+            const int j = col[e];
+            const double sij = si * sc[j];
+            const double2 x0 = xn[j], x1 = xn1[j];
+            const double2 y = make_double2((1.0 - theta) * x0.x + theta * x1.x, (1.0 - theta) * x0.y + theta * x1.y);
+            const double vk = sij * kk[e], vm = sij * mm[e], vi = sij * gi[e], vj = vals[2 * (int64_t)e + 1];
+            aK.x += vk * y.x; aK.y += vk * y.y;
+            aM.x += vm * y.x; aM.y += vm * y.y;
+            aI.x += vi * y.x; aI.y += vi * y.y;
+            aJ0.x += vj * x0.x; aJ0.y += vj * x0.y;
+            aJ1.x += vj * x1.x; aJ1.y += vj * x1.y;
+        }
+        const double2 m = mu[i];
+        const int c = dof_comp ? dof_comp[i] : 0;
+        const double2 pK = c_mul(m, aK), pM = c_mul(m, aM), pI = c_mul(m, aI);
+        const double2 pJ0 = c_mul(m, aJ0), pJ1 = c_mul(m, aJ1);
+        red[2 * c] += pK.x;
+        red[2 * c + 1] += pK.y;
+        red[4 + 2 * c] += pM.x;
+        red[4 + 2 * c + 1] += pM.y;
+        red[8] += pI.x;
+        red[9] += pI.y;
+        red[10] += pJ0.x;
+        red[11] += pJ0.y;
+        red[12] += pJ1.x;
+        red[13] += pJ1.y;
+    }
+    block_reduce<14>(red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 14; ++k) parts[(size_t)blockIdx.x * 14 + k] = red[k];
+}
+
+// lambda^N = c^ (scaled signal weights sc_i w_i / |Omega|)
+__global__ void adj_init_lambda_kernel(int64_t n, const double *w, const double *sc, double inv_vol, double2 *lam) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) lam[i] = make_double2(w[i] * sc[i] * inv_vol, 0.0);
+}
+
 // per-signal partials of sum_i w_i sc_i x[i*S+s]
 __global__ void __launch_bounds__(kThreads) signal_part_kernel(int64_t n, int S, int vrpb, const double *w,
                                                                const double *sc, const double2 *x, double *parts) {
@@ -1562,6 +1624,23 @@ void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s
     init_x_kernel<<<grid_for(n * S, 256), 256, 0, s>>>(n, S, sc, x);
     check_launch();
 }
+int launch_adj_contract(int64_t n, const int *rowptr, const int *col, const double *sc, const int *dof_comp,
+                        const double *mm, const double *kk, const double *gi, const double *vals, const double2 *mu,
+                        const double2 *xn, const double2 *xn1, double theta, double *parts, hipStream_t s) {
+    unsigned g = grid_for(n, kThreads);
+    if (g > (unsigned)kMaxGrid) g = kMaxGrid;
+    if (g == 0) g = 1;
+    adj_contract_kernel<<<g, kThreads, 0, s>>>(n, rowptr, col, sc, dof_comp, mm, kk, gi, vals, mu, xn, xn1, theta,
+                                                parts);
+    check_launch();
+    return (int)g;
+}
+void launch_adj_init_lambda(int64_t n, const double *w, const double *sc, double inv_vol, double2 *lam,
+                            hipStream_t s) {
+    adj_init_lambda_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, w, sc, inv_vol, lam);
+    check_launch();
+}
+
 void launch_copy(const double2 *src, double2 *dst, int64_t n, hipStream_t s) {
     copy_kernel<<<4096, kThreads, 0, s>>>(src, dst, n);
     check_launch();
@@ -1596,7 +1675,7 @@ void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const
     DMRI_DISPATCH_G(g.G_rhs, a.per_gdir,
                     (rhs_residual_kernel<GG, PP><<<g.grid, g.rhs_threads, 0, s>>>(a, g.S, g.rhs_rpb, mode, mask)));
     check_launch();
-    if (a.totU && mode != 2) {
+    if (a.totU && mode < 2) {
         finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partU, g.grid, g.S, a.totU);
         check_launch();
     }

@@ -54,10 +54,22 @@ class _Stats(ctypes.Structure):
                 ("spmv_launches", ctypes.c_int64), ("spmv_bytes", ctypes.c_double)]
 
 
+class _AdjOpts(ctypes.Structure):
+    _fields_ = [("checkpoint_stride", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class _Grads(ctypes.Structure):
+    _fields_ = [("dS_dD", ctypes.c_double * 2), ("dS_dinvT2", ctypes.c_double * 2), ("dS_dkappa", ctypes.c_double),
+                ("dS_dq", ctypes.c_double), ("dS_df_implicit", ctypes.POINTER(ctypes.c_double)),
+                ("dS_df_explicit", ctypes.POINTER(ctypes.c_double)), ("ms_forward", ctypes.c_double),
+                ("ms_adjoint", ctypes.c_double), ("checkpoints", ctypes.c_int32),
+                ("recomputed_steps", ctypes.c_int32)]
+
+
 EXPORTED = ["dmri_create", "dmri_destroy", "dmri_last_error", "dmri_version", "dmri_mesh_set", "dmri_coeffs_set",
             "dmri_assemble", "dmri_get_info", "dmri_export_pattern", "dmri_export_values", "dmri_export_facets",
             "dmri_solve", "dmri_solve_batched", "dmri_tune", "dmri_set_tuning", "dmri_copy_bandwidth",
-            "dmri_get_solution"]
+            "dmri_adjoint", "dmri_get_solution"]
 
 
 def _load():
@@ -84,6 +96,7 @@ def _load():
     L.dmri_tune.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, i32p, i32p, dp]
     L.dmri_set_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     L.dmri_copy_bandwidth.argtypes = [vp, i64, ctypes.c_int32, dp]
+    L.dmri_adjoint.argtypes = [vp, P(_Seq), P(_Opts), P(_AdjOpts), dp, P(_Grads), P(_Stats)]
     for name in EXPORTED:
         if name not in ("dmri_last_error", "dmri_version"):
             getattr(L, name).restype = ctypes.c_int
@@ -243,6 +256,27 @@ class Context:
 
     def set_tuning(self, nsig: int, per_gdir: bool, lanes: int, mapping: int):
         _check(LIB.dmri_set_tuning(self._h, int(nsig), 1 if per_gdir else 0, int(lanes), int(mapping)))
+
+    def adjoint(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
+                checkpoint_stride=0, lanes=0, guess=0, mapping=0):
+        """Signal and its adjoint gradients (see dmri_adjoint): returns (signal, grads dict, stats)."""
+        s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, False, lanes,
+                                    guess, mapping)
+        ao = _AdjOpts()
+        ao.checkpoint_stride = int(checkpoint_stride)
+        n = s.nsteps
+        dfa, dfb = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        g = _Grads()
+        g.dS_df_implicit = _ptr(dfa, ctypes.c_double)
+        g.dS_df_explicit = _ptr(dfb, ctypes.c_double)
+        sig = np.zeros(2)
+        st = _Stats()
+        _check(LIB.dmri_adjoint(self._h, ctypes.byref(s), ctypes.byref(o), ctypes.byref(ao), _ptr(sig, ctypes.c_double),
+                                ctypes.byref(g), ctypes.byref(st)))
+        grads = {"D": np.array(g.dS_dD[:]), "invT2": np.array(g.dS_dinvT2[:]), "kappa": g.dS_dkappa, "q": g.dS_dq,
+                 "f_implicit": dfa[:n].copy(), "f_explicit": dfb[:n].copy(), "ms_forward": g.ms_forward,
+                 "ms_adjoint": g.ms_adjoint, "checkpoints": g.checkpoints, "recomputed_steps": g.recomputed_steps}
+        return complex(sig[0], sig[1]), grads, {k: getattr(st, k) for k, _ in _Stats._fields_}
 
     def copy_bandwidth(self, nbytes: int, reps: int = 10) -> float:
         """GB/s of the library's plain copy kernel (read + write bytes)."""

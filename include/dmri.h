@@ -175,6 +175,32 @@ int dmri_set_tuning(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t lanes
  * kernel for the HBM byte counters.  No reference counterpart. */
 int dmri_copy_bandwidth(dmri_ctx *ctx, int64_t bytes, int32_t reps, double *gbps);
 
+/* Adjoint (reverse-mode) gradients of the normalised signal Re S of one dmri_solve.
+ * With A_n, B_n complex symmetric the discrete adjoint of the theta scheme is
+ *   lambda^N = w/|Omega|,  A_n mu_n = lambda^(n+1),  lambda^n = B_n mu_n,
+ *   dS/dp = Re sum_n mu_n^T (dB_n/dp u^n - dA_n/dp u^(n+1)),
+ * so the backward sweep runs the forward kernels.  States are kept at checkpoints every
+ * checkpoint_stride steps (0 = ceil(sqrt(N))) as (u^k, u^(k-1)) and recomputed per segment, which
+ * reproduces the forward states bit for bit; checkpoint_stride >= N keeps every state instead.
+ * No reference counterpart: the reference computes signals only. */
+typedef struct {
+    int32_t checkpoint_stride;
+    int32_t reserved;
+} dmri_adjoint_opts;
+
+typedef struct {
+    double dS_dD[2];        /* per compartment: d/dD_c for scalar D, d/d(multiplier) for a D_cell field */
+    double dS_dinvT2[2];    /* per compartment d/d(1/T2_c) (NaN with invT2_cell) */
+    double dS_dkappa;       /* d/dkappa (scalar kappa) or d/d(multiplier) of kappa_facet; 0 for one compartment */
+    double dS_dq;           /* at fixed time profile */
+    double *dS_df_implicit; /* optional out, nsteps: d/d f_implicit[n] at fixed q */
+    double *dS_df_explicit; /* optional out, nsteps: d/d f_explicit[n] at fixed q */
+    double ms_forward, ms_adjoint;
+    int32_t checkpoints, recomputed_steps;
+} dmri_grads;
+int dmri_adjoint(dmri_ctx *ctx, const dmri_seq *seq, const dmri_solver_opts *opts, const dmri_adjoint_opts *aopts,
+                 double sig_out[2], dmri_grads *grads, dmri_stats *stats);
+
 /* Copy u^N of signal `sig` of the last launch group to host (2*ndof doubles, interleaved
  * re/im, DOF order; the DOF of (compartment c, vertex v) is dof_of[c*nv+v]). */
 int dmri_get_solution(dmri_ctx *ctx, int32_t sig, double *u);

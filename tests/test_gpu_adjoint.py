@@ -1,0 +1,130 @@
+"""GPU adjoint gradients (dmri_adjoint) against central finite differences of the GPU forward
+solver (h = 1e-6 relative), the CPU oracle, and closed forms.  FD noise with rtol 1e-14 solves is
+about 1e-14 / 1e-6 relative, so gradients are compared at 2e-5 relative (of the larger of the two)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import dmri_mesh as dm  # noqa: E402
+import oracle as O  # noqa: E402
+
+MESH = dm.cylinder_mesh(8.0, 4.0, 2500, seed=1001, interface_radii=(4.0,))
+DELTA, BIGDELTA, K = 10600.0, 43100.0, 1000.0
+FA, FB = O.gcloud_sequence(O.pgse_piecewise(DELTA, BIGDELTA), DELTA + BIGDELTA, K)
+Q = O.q_from_b_pgse(2000.0, DELTA, BIGDELTA)
+G = (0.0, 1.0, 0.3)
+COEF = dict(D=(3e-3, 2e-3), invT2=(1 / 8e4, 1 / 6e4), kappa=1e-5)
+
+
+def _ctx(**over):
+    import dmri_native as N
+
+    kw = dict(COEF)
+    kw.update(over)
+    c = N.Context(0)
+    c.set_mesh(MESH.xyz, MESH.tets, MESH.phase)
+    c.set_coeffs(**kw)
+    c.assemble()
+    return c
+
+
+def _fwd(fA=FA, fB=FB, q=Q, **over):
+    return _ctx(**over).solve(fA, fB, K, q, G, rtol=1e-14)[0].real
+
+
+def _close(a, b, tol=2e-5):
+    return abs(a - b) <= tol * max(abs(a), abs(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def adj():
+    s, g, st = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14)
+    s_all, g_all, _ = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14, checkpoint_stride=10 ** 6)
+    return s, g, s_all, g_all
+
+
+def test_adjoint_signal_equals_forward(adj):
+    s, g, s_all, g_all = adj
+    assert abs(s.real - _fwd()) <= 1e-13
+    assert g["checkpoints"] > 1 and g["recomputed_steps"] == len(FA)
+    assert g_all["checkpoints"] == 0 and g_all["recomputed_steps"] == 0
+    # recomputation from checkpoints reproduces the stored-states gradients exactly
+    for k in ("D", "invT2"):
+        assert np.array_equal(g[k], g_all[k])
+    assert g["kappa"] == g_all["kappa"] and np.array_equal(g["f_implicit"], g_all["f_implicit"])
+
+
+@pytest.mark.parametrize("which", ["D0", "D1", "kappa", "T2_0", "T2_1"])
+def test_coefficient_gradients_match_central_fd(adj, which):
+    s, g, _, _ = adj
+    h = 1e-6
+    if which.startswith("D"):
+        c = int(which[1])
+        base = COEF["D"][c]
+        d = h * base
+        up = list(COEF["D"]); up[c] += d
+        dn = list(COEF["D"]); dn[c] -= d
+        fd = (_fwd(D=tuple(up)) - _fwd(D=tuple(dn))) / (2 * d)
+        got = g["D"][c]
+    elif which == "kappa":
+        d = h * COEF["kappa"]
+        fd = (_fwd(kappa=COEF["kappa"] + d) - _fwd(kappa=COEF["kappa"] - d)) / (2 * d)
+        got = g["kappa"]
+    else:
+        c = int(which[-1])
+        base = COEF["invT2"][c]
+        d = h * base
+        up = list(COEF["invT2"]); up[c] += d
+        dn = list(COEF["invT2"]); dn[c] -= d
+        fd = (_fwd(invT2=tuple(up)) - _fwd(invT2=tuple(dn))) / (2 * d)
+        got = g["invT2"][c]
+    print(f"  {which}: adjoint {got:.10e} fd {fd:.10e}", flush=True)
+    assert _close(got, fd)
+
+
+def test_waveform_and_q_gradients_match_central_fd(adj):
+    s, g, _, _ = adj
+    h = 1e-6
+    for n in (0, 5, 11, 50, len(FA) - 2):
+        for side in ("f_implicit", "f_explicit"):
+            fa, fb = FA.copy(), FB.copy()
+            arr = fa if side == "f_implicit" else fb
+            d = h * max(abs(arr[n]), 1.0)
+            arr[n] += d
+            up = _fwd(fA=fa, fB=fb)
+            arr[n] -= 2 * d
+            dn = _fwd(fA=fa, fB=fb)
+            fd = (up - dn) / (2 * d)
+            got = g[side][n]
+            assert abs(got - fd) <= 2e-5 * max(abs(got), abs(fd)) + 1e-12, (side, n, got, fd)
+    d = h * Q
+    fd = (_fwd(q=Q + d) - _fwd(q=Q - d)) / (2 * d)
+    assert _close(g["q"], fd)
+
+
+def test_adjoint_matches_oracle_fd_for_D():
+    """The same derivative from the CPU oracle (independent forward implementation)."""
+    s, g, _ = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14)
+    h = 1e-6 * COEF["D"][1]
+    f = []
+    for dd in (+h, -h):
+        S = O.System(MESH.xyz, MESH.tets, MESH.phase, D=(COEF["D"][0], COEF["D"][1] + dd), invT2=COEF["invT2"],
+                     kappa=COEF["kappa"])
+        f.append(S.solve(G, Q, K, FA, FB, rtol=1e-14)[0])
+    fd = (f[0] - f[1]) / (2 * h)
+    assert _close(g["D"][1], fd, 5e-5)
+
+
+def test_closed_forms_at_zero_gradient():
+    """b = 0: S = r^N exactly, so dS/dD = dS/dkappa = 0 and dS/d(1/T2) follows from r^N."""
+    T2 = 8e4
+    c = _ctx(invT2=1 / T2)
+    s, g, _ = c.adjoint(FA, FB, K, 0.0, G, rtol=1e-15)
+    th, N = 0.5, len(FA)
+    r = (1 / K - (1 - th) / T2) / (1 / K + th / T2)
+    drdi = (-(1 - th) * (1 / K + th / T2) - th * (1 / K - (1 - th) / T2)) / (1 / K + th / T2) ** 2
+    dS = N * r ** (N - 1) * drdi  # derivative w.r.t. a common 1/T2
+    assert abs(s.real - r ** N) <= 1e-12
+    assert abs(g["invT2"][0] + g["invT2"][1] - dS) <= 1e-8 * abs(dS)
+    assert abs(g["D"][0]) <= 1e-9 and abs(g["D"][1]) <= 1e-9 and abs(g["kappa"]) <= 1e-6
