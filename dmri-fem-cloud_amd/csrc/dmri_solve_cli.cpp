@@ -1,0 +1,292 @@
+// dmri-solve: command-line driver over the libdmri C ABI.
+//
+// Accepts the options of the reference drivers and prints their result lines:
+//   GCloud style (GCloudDmriSolver.py:61-133):  -f mesh.xml -M 0|1 -b B -q Q -p kappa -d delta -D Delta
+//                                               -k dt -K D0 -T2 T2 -gdir gx gy gz
+//   comri style (comri/two-comp/hpc-fenics-cpp/main.cpp:581-648): -m mesh.xml -N steps -b B -q G -d delta
+//                                               -D Delta -k dt -K D0 -p kappa -v gx gy gz -j nskip
+// Extra: -phase file (per-cell phase, one integer per line, or a DOLFIN mesh_function /
+// mesh_value_collection file whose values are partition markers taken modulo 2, as
+// CreatePhaseFunc does, DmriFemLib.py:758-765), -variant gcloud|comri|comri-until-T,
+// -theta, -rtol, -atol, -maxit, -device.
+//
+// Sequences: gcloud = loop t < T + k, t += k, explicit side lagged one step, PGSE profile
+// exclusive at T (DmriFemLib.py:897-910, GCloudDmriSolver.py:188-193); comri = same f on
+// both sides, loop t < T + dt, FT inclusive at Delta + delta (main.cpp:87-92,296-318);
+// comri-until-T = the multilayer loop t < T (comri/multilayer/hpc-fenics-cpp/main.cpp:301).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "dmri.h"
+
+namespace {
+
+struct Mesh {
+    std::vector<double> xyz;
+    std::vector<int32_t> tets;
+};
+
+bool attr(const std::string &line, const char *name, std::string &out) {
+    // a whitespace before the name, so that x=" does not match inside index="
+    std::string key = std::string(" ") + name + "=\"";
+    size_t p = line.find(key);
+    if (p == std::string::npos) return false;
+    p += key.size();
+    size_t e = line.find('"', p);
+    if (e == std::string::npos) return false;
+    out = line.substr(p, e - p);
+    return true;
+}
+
+bool read_dolfin_xml(const std::string &path, Mesh &m) {
+    std::ifstream in(path);
+    if (!in) return false;
+    std::string line, v;
+    while (std::getline(in, line)) {
+        if (line.find("<vertex ") != std::string::npos) {
+            std::string si, sx, sy, sz;
+            if (!attr(line, "index", si) || !attr(line, "x", sx) || !attr(line, "y", sy) || !attr(line, "z", sz))
+                return false;
+            size_t i = std::stoul(si);
+            if (m.xyz.size() < 3 * (i + 1)) m.xyz.resize(3 * (i + 1));
+            m.xyz[3 * i] = std::stod(sx);
+            m.xyz[3 * i + 1] = std::stod(sy);
+            m.xyz[3 * i + 2] = std::stod(sz);
+        } else if (line.find("<tetrahedron ") != std::string::npos) {
+            std::string si, a, b, c, d;
+            if (!attr(line, "index", si) || !attr(line, "v0", a) || !attr(line, "v1", b) || !attr(line, "v2", c) ||
+                !attr(line, "v3", d))
+                return false;
+            size_t i = std::stoul(si);
+            if (m.tets.size() < 4 * (i + 1)) m.tets.resize(4 * (i + 1));
+            m.tets[4 * i] = std::stoi(a);
+            m.tets[4 * i + 1] = std::stoi(b);
+            m.tets[4 * i + 2] = std::stoi(c);
+            m.tets[4 * i + 3] = std::stoi(d);
+        }
+    }
+    return !m.xyz.empty() && !m.tets.empty();
+}
+
+// per-cell phase: plain integers, or <value cell_index="i" ... value="m" /> entries (marker % 2)
+bool read_phase(const std::string &path, size_t nc, std::vector<int32_t> &phase) {
+    std::ifstream in(path);
+    if (!in) return false;
+    phase.assign(nc, 0);
+    std::string line;
+    size_t plain = 0;
+    bool xml = false;
+    while (std::getline(in, line)) {
+        if (line.find("<value ") != std::string::npos) {
+            xml = true;
+            std::string ci, val;
+            if (!attr(line, "cell_index", ci) || !attr(line, "value", val)) return false;
+            size_t i = std::stoul(ci);
+            if (i >= nc) return false;
+            phase[i] = std::stoi(val) % 2;
+        } else if (!xml) {
+            std::istringstream ss(line);
+            long v;
+            while (ss >> v) {
+                if (plain >= nc) return false;
+                phase[plain++] = (int32_t)(v % 2);
+            }
+        }
+    }
+    return xml || plain == nc;
+}
+
+void usage() {
+    std::fprintf(stderr,
+                 "usage: dmri-solve (-f|-m) mesh.xml [-M 0|1] [-phase file] [-b B | -q Q] [-d delta] [-D Delta]\n"
+                 "                  [-k dt | -N steps] [-K D0] [-p kappa] [-T2 T2] [-gdir|-v gx gy gz]\n"
+                 "                  [-variant gcloud|comri|comri-until-T] [-theta t] [-rtol r] [-atol a]\n"
+                 "                  [-maxit n] [-device d]\n");
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    // defaults of GCloudDmriSolver.py:52-55
+    std::string fmesh, fphase, variant = "gcloud";
+    int multiple = 0, nsteps = 100, maxit = 100000, device = 0;
+    double b = 1000, qin = -1, kappa = 1e-5, delta = 10600, Delta = 43100, dt = 200, D0 = 3e-3, T2 = 1e16;
+    double theta = 0.5, rtol = 1e-9, atol = 1e-10;
+    double g[3] = {0, 1, 0};
+    bool have_dt = false, have_b = false, have_q = false;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto next = [&](int k = 1) -> const char * {
+            if (i + k >= argc) {
+                usage();
+                std::exit(2);
+            }
+            return argv[i + k];
+        };
+        if (a == "-f" || a == "-m") { fmesh = next(); ++i; }
+        else if (a == "-M") { multiple = std::atoi(next()); ++i; }
+        else if (a == "-phase" || a == "-c") { fphase = next(); ++i; multiple = 1; }
+        else if (a == "-b") { b = std::atof(next()); have_b = true; ++i; }
+        else if (a == "-q") { qin = std::atof(next()); have_q = true; ++i; }
+        else if (a == "-p") { kappa = std::atof(next()); ++i; }
+        else if (a == "-d") { delta = std::atof(next()); ++i; }
+        else if (a == "-D") { Delta = std::atof(next()); ++i; }
+        else if (a == "-k") { dt = std::atof(next()); have_dt = true; ++i; }
+        else if (a == "-N") { nsteps = std::atoi(next()); ++i; }
+        else if (a == "-K") { D0 = std::atof(next()); ++i; }
+        else if (a == "-T2") { T2 = std::atof(next()); ++i; }
+        else if (a == "-gdir" || a == "-v") {
+            g[0] = std::atof(next(1)); g[1] = std::atof(next(2)); g[2] = std::atof(next(3)); i += 3;
+        }
+        else if (a == "-variant") { variant = next(); ++i; }
+        else if (a == "-theta") { theta = std::atof(next()); ++i; }
+        else if (a == "-rtol") { rtol = std::atof(next()); ++i; }
+        else if (a == "-atol") { atol = std::atof(next()); ++i; }
+        else if (a == "-maxit") { maxit = std::atoi(next()); ++i; }
+        else if (a == "-device") { device = std::atoi(next()); ++i; }
+        else if (a == "-j" || a == "-r" || a == "-s" || a == "-IsPeriodic" || a == "-isperiodic") { ++i; }
+        else if (a == "-h" || a == "--help") { usage(); return 0; }
+        else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return 2; }
+    }
+    if (fmesh.empty()) { usage(); return 2; }
+    const bool comri = variant != "gcloud";
+    if (variant != "gcloud" && variant != "comri" && variant != "comri-until-T") {
+        std::fprintf(stderr, "unknown variant %s\n", variant.c_str());
+        return 2;
+    }
+    const double gnorm_in = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    if (!(gnorm_in > 1e-10)) {
+        std::printf("|g|=0! Please check again the gradient directions!\n");  // DmriFemLib.py:823
+        return 1;
+    }
+    const double T = Delta + delta;
+    const double int4gb = delta * delta * (Delta - delta / 3.0);
+    if (!(int4gb > 0)) {
+        std::fprintf(stderr, "Delta - delta/3 must be positive\n");
+        return 1;
+    }
+    // q = gamma G from b (DmriFemLib.py:840-842, main.cpp:187-191) or from -q (G in T/m for comri)
+    double q;
+    const double g_ratio = 2.675e8;
+    if (have_q && !have_b) {
+        q = comri ? qin * g_ratio * 1e-12 : qin;
+        b = q * q * int4gb;
+    } else {
+        q = std::sqrt(b) / std::sqrt(int4gb);
+    }
+    if (comri && !have_dt) dt = T / nsteps;
+
+    // time profile per step
+    std::vector<double> fA, fB;
+    auto f_py = [&](double s) { return s < delta ? 1.0 : (s < Delta ? 0.0 : (s < T ? -1.0 : 0.0)); };
+    auto f_c = [&](double t) { return 1.0 * (t >= 0 && t < delta) - 1.0 * (t >= Delta && t <= Delta + delta); };
+    if (!comri) {
+        double t = 0, tp = 0;
+        while (t < T + dt) {
+            fA.push_back(f_py(t));
+            fB.push_back(f_py(tp));
+            tp = t;
+            t += dt;
+        }
+    } else {
+        double t = 0;
+        const double end = variant == "comri-until-T" ? T : T + dt;
+        while (t < end) {
+            fA.push_back(f_c(t));
+            t += dt;
+        }
+        fB = fA;
+    }
+
+    Mesh m;
+    if (!read_dolfin_xml(fmesh, m)) {
+        std::fprintf(stderr, "cannot read DOLFIN XML tetrahedral mesh %s\n", fmesh.c_str());
+        return 1;
+    }
+    const int64_t nv = (int64_t)m.xyz.size() / 3, nc = (int64_t)m.tets.size() / 4;
+    std::vector<int32_t> phase;
+    if (multiple) {
+        if (fphase.empty() || !read_phase(fphase, (size_t)nc, phase)) {
+            std::fprintf(stderr, "two compartments need -phase with one value per cell\n");
+            return 1;
+        }
+    }
+    dmri_ctx *ctx = nullptr;
+    auto check = [&](int rc, const char *what) {
+        if (rc != DMRI_OK) {
+            std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, dmri_last_error());
+            if (ctx) dmri_destroy(ctx);
+            std::exit(1);
+        }
+    };
+    check(dmri_create(&ctx, device), "dmri_create");
+    check(dmri_mesh_set(ctx, nv, m.xyz.data(), nc, m.tets.data(), multiple ? phase.data() : nullptr),
+          "dmri_mesh_set");
+    dmri_coeffs co{};
+    co.D[0] = co.D[1] = D0;
+    co.invT2[0] = co.invT2[1] = 1.0 / T2;
+    co.kappa = kappa;
+    check(dmri_coeffs_set(ctx, &co), "dmri_coeffs_set");
+    check(dmri_assemble(ctx), "dmri_assemble");
+    dmri_seq seq{};
+    seq.nsteps = (int32_t)fA.size();
+    seq.f_implicit = fA.data();
+    seq.f_explicit = fB.data();
+    seq.dt = dt;
+    seq.theta = theta;
+    seq.q = q;
+    seq.gdir[0] = g[0];
+    seq.gdir[1] = g[1];
+    seq.gdir[2] = g[2];
+    dmri_solver_opts op{};
+    op.rtol = rtol;
+    op.atol = atol;
+    op.maxit = maxit;
+    double sig[2];
+    dmri_stats st{};
+    check(dmri_solve(ctx, &seq, &op, sig, &st), "dmri_solve");
+    dmri_info info{};
+    check(dmri_get_info(ctx, &info), "dmri_get_info");
+    const double whole = info.volume, elapsed = st.solve_ms * 1e-3;
+    if (!comri) {
+        // PostProcessing line (DmriFemLib.py:933,972) + log.txt append (982-988)
+        char line[1024];
+        const double gval = q / g_ratio * 1e12;
+        if (multiple)
+            std::snprintf(line, sizeof(line),
+                          "b: %.3f, g: %.3f, q: %.3e, Signal: %.3e, Normalized signal: %.6e, kappa: %.3e, dt: %.3f, "
+                          "hmin: %.3e, hmax: %.3e, whole_vol: %.3f, vol_of_interest: %.3f, elasped time %.3f (s)\n",
+                          b, gval, q, sig[0] * whole, sig[0], kappa, dt, info.hmin, info.hmax, whole, whole, elapsed);
+        else
+            std::snprintf(line, sizeof(line),
+                          "b: %.3f, g: %.3f, q: %.3e, Signal: %.3e, Normalized signal: %.6e, dt: %.3f, hmin: %.3e, "
+                          "hmax: %.3e, whole_vol: %.3f, vol_of_interest: %.3f, elasped time %.3f (s)\n",
+                          b, gval, q, sig[0] * whole, sig[0], dt, info.hmin, info.hmax, whole, whole, elapsed);
+        std::fputs(line, stdout);
+        if (FILE *lf = std::fopen("log.txt", "a")) {
+            std::fputs(line, lf);
+            std::fclose(lf);
+        }
+    } else {
+        // comri message (one-comp main.cpp:333; two-comp main.cpp:900): gnorm is q, "q" is G in T/m
+        const double G = q / g_ratio * 1e12, gn = gnorm_in;
+        if (multiple)
+            std::printf("b: %f, gnorm: %e, q: %e, perm: %e, gdir: (%f, %f, %f), s: %f\n", b, q, G, kappa, g[0] / gn,
+                        g[1] / gn, g[2] / gn, sig[0]);
+        else
+            std::printf("b: %f, gnorm: %f, q: %f, gdir: (%f, %f, %f), s: %f\n", b, q, G, g[0] / gn, g[1] / gn,
+                        g[2] / gn, sig[0]);
+        std::printf("Runtime = %f\n", elapsed);
+    }
+    std::fprintf(stderr, "[dmri-solve] nv=%lld nc=%lld ndof=%lld nnz=%lld steps=%d krylov=%lld signal=%.12e\n",
+                 (long long)nv, (long long)nc, (long long)info.ndof, (long long)info.nnz, seq.nsteps,
+                 (long long)st.iters_total, sig[0]);
+    dmri_destroy(ctx);
+    return 0;
+}

@@ -128,3 +128,25 @@ def test_closed_forms_at_zero_gradient():
     assert abs(s.real - r ** N) <= 1e-12
     assert abs(g["invT2"][0] + g["invT2"][1] - dS) <= 1e-8 * abs(dS)
     assert abs(g["D"][0]) <= 1e-9 and abs(g["D"][1]) <= 1e-9 and abs(g["kappa"]) <= 1e-6
+
+
+def test_torch_autograd_function_applies_adjoint_gradients():
+    import torch
+
+    import dmri_native as N
+    import dmri_torch
+
+    c = N.Context(0)
+    c.set_mesh(MESH.xyz, MESH.tets, MESH.phase)
+    D = torch.tensor(COEF["D"], dtype=torch.float64, requires_grad=True)
+    T = torch.tensor(COEF["invT2"], dtype=torch.float64, requires_grad=True)
+    kap = torch.tensor(COEF["kappa"], dtype=torch.float64, requires_grad=True)
+    fa = torch.tensor(FA, requires_grad=True)
+    S = dmri_torch.bloch_torrey_signal(c, D, T, kap, Q, fa, FB, K, G, rtol=1e-14)
+    (3.0 * S).backward()
+    s_ref, g, _ = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14)
+    assert abs(S.item() - s_ref.real) <= 1e-14
+    assert torch.allclose(D.grad, torch.as_tensor(3.0 * g["D"]), rtol=1e-12, atol=0)
+    assert torch.allclose(T.grad, torch.as_tensor(3.0 * g["invT2"]), rtol=1e-12, atol=0)
+    assert abs(kap.grad.item() - 3.0 * g["kappa"]) <= 1e-12 * abs(g["kappa"])
+    assert torch.allclose(fa.grad, torch.as_tensor(3.0 * g["f_implicit"]), rtol=1e-12, atol=1e-300)

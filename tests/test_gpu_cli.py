@@ -1,0 +1,68 @@
+"""dmri-solve (drop-in CLI over the C ABI) on the GPU against the oracle and the reference line formats."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import dmri_mesh as dm  # noqa: E402
+import oracle as O  # noqa: E402
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "dmri-fem-cloud_amd", "lib", "dmri-solve")
+
+
+def _run(args, cwd):
+    r = subprocess.run([CLI] + [str(a) for a in args], capture_output=True, text=True, cwd=cwd, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_gcloud_style_one_compartment(tmp_path):
+    m = dm.box_mesh((-2.5,) * 3, (2.5,) * 3, 6, 6, 6)
+    dm.write_dolfin_xml(m, str(tmp_path / "box.xml"))
+    out = _run(["-f", "box.xml", "-M", 0, "-K", 2e-3, "-b", 1000, "-d", 1000, "-D", 10000, "-k", 50,
+                "-gdir", 0, 0, 1, "-rtol", 1e-13, "-atol", 0], tmp_path)
+    pat = (r"b: 1000\.000, g: [0-9.]+, q: [0-9.e+-]+, Signal: [0-9.e+-]+, Normalized signal: ([0-9.e+-]+), "
+           r"dt: 50\.000, hmin: [0-9.e+-]+, hmax: [0-9.e+-]+, whole_vol: 125\.000, vol_of_interest: 125\.000, "
+           r"elasped time [0-9.]+ \(s\)")
+    mm = re.search(pat, out)
+    assert mm, out
+    S = O.System(m.xyz, m.tets, None, D=2e-3)
+    fA, fB = O.gcloud_sequence(O.pgse_piecewise(1000.0, 10000.0), 11000.0, 50.0)
+    ref, _, _ = S.solve((0, 0, 1), O.q_from_b_pgse(1000.0, 1000.0, 10000.0), 50.0, fA, fB, rtol=1e-13)
+    assert mm.group(1) == f"{ref:.6e}"
+    assert (tmp_path / "log.txt").read_text().strip() == mm.group(0).strip()
+
+
+def test_comri_style_two_compartments_with_marker_file(tmp_path):
+    m = dm.cylinder_mesh(8.0, 4.0, 3000, seed=11, interface_radii=(4.0,))
+    dm.write_dolfin_xml(m, str(tmp_path / "cyl.xml"))
+    markers = np.where(m.phase == 1, 3, 2)  # partition markers; phase = marker % 2
+    with open(tmp_path / "pmk.xml", "w") as fh:
+        fh.write('<?xml version="1.0"?>\n<dolfin>\n  <mesh_function>\n')
+        fh.write(f'    <mesh_value_collection type="uint" dim="3" size="{m.nc}">\n')
+        for i, v in enumerate(markers):
+            fh.write(f'      <value cell_index="{i}" local_entity="0" value="{v}" />\n')
+        fh.write("    </mesh_value_collection>\n  </mesh_function>\n</dolfin>\n")
+    out = _run(["-m", "cyl.xml", "-c", "pmk.xml", "-variant", "comri", "-b", 2000, "-d", 10600, "-D", 43100,
+                "-k", 400, "-K", 3e-3, "-p", 1e-5, "-v", 1, 1, 0, "-rtol", 1e-13, "-atol", 0], tmp_path)
+    mm = re.search(r"b: 2000\.000000, gnorm: ([0-9.e+-]+), q: [0-9.e+-]+, perm: 1\.000000e-05, "
+                   r"gdir: \(0\.707107, 0\.707107, 0\.000000\), s: ([0-9.]+)", out)
+    assert mm, out
+    S = O.System(m.xyz, m.tets, m.phase, D=3e-3, kappa=1e-5)
+    fA, fB = O.comri_sequence(O.pgse_comri(10600.0, 43100.0), 53700.0, 400.0)
+    q = O.q_from_b_pgse(2000.0, 10600.0, 43100.0)
+    ref, _, _ = S.solve((1, 1, 0), q, 400.0, fA, fB, rtol=1e-13)
+    assert mm.group(2) == f"{ref:f}"
+    assert abs(float(mm.group(1)) - q) <= 1e-6 * q
+
+
+def test_cli_rejects_zero_gradient(tmp_path):
+    m = dm.box_mesh((0,) * 3, (1,) * 3, 2, 2, 2)
+    dm.write_dolfin_xml(m, str(tmp_path / "b.xml"))
+    r = subprocess.run([CLI, "-f", "b.xml", "-gdir", "0", "0", "0"], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode != 0 and "|g|=0" in r.stdout
