@@ -530,16 +530,16 @@ struct Stepper {
         int it = 0, it_before = 0, restarts = 0;
         for (;;) {
             const int nl = std::min(chunk, kEventPool / 2);
+            int ntimed = 0;
             for (int k = 0; k < nl; ++k, ++it) {
+                // the SpMV launches of iterations it < profile carry start/end events that take
+                // the timestamps of their own dispatch (no marker packets between kernels)
+                const bool timed = prof && it < opts->profile;
                 launch_bicg_p(geo, a, it, s);
-                if (prof) DMRI_HIP(hipEventRecord(c->ev0[2 * k], s));
-                launch_spmv_v(geo, a, it, s);
-                if (prof) {
-                    DMRI_HIP(hipEventRecord(c->ev1[2 * k], s));
-                    DMRI_HIP(hipEventRecord(c->ev0[2 * k + 1], s));
-                }
-                launch_spmv_t(geo, a, it, s);
-                if (prof) DMRI_HIP(hipEventRecord(c->ev1[2 * k + 1], s));
+                launch_spmv_v(geo, a, it, s, timed ? c->ev0[2 * k] : nullptr, timed ? c->ev1[2 * k] : nullptr);
+                launch_spmv_t(geo, a, it, s, timed ? c->ev0[2 * k + 1] : nullptr,
+                              timed ? c->ev1[2 * k + 1] : nullptr);
+                ntimed += timed ? 1 : 0;
                 launch_bicg_update(geo, a, it, s);
             }
             DMRI_HIP(hipMemcpyAsync(w.h_flag, w.all_done.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -552,7 +552,7 @@ struct Stepper {
                 for (int j = 0; j < S; ++j) it_last = std::max(it_last, w.h_st[j].iters);
             }
             if (prof) {
-                const int real = std::max(0, std::min(nl, it_last - it_before));
+                const int real = std::max(0, std::min(ntimed, it_last - it_before));
                 for (int k = 0; k < real; ++k)
                     for (int j = 0; j < 2; ++j) {
                         float ms = 0;
@@ -791,6 +791,67 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     c->solved = false;  // work.x no longer holds u^N of a plain solve
 }
 
+// Launch spmv_v and spmv_t `reps` times each (each followed by its finalize kernel, as in a
+// solve) on the prepared solver buffers; us_v, us_t = mean duration of one SpMV kernel from the
+// start and end timestamps of its own dispatch (events bound to the kernel, no marker packets).
+void replay_spmv(dmri_ctx *c, const SolveGeom &geo, int S, bool per_gdir, const std::vector<KrylovState> &st0,
+                 int reps, double &us_v, double &us_t) {
+    hipStream_t s = c->stream;
+    SolverWork &w = c->work;
+    const int64_t n = c->ndof;
+    const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
+    if (w.part.n < 3 * np) w.part.alloc(3 * np);
+    SolveArgs a{};
+    a.n = n;
+    a.rowptr = c->rowptr.p;
+    a.col = c->col.p;
+    a.vals = w.vals.p;
+    a.bre = w.bre.p;
+    a.per_gdir = per_gdir;
+    a.coefA = w.coef.p;
+    a.coefB = w.coef.p;
+    a.x = w.x.p;
+    a.r = w.r.p;
+    a.rh = w.rh.p;
+    a.p = w.p.p;
+    a.v = w.v.p;
+    a.t = w.t.p;
+    a.b = w.b.p;
+    a.xprev = w.xprev.p;
+    a.partU = w.part.p;
+    a.partV = w.part.p + np;
+    a.partT = w.part.p + 2 * np;
+    const bool use_fin = S > 1 || std::max(geo.grid, geo.vgrid) > 512;  // as in a solve
+    if (w.tot.n < (size_t)3 * S * kNV) w.tot.alloc(3 * (size_t)S * kNV);
+    a.totU = use_fin ? w.tot.p : nullptr;
+    a.totV = use_fin ? w.tot.p + S * kNV : nullptr;
+    a.totT = use_fin ? w.tot.p + 2 * S * kNV : nullptr;
+    a.st = w.st.p;
+    a.all_done = w.all_done.p;
+    a.rtol = 1e-12;
+    a.maxit = 1;
+    DMRI_HIP(hipMemcpyAsync(w.st.p, st0.data(), sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
+    DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
+    launch_spmv_v(geo, a, 0, s);  // warm-up
+    launch_spmv_t(geo, a, 0, s);
+    reps = std::min(reps, kEventPool / 2);
+    for (int k = 0; k < reps; ++k) {
+        launch_spmv_v(geo, a, 0, s, c->ev0[2 * k], c->ev1[2 * k]);
+        launch_spmv_t(geo, a, 0, s, c->ev0[2 * k + 1], c->ev1[2 * k + 1]);
+    }
+    DMRI_HIP(hipStreamSynchronize(s));
+    double sv = 0, stt = 0;
+    for (int k = 0; k < reps; ++k) {
+        float ms = 0;
+        DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[2 * k], c->ev1[2 * k]));
+        sv += ms;
+        DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[2 * k + 1], c->ev1[2 * k + 1]));
+        stt += ms;
+    }
+    us_v = 1e3 * sv / reps;
+    us_t = 1e3 * stt / reps;
+}
+
 // Time the SpMV kernel of every lanes-per-row candidate for S signals on the assembled pattern
 // and remember the fastest for later solves of this context.
 // Time spmv_v + spmv_t for every (mapping, lanes per row) candidate for S signals on the
@@ -815,6 +876,7 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
     launch_init_x(n, S, w.sc.p, w.p.p, s);
     launch_init_x(n, S, w.sc.p, w.r.p, s);
     launch_init_x(n, S, w.sc.p, w.v.p, s);
+    launch_init_x(n, S, w.sc.p, w.rh.p, s);
     std::vector<double> coef((size_t)S * 3, 1e-4);
     DMRI_HIP(hipMemcpyAsync(w.coef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
     std::vector<KrylovState> st0(S);
@@ -831,45 +893,9 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
             const SolveGeom geo = make_geom(n, G, S, m);
             const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
             if (w.part.n < 3 * np) w.part.alloc(3 * np);
-            SolveArgs a{};
-            a.n = n;
-            a.rowptr = c->rowptr.p;
-            a.col = c->col.p;
-            a.vals = w.vals.p;
-            a.bre = w.bre.p;
-            a.per_gdir = per_gdir;
-            a.coefA = w.coef.p;
-            a.coefB = w.coef.p;
-            a.x = w.x.p;
-            a.r = w.r.p;
-            a.rh = w.p.p;
-            a.p = w.p.p;
-            a.v = w.v.p;
-            a.t = w.t.p;
-            a.b = w.b.p;
-            a.xprev = w.xprev.p;
-            a.partU = w.part.p;
-            a.partV = w.part.p + np;
-            a.partT = w.part.p + 2 * np;
-            a.st = w.st.p;
-            a.all_done = w.all_done.p;
-            a.rtol = 1e-12;
-            a.maxit = 1;
-            const int reps = 4;
-            float total = 0.0f;
-            for (int rep = -1; rep < reps; ++rep) {  // rep -1 warms up
-                DMRI_HIP(hipMemcpyAsync(w.st.p, st0.data(), sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
-                DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
-                DMRI_HIP(hipEventRecord(c->ev0[0], s));
-                launch_spmv_v(geo, a, 0, s);
-                launch_spmv_t(geo, a, 0, s);
-                DMRI_HIP(hipEventRecord(c->ev1[0], s));
-                DMRI_HIP(hipEventSynchronize(c->ev1[0]));
-                float ms = 0;
-                DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[0], c->ev1[0]));
-                if (rep >= 0) total += ms;
-            }
-            slot = 1e3 * total / reps;
+            double uv = 0, ut = 0;
+            replay_spmv(c, geo, S, per_gdir, st0, 4, uv, ut);
+            slot = uv + ut;
             if (slot < best_us) {
                 best_us = slot;
                 *best_lanes = G;
@@ -1145,6 +1171,50 @@ int dmri_tune(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t *best_lanes, 
         tune_impl(c, nsig, per_gdir != 0, &bl, &bm, us);
         *best_lanes = bl;
         *best_mapping = bm;
+    });
+}
+
+int dmri_time_spmv(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t reps, double *us_v, double *us_t,
+                   double *bytes_per_launch) {
+    return guarded([&] {
+        require(c != nullptr && us_v != nullptr && us_t != nullptr, DMRI_ERR_ARG, "null argument");
+        require(c->assembled, DMRI_ERR_STATE, "dmri_time_spmv: call dmri_assemble first");
+        require(nsig >= 1 && nsig <= kMaxSignals && reps >= 1, DMRI_ERR_ARG, "need 1 <= nsig <= 64 and reps >= 1");
+        set_device(c);
+        const bool pg = per_gdir != 0;
+        int mapping, lanes;
+        default_mapping(nsig, mapping, lanes);
+        for (const auto &t : c->tuned)
+            if (t.first == (long long)nsig * 2 + (pg ? 1 : 0)) {
+                mapping = t.second / 64;
+                lanes = t.second % 64;
+            }
+        const SolveGeom geo = make_geom(c->ndof, lanes, nsig, mapping);
+        std::vector<KrylovState> st0(nsig);
+        std::memset(st0.data(), 0, sizeof(KrylovState) * nsig);
+        for (auto &x : st0) x.rho[0] = make_double2(1.0, 0.0);
+        // the buffers keep the last solve's scaled values and vectors when it had this shape
+        if (c->work.x.n != (size_t)c->ndof * nsig || c->work.vals.n != (size_t)c->nnz * (pg ? 4 : 2)) {
+            ensure_group_workspace(c, geo, 1, pg);
+            auto &V = c->val;
+            SolverWork &w = c->work;
+            launch_prep_scale(c->ndof, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
+                              V[DMRI_MAT_I].p, 1.0, 0.5, w.sc.p, c->stream);
+            launch_prep_values(c->ndof, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p,
+                               V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p,
+                               V[DMRI_MAT_JZ].p, pg, 1.0, 0.0, 0.0, 1.0, 0.5, w.vals.p, w.bre.p, c->stream);
+            launch_init_x(c->ndof, nsig, w.sc.p, w.p.p, c->stream);
+            launch_init_x(c->ndof, nsig, w.sc.p, w.r.p, c->stream);
+            launch_init_x(c->ndof, nsig, w.sc.p, w.v.p, c->stream);
+            launch_init_x(c->ndof, nsig, w.sc.p, w.rh.p, c->stream);
+            std::vector<double> coef((size_t)nsig * 3, 1e-4);
+            DMRI_HIP(hipMemcpyAsync(w.coef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice,
+                                    c->stream));
+        }
+        replay_spmv(c, geo, nsig, pg, st0, reps, *us_v, *us_t);
+        if (bytes_per_launch)
+            *bytes_per_launch = (double)c->nnz * (4.0 + 8.0 * (pg ? 4 : 2)) + 4.0 * (double)(c->ndof + 1) +
+                                48.0 * (double)c->ndof * nsig;
     });
 }
 

@@ -1,6 +1,7 @@
 // Internal declarations shared by the libdmri host code and its gfx950 kernels.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -135,8 +136,10 @@ struct SolveArgs {
 // xprev <- 2 x - xprev; 3: b only
 void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const int *mask, hipStream_t s);
 void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
-void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
-void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
+void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr);
+void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr);
 void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
 // per-signal sum_i w_i sc_i x[i*S+s] into out[2*s..2*s+1]
 void launch_signal(int64_t n, int S, const double *w, const double *sc, const double2 *x, double *part, double *out,

@@ -1684,15 +1684,19 @@ void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
     bicg_p_kernel<<<g.vgrid, g.vec_threads, 0, s>>>(a, g.S, g.vec_rpb, g.grid, g.vgrid, it);
     check_launch();
 }
-void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s) {
+// The SpMV kernels go through hipExtLaunchKernelGGL so that e0/e1 (when not null) take the start
+// and end timestamps of the kernel's own dispatch: no marker packet, hence no extra gap.
+void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (g.mapping == kMapSR) {
         DMRI_DISPATCH_G(g.G, a.per_gdir,
-                        (spmv_v_sr_kernel<GG, PP><<<g.grid, g.sr_threads, 0, s>>>(a, g.S, g.sr_rpb, it)));
+                        hipExtLaunchKernelGGL((spmv_v_sr_kernel<GG, PP>), dim3(g.grid), dim3(g.sr_threads), 0, s, e0,
+                                              e1, 0, a, g.S, g.sr_rpb, it));
     } else {
         const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
-        const size_t lds = TR > 0 ? stage_lds_bytes(TR, a.per_gdir) : 0;
+        const uint32_t lds = TR > 0 ? (uint32_t)stage_lds_bytes(TR, a.per_gdir) : 0u;
         DMRI_DISPATCH_G(g.G, a.per_gdir,
-                        (spmv_v_kernel<GG, PP><<<g.grid, g.spmv_threads, lds, s>>>(a, g.S, g.rpb, TR, it)));
+                        hipExtLaunchKernelGGL((spmv_v_kernel<GG, PP>), dim3(g.grid), dim3(g.spmv_threads), lds, s, e0,
+                                              e1, 0, a, g.S, g.rpb, TR, it));
     }
     check_launch();
     if (a.totV) {
@@ -1700,15 +1704,17 @@ void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
         check_launch();
     }
 }
-void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s) {
+void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (g.mapping == kMapSR) {
         DMRI_DISPATCH_G(g.G, a.per_gdir,
-                        (spmv_t_sr_kernel<GG, PP><<<g.grid, g.sr_threads, 0, s>>>(a, g.S, g.sr_rpb, g.grid, it)));
+                        hipExtLaunchKernelGGL((spmv_t_sr_kernel<GG, PP>), dim3(g.grid), dim3(g.sr_threads), 0, s, e0,
+                                              e1, 0, a, g.S, g.sr_rpb, (int)g.grid, it));
     } else {
         const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
-        const size_t lds = TR > 0 ? stage_lds_bytes(TR, a.per_gdir) : 0;
+        const uint32_t lds = TR > 0 ? (uint32_t)stage_lds_bytes(TR, a.per_gdir) : 0u;
         DMRI_DISPATCH_G(g.G, a.per_gdir,
-                        (spmv_t_kernel<GG, PP><<<g.grid, g.spmv_threads, lds, s>>>(a, g.S, g.rpb, TR, g.grid, it)));
+                        hipExtLaunchKernelGGL((spmv_t_kernel<GG, PP>), dim3(g.grid), dim3(g.spmv_threads), lds, s, e0,
+                                              e1, 0, a, g.S, g.rpb, TR, (int)g.grid, it));
     }
     check_launch();
     if (a.totT) {

@@ -68,7 +68,7 @@ class _Grads(ctypes.Structure):
 
 EXPORTED = ["dmri_create", "dmri_destroy", "dmri_last_error", "dmri_version", "dmri_mesh_set", "dmri_coeffs_set",
             "dmri_assemble", "dmri_get_info", "dmri_export_pattern", "dmri_export_values", "dmri_export_facets",
-            "dmri_solve", "dmri_solve_batched", "dmri_tune", "dmri_set_tuning", "dmri_copy_bandwidth",
+            "dmri_solve", "dmri_solve_batched", "dmri_tune", "dmri_set_tuning", "dmri_time_spmv", "dmri_copy_bandwidth",
             "dmri_adjoint", "dmri_get_solution"]
 
 
@@ -95,6 +95,7 @@ def _load():
     L.dmri_get_solution.argtypes = [vp, ctypes.c_int32, dp]
     L.dmri_tune.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, i32p, i32p, dp]
     L.dmri_set_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.dmri_time_spmv.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, dp, dp, dp]
     L.dmri_copy_bandwidth.argtypes = [vp, i64, ctypes.c_int32, dp]
     L.dmri_adjoint.argtypes = [vp, P(_Seq), P(_Opts), P(_AdjOpts), dp, P(_Grads), P(_Stats)]
     for name in EXPORTED:
@@ -211,7 +212,7 @@ class Context:
         for i in range(3):
             s.gdir[i] = float(gdir[i])
         o = _Opts()
-        o.rtol, o.atol, o.maxit, o.profile = float(rtol), float(atol), int(maxit), 1 if profile else 0
+        o.rtol, o.atol, o.maxit, o.profile = float(rtol), float(atol), int(maxit), int(profile)
         o.lanes_per_row = int(lanes)
         o.initial_guess = int(guess)
         o.mapping = int(mapping)
@@ -277,6 +278,14 @@ class Context:
                  "f_implicit": dfa[:n].copy(), "f_explicit": dfb[:n].copy(), "ms_forward": g.ms_forward,
                  "ms_adjoint": g.ms_adjoint, "checkpoints": g.checkpoints, "recomputed_steps": g.recomputed_steps}
         return complex(sig[0], sig[1]), grads, {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    def time_spmv(self, nsig: int, per_gdir: bool = False, reps: int = 50):
+        """(us per spmv_v launch, us per spmv_t launch, algorithmic bytes per launch) of back-to-back
+        replays of the two SpMV kernels with the mapping a solve of nsig signals would use."""
+        uv, ut, nb = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(LIB.dmri_time_spmv(self._h, int(nsig), int(bool(per_gdir)), int(reps), ctypes.byref(uv),
+                                  ctypes.byref(ut), ctypes.byref(nb)))
+        return uv.value, ut.value, nb.value
 
     def copy_bandwidth(self, nbytes: int, reps: int = 10) -> float:
         """GB/s of the library's plain copy kernel (read + write bytes)."""

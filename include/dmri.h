@@ -122,7 +122,9 @@ typedef struct {
     double rtol;           /* ||r|| <= max(rtol ||b||, atol); reference 1e-9 (GCloudDmriSolver.py:220) */
     double atol;
     int32_t maxit;         /* per time step */
-    int32_t profile;       /* 1: time the SpMV launches with HIP events (dmri_stats.spmv_ms) */
+    int32_t profile;       /* k > 0: time the SpMV launches of the first k Krylov iterations of every
+                              time step with HIP events (dmri_stats.spmv_ms, spmv_launches); the
+                              events cost GPU time of their own, so a small k perturbs least */
     int32_t lanes_per_row; /* SpMV lanes per row: 0 = tuned value (dmri_tune / dmri_set_tuning) or the
                               library default; else a power of two in 1..32 */
     int32_t initial_guess; /* Krylov start of step n: 0 = 2u^n - u^(n-1) (extrapolated), 1 = u^n */
@@ -138,7 +140,7 @@ typedef struct {
     int32_t restarts;       /* BiCGStab breakdown restarts */
     double relres_max;      /* max final ||r||/||b|| over the steps */
     double solve_ms;        /* wall time of the time loop (device synchronised) */
-    double spmv_ms;         /* summed device time of the timed SpMV launches (profile=1) */
+    double spmv_ms;         /* summed device time of the timed SpMV launches (profile > 0) */
     int64_t spmv_launches;  /* number of timed SpMV launches */
     double spmv_bytes;      /* algorithmic bytes of one SpMV launch */
 } dmri_stats;
@@ -169,6 +171,16 @@ int dmri_solve_batched(dmri_ctx *ctx, int32_t nsig, const double *q, const doubl
 int dmri_tune(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t *best_lanes, int32_t *best_mapping,
               double *us);
 int dmri_set_tuning(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t lanes, int32_t mapping);
+
+/* Replay the two SpMV kernels of a BiCGStab iteration (A p and A s, s = r - alpha v formed on the
+ * fly) `reps` times each, back to back, on this context's solver buffers for a launch group of
+ * nsig signals, with the mapping a solve of that group would use.  us_v, us_t = microseconds per
+ * launch (HIP events around each batch of reps, so no per-launch event gap is included);
+ * bytes_per_launch (may be NULL) = algorithmic HBM bytes of one launch: nnz*(4 + 8*nv) column
+ * indices and values (nv = 2, or 4 with per-signal gradient directions) + 4*(n+1) row pointers +
+ * 48*n*nsig for the complex vectors read and written.  No reference counterpart. */
+int dmri_time_spmv(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t reps, double *us_v, double *us_t,
+                   double *bytes_per_launch);
 
 /* Bandwidth of a plain device copy kernel over `bytes` (read + write counted), averaged over
  * `reps` launches: the measured reference beside the nominal HBM peak, and the calibration

@@ -227,3 +227,21 @@ def test_initial_guess_modes_agree():
     s1, st1 = ctx.solve(fA, fB, 200.0, q, (0, 1, 0), rtol=1e-13, guess=1)
     print(f"  iterations: extrapolated {st0['iters_total']} previous {st1['iters_total']}", flush=True)
     assert abs(s0 - s1) <= 1e-10 * abs(s1)
+
+
+def test_sampled_event_profile_and_spmv_replay():
+    """profile=k times the SpMV pair of the first k Krylov iterations of each step only, leaves the
+    result unchanged, and the back-to-back replay reports positive per-launch times."""
+    mesh, kw = MESHES["cyl2"]
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    fA, fB = _pgse_gcloud(10600.0, 43100.0, 400.0)
+    q = O.q_from_b_pgse(1000.0, 10600.0, 43100.0)
+    s0, st0 = ctx.solve(fA, fB, 400.0, q, (0, 1, 0), rtol=1e-13)
+    s1, st1 = ctx.solve(fA, fB, 400.0, q, (0, 1, 0), rtol=1e-13, profile=1)
+    assert s0 == s1
+    assert st0["spmv_launches"] == 0
+    assert len(fA) <= st1["spmv_launches"] <= 2 * len(fA) and st1["spmv_ms"] > 0
+    us_v, us_t, nb = ctx.time_spmv(1, False, 10)
+    info = ctx.info()
+    assert us_v > 0 and us_t > 0
+    assert nb == info["nnz"] * 20.0 + 4.0 * (info["ndof"] + 1) + 48.0 * info["ndof"]
