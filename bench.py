@@ -53,6 +53,21 @@ def pgse_sequence(delta, Delta, k):
     return np.asarray(fA), np.asarray(fB)
 
 
+def measured_traffic(info, mapping_lanes):
+    """HBM bytes per SpMV launch from the committed FETCH_SIZE/WRITE_SIZE counter runs of this
+    workload (copy-kernel calibrated), if the mesh and SpMV mapping are the ones measured."""
+    path = os.path.join(ROOT, "profiles", "spmv_traffic_cylinders200k_5sig.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if (d.get("nnz"), d.get("ndof")) != (info["nnz"], info["ndof"]) or \
+            list(d.get("mapping_lanes", [])) != list(mapping_lanes or []):
+        return None
+    return float(d["traffic_per_launch"])
+
+
 def q_from_b(b, delta, Delta):
     return float(np.sqrt(b) / np.sqrt(delta * delta * (Delta - delta / 3.0)))
 
@@ -135,6 +150,8 @@ def main():
     achieved = spmv_bytes / (avg_us * 1e-6) / 1e9 if spmv_n else 0.0
     iters = sum(s["iters_total"] for s in all_stats)
     us_v, us_t, replay_bytes = ctx.time_spmv(len(bvals), False, 50)
+    used = tuned if tuned is not None else ([args.mapping, args.lanes] if args.mapping and args.lanes else None)
+    traffic = measured_traffic(info, used)
     replay_achieved = replay_bytes / (0.5 * (us_v + us_t) * 1e-6) / 1e9
 
     # b = 0 must keep the signal at 1 (mass conservation; size-independent parity property)
@@ -194,7 +211,9 @@ def main():
         "setup_ms": info["setup_ms"],
         "mesh_gen_s": t_mesh,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/spmv_traffic_cylinders200k_5sig.json (rocprofv3 FETCH_SIZE x1.9998 + "
+                                       "WRITE_SIZE x1.0, copy-kernel calibrated)" if traffic else None,
                      "kernel": "spmv_v/spmv_t (batched complex value-split CSR SpMV, 5 signals, fused reductions)",
                      "avg_launch_us": avg_us, "bytes_per_launch": spmv_bytes, "launches": spmv_n,
                      "events": f"first {args.event_iters} Krylov iterations of every theta step",
