@@ -220,8 +220,12 @@ class MyDomain:
         self.D = np.stack(comps, axis=1)
 
     def Apply(self):
-        if sum(self.PeriodicDir) > 0:
-            raise NotImplementedError("pseudo-periodic boundary conditions are not available in this build")
+        # PeriodicDir with IsDomainPeriodic = False: the weak pseudo-periodic condition
+        # (MyDomain.Apply, DmriFemLib.py:618-637) runs in libdmri; the strong form
+        # (IsDomainPeriodic = True, PeriodicBD 327-375) is not available
+        if sum(self.PeriodicDir) > 0 and self.IsDomainPeriodic:
+            raise NotImplementedError("the strong pseudo-periodic formulation (IsDomainPeriodic=True) is not "
+                                      "available in this build; set IsDomainPeriodic=False for the weak one")
         if self.IsDomainMultiple:
             if self.phase is None:
                 raise ValueError("IsDomainMultiple=True needs mydomain.phase (per-cell 0/1)")
@@ -236,15 +240,19 @@ class MRI_simulation:
         self.theta = 0.5
         self.device = 0
 
-    def _sequence(self, mri_para):
-        fA, fB = [], []
+    def _sequence(self, mri_para, integral=False):
+        """f(t_n), f(t_{n-1}) and, for the periodic terms, F(t_{n-1}) = itime_profile(tp)
+        (MRI_simulation.solve, DmriFemLib.py:897-910)."""
+        fA, fB, FB = [], [], []
         t, tp = 0.0, 0.0
         while t < mri_para.T + self.k:
             fA.append(mri_para.time_profile(t))
             fB.append(mri_para.time_profile(tp))
+            if integral:
+                FB.append(mri_para.itime_profile(tp))
             tp = t
             t += self.k
-        return np.asarray(fA), np.asarray(fB)
+        return np.asarray(fA), np.asarray(fB), (np.asarray(FB) if integral else None)
 
     def solve(self, mydomain, mri_para, linsolver, ic=None):
         if ic is not None:
@@ -266,14 +274,17 @@ class MRI_simulation:
         else:
             ctx.set_coeffs(D=0.0, D_cell=np.asarray(D), invT2=invT2 if invT2 is not None else 0.0,
                            kappa=float(mydomain.kappa), invT2_cell=invT2_cell)
+        periodic = sum(mydomain.PeriodicDir) > 0
+        ctx.set_periodic(mydomain.PeriodicDir if periodic else (0, 0, 0))
         ctx.assemble()
-        fA, fB = self._sequence(mri_para)
+        fA, fB, FB = self._sequence(mri_para, integral=periodic)
         p = linsolver.parameters
         start = time.time()
         g = mri_para.gdir.v if isinstance(mri_para.gdir, Point) else np.asarray(mri_para.gdir, dtype=np.float64)
         self.signal_c, self.stats = ctx.solve(fA, fB, self.k, mri_para.qvalue, g, theta=self.theta,
                                               rtol=p["relative_tolerance"], atol=p["absolute_tolerance"],
-                                              maxit=min(int(p["maximum_iterations"]), 2 ** 31 - 1))
+                                              maxit=min(int(p["maximum_iterations"]), 2 ** 31 - 1),
+                                              F_explicit=FB)
         self.elapsed_time = time.time() - start
         self.t = len(fA) * self.k
         self.nsteps = len(fA)

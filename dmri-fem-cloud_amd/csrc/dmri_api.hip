@@ -62,8 +62,8 @@ T read_scalar(const T *d, hipStream_t s) {
 constexpr int kEventPool = 512;
 
 struct SolverWork {
-    DevArray<double2> x, r, rh, p, v, t, b, xprev;
-    DevArray<double> sc, vals, bre, part, tot, coef, sig, spart;
+    DevArray<double2> x, r, rh, p, v, t, b, xprev, ybc, bper;
+    DevArray<double> sc, vals, bre, part, tot, coef, coefP, sig, spart;
     DevArray<KrylovState> st;
     DevArray<int> all_done, mask;
     KrylovState *h_st = nullptr;
@@ -87,6 +87,7 @@ struct dmri_ctx {
     int ncomp = 1;
     bool has_mesh = false;
     double hmin = 0, hmax = 0;
+    double bb_lo[3] = {0, 0, 0}, bb_hi[3] = {0, 0, 0};
     DevArray<double> xyz;
     DevArray<int> tets, phase;
     // coefficients
@@ -104,6 +105,12 @@ struct dmri_ctx {
     DevArray<int> vptr, vlist, fptr, flist, iface_v, iface_c, bnd_v, bnd_c;
     DevArray<double> val[7];
     DevArray<double> w;
+    // weak pseudo-periodic boundary condition
+    int pdir[3] = {0, 0, 0};
+    bool periodic = false;                 // assembled with periodic directions
+    int64_t n_per = 0, n_prow = 0;
+    DevArray<double> pval, per_w, per_ell;
+    DevArray<int> per_dof, per_src, prow, bptr, blist;
     // solver workspace
     SolverWork work;
     DevArray<double> part, dsig;
@@ -209,6 +216,87 @@ std::vector<int64_t> colour_elements(int nvert, const int *ev, int64_t ne, const
     launch_color_scatter(color.p, ne, doffs.p, cursor.p, perm.p, s);
     DMRI_HIP(hipStreamSynchronize(s));
     return offs;
+}
+
+// Weak pseudo-periodic set-up: P on the pattern (rows gather their boundary facets), the rows
+// holding P, and the mirror entries located on facet bins of the opposite faces.
+void assemble_periodic(dmri_ctx *c) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->ndof, nv = c->nv, nbf = c->n_bfacet;
+    PeriodicBox B{};
+    for (int d = 0; d < 3; ++d) {
+        B.dir[d] = c->pdir[d] ? 1 : 0;
+        B.lo[d] = c->bb_lo[d];
+        B.hi[d] = c->bb_hi[d];
+    }
+    B.hmin = c->hmin;
+    const int *phase = c->ncomp == 2 ? c->phase.p : nullptr;
+    build_incidence(3, c->bnd_v.p, nbf, nv, c->bptr, c->blist, s);
+    c->pval.alloc(c->nnz);
+    c->pval.zero(s);
+    DevArray<int> flag, scan, miss;
+    flag.alloc(n);
+    scan.alloc(n + 1);
+    miss.alloc(1);
+    miss.zero(s);
+    launch_periodic_mass(n, nv, c->dof_vert.p, c->dof_comp.p, c->bptr.p, c->blist.p, c->bnd_v.p, c->bnd_c.p, phase,
+                         c->xyz.p, c->dof_of.p, c->rowptr.p, c->col.p, B, c->pval.p, flag.p, miss.p, s);
+    require(read_scalar(miss.p, s) == 0, DMRI_ERR_MESH, "periodic boundary entry outside the CSR pattern");
+    exclusive_scan(flag.p, scan.p, n, s);
+    c->n_prow = read_scalar(scan.p + n, s);
+    c->prow.alloc(std::max<int64_t>(c->n_prow, 1));
+    launch_compact_rows(n, flag.p, scan.p, c->prow.p, s);
+    // facet bins per face: bin edge >= hmax >= any facet's diameter, so the facet that holds a
+    // point has its centroid within one bin of the point's bin
+    PlaneGrids G{};
+    int nbins = 0;
+    for (int p = 0; p < 6; ++p) {
+        const int d = p / 2;
+        PlaneGrid &g = G.p[p];
+        g.u = (d + 1) % 3;
+        g.w = (d + 2) % 3;
+        g.lo_u = c->bb_lo[g.u];
+        g.lo_w = c->bb_lo[g.w];
+        const double eu = c->bb_hi[g.u] - c->bb_lo[g.u], ew = c->bb_hi[g.w] - c->bb_lo[g.w];
+        g.h_u = std::max(c->hmax, eu / 1023.0);
+        g.h_w = std::max(c->hmax, ew / 1023.0);
+        g.nb_u = (int)std::floor(eu / g.h_u) + 1;
+        g.nb_w = (int)std::floor(ew / g.h_w) + 1;
+        g.off = nbins;
+        nbins += c->pdir[d] ? g.nb_u * g.nb_w : 0;
+        if (!c->pdir[d]) g.nb_u = g.nb_w = 0;
+    }
+    DevArray<int> fbin, cnt, binptr, cursor, binlist, bad;
+    fbin.alloc(std::max<int64_t>(nbf, 1));
+    cnt.alloc(std::max(nbins, 1));
+    cnt.zero(s);
+    binptr.alloc(nbins + 1);
+    cursor.alloc(std::max(nbins, 1));
+    cursor.zero(s);
+    launch_bfacet_bin(nbf, c->bnd_v.p, c->xyz.p, B, G, fbin.p, cnt.p, s);
+    exclusive_scan(cnt.p, binptr.p, nbins, s);
+    const int64_t nplane = read_scalar(binptr.p + nbins, s);
+    binlist.alloc(std::max<int64_t>(nplane, 1));
+    launch_bfacet_fill(nbf, fbin.p, binptr.p, cursor.p, binlist.p, s);
+    // mirror entries
+    launch_periodic_flag(n, c->dof_vert.p, c->xyz.p, B, flag.p, s);
+    exclusive_scan(flag.p, scan.p, n, s);
+    c->n_per = read_scalar(scan.p + n, s);
+    const size_t ne = (size_t)std::max<int64_t>(c->n_per, 1);
+    c->per_dof.alloc(ne);
+    c->per_src.alloc(3 * ne);
+    c->per_w.alloc(3 * ne);
+    c->per_ell.alloc(3 * ne);
+    bad.alloc(1);
+    bad.zero(s);
+    if (c->n_per > 0) {
+        require(nplane > 0, DMRI_ERR_MESH, "periodic direction without flat boundary faces on the bounding box");
+        launch_periodic_entries(n, nv, flag.p, scan.p, c->dof_vert.p, c->dof_comp.p, c->dof_of.p, c->xyz.p,
+                                c->bnd_v.p, binptr.p, binlist.p, B, G, c->per_dof.p, c->per_src.p, c->per_w.p,
+                                c->per_ell.p, bad.p, s);
+        require(read_scalar(bad.p, s) == 0, DMRI_ERR_MESH,
+                "a mirror point of a periodic face has no boundary facet on the opposite face");
+    }
 }
 
 void assemble_impl(dmri_ctx *c) {
@@ -326,6 +414,11 @@ void assemble_impl(dmri_ctx *c) {
     }
     require(read_scalar(miss.p, s) == 0, DMRI_ERR_MESH, "element entry outside the CSR pattern");
 
+    // 5b. weak pseudo-periodic boundary mass and mirror entries
+    c->periodic = (c->pdir[0] || c->pdir[1] || c->pdir[2]) && c->n_bfacet > 0;
+    c->n_per = c->n_prow = 0;
+    if (c->periodic) assemble_periodic(c);
+
     // 6. diagonal positions, signal weights w = M 1, volume
     {
         DevArray<int> missing;
@@ -424,6 +517,7 @@ struct Stepper {
     const dmri_solver_opts *opts = nullptr;
     dmri_stats *st = nullptr;
     double2 *xc = nullptr, *xp = nullptr;
+    double omt = 0.5;
     int chunk = 8;
     std::vector<KrylovState> zero_st;
 
@@ -455,11 +549,30 @@ struct Stepper {
         SolverWork &w = c->work;
         auto &V = c->val;
         const double theta = seq->theta, dt = seq->dt;
-        launch_prep_scale(n, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, dt,
+        const double *pv = c->periodic ? c->pval.p : nullptr;
+        launch_prep_scale(n, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, pv, dt,
                           theta, w.sc.p, s);
         launch_prep_values(n, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
-                           V[DMRI_MAT_I].p, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, g[0],
+                           V[DMRI_MAT_I].p, pv, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, g[0],
                            g[1], g[2], dt, theta, w.vals.p, w.bre.p, s);
+        omt = 1.0 - theta;
+        if (c->periodic) {
+            // mirror phase per step and signal: q_s F(t_{n-1}) g_s
+            std::vector<double> hP((size_t)std::max(N, 1) * S * 3, 0.0);
+            for (int k = 0; k < N; ++k)
+                for (int j = 0; j < S; ++j)
+                    for (int d = 0; d < 3; ++d) {
+                        const double gd = g[3 * j + d];
+                        hP[((size_t)k * S + j) * 3 + d] = q[j] * seq->F_explicit[k] * gd;
+                    }
+            if (w.coefP.n < hP.size()) w.coefP.alloc(hP.size());
+            DMRI_HIP(hipMemcpyAsync(w.coefP.p, hP.data(), sizeof(double) * hP.size(), hipMemcpyHostToDevice, s));
+            const size_t nS = (size_t)n * S;
+            if (w.ybc.n != nS) w.ybc.alloc(nS);
+            if (w.bper.n != nS) w.bper.alloc(nS);
+            w.ybc.zero(s);
+            w.bper.zero(s);
+        }
         // per-step imaginary coefficients: implicit side theta q f_A(n) (g), explicit -(1-theta) q f_B(n) (g)
         std::vector<double> coef((size_t)2 * std::max(N, 1) * S * 3, 0.0);
         double *cA = coef.data(), *cB = coef.data() + (size_t)std::max(N, 1) * S * 3;
@@ -505,6 +618,7 @@ struct Stepper {
         a.totT = use_fin ? w.tot.p + 2 * S * kNV : nullptr;
         a.st = w.st.p;
         a.all_done = w.all_done.p;
+        a.bper = c->periodic ? w.bper.p : nullptr;
         a.rtol = opts->rtol;
         a.atol = opts->atol;
         a.maxit = opts->maxit;
@@ -613,6 +727,12 @@ struct Stepper {
         begin_step(k);
         a.x = xc;
         a.xprev = xp;
+        if (c->periodic) {
+            SolverWork &w = c->work;
+            launch_periodic_step(c->n_per, c->n_prow, S, c->per_dof.p, c->per_src.p, c->per_w.p, c->per_ell.p,
+                                 c->prow.p, c->rowptr.p, c->col.p, c->pval.p, w.sc.p, xc,
+                                 w.coefP.p + (size_t)k * S * 3, omt, w.ybc.p, w.bper.p, s);
+        }
         if (extrap && k > 0) {
             // b from u^n; xp <- 2 u^n - u^(n-1); swap so that the guess is iterated on
             launch_rhs_residual(geo, a, 2, nullptr, s);
@@ -868,11 +988,11 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
     ensure_group_workspace(c, g0, 1, per_gdir);
     SolverWork &w = c->work;
     auto &V = c->val;
-    launch_prep_scale(n, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, 1.0, 0.5,
-                      w.sc.p, s);
+    launch_prep_scale(n, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, nullptr,
+                      1.0, 0.5, w.sc.p, s);
     launch_prep_values(n, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
-                       V[DMRI_MAT_I].p, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, 1.0, 0.0, 0.0,
-                       1.0, 0.5, w.vals.p, w.bre.p, s);
+                       V[DMRI_MAT_I].p, nullptr, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, 1.0,
+                       0.0, 0.0, 1.0, 0.5, w.vals.p, w.bre.p, s);
     launch_init_x(n, S, w.sc.p, w.p.p, s);
     launch_init_x(n, S, w.sc.p, w.r.p, s);
     launch_init_x(n, S, w.sc.p, w.v.p, s);
@@ -934,6 +1054,12 @@ void solve_batched_impl(dmri_ctx *c, int nsig, const double *q, const double *gd
     require(c->assembled, DMRI_ERR_STATE, "solve: call dmri_assemble first");
     require(nsig >= 1 && q && gdir && sig, DMRI_ERR_ARG, "solve: need nsig >= 1 and non-null q, gdir, sig_out");
     validate_seq(seq, opts);
+    if (c->periodic && seq->nsteps > 0) {
+        require(seq->F_explicit != nullptr, DMRI_ERR_ARG,
+                "periodic directions are set: dmri_seq.F_explicit (int_0^t f per step) is required");
+        for (int i = 0; i < seq->nsteps; ++i)
+            require(std::isfinite(seq->F_explicit[i]), DMRI_ERR_ARG, "F_explicit has a non-finite value");
+    }
     std::vector<double> g(3 * (size_t)nsig);
     for (int j = 0; j < nsig; ++j) {
         require(std::isfinite(q[j]), DMRI_ERR_ARG, "q is not finite");
@@ -1032,6 +1158,15 @@ int dmri_mesh_set(dmri_ctx *c, int64_t nv, const double *xyz, int64_t nc, const 
         c->ncomp = cell_phase ? 2 : 1;
         c->hmin = hmin;
         c->hmax = hmax;
+        for (int d = 0; d < 3; ++d) {
+            c->bb_lo[d] = 1e300;
+            c->bb_hi[d] = -1e300;
+        }
+        for (int64_t v = 0; v < nv; ++v)
+            for (int d = 0; d < 3; ++d) {
+                c->bb_lo[d] = std::min(c->bb_lo[d], xyz[3 * v + d]);
+                c->bb_hi[d] = std::max(c->bb_hi[d], xyz[3 * v + d]);
+            }
         c->has_mesh = true;
         c->assembled = false;
         c->solved = false;
@@ -1101,6 +1236,8 @@ int dmri_get_info(dmri_ctx *c, dmri_info *info) {
             info->ncolors_iface = c->ncol_iface;
             info->volume = c->volume;
             info->setup_ms = c->setup_ms;
+            info->n_periodic = c->periodic ? c->n_per : 0;
+            info->kappa_e = c->periodic ? 3e-3 / c->hmin : 0.0;
         }
     });
 }
@@ -1127,10 +1264,15 @@ int dmri_export_pattern(dmri_ctx *c, int64_t *rowptr, int32_t *col, int32_t *dof
 int dmri_export_values(dmri_ctx *c, int which, double *val) {
     return guarded([&] {
         require(c != nullptr && val != nullptr, DMRI_ERR_ARG, "null argument");
-        require(which >= 0 && which < 7, DMRI_ERR_ARG, "unknown matrix id");
+        require(which >= 0 && which <= DMRI_MAT_P, DMRI_ERR_ARG, "unknown matrix id");
         require(c->assembled, DMRI_ERR_STATE, "not assembled");
         set_device(c);
-        DMRI_HIP(hipMemcpyAsync(val, c->val[which].p, sizeof(double) * c->nnz, hipMemcpyDeviceToHost, c->stream));
+        if (which == DMRI_MAT_P && !c->periodic) {
+            std::memset(val, 0, sizeof(double) * c->nnz);
+            return;
+        }
+        const double *src = which == DMRI_MAT_P ? c->pval.p : c->val[which].p;
+        DMRI_HIP(hipMemcpyAsync(val, src, sizeof(double) * c->nnz, hipMemcpyDeviceToHost, c->stream));
         DMRI_HIP(hipStreamSynchronize(c->stream));
     });
 }
@@ -1143,6 +1285,31 @@ int dmri_export_facets(dmri_ctx *c, int32_t *fv) {
         if (c->n_iface)
             DMRI_HIP(hipMemcpyAsync(fv, c->iface_v.p, sizeof(int) * 3 * c->n_iface, hipMemcpyDeviceToHost, c->stream));
         DMRI_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int dmri_periodic_set(dmri_ctx *c, const int32_t dir[3]) {
+    return guarded([&] {
+        require(c != nullptr && dir != nullptr, DMRI_ERR_ARG, "null argument");
+        for (int d = 0; d < 3; ++d) c->pdir[d] = dir[d] != 0 ? 1 : 0;
+        c->assembled = false;
+        c->solved = false;
+    });
+}
+
+int dmri_export_periodic(dmri_ctx *c, int32_t *dof, int32_t *src, double *w, double *ell) {
+    return guarded([&] {
+        require(c != nullptr && dof && src && w && ell, DMRI_ERR_ARG, "null argument");
+        require(c->assembled, DMRI_ERR_STATE, "not assembled");
+        if (!c->periodic || c->n_per == 0) return;
+        set_device(c);
+        hipStream_t s = c->stream;
+        const size_t ne = (size_t)c->n_per;
+        DMRI_HIP(hipMemcpyAsync(dof, c->per_dof.p, sizeof(int) * ne, hipMemcpyDeviceToHost, s));
+        DMRI_HIP(hipMemcpyAsync(src, c->per_src.p, sizeof(int) * 3 * ne, hipMemcpyDeviceToHost, s));
+        DMRI_HIP(hipMemcpyAsync(w, c->per_w.p, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, s));
+        DMRI_HIP(hipMemcpyAsync(ell, c->per_ell.p, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, s));
+        DMRI_HIP(hipStreamSynchronize(s));
     });
 }
 
@@ -1199,9 +1366,9 @@ int dmri_time_spmv(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t reps, do
             auto &V = c->val;
             SolverWork &w = c->work;
             launch_prep_scale(c->ndof, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
-                              V[DMRI_MAT_I].p, 1.0, 0.5, w.sc.p, c->stream);
+                              V[DMRI_MAT_I].p, nullptr, 1.0, 0.5, w.sc.p, c->stream);
             launch_prep_values(c->ndof, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p,
-                               V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p,
+                               V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, nullptr, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p,
                                V[DMRI_MAT_JZ].p, pg, 1.0, 0.0, 0.0, 1.0, 0.5, w.vals.p, w.bre.p, c->stream);
             launch_init_x(c->ndof, nsig, w.sc.p, w.p.p, c->stream);
             launch_init_x(c->ndof, nsig, w.sc.p, w.r.p, c->stream);
@@ -1264,6 +1431,7 @@ int dmri_adjoint(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts,
     return guarded([&] {
         require(c != nullptr && grads != nullptr && sig_out != nullptr, DMRI_ERR_ARG, "null argument");
         require(c->assembled, DMRI_ERR_STATE, "dmri_adjoint: call dmri_assemble first");
+        require(!c->periodic, DMRI_ERR_ARG, "dmri_adjoint: the weak pseudo-periodic condition is not supported");
         validate_seq(seq, opts);
         set_device(c);
         adjoint_impl(c, seq, opts, aopts, sig_out, grads, stats);

@@ -106,14 +106,50 @@ void launch_diag_and_weights(int64_t n, const int *rowptr, const int *col, const
 // deterministic sum of x[0..n) into out[0] (two kernels, no atomics); part >= kMaxGrid doubles
 void launch_sum(const double *x, int64_t n, double *part, double *out, hipStream_t s);
 
+// ---------------------------------------------------------------- weak pseudo-periodic set-up
+struct PeriodicBox {
+    int dir[3];            // periodic axes
+    double lo[3], hi[3];   // bounding box of the mesh
+    double hmin;           // min over cells of the longest edge
+};
+struct PlaneGrid {         // centroid bins of the boundary facets lying on one face
+    int u, w;              // in-plane axes
+    int nb_u, nb_w, off;   // bins per axis, first bin id
+    double lo_u, lo_w, h_u, h_w;
+};
+struct PlaneGrids {
+    PlaneGrid p[6];        // face 2d = lo[d], 2d+1 = hi[d]
+};
+void launch_periodic_mass(int64_t n, int64_t nv, const int *dof_vert, const int *dof_comp, const int *bptr,
+                          const int *blist, const int *bnd_v, const int *bnd_c, const int *phase, const double *xyz,
+                          const int *dof_of, const int *rowptr, const int *col, const PeriodicBox &B, double *pv,
+                          int *rowflag, int *miss, hipStream_t s);
+void launch_periodic_flag(int64_t n, const int *dof_vert, const double *xyz, const PeriodicBox &B, int *flag,
+                          hipStream_t s);
+void launch_bfacet_bin(int64_t nbf, const int *bnd_v, const double *xyz, const PeriodicBox &B, const PlaneGrids &G,
+                       int *fbin, int *cnt, hipStream_t s);
+void launch_bfacet_fill(int64_t nbf, const int *fbin, const int *binptr, int *cursor, int *binlist, hipStream_t s);
+void launch_periodic_entries(int64_t n, int64_t nv, const int *flag, const int *scan, const int *dof_vert,
+                             const int *dof_comp, const int *dof_of, const double *xyz, const int *bnd_v,
+                             const int *binptr, const int *binlist, const PeriodicBox &B, const PlaneGrids &G,
+                             int *ent_dof, int *ent_src, double *ent_w, double *ent_ell, int *bad, hipStream_t s);
+void launch_compact_rows(int64_t n, const int *flag, const int *scan, int *rows, hipStream_t s);
+// per step: ybc from x through the mirror entries (phase h = q F g per signal), then
+// bper = (1-theta) S P ybc on the P rows
+void launch_periodic_step(int64_t ne, int64_t nrows, int S, const int *ent_dof, const int *ent_src,
+                          const double *ent_w, const double *ent_ell, const int *rows, const int *rowptr,
+                          const int *col, const double *pv, const double *sc, const double2 *x, const double *h,
+                          double omt, double2 *ybc, double2 *bper, hipStream_t s);
+
 // ---------------------------------------------------------------- solve launchers
+// pv: weak pseudo-periodic boundary mass (implicit side only), or null
 void launch_prep_scale(int64_t n, const int *diag, const double *m, const double *k, const double *r,
-                       const double *gi, double dt, double theta, double *sc, hipStream_t s);
+                       const double *gi, const double *pv, double dt, double theta, double *sc, hipStream_t s);
 // per-gdir == false: val2[e] = {a, g.J} ; per-gdir == true: val4[e] = {a, Jx, Jy, Jz}; all scaled by sc_i sc_j
 void launch_prep_values(int64_t n, const int *rowptr, const int *col, const double *sc, const double *m,
-                        const double *k, const double *r, const double *gi, const double *jx, const double *jy,
-                        const double *jz, bool per_gdir, double gx, double gy, double gz, double dt, double theta,
-                        double *vals, double *bre, hipStream_t s);
+                        const double *k, const double *r, const double *gi, const double *pv, const double *jx,
+                        const double *jy, const double *jz, bool per_gdir, double gx, double gy, double gz, double dt,
+                        double theta, double *vals, double *bre, hipStream_t s);
 void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s);
 
 struct SolveArgs {
@@ -125,6 +161,7 @@ struct SolveArgs {
     const double *coefA;   // [S*3] implicit-side imaginary coefficients of this step
     const double *coefB;   // [S*3] explicit-side coefficients of this step
     double2 *x, *r, *rh, *p, *v, *t, *b, *xprev;
+    const double2 *bper;            // weak pseudo-periodic right-hand side of this step, or null
     double *partU, *partV, *partT;  // [grid][S][kNV] each
     double *totU, *totV, *totT;     // [S][kNV] finalize outputs, or null: consumers reduce the partials
     KrylovState *st;

@@ -640,39 +640,248 @@ __global__ void diag_weights_kernel(int64_t n, const int *rowptr, const int *col
 }
 
 // --------------------------------------------------------------------------------------
+// Weak pseudo-periodic boundary condition (set-up).
+// --------------------------------------------------------------------------------------
+__device__ inline bool near_periodic_face(const double *X, const PeriodicBox &B, double eps) {
+    bool on = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (B.dir[d] && (X[d] < B.lo[d] + eps || X[d] > B.hi[d] - eps)) on = true;
+    return on;
+}
+
+// One thread per DOF row: P_ij += kappa_s |F|/60 sum_k chi_k (1+d_ij)(1+d_ik+d_jk) over the boundary
+// facets F of the row's vertex whose cell lies in the row's compartment (the row owns its entries,
+// so no colouring is needed; facets are visited in the sorted incidence order).  rowflag[i] = 1
+// when the row received a contribution.
+__global__ void periodic_mass_kernel(int64_t n, int64_t nv, const int *dof_vert, const int *dof_comp,
+                                     const int *bptr, const int *blist, const int *bnd_v, const int *bnd_c,
+                                     const int *phase, const double *xyz, const int *dof_of, const int *rowptr,
+                                     const int *col, PeriodicBox B, double *pv, int *rowflag, int *miss) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = dof_vert[i], c = dof_comp[i];
+    const double eps = 1e-2 * B.hmin, ks = 3e-3 / B.hmin;
+    int touched = 0;
+    for (int q = bptr[v]; q < bptr[v + 1]; ++q) {
+        const int f = blist[q];
+        if ((phase ? phase[bnd_c[f]] : 0) != c) continue;
+        int vv[3], a = -1;
+        double X[3][3], chi[3], cs = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            vv[k] = bnd_v[3 * (int64_t)f + k];
+            if (vv[k] == v) a = k;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) X[k][d] = xyz[3 * (int64_t)vv[k] + d];
+            chi[k] = near_periodic_face(X[k], B, eps) ? 1.0 : 0.0;
+            cs += chi[k];
+        }
+        if (cs == 0.0 || a < 0) continue;
+        double e1[3], e2[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            e1[d] = X[1][d] - X[0][d];
+            e2[d] = X[2][d] - X[0][d];
+        }
+        const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                     cz = e1[0] * e2[1] - e1[1] * e2[0];
+        const double area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+        for (int b = 0; b < 3; ++b) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                sum += chi[k] * (a == b ? 2.0 : 1.0) * (1.0 + (a == k ? 1.0 : 0.0) + (b == k ? 1.0 : 0.0));
+            const int j = dof_of[(int64_t)c * nv + vv[b]];
+            const int p = j < 0 ? -1 : csr_find(rowptr, col, (int)i, j);
+            if (p < 0) {
+                atomicOr(miss, 1);
+                continue;
+            }
+            pv[p] += ks * area / 60.0 * sum;
+            touched = 1;
+        }
+    }
+    rowflag[i] = touched;
+}
+
+// DOFs whose vertex lies within 1e-7 of a periodic face take a mirrored value (the tolerance of
+// WeakPseudoPeriodic eval): flag[i] = 1.
+__global__ void periodic_flag_kernel(int64_t n, const int *dof_vert, const double *xyz, PeriodicBox B, int *flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *X = xyz + 3 * (int64_t)dof_vert[i];
+    int hit = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (B.dir[d] && (fabs(X[d] - B.lo[d]) <= 1e-7 || fabs(X[d] - B.hi[d]) <= 1e-7)) hit = 1;
+    flag[i] = hit;
+}
+
+// Plane of a boundary facet: 2d (all vertices within eps of lo[d]) or 2d+1 (of hi[d]) for a
+// periodic axis d, else -1; facets bin by centroid on their plane's grid.
+__global__ void bfacet_bin_kernel(int64_t nbf, const int *bnd_v, const double *xyz, PeriodicBox B, PlaneGrids G,
+                                  int *fbin, int *cnt) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nbf) return;
+    double X[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) X[k][d] = xyz[3 * (int64_t)bnd_v[3 * f + k] + d];
+    const double eps = 1e-2 * B.hmin;
+    int plane = -1;
+    for (int d = 0; d < 3 && plane < 0; ++d) {
+        if (!B.dir[d]) continue;
+        for (int side = 0; side < 2 && plane < 0; ++side) {
+            const double bound = side ? B.hi[d] : B.lo[d];
+            if (fabs(X[0][d] - bound) <= eps && fabs(X[1][d] - bound) <= eps && fabs(X[2][d] - bound) <= eps)
+                plane = 2 * d + side;
+        }
+    }
+    if (plane < 0) {
+        fbin[f] = -1;
+        return;
+    }
+    const PlaneGrid &g = G.p[plane];
+    const double cu = (X[0][g.u] + X[1][g.u] + X[2][g.u]) / 3.0, cw = (X[0][g.w] + X[1][g.w] + X[2][g.w]) / 3.0;
+    const int iu = min(max((int)floor((cu - g.lo_u) / g.h_u), 0), g.nb_u - 1);
+    const int iw = min(max((int)floor((cw - g.lo_w) / g.h_w), 0), g.nb_w - 1);
+    const int bin = g.off + iu * g.nb_w + iw;
+    fbin[f] = bin;
+    atomicAdd(cnt + bin, 1);
+}
+
+__global__ void bfacet_fill_kernel(int64_t nbf, const int *fbin, const int *binptr, int *cursor, int *binlist) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nbf || fbin[f] < 0) return;
+    const int b = fbin[f];
+    binlist[binptr[b] + atomicAdd(cursor + b, 1)] = (int)f;
+}
+
+// Mirror entries, one thread per flagged DOF: target face by the eval's order (a later periodic
+// axis overrides an earlier one), l = mirror - x, and the facet of the target face whose 2-D
+// barycentric coordinates of the mirror point have the largest minimum (ties: smaller facet
+// index) among the facets binned within one bin of it.  bad[0] = 1 when none is found.
+__global__ void periodic_entry_kernel(int64_t n, int64_t nv, const int *flag, const int *scan, const int *dof_vert,
+                                      const int *dof_comp, const int *dof_of, const double *xyz, const int *bnd_v,
+                                      const int *binptr, const int *binlist, PeriodicBox B, PlaneGrids G,
+                                      int *ent_dof, int *ent_src, double *ent_w, double *ent_ell, int *bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    const int e = scan[i];
+    const int v = dof_vert[i], c = dof_comp[i];
+    double X[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) X[d] = xyz[3 * (int64_t)v + d];
+    int plane = -1;
+    double ell[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < 3; ++d) {
+        if (!B.dir[d]) continue;
+        if (fabs(X[d] - B.lo[d]) <= 1e-7) {
+            plane = 2 * d + 1;
+            ell[0] = ell[1] = ell[2] = 0.0;
+            ell[d] = B.hi[d] - X[d];
+        }
+        if (fabs(X[d] - B.hi[d]) <= 1e-7) {
+            plane = 2 * d;
+            ell[0] = ell[1] = ell[2] = 0.0;
+            ell[d] = B.lo[d] - X[d];
+        }
+    }
+    ent_dof[e] = (int)i;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ent_ell[3 * (int64_t)e + d] = ell[d];
+    const PlaneGrid &g = G.p[plane];
+    const double pu = X[g.u], pw = X[g.w];
+    const int iu = min(max((int)floor((pu - g.lo_u) / g.h_u), 0), g.nb_u - 1);
+    const int iw = min(max((int)floor((pw - g.lo_w) / g.h_w), 0), g.nb_w - 1);
+    double best = -1e300, bl[3] = {0.0, 0.0, 0.0};
+    int bf = -1;
+    for (int du = -1; du <= 1; ++du)
+        for (int dw = -1; dw <= 1; ++dw) {
+            const int ju = iu + du, jw = iw + dw;
+            if (ju < 0 || ju >= g.nb_u || jw < 0 || jw >= g.nb_w) continue;
+            const int b = g.off + ju * g.nb_w + jw;
+            for (int q = binptr[b]; q < binptr[b + 1]; ++q) {
+                const int f = binlist[q];
+                double A[2], Bv[2], C[2];
+                const double *xa = xyz + 3 * (int64_t)bnd_v[3 * (int64_t)f];
+                const double *xb = xyz + 3 * (int64_t)bnd_v[3 * (int64_t)f + 1];
+                const double *xc = xyz + 3 * (int64_t)bnd_v[3 * (int64_t)f + 2];
+                A[0] = xa[g.u]; A[1] = xa[g.w];
+                Bv[0] = xb[g.u]; Bv[1] = xb[g.w];
+                C[0] = xc[g.u]; C[1] = xc[g.w];
+                const double det = (Bv[0] - A[0]) * (C[1] - A[1]) - (C[0] - A[0]) * (Bv[1] - A[1]);
+                if (det == 0.0) continue;
+                const double l1 = ((pu - A[0]) * (C[1] - A[1]) - (C[0] - A[0]) * (pw - A[1])) / det;
+                const double l2 = ((Bv[0] - A[0]) * (pw - A[1]) - (pu - A[0]) * (Bv[1] - A[1])) / det;
+                const double l0 = 1.0 - l1 - l2;
+                const double m = fmin(l0, fmin(l1, l2));
+                if (m > best || (m == best && f < bf)) {
+                    best = m;
+                    bf = f;
+                    bl[0] = l0;
+                    bl[1] = l1;
+                    bl[2] = l2;
+                }
+            }
+        }
+    if (bf < 0) {
+        atomicOr(bad, 1);
+        bf = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ent_src[3 * (int64_t)e + k] = dof_of[(int64_t)c * nv + bnd_v[3 * (int64_t)bf + k]];
+        ent_w[3 * (int64_t)e + k] = bl[k];
+    }
+}
+
+__global__ void compact_rows_kernel(int64_t n, const int *flag, const int *scan, int *rows) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) rows[scan[i]] = (int)i;
+}
+
+// --------------------------------------------------------------------------------------
 // Solve-time preparation.  Jacobi preconditioning is applied as the symmetric scaling
 // S = diag(a)^(-1/2) of the whole system (a = M/dt + theta*L): the solver works on
 // S A S, whose real diagonal is 1, with x = S^-1 u.  (PETSc's Jacobi on the reference's
 // real 2x2-block operator uses the same diagonal: the J coupling is off the block diagonal.)
 // --------------------------------------------------------------------------------------
+// pv (weak pseudo-periodic boundary mass, may be null) enters the implicit side only.
 __global__ void prep_scale_kernel(int64_t n, const int *diag, const double *m, const double *k, const double *r,
-                                  const double *gi, double dt, double theta, double *sc) {
+                                  const double *gi, const double *pv, double dt, double theta, double *sc) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int p = diag[i];
-    double a = m[p] / dt + theta * (k[p] + r[p] + gi[p]);
+    double a = m[p] / dt + theta * (k[p] + r[p] + gi[p] + (pv ? pv[p] : 0.0));
     sc[i] = 1.0 / sqrt(a);
 }
 
+// kPrepLanes lanes per row walk the row's entries, so the value arrays are read in runs.
+constexpr int kPrepLanes = 8;
 __global__ void prep_values_kernel(int64_t n, const int *rowptr, const int *col, const double *sc, const double *m,
-                                   const double *k, const double *r, const double *gi, const double *jx,
-                                   const double *jy, const double *jz, int per_gdir, double gx, double gy,
-                                   double gz, double dt, double theta, double *vals, double *bre) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                   const double *k, const double *r, const double *gi, const double *pv,
+                                   const double *jx, const double *jy, const double *jz, int per_gdir, double gx,
+                                   double gy, double gz, double dt, double theta, double *vals, double *bre) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = t / kPrepLanes;
     if (i >= n) return;
     const double si = sc[i];
-    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+    for (int e = rowptr[i] + (int)(t % kPrepLanes); e < rowptr[i + 1]; e += kPrepLanes) {
         const double s = si * sc[col[e]];
         const double l = k[e] + r[e] + gi[e];
         const double mk = m[e] / dt;
         bre[e] = s * (mk - (1.0 - theta) * l);
+        const double impl = s * (mk + theta * (l + (pv ? pv[e] : 0.0)));
         if (per_gdir) {
-            vals[4 * (int64_t)e + 0] = s * (mk + theta * l);
+            vals[4 * (int64_t)e + 0] = impl;
             vals[4 * (int64_t)e + 1] = s * jx[e];
             vals[4 * (int64_t)e + 2] = s * jy[e];
             vals[4 * (int64_t)e + 3] = s * jz[e];
         } else {
-            vals[2 * (int64_t)e + 0] = s * (mk + theta * l);
+            vals[2 * (int64_t)e + 0] = impl;
             vals[2 * (int64_t)e + 1] = s * (gx * jx[e] + gy * jy[e] + gz * jz[e]);
         }
     }
@@ -973,6 +1182,11 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
                 Acc<PG> bacc = mode == 0 ? acc : jacc;
                 bacc.a = sb;
                 b = imag_coef_apply<G, PG>(bacc, a.coefB, s);
+                if (a.bper) {
+                    const double2 bp = a.bper[idx];
+                    b.x += bp.x;
+                    b.y += bp.y;
+                }
                 a.b[idx] = b;
             }
             if (mode == 2) {
@@ -1408,6 +1622,51 @@ __global__ void adj_init_lambda_kernel(int64_t n, const double *w, const double 
 }
 
 // per-signal partials of sum_i w_i sc_i x[i*S+s]
+// Weak pseudo-periodic terms of one step (ThetaMethodL_wBC1c/2c, DmriFemLib.py:70-75,114-120):
+//   ybc[dof_e] = exp(i q_s (g_s.l_e) F) sum_k w_ek u^n[src_ek]   (u = sc x, unscaled)
+// h[3s..3s+2] = q_s F g_s of this step.  One thread per (entry, signal).
+__global__ void periodic_mirror_kernel(int64_t ne, int S, const int *ent_dof, const int *ent_src, const double *ent_w,
+                                       const double *ent_ell, const double *sc, const double2 *x, const double *h,
+                                       double2 *ybc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ne * S) return;
+    const int64_t e = t / S;
+    const int s = (int)(t % S);
+    double yr = 0.0, yi = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int j = ent_src[3 * e + k];
+        if (j < 0) continue;
+        const double wk = ent_w[3 * e + k] * sc[j];
+        const double2 xv = x[(int64_t)j * S + s];
+        yr += wk * xv.x;
+        yi += wk * xv.y;
+    }
+    const double ph = h[3 * s] * ent_ell[3 * e] + h[3 * s + 1] * ent_ell[3 * e + 1] + h[3 * s + 2] * ent_ell[3 * e + 2];
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    ybc[(int64_t)ent_dof[e] * S + s] = make_double2(yr * cs - yi * sn, yr * sn + yi * cs);
+}
+
+// bper[i] = (1-theta) sc_i sum_j P_ij ybc_j on the rows that hold P entries (scaled right-hand side).
+__global__ void periodic_rhs_kernel(int64_t nrows, int S, const int *rows, const int *rowptr, const int *col,
+                                    const double *pv, const double *sc, const double2 *ybc, double omt, double2 *bper) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows * S) return;
+    const int i = rows[t / S];
+    const int s = (int)(t % S);
+    double ar = 0.0, ai = 0.0;
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+        const double pvv = pv[e];
+        if (pvv == 0.0) continue;
+        const double2 y = ybc[(int64_t)col[e] * S + s];
+        ar += pvv * y.x;
+        ai += pvv * y.y;
+    }
+    const double f = omt * sc[i];
+    bper[(int64_t)i * S + s] = make_double2(f * ar, f * ai);
+}
+
 __global__ void __launch_bounds__(kThreads) signal_part_kernel(int64_t n, int S, int vrpb, const double *w,
                                                                const double *sc, const double2 *x, double *parts) {
     const int64_t total = n * S;
@@ -1608,17 +1867,70 @@ void launch_sum(const double *x, int64_t n, double *part, double *out, hipStream
 }
 
 void launch_prep_scale(int64_t n, const int *diag, const double *m, const double *k, const double *r,
-                       const double *gi, double dt, double theta, double *sc, hipStream_t s) {
-    prep_scale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, diag, m, k, r, gi, dt, theta, sc);
+                       const double *gi, const double *pv, double dt, double theta, double *sc, hipStream_t s) {
+    prep_scale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, diag, m, k, r, gi, pv, dt, theta, sc);
     check_launch();
 }
 void launch_prep_values(int64_t n, const int *rowptr, const int *col, const double *sc, const double *m,
-                        const double *k, const double *r, const double *gi, const double *jx, const double *jy,
-                        const double *jz, bool per_gdir, double gx, double gy, double gz, double dt, double theta,
-                        double *vals, double *bre, hipStream_t s) {
-    prep_values_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rowptr, col, sc, m, k, r, gi, jx, jy, jz, per_gdir ? 1 : 0,
-                                                         gx, gy, gz, dt, theta, vals, bre);
+                        const double *k, const double *r, const double *gi, const double *pv, const double *jx,
+                        const double *jy, const double *jz, bool per_gdir, double gx, double gy, double gz, double dt,
+                        double theta, double *vals, double *bre, hipStream_t s) {
+    prep_values_kernel<<<grid_for(n * kPrepLanes, 256), 256, 0, s>>>(n, rowptr, col, sc, m, k, r, gi, pv, jx, jy, jz,
+                                                                      per_gdir ? 1 : 0, gx, gy, gz, dt, theta, vals,
+                                                                      bre);
     check_launch();
+}
+void launch_periodic_mass(int64_t n, int64_t nv, const int *dof_vert, const int *dof_comp, const int *bptr,
+                          const int *blist, const int *bnd_v, const int *bnd_c, const int *phase, const double *xyz,
+                          const int *dof_of, const int *rowptr, const int *col, const PeriodicBox &B, double *pv,
+                          int *rowflag, int *miss, hipStream_t s) {
+    periodic_mass_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, nv, dof_vert, dof_comp, bptr, blist, bnd_v, bnd_c, phase,
+                                                           xyz, dof_of, rowptr, col, B, pv, rowflag, miss);
+    check_launch();
+}
+void launch_periodic_flag(int64_t n, const int *dof_vert, const double *xyz, const PeriodicBox &B, int *flag,
+                          hipStream_t s) {
+    periodic_flag_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, dof_vert, xyz, B, flag);
+    check_launch();
+}
+void launch_bfacet_bin(int64_t nbf, const int *bnd_v, const double *xyz, const PeriodicBox &B, const PlaneGrids &G,
+                       int *fbin, int *cnt, hipStream_t s) {
+    if (nbf <= 0) return;
+    bfacet_bin_kernel<<<grid_for(nbf, 256), 256, 0, s>>>(nbf, bnd_v, xyz, B, G, fbin, cnt);
+    check_launch();
+}
+void launch_bfacet_fill(int64_t nbf, const int *fbin, const int *binptr, int *cursor, int *binlist, hipStream_t s) {
+    if (nbf <= 0) return;
+    bfacet_fill_kernel<<<grid_for(nbf, 256), 256, 0, s>>>(nbf, fbin, binptr, cursor, binlist);
+    check_launch();
+}
+void launch_periodic_entries(int64_t n, int64_t nv, const int *flag, const int *scan, const int *dof_vert,
+                             const int *dof_comp, const int *dof_of, const double *xyz, const int *bnd_v,
+                             const int *binptr, const int *binlist, const PeriodicBox &B, const PlaneGrids &G,
+                             int *ent_dof, int *ent_src, double *ent_w, double *ent_ell, int *bad, hipStream_t s) {
+    periodic_entry_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, nv, flag, scan, dof_vert, dof_comp, dof_of, xyz, bnd_v,
+                                                            binptr, binlist, B, G, ent_dof, ent_src, ent_w, ent_ell,
+                                                            bad);
+    check_launch();
+}
+void launch_compact_rows(int64_t n, const int *flag, const int *scan, int *rows, hipStream_t s) {
+    compact_rows_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, flag, scan, rows);
+    check_launch();
+}
+void launch_periodic_step(int64_t ne, int64_t nrows, int S, const int *ent_dof, const int *ent_src,
+                          const double *ent_w, const double *ent_ell, const int *rows, const int *rowptr,
+                          const int *col, const double *pv, const double *sc, const double2 *x, const double *h,
+                          double omt, double2 *ybc, double2 *bper, hipStream_t s) {
+    if (ne > 0) {
+        periodic_mirror_kernel<<<grid_for(ne * S, 256), 256, 0, s>>>(ne, S, ent_dof, ent_src, ent_w, ent_ell, sc, x,
+                                                                      h, ybc);
+        check_launch();
+    }
+    if (nrows > 0) {
+        periodic_rhs_kernel<<<grid_for(nrows * S, 256), 256, 0, s>>>(nrows, S, rows, rowptr, col, pv, sc, ybc, omt,
+                                                                      bper);
+        check_launch();
+    }
 }
 void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s) {
     init_x_kernel<<<grid_for(n * S, 256), 256, 0, s>>>(n, S, sc, x);

@@ -13,7 +13,7 @@ import numpy as np
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DMRI_LIB", os.path.join(_PKG, "lib", "libdmri.so"))
 
-DMRI_MAT = {"m": 0, "k": 1, "r": 2, "gi": 3, "jx": 4, "jy": 5, "jz": 6}
+DMRI_MAT = {"m": 0, "k": 1, "r": 2, "gi": 3, "jx": 4, "jy": 5, "jz": 6, "p": 7}
 ERRORS = {-1: "ARG", -2: "HIP", -3: "STATE", -4: "NOCONV", -5: "MESH", -6: "ALLOC", -7: "LIMIT"}
 
 
@@ -33,13 +33,15 @@ class _Info(ctypes.Structure):
     _fields_ = [("nv", ctypes.c_int64), ("nc", ctypes.c_int64), ("ncomp", ctypes.c_int64), ("ndof", ctypes.c_int64),
                 ("nnz", ctypes.c_int64), ("n_iface", ctypes.c_int64), ("n_bfacet", ctypes.c_int64),
                 ("ncolors_cells", ctypes.c_int32), ("ncolors_iface", ctypes.c_int32), ("volume", ctypes.c_double),
-                ("hmin", ctypes.c_double), ("hmax", ctypes.c_double), ("setup_ms", ctypes.c_double)]
+                ("hmin", ctypes.c_double), ("hmax", ctypes.c_double), ("setup_ms", ctypes.c_double),
+                ("n_periodic", ctypes.c_int64), ("kappa_e", ctypes.c_double)]
 
 
 class _Seq(ctypes.Structure):
     _fields_ = [("nsteps", ctypes.c_int32), ("f_implicit", ctypes.POINTER(ctypes.c_double)),
                 ("f_explicit", ctypes.POINTER(ctypes.c_double)), ("dt", ctypes.c_double),
-                ("theta", ctypes.c_double), ("q", ctypes.c_double), ("gdir", ctypes.c_double * 3)]
+                ("theta", ctypes.c_double), ("q", ctypes.c_double), ("gdir", ctypes.c_double * 3),
+                ("F_explicit", ctypes.POINTER(ctypes.c_double))]
 
 
 class _Opts(ctypes.Structure):
@@ -68,6 +70,7 @@ class _Grads(ctypes.Structure):
 
 EXPORTED = ["dmri_create", "dmri_destroy", "dmri_last_error", "dmri_version", "dmri_mesh_set", "dmri_coeffs_set",
             "dmri_assemble", "dmri_get_info", "dmri_export_pattern", "dmri_export_values", "dmri_export_facets",
+            "dmri_periodic_set", "dmri_export_periodic",
             "dmri_solve", "dmri_solve_batched", "dmri_tune", "dmri_set_tuning", "dmri_time_spmv", "dmri_copy_bandwidth",
             "dmri_adjoint", "dmri_get_solution"]
 
@@ -90,6 +93,8 @@ def _load():
     L.dmri_export_pattern.argtypes = [vp, P(ctypes.c_int64), i32p, i32p]
     L.dmri_export_values.argtypes = [vp, ctypes.c_int, dp]
     L.dmri_export_facets.argtypes = [vp, i32p]
+    L.dmri_periodic_set.argtypes = [vp, i32p]
+    L.dmri_export_periodic.argtypes = [vp, i32p, i32p, dp, dp]
     L.dmri_solve.argtypes = [vp, P(_Seq), P(_Opts), dp, P(_Stats)]
     L.dmri_solve_batched.argtypes = [vp, ctypes.c_int32, dp, dp, P(_Seq), P(_Opts), dp, P(_Stats)]
     L.dmri_get_solution.argtypes = [vp, ctypes.c_int32, dp]
@@ -196,18 +201,38 @@ class Context:
         _check(LIB.dmri_export_facets(self._h, _ptr(out, ctypes.c_int32)))
         return out[:n]
 
+    def set_periodic(self, dirs):
+        """Weak pseudo-periodic faces normal to the axes with dirs[d] != 0 (next assemble)."""
+        d = np.ascontiguousarray([1 if x else 0 for x in dirs], dtype=np.int32)
+        _check(LIB.dmri_periodic_set(self._h, _ptr(d, ctypes.c_int32)))
+
+    def export_periodic(self):
+        """Mirror entries of the last assemble: dict(dof, src (ne,3), w (ne,3), ell (ne,3))."""
+        ne = self.info()["n_periodic"]
+        dof = np.zeros(max(ne, 1), np.int32)
+        src = np.zeros((max(ne, 1), 3), np.int32)
+        w = np.zeros((max(ne, 1), 3))
+        ell = np.zeros((max(ne, 1), 3))
+        _check(LIB.dmri_export_periodic(self._h, _ptr(dof, ctypes.c_int32), _ptr(src, ctypes.c_int32),
+                                        _ptr(w, ctypes.c_double), _ptr(ell, ctypes.c_double)))
+        return {"dof": dof[:ne], "src": src[:ne], "w": w[:ne], "ell": ell[:ne]}
+
     # -- solve --------------------------------------------------------------------------
     @staticmethod
     def _seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes, guess=0,
-                  mapping=0):
+                  mapping=0, F_explicit=None):
         fA = np.ascontiguousarray(f_implicit, dtype=np.float64)
         fB = np.ascontiguousarray(f_explicit, dtype=np.float64)
         if fA.shape != fB.shape:
             raise ValueError("f_implicit and f_explicit differ in length")
+        FB = None if F_explicit is None else np.ascontiguousarray(F_explicit, dtype=np.float64)
+        if FB is not None and FB.shape != fA.shape:
+            raise ValueError("F_explicit and the time profile differ in length")
         s = _Seq()
         s.nsteps = fA.shape[0]
         s.f_implicit = _ptr(fA, ctypes.c_double)
         s.f_explicit = _ptr(fB, ctypes.c_double)
+        s.F_explicit = _ptr(FB, ctypes.c_double) if FB is not None else None
         s.dt, s.theta, s.q = float(dt), float(theta), float(q)
         for i in range(3):
             s.gdir[i] = float(gdir[i])
@@ -216,12 +241,12 @@ class Context:
         o.lanes_per_row = int(lanes)
         o.initial_guess = int(guess)
         o.mapping = int(mapping)
-        return s, o, (fA, fB)
+        return s, o, (fA, fB, FB)
 
     def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-              profile=False, lanes=0, guess=0, mapping=0):
+              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None):
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes,
-                                    guess, mapping)
+                                    guess, mapping, F_explicit)
         sig = np.zeros(2)
         st = _Stats()
         _check(LIB.dmri_solve(self._h, ctypes.byref(s), ctypes.byref(o), _ptr(sig, ctypes.c_double),
@@ -229,13 +254,13 @@ class Context:
         return complex(sig[0], sig[1]), {k: getattr(st, k) for k, _ in _Stats._fields_}
 
     def solve_batched(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-                      profile=False, lanes=0, guess=0, mapping=0):
+                      profile=False, lanes=0, guess=0, mapping=0, F_explicit=None):
         """Signals for every (q[s], gdir[s]) on the shared pattern; returns complex array (nsig,)."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(gdirs, dtype=np.float64).reshape(-1, 3),
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
-                                    lanes, guess, mapping)
+                                    lanes, guess, mapping, F_explicit)
         sig = np.zeros(2 * qs.shape[0])
         st = _Stats()
         _check(LIB.dmri_solve_batched(self._h, qs.shape[0], _ptr(qs, ctypes.c_double), _ptr(g, ctypes.c_double),

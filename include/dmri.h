@@ -42,7 +42,10 @@ enum dmri_matrix {
     DMRI_MAT_I = 3,  /* interface kappa int_Gamma (u0-u1)(v0-v1)    (icondition_wBC, DmriFemLib.py:47-50) */
     DMRI_MAT_JX = 4, /* int x phi_i phi_j  (GdotX, DmriFemLib.py:33-39; Jg = gx Jx + gy Jy + gz Jz) */
     DMRI_MAT_JY = 5,
-    DMRI_MAT_JZ = 6
+    DMRI_MAT_JZ = 6,
+    DMRI_MAT_P = 7   /* weak pseudo-periodic boundary mass  int_{dOmega} kappa_e phi_i phi_j per compartment,
+                        kappa_e = (3e-3/hmin) * P1 indicator of the periodic faces (ThetaMethodF_wBC1c/2c,
+                        DmriFemLib.py:92,143; WeakPseudoPeridicMarker 601-610); zero without periodic dirs */
 };
 
 /* Create a context on HIP device `device`.
@@ -95,6 +98,8 @@ typedef struct {
     double volume;        /* sum of M (|Omega|; whole_vol of PostProcessing, DmriFemLib.py:923) */
     double hmin, hmax;    /* min/max greatest vertex distance of a cell (mymesh.hmin/hmax) */
     double setup_ms;      /* device time of dmri_assemble */
+    int64_t n_periodic;   /* DOFs on a periodic face that take a mirrored value (dmri_export_periodic) */
+    double kappa_e;       /* 3e-3/hmin when periodic directions are set (MyDomain.kappa_e_scalar), else 0 */
 } dmri_info;
 int dmri_get_info(dmri_ctx *ctx, dmri_info *info);
 
@@ -103,6 +108,27 @@ int dmri_get_info(dmri_ctx *ctx, dmri_info *info);
 int dmri_export_pattern(dmri_ctx *ctx, int64_t *rowptr, int32_t *col, int32_t *dof_of);
 int dmri_export_values(dmri_ctx *ctx, int which, double *val);
 int dmri_export_facets(dmri_ctx *ctx, int32_t *fv);
+
+/* Weak pseudo-periodic boundary condition (MyDomain.PeriodicDir with IsDomainPeriodic = False:
+ * WeakPseudoPeridicMarker, DmriFemLib.py:601-610; ThetaMethodF/L_wBC1c/2c 58-145;
+ * WeakPseudoPeriodic_1c/2c 256-324,386-450).  dir[d] != 0 makes the two faces of the mesh's
+ * bounding box normal to axis d periodic.  Takes effect at the next dmri_assemble, which then adds
+ *   implicit side:  theta * P,              P_ij = int_{dOmega} kappa_e phi_i phi_j  (per compartment)
+ *   explicit side:  (1-theta) * P * u_bc,   u_bc(x) = u^n(x + l) exp(i q (g.l) F(t_{n-1}))
+ * where kappa_e = (3e-3/hmin) * the P1 interpolant of the indicator "within 1e-2 hmin of a periodic
+ * face", and for a vertex x within 1e-7 of a periodic face, x + l is its mirror on the opposite face
+ * (l along that axis; where several periodic faces meet the last axis in x, y, z order wins, as in
+ * the reference's eval); u_bc = 0 at every other vertex.  u^n at the mirror point is the P1
+ * interpolant on the boundary facet of the opposite face that contains it.  Each periodic
+ * direction needs flat opposite faces.  The time profile's integral F must be given
+ * (dmri_seq.F_explicit).  dir = {0,0,0} switches the condition off.  The adjoint does not support
+ * it (DMRI_ERR_ARG). */
+int dmri_periodic_set(dmri_ctx *ctx, const int32_t dir[3]);
+
+/* Parity hook: the n_periodic mirrored DOFs of the last dmri_assemble.  dof[e] is the DOF that takes
+ * the mirrored value; src[3e..3e+2] the DOFs (same compartment, -1 = dead) of the facet vertices
+ * whose P1 interpolant gives u^n at the mirror point, with weights w[3e..3e+2]; ell[3e..3e+2] = l. */
+int dmri_export_periodic(dmri_ctx *ctx, int32_t *dof, int32_t *src, double *w, double *ell);
 
 /* Time sequence for one signal.
  * f_implicit[n] multiplies J on the implicit side of step n, f_explicit[n] on the explicit
@@ -116,6 +142,10 @@ typedef struct {
     double theta;     /* 0.5 (MRI_simulation.theta, DmriFemLib.py:863) */
     double q;         /* gamma*G (MRI_parameters.qvalue; comri gnorm) */
     double gdir[3];   /* gradient direction (normalised inside) */
+    const double *F_explicit; /* nsteps values of F(t) = int_0^t f on the explicit side's time
+                                 (GCloud: t_{n-1}, MRI_parameters.itime_profile, DmriFemLib.py:834-835,
+                                 901-902): the phase of the pseudo-periodic mirror values; required
+                                 when periodic directions are set, ignored otherwise (may be NULL) */
 } dmri_seq;
 
 typedef struct {

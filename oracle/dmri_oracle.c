@@ -18,6 +18,10 @@
  *     DmriFemLib.py:41-50,79-93,125-145,240-254 and comri/one-comp/hpc-fenics-cpp/ufc/
  *     Bloch_Torrey_NoTime3D.ufl): mass, stiffness with scalar or DG0 tensor D, T2
  *     relaxation, gradient moments Jx/Jy/Jz, interface facet mass on phase jumps.
+ *   - Weak pseudo-periodic boundary condition (WeakPseudoPeridicMarker, DmriFemLib.py:601-610;
+ *     ThetaMethodF/L_wBC1c/2c 58-145; WeakPseudoPeriodic_1c/2c eval 256-324,386-450): the
+ *     boundary mass P with the P1-interpolated face indicator, and the mirror map of u^n with its
+ *     phase exp(i q (g.l) F(t_{n-1})), located by brute force over the opposite face's facets.
  *   - Theta time stepping (DmriFemLib.py:878-915): A_n u^{n+1} = B_n u^n with
  *     A_n = M/k + theta*L + i*theta*q*fA[n]*Jg, B_n = M/k - (1-theta)*L - i*(1-theta)*q*fB[n]*Jg,
  *     L = K + R + I, solved by BiCGStab with Jacobi (GCloudDmriSolver.py:219-222,
@@ -171,6 +175,120 @@ int64_t orc_boundary_facets(int64_t nc, const int32_t *tets, int32_t *fv, int32_
 }
 
 /* ------------------------------------------------------------------------- */
+/* Weak pseudo-periodic boundary condition.                                     */
+/*   P_ij += kappa_s * sum_k chi_k int_F phi_k phi_i phi_j over every boundary   */
+/*   facet F, in the compartment of F's cell (the (1-phase)*ds and phase*ds of   */
+/*   ThetaMethodF_wBC2c, DmriFemLib.py:143), kappa_s = 3e-3/hmin (MyDomain 599), */
+/*   chi_k = vertex k lies within eps = 1e-2*hmin of a periodic face (the degree-1 */
+/*   Expression of WeakPseudoPeridicMarker 601-610, interpolated at vertices),   */
+/*   int_F l_k l_i l_j = |F|/60 (1+d_ij)(1+d_ik+d_jk).                            */
+/*   Mirror entries: every live DOF whose vertex lies within 1e-7 of a periodic   */
+/*   face (eval's tol, 256-324) takes u^n at the point with that coordinate moved */
+/*   to the opposite face; later axes override earlier ones (the eval's order);   */
+/*   l = mirror - x.  u^n there is the P1 interpolant on the facet of the         */
+/*   opposite face with the largest smallest barycentric coordinate.              */
+/* Returns the number of entries (<= cap written), or -1 (allocation), -2 (a     */
+/* mirror point with no facet on its face).  pv (nnz) is accumulated into.       */
+/* ------------------------------------------------------------------------- */
+static double tri_area(const double *A, const double *B, const double *C) {
+    double e1[3], e2[3];
+    for (int q = 0; q < 3; ++q) { e1[q] = B[q] - A[q]; e2[q] = C[q] - A[q]; }
+    double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+    return 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+}
+
+static int64_t csr_find64(const int64_t *rowptr, const int32_t *col, int64_t r, int32_t c);
+
+int64_t orc_periodic(int64_t nv, const double *xyz, int64_t nc, const int32_t *tets, const int32_t *phase,
+                     const int32_t *dof_of, const int32_t *pdir, double hmin, const int64_t *rowptr,
+                     const int32_t *col, double *pv, int32_t *ent_dof, int32_t *ent_src, double *ent_w,
+                     double *ent_ell, int64_t cap) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t v = 0; v < nv; ++v)
+        for (int d = 0; d < 3; ++d) {
+            if (xyz[3 * v + d] < lo[d]) lo[d] = xyz[3 * v + d];
+            if (xyz[3 * v + d] > hi[d]) hi[d] = xyz[3 * v + d];
+        }
+    const double ks = 3e-3 / hmin, eps = 1e-2 * hmin, tol = 1e-7;
+    const int ncomp = phase ? 2 : 1;
+    int32_t *fv = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)(4 * nc));
+    int32_t *fc = (int32_t *)malloc(sizeof(int32_t) * (size_t)(4 * nc));
+    if (!fv || !fc) { free(fv); free(fc); return -1; }
+    int64_t nbf = orc_boundary_facets(nc, tets, fv, fc, 4 * nc);
+    /* boundary mass P */
+    for (int64_t f = 0; f < nbf; ++f) {
+        int c = phase ? phase[fc[f]] : 0;
+        int32_t dd[3];
+        double chi[3], ssum = 0;
+        for (int a = 0; a < 3; ++a) {
+            const double *X = xyz + 3 * (int64_t)fv[3 * f + a];
+            int on = 0;
+            for (int d = 0; d < 3; ++d)
+                if (pdir[d] && (X[d] < lo[d] + eps || X[d] > hi[d] - eps)) on = 1;
+            chi[a] = on;
+            ssum += on;
+            dd[a] = dof_of[(int64_t)c * nv + fv[3 * f + a]];
+        }
+        if (ssum == 0) continue;
+        double area = tri_area(xyz + 3 * (int64_t)fv[3 * f], xyz + 3 * (int64_t)fv[3 * f + 1],
+                               xyz + 3 * (int64_t)fv[3 * f + 2]);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                double s = 0;
+                for (int k = 0; k < 3; ++k) s += chi[k] * (1.0 + (a == b)) * (1.0 + (a == k) + (b == k));
+                pv[csr_find64(rowptr, col, dd[a], dd[b])] += ks * area / 60.0 * s;
+            }
+    }
+    /* mirror entries, in DOF order (compartment-major, increasing vertex) */
+    int64_t ne = 0;
+    int rc = 0;
+    for (int c = 0; c < ncomp && !rc; ++c)
+        for (int64_t v = 0; v < nv; ++v) {
+            int32_t i = dof_of[(int64_t)c * nv + v];
+            if (i < 0) continue;
+            const double *X = xyz + 3 * v;
+            int td = -1;
+            double tgt = 0;
+            for (int d = 0; d < 3; ++d) {
+                if (!pdir[d]) continue;
+                if (fabs(X[d] - lo[d]) <= tol) { td = d; tgt = hi[d]; }
+                if (fabs(X[d] - hi[d]) <= tol) { td = d; tgt = lo[d]; }
+            }
+            if (td < 0) continue;
+            const int u = (td + 1) % 3, w = (td + 2) % 3;
+            const double pu = X[u], pw = X[w];
+            double best = -1e300, bl[3] = {0, 0, 0};
+            int64_t bf = -1;
+            for (int64_t f = 0; f < nbf; ++f) {
+                const double *A = xyz + 3 * (int64_t)fv[3 * f], *B = xyz + 3 * (int64_t)fv[3 * f + 1],
+                             *C = xyz + 3 * (int64_t)fv[3 * f + 2];
+                if (fabs(A[td] - tgt) > eps || fabs(B[td] - tgt) > eps || fabs(C[td] - tgt) > eps) continue;
+                double det = (B[u] - A[u]) * (C[w] - A[w]) - (C[u] - A[u]) * (B[w] - A[w]);
+                if (det == 0) continue;
+                double l1 = ((pu - A[u]) * (C[w] - A[w]) - (C[u] - A[u]) * (pw - A[w])) / det;
+                double l2 = ((B[u] - A[u]) * (pw - A[w]) - (pu - A[u]) * (B[w] - A[w])) / det;
+                double l0 = 1.0 - l1 - l2;
+                double m = l0 < l1 ? l0 : l1;
+                m = m < l2 ? m : l2;
+                if (m > best) { best = m; bf = f; bl[0] = l0; bl[1] = l1; bl[2] = l2; }
+            }
+            if (bf < 0) { rc = -2; break; }
+            if (ne < cap) {
+                ent_dof[ne] = i;
+                for (int a = 0; a < 3; ++a) {
+                    ent_src[3 * ne + a] = dof_of[(int64_t)c * nv + fv[3 * bf + a]];
+                    ent_w[3 * ne + a] = bl[a];
+                    ent_ell[3 * ne + a] = a == td ? tgt - X[td] : 0.0;
+                }
+            }
+            ++ne;
+        }
+    free(fv);
+    free(fc);
+    return rc ? rc : ne;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Sparsity pattern: set union of the DOF cliques of every cell (4 DOFs in its  */
 /* own compartment) and every interface facet (3 vertices x 2 compartments).   */
 /* Keys row*ndof+col are radix-sorted and made unique.                          */
@@ -258,6 +376,10 @@ static double tet_geom(const double X[4][3], double G[4][3]) {
     return fabs(det) / 6.0;
 }
 
+static int64_t csr_find(const int64_t *rowptr, const int32_t *col, int64_t r, int32_t c);
+static int64_t csr_find64(const int64_t *rowptr, const int32_t *col, int64_t r, int32_t c) {
+    return csr_find(rowptr, col, r, c);
+}
 static int64_t csr_find(const int64_t *rowptr, const int32_t *col, int64_t r, int32_t c) {
     int64_t lo = rowptr[r], hi = rowptr[r + 1] - 1;
     while (lo <= hi) {
@@ -401,9 +523,14 @@ static double nrm2(int64_t n, const cplx *x) { return sqrt(creal(dotc(n, x, x)))
 /* stats[2] = max final relative residual, stats[3] = breakdown restarts.      */
 /* Returns 0, or -1 when a step did not converge within maxit.                 */
 /* ------------------------------------------------------------------------- */
-int orc_theta_solve(int64_t n, const int64_t *rowptr, const int32_t *col, const double *m, const double *l,
-                    const double *jg, double dt, double theta, double q, int nsteps, const double *fA,
-                    const double *fB, double rtol, double atol, int maxit, double *u_io, double *stats) {
+/* Theta time stepping with the weak pseudo-periodic terms: a = M/k + theta (L + P), and per step
+ * b += (1-theta) P ybc with ybc[ent_dof[e]] = exp(i q (g.ell_e) FB[n]) sum_a w u^n[src] (ThetaMethodL
+ * _wBC1c/2c, DmriFemLib.py:70-75,114-120).  pv == NULL: no periodic terms. */
+int orc_theta_solve_p(int64_t n, const int64_t *rowptr, const int32_t *col, const double *m, const double *l,
+                      const double *jg, double dt, double theta, double q, int nsteps, const double *fA,
+                      const double *fB, double rtol, double atol, int maxit, double *u_io, double *stats,
+                      const double *pv, int64_t ne, const int32_t *ent_dof, const int32_t *ent_src,
+                      const double *ent_w, const double *ent_ell, const double *g, const double *FB) {
     cplx *u = (cplx *)u_io;
     int64_t nnz = rowptr[n];
     double *a = (double *)malloc(sizeof(double) * (size_t)nnz);
@@ -418,9 +545,10 @@ int orc_theta_solve(int64_t n, const int64_t *rowptr, const int32_t *col, const 
     stats[0] = stats[1] = stats[2] = stats[3] = 0;
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < nnz; ++e) {
-        a[e] = m[e] / dt + theta * l[e];
+        a[e] = m[e] / dt + theta * (l[e] + (pv ? pv[e] : 0.0));
         bre[e] = m[e] / dt - (1.0 - theta) * l[e];
     }
+    cplx *ybc = pv ? (cplx *)calloc((size_t)n, sizeof(cplx)) : NULL;
     for (int64_t i = 0; i < n; ++i) {
         double d = 0;
         for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e)
@@ -431,6 +559,21 @@ int orc_theta_solve(int64_t n, const int64_t *rowptr, const int32_t *col, const 
         double sA = theta * q * fA[step];
         double sB = -(1.0 - theta) * q * fB[step];
         spmv_c(n, rowptr, col, bre, jg, sB, u, b);
+        if (pv) {
+            for (int64_t e = 0; e < ne; ++e) {
+                cplx y = 0;
+                for (int k = 0; k < 3; ++k)
+                    if (ent_src[3 * e + k] >= 0) y += ent_w[3 * e + k] * u[ent_src[3 * e + k]];
+                double ph = q * FB[step] * (g[0] * ent_ell[3 * e] + g[1] * ent_ell[3 * e + 1] + g[2] * ent_ell[3 * e + 2]);
+                ybc[ent_dof[e]] = y * (cos(ph) + I * sin(ph));
+            }
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) {
+                cplx acc = 0;
+                for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) acc += pv[e] * ybc[col[e]];
+                b[i] += (1.0 - theta) * acc;
+            }
+        }
         double bn = nrm2(n, b);
         double tol = rtol * bn > atol ? rtol * bn : atol;
         /* warm start x0 = u^n (the previous solution, DOLFIN reuses the vector) */
@@ -484,8 +627,15 @@ int orc_theta_solve(int64_t n, const int64_t *rowptr, const int32_t *col, const 
         if (rc) break;
     }
     free(a); free(bre); free(dinv); free(b); free(r); free(rh); free(p); free(v); free(s); free(t);
-    free(ph); free(sh);
+    free(ph); free(sh); free(ybc);
     return rc;
+}
+
+int orc_theta_solve(int64_t n, const int64_t *rowptr, const int32_t *col, const double *m, const double *l,
+                    const double *jg, double dt, double theta, double q, int nsteps, const double *fA,
+                    const double *fB, double rtol, double atol, int maxit, double *u_io, double *stats) {
+    return orc_theta_solve_p(n, rowptr, col, m, l, jg, dt, theta, q, nsteps, fA, fB, rtol, atol, maxit, u_io, stats,
+                             NULL, 0, NULL, NULL, NULL, NULL, NULL, NULL);
 }
 
 /* y = (va + i*s*vj) x, exported for tests (x, y interleaved complex). */

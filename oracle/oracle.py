@@ -57,6 +57,13 @@ def lib():
         L.orc_theta_solve.restype = ctypes.c_int
         L.orc_theta_solve.argtypes = [i64, i64p, i32p, dp, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, dp]
+        L.orc_theta_solve_p.restype = ctypes.c_int
+        L.orc_theta_solve_p.argtypes = [i64, i64p, i32p, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, dp, dp, dp, i64, i32p, i32p, dp, dp, dp, dp]
+        L.orc_periodic.restype = i64
+        L.orc_periodic.argtypes = [i64, dp, i64, i32p, i32p, i32p, i32p, ctypes.c_double, i64p, i32p, dp, i32p,
+                                   i32p, dp, dp, i64]
         L.orc_spmv.restype = None
         L.orc_spmv.argtypes = [i64, i64p, i32p, dp, dp, ctypes.c_double, dp, dp]
         L.orc_signal.restype = None
@@ -107,6 +114,23 @@ def pgse_comri(delta, Delta):
         return 1.0 * (0 <= t < delta) - 1.0 * (Delta <= t <= Delta + delta)
 
     return f
+
+
+def pgse_integral(delta, Delta):
+    """F(t) = int_0^t f for the Piecewise PGSE of GCloudDmriSolver.py:188-193 (closed form of
+    MRI_parameters.itime_profile_sym, DmriFemLib.py:829-832)."""
+    T = Delta + delta
+
+    def F(t):
+        if t < delta:
+            return float(t)
+        if t < Delta:
+            return float(delta)
+        if t < T:
+            return float(delta - (t - Delta))
+        return 0.0
+
+    return F
 
 
 def q_from_b_pgse(b, delta, Delta):
@@ -163,7 +187,7 @@ class System:
     D: scalar or per-compartment pair; invT2 likewise; kappa scalar (interface)."""
 
     def __init__(self, xyz, tets, phase=None, D=3e-3, invT2=1e-16, kappa=0.0, Dcell=None, invT2cell=None,
-                 kappa_facet=None):
+                 kappa_facet=None, periodic=None):
         L = lib()
         self.xyz = np.ascontiguousarray(xyz, dtype=np.float64)
         self.tets = np.ascontiguousarray(tets, dtype=np.int32)
@@ -215,6 +239,29 @@ class System:
                             *[_p(M[k], ctypes.c_double) for k in ("m", "k", "r", "gi", "jx", "jy", "jz")])
         if rc != 0:
             raise RuntimeError(f"oracle assembly failed ({rc})")
+        # weak pseudo-periodic boundary mass and mirror entries
+        self.periodic = None
+        if periodic is not None and any(periodic):
+            X = self.xyz[self.tets]                                      # nc x 4 x 3
+            ii, jj = np.triu_indices(4, 1)
+            self.hmin = float(np.linalg.norm(X[:, ii] - X[:, jj], axis=2).max(axis=1).min())
+            pdir = np.ascontiguousarray([1 if p else 0 for p in periodic], dtype=np.int32)
+            pv = np.zeros(nnz)
+            args = [nv, _p(self.xyz, ctypes.c_double), nc, _p(self.tets, ctypes.c_int32),
+                    _p(self.phase, ctypes.c_int32), _p(self.dof_of, ctypes.c_int32), _p(pdir, ctypes.c_int32),
+                    self.hmin, _p(self.rowptr, ctypes.c_int64), _p(self.col, ctypes.c_int32)]
+            ne = L.orc_periodic(*args, _p(np.zeros(nnz), ctypes.c_double), None, None, None, None, 0)
+            if ne < 0:
+                raise RuntimeError(f"oracle periodic set-up failed ({ne})")
+            ed = np.zeros(max(ne, 1), np.int32)
+            es = np.zeros((max(ne, 1), 3), np.int32)
+            ew = np.zeros((max(ne, 1), 3))
+            el = np.zeros((max(ne, 1), 3))
+            L.orc_periodic(*args, _p(pv, ctypes.c_double), _p(ed, ctypes.c_int32), _p(es, ctypes.c_int32),
+                           _p(ew, ctypes.c_double), _p(el, ctypes.c_double), ne)
+            self.mats["p"] = pv
+            self.periodic = {"dir": pdir, "dof": ed[:ne], "src": es[:ne], "w": ew[:ne], "ell": el[:ne],
+                             "kappa_e": 3e-3 / self.hmin}
         # signal weights w = M 1 (row sums of the mass matrix on live DOFs)
         self.w = np.add.reduceat(M["m"], self.rowptr[:-1]) if nnz else np.zeros(self.ndof)
         self.volume = float(self.w.sum())
@@ -240,7 +287,8 @@ class System:
         u[:] = 1.0
         return u
 
-    def solve(self, gdir, q, dt, fA, fB, theta=0.5, rtol=1e-13, atol=0.0, maxit=100000, u0=None, l=None):
+    def solve(self, gdir, q, dt, fA, fB, theta=0.5, rtol=1e-13, atol=0.0, maxit=100000, u0=None, l=None, FB=None):
+        """FB: F(t) = int_0^t f at the explicit side's time of every step (periodic systems only)."""
         L = lib()
         fA = np.ascontiguousarray(fA, dtype=np.float64)
         fB = np.ascontiguousarray(fB, dtype=np.float64)
@@ -249,11 +297,23 @@ class System:
         lv = np.ascontiguousarray(self.l() if l is None else l)
         jg = np.ascontiguousarray(self.jg(gdir))
         stats = np.zeros(4)
-        rc = L.orc_theta_solve(self.ndof, _p(self.rowptr, ctypes.c_int64), _p(self.col, ctypes.c_int32),
-                               _p(self.mats["m"], ctypes.c_double), _p(lv, ctypes.c_double), _p(jg, ctypes.c_double),
-                               float(dt), float(theta), float(q), len(fA), _p(fA, ctypes.c_double),
-                               _p(fB, ctypes.c_double), float(rtol), float(atol), int(maxit), _p(uv, ctypes.c_double),
-                               _p(stats, ctypes.c_double))
+        base = [self.ndof, _p(self.rowptr, ctypes.c_int64), _p(self.col, ctypes.c_int32),
+                _p(self.mats["m"], ctypes.c_double), _p(lv, ctypes.c_double), _p(jg, ctypes.c_double),
+                float(dt), float(theta), float(q), len(fA), _p(fA, ctypes.c_double),
+                _p(fB, ctypes.c_double), float(rtol), float(atol), int(maxit), _p(uv, ctypes.c_double),
+                _p(stats, ctypes.c_double)]
+        if self.periodic is None:
+            rc = L.orc_theta_solve(*base)
+        else:
+            P = self.periodic
+            FB = np.ascontiguousarray(np.zeros(len(fA)) if FB is None else FB, dtype=np.float64)
+            assert FB.shape[0] == len(fA)
+            g = np.asarray(gdir, dtype=np.float64)
+            g = np.ascontiguousarray(g / np.linalg.norm(g))
+            rc = L.orc_theta_solve_p(*base, _p(self.mats["p"], ctypes.c_double), int(P["dof"].shape[0]),
+                                     _p(P["dof"], ctypes.c_int32), _p(P["src"], ctypes.c_int32),
+                                     _p(P["w"], ctypes.c_double), _p(P["ell"], ctypes.c_double),
+                                     _p(g, ctypes.c_double), _p(FB, ctypes.c_double))
         if rc != 0:
             raise RuntimeError("oracle Krylov solve did not converge")
         sig = np.zeros(2)
