@@ -18,40 +18,6 @@ namespace {
 thread_local std::string g_last_error;
 
 template <class T>
-struct DevArray {
-    T *p = nullptr;
-    size_t n = 0;
-    DevArray() = default;
-    DevArray(const DevArray &) = delete;
-    DevArray &operator=(const DevArray &) = delete;
-    ~DevArray() { release(); }
-    void alloc(size_t count) {
-        release();
-        if (count) {
-            hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
-            if (e != hipSuccess) {
-                p = nullptr;
-                throw Error(DMRI_ERR_ALLOC, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " +
-                                                hipGetErrorString(e));
-            }
-        }
-        n = count;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void upload(const T *h, size_t count, hipStream_t s) {
-        alloc(count);
-        if (count) DMRI_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
-    }
-    void zero(hipStream_t s) {
-        if (n) DMRI_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
-    }
-};
-
-template <class T>
 T read_scalar(const T *d, hipStream_t s) {
     T h;
     DMRI_HIP(hipMemcpyAsync(&h, d, sizeof(T), hipMemcpyDeviceToHost, s));
@@ -111,6 +77,9 @@ struct dmri_ctx {
     int64_t n_per = 0, n_prow = 0;
     DevArray<double> pval, per_w, per_ell;
     DevArray<int> per_dof, per_src, prow, bptr, blist;
+    // solver numbering (Morton order of the DOF vertices); s* arrays are in that numbering
+    DevArray<int> perm, iperm, srow, scol, emap, sdiag, scomp, sper_dof, sper_src, sprow;
+    DevArray<double> sw, spval;
     // solver workspace
     SolverWork work;
     DevArray<double> part, dsig;
@@ -434,6 +403,36 @@ void assemble_impl(dmri_ctx *c) {
         launch_sum(c->w.p, c->ndof, c->part.p, c->dsig.p, s);
         c->volume = read_scalar(c->dsig.p, s);
     }
+
+    // 7. solver numbering
+    {
+        const int64_t n = c->ndof;
+        c->perm.alloc(n);
+        c->iperm.alloc(n);
+        c->srow.alloc(n + 1);
+        c->scol.alloc(c->nnz);
+        c->emap.alloc(c->nnz);
+        c->sdiag.alloc(n);
+        c->sw.alloc(n);
+        if (c->ncomp == 2) c->scomp.alloc(n);
+        else c->scomp.release();
+        double ext = 0.0;
+        for (int d = 0; d < 3; ++d) ext = std::max(ext, c->bb_hi[d] - c->bb_lo[d]);
+        launch_solver_order(n, c->nnz, c->dof_vert.p, c->dof_comp.p, c->xyz.p, c->bb_lo, ext, c->rowptr.p, c->col.p,
+                            c->diag.p, c->w.p, c->ncomp == 2, c->perm.p, c->iperm.p, c->srow.p, c->scol.p, c->emap.p,
+                            c->sdiag.p, c->sw.p, c->ncomp == 2 ? c->scomp.p : nullptr, s);
+        if (c->periodic) {
+            c->spval.alloc(c->nnz);
+            launch_gather_double(c->nnz, c->emap.p, c->pval.p, c->spval.p, s);
+            const size_t ne = (size_t)std::max<int64_t>(c->n_per, 1);
+            c->sper_dof.alloc(ne);
+            c->sper_src.alloc(3 * ne);
+            c->sprow.alloc(std::max<int64_t>(c->n_prow, 1));
+            launch_renumber(c->n_per, c->iperm.p, c->per_dof.p, c->sper_dof.p, s);
+            launch_renumber(3 * c->n_per, c->iperm.p, c->per_src.p, c->sper_src.p, s);
+            launch_renumber(c->n_prow, c->iperm.p, c->prow.p, c->sprow.p, s);
+        }
+    }
     DMRI_HIP(hipEventRecord(e1, s));
     DMRI_HIP(hipEventSynchronize(e1));
     float ms = 0;
@@ -446,16 +445,17 @@ void assemble_impl(dmri_ctx *c) {
 }
 
 // SpMV mapping and lanes per row when neither the caller nor the tuner chose: vector CSR for
-// one signal, signals in registers up to 8 signals, one lane per (row, signal) beyond.
+// one signal, LDS-staged rows with one lane per (row, signal) up to 8 signals, one wave per row
+// (lane = signal) beyond.
 void default_mapping(int S, int &mapping, int &lanes) {
     if (S == 1) {
         mapping = kMapSL;
         lanes = 8;
     } else if (S <= kMaxSRSignals) {
-        mapping = kMapSR;
-        lanes = 4;
+        mapping = kMapStaged;
+        lanes = 1;
     } else {
-        mapping = kMapSL;
+        mapping = kMapWave;
         lanes = 1;
     }
 }
@@ -550,11 +550,11 @@ struct Stepper {
         auto &V = c->val;
         const double theta = seq->theta, dt = seq->dt;
         const double *pv = c->periodic ? c->pval.p : nullptr;
-        launch_prep_scale(n, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, pv, dt,
+        launch_prep_scale(n, c->sdiag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, pv, dt,
                           theta, w.sc.p, s);
-        launch_prep_values(n, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
-                           V[DMRI_MAT_I].p, pv, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, g[0],
-                           g[1], g[2], dt, theta, w.vals.p, w.bre.p, s);
+        launch_prep_values(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p,
+                           V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, pv, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p,
+                           per_gdir, g[0], g[1], g[2], dt, theta, w.vals.p, w.bre.p, s);
         omt = 1.0 - theta;
         if (c->periodic) {
             // mirror phase per step and signal: q_s F(t_{n-1}) g_s
@@ -594,8 +594,8 @@ struct Stepper {
         const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
         a = SolveArgs{};
         a.n = n;
-        a.rowptr = c->rowptr.p;
-        a.col = c->col.p;
+        a.rowptr = c->srow.p;
+        a.col = c->scol.p;
         a.vals = w.vals.p;
         a.bre = w.bre.p;
         a.per_gdir = per_gdir;
@@ -729,8 +729,8 @@ struct Stepper {
         a.xprev = xp;
         if (c->periodic) {
             SolverWork &w = c->work;
-            launch_periodic_step(c->n_per, c->n_prow, S, c->per_dof.p, c->per_src.p, c->per_w.p, c->per_ell.p,
-                                 c->prow.p, c->rowptr.p, c->col.p, c->pval.p, w.sc.p, xc,
+            launch_periodic_step(c->n_per, c->n_prow, S, c->sper_dof.p, c->sper_src.p, c->per_w.p, c->per_ell.p,
+                                 c->sprow.p, c->srow.p, c->scol.p, c->spval.p, w.sc.p, xc,
                                  w.coefP.p + (size_t)k * S * 3, omt, w.ybc.p, w.bper.p, s);
         }
         if (extrap && k > 0) {
@@ -771,7 +771,7 @@ struct Stepper {
         SolverWork &w = c->work;
         if (xc != w.x.p)
             DMRI_HIP(hipMemcpyAsync(w.x.p, xc, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
-        launch_signal(n, S, c->w.p, w.sc.p, w.x.p, w.spart.p, w.sig.p, s);
+        launch_signal(n, S, c->sw.p, w.sc.p, w.x.p, w.spart.p, w.sig.p, s);
         std::vector<double> hs((size_t)S * kNV);
         DMRI_HIP(hipMemcpyAsync(hs.data(), w.sig.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
         DMRI_HIP(hipStreamSynchronize(s));
@@ -836,7 +836,7 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     // backward sweep
     SolverWork &w = c->work;
     const double theta = seq->theta;
-    launch_adj_init_lambda(n, c->w.p, w.sc.p, 1.0 / c->volume, lam.p, s);
+    launch_adj_init_lambda(n, c->sw.p, w.sc.p, 1.0 / c->volume, lam.p, s);
     DMRI_HIP(hipMemsetAsync(mu.p, 0, vb, s));
     DevArray<double> parts;
     parts.alloc((size_t)kMaxGrid * 14);
@@ -859,8 +859,8 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
         for (int k = k1 - 1; k >= k0; --k) {
             const int base = keep_all ? 0 : k0;
             run.solve_given(k, lam.p, mu.p);  // A_k mu = lambda^(k+1), warm start from mu_(k+1)
-            const int np = launch_adj_contract(n, c->rowptr.p, c->col.p, w.sc.p,
-                                               c->ncomp == 2 ? c->dof_comp.p : nullptr, c->val[DMRI_MAT_M].p,
+            const int np = launch_adj_contract(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p,
+                                               c->ncomp == 2 ? c->scomp.p : nullptr, c->val[DMRI_MAT_M].p,
                                                c->val[DMRI_MAT_K].p, c->val[DMRI_MAT_I].p, w.vals.p, mu.p,
                                                segp(k - base), segp(k - base + 1), theta, parts.p, s);
             run.apply_B(k, mu.p, lam2.p);  // lambda^k = B_k mu_k
@@ -923,8 +923,8 @@ void replay_spmv(dmri_ctx *c, const SolveGeom &geo, int S, bool per_gdir, const 
     if (w.part.n < 3 * np) w.part.alloc(3 * np);
     SolveArgs a{};
     a.n = n;
-    a.rowptr = c->rowptr.p;
-    a.col = c->col.p;
+    a.rowptr = c->srow.p;
+    a.col = c->scol.p;
     a.vals = w.vals.p;
     a.bre = w.bre.p;
     a.per_gdir = per_gdir;
@@ -976,7 +976,7 @@ void replay_spmv(dmri_ctx *c, const SolveGeom &geo, int S, bool per_gdir, const 
 // and remember the fastest for later solves of this context.
 // Time spmv_v + spmv_t for every (mapping, lanes per row) candidate for S signals on the
 // assembled pattern and remember the fastest for later solves of this context.
-// us[3*6]: [mapping-1][log2 lanes] microseconds per (spmv_v + spmv_t), -1 where not valid.
+// us[4*6]: [mapping-1][log2 lanes] microseconds per (spmv_v + spmv_t), -1 where not valid.
 void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_mapping, double *us) {
     require(c->assembled, DMRI_ERR_STATE, "dmri_tune: call dmri_assemble first");
     require(S >= 1 && S <= kMaxSignals, DMRI_ERR_ARG, "dmri_tune: nsig must lie in [1, 64]");
@@ -988,11 +988,11 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
     ensure_group_workspace(c, g0, 1, per_gdir);
     SolverWork &w = c->work;
     auto &V = c->val;
-    launch_prep_scale(n, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, nullptr,
+    launch_prep_scale(n, c->sdiag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, nullptr,
                       1.0, 0.5, w.sc.p, s);
-    launch_prep_values(n, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
-                       V[DMRI_MAT_I].p, nullptr, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, 1.0,
-                       0.0, 0.0, 1.0, 0.5, w.vals.p, w.bre.p, s);
+    launch_prep_values(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p,
+                       V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, nullptr, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p,
+                       V[DMRI_MAT_JZ].p, per_gdir, 1.0, 0.0, 0.0, 1.0, 0.5, w.vals.p, w.bre.p, s);
     launch_init_x(n, S, w.sc.p, w.p.p, s);
     launch_init_x(n, S, w.sc.p, w.r.p, s);
     launch_init_x(n, S, w.sc.p, w.v.p, s);
@@ -1004,12 +1004,12 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
     for (auto &x : st0) x.rho[0] = make_double2(1.0, 0.0);
     double best_us = 1e300;
     default_mapping(S, *best_mapping, *best_lanes);
-    for (int m = kMapSL; m <= kMapSR; ++m)
+    for (int m = kMapSL; m <= kMapWave; ++m)
         for (int i = 0; i < 6; ++i) {
             const int G = cand[i];
             double &slot = us[(m - 1) * 6 + i];
             slot = -1.0;
-            if (m == kMapSR ? S > kMaxSRSignals : G * S > 64) continue;
+            if (m == kMapWave ? G != 1 : (m == kMapSR ? S > kMaxSRSignals : G * S > 64)) continue;
             const SolveGeom geo = make_geom(n, G, S, m);
             const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
             if (w.part.n < 3 * np) w.part.alloc(3 * np);
@@ -1365,9 +1365,9 @@ int dmri_time_spmv(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t reps, do
             ensure_group_workspace(c, geo, 1, pg);
             auto &V = c->val;
             SolverWork &w = c->work;
-            launch_prep_scale(c->ndof, c->diag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
+            launch_prep_scale(c->ndof, c->sdiag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
                               V[DMRI_MAT_I].p, nullptr, 1.0, 0.5, w.sc.p, c->stream);
-            launch_prep_values(c->ndof, c->rowptr.p, c->col.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p,
+            launch_prep_values(c->ndof, c->srow.p, c->scol.p, c->emap.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p,
                                V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, nullptr, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p,
                                V[DMRI_MAT_JZ].p, pg, 1.0, 0.0, 0.0, 1.0, 0.5, w.vals.p, w.bre.p, c->stream);
             launch_init_x(c->ndof, nsig, w.sc.p, w.p.p, c->stream);
@@ -1446,7 +1446,7 @@ int dmri_get_solution(dmri_ctx *c, int32_t sig, double *u) {
         set_device(c);
         DevArray<double2> tmp;
         tmp.alloc(c->ndof);
-        launch_unscale(c->ndof, c->last_S, sig, c->work.sc.p, c->work.x.p, tmp.p, c->stream);
+        launch_unscale_perm(c->ndof, c->last_S, sig, c->perm.p, c->work.sc.p, c->work.x.p, tmp.p, c->stream);
         DMRI_HIP(hipMemcpyAsync(u, tmp.p, sizeof(double2) * c->ndof, hipMemcpyDeviceToHost, c->stream));
         DMRI_HIP(hipStreamSynchronize(c->stream));
     });

@@ -25,6 +25,41 @@ struct Error : std::runtime_error {
                                         std::to_string(__LINE__));                                       \
     } while (0)
 
+// Device buffer owned by the host object (freed on destruction).
+template <class T>
+struct DevArray {
+    T *p = nullptr;
+    size_t n = 0;
+    DevArray() = default;
+    DevArray(const DevArray &) = delete;
+    DevArray &operator=(const DevArray &) = delete;
+    ~DevArray() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count) {
+            hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+            if (e != hipSuccess) {
+                p = nullptr;
+                throw Error(-6 /* DMRI_ERR_ALLOC */, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " +
+                                                hipGetErrorString(e));
+            }
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) DMRI_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void zero(hipStream_t s) {
+        if (n) DMRI_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+    }
+};
+
 // Per-signal BiCGStab state.  Scalars of iteration i live in slot i&1: a kernel only
 // writes the slot of its own iteration and reads slots written by earlier kernels, so no
 // block ever reads a value another block of the same launch writes.
@@ -48,6 +83,7 @@ constexpr int kStageBatch = 8;      // independent loads per thread per batch wh
 constexpr int kMapSL = 1;
 constexpr int kMapStaged = 2;
 constexpr int kMapSR = 3;
+constexpr int kMapWave = 4;  // one wave per row, lane = signal; row data through the scalar cache
 constexpr int kMaxSRSignals = 8;
 
 // Launch geometry of the solver kernels for a (mapping, G lanes per row, S signals) choice.
@@ -145,11 +181,25 @@ void launch_periodic_step(int64_t ne, int64_t nrows, int S, const int *ent_dof, 
 // pv: weak pseudo-periodic boundary mass (implicit side only), or null
 void launch_prep_scale(int64_t n, const int *diag, const double *m, const double *k, const double *r,
                        const double *gi, const double *pv, double dt, double theta, double *sc, hipStream_t s);
-// per-gdir == false: val2[e] = {a, g.J} ; per-gdir == true: val4[e] = {a, Jx, Jy, Jz}; all scaled by sc_i sc_j
-void launch_prep_values(int64_t n, const int *rowptr, const int *col, const double *sc, const double *m,
-                        const double *k, const double *r, const double *gi, const double *pv, const double *jx,
-                        const double *jy, const double *jz, bool per_gdir, double gx, double gy, double gz, double dt,
-                        double theta, double *vals, double *bre, hipStream_t s);
+// per-gdir == false: val2[e] = {a, g.J} ; per-gdir == true: val4[e] = {a, Jx, Jy, Jz}; all scaled by sc_i sc_j.
+// rowptr/col in the solver numbering; m..jz are the assembled arrays, read at emap[e].
+void launch_prep_values(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
+                        const double *m, const double *k, const double *r, const double *gi, const double *pv,
+                        const double *jx, const double *jy, const double *jz, bool per_gdir, double gx, double gy,
+                        double gz, double dt, double theta, double *vals, double *bre, hipStream_t s);
+// Solver numbering along a Morton curve of the DOF vertices: perm (new -> old), iperm, the
+// renumbered CSR (srow, scol) with emap[e'] = old entry index, and per new row the old diagonal
+// position sdiag, the signal weight sw and (two compartments) the compartment scomp.
+void launch_solver_order(int64_t n, int64_t nnz, const int *dof_vert, const int *dof_comp, const double *xyz,
+                         const double lo[3], double ext, const int *rowptr, const int *col, const int *diag,
+                         const double *w, bool two_comp, int *perm, int *iperm, int *srow, int *scol, int *emap,
+                         int *sdiag, double *sw, int *scomp, hipStream_t s);
+void launch_gather_double(int64_t n, const int *map, const double *src, double *dst, hipStream_t s);
+// dst[i] = iperm[src[i]] (-1 stays -1)
+void launch_renumber(int64_t n, const int *iperm, const int *src, int *dst, hipStream_t s);
+// u[perm[i]] = sc_i x[i*S+sig]: a solver-numbered state back in the assembly numbering
+void launch_unscale_perm(int64_t n, int S, int sig, const int *perm, const double *sc, const double2 *x, double2 *u,
+                         hipStream_t s);
 void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s);
 
 struct SolveArgs {
@@ -184,9 +234,10 @@ void launch_signal(int64_t n, int S, const double *w, const double *sc, const do
 void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *x, double2 *u, hipStream_t s);
 void launch_copy(const double2 *src, double2 *dst, int64_t n, hipStream_t s);
 // adjoint contraction of one step; returns the number of block partials (14 doubles each)
-int launch_adj_contract(int64_t n, const int *rowptr, const int *col, const double *sc, const int *dof_comp,
-                        const double *mm, const double *kk, const double *gi, const double *vals, const double2 *mu,
-                        const double2 *xn, const double2 *xn1, double theta, double *parts, hipStream_t s);
+int launch_adj_contract(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
+                        const int *dof_comp, const double *mm, const double *kk, const double *gi, const double *vals,
+                        const double2 *mu, const double2 *xn, const double2 *xn1, double theta, double *parts,
+                        hipStream_t s);
 void launch_adj_init_lambda(int64_t n, const double *w, const double *sc, double inv_vol, double2 *lam,
                             hipStream_t s);
 

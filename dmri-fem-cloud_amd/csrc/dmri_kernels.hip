@@ -2,9 +2,11 @@
 // element colouring, closed-form P1 assembly) and the FP64 theta-scheme / BiCGStab solve.
 //
 // Wave64 throughout.  Every reduction is a fixed-shape tree (wave butterfly, then the
-// waves of a block in index order, then the blocks of a grid in index order by the last
-// block to finish), so results are bitwise reproducible run to run.
+// waves of a block in index order, then the blocks' partials in index order, summed by the
+// consuming kernel or a small finalize kernel), so results are bitwise reproducible run to run.
 #include <algorithm>
+
+#include <hipcub/hipcub.hpp>
 
 #include "dmri_internal.hpp"
 
@@ -844,6 +846,95 @@ __global__ void compact_rows_kernel(int64_t n, const int *flag, const int *scan,
 }
 
 // --------------------------------------------------------------------------------------
+// Solver ordering.  The solver numbers DOFs along a Morton (Z-order) curve of their vertex
+// coordinates (the two compartments of a vertex next to each other), so that the rows a block
+// or an XCD works on at once form a compact region and the columns they gather hit in L2.
+// Assembly and every export keep the compartment-major numbering.
+// --------------------------------------------------------------------------------------
+__device__ inline uint64_t spread_bits_20(uint64_t v) {
+    // bit k of v -> bit 3k
+    v &= 0xFFFFFull;
+    v = (v | (v << 32)) & 0x1F00000000FFFFull;
+    v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+    v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+    v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__global__ void morton_key_kernel(int64_t n, const int *dof_vert, const int *dof_comp, const double *xyz, double lo0,
+                                  double lo1, double lo2, double inv_ext, uint64_t *key, int *idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *X = xyz + 3 * (int64_t)dof_vert[i];
+    const double lo[3] = {lo0, lo1, lo2};
+    uint64_t code = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double t = (X[d] - lo[d]) * inv_ext;
+        t = fmin(fmax(t, 0.0), 1.0);
+        const uint64_t q = (uint64_t)(t * 1048575.0);
+        code |= spread_bits_20(q) << d;
+    }
+    key[i] = (code << 1) | (uint64_t)(dof_comp[i] & 1);
+    idx[i] = (int)i;
+}
+
+__global__ void invert_perm_kernel(int64_t n, const int *perm, int *iperm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) iperm[perm[i]] = (int)i;
+}
+
+__global__ void perm_row_len_kernel(int64_t n, const int *perm, const int *rowptr, int *len) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) len[i] = rowptr[perm[i] + 1] - rowptr[perm[i]];
+}
+
+// New row i = old row perm[i] with columns renumbered and sorted; emap[e'] = old entry index.
+// Also the per-row arrays the solver reads in the new numbering.
+__global__ void perm_fill_kernel(int64_t n, const int *perm, const int *iperm, const int *rowptr, const int *col,
+                                 const int *srow, const int *diag, const double *w, const int *dof_comp, int *scol,
+                                 int *emap, int *sdiag, double *sw, int *scomp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r = perm[i];
+    const int e0 = rowptr[r], len = rowptr[r + 1] - e0, o = srow[i];
+    for (int k = 0; k < len; ++k) {
+        const int c = iperm[col[e0 + k]];
+        int p = k;
+        while (p > 0 && scol[o + p - 1] > c) {
+            scol[o + p] = scol[o + p - 1];
+            emap[o + p] = emap[o + p - 1];
+            --p;
+        }
+        scol[o + p] = c;
+        emap[o + p] = e0 + k;
+    }
+    sdiag[i] = diag[r];
+    sw[i] = w[r];
+    if (scomp) scomp[i] = dof_comp[r];
+}
+
+__global__ void gather_double_kernel(int64_t n, const int *map, const double *src, double *dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[map[i]];
+}
+
+__global__ void renumber_kernel(int64_t n, const int *iperm, const int *src, int *dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] < 0 ? -1 : iperm[src[i]];
+}
+
+__global__ void unscale_perm_kernel(int64_t n, int S, int sig, const int *perm, const double *sc, const double2 *x,
+                                    double2 *u) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double2 xi = x[i * S + sig];
+        u[perm[i]] = make_double2(sc[i] * xi.x, sc[i] * xi.y);
+    }
+}
+
+// --------------------------------------------------------------------------------------
 // Solve-time preparation.  Jacobi preconditioning is applied as the symmetric scaling
 // S = diag(a)^(-1/2) of the whole system (a = M/dt + theta*L): the solver works on
 // S A S, whose real diagonal is 1, with x = S^-1 u.  (PETSc's Jacobi on the reference's
@@ -861,28 +952,32 @@ __global__ void prep_scale_kernel(int64_t n, const int *diag, const double *m, c
 
 // kPrepLanes lanes per row walk the row's entries, so the value arrays are read in runs.
 constexpr int kPrepLanes = 8;
-__global__ void prep_values_kernel(int64_t n, const int *rowptr, const int *col, const double *sc, const double *m,
-                                   const double *k, const double *r, const double *gi, const double *pv,
-                                   const double *jx, const double *jy, const double *jz, int per_gdir, double gx,
-                                   double gy, double gz, double dt, double theta, double *vals, double *bre) {
+// Solver rows/columns (rowptr, col) in the solver numbering; the assembled arrays are read at the
+// old entry index emap[e].
+__global__ void prep_values_kernel(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
+                                   const double *m, const double *k, const double *r, const double *gi,
+                                   const double *pv, const double *jx, const double *jy, const double *jz,
+                                   int per_gdir, double gx, double gy, double gz, double dt, double theta,
+                                   double *vals, double *bre) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t i = t / kPrepLanes;
     if (i >= n) return;
     const double si = sc[i];
     for (int e = rowptr[i] + (int)(t % kPrepLanes); e < rowptr[i + 1]; e += kPrepLanes) {
         const double s = si * sc[col[e]];
-        const double l = k[e] + r[e] + gi[e];
-        const double mk = m[e] / dt;
+        const int o = emap[e];
+        const double l = k[o] + r[o] + gi[o];
+        const double mk = m[o] / dt;
         bre[e] = s * (mk - (1.0 - theta) * l);
-        const double impl = s * (mk + theta * (l + (pv ? pv[e] : 0.0)));
+        const double impl = s * (mk + theta * (l + (pv ? pv[o] : 0.0)));
         if (per_gdir) {
             vals[4 * (int64_t)e + 0] = impl;
-            vals[4 * (int64_t)e + 1] = s * jx[e];
-            vals[4 * (int64_t)e + 2] = s * jy[e];
-            vals[4 * (int64_t)e + 3] = s * jz[e];
+            vals[4 * (int64_t)e + 1] = s * jx[o];
+            vals[4 * (int64_t)e + 2] = s * jy[o];
+            vals[4 * (int64_t)e + 3] = s * jz[o];
         } else {
             vals[2 * (int64_t)e + 0] = impl;
-            vals[2 * (int64_t)e + 1] = s * (gx * jx[e] + gy * jy[e] + gz * jz[e]);
+            vals[2 * (int64_t)e + 1] = s * (gx * jx[o] + gy * jy[o] + gz * jz[o]);
         }
     }
 }
@@ -1017,22 +1112,24 @@ __device__ __forceinline__ void lds_row_accumulate(const int *srp, const int *sc
     for (; e < e1; e += G) acc_entry<PG>(svals, e, X(scol[e]), acc);
 }
 
-// Rows of this block: a contiguous chunk, with the chunks of consecutive virtual blocks on
-// the same XCD.  Workgroups are dealt to the 8 XCDs round-robin, so block b runs on XCD b % 8;
-// virtual block (b % 8) * (grid / 8) + b / 8 gives every XCD one contiguous eighth of the rows,
-// and the columns a row gathers are mostly in that XCD's L2.  The grid is a multiple of 8.
+// Rows of this block.  Workgroups are dealt to the 8 XCDs round-robin, so block b runs on XCD
+// b % 8: every XCD owns one contiguous eighth of the rows (with the solver's Morton numbering,
+// a compact region whose gathered columns stay in that XCD's L2), and its blocks sweep that
+// eighth together in tiles of TW rows: block k of the XCD takes tiles k, k + grid/8, ...  The
+// XCD's rows in flight at any time are one window of (grid/8)*TW consecutive rows, whose
+// neighbours' entries are reused from L2 instead of re-fetched.  The grid is a multiple of 8.
 constexpr int kXcds = 8;
-struct RowRange {
-    int64_t r0, r1;
+struct TileWalk {
+    int64_t first, stride, end;  // tile starts first, first + stride, ... < end; a tile is TW rows, cut at end
 };
-__device__ __forceinline__ RowRange block_rows(int64_t n) {
-    const int g = gridDim.x;
-    const int per = g / kXcds;
-    const int vb = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
-    const int64_t chunk = (n + g - 1) / g;
-    const int64_t r0 = (int64_t)vb * chunk;
-    const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
-    return RowRange{r0 < n ? r0 : n, r1};
+__device__ __forceinline__ TileWalk xcd_tiles(int64_t n, int TW) {
+    const int per = (int)gridDim.x / kXcds;
+    const int x = (int)(blockIdx.x % kXcds), k = (int)(blockIdx.x / kXcds);
+    int64_t q = (n + kXcds - 1) / kXcds;
+    q = (q + TW - 1) / TW * TW;
+    const int64_t xb = (int64_t)x * q < n ? (int64_t)x * q : n;
+    const int64_t xe = xb + q < n ? xb + q : n;
+    return TileWalk{xb + (int64_t)k * TW, (int64_t)per * TW, xe};
 }
 
 struct Lane {
@@ -1055,9 +1152,11 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
                                             Emit emit) {
     constexpr int VW = PG ? 4 : 2;
     if (TR <= 0) {  // direct: entries straight from global memory
-        const RowRange R = block_rows(a.n);
+        const TileWalk W = xcd_tiles(a.n, rpb);
         if (!act) return;
-        for (int64_t row = R.r0 + L.lrow; row < R.r1; row += rpb) {
+        for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
+            const int64_t row = t0 + L.lrow;
+            if (row >= W.end) continue;
             Acc<PG> acc;
             row_accumulate<G, PG>(a.rowptr, a.col, a.vals, row, L.g, X, acc);
             group_reduce_acc<G, PG>(acc);
@@ -1070,9 +1169,9 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
     double *svals = reinterpret_cast<double *>(dmri_stage_lds);
     int *scol = reinterpret_cast<int *>(svals + (size_t)ECAP * VW);
     int *srp = scol + ECAP;
-    const RowRange R = block_rows(a.n);
-    for (int64_t t0 = R.r0; t0 < R.r1; t0 += TR) {
-        const int nr = (int)((R.r1 - t0) < TR ? (R.r1 - t0) : TR);
+    const TileWalk W = xcd_tiles(a.n, TR);
+    for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
+        const int nr = (int)((W.end - t0) < TR ? (W.end - t0) : TR);
         const int E0 = a.rowptr[t0];
         const int ne = a.rowptr[t0 + nr] - E0;
         const bool fits = ne <= ECAP;
@@ -1134,9 +1233,10 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
     const bool sel = mode != 1 || mask == nullptr || mask[L.s];
     const int S_ = S;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
-    const RowRange R = block_rows(a.n);
-    for (int64_t row = R.r0 + L.lrow; row < R.r1; row += rpb) {
-        if (!sel) continue;
+    const TileWalk W = xcd_tiles(a.n, rpb);
+    for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
+        const int64_t row = t0 + L.lrow;
+        if (!sel || row >= W.end) continue;
         const double2 *x = a.x;
         const int s = L.s;
         Acc<PG> acc;
@@ -1315,6 +1415,38 @@ __global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, in
     block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
 }
 
+// Head of every t = A s kernel: alpha_s = rho_s / <r^, v>_s per signal into salpha, the
+// active flags into sact (breakdown when <r^, v> = 0).
+__device__ __forceinline__ void spmv_t_head(const SolveArgs &a, int S, int nparts, int it, double2 *salpha,
+                                            int *sact) {
+    __shared__ double tot[kMaxSignals][kNV];
+    signal_totals(a.partV, a.totV, nparts, S, tot);
+    if ((int)threadIdx.x < S) {
+        const int s = threadIdx.x;
+        KrylovState &st = a.st[s];
+        int act = !st.done;
+        const int local = it - st.it0;
+        if (act) {
+            const double2 rv = make_double2(tot[s][0], tot[s][1]);
+            if (rv.x == 0.0 && rv.y == 0.0) {
+                if (blockIdx.x == 0) {
+                    st.done = 1;
+                    st.breakdown = 1;
+                    st.conv = 0;
+                    st.iters = it;
+                }
+                act = 0;
+            } else {
+                const double2 al = c_div(st.rho[local & 1], rv);
+                salpha[s] = al;
+                if (blockIdx.x == 0) st.alpha[local & 1] = al;
+            }
+        }
+        sact[s] = act;
+    }
+    __syncthreads();
+}
+
 // alpha = rho / <r^, v> ; t = A (r - alpha v) with s = r - alpha v formed on the fly;
 // partials T: [<t, s>.re, <t, s>.im, |t|^2, |s|^2]
 template <int G, bool PG>
@@ -1375,6 +1507,102 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
     block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
 }
 
+// ---- one wave per row, lane = signal (any S <= 64).  The row's column indices and values are
+// the same for every lane: with the row index made wave-uniform they load through the scalar
+// cache once per wave (s_load), the vector registers hold only the gathered x and the
+// accumulators, and each lane gathers its own signal's 16 B of x (64 signals: one 1 KB run).
+template <bool PG, class Gather, class Emit>
+__device__ __forceinline__ void wave_rows(int64_t n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                                          const double *__restrict__ vals, bool act, Gather X, Emit emit) {
+    const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
+    const TileWalk W = xcd_tiles(n, nw);
+    for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
+        const int64_t row = t0 + w;
+        if (row >= W.end) continue;
+        const int r = __builtin_amdgcn_readfirstlane((int)row);
+        const int e0 = rowptr[r], e1 = rowptr[r + 1];
+        Acc<PG> acc;
+        acc.a = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int d = 0; d < (PG ? 3 : 1); ++d) acc.j[d] = make_double2(0.0, 0.0);
+        int e = e0;
+        for (; e + 4 <= e1; e += 4) {
+            int c[4];
+            double2 xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) c[u] = col[e + u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xv[u] = act ? X(c[u]) : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc_entry<PG>(vals, e + u, xv[u], acc);
+        }
+        for (; e < e1; ++e) {
+            const double2 xv = act ? X(col[e]) : make_double2(0.0, 0.0);
+            acc_entry<PG>(vals, e, xv, acc);
+        }
+        emit(row, acc);
+    }
+}
+
+template <bool PG>
+__global__ void __launch_bounds__(kThreads) spmv_v_wave_kernel(SolveArgs a, const int *__restrict__ rowptr,
+                                                               const int *__restrict__ col,
+                                                               const double *__restrict__ vals, int S, int it) {
+    if (*a.all_done) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const bool act = lane < S && !a.st[lane].done;
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    const double2 *p = a.p;
+    wave_rows<PG>(
+        a.n, rowptr, col, vals, act, [&](int c) { return p[(int64_t)c * S + lane]; },
+        [&](int64_t row, const Acc<PG> &acc) {
+            if (!act) return;
+            const int64_t idx = row * S + lane;
+            const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
+            a.v[idx] = y;
+            const double2 h = a.rh[idx];
+            red[0] += h.x * y.x + h.y * y.y;
+            red[1] += h.x * y.y - h.y * y.x;
+        });
+    block_signal_partials(red, lane < S, (int)(threadIdx.x >> 6) * S + lane, (int)(blockDim.x >> 6) * S, S,
+                          a.partV);
+}
+
+template <bool PG>
+__global__ void __launch_bounds__(kThreads) spmv_t_wave_kernel(SolveArgs a, const int *__restrict__ rowptr,
+                                                               const int *__restrict__ col,
+                                                               const double *__restrict__ vals, int S, int nparts,
+                                                               int it) {
+    if (*a.all_done) return;
+    __shared__ double2 salpha[kMaxSignals];
+    __shared__ int sact[kMaxSignals];
+    spmv_t_head(a, S, nparts, it, salpha, sact);
+    const int lane = (int)(threadIdx.x & 63);
+    const bool act = lane < S && sact[lane];
+    const double2 alpha = act ? salpha[lane] : make_double2(0.0, 0.0);
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    const double2 *r = a.r, *v = a.v;
+    wave_rows<PG>(
+        a.n, rowptr, col, vals, act,
+        [&](int c) {
+            const int64_t j = (int64_t)c * S + lane;
+            return c_sub(r[j], c_mul(alpha, v[j]));
+        },
+        [&](int64_t row, const Acc<PG> &acc) {
+            if (!act) return;
+            const int64_t idx = row * S + lane;
+            const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
+            a.t[idx] = y;
+            const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
+            red[0] += y.x * sv.x + y.y * sv.y;
+            red[1] += y.x * sv.y - y.y * sv.x;
+            red[2] += y.x * y.x + y.y * y.y;
+            red[3] += sv.x * sv.x + sv.y * sv.y;
+        });
+    block_signal_partials(red, lane < S, (int)(threadIdx.x >> 6) * S + lane, (int)(blockDim.x >> 6) * S, S,
+                          a.partT);
+}
+
 // ---- signals in registers: each lane takes entries g, g+G, ... of a row for all S <= 8
 // signals, so column indices and values are loaded once per entry instead of once per signal.
 constexpr int kMaxSR = 8;
@@ -1426,8 +1654,10 @@ __global__ void __launch_bounds__(kThreads) spmv_v_sr_kernel(SolveArgs a, int S,
         red[s][0] = red[s][1] = 0.0;
     }
     const double2 *p = a.p;
-    const RowRange R = block_rows(a.n);
-    for (int64_t row = R.r0 + lrow; row < R.r1; row += rpb) {
+    const TileWalk W = xcd_tiles(a.n, rpb);
+    for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
+        const int64_t row = t0 + lrow;
+        if (row >= W.end) continue;
         Acc<PG> acc[kMaxSR];
         sr_row<G, PG>(a, row, g, S, [&](int c, int s) { return p[(int64_t)c * S + s]; }, acc);
         if (g != 0) continue;
@@ -1487,8 +1717,10 @@ __global__ void __launch_bounds__(kThreads) spmv_t_sr_kernel(SolveArgs a, int S,
         red[s][0] = red[s][1] = red[s][2] = red[s][3] = 0.0;
     }
     const double2 *r = a.r, *v = a.v;
-    const RowRange R = block_rows(a.n);
-    for (int64_t row = R.r0 + lrow; row < R.r1; row += rpb) {
+    const TileWalk W = xcd_tiles(a.n, rpb);
+    for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
+        const int64_t row = t0 + lrow;
+        if (row >= W.end) continue;
         Acc<PG> acc[kMaxSR];
         sr_row<G, PG>(
             a, row, g, S,
@@ -1571,8 +1803,9 @@ __global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int 
 //   k[12,13] mu_i sum_j Jg_ij x_n1,j
 // Block partials go to parts[block * 14 + k] and are summed on the host in block order.
 __global__ void __launch_bounds__(kThreads) adj_contract_kernel(int64_t n, const int *rowptr, const int *col,
-                                                                const double *sc, const int *dof_comp,
-                                                                const double *mm, const double *kk,
+                                                                const int *emap, const double *sc,
+                                                                const int *dof_comp, const double *mm,
+                                                                const double *kk,
                                                                 const double *gi, const double *vals,
                                                                 const double2 *mu, const double2 *xn,
                                                                 const double2 *xn1, double theta, double *parts) {
@@ -1588,7 +1821,8 @@ __global__ void __launch_bounds__(kThreads) adj_contract_kernel(int64_t n, const
             const double sij = si * sc[j];
             const double2 x0 = xn[j], x1 = xn1[j];
             const double2 y = make_double2((1.0 - theta) * x0.x + theta * x1.x, (1.0 - theta) * x0.y + theta * x1.y);
-            const double vk = sij * kk[e], vm = sij * mm[e], vi = sij * gi[e], vj = vals[2 * (int64_t)e + 1];
+            const int o = emap[e];
+            const double vk = sij * kk[o], vm = sij * mm[o], vi = sij * gi[o], vj = vals[2 * (int64_t)e + 1];
             aK.x += vk * y.x; aK.y += vk * y.y;
             aM.x += vm * y.x; aM.y += vm * y.y;
             aI.x += vi * y.x; aI.y += vi * y.y;
@@ -1818,7 +2052,9 @@ void launch_diag_and_weights(int64_t n, const int *rowptr, const int *col, const
 SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
     if (S < 1 || S > kMaxSignals) throw Error(-1, "signals per launch must lie in [1, 64]");
     if (G < 1 || G > 32 || (G & (G - 1))) throw Error(-1, "lanes per row must be a power of two in [1, 32]");
-    if (mapping != kMapSL && mapping != kMapStaged && mapping != kMapSR) throw Error(-1, "unknown SpMV mapping");
+    if (mapping != kMapSL && mapping != kMapStaged && mapping != kMapSR && mapping != kMapWave)
+        throw Error(-1, "unknown SpMV mapping");
+    if (mapping == kMapWave && G != 1) throw Error(-1, "the wave-per-row mapping takes one lane per signal (lanes 1)");
     if (mapping == kMapSR && S > kMaxSRSignals)
         throw Error(-1, "signals-in-registers mapping needs at most 8 signals per launch group");
     if (mapping != kMapSR && G * S > 64) throw Error(-1, "lanes per row times signals must not exceed 64");
@@ -1833,6 +2069,10 @@ SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
     g.rhs_threads = g.rhs_rpb * g.G_rhs * S;
     g.rpb = mapping == kMapSR ? g.rhs_rpb : std::max(1, kThreads / (G * S));
     g.spmv_threads = mapping == kMapSR ? g.rhs_threads : g.rpb * G * S;
+    if (mapping == kMapWave) {  // one wave per row
+        g.rpb = kThreads / 64;
+        g.spmv_threads = kThreads;
+    }
     g.sr_rpb = std::max(1, kThreads / G);
     g.sr_threads = g.sr_rpb * G;
     g.vec_rpb = std::max(1, kThreads / S);
@@ -1871,13 +2111,59 @@ void launch_prep_scale(int64_t n, const int *diag, const double *m, const double
     prep_scale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, diag, m, k, r, gi, pv, dt, theta, sc);
     check_launch();
 }
-void launch_prep_values(int64_t n, const int *rowptr, const int *col, const double *sc, const double *m,
-                        const double *k, const double *r, const double *gi, const double *pv, const double *jx,
-                        const double *jy, const double *jz, bool per_gdir, double gx, double gy, double gz, double dt,
-                        double theta, double *vals, double *bre, hipStream_t s) {
-    prep_values_kernel<<<grid_for(n * kPrepLanes, 256), 256, 0, s>>>(n, rowptr, col, sc, m, k, r, gi, pv, jx, jy, jz,
-                                                                      per_gdir ? 1 : 0, gx, gy, gz, dt, theta, vals,
-                                                                      bre);
+void launch_prep_values(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
+                        const double *m, const double *k, const double *r, const double *gi, const double *pv,
+                        const double *jx, const double *jy, const double *jz, bool per_gdir, double gx, double gy,
+                        double gz, double dt, double theta, double *vals, double *bre, hipStream_t s) {
+    prep_values_kernel<<<grid_for(n * kPrepLanes, 256), 256, 0, s>>>(n, rowptr, col, emap, sc, m, k, r, gi, pv, jx,
+                                                                      jy, jz, per_gdir ? 1 : 0, gx, gy, gz, dt, theta,
+                                                                      vals, bre);
+    check_launch();
+}
+
+void launch_solver_order(int64_t n, int64_t nnz, const int *dof_vert, const int *dof_comp, const double *xyz,
+                         const double lo[3], double ext, const int *rowptr, const int *col, const int *diag,
+                         const double *w, bool two_comp, int *perm, int *iperm, int *srow, int *scol, int *emap,
+                         int *sdiag, double *sw, int *scomp, hipStream_t s) {
+    DevArray<uint64_t> key, key_sorted;
+    DevArray<int> idx, len;
+    key.alloc(n);
+    key_sorted.alloc(n);
+    idx.alloc(n);
+    len.alloc(n);
+    morton_key_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, dof_vert, dof_comp, xyz, lo[0], lo[1], lo[2],
+                                                        ext > 0 ? 1.0 / ext : 0.0, key.p, idx.p);
+    check_launch();
+    size_t tmp_bytes = 0;
+    DMRI_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key.p, key_sorted.p, idx.p, perm, (int)n, 0, 61,
+                                                s));
+    DevArray<char> tmp;
+    tmp.alloc(std::max<size_t>(tmp_bytes, 1));
+    DMRI_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, key.p, key_sorted.p, idx.p, perm, (int)n, 0, 61, s));
+    invert_perm_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, perm, iperm);
+    check_launch();
+    perm_row_len_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, perm, rowptr, len.p);
+    check_launch();
+    exclusive_scan(len.p, srow, n, s);
+    perm_fill_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, perm, iperm, rowptr, col, srow, diag, w, dof_comp, scol, emap,
+                                                       sdiag, sw, two_comp ? scomp : nullptr);
+    check_launch();
+    (void)nnz;
+    DMRI_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+}
+void launch_gather_double(int64_t n, const int *map, const double *src, double *dst, hipStream_t s) {
+    if (n <= 0) return;
+    gather_double_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, map, src, dst);
+    check_launch();
+}
+void launch_renumber(int64_t n, const int *iperm, const int *src, int *dst, hipStream_t s) {
+    if (n <= 0) return;
+    renumber_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, iperm, src, dst);
+    check_launch();
+}
+void launch_unscale_perm(int64_t n, int S, int sig, const int *perm, const double *sc, const double2 *x, double2 *u,
+                         hipStream_t s) {
+    unscale_perm_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, S, sig, perm, sc, x, u);
     check_launch();
 }
 void launch_periodic_mass(int64_t n, int64_t nv, const int *dof_vert, const int *dof_comp, const int *bptr,
@@ -1936,14 +2222,15 @@ void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s
     init_x_kernel<<<grid_for(n * S, 256), 256, 0, s>>>(n, S, sc, x);
     check_launch();
 }
-int launch_adj_contract(int64_t n, const int *rowptr, const int *col, const double *sc, const int *dof_comp,
-                        const double *mm, const double *kk, const double *gi, const double *vals, const double2 *mu,
-                        const double2 *xn, const double2 *xn1, double theta, double *parts, hipStream_t s) {
+int launch_adj_contract(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
+                        const int *dof_comp, const double *mm, const double *kk, const double *gi, const double *vals,
+                        const double2 *mu, const double2 *xn, const double2 *xn1, double theta, double *parts,
+                        hipStream_t s) {
     unsigned g = grid_for(n, kThreads);
     if (g > (unsigned)kMaxGrid) g = kMaxGrid;
     if (g == 0) g = 1;
-    adj_contract_kernel<<<g, kThreads, 0, s>>>(n, rowptr, col, sc, dof_comp, mm, kk, gi, vals, mu, xn, xn1, theta,
-                                                parts);
+    adj_contract_kernel<<<g, kThreads, 0, s>>>(n, rowptr, col, emap, sc, dof_comp, mm, kk, gi, vals, mu, xn, xn1,
+                                                theta, parts);
     check_launch();
     return (int)g;
 }
@@ -1999,7 +2286,14 @@ void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
 // The SpMV kernels go through hipExtLaunchKernelGGL so that e0/e1 (when not null) take the start
 // and end timestamps of the kernel's own dispatch: no marker packet, hence no extra gap.
 void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if (g.mapping == kMapSR) {
+    if (g.mapping == kMapWave) {
+        if (a.per_gdir)
+            hipExtLaunchKernelGGL((spmv_v_wave_kernel<true>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1, 0, a,
+                                  a.rowptr, a.col, a.vals, g.S, it);
+        else
+            hipExtLaunchKernelGGL((spmv_v_wave_kernel<false>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1, 0,
+                                  a, a.rowptr, a.col, a.vals, g.S, it);
+    } else if (g.mapping == kMapSR) {
         DMRI_DISPATCH_G(g.G, a.per_gdir,
                         hipExtLaunchKernelGGL((spmv_v_sr_kernel<GG, PP>), dim3(g.grid), dim3(g.sr_threads), 0, s, e0,
                                               e1, 0, a, g.S, g.sr_rpb, it));
@@ -2017,7 +2311,14 @@ void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
     }
 }
 void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if (g.mapping == kMapSR) {
+    if (g.mapping == kMapWave) {
+        if (a.per_gdir)
+            hipExtLaunchKernelGGL((spmv_t_wave_kernel<true>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1, 0, a,
+                                  a.rowptr, a.col, a.vals, g.S, (int)g.grid, it);
+        else
+            hipExtLaunchKernelGGL((spmv_t_wave_kernel<false>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1, 0,
+                                  a, a.rowptr, a.col, a.vals, g.S, (int)g.grid, it);
+    } else if (g.mapping == kMapSR) {
         DMRI_DISPATCH_G(g.G, a.per_gdir,
                         hipExtLaunchKernelGGL((spmv_t_sr_kernel<GG, PP>), dim3(g.grid), dim3(g.sr_threads), 0, s, e0,
                                               e1, 0, a, g.S, g.sr_rpb, (int)g.grid, it));

@@ -14,6 +14,10 @@ module builds the meshes the workloads need, deterministically from a seed:
                        order, so neighbouring prisms always agree on a shared quad face.
 * ``cylinders_box_mesh`` -- a rectangular box holding seeded non-overlapping cylinders along z,
                        built the same way (2-D Delaunay + extrusion).
+* ``neuron_mesh``   -- a neuron-like surrogate: a soma sphere and seeded branching cylindrical
+                       dendrites, voxelised on a uniform grid (every voxel whose centre lies inside
+                       the union is kept) and split like ``box_mesh``, so the boundary is
+                       stair-stepped.  One compartment.
 
 Every generator returns a :class:`TetMesh` with FP64 coordinates in micrometres,
 int32 vertex indices and, for two compartments, an int32 cell phase in {0, 1}
@@ -32,6 +36,8 @@ __all__ = [
     "box_mesh",
     "cylinder_mesh",
     "cylinders_box_mesh",
+    "neuron_mesh",
+    "neuron_segments",
     "extrude_triangles",
     "cell_volumes",
     "read_dolfin_xml",
@@ -240,18 +246,47 @@ def cylinder_mesh(radius: float, length: float, target_nv: int = 20000, seed: in
 
 
 def _place_circles(Lx, Ly, n, rmin, rmax, seed, gap, margin):
+    """n non-overlapping circles, radii uniform in [rmin, rmax] (seeded), at least ``gap`` apart and
+    ``margin`` inside the rectangle.  Random sequential placement first; when that jams (dense
+    packings), seeded random centres relaxed by pairwise overlap removal."""
     rng = np.random.default_rng(seed)
     radii = np.sort(rng.uniform(rmin, rmax, size=n))[::-1]
     centers = []
     for r in radii:
-        for _ in range(200000):
+        for _ in range(20000):
             c = rng.uniform([margin + r, margin + r], [Lx - margin - r, Ly - margin - r])
             if all(np.hypot(*(c - cc)) >= r + rr + gap for cc, rr in centers):
                 centers.append((c, r))
                 break
         else:
-            raise RuntimeError("could not place non-overlapping cylinders; lower the radii")
-    return [(float(c[0]), float(c[1]), float(r)) for c, r in centers]
+            break
+    if len(centers) == n:
+        return [(float(c[0]), float(c[1]), float(r)) for c, r in centers]
+    # seeded random centres, radii grown from half size in stages with overlap removal between
+    pos = rng.uniform([margin + rmax, margin + rmax], [Lx - margin - rmax, Ly - margin - rmax], size=(n, 2))
+    eye = np.eye(n, dtype=bool)
+    for s in np.linspace(0.5, 1.0, 26):
+        rr = s * radii
+        lo = np.stack([margin + rr, margin + rr], axis=1)
+        hi = np.stack([Lx - margin - rr, Ly - margin - rr], axis=1)
+        need = rr[:, None] + rr[None, :] + gap
+        for _ in range(4000):
+            pos = np.clip(pos, lo, hi)
+            d = pos[:, None, :] - pos[None, :, :]
+            dist = np.hypot(d[..., 0], d[..., 1])
+            ov = np.where(eye, -1.0, need - dist)
+            if ov.max() <= 0.0:
+                break
+            # push overlapping pairs apart to a little beyond contact so the relaxation ends
+            ov = np.where(ov > -1e-3, ov + 2e-3, 0.0)
+            unit = d / np.where(dist > 1e-12, dist, 1.0)[..., None]
+            pos = pos + 0.5 * (ov[..., None] * unit).sum(axis=1)
+        pos = np.clip(pos, lo, hi)
+    d = pos[:, None, :] - pos[None, :, :]
+    dist = np.hypot(d[..., 0], d[..., 1])
+    if np.where(eye, 0.0, radii[:, None] + radii[None, :] + gap - dist).max() > 0.0:
+        raise RuntimeError("could not place non-overlapping cylinders; lower the radii")
+    return [(float(pos[i, 0]), float(pos[i, 1]), float(radii[i])) for i in range(n)]
 
 
 def cylinders_box_mesh(Lx=30.0, Ly=30.0, Lz=10.0, ncyl=20, rmin=2.0, rmax=4.0, target_nv=500000,
@@ -307,6 +342,128 @@ def cylinders_box_mesh(Lx=30.0, Ly=30.0, Lz=10.0, ncyl=20, rmin=2.0, rmax=4.0, t
 # --------------------------------------------------------------------------------------
 # DOLFIN XML mesh files (comri/meshes/*.xml, read by Mesh(fmesh), main.cpp:200)
 # --------------------------------------------------------------------------------------
+# --------------------------------------------------------------------------------------
+# neuron-like surrogate (voxelised union of a sphere and capsules)
+# --------------------------------------------------------------------------------------
+def neuron_segments(seed=1004, soma_r=8.0, ndend=6, levels=3, rmin=0.5, rmax=2.0, lmin=20.0, lmax=60.0,
+                    branch_angle=35.0):
+    """Seeded dendrite tree: ``ndend`` roots leaving the soma along jittered Fibonacci-sphere
+    directions, each bifurcating ``levels`` times (2**(levels+1)-1 segments per dendrite); root
+    radius uniform in [0.75 rmax, rmax], radius x 0.8 per level (>= rmin), segment lengths uniform
+    in [lmin, lmax].  Returns a list of (p0, p1, r) capsules."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(ndend) + 0.5
+    phi = np.arccos(1 - 2 * k / ndend)
+    th = math.pi * (1 + 5 ** 0.5) * k
+    dirs = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], axis=1)
+    dirs += 0.15 * rng.normal(size=dirs.shape)
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    segs = []
+
+    def grow(p0, d, r, level):
+        L = rng.uniform(lmin, lmax)
+        p1 = p0 + L * d
+        segs.append((p0, p1, r))
+        if level == levels:
+            return
+        a = rng.normal(size=3)
+        a -= a.dot(d) * d
+        a /= np.linalg.norm(a)
+        ang = math.radians(branch_angle)
+        for sgn in (1.0, -1.0):
+            c = math.cos(sgn * ang)
+            s = math.sin(sgn * ang)
+            nd = d * c + np.cross(a, d) * s
+            grow(p1, nd / np.linalg.norm(nd), max(rmin, 0.8 * r), level + 1)
+
+    for d in dirs:
+        grow((soma_r - 0.5) * d, d, rng.uniform(0.75 * rmax, rmax), 0)
+    return segs
+
+
+def _capsule_voxels(p0, p1, r, h):
+    """Integer voxel indices (n,3) whose centres (i+0.5)h lie within r of the segment p0-p1."""
+    d = p1 - p0
+    L = float(np.linalg.norm(d))
+    ax = int(np.argmax(np.abs(d)))
+    u, w = (ax + 1) % 3, (ax + 2) % 3
+    lo = min(p0[ax], p1[ax]) - r
+    hi = max(p0[ax], p1[ax]) + r
+    ia = np.arange(int(math.floor(lo / h)), int(math.ceil(hi / h)) + 1)
+    ca = (ia + 0.5) * h
+    # axis point at each slice (clamped to the segment), the capsule's slice lies within r/|cos| of it
+    t = np.clip((ca - p0[ax]) / d[ax], 0.0, 1.0) if abs(d[ax]) > 0 else np.zeros_like(ca)
+    cen = p0[None, :] + t[:, None] * d[None, :]
+    ext = r / max(abs(d[ax]) / max(L, 1e-300), 0.3) + h
+    m = int(math.ceil(ext / h)) + 1
+    off = np.arange(-m, m + 1)
+    iu = np.floor(cen[:, u] / h).astype(np.int64)[:, None, None] + off[None, :, None]
+    iw = np.floor(cen[:, w] / h).astype(np.int64)[:, None, None] + off[None, None, :]
+    iu, iw = np.broadcast_arrays(iu, iw)
+    ia3 = np.broadcast_to(ia[:, None, None], iu.shape)
+    P = np.zeros(iu.shape + (3,))
+    P[..., ax] = (ia3 + 0.5) * h
+    P[..., u] = (iu + 0.5) * h
+    P[..., w] = (iw + 0.5) * h
+    q = P - p0
+    tt = np.clip((q @ d) / max(L * L, 1e-300), 0.0, 1.0)
+    dist2 = np.sum((q - tt[..., None] * d) ** 2, axis=-1)
+    keep = dist2 <= r * r
+    out = np.zeros((int(keep.sum()), 3), dtype=np.int64)
+    out[:, ax] = ia3[keep]
+    out[:, u] = iu[keep]
+    out[:, w] = iw[keep]
+    return out
+
+
+def _neuron_voxels(h, soma_r, segs):
+    m = int(math.ceil(soma_r / h)) + 1
+    g = np.arange(-m, m + 1)
+    I, J, K = np.meshgrid(g, g, g, indexing="ij")
+    C = (np.stack([I, J, K], axis=-1) + 0.5) * h
+    soma = np.stack([I, J, K], axis=-1)[np.sum(C * C, axis=-1) <= soma_r * soma_r]
+    parts = [soma.reshape(-1, 3).astype(np.int64)]
+    for p0, p1, r in segs:
+        parts.append(_capsule_voxels(np.asarray(p0, float), np.asarray(p1, float), float(r), h))
+    vox = np.concatenate(parts, axis=0)
+    return np.unique(vox, axis=0)
+
+
+def neuron_mesh(target_nv: int = 2_000_000, seed: int = 1004, soma_r: float = 8.0, ndend: int = 6, levels: int = 3,
+                h: float | None = None, **tree) -> TetMesh:
+    """Neuron-like surrogate of about ``target_nv`` vertices: soma sphere of radius ``soma_r``
+    centred at the origin plus the dendrites of :func:`neuron_segments`, voxelised with edge h
+    (chosen for the vertex target when not given) and split into six tetrahedra per voxel."""
+    segs = neuron_segments(seed=seed, soma_r=soma_r, ndend=ndend, levels=levels, **tree)
+    corner = np.array([[dx, dy, dz] for dz in (0, 1) for dy in (0, 1) for dx in (0, 1)], dtype=np.int64)
+
+    def corners(vox):
+        lo = vox.min(axis=0)
+        ext = vox.max(axis=0) - lo + 2
+        cid = (vox[:, None, :] - lo + corner[None, :, :])  # (nvox, 8, 3) corner index triples
+        key = cid[..., 0] + ext[0] * (cid[..., 1] + ext[1] * cid[..., 2])
+        return lo, ext, key
+
+    if h is None:
+        h = 0.6
+        for _ in range(4):  # vertices scale like h^-3 (a little faster on thin branches)
+            nvert = np.unique(corners(_neuron_voxels(h, soma_r, segs))[2].ravel()).shape[0]
+            h *= (nvert / target_nv) ** (1.0 / 3.0)
+    vox = _neuron_voxels(h, soma_r, segs)
+    lo, ext, key = corners(vox)
+    ukey, inv = np.unique(key.ravel(), return_inverse=True)
+    inv = inv.reshape(-1, 8).astype(np.int64)
+    iz, rem = np.divmod(ukey, ext[0] * ext[1])
+    iy, ix = np.divmod(rem, ext[0])
+    xyz = (np.stack([ix, iy, iz], axis=1) + lo).astype(np.float64) * h
+    v = [inv[:, i] for i in range(8)]
+    tets = np.stack([np.stack(t, axis=1) for t in (
+        (v[0], v[1], v[3], v[7]), (v[0], v[1], v[7], v[5]), (v[0], v[5], v[7], v[4]),
+        (v[0], v[3], v[2], v[7]), (v[0], v[6], v[4], v[7]), (v[0], v[2], v[6], v[7]))], axis=1).reshape(-1, 4)
+    return TetMesh(xyz, tets.astype(np.int32), None,
+                   {"kind": "neuron", "seed": seed, "h": h, "nvox": int(vox.shape[0]), "segments": len(segs)})
+
+
 def read_dolfin_xml(path: str) -> TetMesh:
     import re
 
