@@ -805,8 +805,15 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     run.setup(c, 1, &seq->q, g, false, seq, opts, &st);
     const int N = run.N;
     const int64_t n = run.n;
-    int K = aopts && aopts->checkpoint_stride > 0 ? aopts->checkpoint_stride
-                                                  : (int)std::ceil(std::sqrt((double)std::max(N, 1)));
+    // stride 0: keep every state when the N+1 states fit in a quarter of the free HBM (no step is
+    // recomputed), else checkpoints every ceil(sqrt(N)) steps
+    int K = aopts ? aopts->checkpoint_stride : 0;
+    if (K <= 0) {
+        size_t fr = 0, tot = 0;
+        DMRI_HIP(hipMemGetInfo(&fr, &tot));
+        const double all = (double)(N + 1) * (double)n * sizeof(double2);
+        K = all <= 0.25 * (double)fr ? std::max(N, 1) : (int)std::ceil(std::sqrt((double)std::max(N, 1)));
+    }
     K = std::max(1, std::min(K, std::max(N, 1)));
     const bool keep_all = K >= N;
     const int nck = (N + K - 1) / K;
@@ -838,9 +845,10 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     const double theta = seq->theta;
     launch_adj_init_lambda(n, c->sw.p, w.sc.p, 1.0 / c->volume, lam.p, s);
     DMRI_HIP(hipMemsetAsync(mu.p, 0, vb, s));
+    // per-step contraction partials stay on the device and are summed on the host at the end
     DevArray<double> parts;
-    parts.alloc((size_t)kMaxGrid * 14);
-    std::vector<double> hp((size_t)kMaxGrid * 14);
+    parts.alloc((size_t)std::max(N, 1) * kMaxGrid * 14);
+    std::vector<int> nparts_of(std::max(N, 1), 0);
     double gK[2] = {0, 0}, gM[2] = {0, 0}, gI = 0, gq = 0;
     std::vector<double> dfa(std::max(N, 1), 0.0), dfb(std::max(N, 1), 0.0);
     int recomputed = 0;
@@ -859,17 +867,25 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
         for (int k = k1 - 1; k >= k0; --k) {
             const int base = keep_all ? 0 : k0;
             run.solve_given(k, lam.p, mu.p);  // A_k mu = lambda^(k+1), warm start from mu_(k+1)
-            const int np = launch_adj_contract(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p,
+            nparts_of[k] = launch_adj_contract(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p,
                                                c->ncomp == 2 ? c->scomp.p : nullptr, c->val[DMRI_MAT_M].p,
                                                c->val[DMRI_MAT_K].p, c->val[DMRI_MAT_I].p, w.vals.p, mu.p,
-                                               segp(k - base), segp(k - base + 1), theta, parts.p, s);
+                                               segp(k - base), segp(k - base + 1), theta,
+                                               parts.p + (size_t)k * kMaxGrid * 14, s);
             run.apply_B(k, mu.p, lam2.p);  // lambda^k = B_k mu_k
-            DMRI_HIP(hipMemcpyAsync(hp.data(), parts.p, sizeof(double) * 14 * np, hipMemcpyDeviceToHost, s));
-            DMRI_HIP(hipStreamSynchronize(s));
             std::swap(lam.p, lam2.p);
+        }
+    }
+    std::vector<double> hp((size_t)std::max(N, 1) * kMaxGrid * 14);
+    DMRI_HIP(hipMemcpyAsync(hp.data(), parts.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
+    DMRI_HIP(hipStreamSynchronize(s));
+    for (int j = nck - 1; j >= 0; --j) {
+        const int k0 = j * K, k1 = std::min(N, k0 + K);
+        for (int k = k1 - 1; k >= k0; --k) {
+            const double *pk = hp.data() + (size_t)k * kMaxGrid * 14;
             double t[14] = {0};
-            for (int b = 0; b < np; ++b)
-                for (int q = 0; q < 14; ++q) t[q] += hp[(size_t)b * 14 + q];
+            for (int b = 0; b < nparts_of[k]; ++b)
+                for (int q = 0; q < 14; ++q) t[q] += pk[(size_t)b * 14 + q];
             // dS/dp = Re sum_k mu_k^T (dB_k x^k - dA_k x^(k+1))
             gK[0] -= t[0];
             gK[1] -= t[2];

@@ -39,9 +39,16 @@ def _close(a, b, tol=2e-5):
 
 @pytest.fixture(scope="module")
 def adj():
-    s, g, st = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14)
+    stride = int(np.ceil(np.sqrt(len(FA))))
+    s, g, st = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14, checkpoint_stride=stride)
     s_all, g_all, _ = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14, checkpoint_stride=10 ** 6)
     return s, g, s_all, g_all
+
+
+def test_automatic_checkpointing_keeps_every_state_when_it_fits():
+    # a few MB of states against hundreds of GB of HBM: nothing is recomputed
+    _, g, _ = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14)
+    assert g["checkpoints"] == 0 and g["recomputed_steps"] == 0
 
 
 def test_adjoint_signal_equals_forward(adj):
