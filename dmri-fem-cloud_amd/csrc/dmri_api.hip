@@ -28,7 +28,7 @@ T read_scalar(const T *d, hipStream_t s) {
 constexpr int kEventPool = 512;
 
 struct SolverWork {
-    DevArray<double2> x, r, rh, p, v, t, b, xprev, ybc, bper;
+    DevArray<double2> x, r, rh, p, v, t, b, xprev, xprev2, ybc, bper;
     DevArray<double> sc, vals, bre, part, tot, coef, coefP, sig, spart;
     DevArray<KrylovState> st;
     DevArray<int> all_done, mask;
@@ -478,6 +478,7 @@ void ensure_group_workspace(dmri_ctx *c, const SolveGeom &g, int nsteps, bool pe
         w.t.alloc(nS);
         w.b.alloc(nS);
         w.xprev.alloc(nS);
+        w.xprev2.alloc(nS);
     }
     if (w.sc.n != (size_t)c->ndof) w.sc.alloc(c->ndof);
     const size_t nval = (size_t)c->nnz * (per_gdir ? 4 : 2);
@@ -516,7 +517,8 @@ struct Stepper {
     SolveArgs a{};
     const dmri_solver_opts *opts = nullptr;
     dmri_stats *st = nullptr;
-    double2 *xc = nullptr, *xp = nullptr;
+    double2 *xc = nullptr, *xp = nullptr, *xq = nullptr;  // u^n, u^(n-1), u^(n-2)
+    int guess_order = 2;
     double omt = 0.5;
     int chunk = 8;
     std::vector<KrylovState> zero_st;
@@ -601,7 +603,11 @@ struct Stepper {
         a.per_gdir = per_gdir;
         xc = w.x.p;
         xp = w.xprev.p;
-        extrap = opts->initial_guess == 0;
+        xq = w.xprev2.p;
+        // initial guess of step n: 0 = quadratic extrapolation 3u^n - 3u^(n-1) + u^(n-2),
+        // 2 = linear 2u^n - u^(n-1), 1 = u^n
+        guess_order = opts->initial_guess == 1 ? 0 : (opts->initial_guess == 2 ? 1 : 2);
+        extrap = guess_order > 0;
         a.x = xc;
         a.xprev = xp;
         a.r = w.r.p;
@@ -722,7 +728,7 @@ struct Stepper {
         chunk = std::max(1, step_max);
     }
 
-    // One theta step of the forward problem on xc (xp holds the previous state).
+    // One theta step of the forward problem on xc (xp, xq hold the two previous states).
     void step(int k) {
         begin_step(k);
         a.x = xc;
@@ -733,10 +739,18 @@ struct Stepper {
                                  c->sprow.p, c->srow.p, c->scol.p, c->spval.p, w.sc.p, xc,
                                  w.coefP.p + (size_t)k * S * 3, omt, w.ybc.p, w.bper.p, s);
         }
-        if (extrap && k > 0) {
-            // b from u^n; xp <- 2 u^n - u^(n-1); swap so that the guess is iterated on
+        const int order = std::min(guess_order, k);
+        if (order > 0) {
+            // b from u^n; xq <- the extrapolated guess; iterate on it, then rotate the history
+            a.xpp = xq;
+            a.gc0 = order == 1 ? 2.0 : 3.0;
+            a.gc1 = order == 1 ? -1.0 : -3.0;
+            a.gc2 = order == 1 ? 0.0 : 1.0;
             launch_rhs_residual(geo, a, 2, nullptr, s);
-            std::swap(xc, xp);
+            double2 *u_n = xc, *u_n1 = xp;
+            xc = xq;
+            xp = u_n;
+            xq = u_n1;
             a.x = xc;
             a.xprev = xp;
             launch_rhs_residual(geo, a, 1, nullptr, s);
@@ -817,9 +831,11 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     K = std::max(1, std::min(K, std::max(N, 1)));
     const bool keep_all = K >= N;
     const int nck = (N + K - 1) / K;
-    // checkpoints (x^k, x^(k-1)) at k = 0, K, 2K, ...; a segment holds x^k0 .. x^k1
+    // checkpoints (x^k, x^(k-1), x^(k-2)) at k = 0, K, 2K, ... (the history the initial guess of
+    // step k extrapolates from, so a recomputed segment repeats the forward sweep bit for bit);
+    // a segment holds x^k0 .. x^k1
     DevArray<double2> ck, seg, mu, lam, lam2;
-    ck.alloc((size_t)std::max(nck, 1) * 2 * n);
+    ck.alloc((size_t)std::max(nck, 1) * 3 * n);
     seg.alloc((size_t)(std::min(K, std::max(N, 1)) + 1) * n);
     mu.alloc(n);
     lam.alloc(n);
@@ -830,8 +846,10 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     for (int k = 0; k < N; ++k) {
         if (k % K == 0) {
             const int j = k / K;
-            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(2 * j) * n, run.xc, vb, hipMemcpyDeviceToDevice, s));
-            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(2 * j + 1) * n, k > 0 ? run.xp : run.xc, vb,
+            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j) * n, run.xc, vb, hipMemcpyDeviceToDevice, s));
+            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j + 2) * n, k > 1 ? run.xq : run.xc, vb,
+                                    hipMemcpyDeviceToDevice, s));
+            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j + 1) * n, k > 0 ? run.xp : run.xc, vb,
                                     hipMemcpyDeviceToDevice, s));
         }
         if (keep_all && k == 0) DMRI_HIP(hipMemcpyAsync(segp(0), run.xc, vb, hipMemcpyDeviceToDevice, s));
@@ -855,8 +873,9 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     for (int j = nck - 1; j >= 0; --j) {
         const int k0 = j * K, k1 = std::min(N, k0 + K);
         if (!keep_all) {
-            DMRI_HIP(hipMemcpyAsync(run.xc, ck.p + (size_t)(2 * j) * n, vb, hipMemcpyDeviceToDevice, s));
-            DMRI_HIP(hipMemcpyAsync(run.xp, ck.p + (size_t)(2 * j + 1) * n, vb, hipMemcpyDeviceToDevice, s));
+            DMRI_HIP(hipMemcpyAsync(run.xc, ck.p + (size_t)(3 * j) * n, vb, hipMemcpyDeviceToDevice, s));
+            DMRI_HIP(hipMemcpyAsync(run.xp, ck.p + (size_t)(3 * j + 1) * n, vb, hipMemcpyDeviceToDevice, s));
+            DMRI_HIP(hipMemcpyAsync(run.xq, ck.p + (size_t)(3 * j + 2) * n, vb, hipMemcpyDeviceToDevice, s));
             DMRI_HIP(hipMemcpyAsync(segp(0), run.xc, vb, hipMemcpyDeviceToDevice, s));
             for (int k = k0; k < k1; ++k) {
                 run.step(k);

@@ -211,6 +211,8 @@ struct SolveArgs {
     const double *coefA;   // [S*3] implicit-side imaginary coefficients of this step
     const double *coefB;   // [S*3] explicit-side coefficients of this step
     double2 *x, *r, *rh, *p, *v, *t, *b, *xprev;
+    double2 *xpp;                   // mode 2 writes the initial guess gc0 x + gc1 xprev + gc2 xpp here
+    double gc0, gc1, gc2;
     const double2 *bper;            // weak pseudo-periodic right-hand side of this step, or null
     double *partU, *partV, *partT;  // [grid][S][kNV] each
     double *totU, *totV, *totT;     // [S][kNV] finalize outputs, or null: consumers reduce the partials
@@ -220,7 +222,8 @@ struct SolveArgs {
     int maxit;
 };
 // mode 0: b and r from x; 1: r from the stored b (masked signals, nullptr = all); 2: b only and
-// xprev <- 2 x - xprev; 3: b only
+// xpp <- gc0 x + gc1 xprev + gc2 xpp (the extrapolated initial guess; xpp is not read when
+// gc2 = 0); 3: b only
 void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const int *mask, hipStream_t s);
 void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
 void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,

@@ -1223,7 +1223,7 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
 // mode 0: b = (Bre + i cB.J) x, r = b - (Are + i cA.J) x, r^ = r          (start of a step)
 // mode 1: r = b - (Are + i cA.J) x with the stored b, for the masked signals (restart, or
 //         after an extrapolated initial guess; mask == nullptr selects every signal)
-// mode 2: b = (Bre + i cB.J) x only, and xprev <- 2 x - xprev (the extrapolated guess)
+// mode 2: b = (Bre + i cB.J) x only, and xpp <- gc0 x + gc1 xprev + gc2 xpp (the extrapolated guess)
 // mode 3: b = (Bre + i cB.J) x only (the adjoint's lambda^n = B_n mu_n)
 // partials U (modes 0, 1): [|r|^2, 0, |r|^2, |b|^2]
 template <int G, bool PG>
@@ -1291,7 +1291,13 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
             }
             if (mode == 2) {
                 const double2 xi = x[idx], xp = a.xprev[idx];
-                a.xprev[idx] = make_double2(2.0 * xi.x - xp.x, 2.0 * xi.y - xp.y);
+                double2 g = make_double2(a.gc0 * xi.x + a.gc1 * xp.x, a.gc0 * xi.y + a.gc1 * xp.y);
+                if (a.gc2 != 0.0) {
+                    const double2 xq = a.xpp[idx];
+                    g.x += a.gc2 * xq.x;
+                    g.y += a.gc2 * xq.y;
+                }
+                a.xpp[idx] = g;
             } else if (mode == 3) {
             } else {
                 const double2 ax = imag_coef_apply<G, PG>(acc, a.coefA, s);

@@ -157,7 +157,8 @@ typedef struct {
                               events cost GPU time of their own, so a small k perturbs least */
     int32_t lanes_per_row; /* SpMV lanes per row: 0 = tuned value (dmri_tune / dmri_set_tuning) or the
                               library default; else a power of two in 1..32 */
-    int32_t initial_guess; /* Krylov start of step n: 0 = 2u^n - u^(n-1) (extrapolated), 1 = u^n */
+    int32_t initial_guess; /* Krylov start of step n: 0 = 3u^n - 3u^(n-1) + u^(n-2) (quadratic
+                              extrapolation; linear at n = 1), 1 = u^n, 2 = 2u^n - u^(n-1) */
     int32_t mapping;       /* SpMV mapping: 0 = tuned/default, 1 = signals spread over lanes
                               (lanes * signals <= 64), 2 = the same with rows staged in LDS,
                               3 = signals kept in registers (at most 8 signals per launch group),

@@ -225,8 +225,14 @@ def test_initial_guess_modes_agree():
     q = O.q_from_b_pgse(2000.0, 10600.0, 43100.0)
     s0, st0 = ctx.solve(fA, fB, 200.0, q, (0, 1, 0), rtol=1e-13, guess=0)
     s1, st1 = ctx.solve(fA, fB, 200.0, q, (0, 1, 0), rtol=1e-13, guess=1)
-    print(f"  iterations: extrapolated {st0['iters_total']} previous {st1['iters_total']}", flush=True)
-    assert abs(s0 - s1) <= 1e-10 * abs(s1)
+    s2, st2 = ctx.solve(fA, fB, 200.0, q, (0, 1, 0), rtol=1e-13, guess=2)
+    print(f"  iterations: quadratic {st0['iters_total']} linear {st2['iters_total']} previous {st1['iters_total']}",
+          flush=True)
+    assert abs(s0 - s1) <= 1e-10 * abs(s1) and abs(s2 - s1) <= 1e-10 * abs(s1)
+    assert st0["iters_total"] < st2["iters_total"] < st1["iters_total"]
+    S = O.System(mesh.xyz, mesh.tets, mesh.phase, D=kw["D"], kappa=kw["kappa"])
+    ref, _, _ = S.solve((0, 1, 0), q, 200.0, fA, fB, rtol=1e-13)
+    assert abs(s0.real - ref) <= 1e-9 * abs(ref)
 
 
 def test_sampled_event_profile_and_spmv_replay():
