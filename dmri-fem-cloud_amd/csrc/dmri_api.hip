@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -80,6 +82,8 @@ struct dmri_ctx {
     // solver numbering (Morton order of the DOF vertices); s* arrays are in that numbering
     DevArray<int> perm, iperm, srow, scol, emap, sdiag, scomp, sper_dof, sper_src, sprow;
     DevArray<double> sw, spval;
+    // column compression of the solver CSR per tile height (built on first use by a solve)
+    std::map<int, std::unique_ptr<TileCompression>> tiles;
     // solver workspace
     SolverWork work;
     DevArray<double> part, dsig;
@@ -270,6 +274,7 @@ void assemble_periodic(dmri_ctx *c) {
 
 void assemble_impl(dmri_ctx *c) {
     require(c->has_mesh, DMRI_ERR_STATE, "dmri_assemble: no mesh (call dmri_mesh_set first)");
+    c->tiles.clear();
     set_device(c);
     hipStream_t s = c->stream;
     hipEvent_t e0, e1;
@@ -507,6 +512,27 @@ void ensure_group_workspace(dmri_ctx *c, const SolveGeom &g, int nsteps, bool pe
 
 // One launch group of S <= 64 signals sharing the time profile.  q[s], g[3*s] (unit).
 // Theta-scheme stepping engine for one launch group of S <= 64 signals sharing the time profile.
+// Tile compression for a kMapTile geometry (built once per tile height and kept), with the
+// staged-column capacity of the launch set from its largest tile; null for other mappings.
+const TileCompression *tiles_for(dmri_ctx *c, SolveGeom &geo) {
+    if (geo.mapping != kMapTile) return nullptr;
+    auto it = c->tiles.find(geo.tile_rows);
+    if (it == c->tiles.end()) {
+        auto tc = std::make_unique<TileCompression>();
+        build_tile_compression(c->ndof, c->nnz, c->srow.p, c->scol.p, geo.tile_rows, *tc, c->stream);
+        it = c->tiles.emplace(geo.tile_rows, std::move(tc)).first;
+    }
+    geo.tile_ucap = std::max(1, std::min(kTileLdsBytes / (16 * geo.S), it->second->maxnu));
+    return it->second.get();
+}
+void attach_tiles(SolveArgs &a, const TileCompression *tc) {
+    if (!tc) return;
+    a.tinfo = tc->tinfo.p;
+    a.tptr = tc->tptr.p;
+    a.ucol = tc->ucol.p;
+    a.lcol = tc->lcol.p;
+}
+
 struct Stepper {
     dmri_ctx *c = nullptr;
     hipStream_t s = nullptr;
@@ -514,6 +540,7 @@ struct Stepper {
     int64_t n = 0;
     bool per_gdir = false, extrap = true, prof = false;
     SolveGeom geo;
+    const TileCompression *tiles = nullptr;
     SolveArgs a{};
     const dmri_solver_opts *opts = nullptr;
     dmri_stats *st = nullptr;
@@ -547,6 +574,7 @@ struct Stepper {
             if (opts->lanes_per_row > 0) lanes = opts->lanes_per_row;
         }
         geo = make_geom(n, lanes, S, mapping);
+        tiles = tiles_for(c, geo);
         ensure_group_workspace(c, geo, N, per_gdir);
         SolverWork &w = c->work;
         auto &V = c->val;
@@ -625,6 +653,7 @@ struct Stepper {
         a.st = w.st.p;
         a.all_done = w.all_done.p;
         a.bper = c->periodic ? w.bper.p : nullptr;
+        attach_tiles(a, tiles);
         a.rtol = opts->rtol;
         a.atol = opts->atol;
         a.maxit = opts->maxit;
@@ -949,9 +978,10 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
 // Launch spmv_v and spmv_t `reps` times each (each followed by its finalize kernel, as in a
 // solve) on the prepared solver buffers; us_v, us_t = mean duration of one SpMV kernel from the
 // start and end timestamps of its own dispatch (events bound to the kernel, no marker packets).
-void replay_spmv(dmri_ctx *c, const SolveGeom &geo, int S, bool per_gdir, const std::vector<KrylovState> &st0,
+void replay_spmv(dmri_ctx *c, SolveGeom geo, int S, bool per_gdir, const std::vector<KrylovState> &st0,
                  int reps, double &us_v, double &us_t) {
     hipStream_t s = c->stream;
+    const TileCompression *tiles = tiles_for(c, geo);
     SolverWork &w = c->work;
     const int64_t n = c->ndof;
     const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
@@ -985,6 +1015,7 @@ void replay_spmv(dmri_ctx *c, const SolveGeom &geo, int S, bool per_gdir, const 
     a.all_done = w.all_done.p;
     a.rtol = 1e-12;
     a.maxit = 1;
+    attach_tiles(a, tiles);
     DMRI_HIP(hipMemcpyAsync(w.st.p, st0.data(), sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
     DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
     launch_spmv_v(geo, a, 0, s);  // warm-up
@@ -1011,7 +1042,7 @@ void replay_spmv(dmri_ctx *c, const SolveGeom &geo, int S, bool per_gdir, const 
 // and remember the fastest for later solves of this context.
 // Time spmv_v + spmv_t for every (mapping, lanes per row) candidate for S signals on the
 // assembled pattern and remember the fastest for later solves of this context.
-// us[4*6]: [mapping-1][log2 lanes] microseconds per (spmv_v + spmv_t), -1 where not valid.
+// us[5*6]: [mapping-1][log2 lanes] microseconds per (spmv_v + spmv_t), -1 where not valid.
 void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_mapping, double *us) {
     require(c->assembled, DMRI_ERR_STATE, "dmri_tune: call dmri_assemble first");
     require(S >= 1 && S <= kMaxSignals, DMRI_ERR_ARG, "dmri_tune: nsig must lie in [1, 64]");
@@ -1039,7 +1070,7 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
     for (auto &x : st0) x.rho[0] = make_double2(1.0, 0.0);
     double best_us = 1e300;
     default_mapping(S, *best_mapping, *best_lanes);
-    for (int m = kMapSL; m <= kMapWave; ++m)
+    for (int m = kMapSL; m <= kMapTile; ++m)
         for (int i = 0; i < 6; ++i) {
             const int G = cand[i];
             double &slot = us[(m - 1) * 6 + i];

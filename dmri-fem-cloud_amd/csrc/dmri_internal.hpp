@@ -76,7 +76,7 @@ constexpr int kMaxGridSpmv = 2048; // blocks of the SpMV kernels: 8 blocks of 4 
 constexpr int kMaxSignals = 64; // signals per batched launch
 constexpr int kNV = 4;          // doubles per (block, signal) partial slot
 constexpr int kStageNnzPerRow = 24; // LDS budget of a staged SpMV tile, entries per row
-constexpr int kStageBatch = 8;      // independent loads per thread per batch when staging a tile
+constexpr int kStageBatch = 4;      // independent loads per thread per batch when staging a tile
 
 // SpMV mappings: signals spread over lanes (entries from global memory, or staged in LDS per
 // tile of rows), or all signals of a lane's entries kept in registers (S <= 8).
@@ -84,6 +84,9 @@ constexpr int kMapSL = 1;
 constexpr int kMapStaged = 2;
 constexpr int kMapSR = 3;
 constexpr int kMapWave = 4;  // one wave per row, lane = signal; row data through the scalar cache
+constexpr int kMapTile = 5;  // tiles of rpb rows: the tile's distinct columns staged once in LDS
+constexpr int kTileMaxEntries = 4096;  // entries of a tile the compression build sorts in LDS
+constexpr int kTileLdsBytes = 32768;   // LDS cap of the staged x values of one tile (distinct columns x S)
 constexpr int kMaxSRSignals = 8;
 
 // Launch geometry of the solver kernels for a (mapping, G lanes per row, S signals) choice.
@@ -97,6 +100,8 @@ struct SolveGeom {
     int vec_rpb = 256;       // rows per block in the vector kernels
     int vec_threads = 256;   // vec_rpb * S
     int grid = 8;            // blocks of every SpMV-type kernel (a multiple of 8)
+    int tile_rows = 0;       // kMapTile: rows of a compressed tile (= rpb)
+    int tile_ucap = 0;       // kMapTile: distinct columns staged per tile (LDS), set from the build
     int vgrid = 1;           // blocks of the vector kernels
 };
 SolveGeom make_geom(int64_t n, int G, int S, int mapping = kMapSL);
@@ -195,6 +200,18 @@ void launch_solver_order(int64_t n, int64_t nnz, const int *dof_vert, const int 
                          const double *w, bool two_comp, int *perm, int *iperm, int *srow, int *scol, int *emap,
                          int *sdiag, double *sw, int *scomp, hipStream_t s);
 void launch_gather_double(int64_t n, const int *map, const double *src, double *dst, hipStream_t s);
+// Column compression of the solver CSR in tiles of TR rows: tinfo (ntiles), tptr (ntiles+1),
+// ucol (tptr[ntiles]) and lcol (nnz) as used by the kMapTile SpMV kernels.  The arrays are
+// allocated by the callee through the allocator callback.
+struct TileCompression {
+    DevArray<int> tinfo, tptr, ucol;
+    DevArray<unsigned short> lcol;
+    int TR = 0;
+    int64_t nunique = 0;
+    int maxnu = 0;  // most distinct columns of a compressed tile
+};
+void build_tile_compression(int64_t n, int64_t nnz, const int *srow, const int *scol, int TR, TileCompression &tc,
+                            hipStream_t s);
 // dst[i] = iperm[src[i]] (-1 stays -1)
 void launch_renumber(int64_t n, const int *iperm, const int *src, int *dst, hipStream_t s);
 // u[perm[i]] = sc_i x[i*S+sig]: a solver-numbered state back in the assembly numbering
@@ -212,6 +229,10 @@ struct SolveArgs {
     const double *coefB;   // [S*3] explicit-side coefficients of this step
     double2 *x, *r, *rh, *p, *v, *t, *b, *xprev;
     double2 *xpp;                   // mode 2 writes the initial guess gc0 x + gc1 xprev + gc2 xpp here
+    // kMapTile: per tile of tile_rows rows, tinfo = number of distinct columns (-1: not compressed),
+    // tptr = offset of its sorted distinct columns ucol; lcol = per entry, index into that list
+    const int *tinfo, *tptr, *ucol;
+    const unsigned short *lcol;
     double gc0, gc1, gc2;
     const double2 *bper;            // weak pseudo-periodic right-hand side of this step, or null
     double *partU, *partV, *partT;  // [grid][S][kNV] each

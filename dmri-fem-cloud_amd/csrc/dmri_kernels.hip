@@ -920,6 +920,94 @@ __global__ void gather_double_kernel(int64_t n, const int *map, const double *sr
     if (i < n) dst[i] = src[map[i]];
 }
 
+// Column compression of a tile of TR rows (one block per tile): sort the tile's column indices
+// in LDS (bitonic), keep the distinct ones.  pass 0 writes their count (or -1 when the tile has
+// more than kTileMaxEntries entries); pass 1 writes them to ucol at tptr[tile] and the index of
+// every entry's column among them to lcol.
+__global__ void __launch_bounds__(256) tile_compress_kernel(int pass, int64_t n, int TR, const int *srow,
+                                                            const int *scol, int *tinfo, const int *tptr, int *ucol,
+                                                            unsigned short *lcol) {
+    __shared__ int sk[kTileMaxEntries];
+    __shared__ int sflag[kTileMaxEntries];
+    const int64_t t = blockIdx.x;
+    const int64_t r0 = t * TR;
+    const int64_t r1 = r0 + TR < n ? r0 + TR : n;
+    const int E0 = srow[r0], ne = srow[r1] - E0;
+    if (ne > kTileMaxEntries) {
+        if (pass == 0 && threadIdx.x == 0) tinfo[t] = -1;
+        return;
+    }
+    int m = 1;
+    while (m < ne) m <<= 1;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) sk[i] = i < ne ? scol[E0 + i] : 0x7fffffff;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int x = sk[i], y = sk[l];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) {
+                        sk[i] = y;
+                        sk[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // first occurrences of each column, their rank by a block scan over the flags
+    for (int i = threadIdx.x; i < m; i += blockDim.x) sflag[i] = (i < ne && (i == 0 || sk[i] != sk[i - 1])) ? 1 : 0;
+    __syncthreads();
+    {  // inclusive scan: serial within each thread's chunk, then the chunk offsets
+        __shared__ int part[256];
+        const int per = (m + (int)blockDim.x - 1) / (int)blockDim.x;
+        const int b0 = (int)threadIdx.x * per, b1 = b0 + per < m ? b0 + per : m;
+        int run = 0;
+        for (int i = b0; i < b1; ++i) {
+            run += sflag[i];
+            sflag[i] = run;
+        }
+        part[threadIdx.x] = run;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int j = 0; j < (int)blockDim.x; ++j) {
+                const int v = part[j];
+                part[j] = acc;
+                acc += v;
+            }
+        }
+        __syncthreads();
+        for (int i = b0; i < b1; ++i) sflag[i] += part[threadIdx.x];
+        __syncthreads();
+    }
+    const int nu = ne > 0 ? sflag[ne - 1] : 0;
+    if (pass == 0) {
+        if (threadIdx.x == 0) tinfo[t] = nu;
+        return;
+    }
+    const int u0 = tptr[t];
+    for (int i = threadIdx.x; i < ne; i += blockDim.x)
+        if (i == 0 || sk[i] != sk[i - 1]) ucol[u0 + sflag[i] - 1] = sk[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+        const int c = scol[E0 + i];
+        int lo = 0, hi = ne - 1;  // first position of c in the sorted keys
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sk[mid] < c) lo = mid + 1;
+            else hi = mid;
+        }
+        lcol[E0 + i] = (unsigned short)(sflag[lo] - 1);
+    }
+}
+
+__global__ void clamp_count_kernel(int64_t n, const int *in, int *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i] > 0 ? in[i] : 0;
+}
+
 __global__ void renumber_kernel(int64_t n, const int *iperm, const int *src, int *dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[i] < 0 ? -1 : iperm[src[i]];
@@ -1513,6 +1601,118 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
     block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
 }
 
+// ---- tile-compressed rows: a tile is the TR rows of one pass of the (row, signal) lane groups.
+// Per tile, the values at the tile's distinct columns (sorted, listed at build time) are staged
+// in LDS once for every active signal: neighbouring rows share most columns, so each x value
+// (or s = r - alpha v) is read from memory once per tile instead of once per entry, and the
+// entries index the staged values through 16-bit local column numbers.
+template <int G, bool PG, class Gather>
+__device__ __forceinline__ void tile_row_accumulate(const int *rowptr, const unsigned short *lcol,
+                                                    const double *vals, int64_t row, int g, Gather X,
+                                                    Acc<PG> &acc) {
+    acc.a = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < (PG ? 3 : 1); ++d) acc.j[d] = make_double2(0.0, 0.0);
+    const int e1 = rowptr[row + 1];
+    int e = rowptr[row] + g;
+    for (; e + 3 * G < e1; e += 4 * G) {
+        const int l0 = lcol[e], l1 = lcol[e + G], l2 = lcol[e + 2 * G], l3 = lcol[e + 3 * G];
+        const double2 x0 = X(l0), x1 = X(l1), x2 = X(l2), x3 = X(l3);
+        acc_entry<PG>(vals, e, x0, acc);
+        acc_entry<PG>(vals, e + G, x1, acc);
+        acc_entry<PG>(vals, e + 2 * G, x2, acc);
+        acc_entry<PG>(vals, e + 3 * G, x3, acc);
+    }
+    for (; e < e1; e += G) acc_entry<PG>(vals, e, X(lcol[e]), acc);
+}
+
+// X(c, s): value of signal s at column c.  ucap: distinct columns the dynamic LDS holds.
+template <int G, bool PG, class Stage, class Emit>
+__device__ __forceinline__ void tile_spmv(const SolveArgs &a, int S, int TR, int ucap, const Lane &L, bool act,
+                                          const int *sact, Stage X, Emit emit) {
+    extern __shared__ double2 tile_x[];
+    const TileWalk W = xcd_tiles(a.n, TR);
+    for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
+        const int64_t tile = t0 / TR;
+        const int nu = a.tinfo[tile];
+        const bool comp = nu >= 0 && nu <= ucap;
+        __syncthreads();  // the previous tile's readers are done
+        if (comp) {
+            const int u0 = a.tptr[tile];
+            for (int i = threadIdx.x; i < nu * S; i += blockDim.x) {
+                const int u = i / S, s = i - u * S;
+                tile_x[i] = sact[s] ? X(a.ucol[u0 + u], s) : make_double2(0.0, 0.0);
+            }
+        }
+        __syncthreads();
+        const int64_t row = t0 + L.lrow;
+        if (!act || row >= W.end) continue;
+        Acc<PG> acc;
+        if (comp)
+            tile_row_accumulate<G, PG>(a.rowptr, a.lcol, a.vals, row, L.g,
+                                       [&](int lc) { return tile_x[lc * S + L.s]; }, acc);
+        else
+            row_accumulate<G, PG>(a.rowptr, a.col, a.vals, row, L.g, [&](int c) { return X(c, L.s); }, acc);
+        group_reduce_acc<G, PG>(acc);
+        emit(row, acc);
+    }
+}
+
+template <int G, bool PG>
+__global__ void __launch_bounds__(kThreads) spmv_v_tile_kernel(SolveArgs a, int S, int TR, int ucap, int it) {
+    if (*a.all_done) return;
+    __shared__ int sact[kMaxSignals];
+    if ((int)threadIdx.x < S) sact[threadIdx.x] = !a.st[threadIdx.x].done;
+    __syncthreads();
+    const Lane L = spmv_lane<G>(S);
+    const bool act = sact[L.s];
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    const double2 *p = a.p;
+    tile_spmv<G, PG>(
+        a, S, TR, ucap, L, act, sact, [&](int c, int s) { return p[(int64_t)c * S + s]; },
+        [&](int64_t row, const Acc<PG> &acc) {
+            if (L.g != 0) return;
+            const int64_t idx = row * S + L.s;
+            const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, L.s);
+            a.v[idx] = y;
+            const double2 h = a.rh[idx];
+            red[0] += h.x * y.x + h.y * y.y;
+            red[1] += h.x * y.y - h.y * y.x;
+        });
+    block_signal_partials(red, L.g == 0, L.lrow * S + L.s, TR * S, S, a.partV);
+}
+
+template <int G, bool PG>
+__global__ void __launch_bounds__(kThreads) spmv_t_tile_kernel(SolveArgs a, int S, int TR, int ucap, int nparts,
+                                                               int it) {
+    if (*a.all_done) return;
+    __shared__ double2 salpha[kMaxSignals];
+    __shared__ int sact[kMaxSignals];
+    spmv_t_head(a, S, nparts, it, salpha, sact);
+    const Lane L = spmv_lane<G>(S);
+    const bool act = sact[L.s];
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    const double2 *r = a.r, *v = a.v;
+    tile_spmv<G, PG>(
+        a, S, TR, ucap, L, act, sact,
+        [&](int c, int s) {
+            const int64_t j = (int64_t)c * S + s;
+            return c_sub(r[j], c_mul(salpha[s], v[j]));
+        },
+        [&](int64_t row, const Acc<PG> &acc) {
+            if (L.g != 0) return;
+            const int64_t idx = row * S + L.s;
+            const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, L.s);
+            a.t[idx] = y;
+            const double2 sv = c_sub(r[idx], c_mul(salpha[L.s], v[idx]));
+            red[0] += y.x * sv.x + y.y * sv.y;
+            red[1] += y.x * sv.y - y.y * sv.x;
+            red[2] += y.x * y.x + y.y * y.y;
+            red[3] += sv.x * sv.x + sv.y * sv.y;
+        });
+    block_signal_partials(red, L.g == 0, L.lrow * S + L.s, TR * S, S, a.partT);
+}
+
 // ---- one wave per row, lane = signal (any S <= 64).  The row's column indices and values are
 // the same for every lane: with the row index made wave-uniform they load through the scalar
 // cache once per wave (s_load), the vector registers hold only the gathered x and the
@@ -2058,7 +2258,7 @@ void launch_diag_and_weights(int64_t n, const int *rowptr, const int *col, const
 SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
     if (S < 1 || S > kMaxSignals) throw Error(-1, "signals per launch must lie in [1, 64]");
     if (G < 1 || G > 32 || (G & (G - 1))) throw Error(-1, "lanes per row must be a power of two in [1, 32]");
-    if (mapping != kMapSL && mapping != kMapStaged && mapping != kMapSR && mapping != kMapWave)
+    if (mapping != kMapSL && mapping != kMapStaged && mapping != kMapSR && mapping != kMapWave && mapping != kMapTile)
         throw Error(-1, "unknown SpMV mapping");
     if (mapping == kMapWave && G != 1) throw Error(-1, "the wave-per-row mapping takes one lane per signal (lanes 1)");
     if (mapping == kMapSR && S > kMaxSRSignals)
@@ -2079,6 +2279,7 @@ SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
         g.rpb = kThreads / 64;
         g.spmv_threads = kThreads;
     }
+    if (mapping == kMapTile) g.tile_rows = g.rpb;
     g.sr_rpb = std::max(1, kThreads / G);
     g.sr_threads = g.sr_rpb * G;
     g.vec_rpb = std::max(1, kThreads / S);
@@ -2156,6 +2357,36 @@ void launch_solver_order(int64_t n, int64_t nnz, const int *dof_vert, const int 
     check_launch();
     (void)nnz;
     DMRI_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+}
+void build_tile_compression(int64_t n, int64_t nnz, const int *srow, const int *scol, int TR, TileCompression &tc,
+                            hipStream_t s) {
+    const int64_t nt = (n + TR - 1) / TR;
+    tc.TR = TR;
+    tc.tinfo.alloc(nt);
+    tc.tptr.alloc(nt + 1);
+    tc.lcol.alloc(nnz);
+    tile_compress_kernel<<<(unsigned)nt, 256, 0, s>>>(0, n, TR, srow, scol, tc.tinfo.p, nullptr, nullptr, nullptr);
+    check_launch();
+    DevArray<int> cnt;
+    cnt.alloc(nt);
+    clamp_count_kernel<<<grid_for(nt, 256), 256, 0, s>>>(nt, tc.tinfo.p, cnt.p);
+    check_launch();
+    exclusive_scan(cnt.p, tc.tptr.p, nt, s);
+    int total = 0;
+    DMRI_HIP(hipMemcpyAsync(&total, tc.tptr.p + nt, sizeof(int), hipMemcpyDeviceToHost, s));
+    DMRI_HIP(hipStreamSynchronize(s));
+    tc.nunique = total;
+    {
+        std::vector<int> ti(nt);
+        DMRI_HIP(hipMemcpy(ti.data(), tc.tinfo.p, sizeof(int) * nt, hipMemcpyDeviceToHost));
+        tc.maxnu = 0;
+        for (int v : ti) tc.maxnu = std::max(tc.maxnu, v);
+    }
+    tc.ucol.alloc(std::max(total, 1));
+    tile_compress_kernel<<<(unsigned)nt, 256, 0, s>>>(1, n, TR, srow, scol, tc.tinfo.p, tc.tptr.p, tc.ucol.p,
+                                                       tc.lcol.p);
+    check_launch();
+    DMRI_HIP(hipStreamSynchronize(s));
 }
 void launch_gather_double(int64_t n, const int *map, const double *src, double *dst, hipStream_t s) {
     if (n <= 0) return;
@@ -2291,8 +2522,21 @@ void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
 }
 // The SpMV kernels go through hipExtLaunchKernelGGL so that e0/e1 (when not null) take the start
 // and end timestamps of the kernel's own dispatch: no marker packet, hence no extra gap.
+// dynamic LDS of a tile kernel: the staged values of up to ucap distinct columns
+static void tile_lds(const SolveGeom &g, int &ucap, uint32_t &bytes) {
+    ucap = kTileLdsBytes / (16 * g.S);
+    if (g.tile_ucap > 0 && g.tile_ucap < ucap) ucap = g.tile_ucap;
+    bytes = (uint32_t)(ucap * 16 * g.S);
+}
 void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if (g.mapping == kMapWave) {
+    if (g.mapping == kMapTile) {
+        int ucap;
+        uint32_t lds;
+        tile_lds(g, ucap, lds);
+        DMRI_DISPATCH_G(g.G, a.per_gdir,
+                        hipExtLaunchKernelGGL((spmv_v_tile_kernel<GG, PP>), dim3(g.grid), dim3(g.spmv_threads), lds,
+                                              s, e0, e1, 0, a, g.S, g.tile_rows, ucap, it));
+    } else if (g.mapping == kMapWave) {
         if (a.per_gdir)
             hipExtLaunchKernelGGL((spmv_v_wave_kernel<true>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1, 0, a,
                                   a.rowptr, a.col, a.vals, g.S, it);
@@ -2317,7 +2561,14 @@ void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
     }
 }
 void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if (g.mapping == kMapWave) {
+    if (g.mapping == kMapTile) {
+        int ucap;
+        uint32_t lds;
+        tile_lds(g, ucap, lds);
+        DMRI_DISPATCH_G(g.G, a.per_gdir,
+                        hipExtLaunchKernelGGL((spmv_t_tile_kernel<GG, PP>), dim3(g.grid), dim3(g.spmv_threads), lds,
+                                              s, e0, e1, 0, a, g.S, g.tile_rows, ucap, (int)g.grid, it));
+    } else if (g.mapping == kMapWave) {
         if (a.per_gdir)
             hipExtLaunchKernelGGL((spmv_t_wave_kernel<true>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1, 0, a,
                                   a.rowptr, a.col, a.vals, g.S, (int)g.grid, it);

@@ -268,15 +268,15 @@ class Context:
                                       ctypes.byref(st)))
         return sig.view(np.complex128), {k: getattr(st, k) for k, _ in _Stats._fields_}
 
-    MAPPINGS = {1: "spread", 2: "spread-staged", 3: "registers", 4: "wave-per-row"}
+    MAPPINGS = {1: "spread", 2: "spread-staged", 3: "registers", 4: "wave-per-row", 5: "tile-compressed"}
 
     def tune(self, nsig: int, per_gdir: bool = False):
         """Time every (mapping, lanes) candidate; returns ((mapping, lanes), {(mapping, lanes): us})."""
         bl, bm = ctypes.c_int32(), ctypes.c_int32()
-        us = np.zeros(24)
+        us = np.zeros(30)
         _check(LIB.dmri_tune(self._h, int(nsig), 1 if per_gdir else 0, ctypes.byref(bl), ctypes.byref(bm),
                              _ptr(us, ctypes.c_double)))
-        times = {(m, l): us[(m - 1) * 6 + i] for m in (1, 2, 3, 4) for i, l in enumerate((1, 2, 4, 8, 16, 32))
+        times = {(m, l): us[(m - 1) * 6 + i] for m in (1, 2, 3, 4, 5) for i, l in enumerate((1, 2, 4, 8, 16, 32))
                  if us[(m - 1) * 6 + i] >= 0}
         return (bm.value, bl.value), times
 

@@ -163,7 +163,9 @@ typedef struct {
                               (lanes * signals <= 64), 2 = the same with rows staged in LDS,
                               3 = signals kept in registers (at most 8 signals per launch group),
                               4 = one wave per row, lane = signal, row data through the scalar
-                              cache (lanes_per_row 1; for large launch groups) */
+                              cache (lanes_per_row 1; for large launch groups),
+                              5 = mapping 1 over tiles whose distinct columns are staged once
+                              in LDS (lanes * signals <= 64) */
     int32_t reserved;
 } dmri_solver_opts;
 
@@ -195,8 +197,8 @@ int dmri_solve(dmri_ctx *ctx, const dmri_seq *seq, const dmri_solver_opts *opts,
 int dmri_solve_batched(dmri_ctx *ctx, int32_t nsig, const double *q, const double *gdir, const dmri_seq *seq,
                        const dmri_solver_opts *opts, double *sig_out, dmri_stats *stats);
 
-/* Autotuner: time the two SpMV kernels of a Krylov iteration for every valid (mapping 1..4,
- * lanes per row 1..32) on the assembled pattern (us[24] = [mapping-1][log2 lanes] microseconds,
+/* Autotuner: time the two SpMV kernels of a Krylov iteration for every valid (mapping 1..5,
+ * lanes per row 1..32) on the assembled pattern (us[30] = [mapping-1][log2 lanes] microseconds,
  * -1 = not valid) and keep the fastest for later solves of this context whose options leave
  * mapping and lanes_per_row at 0.  dmri_set_tuning installs a stored choice (e.g. from a tuning
  * database) without timing.  No reference counterpart (the reference has no GPU kernels);

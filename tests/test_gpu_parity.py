@@ -195,7 +195,7 @@ def test_autotuner_picks_valid_choice_and_results_agree():
         best, times = ctx.tune(nsig, pg)
         assert best in times and all(t > 0 for t in times.values())
         m, l = best
-        assert (m == 3 and nsig <= 8) or (m in (1, 2) and l * nsig <= 64) or (m == 4 and l == 1)
+        assert (m == 3 and nsig <= 8) or (m in (1, 2, 5) and l * nsig <= 64) or (m == 4 and l == 1)
     fA, fB = _pgse_gcloud(10600.0, 43100.0, 400.0)
     q = O.q_from_b_pgse(1000.0, 10600.0, 43100.0)
     s_tuned, _ = ctx.solve(fA, fB, 400.0, q, (0, 1, 0), rtol=1e-13)
@@ -204,7 +204,8 @@ def test_autotuner_picks_valid_choice_and_results_agree():
     assert abs(s_tuned - s_set) <= 1e-11 * abs(s_set)
 
 
-@pytest.mark.parametrize("mapping,lanes", [(1, 1), (1, 4), (2, 1), (2, 4), (3, 1), (3, 2), (3, 8), (4, 1)])
+@pytest.mark.parametrize("mapping,lanes", [(1, 1), (1, 4), (2, 1), (2, 4), (3, 1), (3, 2), (3, 8), (4, 1), (5, 1),
+                                           (5, 2), (5, 8)])
 @pytest.mark.parametrize("pg", [False, True])
 def test_spmv_mappings_agree_batched(mapping, lanes, pg):
     mesh, kw = MESHES["cyl2"]
@@ -265,7 +266,21 @@ def test_wave_mapping_full_launch_group_matches_oracle():
     got, _ = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13, mapping=4, lanes=1)
     ref, _ = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13, mapping=1, lanes=1)
     assert np.all(np.abs(got - ref) <= 1e-11 * np.abs(ref))
+    til, _ = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13, mapping=5, lanes=1)
+    assert np.all(np.abs(til - ref) <= 1e-11 * np.abs(ref))
     S = O.System(mesh.xyz, mesh.tets, mesh.phase, D=kw["D"], kappa=kw["kappa"])
     for j in (0, 17, 63):
         s, _, _ = S.solve(gd[j], qs[j], k, fA, fB, rtol=1e-13)
         assert abs(got[j].real - s) <= 1e-9 * abs(s)
+
+
+def test_tile_mapping_single_signal_matches_oracle():
+    """One signal through the tile-compressed kernels (8 lanes per row, 32-row tiles)."""
+    mesh, kw = MESHES["cyl1"]
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    fA, fB = _pgse_gcloud(10600.0, 43100.0, 400.0)
+    q = O.q_from_b_pgse(1500.0, 10600.0, 43100.0)
+    got, _ = ctx.solve(fA, fB, 400.0, q, (0.2, 1.0, -0.3), rtol=1e-13, mapping=5, lanes=8)
+    S = O.System(mesh.xyz, mesh.tets, None, D=kw["D"])
+    ref, _, _ = S.solve((0.2, 1.0, -0.3), q, 400.0, fA, fB, rtol=1e-13)
+    assert abs(got.real - ref) <= 1e-9 * abs(ref)
