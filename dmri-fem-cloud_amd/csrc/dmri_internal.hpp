@@ -75,7 +75,7 @@ constexpr int kMaxGrid = 512;      // blocks of the vector kernels (grid-stride)
 constexpr int kMaxGridSpmv = 2048; // blocks of the SpMV kernels: 8 blocks of 4 waves on each of 256 CUs
 constexpr int kMaxSignals = 64; // signals per batched launch
 constexpr int kNV = 4;          // doubles per (block, signal) partial slot
-constexpr int kStageNnzPerRow = 24; // LDS budget of a staged SpMV tile, entries per row
+constexpr int kStageNnzPerRow = 18; // LDS budget of a staged SpMV tile, entries per row
 constexpr int kStageBatch = 4;      // independent loads per thread per batch when staging a tile
 
 // SpMV mappings: signals spread over lanes (entries from global memory, or staged in LDS per

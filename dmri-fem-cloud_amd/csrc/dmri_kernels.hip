@@ -73,11 +73,19 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
     __syncthreads();
 }
 
+// One LDS scratch of kNV x kThreads doubles per kernel, shared by the reductions below (the
+// kernel's prologue reduction and its epilogue partials use it at different times), so a kernel
+// that does both holds 8 KB of it instead of 16 KB and fits more blocks per CU.
+__device__ __forceinline__ double (&reduce_scratch())[kNV][kThreads] {
+    __shared__ double buf[kNV][kThreads];
+    return buf;
+}
+
 // Per-signal block partials.  slot = (row in block)*S + signal; every slot in [0, nslots)
 // is written by exactly one thread (writer == true).
 __device__ __forceinline__ void block_signal_partials(const double (&v)[kNV], bool writer, int slot, int nslots,
                                                       int S, double *parts) {
-    __shared__ double buf[kNV][kThreads];
+    double(&buf)[kNV][kThreads] = reduce_scratch();
     if (writer) {
 #pragma unroll
         for (int k = 0; k < kNV; ++k) buf[k][slot] = v[k];
@@ -104,7 +112,7 @@ __device__ __forceinline__ void signal_totals(const double *parts, const double 
         __syncthreads();
         return;
     }
-    __shared__ double buf[kNV][kThreads];
+    double(&buf)[kNV][kThreads] = reduce_scratch();
     const int J = blockDim.x / S;
     const int j = threadIdx.x / S, s = threadIdx.x % S;
     if (j < J) {
