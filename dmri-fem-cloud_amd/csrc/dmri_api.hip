@@ -30,7 +30,7 @@ T read_scalar(const T *d, hipStream_t s) {
 constexpr int kEventPool = 512;
 
 struct SolverWork {
-    DevArray<double2> x, r, rh, p, v, t, b, xprev, xprev2, ybc, bper;
+    DevArray<double2> x, r, rh, p, v, t, b, xprev, xprev2, ybc, bper, ph, sh;
     DevArray<double> sc, vals, bre, part, tot, coef, coefP, sig, spart;
     DevArray<KrylovState> st;
     DevArray<int> all_done, mask;
@@ -546,6 +546,7 @@ struct Stepper {
     dmri_stats *st = nullptr;
     double2 *xc = nullptr, *xp = nullptr, *xq = nullptr;  // u^n, u^(n-1), u^(n-2)
     int guess_order = 2;
+    bool poly = true;  // degree-1 polynomial preconditioner
     double omt = 0.5;
     int chunk = 8;
     std::vector<KrylovState> zero_st;
@@ -572,6 +573,11 @@ struct Stepper {
         } else {
             if (opts->mapping > 0) mapping = opts->mapping;
             if (opts->lanes_per_row > 0) lanes = opts->lanes_per_row;
+        }
+        poly = opts->preconditioner == 0;
+        if (poly && (mapping == kMapSR || mapping == kMapTile)) {  // kernels without the polynomial ops
+            mapping = kMapStaged;
+            while (lanes > 1 && lanes * S > 64) lanes >>= 1;
         }
         geo = make_geom(n, lanes, S, mapping);
         tiles = tiles_for(c, geo);
@@ -654,11 +660,21 @@ struct Stepper {
         a.all_done = w.all_done.p;
         a.bper = c->periodic ? w.bper.p : nullptr;
         attach_tiles(a, tiles);
+        if (poly) {
+            const size_t nS = (size_t)n * S;
+            if (w.ph.n != nS) w.ph.alloc(nS);
+            if (w.sh.n != nS) w.sh.alloc(nS);
+            a.ph = w.ph.p;
+            a.sh = w.sh.p;
+        }
         a.rtol = opts->rtol;
         a.atol = opts->atol;
         a.maxit = opts->maxit;
+        // timed launches: v = A p (p gathered, r^ read, v written: 48 B per row and signal) and
+        // t = A s (plain: r, v gathered, t written: 48 B; preconditioned: sh gathered, r, v read at
+        // the own row, t written: 64 B), so 48 or 56 B on average
         st->spmv_bytes = (double)c->nnz * (4.0 + 8.0 * (per_gdir ? 4 : 2)) + 4.0 * (double)(n + 1) +
-                         48.0 * (double)n * S;
+                         (poly ? 56.0 : 48.0) * (double)n * S;
         prof = opts->profile != 0;
         chunk = 8;
         zero_st.assign(S, KrylovState{});
@@ -685,9 +701,20 @@ struct Stepper {
                 // the timestamps of their own dispatch (no marker packets between kernels)
                 const bool timed = prof && it < opts->profile;
                 launch_bicg_p(geo, a, it, s);
-                launch_spmv_v(geo, a, it, s, timed ? c->ev0[2 * k] : nullptr, timed ? c->ev1[2 * k] : nullptr);
-                launch_spmv_t(geo, a, it, s, timed ? c->ev0[2 * k + 1] : nullptr,
-                              timed ? c->ev1[2 * k + 1] : nullptr);
+                if (poly) {
+                    // ph = 2p - Ap, v = A ph; sh = 2s - As, t = A sh (right preconditioning)
+                    launch_spmv_v(geo, a, it, s, nullptr, nullptr, kOpPre);
+                    SolveArgs av = a;
+                    av.p = a.ph;
+                    launch_spmv_v(geo, av, it, s, timed ? c->ev0[2 * k] : nullptr, timed ? c->ev1[2 * k] : nullptr);
+                    launch_spmv_t(geo, a, it, s, nullptr, nullptr, kOpPre);
+                    launch_spmv_t(geo, a, it, s, timed ? c->ev0[2 * k + 1] : nullptr,
+                                  timed ? c->ev1[2 * k + 1] : nullptr, kOpTP);
+                } else {
+                    launch_spmv_v(geo, a, it, s, timed ? c->ev0[2 * k] : nullptr, timed ? c->ev1[2 * k] : nullptr);
+                    launch_spmv_t(geo, a, it, s, timed ? c->ev0[2 * k + 1] : nullptr,
+                                  timed ? c->ev1[2 * k + 1] : nullptr);
+                }
                 ntimed += timed ? 1 : 0;
                 launch_bicg_update(geo, a, it, s);
             }

@@ -85,6 +85,9 @@ constexpr int kMapStaged = 2;
 constexpr int kMapSR = 3;
 constexpr int kMapWave = 4;  // one wave per row, lane = signal; row data through the scalar cache
 constexpr int kMapTile = 5;  // tiles of rpb rows: the tile's distinct columns staged once in LDS
+// SpMV kernel operations: v = A p / t = A s (plain BiCGStab), the degree-1 polynomial
+// preconditioner z -> 2z - A z applied to p or s, and t = A sh (preconditioned BiCGStab)
+constexpr int kOpV = 0, kOpT = 0, kOpPre = 1, kOpTP = 2;
 constexpr int kTileMaxEntries = 4096;  // entries of a tile the compression build sorts in LDS
 constexpr int kTileLdsBytes = 32768;   // LDS cap of the staged x values of one tile (distinct columns x S)
 constexpr int kMaxSRSignals = 8;
@@ -235,6 +238,7 @@ struct SolveArgs {
     const unsigned short *lcol;
     double gc0, gc1, gc2;
     const double2 *bper;            // weak pseudo-periodic right-hand side of this step, or null
+    double2 *ph, *sh;               // preconditioned p and s (polynomial preconditioner), or null
     double *partU, *partV, *partT;  // [grid][S][kNV] each
     double *totU, *totV, *totT;     // [S][kNV] finalize outputs, or null: consumers reduce the partials
     KrylovState *st;
@@ -247,10 +251,13 @@ struct SolveArgs {
 // gc2 = 0); 3: b only
 void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const int *mask, hipStream_t s);
 void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
+// op: kOpV / kOpPre for the v-kernel (kOpPre writes ph from p), kOpT / kOpPre / kOpTP for the
+// t-kernel (kOpPre writes sh from s = r - alpha v; kOpTP gathers sh).  Mappings 1, 2 and 4 take
+// every op, mappings 3 and 5 only kOpV / kOpT.
 void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
-                   hipEvent_t e1 = nullptr);
+                   hipEvent_t e1 = nullptr, int op = kOpV);
 void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
-                   hipEvent_t e1 = nullptr);
+                   hipEvent_t e1 = nullptr, int op = kOpT);
 void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
 // per-signal sum_i w_i sc_i x[i*S+s] into out[2*s..2*s+1]
 void launch_signal(int64_t n, int S, const double *w, const double *sc, const double2 *x, double *part, double *out,

@@ -1493,8 +1493,10 @@ __global__ void __launch_bounds__(kThreads) bicg_p_kernel(SolveArgs a, int S, in
     }
 }
 
-// v = A p ; partials V: [<r^, v>.re, <r^, v>.im, 0, 0]
-template <int G, bool PG>
+// OP kOpV:  v = A p ; partials V: [<r^, v>.re, <r^, v>.im, 0, 0]
+// OP kOpPre: ph = 2 p - A p (the degree-1 Neumann polynomial preconditioner of the scaled
+//            system, whose diagonal is 1), no partials
+template <int G, bool PG, int OP>
 __global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
     if (*a.all_done) return;
     const Lane L = spmv_lane<G>(S);
@@ -1509,12 +1511,17 @@ __global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, in
             if (L.g != 0) return;
             const int64_t idx = row * S_ + s;
             const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
+            if (OP == kOpPre) {
+                const double2 z = p[idx];
+                a.ph[idx] = make_double2(2.0 * z.x - y.x, 2.0 * z.y - y.y);
+                return;
+            }
             a.v[idx] = y;
             const double2 h = a.rh[idx];
             red[0] += h.x * y.x + h.y * y.y;
             red[1] += h.x * y.y - h.y * y.x;
         });
-    block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
+    if (OP == kOpV) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
 }
 
 // Head of every t = A s kernel: alpha_s = rho_s / <r^, v>_s per signal into salpha, the
@@ -1549,64 +1556,46 @@ __device__ __forceinline__ void spmv_t_head(const SolveArgs &a, int S, int npart
     __syncthreads();
 }
 
-// alpha = rho / <r^, v> ; t = A (r - alpha v) with s = r - alpha v formed on the fly;
-// partials T: [<t, s>.re, <t, s>.im, |t|^2, |s|^2]
-template <int G, bool PG>
+// alpha = rho / <r^, v> and s = r - alpha v (formed for the gathered columns), then
+// OP kOpT:   t = A s ; partials T: [<t, s>.re, <t, s>.im, |t|^2, |s|^2]
+// OP kOpPre: sh = 2 s - A s (the polynomial preconditioner applied to s), no partials
+// OP kOpTP:  t = A sh with sh gathered from a.sh; partials T as for kOpT (s at the own row)
+template <int G, bool PG, int OP>
 __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, int rpb, int TR, int nparts, int it) {
     if (*a.all_done) return;
-    __shared__ double tot[kMaxSignals][kNV];
     __shared__ double2 salpha[kMaxSignals];
     __shared__ int sact[kMaxSignals];
-    signal_totals(a.partV, a.totV, nparts, S, tot);
-    if ((int)threadIdx.x < S) {
-        const int s = threadIdx.x;
-        KrylovState &st = a.st[s];
-        int act = !st.done;
-        const int local = it - st.it0;
-        if (act) {
-            const double2 rv = make_double2(tot[s][0], tot[s][1]);
-            if (rv.x == 0.0 && rv.y == 0.0) {
-                if (blockIdx.x == 0) {
-                    st.done = 1;
-                    st.breakdown = 1;
-                    st.conv = 0;
-                    st.iters = it;
-                }
-                act = 0;
-            } else {
-                const double2 al = c_div(st.rho[local & 1], rv);
-                salpha[s] = al;
-                if (blockIdx.x == 0) st.alpha[local & 1] = al;
-            }
-        }
-        sact[s] = act;
-    }
-    __syncthreads();
+    spmv_t_head(a, S, nparts, it, salpha, sact);
     const Lane L = spmv_lane<G>(S);
     const bool act = sact[L.s];
     const double2 alpha = salpha[L.s];
     const int S_ = S;
     const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
-    const double2 *r = a.r, *v = a.v;
+    const double2 *r = a.r, *v = a.v, *sh = a.sh;
     staged_spmv<G, PG>(
         a, rpb, TR, L, act,
         [&](int c) {
             const int64_t j = (int64_t)c * S_ + s;
+            if (OP == kOpTP) return sh[j];
             return c_sub(r[j], c_mul(alpha, v[j]));
         },
         [&](int64_t row, const Acc<PG> &acc) {
             if (L.g != 0) return;
             const int64_t idx = row * S_ + s;
             const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
-            a.t[idx] = y;
             const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
+            if (OP == kOpPre) {
+                a.sh[idx] = make_double2(2.0 * sv.x - y.x, 2.0 * sv.y - y.y);
+                return;
+            }
+            a.t[idx] = y;
             red[0] += y.x * sv.x + y.y * sv.y;
             red[1] += y.x * sv.y - y.y * sv.x;
             red[2] += y.x * y.x + y.y * y.y;
             red[3] += sv.x * sv.x + sv.y * sv.y;
         });
-    block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
+    if (OP != kOpPre) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
 }
 
 // ---- tile-compressed rows: a tile is the TR rows of one pass of the (row, signal) lane groups.
@@ -1758,7 +1747,7 @@ __device__ __forceinline__ void wave_rows(int64_t n, const int *__restrict__ row
     }
 }
 
-template <bool PG>
+template <bool PG, int OP>
 __global__ void __launch_bounds__(kThreads) spmv_v_wave_kernel(SolveArgs a, const int *__restrict__ rowptr,
                                                                const int *__restrict__ col,
                                                                const double *__restrict__ vals, int S, int it) {
@@ -1773,16 +1762,22 @@ __global__ void __launch_bounds__(kThreads) spmv_v_wave_kernel(SolveArgs a, cons
             if (!act) return;
             const int64_t idx = row * S + lane;
             const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
+            if (OP == kOpPre) {
+                const double2 z = p[idx];
+                a.ph[idx] = make_double2(2.0 * z.x - y.x, 2.0 * z.y - y.y);
+                return;
+            }
             a.v[idx] = y;
             const double2 h = a.rh[idx];
             red[0] += h.x * y.x + h.y * y.y;
             red[1] += h.x * y.y - h.y * y.x;
         });
-    block_signal_partials(red, lane < S, (int)(threadIdx.x >> 6) * S + lane, (int)(blockDim.x >> 6) * S, S,
-                          a.partV);
+    if (OP == kOpV)
+        block_signal_partials(red, lane < S, (int)(threadIdx.x >> 6) * S + lane, (int)(blockDim.x >> 6) * S, S,
+                              a.partV);
 }
 
-template <bool PG>
+template <bool PG, int OP>
 __global__ void __launch_bounds__(kThreads) spmv_t_wave_kernel(SolveArgs a, const int *__restrict__ rowptr,
                                                                const int *__restrict__ col,
                                                                const double *__restrict__ vals, int S, int nparts,
@@ -1795,26 +1790,32 @@ __global__ void __launch_bounds__(kThreads) spmv_t_wave_kernel(SolveArgs a, cons
     const bool act = lane < S && sact[lane];
     const double2 alpha = act ? salpha[lane] : make_double2(0.0, 0.0);
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
-    const double2 *r = a.r, *v = a.v;
+    const double2 *r = a.r, *v = a.v, *sh = a.sh;
     wave_rows<PG>(
         a.n, rowptr, col, vals, act,
         [&](int c) {
             const int64_t j = (int64_t)c * S + lane;
+            if (OP == kOpTP) return sh[j];
             return c_sub(r[j], c_mul(alpha, v[j]));
         },
         [&](int64_t row, const Acc<PG> &acc) {
             if (!act) return;
             const int64_t idx = row * S + lane;
             const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
-            a.t[idx] = y;
             const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
+            if (OP == kOpPre) {
+                a.sh[idx] = make_double2(2.0 * sv.x - y.x, 2.0 * sv.y - y.y);
+                return;
+            }
+            a.t[idx] = y;
             red[0] += y.x * sv.x + y.y * sv.y;
             red[1] += y.x * sv.y - y.y * sv.x;
             red[2] += y.x * y.x + y.y * y.y;
             red[3] += sv.x * sv.x + sv.y * sv.y;
         });
-    block_signal_partials(red, lane < S, (int)(threadIdx.x >> 6) * S + lane, (int)(blockDim.x >> 6) * S, S,
-                          a.partT);
+    if (OP != kOpPre)
+        block_signal_partials(red, lane < S, (int)(threadIdx.x >> 6) * S + lane, (int)(blockDim.x >> 6) * S, S,
+                              a.partT);
 }
 
 // ---- signals in registers: each lane takes entries g, g+G, ... of a row for all S <= 8
@@ -1995,7 +1996,11 @@ __global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int 
         for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total;
              i += (int64_t)gridDim.x * vrpb * S) {
             const double2 sv = c_sub(a.r[i], c_mul(alpha, a.v[i]));
-            a.x[i] = c_add(a.x[i], c_add(c_mul(alpha, a.p[i]), c_mul(omega, sv)));
+            // right preconditioning: the iterate moves along the preconditioned directions
+            if (a.ph)
+                a.x[i] = c_add(a.x[i], c_add(c_mul(alpha, a.ph[i]), c_mul(omega, a.sh[i])));
+            else
+                a.x[i] = c_add(a.x[i], c_add(c_mul(alpha, a.p[i]), c_mul(omega, sv)));
             const double2 rn = c_sub(sv, c_mul(omega, a.t[i]));
             a.r[i] = rn;
             const double2 h = a.rh[i];
@@ -2536,7 +2541,47 @@ static void tile_lds(const SolveGeom &g, int &ucap, uint32_t &bytes) {
     if (g.tile_ucap > 0 && g.tile_ucap < ucap) ucap = g.tile_ucap;
     bytes = (uint32_t)(ucap * 16 * g.S);
 }
-void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+template <int OP>
+static void spmv_v_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
+                            hipEvent_t e1) {
+    if (g.mapping == kMapWave) {
+        if (a.per_gdir)
+            hipExtLaunchKernelGGL((spmv_v_wave_kernel<true, OP>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1,
+                                  0, a, a.rowptr, a.col, a.vals, g.S, it);
+        else
+            hipExtLaunchKernelGGL((spmv_v_wave_kernel<false, OP>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1,
+                                  0, a, a.rowptr, a.col, a.vals, g.S, it);
+    } else {
+        const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
+        const uint32_t lds = TR > 0 ? (uint32_t)stage_lds_bytes(TR, a.per_gdir) : 0u;
+        DMRI_DISPATCH_G(g.G, a.per_gdir,
+                        hipExtLaunchKernelGGL((spmv_v_kernel<GG, PP, OP>), dim3(g.grid), dim3(g.spmv_threads), lds, s,
+                                              e0, e1, 0, a, g.S, g.rpb, TR, it));
+    }
+}
+template <int OP>
+static void spmv_t_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
+                            hipEvent_t e1) {
+    if (g.mapping == kMapWave) {
+        if (a.per_gdir)
+            hipExtLaunchKernelGGL((spmv_t_wave_kernel<true, OP>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1,
+                                  0, a, a.rowptr, a.col, a.vals, g.S, (int)g.grid, it);
+        else
+            hipExtLaunchKernelGGL((spmv_t_wave_kernel<false, OP>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1,
+                                  0, a, a.rowptr, a.col, a.vals, g.S, (int)g.grid, it);
+    } else {
+        const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
+        const uint32_t lds = TR > 0 ? (uint32_t)stage_lds_bytes(TR, a.per_gdir) : 0u;
+        DMRI_DISPATCH_G(g.G, a.per_gdir,
+                        hipExtLaunchKernelGGL((spmv_t_kernel<GG, PP, OP>), dim3(g.grid), dim3(g.spmv_threads), lds, s,
+                                              e0, e1, 0, a, g.S, g.rpb, TR, (int)g.grid, it));
+    }
+}
+
+void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                   int op) {
+    if (op != kOpV && (g.mapping == kMapSR || g.mapping == kMapTile))
+        throw Error(-1, "the polynomial preconditioner needs SpMV mapping 1, 2 or 4");
     if (g.mapping == kMapTile) {
         int ucap;
         uint32_t lds;
@@ -2544,31 +2589,25 @@ void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
         DMRI_DISPATCH_G(g.G, a.per_gdir,
                         hipExtLaunchKernelGGL((spmv_v_tile_kernel<GG, PP>), dim3(g.grid), dim3(g.spmv_threads), lds,
                                               s, e0, e1, 0, a, g.S, g.tile_rows, ucap, it));
-    } else if (g.mapping == kMapWave) {
-        if (a.per_gdir)
-            hipExtLaunchKernelGGL((spmv_v_wave_kernel<true>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1, 0, a,
-                                  a.rowptr, a.col, a.vals, g.S, it);
-        else
-            hipExtLaunchKernelGGL((spmv_v_wave_kernel<false>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1, 0,
-                                  a, a.rowptr, a.col, a.vals, g.S, it);
     } else if (g.mapping == kMapSR) {
         DMRI_DISPATCH_G(g.G, a.per_gdir,
                         hipExtLaunchKernelGGL((spmv_v_sr_kernel<GG, PP>), dim3(g.grid), dim3(g.sr_threads), 0, s, e0,
                                               e1, 0, a, g.S, g.sr_rpb, it));
+    } else if (op == kOpPre) {
+        spmv_v_dispatch<kOpPre>(g, a, it, s, e0, e1);
     } else {
-        const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
-        const uint32_t lds = TR > 0 ? (uint32_t)stage_lds_bytes(TR, a.per_gdir) : 0u;
-        DMRI_DISPATCH_G(g.G, a.per_gdir,
-                        hipExtLaunchKernelGGL((spmv_v_kernel<GG, PP>), dim3(g.grid), dim3(g.spmv_threads), lds, s, e0,
-                                              e1, 0, a, g.S, g.rpb, TR, it));
+        spmv_v_dispatch<kOpV>(g, a, it, s, e0, e1);
     }
     check_launch();
-    if (a.totV) {
+    if (a.totV && op == kOpV) {
         finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partV, g.grid, g.S, a.totV);
         check_launch();
     }
 }
-void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                   int op) {
+    if (op != kOpT && (g.mapping == kMapSR || g.mapping == kMapTile))
+        throw Error(-1, "the polynomial preconditioner needs SpMV mapping 1, 2 or 4");
     if (g.mapping == kMapTile) {
         int ucap;
         uint32_t lds;
@@ -2576,26 +2615,19 @@ void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
         DMRI_DISPATCH_G(g.G, a.per_gdir,
                         hipExtLaunchKernelGGL((spmv_t_tile_kernel<GG, PP>), dim3(g.grid), dim3(g.spmv_threads), lds,
                                               s, e0, e1, 0, a, g.S, g.tile_rows, ucap, (int)g.grid, it));
-    } else if (g.mapping == kMapWave) {
-        if (a.per_gdir)
-            hipExtLaunchKernelGGL((spmv_t_wave_kernel<true>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1, 0, a,
-                                  a.rowptr, a.col, a.vals, g.S, (int)g.grid, it);
-        else
-            hipExtLaunchKernelGGL((spmv_t_wave_kernel<false>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1, 0,
-                                  a, a.rowptr, a.col, a.vals, g.S, (int)g.grid, it);
     } else if (g.mapping == kMapSR) {
         DMRI_DISPATCH_G(g.G, a.per_gdir,
                         hipExtLaunchKernelGGL((spmv_t_sr_kernel<GG, PP>), dim3(g.grid), dim3(g.sr_threads), 0, s, e0,
                                               e1, 0, a, g.S, g.sr_rpb, (int)g.grid, it));
+    } else if (op == kOpPre) {
+        spmv_t_dispatch<kOpPre>(g, a, it, s, e0, e1);
+    } else if (op == kOpTP) {
+        spmv_t_dispatch<kOpTP>(g, a, it, s, e0, e1);
     } else {
-        const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
-        const uint32_t lds = TR > 0 ? (uint32_t)stage_lds_bytes(TR, a.per_gdir) : 0u;
-        DMRI_DISPATCH_G(g.G, a.per_gdir,
-                        hipExtLaunchKernelGGL((spmv_t_kernel<GG, PP>), dim3(g.grid), dim3(g.spmv_threads), lds, s, e0,
-                                              e1, 0, a, g.S, g.rpb, TR, (int)g.grid, it));
+        spmv_t_dispatch<kOpT>(g, a, it, s, e0, e1);
     }
     check_launch();
-    if (a.totT) {
+    if (a.totT && op != kOpPre) {
         finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partT, g.grid, g.S, a.totT);
         check_launch();
     }

@@ -47,7 +47,7 @@ class _Seq(ctypes.Structure):
 class _Opts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxit", ctypes.c_int32),
                 ("profile", ctypes.c_int32), ("lanes_per_row", ctypes.c_int32), ("initial_guess", ctypes.c_int32),
-                ("mapping", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("mapping", ctypes.c_int32), ("preconditioner", ctypes.c_int32)]
 
 
 class _Stats(ctypes.Structure):
@@ -220,7 +220,7 @@ class Context:
     # -- solve --------------------------------------------------------------------------
     @staticmethod
     def _seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes, guess=0,
-                  mapping=0, F_explicit=None):
+                  mapping=0, F_explicit=None, precond=0):
         fA = np.ascontiguousarray(f_implicit, dtype=np.float64)
         fB = np.ascontiguousarray(f_explicit, dtype=np.float64)
         if fA.shape != fB.shape:
@@ -241,12 +241,14 @@ class Context:
         o.lanes_per_row = int(lanes)
         o.initial_guess = int(guess)
         o.mapping = int(mapping)
+        o.preconditioner = int(precond)
         return s, o, (fA, fB, FB)
 
     def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None):
+              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0):
+        """precond: 0 = Jacobi + degree-1 polynomial (default), 1 = Jacobi only."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes,
-                                    guess, mapping, F_explicit)
+                                    guess, mapping, F_explicit, precond)
         sig = np.zeros(2)
         st = _Stats()
         _check(LIB.dmri_solve(self._h, ctypes.byref(s), ctypes.byref(o), _ptr(sig, ctypes.c_double),
@@ -254,13 +256,13 @@ class Context:
         return complex(sig[0], sig[1]), {k: getattr(st, k) for k, _ in _Stats._fields_}
 
     def solve_batched(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-                      profile=False, lanes=0, guess=0, mapping=0, F_explicit=None):
+                      profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0):
         """Signals for every (q[s], gdir[s]) on the shared pattern; returns complex array (nsig,)."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(gdirs, dtype=np.float64).reshape(-1, 3),
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
-                                    lanes, guess, mapping, F_explicit)
+                                    lanes, guess, mapping, F_explicit, precond)
         sig = np.zeros(2 * qs.shape[0])
         st = _Stats()
         _check(LIB.dmri_solve_batched(self._h, qs.shape[0], _ptr(qs, ctypes.c_double), _ptr(g, ctypes.c_double),
@@ -284,10 +286,10 @@ class Context:
         _check(LIB.dmri_set_tuning(self._h, int(nsig), 1 if per_gdir else 0, int(lanes), int(mapping)))
 
     def adjoint(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-                checkpoint_stride=0, lanes=0, guess=0, mapping=0):
+                checkpoint_stride=0, lanes=0, guess=0, mapping=0, precond=0):
         """Signal and its adjoint gradients (see dmri_adjoint): returns (signal, grads dict, stats)."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, False, lanes,
-                                    guess, mapping)
+                                    guess, mapping, None, precond)
         ao = _AdjOpts()
         ao.checkpoint_stride = int(checkpoint_stride)
         n = s.nsteps
