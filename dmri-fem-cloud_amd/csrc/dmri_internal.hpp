@@ -221,6 +221,16 @@ void launch_renumber(int64_t n, const int *iperm, const int *src, int *dst, hipS
 void launch_unscale_perm(int64_t n, int S, int sig, const int *perm, const double *sc, const double2 *x, double2 *u,
                          hipStream_t s);
 void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s);
+// Gershgorin bound max_i sum_j |a_ij| of the real implicit part of the scaled operator (the
+// first value of every vals entry, vw values per entry) into out[0]; part >= kMaxGrid doubles
+void launch_gershgorin(int64_t n, const int *rowptr, const double *vals, int vw, double *part, double *out,
+                       hipStream_t s);
+// y = Re(A) x for the real implicit part (vw values per entry), and xy = x.*y, yy = y.*y
+void launch_real_spmv(int64_t n, const int *rowptr, const int *col, const double *vals, int vw, const double *x,
+                      double *y, double *xy, double *yy, hipStream_t s);
+// x = y * scale, and the deterministic start vector of the power iteration
+void launch_scale_vec(int64_t n, const double *y, double scale, double *x, hipStream_t s);
+void launch_power_start(int64_t n, double *x, hipStream_t s);
 
 struct SolveArgs {
     int64_t n;
@@ -239,6 +249,7 @@ struct SolveArgs {
     double gc0, gc1, gc2;
     const double2 *bper;            // weak pseudo-periodic right-hand side of this step, or null
     double2 *ph, *sh;               // preconditioned p and s (polynomial preconditioner), or null
+    double pc;                      // polynomial scale c: z -> c (2 z - c A z)
     double *partU, *partV, *partT;  // [grid][S][kNV] each
     double *totU, *totV, *totT;     // [S][kNV] finalize outputs, or null: consumers reduce the partials
     KrylovState *st;

@@ -167,6 +167,37 @@ __global__ void __launch_bounds__(kThreads) sum_part_kernel(const double *x, int
 
 inline unsigned grid_for(int64_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
 
+// Largest absolute row sum of the real implicit part (block maxima, then one block over them).
+__global__ void __launch_bounds__(kThreads) row_absmax_kernel(int64_t n, const int *rowptr, const double *vals,
+                                                              int vw, double *parts) {
+    __shared__ double sm[kThreads];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) s += fabs(vals[(int64_t)vw * e]);
+        m = fmax(m, s);
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x] = sm[0];
+}
+__global__ void __launch_bounds__(kThreads) max_final_kernel(const double *parts, int np, double *out) {
+    __shared__ double sm[kThreads];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) m = fmax(m, parts[i]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sm[0];
+}
+
 // --------------------------------------------------------------------------------------
 // exclusive scan (int32, non-negative counts)
 // --------------------------------------------------------------------------------------
@@ -1494,8 +1525,8 @@ __global__ void __launch_bounds__(kThreads) bicg_p_kernel(SolveArgs a, int S, in
 }
 
 // OP kOpV:  v = A p ; partials V: [<r^, v>.re, <r^, v>.im, 0, 0]
-// OP kOpPre: ph = 2 p - A p (the degree-1 Neumann polynomial preconditioner of the scaled
-//            system, whose diagonal is 1), no partials
+// OP kOpPre: ph = c (2 p - c A p) (a degree-1 polynomial preconditioner of the scaled system,
+//            whose diagonal is 1; c from a bound of its spectrum), no partials
 template <int G, bool PG, int OP>
 __global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
     if (*a.all_done) return;
@@ -1513,7 +1544,7 @@ __global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, in
             const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
             if (OP == kOpPre) {
                 const double2 z = p[idx];
-                a.ph[idx] = make_double2(2.0 * z.x - y.x, 2.0 * z.y - y.y);
+                a.ph[idx] = make_double2(a.pc * (2.0 * z.x - a.pc * y.x), a.pc * (2.0 * z.y - a.pc * y.y));
                 return;
             }
             a.v[idx] = y;
@@ -1558,7 +1589,7 @@ __device__ __forceinline__ void spmv_t_head(const SolveArgs &a, int S, int npart
 
 // alpha = rho / <r^, v> and s = r - alpha v (formed for the gathered columns), then
 // OP kOpT:   t = A s ; partials T: [<t, s>.re, <t, s>.im, |t|^2, |s|^2]
-// OP kOpPre: sh = 2 s - A s (the polynomial preconditioner applied to s), no partials
+// OP kOpPre: sh = c (2 s - c A s) (the polynomial preconditioner applied to s), no partials
 // OP kOpTP:  t = A sh with sh gathered from a.sh; partials T as for kOpT (s at the own row)
 template <int G, bool PG, int OP>
 __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, int rpb, int TR, int nparts, int it) {
@@ -1586,7 +1617,7 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
             const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
             const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
             if (OP == kOpPre) {
-                a.sh[idx] = make_double2(2.0 * sv.x - y.x, 2.0 * sv.y - y.y);
+                a.sh[idx] = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
                 return;
             }
             a.t[idx] = y;
@@ -1764,7 +1795,7 @@ __global__ void __launch_bounds__(kThreads) spmv_v_wave_kernel(SolveArgs a, cons
             const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
             if (OP == kOpPre) {
                 const double2 z = p[idx];
-                a.ph[idx] = make_double2(2.0 * z.x - y.x, 2.0 * z.y - y.y);
+                a.ph[idx] = make_double2(a.pc * (2.0 * z.x - a.pc * y.x), a.pc * (2.0 * z.y - a.pc * y.y));
                 return;
             }
             a.v[idx] = y;
@@ -1804,7 +1835,7 @@ __global__ void __launch_bounds__(kThreads) spmv_t_wave_kernel(SolveArgs a, cons
             const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
             const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
             if (OP == kOpPre) {
-                a.sh[idx] = make_double2(2.0 * sv.x - y.x, 2.0 * sv.y - y.y);
+                a.sh[idx] = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
                 return;
             }
             a.t[idx] = y;
@@ -2467,6 +2498,47 @@ void launch_periodic_step(int64_t ne, int64_t nrows, int S, const int *ent_dof, 
                                                                       bper);
         check_launch();
     }
+}
+__global__ void real_spmv_kernel(int64_t n, const int *rowptr, const int *col, const double *vals, int vw,
+                                 const double *x, double *y, double *xy, double *yy) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) acc += vals[(int64_t)vw * e] * x[col[e]];
+    y[i] = acc;
+    xy[i] = x[i] * acc;
+    yy[i] = acc * acc;
+}
+__global__ void scale_vec_kernel(int64_t n, const double *y, double scale, double *x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = y[i] * scale;
+}
+__global__ void power_start_kernel(int64_t n, double *x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 1.0 + 0.5 * sin(0.7 * (double)i);  // no symmetry the top eigenvector could miss
+}
+void launch_real_spmv(int64_t n, const int *rowptr, const int *col, const double *vals, int vw, const double *x,
+                      double *y, double *xy, double *yy, hipStream_t s) {
+    real_spmv_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rowptr, col, vals, vw, x, y, xy, yy);
+    check_launch();
+}
+void launch_scale_vec(int64_t n, const double *y, double scale, double *x, hipStream_t s) {
+    scale_vec_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, y, scale, x);
+    check_launch();
+}
+void launch_power_start(int64_t n, double *x, hipStream_t s) {
+    power_start_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, x);
+    check_launch();
+}
+void launch_gershgorin(int64_t n, const int *rowptr, const double *vals, int vw, double *part, double *out,
+                       hipStream_t s) {
+    unsigned g = grid_for(n, kThreads);
+    if (g > (unsigned)kMaxGrid) g = kMaxGrid;
+    if (g == 0) g = 1;
+    row_absmax_kernel<<<g, kThreads, 0, s>>>(n, rowptr, vals, vw, part);
+    check_launch();
+    max_final_kernel<<<1, kThreads, 0, s>>>(part, (int)g, out);
+    check_launch();
 }
 void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s) {
     init_x_kernel<<<grid_for(n * S, 256), 256, 0, s>>>(n, S, sc, x);

@@ -22,7 +22,8 @@ import collections
 import csv
 import statistics
 
-SPMV = ("spmv_v_kernel", "spmv_t_kernel", "spmv_v_sr_kernel", "spmv_t_sr_kernel")
+SPMV = ("spmv_v_kernel", "spmv_t_kernel", "spmv_v_sr_kernel", "spmv_t_sr_kernel", "spmv_v_wave_kernel",
+        "spmv_t_wave_kernel", "spmv_v_tile_kernel", "spmv_t_tile_kernel")
 
 
 def short(name: str) -> str:
@@ -43,9 +44,10 @@ def sampled(rows, k):
         if b == "rhs_residual_kernel":
             seen.clear()
         elif b in SPMV:
-            if seen[b] < k:
+            key = short(r["Kernel_Name"])  # each template instance (operation) counts on its own
+            if seen[key] < k:
                 out.add(i)
-            seen[b] += 1
+            seen[key] += 1
     return out
 
 

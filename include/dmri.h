@@ -167,9 +167,10 @@ typedef struct {
                               5 = mapping 1 over tiles whose distinct columns are staged once
                               in LDS (lanes * signals <= 64) */
     int32_t preconditioner; /* 0 = Jacobi (the symmetric diagonal scaling) with the degree-1
-                               Neumann polynomial 2I - A of the scaled system, applied from the
-                               right (the residual the tolerance tests stays the true one; the
-                               default), 1 = Jacobi only, as the reference's
+                               polynomial c (2I - c A) of the scaled system, c = 1.8 / (Gershgorin
+                               bound of its real part), applied from the right (the residual the
+                               tolerance tests stays the true one; the default), 1 = Jacobi only,
+                               as the reference's
                                KrylovSolver("bicgstab", "jacobi").  Both converge to the same
                                solution; the polynomial needs SpMV mapping 1, 2 or 4 (others fall
                                back to mapping 2). */
