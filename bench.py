@@ -218,6 +218,20 @@ def cpu_baseline(args, w, ctx, info, gdir, q):
     return cpu, parity
 
 
+def parity_block(w, b0, prefix_parity, sig):
+    out = {"b0_signal_abs_err_max": max(b0) if b0 else None, "prefix_rel_err_vs_cpu": prefix_parity,
+           "signals_first": [s.real for s in sig[:8]]}
+    if w["periodic"] and not all(w["periodic"]) and b0:
+        # x/y periodic, z faces insulating: the weak periodic term integrates kappa_e (u - u_bc) over every
+        # boundary facet, and u_bc is 0 at vertices off the periodic faces (DmriFemLib.py:267-318), so the
+        # z-face facets touching a periodic edge remove magnetisation even at b = 0; the oracle shows the
+        # same loss (all faces periodic keeps S(b=0) = 1 to 1e-10)
+        out["b0_signal_minus_1_max"] = out.pop("b0_signal_abs_err_max")
+        out["b0_note"] = ("partially periodic box: S(b=0) < 1 is a property of the weak periodic form, "
+                          "reproduced by the oracle")
+    return out
+
+
 def run_forward(args, name):
     import dmri_native as N
 
@@ -322,8 +336,7 @@ def run_forward(args, name):
                      "frac_of_copy": (achieved / copy_gbs) if copy_gbs else None},
         "cpu_baseline": cpu,
         "plain_one_signal_at_a_time": plain,
-        "parity": {"b0_signal_abs_err_max": max(b0) if b0 else None, "prefix_rel_err_vs_cpu": prefix_parity,
-                   "signals_first": [s.real for s in sig[:8]]},
+        "parity": parity_block(w, b0, prefix_parity, sig),
     }
     print(json.dumps(out))
 
@@ -405,10 +418,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rtol", type=float, default=1e-12)
     ap.add_argument("--lanes", type=int, default=0, help="SpMV lanes per row; 0 = tuned")
-    ap.add_argument("--mapping", type=int, default=0, help="SpMV mapping 1 spread, 2 staged, 3 registers; 0 = tuned")
+    ap.add_argument("--mapping", type=int, default=0, help="SpMV mapping 1 spread, 2 staged, 3 registers, 4 wave per row, "
+                    "5 tile-compressed; 0 = tuned")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-signal-at-a-time comparison")
-    ap.add_argument("--guess", type=int, default=0, help="Krylov initial guess: 0 extrapolated, 1 previous step")
+    ap.add_argument("--guess", type=int, default=0, help="Krylov initial guess: 0 residual-minimising projection on the last states, 1 previous "
+                    "step, 2 linear, 3 quadratic extrapolation")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="counter runs only: truncate to the first K theta steps (result line not comparable)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel reference")

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <map>
 #include <memory>
@@ -31,7 +32,8 @@ constexpr int kEventPool = 512;
 
 struct SolverWork {
     DevArray<double2> x, r, rh, p, v, t, b, xprev, xprev2, ybc, bper, ph, sh;
-    DevArray<double> sc, vals, bre, part, tot, coef, coefP, sig, spart;
+    DevArray<double2> yh[kMaxHist], hp[kMaxHist], hq[kMaxHist];  // projection history
+    DevArray<double> sc, vals, bre, part, tot, coef, coefP, sig, spart, ppart, ptot, pgam, gsig;
     DevArray<KrylovState> st;
     DevArray<int> all_done, mask;
     KrylovState *h_st = nullptr;
@@ -568,6 +570,63 @@ void attach_tiles(SolveArgs &a, const TileCompression *tc) {
     a.lcol = tc->lcol.p;
 }
 
+// History of the projection guess: up to m states newest first, the Are / Jg products of each
+// (same order), and the free buffers.  m + 1 state buffers circulate: the m states and the one
+// the next guess is written to.
+struct ProjHistory {
+    int m = 0;  // capacity, 0 = off
+    std::deque<double2 *> ys, yfree;
+    std::deque<std::pair<double2 *, double2 *>> pq, pqfree;
+    void reset(int m_, const DevArray<double2> *yb, const DevArray<double2> *pb, const DevArray<double2> *qb) {
+        m = m_;
+        ys.clear();
+        yfree.clear();
+        pq.clear();
+        pqfree.clear();
+        for (int j = 0; j < m; ++j) {
+            yfree.push_back(yb[j].p);
+            pqfree.emplace_back(pb[j].p, qb[j].p);
+        }
+    }
+    int nh() const { return (int)std::min(ys.size(), pq.size()); }
+    // buffers for the products of the state that becomes the newest (the oldest pair when full)
+    std::pair<double2 *, double2 *> take_slot() {
+        std::pair<double2 *, double2 *> sl;
+        if (!pqfree.empty()) {
+            sl = pqfree.front();
+            pqfree.pop_front();
+        } else {
+            sl = pq.back();
+            pq.pop_back();
+        }
+        return sl;
+    }
+    double2 *take_free() {
+        double2 *y = yfree.front();
+        yfree.pop_front();
+        return y;
+    }
+    void push(double2 *y, std::pair<double2 *, double2 *> sl) {
+        pq.push_front(sl);
+        ys.push_front(y);
+        if ((int)ys.size() > m) {
+            yfree.push_back(ys.back());
+            ys.pop_back();
+        }
+    }
+};
+
+// capacity of a projection history of S-signal states: 8, 6 or 4 states when 3 x capacity
+// state-sized buffers take at most half of the free device memory (plus what the caller already
+// holds for it), else 0
+int proj_capacity(size_t nS, size_t held_bytes) {
+    size_t fr = 0, tot = 0;
+    DMRI_HIP(hipMemGetInfo(&fr, &tot));
+    for (int m : {8, 6, 4})
+        if ((double)m * 3.0 * (double)nS * sizeof(double2) <= 0.5 * (double)(fr + held_bytes)) return m;
+    return 0;
+}
+
 struct Stepper {
     dmri_ctx *c = nullptr;
     hipStream_t s = nullptr;
@@ -581,6 +640,7 @@ struct Stepper {
     dmri_stats *st = nullptr;
     double2 *xc = nullptr, *xp = nullptr, *xq = nullptr;  // u^n, u^(n-1), u^(n-2)
     int guess_order = 2;
+    ProjHistory fh;  // projection guess of the forward steps (fh.m = 0: off); fh.ys[0] = xc
     bool poly = true;  // degree-1 polynomial preconditioner
     double omt = 0.5;
     int chunk = 8;
@@ -673,10 +733,12 @@ struct Stepper {
         xc = w.x.p;
         xp = w.xprev.p;
         xq = w.xprev2.p;
-        // initial guess of step n: 0 = quadratic extrapolation 3u^n - 3u^(n-1) + u^(n-2),
-        // 2 = linear 2u^n - u^(n-1), 1 = u^n
+        // initial guess of step n: 0 = residual-minimising projection on the last states (the
+        // quadratic extrapolation when the history does not fit), 1 = u^n, 2 = linear
+        // 2u^n - u^(n-1), 3 = quadratic extrapolation 3u^n - 3u^(n-1) + u^(n-2)
         guess_order = opts->initial_guess == 1 ? 0 : (opts->initial_guess == 2 ? 1 : 2);
         extrap = guess_order > 0;
+        if (opts->initial_guess == 0) setup_projection(g);
         a.x = xc;
         a.xprev = xp;
         a.r = w.r.p;
@@ -714,6 +776,122 @@ struct Stepper {
         prof = opts->profile != 0;
         chunk = 8;
         zero_st.assign(S, KrylovState{});
+    }
+
+    // Residual-minimising guess over the last fh.m states (see proj_capacity); the quadratic
+    // extrapolation stays in use when the history does not fit.
+    void setup_projection(const double *g) {
+        SolverWork &w = c->work;
+        const size_t nS = (size_t)n * S;
+        size_t held = 0;  // history buffers this context already holds count as free
+        for (int j = 0; j < kMaxHist; ++j) held += (w.yh[j].n + w.hp[j].n + w.hq[j].n) * sizeof(double2);
+        const int m = proj_capacity(nS, held);
+        fh.reset(0, nullptr, nullptr, nullptr);
+        if (m == 0) return;
+        for (int j = 0; j < kMaxHist; ++j) {
+            if (j < m) {
+                if (w.yh[j].n != nS) w.yh[j].alloc(nS);
+                if (w.hp[j].n != nS) w.hp[j].alloc(nS);
+                if (w.hq[j].n != nS) w.hq[j].alloc(nS);
+            } else {
+                w.yh[j].release();
+                w.hp[j].release();
+                w.hq[j].release();
+            }
+        }
+        ensure_proj_scratch(m, g);
+        fh.reset(m, w.yh, w.hp, w.hq);
+        fh.ys.push_back(xc);  // u^0: its products are formed at step 0
+        extrap = false;
+        guess_order = 0;
+    }
+
+    void ensure_proj_scratch(int m, const double *g) {
+        SolverWork &w = c->work;
+        const size_t npart = (size_t)proj_grid(geo) * S * proj_nv(m);
+        if (w.ppart.n < npart) w.ppart.alloc(npart);
+        if (w.ptot.n < (size_t)S * proj_nv(m)) w.ptot.alloc((size_t)S * proj_nv(m));
+        if (w.pgam.n < (size_t)S * 2 * kMaxHist) w.pgam.alloc((size_t)S * 2 * kMaxHist);
+        if (per_gdir) {
+            if (w.gsig.n < (size_t)S * 3) w.gsig.alloc((size_t)S * 3);
+            DMRI_HIP(hipMemcpyAsync(w.gsig.p, g, sizeof(double) * S * 3, hipMemcpyHostToDevice, s));
+            a.gsig = w.gsig.p;
+        }
+    }
+
+    // Guess of the current step (coefficients set by begin_step, b in work.b) from history h
+    // into a free buffer of h, which it returns.
+    double2 *projected_guess(ProjHistory &h) {
+        SolverWork &w = c->work;
+        double2 *y0 = h.take_free();
+        ProjArgs pa{};
+        pa.n = n;
+        pa.S = S;
+        pa.nh = h.nh();
+        pa.per_gdir = per_gdir;
+        for (int j = 0; j < pa.nh; ++j) {
+            pa.Y[j] = h.ys[j];
+            pa.P[j] = h.pq[j].first;
+            pa.Q[j] = h.pq[j].second;
+        }
+        pa.b = w.b.p;
+        pa.coefA = a.coefA;
+        pa.gsig = per_gdir ? w.gsig.p : nullptr;
+        pa.part = w.ppart.p;
+        pa.tot = w.ptot.p;
+        pa.gam = w.pgam.p;
+        pa.y0 = y0;
+        launch_proj_guess(geo, pa, h.m, s);
+        return y0;
+    }
+
+    // Step k with the projection guess: b and the history products of u^n (= fh.ys[0]) in one
+    // pass, the guess into a free state buffer, r = b - A x0, Krylov; the solution becomes ys[0].
+    void projection_step(int k) {
+        const auto slot = fh.take_slot();
+        a.x = xc;
+        a.hp = slot.first;
+        a.hq = slot.second;
+        launch_rhs_residual(geo, a, 4, nullptr, s);
+        fh.pq.push_front(slot);
+        double2 *y0 = projected_guess(fh);
+        a.x = y0;
+        launch_rhs_residual(geo, a, 1, nullptr, s);
+        krylov(k);
+        xc = y0;
+        a.x = xc;
+        fh.ys.push_front(y0);
+        if ((int)fh.ys.size() > fh.m) {
+            fh.yfree.push_back(fh.ys.back());
+            fh.ys.pop_back();
+        }
+    }
+
+    // Adjoint solve of step k: A_k y = rhs, from the projection over h (from *start when h is
+    // empty); returns the buffer holding the solution.
+    double2 *solve_projected(int k, const double2 *rhs, ProjHistory &h, double2 *start) {
+        begin_step(k);
+        DMRI_HIP(hipMemcpyAsync(c->work.b.p, rhs, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        double2 *y = h.nh() > 0 ? projected_guess(h) : start;
+        a.x = y;
+        launch_rhs_residual(geo, a, 1, nullptr, s);
+        krylov(k);
+        a.x = xc;
+        return y;
+    }
+
+    // out = B_k y, with the history products of y (y becomes the newest state of h)
+    void apply_B_record(int k, double2 *y, double2 *out, ProjHistory &h) {
+        begin_step(k);
+        const auto slot = h.take_slot();
+        a.x = y;
+        a.hp = slot.first;
+        a.hq = slot.second;
+        launch_rhs_residual(geo, a, 4, nullptr, s);
+        h.push(y, slot);
+        DMRI_HIP(hipMemcpyAsync(out, c->work.b.p, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        a.x = xc;
+        a.hp = a.hq = nullptr;
     }
 
     void begin_step(int step) {
@@ -825,11 +1003,16 @@ struct Stepper {
         begin_step(k);
         a.x = xc;
         a.xprev = xp;
+        a.hp = a.hq = nullptr;
         if (c->periodic) {
             SolverWork &w = c->work;
             launch_periodic_step(c->n_per, c->n_prow, S, c->sper_dof.p, c->sper_src.p, c->per_w.p, c->per_ell.p,
                                  c->sprow.p, c->srow.p, c->scol.p, c->spval.p, w.sc.p, xc,
                                  w.coefP.p + (size_t)k * S * 3, omt, w.ybc.p, w.bper.p, s);
+        }
+        if (fh.m > 0) {
+            projection_step(k);
+            return;
         }
         const int order = std::min(guess_order, k);
         if (order > 0) {
@@ -908,9 +1091,8 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     dmri_stats st{};
     Stepper run;
     const auto t0 = std::chrono::steady_clock::now();
-    run.setup(c, 1, &seq->q, g, false, seq, opts, &st);
-    const int N = run.N;
-    const int64_t n = run.n;
+    const int N = seq->nsteps;
+    const int64_t n = c->ndof;
     // stride 0: keep every state when the N+1 states fit in a quarter of the free HBM (no step is
     // recomputed), else checkpoints every ceil(sqrt(N)) steps
     int K = aopts ? aopts->checkpoint_stride : 0;
@@ -922,6 +1104,14 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     }
     K = std::max(1, std::min(K, std::max(N, 1)));
     const bool keep_all = K >= N;
+    // With checkpoints the forward sweep extrapolates from at most three states (a checkpoint
+    // holds three, so a recomputed segment repeats the sweep bit for bit): the projection guess
+    // becomes the quadratic extrapolation there.  The backward sweep runs once and keeps its own
+    // projection history of adjoint states.
+    const bool proj = opts->initial_guess == 0;
+    dmri_solver_opts fopts = *opts;
+    if (proj && !keep_all) fopts.initial_guess = 3;
+    run.setup(c, 1, &seq->q, g, false, seq, &fopts, &st);
     const int nck = (N + K - 1) / K;
     // checkpoints (x^k, x^(k-1), x^(k-2)) at k = 0, K, 2K, ... (the history the initial guess of
     // step k extrapolates from, so a recomputed segment repeats the forward sweep bit for bit);
@@ -962,6 +1152,21 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     double gK[2] = {0, 0}, gM[2] = {0, 0}, gI = 0, gq = 0;
     std::vector<double> dfa(std::max(N, 1), 0.0), dfb(std::max(N, 1), 0.0);
     int recomputed = 0;
+    ProjHistory bh;
+    DevArray<double2> byh[kMaxHist], bhp[kMaxHist], bhq[kMaxHist];
+    if (proj) {
+        const int m = proj_capacity((size_t)n, 0);
+        if (m > 0) {
+            for (int j = 0; j < m; ++j) {
+                byh[j].alloc(n);
+                bhp[j].alloc(n);
+                bhq[j].alloc(n);
+            }
+            run.ensure_proj_scratch(m, g);
+            bh.reset(m, byh, bhp, bhq);
+        }
+    }
+    double2 *mu_cur = mu.p;
     for (int j = nck - 1; j >= 0; --j) {
         const int k0 = j * K, k1 = std::min(N, k0 + K);
         if (!keep_all) {
@@ -977,13 +1182,21 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
         }
         for (int k = k1 - 1; k >= k0; --k) {
             const int base = keep_all ? 0 : k0;
-            run.solve_given(k, lam.p, mu.p);  // A_k mu = lambda^(k+1), warm start from mu_(k+1)
+            // A_k mu = lambda^(k+1): from the projection over the later adjoint states, or warm
+            // started from mu_(k+1)
+            if (bh.m > 0)
+                mu_cur = run.solve_projected(k, lam.p, bh, mu_cur);
+            else
+                run.solve_given(k, lam.p, mu_cur);
             nparts_of[k] = launch_adj_contract(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p,
                                                c->ncomp == 2 ? c->scomp.p : nullptr, c->val[DMRI_MAT_M].p,
-                                               c->val[DMRI_MAT_K].p, c->val[DMRI_MAT_I].p, w.vals.p, mu.p,
+                                               c->val[DMRI_MAT_K].p, c->val[DMRI_MAT_I].p, w.vals.p, mu_cur,
                                                segp(k - base), segp(k - base + 1), theta,
                                                parts.p + (size_t)k * kMaxGrid * 14, s);
-            run.apply_B(k, mu.p, lam2.p);  // lambda^k = B_k mu_k
+            if (bh.m > 0)
+                run.apply_B_record(k, mu_cur, lam2.p, bh);  // lambda^k = B_k mu_k, mu_k into the history
+            else
+                run.apply_B(k, mu_cur, lam2.p);  // lambda^k = B_k mu_k
             std::swap(lam.p, lam2.p);
         }
     }
@@ -1183,6 +1396,7 @@ void validate_seq(const dmri_seq *seq, const dmri_solver_opts *opts) {
     require(opts->rtol >= 0 && opts->atol >= 0 && (opts->rtol > 0 || opts->atol > 0), DMRI_ERR_ARG,
             "need rtol > 0 or atol > 0");
     require(opts->lanes_per_row >= 0, DMRI_ERR_ARG, "lanes_per_row must be >= 0");
+    require(opts->initial_guess >= 0 && opts->initial_guess <= 3, DMRI_ERR_ARG, "initial_guess must lie in [0, 3]");
     for (int i = 0; i < seq->nsteps; ++i)
         require(std::isfinite(seq->f_implicit[i]) && std::isfinite(seq->f_explicit[i]), DMRI_ERR_ARG,
                 "time profile has a non-finite value");

@@ -250,6 +250,8 @@ struct SolveArgs {
     const double2 *bper;            // weak pseudo-periodic right-hand side of this step, or null
     double2 *ph, *sh;               // preconditioned p and s (polynomial preconditioner), or null
     double pc;                      // polynomial scale c: z -> c (2 z - c A z)
+    double2 *hp, *hq;               // mode 4 writes Are x and Jg x here (projection history)
+    const double *gsig;             // [S*3] unit gradient directions (per-direction values only)
     double *partU, *partV, *partT;  // [grid][S][kNV] each
     double *totU, *totV, *totT;     // [S][kNV] finalize outputs, or null: consumers reduce the partials
     KrylovState *st;
@@ -259,8 +261,36 @@ struct SolveArgs {
 };
 // mode 0: b and r from x; 1: r from the stored b (masked signals, nullptr = all); 2: b only and
 // xpp <- gc0 x + gc1 xprev + gc2 xpp (the extrapolated initial guess; xpp is not read when
-// gc2 = 0); 3: b only
+// gc2 = 0); 3: b only; 4: b, hp = Are x and hq = Jg x (Jg = g_s.J with per-signal directions)
 void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const int *mask, hipStream_t s);
+
+// Residual-minimising initial guess over the last nh <= kMaxHist states y_0 (newest) .. y_nh-1 of
+// the scaled system A_n = Are + i c_s Jg.  With P_j = Are y_j and Q_j = Jg y_j kept from the step
+// that produced y_j, the columns W_j = P_j + i c_s Q_j span A_n times the history; the guess
+// y0 = sum_j gamma_j y_j minimises |b - A_n y0| per signal.  The normal equations are formed on
+// the backward differences of the history (the raw states are nearly parallel, their Gram matrix
+// is singular to working precision), Jacobi-scaled and solved by a Cholesky factorisation that
+// stops at the first pivot below kProjPivot (the remaining differences are dropped).
+constexpr int kMaxHist = 8;
+constexpr double kProjPivot = 1e-12;
+struct ProjArgs {
+    int64_t n;
+    int S, nh;
+    bool per_gdir;
+    const double2 *Y[kMaxHist], *P[kMaxHist], *Q[kMaxHist];  // newest first
+    const double2 *b;
+    const double *coefA;  // [S*3] this step's implicit imaginary coefficients (c_s g_s per direction)
+    const double *gsig;   // [S*3] unit directions (per-direction values only)
+    double *part;         // [pgrid][S][proj_nv(mh)] block partials
+    double *tot;          // [S][proj_nv(mh)]
+    double *gam;          // [S][kMaxHist] complex coefficients on the raw states
+    double2 *y0;          // the guess
+};
+// doubles per signal of the normal equations: upper triangle of the Gram matrix and W^H b, complex
+constexpr int proj_nv(int mh) { return 2 * (mh * (mh + 1) / 2 + mh); }
+// mh: history capacity (4, 6 or 8) >= p.nh; part >= proj_grid() * S * proj_nv(mh) doubles
+int proj_grid(const SolveGeom &g);
+void launch_proj_guess(const SolveGeom &g, const ProjArgs &p, int mh, hipStream_t s);
 void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
 // op: kOpV / kOpPre for the v-kernel (kOpPre writes ph from p), kOpT / kOpPre / kOpTP for the
 // t-kernel (kOpPre writes sh from s = r - alpha v; kOpTP gathers sh).  Mappings 1, 2 and 4 take

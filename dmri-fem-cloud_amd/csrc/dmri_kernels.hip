@@ -1352,6 +1352,7 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
 //         after an extrapolated initial guess; mask == nullptr selects every signal)
 // mode 2: b = (Bre + i cB.J) x only, and xpp <- gc0 x + gc1 xprev + gc2 xpp (the extrapolated guess)
 // mode 3: b = (Bre + i cB.J) x only (the adjoint's lambda^n = B_n mu_n)
+// mode 4: b = (Bre + i cB.J) x, hp = Are x, hq = Jg x (the projection history of state x)
 // partials U (modes 0, 1): [|r|^2, 0, |r|^2, |b|^2]
 template <int G, bool PG>
 __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int S, int rpb, int mode,
@@ -1374,6 +1375,7 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
         }
         double2 sb = make_double2(0.0, 0.0);
         Acc<PG> jacc;
+        const bool jsep = mode == 2 || mode == 3;  // J part of x not in acc: accumulate it here
         if (mode != 1) {
             jacc.a = make_double2(0.0, 0.0);
 #pragma unroll
@@ -1384,8 +1386,8 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
                 const double2 xv = x[(int64_t)a.col[e] * S_ + s];
                 sb.x += bv * xv.x;
                 sb.y += bv * xv.y;
-                if (mode >= 2) {
-                    // J part of the explicit side (mode 0 reuses the implicit accumulation)
+                if (jsep) {
+                    // J part of the explicit side (modes 0 and 4 reuse the implicit accumulation)
                     if (PG) {
                         const double4 v = *reinterpret_cast<const double4 *>(a.vals + 4 * (int64_t)e);
                         jacc.j[0].x += v.y * xv.x; jacc.j[0].y += v.y * xv.y;
@@ -1398,7 +1400,7 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
                 }
             }
             group_reduce<G>(sb);
-            if (mode >= 2) group_reduce_acc<G, PG>(jacc);
+            if (jsep) group_reduce_acc<G, PG>(jacc);
         }
         if (L.g == 0) {
             const int64_t idx = row * S_ + s;
@@ -1406,7 +1408,7 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
             if (mode == 1) {
                 b = a.b[idx];
             } else {
-                Acc<PG> bacc = mode == 0 ? acc : jacc;
+                Acc<PG> bacc = jsep ? jacc : acc;
                 bacc.a = sb;
                 b = imag_coef_apply<G, PG>(bacc, a.coefB, s);
                 if (a.bper) {
@@ -1426,6 +1428,15 @@ __global__ void __launch_bounds__(kThreads) rhs_residual_kernel(SolveArgs a, int
                 }
                 a.xpp[idx] = g;
             } else if (mode == 3) {
+            } else if (mode == 4) {
+                a.hp[idx] = acc.a;
+                double2 q = acc.j[0];
+                if (PG) {
+                    const double g0 = a.gsig[3 * s], g1 = a.gsig[3 * s + 1], g2 = a.gsig[3 * s + 2];
+                    q.x = g0 * acc.j[0].x + g1 * acc.j[1].x + g2 * acc.j[2].x;
+                    q.y = g0 * acc.j[0].y + g1 * acc.j[1].y + g2 * acc.j[2].y;
+                }
+                a.hq[idx] = q;
             } else {
                 const double2 ax = imag_coef_apply<G, PG>(acc, a.coefA, s);
                 const double2 r = make_double2(b.x - ax.x, b.y - ax.y);
@@ -2041,6 +2052,241 @@ __global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int 
         }
     }
     block_signal_partials(red, valid, threadIdx.x, vrpb * S, S, a.partU);
+}
+
+// ---- residual-minimising initial guess over the last states (see ProjArgs)
+__device__ __forceinline__ double2 c_conj_mul(double2 a, double2 b) {  // conj(a) b
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+// d[j] <- j-th backward difference of d[0] (newest) .. d[nh-1]: sum_i (-1)^i C(j,i) d[i]
+template <int MH>
+__device__ __forceinline__ void backward_differences(double2 (&d)[MH], int nh) {
+#pragma unroll
+    for (int lvl = 1; lvl < MH; ++lvl)
+#pragma unroll
+        for (int j = MH - 1; j >= lvl; --j)
+            if (j < nh) d[j] = c_sub(d[j - 1], d[j]);
+}
+
+__device__ __forceinline__ double proj_coef(const ProjArgs &p, int s) {
+    if (!p.per_gdir) return p.coefA[3 * s];
+    return p.coefA[3 * s] * p.gsig[3 * s] + p.coefA[3 * s + 1] * p.gsig[3 * s + 1] +
+           p.coefA[3 * s + 2] * p.gsig[3 * s + 2];
+}
+
+// Lanes: Sp = S rounded up to a power of two, lane = row_in_wave * Sp + signal (lanes with
+// signal >= S idle), so the lanes of one signal in a wave are an xor-closed set and the
+// per-signal sums reduce with shuffles before one LDS pass over the 4 waves.  Block partials:
+// part[block][s][k], k over the packed upper triangle of W^H W (j <= l, row-major) and then W^H b.
+template <int MH>
+__global__ void __launch_bounds__(kThreads) proj_gram_kernel(ProjArgs p, int Sp) {
+    constexpr int NG = MH * (MH + 1) / 2;
+    constexpr int NV = proj_nv(MH);
+    const int S = p.S;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int s = lane & (Sp - 1);
+    const int rw = 64 / Sp;                  // rows per wave
+    const int rb = rw * (kThreads / 64);     // rows per block pass
+    const bool act = s < S;
+    const int nh = p.nh;
+    double2 g[NG], h[MH];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) g[k] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < MH; ++k) h[k] = make_double2(0.0, 0.0);
+    if (act) {
+        const double c = proj_coef(p, s);
+        for (int64_t row = (int64_t)blockIdx.x * rb + wave * rw + (lane / Sp); row < p.n;
+             row += (int64_t)gridDim.x * rb) {
+            const int64_t i = row * S + s;
+            double2 w[MH];
+#pragma unroll
+            for (int j = 0; j < MH; ++j) {
+                w[j] = make_double2(0.0, 0.0);
+                if (j < nh) {
+                    const double2 P = p.P[j][i], Q = p.Q[j][i];
+                    w[j] = make_double2(P.x - c * Q.y, P.y + c * Q.x);  // P + i c Q
+                }
+            }
+            backward_differences<MH>(w, nh);
+            const double2 bv = p.b[i];
+            int k = 0;
+#pragma unroll
+            for (int j = 0; j < MH; ++j) {
+                h[j] = c_add(h[j], c_conj_mul(w[j], bv));
+#pragma unroll
+                for (int l = j; l < MH; ++l, ++k) g[k] = c_add(g[k], c_conj_mul(w[j], w[l]));
+            }
+        }
+    }
+    // per-signal sums: shuffles inside the wave, then the 4 waves through LDS, kNV doubles a pass
+    double(&buf)[kNV][kThreads] = reduce_scratch();
+#pragma unroll
+    for (int k0 = 0; k0 < NV; k0 += kNV) {
+        double v[kNV];
+#pragma unroll
+        for (int k = 0; k < kNV; ++k) {
+            const int kk = k0 + k;
+            double x = 0.0;
+            if (kk < 2 * NG) x = (kk & 1) ? g[kk >> 1].y : g[kk >> 1].x;
+            else if (kk < NV) x = ((kk - 2 * NG) & 1) ? h[(kk - 2 * NG) >> 1].y : h[(kk - 2 * NG) >> 1].x;
+            for (int off = 32; off >= Sp; off >>= 1) x += __shfl_xor(x, off, 64);
+            v[k] = x;
+        }
+        if (lane < Sp) {
+#pragma unroll
+            for (int k = 0; k < kNV; ++k) buf[k][wave * 64 + lane] = v[k];
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < S * kNV) {
+            const int ss = threadIdx.x % S, k = threadIdx.x / S;
+            if (k0 + k < NV) {
+                double acc = 0.0;
+#pragma unroll
+                for (int wv = 0; wv < kThreads / 64; ++wv) acc += buf[k][wv * 64 + ss];
+                p.part[((size_t)blockIdx.x * S + ss) * NV + k0 + k] = acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// tot[s][k] = sum over the nparts blocks of part[b][s][k], in block order per thread then a
+// fixed block reduction; one block per (signal, k).
+__global__ void __launch_bounds__(kThreads) proj_finalize_kernel(const double *part, int nparts, int S, int nv,
+                                                                 double *tot) {
+    const int s = blockIdx.x % S, k = blockIdx.x / S;
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) v[0] += part[((size_t)b * S + s) * nv + k];
+    block_reduce<1>(v);
+    if (threadIdx.x == 0) tot[s * nv + k] = v[0];
+}
+
+// Per signal (one thread): Jacobi-scaled Cholesky of the Gram matrix with truncation at the
+// first pivot below kProjPivot, solve, and the coefficients on the raw states.
+template <int MH>
+__global__ void proj_solve_kernel(ProjArgs p) {
+    constexpr int NG = MH * (MH + 1) / 2;
+    constexpr int NV = proj_nv(MH);
+    const int s = threadIdx.x;
+    if (s >= p.S) return;
+    const double *t = p.tot + (size_t)s * NV;
+    const int nh = p.nh;
+    double2 G[MH][MH], L[MH][MH], z[MH], c[MH];
+    double D[MH];
+    {
+        int k = 0;
+#pragma unroll
+        for (int j = 0; j < MH; ++j)
+#pragma unroll
+            for (int l = j; l < MH; ++l, ++k) {
+                G[j][l] = make_double2(t[2 * k], t[2 * k + 1]);
+                G[l][j] = make_double2(t[2 * k], -t[2 * k + 1]);
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < MH; ++j) {
+        D[j] = (j < nh && G[j][j].x > 0.0) ? sqrt(G[j][j].x) : 0.0;
+        z[j] = make_double2(t[2 * NG + 2 * j], t[2 * NG + 2 * j + 1]);
+        c[j] = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int l = 0; l < MH; ++l) L[j][l] = make_double2(0.0, 0.0);
+    }
+    int m = 0;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < MH; ++j) {
+        if (ok && j < nh && D[j] > 0.0) {
+            double d = 1.0;
+#pragma unroll
+            for (int k = 0; k < j; ++k) d -= L[j][k].x * L[j][k].x + L[j][k].y * L[j][k].y;
+            if (d > kProjPivot) {
+                const double ljj = sqrt(d);
+                L[j][j] = make_double2(ljj, 0.0);
+#pragma unroll
+                for (int i = j + 1; i < MH; ++i) {
+                    if (i < nh && D[i] > 0.0) {
+                        const double sc = 1.0 / (D[i] * D[j]);
+                        double2 v = make_double2(G[i][j].x * sc, G[i][j].y * sc);
+#pragma unroll
+                        for (int k = 0; k < j; ++k) v = c_sub(v, c_mul(L[i][k], make_double2(L[j][k].x, -L[j][k].y)));
+                        L[i][j] = make_double2(v.x / ljj, v.y / ljj);
+                    }
+                }
+                m = j + 1;
+            } else {
+                ok = false;
+            }
+        } else {
+            ok = false;
+        }
+    }
+    // forward: L y = W^H b / D ; back: L^H x = y ; c = x / D
+#pragma unroll
+    for (int j = 0; j < MH; ++j) {
+        if (j < m) {
+            double2 v = make_double2(z[j].x / D[j], z[j].y / D[j]);
+#pragma unroll
+            for (int k = 0; k < j; ++k) v = c_sub(v, c_mul(L[j][k], z[k]));
+            z[j] = make_double2(v.x / L[j][j].x, v.y / L[j][j].x);
+        }
+    }
+#pragma unroll
+    for (int j = MH - 1; j >= 0; --j) {
+        if (j < m) {
+            double2 v = z[j];
+#pragma unroll
+            for (int k = j + 1; k < MH; ++k)
+                if (k < m) v = c_sub(v, c_mul(make_double2(L[k][j].x, -L[k][j].y), c[k]));
+            c[j] = make_double2(v.x / L[j][j].x, v.y / L[j][j].x);
+        }
+    }
+    // y0 = sum_j c_j Delta^j = sum_i gamma_i y_i, gamma_i = sum_{j >= i} c_j (-1)^i C(j, i)
+    double2 gam[MH];
+#pragma unroll
+    for (int i = 0; i < MH; ++i) gam[i] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < MH; ++j) {
+        if (j < m) {
+            const double2 cj = make_double2(c[j].x / D[j], c[j].y / D[j]);
+            double binom = 1.0;  // C(j, i)
+#pragma unroll
+            for (int i = 0; i <= j; ++i) {
+                const double f = (i & 1) ? -binom : binom;
+                gam[i].x += f * cj.x;
+                gam[i].y += f * cj.y;
+                binom = binom * (double)(j - i) / (double)(i + 1);
+            }
+        }
+    }
+    if (m == 0) gam[0] = make_double2(1.0, 0.0);  // no usable direction: keep the newest state
+#pragma unroll
+    for (int i = 0; i < MH; ++i) {
+        p.gam[(size_t)s * 2 * kMaxHist + 2 * i] = gam[i].x;
+        p.gam[(size_t)s * 2 * kMaxHist + 2 * i + 1] = gam[i].y;
+    }
+}
+
+// y0 = sum_j gamma_{s,j} y_j (vector lanes: element i of the [n][S] arrays, signal i mod S)
+template <int MH>
+__global__ void __launch_bounds__(kThreads) proj_combine_kernel(ProjArgs p, int vrpb) {
+    const int S = p.S;
+    const int s = threadIdx.x % S;
+    if ((int)threadIdx.x >= vrpb * S) return;
+    const int nh = p.nh;
+    double2 gm[MH];
+#pragma unroll
+    for (int j = 0; j < MH; ++j)
+        gm[j] = make_double2(p.gam[(size_t)s * 2 * kMaxHist + 2 * j], p.gam[(size_t)s * 2 * kMaxHist + 2 * j + 1]);
+    const int64_t total = p.n * S;
+    for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total; i += (int64_t)gridDim.x * vrpb * S) {
+        double2 y = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < MH; ++j)
+            if (j < nh) y = c_add(y, c_mul(gm[j], p.Y[j][i]));
+        p.y0[i] = y;
+    }
 }
 
 // Adjoint contraction of one theta step (one signal).  With the scaled states x_n, x_n1, the
@@ -2712,6 +2958,35 @@ void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStrea
         check_launch();
     }
 }
+int proj_grid(const SolveGeom &g) { return g.vgrid; }
+
+void launch_proj_guess(const SolveGeom &g, const ProjArgs &p, int mh, hipStream_t s) {
+    if (p.nh < 1 || p.nh > mh || !(mh == 4 || mh == 6 || mh == 8))
+        throw Error(-1, "projection guess: history size must satisfy 1 <= nh <= mh, mh in {4, 6, 8}");
+    if (p.S < 1 || p.S > kMaxSignals) throw Error(-1, "projection guess: signals per launch must lie in [1, 64]");
+    int Sp = 1;
+    while (Sp < p.S) Sp <<= 1;
+    const int grid = proj_grid(g);
+    const int nv = proj_nv(mh);
+#define DMRI_PROJ(M)                                                                   \
+    proj_gram_kernel<M><<<grid, kThreads, 0, s>>>(p, Sp);                               \
+    check_launch();                                                                    \
+    proj_finalize_kernel<<<p.S * nv, kThreads, 0, s>>>(p.part, grid, p.S, nv, p.tot); \
+    check_launch();                                                                    \
+    proj_solve_kernel<M><<<1, 64, 0, s>>>(p);                                          \
+    check_launch();                                                                    \
+    proj_combine_kernel<M><<<g.vgrid, g.vec_threads, 0, s>>>(p, g.vec_rpb);            \
+    check_launch();
+    if (mh == 4) {
+        DMRI_PROJ(4)
+    } else if (mh == 6) {
+        DMRI_PROJ(6)
+    } else {
+        DMRI_PROJ(8)
+    }
+#undef DMRI_PROJ
+}
+
 void launch_signal(int64_t n, int S, const double *w, const double *sc, const double2 *x, double *part, double *out,
                    hipStream_t s) {
     const int vrpb = std::max(1, kThreads / S);

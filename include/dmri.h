@@ -157,8 +157,14 @@ typedef struct {
                               events cost GPU time of their own, so a small k perturbs least */
     int32_t lanes_per_row; /* SpMV lanes per row: 0 = tuned value (dmri_tune / dmri_set_tuning) or the
                               library default; else a power of two in 1..32 */
-    int32_t initial_guess; /* Krylov start of step n: 0 = 3u^n - 3u^(n-1) + u^(n-2) (quadratic
-                              extrapolation; linear at n = 1), 1 = u^n, 2 = 2u^n - u^(n-1) */
+    int32_t initial_guess; /* Krylov start of step n: 0 = the combination of the last 8 states
+                              (fewer in the first steps) that minimises the residual of step n
+                              (the default; the history takes 24 state-sized buffers, 6 or 4
+                              states are kept when those do not fit in half of the free device
+                              memory, and the quadratic extrapolation below when even 4 do not;
+                              dmri_adjoint's forward sweep always uses the quadratic one),
+                              1 = u^n, 2 = 2u^n - u^(n-1) (linear extrapolation), 3 = 3u^n -
+                              3u^(n-1) + u^(n-2) (quadratic extrapolation; linear at n = 1) */
     int32_t mapping;       /* SpMV mapping: 0 = tuned/default, 1 = signals spread over lanes
                               (lanes * signals <= 64), 2 = the same with rows staged in LDS,
                               3 = signals kept in registers (at most 8 signals per launch group),
@@ -167,8 +173,10 @@ typedef struct {
                               5 = mapping 1 over tiles whose distinct columns are staged once
                               in LDS (lanes * signals <= 64) */
     int32_t preconditioner; /* 0 = Jacobi (the symmetric diagonal scaling) with the degree-1
-                               polynomial c (2I - c A) of the scaled system, c = 1.8 / (Gershgorin
-                               bound of its real part), applied from the right (the residual the
+                               polynomial c (2I - c A) of the scaled system, c = 1.8 / b where b
+                               bounds the largest eigenvalue of its real part (1.05 x a 30-step
+                               power iteration estimate, capped by the Gershgorin bound),
+                               applied from the right (the residual the
                                tolerance tests stays the true one; the default), 1 = Jacobi only,
                                as the reference's
                                KrylovSolver("bicgstab", "jacobi").  Both converge to the same

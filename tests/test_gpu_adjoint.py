@@ -30,7 +30,8 @@ def _ctx(**over):
 
 
 def _fwd(fA=FA, fB=FB, q=Q, **over):
-    return _ctx(**over).solve(fA, fB, K, q, G, rtol=1e-14)[0].real
+    # the adjoint's forward sweep starts each Krylov solve from the quadratic extrapolation
+    return _ctx(**over).solve(fA, fB, K, q, G, rtol=1e-14, guess=3)[0].real
 
 
 def _close(a, b, tol=2e-5):
@@ -39,10 +40,25 @@ def _close(a, b, tol=2e-5):
 
 @pytest.fixture(scope="module")
 def adj():
+    # quadratic extrapolation in both sweeps: the checkpointed and the stored-states runs then
+    # take the same Krylov iterates, so their gradients agree bit for bit
     stride = int(np.ceil(np.sqrt(len(FA))))
-    s, g, st = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14, checkpoint_stride=stride)
-    s_all, g_all, _ = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14, checkpoint_stride=10 ** 6)
+    s, g, st = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14, checkpoint_stride=stride, guess=3)
+    s_all, g_all, _ = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14, checkpoint_stride=10 ** 6, guess=3)
     return s, g, s_all, g_all
+
+
+@pytest.mark.parametrize("stride", [0, 17])
+def test_projection_guess_gradients_match_quadratic(adj, stride):
+    """Default guess: the projection over the last states in the backward sweep (and in the
+    forward sweep when every state is kept); the gradients agree with the quadratic runs."""
+    _, g_q, _, _ = adj
+    s, g, _ = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14, checkpoint_stride=stride)
+    assert abs(s.real - _fwd()) <= 1e-12
+    for k in ("D", "invT2"):
+        assert np.all(np.abs(g[k] - g_q[k]) <= 1e-9 * np.abs(g_q[k]).max())
+    assert abs(g["kappa"] - g_q["kappa"]) <= 1e-9 * abs(g_q["kappa"])
+    assert np.abs(g["f_implicit"] - g_q["f_implicit"]).max() <= 1e-9 * np.abs(g_q["f_implicit"]).max()
 
 
 def test_automatic_checkpointing_keeps_every_state_when_it_fits():

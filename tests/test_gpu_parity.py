@@ -227,13 +227,37 @@ def test_initial_guess_modes_agree():
     s0, st0 = ctx.solve(fA, fB, 200.0, q, (0, 1, 0), rtol=1e-13, guess=0)
     s1, st1 = ctx.solve(fA, fB, 200.0, q, (0, 1, 0), rtol=1e-13, guess=1)
     s2, st2 = ctx.solve(fA, fB, 200.0, q, (0, 1, 0), rtol=1e-13, guess=2)
-    print(f"  iterations: quadratic {st0['iters_total']} linear {st2['iters_total']} previous {st1['iters_total']}",
-          flush=True)
-    assert abs(s0 - s1) <= 1e-10 * abs(s1) and abs(s2 - s1) <= 1e-10 * abs(s1)
-    assert st0["iters_total"] < st2["iters_total"] < st1["iters_total"]
+    s3, st3 = ctx.solve(fA, fB, 200.0, q, (0, 1, 0), rtol=1e-13, guess=3)
+    print(f"  iterations: projection {st0['iters_total']} quadratic {st3['iters_total']} linear {st2['iters_total']} "
+          f"previous {st1['iters_total']}", flush=True)
+    assert abs(s0 - s1) <= 1e-10 * abs(s1) and abs(s2 - s1) <= 1e-10 * abs(s1) and abs(s3 - s1) <= 1e-10 * abs(s1)
+    assert st0["iters_total"] < st3["iters_total"] < st2["iters_total"] < st1["iters_total"]
     S = O.System(mesh.xyz, mesh.tets, mesh.phase, D=kw["D"], kappa=kw["kappa"])
     ref, _, _ = S.solve((0, 1, 0), q, 200.0, fA, fB, rtol=1e-13)
     assert abs(s0.real - ref) <= 1e-9 * abs(ref)
+
+
+@pytest.mark.parametrize("pg", [False, True])
+def test_projection_guess_batched_matches_quadratic(pg):
+    """The residual-minimising guess over the last 8 states changes only where Krylov starts:
+    batched signals (shared or per-signal directions) agree with the quadratic extrapolation's
+    and with single solves, in fewer iterations; a second launch group on the same context
+    (history buffers reused) gives the same values."""
+    mesh, kw = MESHES["cyl2"]
+    delta, Delta, k = 10600.0, 43100.0, 400.0
+    fA, fB = _pgse_gcloud(delta, Delta, k)
+    qs = np.array([O.q_from_b_pgse(b, delta, Delta) for b in (0.0, 1000.0, 2500.0, 4000.0, 500.0)])
+    gd = np.random.default_rng(11).normal(size=(5, 3)) if pg else np.tile([0.0, 1.0, 0.0], (5, 1))
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    ref, st_q = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13, guess=3)
+    got, st_p = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13, guess=0)
+    again, _ = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13, guess=0)
+    print(f"  iterations: projection {st_p['iters_total']} quadratic {st_q['iters_total']}", flush=True)
+    assert np.all(np.abs(got - ref) <= 1e-11 * np.abs(ref))
+    assert np.array_equal(got, again)
+    assert st_p["iters_total"] < st_q["iters_total"]
+    one, _ = ctx.solve(fA, fB, k, qs[3], gd[3], rtol=1e-13, guess=0)
+    assert abs(one - got[3]) <= 1e-11 * abs(got[3])
 
 
 def test_sampled_event_profile_and_spmv_replay():

@@ -106,6 +106,19 @@ def test_periodic_batched_equals_single_and_oracle():
         assert abs(sig[j].real - ref) <= 1e-9 * abs(ref)
 
 
+def test_periodic_projection_guess_matches_oracle():
+    ctx, S = _pair("box_xy_2c")
+    k = 400.0
+    fA, fB, FB = _seq(k)
+    q = O.q_from_b_pgse(2000.0, DELTA, BIGDELTA)
+    ref, _, _ = S.solve((0.6, 0.8, 0.0), q, k, fA, fB, FB=FB, rtol=1e-13)
+    got, st = ctx.solve(fA, fB, k, q, (0.6, 0.8, 0.0), rtol=1e-13, F_explicit=FB, guess=0)
+    quad, stq = ctx.solve(fA, fB, k, q, (0.6, 0.8, 0.0), rtol=1e-13, F_explicit=FB, guess=3)
+    assert abs(got.real - ref) <= 1e-9 * abs(ref)
+    assert abs(got - quad) <= 1e-11 * abs(quad)
+    assert st["iters_total"] < stq["iters_total"]
+
+
 def test_all_faces_periodic_keeps_b0_signal_one():
     ctx, _ = _pair("box_xyz")
     fA, fB, FB = _seq(400.0)
