@@ -2164,54 +2164,70 @@ __global__ void __launch_bounds__(kThreads) proj_finalize_kernel(const double *p
 }
 
 // Per signal (one thread): Jacobi-scaled Cholesky of the Gram matrix with truncation at the
-// first pivot below kProjPivot, solve, and the coefficients on the raw states.
+// first pivot below kProjPivot, solve, and the coefficients on the raw states.  The totals are
+// staged in LDS by the whole block first; the factor overwrites the packed lower triangle.
 template <int MH>
-__global__ void proj_solve_kernel(ProjArgs p) {
+__global__ void __launch_bounds__(64) proj_solve_kernel(ProjArgs p) {
     constexpr int NG = MH * (MH + 1) / 2;
     constexpr int NV = proj_nv(MH);
+    __shared__ double sm[kMaxSignals * NV];
+    for (int i = threadIdx.x; i < p.S * NV; i += blockDim.x) sm[i] = p.tot[i];
+    __syncthreads();
     const int s = threadIdx.x;
     if (s >= p.S) return;
-    const double *t = p.tot + (size_t)s * NV;
+    const double *t = sm + (size_t)s * NV;
     const int nh = p.nh;
-    double2 G[MH][MH], L[MH][MH], z[MH], c[MH];
+    // lower triangle, packed row-major: (i, j), j <= i  ->  i (i + 1) / 2 + j
+    // the totals hold the upper triangle (j, l), l >= j, row-major: G(i, j) = conj(G(j, i))
+    double2 A[NG], z[MH];
     double D[MH];
-    {
-        int k = 0;
 #pragma unroll
-        for (int j = 0; j < MH; ++j)
+    for (int i = 0; i < MH; ++i)
 #pragma unroll
-            for (int l = j; l < MH; ++l, ++k) {
-                G[j][l] = make_double2(t[2 * k], t[2 * k + 1]);
-                G[l][j] = make_double2(t[2 * k], -t[2 * k + 1]);
-            }
-    }
+        for (int j = 0; j <= i; ++j) {
+            const int k = j * MH - j * (j - 1) / 2 + (i - j);  // packed upper index of (j, i)
+            A[i * (i + 1) / 2 + j] = make_double2(t[2 * k], -t[2 * k + 1]);
+        }
 #pragma unroll
     for (int j = 0; j < MH; ++j) {
-        D[j] = (j < nh && G[j][j].x > 0.0) ? sqrt(G[j][j].x) : 0.0;
+        const double gjj = A[j * (j + 1) / 2 + j].x;
+        D[j] = (j < nh && gjj > 0.0) ? sqrt(gjj) : 0.0;
         z[j] = make_double2(t[2 * NG + 2 * j], t[2 * NG + 2 * j + 1]);
-        c[j] = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int l = 0; l < MH; ++l) L[j][l] = make_double2(0.0, 0.0);
     }
+    // scaled matrix: A(i, j) / (D_i D_j), unit diagonal
+#pragma unroll
+    for (int i = 0; i < MH; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            const double sc = (D[i] > 0.0 && D[j] > 0.0) ? 1.0 / (D[i] * D[j]) : 0.0;
+            double2 &v = A[i * (i + 1) / 2 + j];
+            v.x *= sc;
+            v.y *= sc;
+        }
     int m = 0;
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < MH; ++j) {
         if (ok && j < nh && D[j] > 0.0) {
-            double d = 1.0;
+            double d = A[j * (j + 1) / 2 + j].x;
 #pragma unroll
-            for (int k = 0; k < j; ++k) d -= L[j][k].x * L[j][k].x + L[j][k].y * L[j][k].y;
+            for (int k = 0; k < j; ++k) {
+                const double2 l = A[j * (j + 1) / 2 + k];
+                d -= l.x * l.x + l.y * l.y;
+            }
             if (d > kProjPivot) {
                 const double ljj = sqrt(d);
-                L[j][j] = make_double2(ljj, 0.0);
+                A[j * (j + 1) / 2 + j] = make_double2(ljj, 0.0);
 #pragma unroll
                 for (int i = j + 1; i < MH; ++i) {
-                    if (i < nh && D[i] > 0.0) {
-                        const double sc = 1.0 / (D[i] * D[j]);
-                        double2 v = make_double2(G[i][j].x * sc, G[i][j].y * sc);
+                    if (i < nh) {
+                        double2 v = A[i * (i + 1) / 2 + j];
 #pragma unroll
-                        for (int k = 0; k < j; ++k) v = c_sub(v, c_mul(L[i][k], make_double2(L[j][k].x, -L[j][k].y)));
-                        L[i][j] = make_double2(v.x / ljj, v.y / ljj);
+                        for (int k = 0; k < j; ++k) {
+                            const double2 ljk = A[j * (j + 1) / 2 + k];
+                            v = c_sub(v, c_mul(A[i * (i + 1) / 2 + k], make_double2(ljk.x, -ljk.y)));
+                        }
+                        A[i * (i + 1) / 2 + j] = make_double2(v.x / ljj, v.y / ljj);
                     }
                 }
                 m = j + 1;
@@ -2222,14 +2238,15 @@ __global__ void proj_solve_kernel(ProjArgs p) {
             ok = false;
         }
     }
-    // forward: L y = W^H b / D ; back: L^H x = y ; c = x / D
+    // forward: L y = W^H b / D ; back: L^H x = y ; coefficient of Delta^j: x_j / D_j
 #pragma unroll
     for (int j = 0; j < MH; ++j) {
         if (j < m) {
             double2 v = make_double2(z[j].x / D[j], z[j].y / D[j]);
 #pragma unroll
-            for (int k = 0; k < j; ++k) v = c_sub(v, c_mul(L[j][k], z[k]));
-            z[j] = make_double2(v.x / L[j][j].x, v.y / L[j][j].x);
+            for (int k = 0; k < j; ++k) v = c_sub(v, c_mul(A[j * (j + 1) / 2 + k], z[k]));
+            const double ljj = A[j * (j + 1) / 2 + j].x;
+            z[j] = make_double2(v.x / ljj, v.y / ljj);
         }
     }
 #pragma unroll
@@ -2238,8 +2255,12 @@ __global__ void proj_solve_kernel(ProjArgs p) {
             double2 v = z[j];
 #pragma unroll
             for (int k = j + 1; k < MH; ++k)
-                if (k < m) v = c_sub(v, c_mul(make_double2(L[k][j].x, -L[k][j].y), c[k]));
-            c[j] = make_double2(v.x / L[j][j].x, v.y / L[j][j].x);
+                if (k < m) {
+                    const double2 lkj = A[k * (k + 1) / 2 + j];
+                    v = c_sub(v, c_mul(make_double2(lkj.x, -lkj.y), z[k]));
+                }
+            const double ljj = A[j * (j + 1) / 2 + j].x;
+            z[j] = make_double2(v.x / ljj, v.y / ljj);  // back substitution in place (k > j done)
         }
     }
     // y0 = sum_j c_j Delta^j = sum_i gamma_i y_i, gamma_i = sum_{j >= i} c_j (-1)^i C(j, i)
@@ -2249,7 +2270,7 @@ __global__ void proj_solve_kernel(ProjArgs p) {
 #pragma unroll
     for (int j = 0; j < MH; ++j) {
         if (j < m) {
-            const double2 cj = make_double2(c[j].x / D[j], c[j].y / D[j]);
+            const double2 cj = make_double2(z[j].x / D[j], z[j].y / D[j]);
             double binom = 1.0;  // C(j, i)
 #pragma unroll
             for (int i = 0; i <= j; ++i) {
