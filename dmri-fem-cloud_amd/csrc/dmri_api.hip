@@ -354,29 +354,17 @@ void assemble_impl(dmri_ctx *c) {
                        flist, c->iface_v.p, c->dof_of.p, nullptr, c->rowptr.p, c->col.p, overflow.p, s);
     }
 
-    // 5. colouring + assembly
-    for (auto &a : c->val) {
-        a.alloc(c->nnz);
-        a.zero(s);
-    }
+    // 5. cell matrices by row gather (every entry written once); interface facets by colour
+    for (auto &a : c->val) a.alloc(c->nnz);
+    c->val[DMRI_MAT_I].zero(s);
     DevArray<int> miss;
     miss.alloc(1);
     miss.zero(s);
-    {
-        DevArray<int> perm, ct, cph;
-        std::vector<int64_t> offs = colour_elements(4, c->tets.p, nc, c->vptr.p, c->vlist.p, perm, s);
-        c->ncol_cells = (int)offs.size() - 1;
-        ct.alloc(4 * nc);
-        cph.alloc(nc);
-        launch_gather_cells(perm.p, nc, c->tets.p, phase, ct.p, cph.p, s);
-        for (int k = 0; k < c->ncol_cells; ++k)
-            launch_assemble_cells(offs[k], offs[k + 1] - offs[k], nc, perm.p, ct.p, cph.p, c->xyz.p, nv, c->dof_of.p,
-                                  c->rowptr.p, c->col.p, c->Dc.p, c->has_Dcell ? c->Dcell.p : nullptr, c->iT2c.p,
-                                  c->has_iT2cell ? c->iT2cell.p : nullptr, c->val[DMRI_MAT_M].p,
-                                  c->val[DMRI_MAT_K].p, c->val[DMRI_MAT_R].p, c->val[DMRI_MAT_JX].p,
-                                  c->val[DMRI_MAT_JY].p, c->val[DMRI_MAT_JZ].p, miss.p, s);
-        DMRI_HIP(hipStreamSynchronize(s));
-    }
+    c->ncol_cells = 0;
+    launch_assemble_rows(c->ndof, c->dof_vert.p, c->dof_comp.p, c->rowptr.p, c->col.p, c->tets.p, phase, c->xyz.p,
+                         c->vptr.p, c->vlist.p, c->Dc.p, c->has_Dcell ? c->Dcell.p : nullptr, c->iT2c.p,
+                         c->has_iT2cell ? c->iT2cell.p : nullptr, c->val[DMRI_MAT_M].p, c->val[DMRI_MAT_K].p,
+                         c->val[DMRI_MAT_R].p, c->val[DMRI_MAT_JX].p, c->val[DMRI_MAT_JY].p, c->val[DMRI_MAT_JZ].p, s);
     c->ncol_iface = 0;
     if (have_iface) {
         DevArray<int> perm;

@@ -135,13 +135,12 @@ void launch_jp_round(int k, int64_t ne, int nvert, const int *ev, const int *vpt
                      int *remaining, hipStream_t s);
 void launch_color_hist(const int *color, int64_t ne, int *hist, hipStream_t s);
 void launch_color_scatter(const int *color, int64_t ne, const int *offs, int *cursor, int *perm, hipStream_t s);
-void launch_gather_cells(const int *perm, int64_t ne, const int *tets, const int *phase, int *ct, int *cph,
-                         hipStream_t s);
-void launch_assemble_cells(int64_t start, int64_t count, int64_t ne, const int *perm, const int *ct, const int *cph,
-                           const double *xyz, int64_t nv, const int *dof_of, const int *rowptr, const int *col,
-                           const double *Dc, const double *Dcell, const double *iT2c, const double *iT2cell,
-                           double *m, double *k, double *r, double *jx, double *jy, double *jz, int *miss,
-                           hipStream_t s);
+// M, K, R, Jx, Jy, Jz of every pattern entry by row gather over the incident cells (writes every
+// entry; phase may be null for one compartment)
+void launch_assemble_rows(int64_t n, const int *dof_vert, const int *dof_comp, const int *rowptr, const int *col,
+                          const int *tets, const int *phase, const double *xyz, const int *vptr, const int *vlist,
+                          const double *Dc, const double *Dcell, const double *iT2c, const double *iT2cell, double *m,
+                          double *k, double *r, double *jx, double *jy, double *jz, hipStream_t s);
 void launch_assemble_facets(int64_t start, int64_t count, const int *perm, const int *iface_v, const double *xyz,
                             int64_t nv, const int *dof_of, const int *rowptr, const int *col, double kappa,
                             const double *kappa_f, double *gi, int *miss, hipStream_t s);

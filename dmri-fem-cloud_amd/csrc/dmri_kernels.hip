@@ -515,21 +515,12 @@ __global__ void color_scatter_kernel(const int *color, int64_t ne, const int *of
     perm[offs[c] + pos] = (int)e;
 }
 
-// structure-of-arrays copy of the cells in colour order (coalesced vertex-index reads)
-__global__ void gather_cells_kernel(const int *perm, int64_t ne, const int *tets, const int *phase, int *ct,
-                                    int *cph) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ne) return;
-    int t = perm[i];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) ct[(int64_t)a * ne + i] = tets[4 * (int64_t)t + a];
-    cph[i] = phase ? phase[t] : 0;
-}
-
 // --------------------------------------------------------------------------------------
-// closed-form P1 element assembly (one thread per element of one colour; no two elements
-// of a colour share a vertex, so the read-modify-write of the CSR values is race free and
-// every entry receives its contributions in colour order).
+// closed-form P1 element assembly by row gather: a group of kRowLanes lanes per DOF row, lane j
+// owning the row's entries j, j + kRowLanes, ...  Each lane walks the incident cells of the row's
+// vertex (sorted) and adds, for every cell of the row's compartment that holds the entry's
+// column vertex, the element entry.  Every entry is written once by one lane, so there is no
+// colouring and no atomic; the summation order per entry is the sorted cell order.
 // --------------------------------------------------------------------------------------
 __device__ __forceinline__ int csr_find(const int *rowptr, const int *col, int r, int c) {
     int lo = rowptr[r], hi = rowptr[r + 1] - 1;
@@ -543,84 +534,114 @@ __device__ __forceinline__ int csr_find(const int *rowptr, const int *col, int r
     return -1;
 }
 
-__global__ void assemble_cells_kernel(int64_t start, int64_t count, int64_t ne, const int *perm, const int *ct,
-                                      const int *cph, const double *xyz, int64_t nv, const int *dof_of,
-                                      const int *rowptr, const int *col, const double *Dc, const double *Dcell,
-                                      const double *iT2c, const double *iT2cell, double *mv, double *kv, double *rv,
-                                      double *jxv, double *jyv, double *jzv, int *miss) {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= count) return;
-    const int64_t i = start + g;
-    const int t = perm[i];
-    const int c = cph[i];
-    int vid[4];
-    double X[4][3];
+constexpr int kRowLanes = 16;
+
+__global__ void __launch_bounds__(256) assemble_rows_kernel(int64_t n, const int *dof_vert, const int *dof_comp,
+                                                            const int *rowptr, const int *col, const int *tets,
+                                                            const int *phase, const double *xyz, const int *vptr,
+                                                            const int *vlist, const double *Dc, const double *Dcell,
+                                                            const double *iT2c, const double *iT2cell, double *mv,
+                                                            double *kv, double *rv, double *jxv, double *jyv,
+                                                            double *jzv) {
+    const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes;
+    const int lane = threadIdx.x % kRowLanes;
+    if (r >= n) return;
+    const int v = dof_vert[r], cr = dof_comp[r];
+    const int e0 = rowptr[r], e1 = rowptr[r + 1];
+    const int q0 = vptr[v], q1 = vptr[v + 1];
+    for (int e = e0 + lane; e < e1; e += kRowLanes) {
+        const int cl = col[e];
+        const int vj = dof_vert[cl];
+        double m = 0.0, kk = 0.0, rr = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+        if (dof_comp[cl] == cr) {
+            for (int q = q0; q < q1; ++q) {
+                const int t = vlist[q];
+                if ((phase ? phase[t] : 0) != cr) continue;
+                const int4 tv = *reinterpret_cast<const int4 *>(tets + 4 * (int64_t)t);
+                const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+                int a = 0, b = -1;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        vid[a] = ct[(int64_t)a * ne + i];
+                for (int k = 0; k < 4; ++k) {
+                    if (vid[k] == v) a = k;
+                    if (vid[k] == vj) b = k;
+                }
+                if (b < 0) continue;
+                double X[4][3];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) X[a][q] = xyz[3 * (int64_t)vid[a] + q];
-    }
-    double e[3][3];
+                for (int k = 0; k < 4; ++k)
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+                    for (int d = 0; d < 3; ++d) X[k][d] = xyz[3 * (int64_t)vid[k] + d];
+                double E[3][3];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) e[a][q] = X[a + 1][q] - X[0][q];
-    const double c23[3] = {e[1][1] * e[2][2] - e[1][2] * e[2][1], e[1][2] * e[2][0] - e[1][0] * e[2][2],
-                           e[1][0] * e[2][1] - e[1][1] * e[2][0]};
-    const double c31[3] = {e[2][1] * e[0][2] - e[2][2] * e[0][1], e[2][2] * e[0][0] - e[2][0] * e[0][2],
-                           e[2][0] * e[0][1] - e[2][1] * e[0][0]};
-    const double c12[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
-                           e[0][0] * e[1][1] - e[0][1] * e[1][0]};
-    const double det = e[0][0] * c23[0] + e[0][1] * c23[1] + e[0][2] * c23[2];
-    double G[4][3];
+                for (int k = 0; k < 3; ++k)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        G[1][q] = c23[q] / det;
-        G[2][q] = c31[q] / det;
-        G[3][q] = c12[q] / det;
-        G[0][q] = -(G[1][q] + G[2][q] + G[3][q]);
-    }
-    const double vol = fabs(det) / 6.0;
-    double D[9];
-    if (Dcell) {
+                    for (int d = 0; d < 3; ++d) E[k][d] = X[k + 1][d] - X[0][d];
+                // rows of the inverse Jacobian (gradients of the barycentric functions 1..3)
+                const double c23[3] = {E[1][1] * E[2][2] - E[1][2] * E[2][1], E[1][2] * E[2][0] - E[1][0] * E[2][2],
+                                       E[1][0] * E[2][1] - E[1][1] * E[2][0]};
+                const double c31[3] = {E[2][1] * E[0][2] - E[2][2] * E[0][1], E[2][2] * E[0][0] - E[2][0] * E[0][2],
+                                       E[2][0] * E[0][1] - E[2][1] * E[0][0]};
+                const double c12[3] = {E[0][1] * E[1][2] - E[0][2] * E[1][1], E[0][2] * E[1][0] - E[0][0] * E[1][2],
+                                       E[0][0] * E[1][1] - E[0][1] * E[1][0]};
+                const double det = E[0][0] * c23[0] + E[0][1] * c23[1] + E[0][2] * c23[2];
+                double G[4][3];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) D[q] = Dcell[9 * (int64_t)t + q];
-    } else {
+                for (int d = 0; d < 3; ++d) {
+                    G[1][d] = c23[d] / det;
+                    G[2][d] = c31[d] / det;
+                    G[3][d] = c12[d] / det;
+                    G[0][d] = -(G[1][d] + G[2][d] + G[3][d]);
+                }
+                const double vol = fabs(det) / 6.0;
+                double Ga[3], Gb[3], Xa[3], Xb[3];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) D[q] = 0.0;
-        D[0] = D[4] = D[8] = Dc[c];
-    }
-    const double it2 = iT2cell ? iT2cell[t] : iT2c[c];
-    double S[3] = {0.0, 0.0, 0.0};
+                for (int d = 0; d < 3; ++d) {
+                    Ga[d] = G[0][d];
+                    Gb[d] = G[0][d];
+                    Xa[d] = X[0][d];
+                    Xb[d] = X[0][d];
+                }
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+                for (int k = 1; k < 4; ++k)
+                    if (a == k || b == k)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) S[q] += X[a][q];
-    int d[4];
+                        for (int d = 0; d < 3; ++d) {
+                            if (a == k) { Ga[d] = G[k][d]; Xa[d] = X[k][d]; }
+                            if (b == k) { Gb[d] = G[k][d]; Xb[d] = X[k][d]; }
+                        }
+                double Dm[9];
+                if (Dcell) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) d[a] = dof_of[(int64_t)c * nv + vid[a]];
+                    for (int d = 0; d < 9; ++d) Dm[d] = Dcell[9 * (int64_t)t + d];
+                } else {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        double DG[3];
+                    for (int d = 0; d < 9; ++d) Dm[d] = 0.0;
+                    Dm[0] = Dm[4] = Dm[8] = Dc[cr];
+                }
+                const double it2 = iT2cell ? iT2cell[t] : iT2c[cr];
+                const double dab = (a == b) ? 2.0 : 1.0;
+                const double mab = vol * dab / 20.0;
+                m += mab;
+                rr += it2 * mab;
+                double kab = 0.0;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) DG[q] = D[3 * q] * G[a][0] + D[3 * q + 1] * G[a][1] + D[3 * q + 2] * G[a][2];
-        for (int b = 0; b < 4; ++b) {
-            int pos = csr_find(rowptr, col, d[a], d[b]);
-            if (pos < 0) {
-                atomicOr(miss, 1);
-                continue;
+                for (int d = 0; d < 3; ++d) kab += Gb[d] * (Dm[3 * d] * Ga[0] + Dm[3 * d + 1] * Ga[1] + Dm[3 * d + 2] * Ga[2]);
+                kk += vol * kab;
+                const double w = vol * dab / 120.0;
+                const double S0 = X[0][0] + X[1][0] + X[2][0] + X[3][0];
+                const double S1 = X[0][1] + X[1][1] + X[2][1] + X[3][1];
+                const double S2 = X[0][2] + X[1][2] + X[2][2] + X[3][2];
+                jx += w * (S0 + Xa[0] + Xb[0]);
+                jy += w * (S1 + Xa[1] + Xb[1]);
+                jz += w * (S2 + Xa[2] + Xb[2]);
             }
-            const double dab = (a == b) ? 2.0 : 1.0;
-            const double mab = vol * dab / 20.0;
-            mv[pos] += mab;
-            rv[pos] += it2 * mab;
-            kv[pos] += vol * (G[b][0] * DG[0] + G[b][1] * DG[1] + G[b][2] * DG[2]);
-            const double w = vol * dab / 120.0;
-            jxv[pos] += w * (S[0] + X[a][0] + X[b][0]);
-            jyv[pos] += w * (S[1] + X[a][1] + X[b][1]);
-            jzv[pos] += w * (S[2] + X[a][2] + X[b][2]);
         }
+        mv[e] = m;
+        kv[e] = kk;
+        rv[e] = rr;
+        jxv[e] = jx;
+        jyv[e] = jy;
+        jzv[e] = jz;
     }
 }
 
@@ -2536,20 +2557,14 @@ void launch_color_scatter(const int *color, int64_t ne, const int *offs, int *cu
     color_scatter_kernel<<<grid_for(ne, 256), 256, 0, s>>>(color, ne, offs, cursor, perm);
     check_launch();
 }
-void launch_gather_cells(const int *perm, int64_t ne, const int *tets, const int *phase, int *ct, int *cph,
-                         hipStream_t s) {
-    gather_cells_kernel<<<grid_for(ne, 256), 256, 0, s>>>(perm, ne, tets, phase, ct, cph);
-    check_launch();
-}
-void launch_assemble_cells(int64_t start, int64_t count, int64_t ne, const int *perm, const int *ct, const int *cph,
-                           const double *xyz, int64_t nv, const int *dof_of, const int *rowptr, const int *col,
-                           const double *Dc, const double *Dcell, const double *iT2c, const double *iT2cell,
-                           double *m, double *k, double *r, double *jx, double *jy, double *jz, int *miss,
-                           hipStream_t s) {
-    if (count <= 0) return;
-    assemble_cells_kernel<<<grid_for(count, 128), 128, 0, s>>>(start, count, ne, perm, ct, cph, xyz, nv,
-                                                               dof_of, rowptr, col, Dc, Dcell, iT2c, iT2cell, m, k, r,
-                                                               jx, jy, jz, miss);
+void launch_assemble_rows(int64_t n, const int *dof_vert, const int *dof_comp, const int *rowptr, const int *col,
+                          const int *tets, const int *phase, const double *xyz, const int *vptr, const int *vlist,
+                          const double *Dc, const double *Dcell, const double *iT2c, const double *iT2cell, double *m,
+                          double *k, double *r, double *jx, double *jy, double *jz, hipStream_t s) {
+    if (n <= 0) return;
+    assemble_rows_kernel<<<grid_for(n * kRowLanes, 256), 256, 0, s>>>(n, dof_vert, dof_comp, rowptr, col, tets, phase,
+                                                                       xyz, vptr, vlist, Dc, Dcell, iT2c, iT2cell, m,
+                                                                       k, r, jx, jy, jz);
     check_launch();
 }
 void launch_assemble_facets(int64_t start, int64_t count, const int *perm, const int *iface_v, const double *xyz,

@@ -94,7 +94,8 @@ int dmri_assemble(dmri_ctx *ctx);
 
 typedef struct {
     int64_t nv, nc, ncomp, ndof, nnz, n_iface, n_bfacet;
-    int32_t ncolors_cells, ncolors_iface;
+    int32_t ncolors_cells;  /* 0: the cell matrices are assembled by row gather, without colouring */
+    int32_t ncolors_iface;  /* colours of the interface-facet assembly */
     double volume;        /* sum of M (|Omega|; whole_vol of PostProcessing, DmriFemLib.py:923) */
     double hmin, hmax;    /* min/max greatest vertex distance of a cell (mymesh.hmin/hmax) */
     double setup_ms;      /* device time of dmri_assemble */
