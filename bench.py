@@ -427,7 +427,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="counter runs only: truncate to the first K theta steps (result line not comparable)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel reference")
-    ap.add_argument("--event-iters", type=int, default=2,
+    ap.add_argument("--event-iters", type=int, default=1,
                     help="Krylov iterations per theta step whose SpMV launches are event-timed")
     args = ap.parse_args()
 

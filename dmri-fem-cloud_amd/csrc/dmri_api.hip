@@ -36,8 +36,8 @@ struct SolverWork {
     DevArray<double> sc, vals, bre, part, tot, coef, coefP, sig, spart, ppart, ptot, pgam, gsig;
     DevArray<KrylovState> st;
     DevArray<int> all_done, mask;
-    KrylovState *h_st = nullptr;
-    int *h_flag = nullptr;
+    KrylovState *h_st = nullptr, *d_h_st = nullptr;  // pinned host copy and its device address
+    int *h_flag = nullptr, *d_h_flag = nullptr;
     SolverWork() = default;
     SolverWork(const SolverWork &) = delete;
     SolverWork &operator=(const SolverWork &) = delete;
@@ -488,8 +488,15 @@ void ensure_group_workspace(dmri_ctx *c, const SolveGeom &g, int nsteps, bool pe
     if (!w.mask.p) w.mask.alloc(kMaxSignals);
     if (w.sig.n < (size_t)kMaxSignals * kNV) w.sig.alloc(kMaxSignals * kNV);
     if (w.spart.n < (size_t)kMaxGrid * kMaxSignals * kNV) w.spart.alloc((size_t)kMaxGrid * kMaxSignals * kNV);
-    if (!w.h_st) DMRI_HIP(hipHostMalloc((void **)&w.h_st, sizeof(KrylovState) * kMaxSignals, hipHostMallocDefault));
-    if (!w.h_flag) DMRI_HIP(hipHostMalloc((void **)&w.h_flag, sizeof(int) * 4, hipHostMallocDefault));
+    if (!w.h_st) {
+        DMRI_HIP(hipHostMalloc((void **)&w.h_st, sizeof(KrylovState) * kMaxSignals,
+                               hipHostMallocMapped | hipHostMallocCoherent));
+        DMRI_HIP(hipHostGetDevicePointer((void **)&w.d_h_st, w.h_st, 0));
+    }
+    if (!w.h_flag) {
+        DMRI_HIP(hipHostMalloc((void **)&w.h_flag, sizeof(int) * 4, hipHostMallocMapped | hipHostMallocCoherent));
+        DMRI_HIP(hipHostGetDevicePointer((void **)&w.d_h_flag, w.h_flag, 0));
+    }
     if (!c->events_made) {
         for (int i = 0; i < kEventPool; ++i) {
             DMRI_HIP(hipEventCreate(&c->ev0[i]));
@@ -632,7 +639,6 @@ struct Stepper {
     bool poly = true;  // degree-1 polynomial preconditioner
     double omt = 0.5;
     int chunk = 8;
-    std::vector<KrylovState> zero_st;
 
     // q[s], g[3*s] (unit).  Scales the system, uploads the per-step coefficients, sets u0 = 1.
     void setup(dmri_ctx *ctx, int S_, const double *q, const double *g, bool pg, const dmri_seq *seq,
@@ -763,7 +769,6 @@ struct Stepper {
                          (poly ? 56.0 : 48.0) * (double)n * S;
         prof = opts->profile != 0;
         chunk = 8;
-        zero_st.assign(S, KrylovState{});
     }
 
     // Residual-minimising guess over the last fh.m states (see proj_capacity); the quadratic
@@ -886,8 +891,7 @@ struct Stepper {
         SolverWork &w = c->work;
         a.coefA = w.coef.p + (size_t)step * S * 3;
         a.coefB = w.coef.p + (size_t)std::max(N, 1) * S * 3 + (size_t)step * S * 3;
-        std::memset(zero_st.data(), 0, sizeof(KrylovState) * S);
-        DMRI_HIP(hipMemcpyAsync(w.st.p, zero_st.data(), sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
+        DMRI_HIP(hipMemsetAsync(w.st.p, 0, sizeof(KrylovState) * S, s));
         DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
     }
 
@@ -920,8 +924,7 @@ struct Stepper {
                 ntimed += timed ? 1 : 0;
                 launch_bicg_update(geo, a, it, s);
             }
-            DMRI_HIP(hipMemcpyAsync(w.h_flag, w.all_done.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            DMRI_HIP(hipMemcpyAsync(w.h_st, w.st.p, sizeof(KrylovState) * S, hipMemcpyDeviceToHost, s));
+            launch_publish_state(w.st.p, S, w.all_done.p, w.d_h_st, w.d_h_flag, s);
             DMRI_HIP(hipStreamSynchronize(s));
             const bool all = w.h_flag[0] != 0;
             int it_last = it;
@@ -983,7 +986,9 @@ struct Stepper {
         }
         st->iters_max = std::max(st->iters_max, step_max);
         st->restarts += restarts;
-        chunk = std::max(1, step_max);
+        // the next step converges in about as many iterations; its convergence shows in the
+        // head of the iteration after the last one, so that iteration is queued too
+        chunk = std::max(1, step_max) + 1;
     }
 
     // One theta step of the forward problem on xc (xp, xq hold the two previous states).

@@ -2994,6 +2994,20 @@ void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStrea
         check_launch();
     }
 }
+// Krylov states and the all-done flag into pinned host memory: one dispatch at every convergence
+// check instead of two device-to-host copies
+__global__ void __launch_bounds__(256) publish_state_kernel(const int *st, int nwords, const int *flag, int *hst,
+                                                            int *hflag) {
+    for (int i = threadIdx.x; i < nwords; i += blockDim.x) hst[i] = st[i];
+    if (threadIdx.x == 0) hflag[0] = flag[0];
+}
+void launch_publish_state(const KrylovState *st, int S, const int *flag, KrylovState *hst, int *hflag, hipStream_t s) {
+    const int nwords = (int)(S * sizeof(KrylovState) / sizeof(int));
+    publish_state_kernel<<<1, 256, 0, s>>>(reinterpret_cast<const int *>(st), nwords, flag,
+                                           reinterpret_cast<int *>(hst), hflag);
+    check_launch();
+}
+
 int proj_grid(const SolveGeom &g) { return g.vgrid; }
 
 void launch_proj_guess(const SolveGeom &g, const ProjArgs &p, int mh, hipStream_t s) {
