@@ -860,6 +860,38 @@ struct Stepper {
         }
     }
 
+    // State b steps back from the newest (0 = xc), for checkpoints; the newest when there is none
+    const double2 *state_back(int b) const {
+        if (fh.m > 0) return b < (int)fh.ys.size() ? fh.ys[b] : xc;
+        return b == 0 ? xc : (b == 1 ? xp : xq);
+    }
+
+    // Restart the projection history from count stored states (newest first): the older ones get
+    // their history products, the newest becomes xc (its products come with the next step).
+    void reseed_projection(const double2 *const *states, int count) {
+        SolverWork &w = c->work;
+        fh.reset(fh.m, w.yh, w.hp, w.hq);
+        fh.yfree.push_back(w.x.p);
+        const size_t vb = sizeof(double2) * (size_t)n * S;
+        for (int i = count - 1; i >= 0; --i) {
+            double2 *y = fh.take_free();
+            DMRI_HIP(hipMemcpyAsync(y, states[i], vb, hipMemcpyDeviceToDevice, s));
+            if (i > 0) {
+                const auto slot = fh.take_slot();
+                a.x = y;
+                a.hp = slot.first;
+                a.hq = slot.second;
+                launch_rhs_residual(geo, a, 4, nullptr, s);  // products; b is formed again by the step
+                fh.push(y, slot);
+            } else {
+                fh.ys.push_front(y);
+            }
+        }
+        xc = fh.ys[0];
+        a.x = xc;
+        a.hp = a.hq = nullptr;
+    }
+
     // Adjoint solve of step k: A_k y = rhs, from the projection over h (from *start when h is
     // empty); returns the buffer holding the solution.
     double2 *solve_projected(int k, const double2 *rhs, ProjHistory &h, double2 *start) {
@@ -1097,18 +1129,15 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     }
     K = std::max(1, std::min(K, std::max(N, 1)));
     const bool keep_all = K >= N;
-    // With checkpoints the forward sweep extrapolates from at most three states (a checkpoint
-    // holds three, so a recomputed segment repeats the sweep bit for bit): the projection guess
-    // becomes the quadratic extrapolation there.  The backward sweep runs once and keeps its own
-    // projection history of adjoint states.
+    // A checkpoint holds three states.  With the extrapolated guesses (initial_guess 1-3) they are
+    // the whole history of the next step, so a recomputed segment repeats the forward sweep bit
+    // for bit.  The projection guess (the default) restarts its history from them, so recomputed
+    // states agree with the first sweep's to the solver tolerance.  The backward sweep keeps its
+    // own projection history of adjoint states.
     const bool proj = opts->initial_guess == 0;
-    dmri_solver_opts fopts = *opts;
-    if (proj && !keep_all) fopts.initial_guess = 3;
-    run.setup(c, 1, &seq->q, g, false, seq, &fopts, &st);
+    run.setup(c, 1, &seq->q, g, false, seq, opts, &st);
     const int nck = (N + K - 1) / K;
-    // checkpoints (x^k, x^(k-1), x^(k-2)) at k = 0, K, 2K, ... (the history the initial guess of
-    // step k extrapolates from, so a recomputed segment repeats the forward sweep bit for bit);
-    // a segment holds x^k0 .. x^k1
+    // checkpoints (x^k, x^(k-1), x^(k-2)) at k = 0, K, 2K, ...; a segment holds x^k0 .. x^k1
     DevArray<double2> ck, seg, mu, lam, lam2;
     ck.alloc((size_t)std::max(nck, 1) * 3 * n);
     seg.alloc((size_t)(std::min(K, std::max(N, 1)) + 1) * n);
@@ -1119,12 +1148,12 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     auto segp = [&](int j) { return seg.p + (size_t)j * n; };
     // forward sweep
     for (int k = 0; k < N; ++k) {
-        if (k % K == 0) {
+        if (k % K == 0 && !keep_all) {
             const int j = k / K;
             DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j) * n, run.xc, vb, hipMemcpyDeviceToDevice, s));
-            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j + 2) * n, k > 1 ? run.xq : run.xc, vb,
+            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j + 2) * n, run.state_back(k > 1 ? 2 : 0), vb,
                                     hipMemcpyDeviceToDevice, s));
-            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j + 1) * n, k > 0 ? run.xp : run.xc, vb,
+            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j + 1) * n, run.state_back(k > 0 ? 1 : 0), vb,
                                     hipMemcpyDeviceToDevice, s));
         }
         if (keep_all && k == 0) DMRI_HIP(hipMemcpyAsync(segp(0), run.xc, vb, hipMemcpyDeviceToDevice, s));
@@ -1163,9 +1192,15 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     for (int j = nck - 1; j >= 0; --j) {
         const int k0 = j * K, k1 = std::min(N, k0 + K);
         if (!keep_all) {
-            DMRI_HIP(hipMemcpyAsync(run.xc, ck.p + (size_t)(3 * j) * n, vb, hipMemcpyDeviceToDevice, s));
-            DMRI_HIP(hipMemcpyAsync(run.xp, ck.p + (size_t)(3 * j + 1) * n, vb, hipMemcpyDeviceToDevice, s));
-            DMRI_HIP(hipMemcpyAsync(run.xq, ck.p + (size_t)(3 * j + 2) * n, vb, hipMemcpyDeviceToDevice, s));
+            if (run.fh.m > 0) {
+                const double2 *states[3] = {ck.p + (size_t)(3 * j) * n, ck.p + (size_t)(3 * j + 1) * n,
+                                            ck.p + (size_t)(3 * j + 2) * n};
+                run.reseed_projection(states, std::min(3, k0 + 1));
+            } else {
+                DMRI_HIP(hipMemcpyAsync(run.xc, ck.p + (size_t)(3 * j) * n, vb, hipMemcpyDeviceToDevice, s));
+                DMRI_HIP(hipMemcpyAsync(run.xp, ck.p + (size_t)(3 * j + 1) * n, vb, hipMemcpyDeviceToDevice, s));
+                DMRI_HIP(hipMemcpyAsync(run.xq, ck.p + (size_t)(3 * j + 2) * n, vb, hipMemcpyDeviceToDevice, s));
+            }
             DMRI_HIP(hipMemcpyAsync(segp(0), run.xc, vb, hipMemcpyDeviceToDevice, s));
             for (int k = k0; k < k1; ++k) {
                 run.step(k);
