@@ -611,13 +611,13 @@ struct ProjHistory {
     }
 };
 
-// capacity of a projection history of S-signal states: 8, 6 or 4 states when 3 x capacity
+// capacity of a projection history of S-signal states: 12, 10, 8, 6 or 4 states when 3 x capacity
 // state-sized buffers take at most half of the free device memory (plus what the caller already
 // holds for it), else 0
 int proj_capacity(size_t nS, size_t held_bytes) {
     size_t fr = 0, tot = 0;
     DMRI_HIP(hipMemGetInfo(&fr, &tot));
-    for (int m : {8, 6, 4})
+    for (int m : {12, 10, 8, 6, 4})
         if ((double)m * 3.0 * (double)nS * sizeof(double2) <= 0.5 * (double)(fr + held_bytes)) return m;
     return 0;
 }

@@ -106,6 +106,7 @@ struct SolveGeom {
     int tile_rows = 0;       // kMapTile: rows of a compressed tile (= rpb)
     int tile_ucap = 0;       // kMapTile: distinct columns staged per tile (LDS), set from the build
     int vgrid = 1;           // blocks of the vector kernels
+    int pgrid = 1;           // blocks of the projection Gram kernel
 };
 SolveGeom make_geom(int64_t n, int G, int S, int mapping = kMapSL);
 int stage_rows(const SolveGeom &g, bool per_gdir);
@@ -270,7 +271,8 @@ void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const
 // the backward differences of the history (the raw states are nearly parallel, their Gram matrix
 // is singular to working precision), Jacobi-scaled and solved by a Cholesky factorisation that
 // stops at the first pivot below kProjPivot (the remaining differences are dropped).
-constexpr int kMaxHist = 8;
+constexpr int kMaxHist = 12;
+constexpr int kMaxProjGrid = 1024;
 constexpr double kProjPivot = 1e-12;
 struct ProjArgs {
     int64_t n;
@@ -287,7 +289,7 @@ struct ProjArgs {
 };
 // doubles per signal of the normal equations: upper triangle of the Gram matrix and W^H b, complex
 constexpr int proj_nv(int mh) { return 2 * (mh * (mh + 1) / 2 + mh); }
-// mh: history capacity (4, 6 or 8) >= p.nh; part >= proj_grid() * S * proj_nv(mh) doubles
+// mh: history capacity (4, 6, 8, 10 or 12) >= p.nh; part >= proj_grid() * S * proj_nv(mh) doubles
 int proj_grid(const SolveGeom &g);
 void launch_proj_guess(const SolveGeom &g, const ProjArgs &p, int mh, hipStream_t s);
 void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);

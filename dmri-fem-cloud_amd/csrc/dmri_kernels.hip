@@ -2096,80 +2096,96 @@ __device__ __forceinline__ double proj_coef(const ProjArgs &p, int s) {
            p.coefA[3 * s + 2] * p.gsig[3 * s + 2];
 }
 
-// Lanes: Sp = S rounded up to a power of two, lane = row_in_wave * Sp + signal (lanes with
-// signal >= S idle), so the lanes of one signal in a wave are an xor-closed set and the
-// per-signal sums reduce with shuffles before one LDS pass over the 4 waves.  Block partials:
-// part[block][s][k], k over the packed upper triangle of W^H W (j <= l, row-major) and then W^H b.
+// Blocks (x, y): y selects a window of Sw signals [y Sw, y Sw + Sl); x walks tiles of
+// rpt = 256 / Sl rows (thread t <-> element t of the tile: row t / Sl, window signal t % Sl).
+// Load phase: each thread forms W_j = P_j + i c_s Q_j of its element for the nh states, takes the
+// backward differences in registers and stores them, with b, in LDS.  Pair phase: thread t owns
+// the (pair, signal) combinations q = t, t + 256 (pairs: the upper triangle (j, l), j <= l,
+// row-major, then (j, b)) and sums conj(W_j) W_l over the tile's rows from LDS.  Sw keeps the
+// combinations of a window within 2 per thread.  Every combination belongs to one thread, so the
+// block partials need no further reduction: part[x][s][2 pair + re/im].
 template <int MH>
-__global__ void __launch_bounds__(kThreads) proj_gram_kernel(ProjArgs p, int Sp) {
+__global__ void __launch_bounds__(kThreads) proj_gram_kernel(ProjArgs p, int Sw) {
     constexpr int NG = MH * (MH + 1) / 2;
-    constexpr int NV = proj_nv(MH);
+    constexpr int NP = NG + MH;
+    constexpr int NC = 2;
+    extern __shared__ double2 sw[];  // [MH + 1][kThreads]
     const int S = p.S;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int s = lane & (Sp - 1);
-    const int rw = 64 / Sp;                  // rows per wave
-    const int rb = rw * (kThreads / 64);     // rows per block pass
-    const bool act = s < S;
+    const int s0 = blockIdx.y * Sw;
+    const int Sl = S - s0 < Sw ? S - s0 : Sw;
+    const int rpt = kThreads / Sl;
+    const int nel = rpt * Sl;
+    const int t = threadIdx.x;
     const int nh = p.nh;
-    double2 g[NG], h[MH];
+    double2 acc[NC];
+    int cx[NC], cy[NC];  // LDS offsets of the two columns of each combination (signal included)
 #pragma unroll
-    for (int k = 0; k < NG; ++k) g[k] = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int k = 0; k < MH; ++k) h[k] = make_double2(0.0, 0.0);
-    if (act) {
-        const double c = proj_coef(p, s);
-        for (int64_t row = (int64_t)blockIdx.x * rb + wave * rw + (lane / Sp); row < p.n;
-             row += (int64_t)gridDim.x * rb) {
-            const int64_t i = row * S + s;
+    for (int k = 0; k < NC; ++k) {
+        acc[k] = make_double2(0.0, 0.0);
+        const int q = t + k * kThreads;
+        int pp = q / Sl, j = 0, l;
+        const int sl = q % Sl;
+        if (pp < NG) {
+            while (pp >= MH - j) {
+                pp -= MH - j;
+                ++j;
+            }
+            l = j + pp;
+        } else {
+            j = pp - NG;
+            l = MH;  // the b column
+        }
+        cx[k] = j * kThreads + sl;
+        cy[k] = l * kThreads + sl;
+    }
+    const int64_t ntiles = (p.n + rpt - 1) / rpt;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();  // the previous tile's pair phase is done with the LDS
+        if (t < nel) {
+            const int64_t row = tile * rpt + t / Sl;
+            const int s = s0 + t % Sl;
             double2 w[MH];
 #pragma unroll
-            for (int j = 0; j < MH; ++j) {
-                w[j] = make_double2(0.0, 0.0);
-                if (j < nh) {
-                    const double2 P = p.P[j][i], Q = p.Q[j][i];
-                    w[j] = make_double2(P.x - c * Q.y, P.y + c * Q.x);  // P + i c Q
-                }
+            for (int j = 0; j < MH; ++j) w[j] = make_double2(0.0, 0.0);
+            double2 bv = make_double2(0.0, 0.0);
+            if (row < p.n) {
+                const int64_t e = row * S + s;
+                const double c = proj_coef(p, s);
+#pragma unroll
+                for (int j = 0; j < MH; ++j)
+                    if (j < nh) {
+                        const double2 P = p.P[j][e], Q = p.Q[j][e];
+                        w[j] = make_double2(P.x - c * Q.y, P.y + c * Q.x);  // P + i c Q
+                    }
+                backward_differences<MH>(w, nh);
+                bv = p.b[e];
             }
-            backward_differences<MH>(w, nh);
-            const double2 bv = p.b[i];
-            int k = 0;
 #pragma unroll
-            for (int j = 0; j < MH; ++j) {
-                h[j] = c_add(h[j], c_conj_mul(w[j], bv));
+            for (int j = 0; j < MH; ++j) sw[j * kThreads + t] = w[j];
+            sw[MH * kThreads + t] = bv;
+        }
+        __syncthreads();
 #pragma unroll
-                for (int l = j; l < MH; ++l, ++k) g[k] = c_add(g[k], c_conj_mul(w[j], w[l]));
+        for (int k = 0; k < NC; ++k) {
+            if (t + k * kThreads < NP * Sl) {
+                const double2 *x = sw + cx[k];
+                const double2 *y = sw + cy[k];
+                double2 v = acc[k];
+#pragma unroll 2
+                for (int r = 0; r < rpt; ++r) v = c_add(v, c_conj_mul(x[r * Sl], y[r * Sl]));
+                acc[k] = v;
             }
         }
     }
-    // per-signal sums: shuffles inside the wave, then the 4 waves through LDS, kNV doubles a pass
-    double(&buf)[kNV][kThreads] = reduce_scratch();
+    double *part = p.part + (size_t)blockIdx.x * S * (2 * NP);
 #pragma unroll
-    for (int k0 = 0; k0 < NV; k0 += kNV) {
-        double v[kNV];
-#pragma unroll
-        for (int k = 0; k < kNV; ++k) {
-            const int kk = k0 + k;
-            double x = 0.0;
-            if (kk < 2 * NG) x = (kk & 1) ? g[kk >> 1].y : g[kk >> 1].x;
-            else if (kk < NV) x = ((kk - 2 * NG) & 1) ? h[(kk - 2 * NG) >> 1].y : h[(kk - 2 * NG) >> 1].x;
-            for (int off = 32; off >= Sp; off >>= 1) x += __shfl_xor(x, off, 64);
-            v[k] = x;
+    for (int k = 0; k < NC; ++k) {
+        const int q = t + k * kThreads;
+        if (q < NP * Sl) {
+            const int pp = q / Sl, s = s0 + q % Sl;
+            part[(size_t)s * (2 * NP) + 2 * pp] = acc[k].x;
+            part[(size_t)s * (2 * NP) + 2 * pp + 1] = acc[k].y;
         }
-        if (lane < Sp) {
-#pragma unroll
-            for (int k = 0; k < kNV; ++k) buf[k][wave * 64 + lane] = v[k];
-        }
-        __syncthreads();
-        if ((int)threadIdx.x < S * kNV) {
-            const int ss = threadIdx.x % S, k = threadIdx.x / S;
-            if (k0 + k < NV) {
-                double acc = 0.0;
-#pragma unroll
-                for (int wv = 0; wv < kThreads / 64; ++wv) acc += buf[k][wv * 64 + ss];
-                p.part[((size_t)blockIdx.x * S + ss) * NV + k0 + k] = acc;
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -2184,19 +2200,19 @@ __global__ void __launch_bounds__(kThreads) proj_finalize_kernel(const double *p
     if (threadIdx.x == 0) tot[s * nv + k] = v[0];
 }
 
-// Per signal (one thread): Jacobi-scaled Cholesky of the Gram matrix with truncation at the
-// first pivot below kProjPivot, solve, and the coefficients on the raw states.  The totals are
-// staged in LDS by the whole block first; the factor overwrites the packed lower triangle.
+// Per signal (one block; its totals staged in LDS, then one thread): Jacobi-scaled Cholesky of
+// the Gram matrix with truncation at the first pivot below kProjPivot, solve, and the
+// coefficients on the raw states.  The factor overwrites the packed lower triangle.
 template <int MH>
 __global__ void __launch_bounds__(64) proj_solve_kernel(ProjArgs p) {
     constexpr int NG = MH * (MH + 1) / 2;
     constexpr int NV = proj_nv(MH);
-    __shared__ double sm[kMaxSignals * NV];
-    for (int i = threadIdx.x; i < p.S * NV; i += blockDim.x) sm[i] = p.tot[i];
+    __shared__ double sm[NV];
+    const int s = blockIdx.x;
+    for (int i = threadIdx.x; i < NV; i += blockDim.x) sm[i] = p.tot[(size_t)s * NV + i];
     __syncthreads();
-    const int s = threadIdx.x;
-    if (s >= p.S) return;
-    const double *t = sm + (size_t)s * NV;
+    if (threadIdx.x != 0) return;
+    const double *t = sm;
     const int nh = p.nh;
     // lower triangle, packed row-major: (i, j), j <= i  ->  i (i + 1) / 2 + j
     // the totals hold the upper triangle (j, l), l >= j, row-major: G(i, j) = conj(G(j, i))
@@ -2615,6 +2631,7 @@ SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
     g.grid = (int)std::max<int64_t>(8, std::min<int64_t>(need, kMaxGridSpmv));
     int64_t vneed = (n + g.vec_rpb - 1) / g.vec_rpb;
     g.vgrid = (int)std::max<int64_t>(1, std::min<int64_t>(vneed, kMaxGrid));
+    g.pgrid = (int)std::max<int64_t>(1, std::min<int64_t>(vneed, kMaxProjGrid));
     return g;
 }
 
@@ -3008,22 +3025,24 @@ void launch_publish_state(const KrylovState *st, int S, const int *flag, KrylovS
     check_launch();
 }
 
-int proj_grid(const SolveGeom &g) { return g.vgrid; }
+int proj_grid(const SolveGeom &g) { return g.pgrid; }
 
 void launch_proj_guess(const SolveGeom &g, const ProjArgs &p, int mh, hipStream_t s) {
-    if (p.nh < 1 || p.nh > mh || !(mh == 4 || mh == 6 || mh == 8))
-        throw Error(-1, "projection guess: history size must satisfy 1 <= nh <= mh, mh in {4, 6, 8}");
+    if (p.nh < 1 || p.nh > mh || !(mh == 4 || mh == 6 || mh == 8 || mh == 10 || mh == 12))
+        throw Error(-1, "projection guess: history size must satisfy 1 <= nh <= mh, mh in {4, 6, 8, 10, 12}");
     if (p.S < 1 || p.S > kMaxSignals) throw Error(-1, "projection guess: signals per launch must lie in [1, 64]");
-    int Sp = 1;
-    while (Sp < p.S) Sp <<= 1;
     const int grid = proj_grid(g);
     const int nv = proj_nv(mh);
+    const size_t lds = (size_t)(mh + 1) * kThreads * sizeof(double2);
+    // signal window of the Gram kernel: at most 2 (pair, signal) combinations per thread
+    const int Sw = std::max(1, std::min(p.S, 2 * kThreads / (nv / 2)));
+    const dim3 ggrid(grid, (p.S + Sw - 1) / Sw);
 #define DMRI_PROJ(M)                                                                   \
-    proj_gram_kernel<M><<<grid, kThreads, 0, s>>>(p, Sp);                               \
+    proj_gram_kernel<M><<<ggrid, kThreads, lds, s>>>(p, Sw);                            \
     check_launch();                                                                    \
     proj_finalize_kernel<<<p.S * nv, kThreads, 0, s>>>(p.part, grid, p.S, nv, p.tot); \
     check_launch();                                                                    \
-    proj_solve_kernel<M><<<1, 64, 0, s>>>(p);                                          \
+    proj_solve_kernel<M><<<p.S, 64, 0, s>>>(p);                                        \
     check_launch();                                                                    \
     proj_combine_kernel<M><<<g.vgrid, g.vec_threads, 0, s>>>(p, g.vec_rpb);            \
     check_launch();
@@ -3031,8 +3050,12 @@ void launch_proj_guess(const SolveGeom &g, const ProjArgs &p, int mh, hipStream_
         DMRI_PROJ(4)
     } else if (mh == 6) {
         DMRI_PROJ(6)
-    } else {
+    } else if (mh == 8) {
         DMRI_PROJ(8)
+    } else if (mh == 10) {
+        DMRI_PROJ(10)
+    } else {
+        DMRI_PROJ(12)
     }
 #undef DMRI_PROJ
 }

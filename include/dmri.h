@@ -158,12 +158,12 @@ typedef struct {
                               events cost GPU time of their own, so a small k perturbs least */
     int32_t lanes_per_row; /* SpMV lanes per row: 0 = tuned value (dmri_tune / dmri_set_tuning) or the
                               library default; else a power of two in 1..32 */
-    int32_t initial_guess; /* Krylov start of step n: 0 = the combination of the last 8 states
+    int32_t initial_guess; /* Krylov start of step n: 0 = the combination of the last 12 states
                               (fewer in the first steps) that minimises the residual of step n
-                              (the default; the history takes 24 state-sized buffers, 6 or 4
-                              states are kept when those do not fit in half of the free device
-                              memory, and the quadratic extrapolation below when even 4 do not;
-                              dmri_adjoint's forward sweep always uses the quadratic one),
+                              (the default; the history takes 36 state-sized buffers; 10, 8, 6
+                              or 4 states are kept when those do not fit in half of the free
+                              device memory, and the quadratic extrapolation below when even 4
+                              do not; dmri_adjoint restarts the history at its checkpoints),
                               1 = u^n, 2 = 2u^n - u^(n-1) (linear extrapolation), 3 = 3u^n -
                               3u^(n-1) + u^(n-2) (quadratic extrapolation; linear at n = 1) */
     int32_t mapping;       /* SpMV mapping: 0 = tuned/default, 1 = signals spread over lanes

@@ -239,7 +239,7 @@ def test_initial_guess_modes_agree():
 
 @pytest.mark.parametrize("pg", [False, True])
 def test_projection_guess_batched_matches_quadratic(pg):
-    """The residual-minimising guess over the last 8 states changes only where Krylov starts:
+    """The residual-minimising guess over the last states changes only where Krylov starts:
     batched signals (shared or per-signal directions) agree with the quadratic extrapolation's
     and with single solves, in fewer iterations; a second launch group on the same context
     (history buffers reused) gives the same values."""
