@@ -923,8 +923,7 @@ struct Stepper {
         SolverWork &w = c->work;
         a.coefA = w.coef.p + (size_t)step * S * 3;
         a.coefB = w.coef.p + (size_t)std::max(N, 1) * S * 3 + (size_t)step * S * 3;
-        DMRI_HIP(hipMemsetAsync(w.st.p, 0, sizeof(KrylovState) * S, s));
-        DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
+        launch_reset_state(w.st.p, S, w.all_done.p, s);
     }
 
     // BiCGStab iterations on a.x with the residual already formed; restarts on breakdown.

@@ -301,6 +301,8 @@ void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
 void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
                    hipEvent_t e1 = nullptr, int op = kOpT);
 void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
+// zero st[0..S) and *flag
+void launch_reset_state(KrylovState *st, int S, int *flag, hipStream_t s);
 // st[0..S) and *flag into host-mapped pinned memory (device addresses of hipHostMalloc buffers)
 void launch_publish_state(const KrylovState *st, int S, const int *flag, KrylovState *hst, int *hflag, hipStream_t s);
 // per-signal sum_i w_i sc_i x[i*S+s] into out[2*s..2*s+1]

@@ -1587,6 +1587,33 @@ __global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, in
     if (OP == kOpV) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
 }
 
+// r = b - (Are + i cA.J) x with the stored b, r^ = r, in the SpMV kernels' staged row mapping
+// (the unmasked mode 1 of the residual kernel); partials U: [|r|^2, 0, |r|^2, |b|^2]
+template <int G, bool PG>
+__global__ void __launch_bounds__(kThreads) residual_kernel(SolveArgs a, int S, int rpb, int TR) {
+    const Lane L = spmv_lane<G>(S);
+    const int S_ = S;
+    const int s = L.s;
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    const double2 *x = a.x;
+    staged_spmv<G, PG>(
+        a, rpb, TR, L, true, [&](int c) { return x[(int64_t)c * S_ + s]; },
+        [&](int64_t row, const Acc<PG> &acc) {
+            if (L.g != 0) return;
+            const int64_t idx = row * S_ + s;
+            const double2 ax = imag_coef_apply<G, PG>(acc, a.coefA, s);
+            const double2 b = a.b[idx];
+            const double2 r = make_double2(b.x - ax.x, b.y - ax.y);
+            a.r[idx] = r;
+            a.rh[idx] = r;
+            const double rr = r.x * r.x + r.y * r.y;
+            red[0] += rr;
+            red[2] += rr;
+            red[3] += b.x * b.x + b.y * b.y;
+        });
+    block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partU);
+}
+
 // Head of every t = A s kernel: alpha_s = rho_s / <r^, v>_s per signal into salpha, the
 // active flags into sact (breakdown when <r^, v> = 0).
 __device__ __forceinline__ void spmv_t_head(const SolveArgs &a, int S, int nparts, int it, double2 *salpha,
@@ -2200,129 +2227,99 @@ __global__ void __launch_bounds__(kThreads) proj_finalize_kernel(const double *p
     if (threadIdx.x == 0) tot[s * nv + k] = v[0];
 }
 
-// Per signal (one block; its totals staged in LDS, then one thread): Jacobi-scaled Cholesky of
-// the Gram matrix with truncation at the first pivot below kProjPivot, solve, and the
-// coefficients on the raw states.  The factor overwrites the packed lower triangle.
+// Per signal (one block of one wave): Jacobi-scaled Cholesky of the Gram matrix in LDS with
+// truncation at the first pivot below kProjPivot (column by column, the rows of a column in
+// parallel), the two triangular solves, and the coefficients on the raw states.
 template <int MH>
 __global__ void __launch_bounds__(64) proj_solve_kernel(ProjArgs p) {
     constexpr int NG = MH * (MH + 1) / 2;
     constexpr int NV = proj_nv(MH);
     __shared__ double sm[NV];
+    __shared__ double2 L[MH][MH];  // lower triangle: the scaled matrix, then its factor
+    __shared__ double2 z[MH];
+    __shared__ double D[MH];
+    __shared__ int sm_m, sm_ok;
     const int s = blockIdx.x;
-    for (int i = threadIdx.x; i < NV; i += blockDim.x) sm[i] = p.tot[(size_t)s * NV + i];
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    const double *t = sm;
+    const int t = threadIdx.x;
     const int nh = p.nh;
-    // lower triangle, packed row-major: (i, j), j <= i  ->  i (i + 1) / 2 + j
-    // the totals hold the upper triangle (j, l), l >= j, row-major: G(i, j) = conj(G(j, i))
-    double2 A[NG], z[MH];
-    double D[MH];
-#pragma unroll
-    for (int i = 0; i < MH; ++i)
-#pragma unroll
-        for (int j = 0; j <= i; ++j) {
-            const int k = j * MH - j * (j - 1) / 2 + (i - j);  // packed upper index of (j, i)
-            A[i * (i + 1) / 2 + j] = make_double2(t[2 * k], -t[2 * k + 1]);
-        }
-#pragma unroll
-    for (int j = 0; j < MH; ++j) {
-        const double gjj = A[j * (j + 1) / 2 + j].x;
-        D[j] = (j < nh && gjj > 0.0) ? sqrt(gjj) : 0.0;
-        z[j] = make_double2(t[2 * NG + 2 * j], t[2 * NG + 2 * j + 1]);
+    for (int i = t; i < NV; i += blockDim.x) sm[i] = p.tot[(size_t)s * NV + i];
+    __syncthreads();
+    if (t < MH) {
+        const int k = t * MH - t * (t - 1) / 2;  // packed index of (t, t)
+        const double g = sm[2 * k];
+        D[t] = (t < nh && g > 0.0) ? sqrt(g) : 0.0;
+        z[t] = make_double2(sm[2 * NG + 2 * t], sm[2 * NG + 2 * t + 1]);
     }
-    // scaled matrix: A(i, j) / (D_i D_j), unit diagonal
-#pragma unroll
-    for (int i = 0; i < MH; ++i)
-#pragma unroll
-        for (int j = 0; j <= i; ++j) {
+    if (t == 0) {
+        sm_m = 0;
+        sm_ok = 1;
+    }
+    __syncthreads();
+    // L(i, j) = conj(G(j, i)) / (D_i D_j), j <= i (G holds the upper triangle, row-major)
+    for (int q = t; q < MH * MH; q += blockDim.x) {
+        const int i = q / MH, j = q % MH;
+        if (j <= i) {
+            const int k = j * MH - j * (j - 1) / 2 + (i - j);
             const double sc = (D[i] > 0.0 && D[j] > 0.0) ? 1.0 / (D[i] * D[j]) : 0.0;
-            double2 &v = A[i * (i + 1) / 2 + j];
-            v.x *= sc;
-            v.y *= sc;
+            L[i][j] = make_double2(sm[2 * k] * sc, -sm[2 * k + 1] * sc);
         }
-    int m = 0;
-    bool ok = true;
-#pragma unroll
+    }
+    __syncthreads();
     for (int j = 0; j < MH; ++j) {
-        if (ok && j < nh && D[j] > 0.0) {
-            double d = A[j * (j + 1) / 2 + j].x;
-#pragma unroll
-            for (int k = 0; k < j; ++k) {
-                const double2 l = A[j * (j + 1) / 2 + k];
-                d -= l.x * l.x + l.y * l.y;
-            }
-            if (d > kProjPivot) {
-                const double ljj = sqrt(d);
-                A[j * (j + 1) / 2 + j] = make_double2(ljj, 0.0);
-#pragma unroll
-                for (int i = j + 1; i < MH; ++i) {
-                    if (i < nh) {
-                        double2 v = A[i * (i + 1) / 2 + j];
-#pragma unroll
-                        for (int k = 0; k < j; ++k) {
-                            const double2 ljk = A[j * (j + 1) / 2 + k];
-                            v = c_sub(v, c_mul(A[i * (i + 1) / 2 + k], make_double2(ljk.x, -ljk.y)));
-                        }
-                        A[i * (i + 1) / 2 + j] = make_double2(v.x / ljj, v.y / ljj);
-                    }
+        if (t == 0) {
+            if (sm_ok && j < nh && D[j] > 0.0) {
+                double d = L[j][j].x;
+                for (int k = 0; k < j; ++k) d -= L[j][k].x * L[j][k].x + L[j][k].y * L[j][k].y;
+                if (d > kProjPivot) {
+                    L[j][j] = make_double2(sqrt(d), 0.0);
+                    sm_m = j + 1;
+                } else {
+                    sm_ok = 0;
                 }
-                m = j + 1;
             } else {
-                ok = false;
+                sm_ok = 0;
             }
-        } else {
-            ok = false;
         }
+        __syncthreads();
+        if (!sm_ok) break;
+        const int i = j + 1 + t;
+        if (i < MH && i < nh) {
+            double2 v = L[i][j];
+            for (int k = 0; k < j; ++k) v = c_sub(v, c_mul(L[i][k], make_double2(L[j][k].x, -L[j][k].y)));
+            L[i][j] = make_double2(v.x / L[j][j].x, v.y / L[j][j].x);
+        }
+        __syncthreads();
     }
-    // forward: L y = W^H b / D ; back: L^H x = y ; coefficient of Delta^j: x_j / D_j
-#pragma unroll
-    for (int j = 0; j < MH; ++j) {
-        if (j < m) {
+    const int m = sm_m;
+    if (t == 0) {
+        // forward: L y = W^H b / D ; back: L^H x = y (in place in z)
+        for (int j = 0; j < m; ++j) {
             double2 v = make_double2(z[j].x / D[j], z[j].y / D[j]);
-#pragma unroll
-            for (int k = 0; k < j; ++k) v = c_sub(v, c_mul(A[j * (j + 1) / 2 + k], z[k]));
-            const double ljj = A[j * (j + 1) / 2 + j].x;
-            z[j] = make_double2(v.x / ljj, v.y / ljj);
+            for (int k = 0; k < j; ++k) v = c_sub(v, c_mul(L[j][k], z[k]));
+            z[j] = make_double2(v.x / L[j][j].x, v.y / L[j][j].x);
         }
-    }
-#pragma unroll
-    for (int j = MH - 1; j >= 0; --j) {
-        if (j < m) {
+        for (int j = m - 1; j >= 0; --j) {
             double2 v = z[j];
-#pragma unroll
-            for (int k = j + 1; k < MH; ++k)
-                if (k < m) {
-                    const double2 lkj = A[k * (k + 1) / 2 + j];
-                    v = c_sub(v, c_mul(make_double2(lkj.x, -lkj.y), z[k]));
-                }
-            const double ljj = A[j * (j + 1) / 2 + j].x;
-            z[j] = make_double2(v.x / ljj, v.y / ljj);  // back substitution in place (k > j done)
+            for (int k = j + 1; k < m; ++k) v = c_sub(v, c_mul(make_double2(L[k][j].x, -L[k][j].y), z[k]));
+            z[j] = make_double2(v.x / L[j][j].x, v.y / L[j][j].x);
         }
     }
-    // y0 = sum_j c_j Delta^j = sum_i gamma_i y_i, gamma_i = sum_{j >= i} c_j (-1)^i C(j, i)
-    double2 gam[MH];
-#pragma unroll
-    for (int i = 0; i < MH; ++i) gam[i] = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int j = 0; j < MH; ++j) {
-        if (j < m) {
-            const double2 cj = make_double2(z[j].x / D[j], z[j].y / D[j]);
-            double binom = 1.0;  // C(j, i)
-#pragma unroll
-            for (int i = 0; i <= j; ++i) {
-                const double f = (i & 1) ? -binom : binom;
-                gam[i].x += f * cj.x;
-                gam[i].y += f * cj.y;
-                binom = binom * (double)(j - i) / (double)(i + 1);
+    __syncthreads();
+    // y0 = sum_j c_j Delta^j = sum_i gamma_i y_i, gamma_i = sum_{j >= i} c_j (-1)^i C(j, i), c_j = x_j / D_j
+    if (t < kMaxHist) {
+        double2 g = make_double2(0.0, 0.0);
+        if (t < MH) {
+            for (int j = t; j < m; ++j) {
+                double binom = 1.0;  // C(j, t)
+                for (int r = 0; r < t; ++r) binom = binom * (double)(j - r) / (double)(r + 1);
+                const double f = (t & 1) ? -binom : binom;
+                g.x += f * z[j].x / D[j];
+                g.y += f * z[j].y / D[j];
             }
+            if (m == 0 && t == 0) g = make_double2(1.0, 0.0);  // no usable direction: keep the newest state
         }
-    }
-    if (m == 0) gam[0] = make_double2(1.0, 0.0);  // no usable direction: keep the newest state
-#pragma unroll
-    for (int i = 0; i < MH; ++i) {
-        p.gam[(size_t)s * 2 * kMaxHist + 2 * i] = gam[i].x;
-        p.gam[(size_t)s * 2 * kMaxHist + 2 * i + 1] = gam[i].y;
+        p.gam[(size_t)s * 2 * kMaxHist + 2 * t] = g.x;
+        p.gam[(size_t)s * 2 * kMaxHist + 2 * t + 1] = g.y;
     }
 }
 
@@ -2892,8 +2889,16 @@ void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *
     } while (0)
 
 void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const int *mask, hipStream_t s) {
-    DMRI_DISPATCH_G(g.G_rhs, a.per_gdir,
-                    (rhs_residual_kernel<GG, PP><<<g.grid, g.rhs_threads, 0, s>>>(a, g.S, g.rhs_rpb, mode, mask)));
+    if (mode == 1 && mask == nullptr && (g.mapping == kMapStaged || g.mapping == kMapSL)) {
+        // the plain residual takes the SpMV kernels' (staged) rows; same grid, same partials
+        const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
+        const size_t lds = TR > 0 ? stage_lds_bytes(TR, a.per_gdir) : 0;
+        DMRI_DISPATCH_G(g.G, a.per_gdir,
+                        (residual_kernel<GG, PP><<<g.grid, g.spmv_threads, lds, s>>>(a, g.S, g.rpb, TR)));
+    } else {
+        DMRI_DISPATCH_G(g.G_rhs, a.per_gdir,
+                        (rhs_residual_kernel<GG, PP><<<g.grid, g.rhs_threads, 0, s>>>(a, g.S, g.rhs_rpb, mode, mask)));
+    }
     check_launch();
     if (a.totU && mode < 2) {
         finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partU, g.grid, g.S, a.totU);
@@ -3011,6 +3016,17 @@ void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStrea
         check_launch();
     }
 }
+// Zero the Krylov states of a step and the all-done flag (one dispatch instead of two fills)
+__global__ void __launch_bounds__(256) reset_state_kernel(int *st, int nwords, int *flag) {
+    for (int i = threadIdx.x; i < nwords; i += blockDim.x) st[i] = 0;
+    if (threadIdx.x == 0) flag[0] = 0;
+}
+void launch_reset_state(KrylovState *st, int S, int *flag, hipStream_t s) {
+    const int nwords = (int)(S * sizeof(KrylovState) / sizeof(int));
+    reset_state_kernel<<<1, 256, 0, s>>>(reinterpret_cast<int *>(st), nwords, flag);
+    check_launch();
+}
+
 // Krylov states and the all-done flag into pinned host memory: one dispatch at every convergence
 // check instead of two device-to-host copies
 __global__ void __launch_bounds__(256) publish_state_kernel(const int *st, int nwords, const int *flag, int *hst,
