@@ -1368,6 +1368,149 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
     }
 }
 
+// Accumulation of entries e0 + g, e0 + g + G, ... < e1 of a row: the implicit values (acc) and the
+// explicit real part sb = sum bre_e x_col(e); the pointers are the staged LDS copies or the
+// global arrays.
+template <int G, bool PG, class Gather>
+__device__ __forceinline__ void accumulate_with_bre(const int *col, const double *vals, const double *bre, int e0,
+                                                    int e1, int g, Gather X, Acc<PG> &acc, double2 &sb) {
+    acc.a = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < (PG ? 3 : 1); ++d) acc.j[d] = make_double2(0.0, 0.0);
+    sb = make_double2(0.0, 0.0);
+    int e = e0 + g;
+    for (; e + G < e1; e += 2 * G) {
+        const int c0 = col[e], c1 = col[e + G];
+        const double2 x0 = X(c0), x1 = X(c1);
+        acc_entry<PG>(vals, e, x0, acc);
+        acc_entry<PG>(vals, e + G, x1, acc);
+        const double b0 = bre[e], b1 = bre[e + G];
+        sb.x += b0 * x0.x + b1 * x1.x;
+        sb.y += b0 * x0.y + b1 * x1.y;
+    }
+    if (e < e1) {
+        const double2 x0 = X(col[e]);
+        acc_entry<PG>(vals, e, x0, acc);
+        sb.x += bre[e] * x0.x;
+        sb.y += bre[e] * x0.y;
+    }
+}
+
+// staged_spmv with the explicit real part: a tile's column indices, values and bre are staged in
+// LDS (stage_lds_bytes_bre); emit(row, acc, sb) receives the group-reduced sums.
+template <int G, bool PG, class Gather, class Emit>
+__device__ __forceinline__ void staged_spmv_bre(const SolveArgs &a, int rpb, int TR, const Lane &L, Gather X,
+                                                Emit emit) {
+    constexpr int VW = PG ? 4 : 2;
+    if (TR <= 0) {
+        const TileWalk W = xcd_tiles(a.n, rpb);
+        for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
+            const int64_t row = t0 + L.lrow;
+            if (row >= W.end) continue;
+            Acc<PG> acc;
+            double2 sb;
+            accumulate_with_bre<G, PG>(a.col, a.vals, a.bre, a.rowptr[row], a.rowptr[row + 1], L.g, X, acc, sb);
+            group_reduce_acc<G, PG>(acc);
+            group_reduce<G>(sb);
+            emit(row, acc, sb);
+        }
+        return;
+    }
+    const int ECAP = TR * kStageNnzPerRow;
+    extern __shared__ double4 dmri_stage_lds[];
+    double *svals = reinterpret_cast<double *>(dmri_stage_lds);
+    double *sbre = svals + (size_t)ECAP * VW;
+    int *scol = reinterpret_cast<int *>(sbre + ECAP);
+    int *srp = scol + ECAP;
+    const TileWalk W = xcd_tiles(a.n, TR);
+    for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
+        const int nr = (int)((W.end - t0) < TR ? (W.end - t0) : TR);
+        const int E0 = a.rowptr[t0];
+        const int ne = a.rowptr[t0 + nr] - E0;
+        const bool fits = ne <= ECAP;
+        __syncthreads();
+        if (fits) {
+            const double2 *gv = reinterpret_cast<const double2 *>(a.vals + (int64_t)E0 * VW);
+            double2 *lv = reinterpret_cast<double2 *>(svals);
+            const int nt = blockDim.x;
+            const int rp = threadIdx.x <= nr ? a.rowptr[t0 + threadIdx.x] : 0;
+            for (int base = 0; base < ne; base += kStageBatch * nt) {
+                int cv[kStageBatch];
+                double bv[kStageBatch];
+                double2 vv[kStageBatch][VW / 2];
+#pragma unroll
+                for (int u = 0; u < kStageBatch; ++u) {
+                    const int i = base + u * nt + threadIdx.x;
+                    const bool ok = i < ne;
+                    cv[u] = ok ? a.col[E0 + i] : 0;
+                    bv[u] = ok ? a.bre[E0 + i] : 0.0;
+#pragma unroll
+                    for (int h = 0; h < VW / 2; ++h)
+                        vv[u][h] = ok ? gv[(int64_t)i * (VW / 2) + h] : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int u = 0; u < kStageBatch; ++u) {
+                    const int i = base + u * nt + threadIdx.x;
+                    if (i < ne) {
+                        scol[i] = cv[u];
+                        sbre[i] = bv[u];
+#pragma unroll
+                        for (int h = 0; h < VW / 2; ++h) lv[i * (VW / 2) + h] = vv[u][h];
+                    }
+                }
+            }
+            if ((int)threadIdx.x <= nr) srp[threadIdx.x] = rp - E0;
+            for (int i = threadIdx.x + nt; i <= nr; i += nt) srp[i] = a.rowptr[t0 + i] - E0;
+        }
+        __syncthreads();
+        for (int lr = L.lrow; lr < nr; lr += rpb) {
+            Acc<PG> acc;
+            double2 sb;
+            if (fits)
+                accumulate_with_bre<G, PG>(scol, svals, sbre, srp[lr], srp[lr + 1], L.g, X, acc, sb);
+            else
+                accumulate_with_bre<G, PG>(a.col, a.vals, a.bre, a.rowptr[t0 + lr], a.rowptr[t0 + lr + 1], L.g, X,
+                                           acc, sb);
+            group_reduce_acc<G, PG>(acc);
+            group_reduce<G>(sb);
+            emit(t0 + lr, acc, sb);
+        }
+    }
+}
+
+// Mode 4 of the residual kernel in the SpMV kernels' (staged) row mapping: b = (Bre + i cB.J) x
+// (+ the periodic right-hand side), hp = Are x, hq = Jg x.
+template <int G, bool PG>
+__global__ void __launch_bounds__(kThreads) rhs_history_kernel(SolveArgs a, int S, int rpb, int TR) {
+    const Lane L = spmv_lane<G>(S);
+    const int S_ = S;
+    const int s = L.s;
+    const double2 *x = a.x;
+    staged_spmv_bre<G, PG>(
+        a, rpb, TR, L, [&](int c) { return x[(int64_t)c * S_ + s]; },
+        [&](int64_t row, const Acc<PG> &acc, double2 sb) {
+            if (L.g != 0) return;
+            const int64_t idx = row * S_ + s;
+            Acc<PG> bacc = acc;
+            bacc.a = sb;
+            double2 b = imag_coef_apply<G, PG>(bacc, a.coefB, s);
+            if (a.bper) {
+                const double2 bp = a.bper[idx];
+                b.x += bp.x;
+                b.y += bp.y;
+            }
+            a.b[idx] = b;
+            a.hp[idx] = acc.a;
+            double2 q = acc.j[0];
+            if (PG) {
+                const double g0 = a.gsig[3 * s], g1 = a.gsig[3 * s + 1], g2 = a.gsig[3 * s + 2];
+                q.x = g0 * acc.j[0].x + g1 * acc.j[1].x + g2 * acc.j[2].x;
+                q.y = g0 * acc.j[0].y + g1 * acc.j[1].y + g2 * acc.j[2].y;
+            }
+            a.hq[idx] = q;
+        });
+}
+
 // mode 0: b = (Bre + i cB.J) x, r = b - (Are + i cA.J) x, r^ = r          (start of a step)
 // mode 1: r = b - (Are + i cA.J) x with the stored b, for the masked signals (restart, or
 //         after an extrapolated initial guess; mask == nullptr selects every signal)
@@ -2642,6 +2785,9 @@ size_t stage_lds_bytes(int TR, bool per_gdir) {
     const size_t ecap = (size_t)TR * kStageNnzPerRow;
     return ecap * (8 * (per_gdir ? 4 : 2) + 4) + 4 * (size_t)(TR + 1) + 32;
 }
+static size_t stage_lds_bytes_bre(int TR, bool per_gdir) {
+    return stage_lds_bytes(TR, per_gdir) + (size_t)TR * kStageNnzPerRow * 8;
+}
 
 void launch_sum(const double *x, int64_t n, double *part, double *out, hipStream_t s) {
     unsigned g = grid_for(n, kThreads);
@@ -2889,7 +3035,12 @@ void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *
     } while (0)
 
 void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const int *mask, hipStream_t s) {
-    if (mode == 1 && mask == nullptr && (g.mapping == kMapStaged || g.mapping == kMapSL)) {
+    if (mode == 4 && (g.mapping == kMapStaged || g.mapping == kMapSL)) {
+        const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
+        const size_t lds = TR > 0 ? stage_lds_bytes_bre(TR, a.per_gdir) : 0;
+        DMRI_DISPATCH_G(g.G, a.per_gdir,
+                        (rhs_history_kernel<GG, PP><<<g.grid, g.spmv_threads, lds, s>>>(a, g.S, g.rpb, TR)));
+    } else if (mode == 1 && mask == nullptr && (g.mapping == kMapStaged || g.mapping == kMapSL)) {
         // the plain residual takes the SpMV kernels' (staged) rows; same grid, same partials
         const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
         const size_t lds = TR > 0 ? stage_lds_bytes(TR, a.per_gdir) : 0;
