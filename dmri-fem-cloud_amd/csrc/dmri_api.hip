@@ -669,6 +669,7 @@ struct Stepper {
             while (lanes > 1 && lanes * S > 64) lanes >>= 1;
         }
         geo = make_geom(n, lanes, S, mapping);
+        fit_spmv_grid(geo, per_gdir);
         tiles = tiles_for(c, geo);
         ensure_group_workspace(c, geo, N, per_gdir);
         SolverWork &w = c->work;
@@ -1385,7 +1386,8 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
             double &slot = us[(m - 1) * 6 + i];
             slot = -1.0;
             if (m == kMapWave ? G != 1 : (m == kMapSR ? S > kMaxSRSignals : G * S > 64)) continue;
-            const SolveGeom geo = make_geom(n, G, S, m);
+            SolveGeom geo = make_geom(n, G, S, m);
+            fit_spmv_grid(geo, per_gdir);
             const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
             if (w.part.n < 3 * np) w.part.alloc(3 * np);
             double uv = 0, ut = 0;
@@ -1737,7 +1739,8 @@ int dmri_time_spmv(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t reps, do
                 mapping = t.second / 64;
                 lanes = t.second % 64;
             }
-        const SolveGeom geo = make_geom(c->ndof, lanes, nsig, mapping);
+        SolveGeom geo = make_geom(c->ndof, lanes, nsig, mapping);
+        fit_spmv_grid(geo, pg);
         std::vector<KrylovState> st0(nsig);
         std::memset(st0.data(), 0, sizeof(KrylovState) * nsig);
         for (auto &x : st0) x.rho[0] = make_double2(1.0, 0.0);

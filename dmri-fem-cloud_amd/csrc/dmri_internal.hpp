@@ -72,7 +72,8 @@ struct KrylovState {
 
 constexpr int kThreads = 256;   // target threads per block of every solver kernel
 constexpr int kMaxGrid = 512;      // blocks of the vector kernels (grid-stride)
-constexpr int kMaxGridSpmv = 2048; // blocks of the SpMV kernels: 8 blocks of 4 waves on each of 256 CUs
+// upper bound of the SpMV grid; fit_spmv_grid lowers it to the blocks that stay resident at once
+constexpr int kMaxGridSpmv = 2048;
 constexpr int kMaxSignals = 64; // signals per batched launch
 constexpr int kNV = 4;          // doubles per (block, signal) partial slot
 constexpr int kStageNnzPerRow = 18; // LDS budget of a staged SpMV tile, entries per row
@@ -109,6 +110,10 @@ struct SolveGeom {
     int pgrid = 1;           // blocks of the projection Gram kernel
 };
 SolveGeom make_geom(int64_t n, int G, int S, int mapping = kMapSL);
+// Lower g.grid to the number of SpMV blocks the device keeps resident at once (occupancy of the
+// mapping's preconditioner t-kernel, the most constrained one, times the CU count, a multiple of
+// 8), so every block starts in the first round and the launch has no partial last round.
+void fit_spmv_grid(SolveGeom &g, bool per_gdir);
 int stage_rows(const SolveGeom &g, bool per_gdir);
 size_t stage_lds_bytes(int TR, bool per_gdir);
 

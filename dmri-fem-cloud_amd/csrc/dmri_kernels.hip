@@ -3034,6 +3034,27 @@ void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *
         }                                                                                   \
     } while (0)
 
+void fit_spmv_grid(SolveGeom &g, bool per_gdir) {
+    const void *fn = nullptr;
+    size_t lds = 0;
+    if (g.mapping == kMapStaged || g.mapping == kMapSL) {
+        const int TR = g.mapping == kMapStaged ? stage_rows(g, per_gdir) : 0;
+        lds = TR > 0 ? stage_lds_bytes(TR, per_gdir) : 0;
+        DMRI_DISPATCH_G(g.G, per_gdir, fn = reinterpret_cast<const void *>(&spmv_t_kernel<GG, PP, kOpPre>));
+    } else if (g.mapping == kMapWave) {
+        fn = per_gdir ? reinterpret_cast<const void *>(&spmv_t_wave_kernel<true, kOpPre>)
+                      : reinterpret_cast<const void *>(&spmv_t_wave_kernel<false, kOpPre>);
+    }
+    if (!fn) return;
+    int dev = 0, cus = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, g.spmv_threads, lds) != hipSuccess)
+        throw Error(-2, "occupancy query of the SpMV kernels failed");
+    const int64_t cap = (int64_t)nb * cus / 8 * 8;
+    if (cap >= 8 && cap < g.grid) g.grid = (int)cap;
+}
+
 void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const int *mask, hipStream_t s) {
     if (mode == 4 && (g.mapping == kMapStaged || g.mapping == kMapSL)) {
         const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
