@@ -3219,9 +3219,27 @@ void launch_proj_guess(const SolveGeom &g, const ProjArgs &p, int mh, hipStream_
     if (p.nh < 1 || p.nh > mh || !(mh == 4 || mh == 6 || mh == 8 || mh == 10 || mh == 12))
         throw Error(-1, "projection guess: history size must satisfy 1 <= nh <= mh, mh in {4, 6, 8, 10, 12}");
     if (p.S < 1 || p.S > kMaxSignals) throw Error(-1, "projection guess: signals per launch must lie in [1, 64]");
-    const int grid = proj_grid(g);
     const int nv = proj_nv(mh);
     const size_t lds = (size_t)(mh + 1) * kThreads * sizeof(double2);
+    // Gram blocks: at most what stays resident at once (its LDS tile bounds it), so no block
+    // waits for a second round
+    int grid = proj_grid(g);
+    {
+        const void *fn = mh == 4    ? reinterpret_cast<const void *>(&proj_gram_kernel<4>)
+                         : mh == 6  ? reinterpret_cast<const void *>(&proj_gram_kernel<6>)
+                         : mh == 8  ? reinterpret_cast<const void *>(&proj_gram_kernel<8>)
+                         : mh == 10 ? reinterpret_cast<const void *>(&proj_gram_kernel<10>)
+                                    : reinterpret_cast<const void *>(&proj_gram_kernel<12>);
+        int dev = 0, cus = 0, nb = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kThreads, lds) != hipSuccess)
+            throw Error(-2, "occupancy query of the projection Gram kernel failed");
+        const int nwin = (p.S + std::max(1, std::min(p.S, 2 * kThreads / (nv / 2))) - 1) /
+                         std::max(1, std::min(p.S, 2 * kThreads / (nv / 2)));
+        const int cap = nb * cus / nwin;
+        if (cap >= 1 && cap < grid) grid = cap;
+    }
     // signal window of the Gram kernel: at most 2 (pair, signal) combinations per thread
     const int Sw = std::max(1, std::min(p.S, 2 * kThreads / (nv / 2)));
     const dim3 ggrid(grid, (p.S + Sw - 1) / Sw);
