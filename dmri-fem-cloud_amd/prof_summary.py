@@ -6,7 +6,7 @@
 
 trace: per kernel the number of dispatches, mean and median duration, and the mean idle time
 between the end of the previous dispatch and its start.  For the SpMV kernels it also prints the
-mean over the first K dispatches after every rhs_residual_kernel (the first K Krylov iterations
+mean over the first K dispatches after every residual kernel (the first K Krylov iterations
 of each theta step, where every signal of the launch group is still active): the same launches
 that the solver's sampled kernel events time (dmri_solver_opts.profile = K).
 
@@ -26,6 +26,10 @@ SPMV = ("spmv_v_kernel", "spmv_t_kernel", "spmv_v_sr_kernel", "spmv_t_sr_kernel"
         "spmv_t_wave_kernel", "spmv_v_tile_kernel", "spmv_t_tile_kernel")
 
 
+# kernels that form a theta step's residual just before its Krylov iterations
+STEP_START = ("rhs_residual_kernel", "rhs_history_kernel", "residual_kernel")
+
+
 def short(name: str) -> str:
     n = name.replace("void ", "").replace("dmri::(anonymous namespace)::", "").replace("dmri::", "")
     return n.split("(")[0]
@@ -36,12 +40,12 @@ def base(name: str) -> str:
 
 
 def sampled(rows, k):
-    """Indices of the first k SpMV dispatches (per SpMV kernel) after each rhs_residual_kernel."""
+    """Indices of the first k SpMV dispatches (per SpMV kernel) after each step's residual kernel."""
     out = set()
     seen = collections.Counter()
     for i, r in enumerate(rows):
         b = base(r["Kernel_Name"])
-        if b == "rhs_residual_kernel":
+        if b in STEP_START:
             seen.clear()
         elif b in SPMV:
             key = short(r["Kernel_Name"])  # each template instance (operation) counts on its own
