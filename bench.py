@@ -165,7 +165,7 @@ def workload(name, args):
                  k=k, qs=qs, gdirs=gd, cpu_signal=3,
                  desc="periodic box 30x30x10 um, 20 seeded cylinders r~U[2,4] um (two compartments), weak "
                  "pseudo-periodic faces x and y, kappa=1e-5, D=3e-3, PGSE 10600/43100 us, dt=200 us, 64 "
-                 "Fibonacci-sphere directions x 8 b-values linspace(0,4000) = 512 signals in launch groups of 64; "
+                 "Fibonacci-sphere directions x 8 b-values linspace(0,4000) = 512 signals in launch groups of --group; "
                  "step = device assembly + all signals", extra={"b_values": bvals.tolist(), "n_dirs": 64})
     elif name == "c4":
         f, T, int4gb = cos_ogse(4, 19900.0, 19900.0)
@@ -180,7 +180,7 @@ def workload(name, args):
                  qs=qs, gdirs=gd, cpu_signal=1,
                  desc="neuron-like surrogate (soma r=8 um + 6 dendrite trees, 3 bifurcation levels, voxelised), "
                  "cos-OGSE 4 periods delta=Delta=19900 us, T=40 ms, dt=100 us, D=3e-3, 32 Fibonacci-sphere "
-                 "directions x b in {1000,2000,3000,4000} = 128 signals in launch groups of 64; step = device "
+                 "directions x b in {1000,2000,3000,4000} = 128 signals in launch groups of --group; step = device "
                  "assembly + all signals", extra={"b_values": bvals, "n_dirs": 32})
     else:
         raise SystemExit(f"unknown workload {name}")
@@ -240,7 +240,7 @@ def run_forward(args, name):
     qs, gd = w["qs"], w["gdirs"]
     nsig = len(qs)
     per_gdir = bool(np.any(np.abs(gd - gd[0]) > 0))
-    S_group = min(64, nsig)
+    S_group = min(args.group, nsig)
     if args.profile_steps > 0:
         fA, fB = fA[:args.profile_steps], fB[:args.profile_steps]
         FB = None if FB is None else FB[:args.profile_steps]
@@ -251,9 +251,28 @@ def run_forward(args, name):
         ctx.set_periodic(w["periodic"])
 
     def one_step(profile=0):
+        """Device assembly + every signal.  Several launch groups are solved one call each, with a
+        progress line on stderr after each (a full C2 job runs for minutes)."""
         ctx.assemble()
-        return ctx.solve_batched(fA, fB, k, qs, gd, rtol=args.rtol, profile=profile, lanes=args.lanes,
-                                 guess=args.guess, mapping=args.mapping, F_explicit=FB)
+        if nsig <= S_group:
+            return ctx.solve_batched(fA, fB, k, qs, gd, rtol=args.rtol, profile=profile, lanes=args.lanes,
+                                     guess=args.guess, mapping=args.mapping, F_explicit=FB)
+        sigs, stats = [], []
+        t0 = time.perf_counter()
+        for g0 in range(0, nsig, S_group):
+            sl = slice(g0, min(nsig, g0 + S_group))
+            s_, st_ = ctx.solve_batched(fA, fB, k, qs[sl], gd[sl], rtol=args.rtol, profile=profile, lanes=args.lanes,
+                                        guess=args.guess, mapping=args.mapping, F_explicit=FB)
+            sigs.append(s_)
+            stats.append(st_)
+            print(f"  signals {sl.stop}/{nsig} done, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        merged = dict(stats[0])
+        for key in merged:
+            if key in ("iters_max", "relres_max"):
+                merged[key] = max(s_[key] for s_ in stats)
+            elif key != "spmv_bytes":
+                merged[key] = sum(s_[key] for s_ in stats)
+        return np.concatenate(sigs), merged
 
     copy_gbs = ctx.copy_bandwidth(args.copy_bytes, 10) if args.copy_bytes > 0 else None
     tuned = None
@@ -417,6 +436,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60, help="theta steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rtol", type=float, default=1e-12)
+    ap.add_argument("--group", type=int, default=32,
+                    help="signals per launch group (1..64); 32 was the fastest for the 512- and 128-signal "
+                         "workloads (64: 2-4 %% slower, 16: 5-8 %% slower)")
     ap.add_argument("--lanes", type=int, default=0, help="SpMV lanes per row; 0 = tuned")
     ap.add_argument("--mapping", type=int, default=0, help="SpMV mapping 1 spread, 2 staged, 3 registers, 4 wave per row, "
                     "5 tile-compressed; 0 = tuned")
