@@ -14,7 +14,7 @@ Other workloads (each prints the same kind of line; --theta-steps cuts the seque
 first K steps for a shorter measurement, stated in the line):
   c2  periodic box 30x30x10 um with 20 seeded cylinders (r uniform 2-4 um), ~500k vertices,
       weak pseudo-periodic faces in x and y, kappa=1e-5, 64 Fibonacci-sphere directions x
-      8 b-values (linspace 0..4000) = 512 signals in launch groups of 64.
+      8 b-values (linspace 0..4000) = 512 signals in launch groups of 32 (--group).
   c4  neuron-like surrogate (soma r=8 um + 6 seeded dendrite trees, 3 bifurcation levels),
       ~2M vertices, cos-OGSE 4 periods, T=40 ms, dt=100 us (401 steps), 32 directions x
       4 b-values = 128 signals.
@@ -165,7 +165,7 @@ def workload(name, args):
                  k=k, qs=qs, gdirs=gd, cpu_signal=3,
                  desc="periodic box 30x30x10 um, 20 seeded cylinders r~U[2,4] um (two compartments), weak "
                  "pseudo-periodic faces x and y, kappa=1e-5, D=3e-3, PGSE 10600/43100 us, dt=200 us, 64 "
-                 "Fibonacci-sphere directions x 8 b-values linspace(0,4000) = 512 signals in launch groups of --group; "
+                 f"Fibonacci-sphere directions x 8 b-values linspace(0,4000) = 512 signals in launch groups of {args.group}; "
                  "step = device assembly + all signals", extra={"b_values": bvals.tolist(), "n_dirs": 64})
     elif name == "c4":
         f, T, int4gb = cos_ogse(4, 19900.0, 19900.0)
@@ -180,7 +180,7 @@ def workload(name, args):
                  qs=qs, gdirs=gd, cpu_signal=1,
                  desc="neuron-like surrogate (soma r=8 um + 6 dendrite trees, 3 bifurcation levels, voxelised), "
                  "cos-OGSE 4 periods delta=Delta=19900 us, T=40 ms, dt=100 us, D=3e-3, 32 Fibonacci-sphere "
-                 "directions x b in {1000,2000,3000,4000} = 128 signals in launch groups of --group; step = device "
+                 f"directions x b in {{1000,2000,3000,4000}} = 128 signals in launch groups of {args.group}; step = device "
                  "assembly + all signals", extra={"b_values": bvals, "n_dirs": 32})
     else:
         raise SystemExit(f"unknown workload {name}")
