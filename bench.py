@@ -256,13 +256,13 @@ def run_forward(args, name):
         ctx.assemble()
         if nsig <= S_group:
             return ctx.solve_batched(fA, fB, k, qs, gd, rtol=args.rtol, profile=profile, lanes=args.lanes,
-                                     guess=args.guess, mapping=args.mapping, F_explicit=FB)
+                                     guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond)
         sigs, stats = [], []
         t0 = time.perf_counter()
         for g0 in range(0, nsig, S_group):
             sl = slice(g0, min(nsig, g0 + S_group))
             s_, st_ = ctx.solve_batched(fA, fB, k, qs[sl], gd[sl], rtol=args.rtol, profile=profile, lanes=args.lanes,
-                                        guess=args.guess, mapping=args.mapping, F_explicit=FB)
+                                        guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond)
             sigs.append(s_)
             stats.append(st_)
             print(f"  signals {sl.stop}/{nsig} done, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -436,6 +436,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60, help="theta steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rtol", type=float, default=1e-12)
+    ap.add_argument("--precond", type=int, default=0,
+                    help="0 = Jacobi + degree-1 polynomial (library default), 1 = Jacobi only")
     ap.add_argument("--group", type=int, default=32,
                     help="signals per launch group (1..64); 32 was the fastest for the 512- and 128-signal "
                          "workloads (64: 2-4 %% slower, 16: 5-8 %% slower)")

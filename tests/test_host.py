@@ -135,3 +135,21 @@ def test_bench_sequence_matches_oracle():
     a = bench.pgse_sequence(10600.0, 43100.0, 200.0)
     b = O.gcloud_sequence(O.pgse_piecewise(10600.0, 43100.0), 53700.0, 200.0)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_profile_summary_samples_first_iterations_of_every_step(tmp_path):
+    """prof_summary's sample: the first k dispatches of every SpMV kernel after each step's residual
+    kernel (the launches bench.py times with events), from a synthetic kernel trace."""
+    import prof_summary as ps
+
+    names = []
+    for _ in range(3):  # three theta steps
+        names += ["rhs_history_kernel<1, false>", "proj_gram_kernel<12>", "residual_kernel<1, false>"]
+        for _ in range(4):  # four Krylov iterations
+            names += ["bicg_p_kernel", "spmv_v_kernel<1, false, 1>", "spmv_v_kernel<1, false, 0>",
+                      "spmv_t_kernel<1, false, 1>", "spmv_t_kernel<1, false, 2>", "bicg_update_kernel"]
+    rows = [{"Kernel_Name": f"void dmri::(anonymous namespace)::{n}(dmri::SolveArgs, int)"} for n in names]
+    picked = [ps.short(rows[i]["Kernel_Name"]) for i in sorted(ps.sampled(rows, 1))]
+    assert len(picked) == 3 * 4
+    assert picked.count("spmv_v_kernel<1, false, 0>") == 3 and picked.count("spmv_t_kernel<1, false, 2>") == 3
+    assert len(ps.sampled(rows, 2)) == 3 * 4 * 2
