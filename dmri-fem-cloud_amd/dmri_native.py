@@ -246,7 +246,9 @@ class Context:
 
     def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
               profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0):
-        """precond: 0 = Jacobi + degree-1 polynomial (default), 1 = Jacobi only."""
+        """precond: 0 = Jacobi + degree-1 polynomial (default), 1 = Jacobi only.  guess: Krylov start
+        of each step, 0 = residual-minimising projection on the last states (default), 1 = previous
+        state, 2 = linear, 3 = quadratic extrapolation."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes,
                                     guess, mapping, F_explicit, precond)
         sig = np.zeros(2)
