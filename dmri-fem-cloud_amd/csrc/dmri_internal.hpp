@@ -89,6 +89,9 @@ constexpr int kMapTile = 5;  // tiles of rpb rows: the tile's distinct columns s
 // SpMV kernel operations: v = A p / t = A s (plain BiCGStab), the degree-1 polynomial
 // preconditioner z -> 2z - A z applied to p or s, and t = A sh (preconditioned BiCGStab)
 constexpr int kOpV = 0, kOpT = 0, kOpPre = 1, kOpTP = 2;
+// the preconditioner polynomial takes the real part alone when the imaginary part's bound is at
+// most this fraction of the real part's (measured: 0.002 -> same iterations, 3.3 -> twice as many)
+constexpr double kRealPreconditioner = 0.05;
 constexpr int kTileMaxEntries = 4096;  // entries of a tile the compression build sorts in LDS
 constexpr int kTileLdsBytes = 32768;   // LDS cap of the staged x values of one tile (distinct columns x S)
 constexpr int kMaxSRSignals = 8;
@@ -254,7 +257,8 @@ struct SolveArgs {
     double gc0, gc1, gc2;
     const double2 *bper;            // weak pseudo-periodic right-hand side of this step, or null
     double2 *ph, *sh;               // preconditioned p and s (polynomial preconditioner), or null
-    double pc;                      // polynomial scale c: z -> c (2 z - c A z)
+    double pc;                      // polynomial scale c: z -> c (2 z - c Are z)
+    const double *areal;            // [nnz] the scaled real part alone (preconditioner SpMVs)
     double2 *hp, *hq;               // mode 4 writes Are x and Jg x here (projection history)
     const double *gsig;             // [S*3] unit gradient directions (per-direction values only)
     double *partU, *partV, *partT;  // [grid][S][kNV] each
@@ -306,6 +310,8 @@ void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
 void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
                    hipEvent_t e1 = nullptr, int op = kOpT);
 void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
+// areal[e] = vals[vw e]
+void launch_extract_real(int64_t nnz, const double *vals, int vw, double *areal, hipStream_t s);
 // zero st[0..S) and *flag
 void launch_reset_state(KrylovState *st, int S, int *flag, hipStream_t s);
 // st[0..S) and *flag into host-mapped pinned memory (device addresses of hipHostMalloc buffers)

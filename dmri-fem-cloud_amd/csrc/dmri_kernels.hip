@@ -1295,10 +1295,33 @@ __device__ __forceinline__ Lane spmv_lane(int S) {
 // reads its entries from LDS and gathers x from global memory.  A tile whose entries exceed the
 // LDS budget (TR * kStageNnzPerRow) is processed straight from global memory instead.
 // emit(row, acc) receives the group-reduced accumulators (acted on by lane g == 0).
-template <int G, bool PG, class Gather, class Emit>
+// Real part only (RO): y = sum a_e x_col(e) from the separate array a.areal, one value per entry;
+// acc.a receives it and acc.j stays zero.
+template <int G, bool PG, class Gather>
+__device__ __forceinline__ void real_row_accumulate(const int *col, const double *av, int e0, int e1, int g, Gather X,
+                                                    Acc<PG> &acc) {
+    acc.a = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < (PG ? 3 : 1); ++d) acc.j[d] = make_double2(0.0, 0.0);
+    int e = e0 + g;
+    for (; e + 3 * G < e1; e += 4 * G) {
+        const int c0 = col[e], c1 = col[e + G], c2 = col[e + 2 * G], c3 = col[e + 3 * G];
+        const double2 x0 = X(c0), x1 = X(c1), x2 = X(c2), x3 = X(c3);
+        const double v0 = av[e], v1 = av[e + G], v2 = av[e + 2 * G], v3 = av[e + 3 * G];
+        acc.a.x += v0 * x0.x + v1 * x1.x + v2 * x2.x + v3 * x3.x;
+        acc.a.y += v0 * x0.y + v1 * x1.y + v2 * x2.y + v3 * x3.y;
+    }
+    for (; e < e1; e += G) {
+        const double2 x0 = X(col[e]);
+        acc.a.x += av[e] * x0.x;
+        acc.a.y += av[e] * x0.y;
+    }
+}
+
+template <int G, bool PG, bool RO = false, class Gather, class Emit>
 __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR, const Lane &L, bool act, Gather X,
                                             Emit emit) {
-    constexpr int VW = PG ? 4 : 2;
+    constexpr int VW = RO ? 1 : (PG ? 4 : 2);
     if (TR <= 0) {  // direct: entries straight from global memory
         const TileWalk W = xcd_tiles(a.n, rpb);
         if (!act) return;
@@ -1306,7 +1329,10 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
             const int64_t row = t0 + L.lrow;
             if (row >= W.end) continue;
             Acc<PG> acc;
-            row_accumulate<G, PG>(a.rowptr, a.col, a.vals, row, L.g, X, acc);
+            if (RO)
+                real_row_accumulate<G, PG>(a.col, a.areal, a.rowptr[row], a.rowptr[row + 1], L.g, X, acc);
+            else
+                row_accumulate<G, PG>(a.rowptr, a.col, a.vals, row, L.g, X, acc);
             group_reduce_acc<G, PG>(acc);
             emit(row, acc);
         }
@@ -1327,29 +1353,52 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
         if (fits) {
             // Copy through registers in batches of kStageBatch independent loads per thread, so
             // a thread waits on memory once per batch rather than once per element.
-            const double2 *gv = reinterpret_cast<const double2 *>(a.vals + (int64_t)E0 * VW);
-            double2 *lv = reinterpret_cast<double2 *>(svals);
             const int nt = blockDim.x;
             const int rp = threadIdx.x <= nr ? a.rowptr[t0 + threadIdx.x] : 0;
-            for (int base = 0; base < ne; base += kStageBatch * nt) {
-                int cv[kStageBatch];
-                double2 vv[kStageBatch][VW / 2];
+            if (RO) {
+                const double *gv = a.areal + E0;
+                for (int base = 0; base < ne; base += kStageBatch * nt) {
+                    int cv[kStageBatch];
+                    double vv[kStageBatch];
 #pragma unroll
-                for (int u = 0; u < kStageBatch; ++u) {
-                    const int i = base + u * nt + threadIdx.x;
-                    const bool ok = i < ne;
-                    cv[u] = ok ? a.col[E0 + i] : 0;
+                    for (int u = 0; u < kStageBatch; ++u) {
+                        const int i = base + u * nt + threadIdx.x;
+                        const bool ok = i < ne;
+                        cv[u] = ok ? a.col[E0 + i] : 0;
+                        vv[u] = ok ? gv[i] : 0.0;
+                    }
 #pragma unroll
-                    for (int h = 0; h < VW / 2; ++h)
-                        vv[u][h] = ok ? gv[(int64_t)i * (VW / 2) + h] : make_double2(0.0, 0.0);
+                    for (int u = 0; u < kStageBatch; ++u) {
+                        const int i = base + u * nt + threadIdx.x;
+                        if (i < ne) {
+                            scol[i] = cv[u];
+                            svals[i] = vv[u];
+                        }
+                    }
                 }
+            } else {
+                const double2 *gv = reinterpret_cast<const double2 *>(a.vals + (int64_t)E0 * VW);
+                double2 *lv = reinterpret_cast<double2 *>(svals);
+                for (int base = 0; base < ne; base += kStageBatch * nt) {
+                    int cv[kStageBatch];
+                    double2 vv[kStageBatch][VW / 2 > 0 ? VW / 2 : 1];
 #pragma unroll
-                for (int u = 0; u < kStageBatch; ++u) {
-                    const int i = base + u * nt + threadIdx.x;
-                    if (i < ne) {
-                        scol[i] = cv[u];
+                    for (int u = 0; u < kStageBatch; ++u) {
+                        const int i = base + u * nt + threadIdx.x;
+                        const bool ok = i < ne;
+                        cv[u] = ok ? a.col[E0 + i] : 0;
 #pragma unroll
-                        for (int h = 0; h < VW / 2; ++h) lv[i * (VW / 2) + h] = vv[u][h];
+                        for (int h = 0; h < VW / 2; ++h)
+                            vv[u][h] = ok ? gv[(int64_t)i * (VW / 2) + h] : make_double2(0.0, 0.0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kStageBatch; ++u) {
+                        const int i = base + u * nt + threadIdx.x;
+                        if (i < ne) {
+                            scol[i] = cv[u];
+#pragma unroll
+                            for (int h = 0; h < VW / 2; ++h) lv[i * (VW / 2) + h] = vv[u][h];
+                        }
                     }
                 }
             }
@@ -1360,8 +1409,13 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
         if (!act) continue;
         for (int lr = L.lrow; lr < nr; lr += rpb) {
             Acc<PG> acc;
-            if (fits) lds_row_accumulate<G, PG>(srp, scol, svals, lr, L.g, X, acc);
-            else row_accumulate<G, PG>(a.rowptr, a.col, a.vals, t0 + lr, L.g, X, acc);
+            if (RO) {
+                if (fits) real_row_accumulate<G, PG>(scol, svals, srp[lr], srp[lr + 1], L.g, X, acc);
+                else real_row_accumulate<G, PG>(a.col, a.areal, a.rowptr[t0 + lr], a.rowptr[t0 + lr + 1], L.g, X, acc);
+            } else {
+                if (fits) lds_row_accumulate<G, PG>(srp, scol, svals, lr, L.g, X, acc);
+                else row_accumulate<G, PG>(a.rowptr, a.col, a.vals, t0 + lr, L.g, X, acc);
+            }
             group_reduce_acc<G, PG>(acc);
             emit(t0 + lr, acc);
         }
@@ -1700,7 +1754,7 @@ __global__ void __launch_bounds__(kThreads) bicg_p_kernel(SolveArgs a, int S, in
 }
 
 // OP kOpV:  v = A p ; partials V: [<r^, v>.re, <r^, v>.im, 0, 0]
-// OP kOpPre: ph = c (2 p - c A p) (a degree-1 polynomial preconditioner of the scaled system,
+// OP kOpPre: ph = c (2 p - c Are p) (a degree-1 polynomial preconditioner of the scaled system,
 //            whose diagonal is 1; c from a bound of its spectrum), no partials
 template <int G, bool PG, int OP>
 __global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
@@ -1711,22 +1765,26 @@ __global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, in
     const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     const double2 *p = a.p;
-    staged_spmv<G, PG>(
-        a, rpb, TR, L, act, [&](int c) { return p[(int64_t)c * S_ + s]; },
-        [&](int64_t row, const Acc<PG> &acc) {
-            if (L.g != 0) return;
-            const int64_t idx = row * S_ + s;
-            const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
-            if (OP == kOpPre) {
-                const double2 z = p[idx];
-                a.ph[idx] = make_double2(a.pc * (2.0 * z.x - a.pc * y.x), a.pc * (2.0 * z.y - a.pc * y.y));
-                return;
-            }
-            a.v[idx] = y;
-            const double2 h = a.rh[idx];
-            red[0] += h.x * y.x + h.y * y.y;
-            red[1] += h.x * y.y - h.y * y.x;
-        });
+    auto gather = [&](int c) { return p[(int64_t)c * S_ + s]; };
+    auto emit = [&](int64_t row, const Acc<PG> &acc) {
+        if (L.g != 0) return;
+        const int64_t idx = row * S_ + s;
+        const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
+        if (OP == kOpPre) {
+            const double2 z = p[idx];
+            a.ph[idx] = make_double2(a.pc * (2.0 * z.x - a.pc * y.x), a.pc * (2.0 * z.y - a.pc * y.y));
+            return;
+        }
+        a.v[idx] = y;
+        const double2 h = a.rh[idx];
+        red[0] += h.x * y.x + h.y * y.y;
+        red[1] += h.x * y.y - h.y * y.x;
+    };
+    // the preconditioner takes the real part alone when a.areal is set
+    if (OP == kOpPre && a.areal)
+        staged_spmv<G, PG, true>(a, rpb, TR, L, act, gather, emit);
+    else
+        staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit);
     if (OP == kOpV) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
 }
 
@@ -1791,7 +1849,7 @@ __device__ __forceinline__ void spmv_t_head(const SolveArgs &a, int S, int npart
 
 // alpha = rho / <r^, v> and s = r - alpha v (formed for the gathered columns), then
 // OP kOpT:   t = A s ; partials T: [<t, s>.re, <t, s>.im, |t|^2, |s|^2]
-// OP kOpPre: sh = c (2 s - c A s) (the polynomial preconditioner applied to s), no partials
+// OP kOpPre: sh = c (2 s - c Are s) (the polynomial preconditioner applied to s), no partials
 // OP kOpTP:  t = A sh with sh gathered from a.sh; partials T as for kOpT (s at the own row)
 template <int G, bool PG, int OP>
 __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, int rpb, int TR, int nparts, int it) {
@@ -1806,28 +1864,30 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
     const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     const double2 *r = a.r, *v = a.v, *sh = a.sh;
-    staged_spmv<G, PG>(
-        a, rpb, TR, L, act,
-        [&](int c) {
-            const int64_t j = (int64_t)c * S_ + s;
-            if (OP == kOpTP) return sh[j];
-            return c_sub(r[j], c_mul(alpha, v[j]));
-        },
-        [&](int64_t row, const Acc<PG> &acc) {
-            if (L.g != 0) return;
-            const int64_t idx = row * S_ + s;
-            const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
-            const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
-            if (OP == kOpPre) {
-                a.sh[idx] = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
-                return;
-            }
-            a.t[idx] = y;
-            red[0] += y.x * sv.x + y.y * sv.y;
-            red[1] += y.x * sv.y - y.y * sv.x;
-            red[2] += y.x * y.x + y.y * y.y;
-            red[3] += sv.x * sv.x + sv.y * sv.y;
-        });
+    auto gather = [&](int c) {
+        const int64_t j = (int64_t)c * S_ + s;
+        if (OP == kOpTP) return sh[j];
+        return c_sub(r[j], c_mul(alpha, v[j]));
+    };
+    auto emit = [&](int64_t row, const Acc<PG> &acc) {
+        if (L.g != 0) return;
+        const int64_t idx = row * S_ + s;
+        const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
+        const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
+        if (OP == kOpPre) {
+            a.sh[idx] = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
+            return;
+        }
+        a.t[idx] = y;
+        red[0] += y.x * sv.x + y.y * sv.y;
+        red[1] += y.x * sv.y - y.y * sv.x;
+        red[2] += y.x * y.x + y.y * y.y;
+        red[3] += sv.x * sv.x + sv.y * sv.y;
+    };
+    if (OP == kOpPre && a.areal)
+        staged_spmv<G, PG, true>(a, rpb, TR, L, act, gather, emit);
+    else
+        staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit);
     if (OP != kOpPre) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
 }
 
@@ -2785,6 +2845,11 @@ size_t stage_lds_bytes(int TR, bool per_gdir) {
     const size_t ecap = (size_t)TR * kStageNnzPerRow;
     return ecap * (8 * (per_gdir ? 4 : 2) + 4) + 4 * (size_t)(TR + 1) + 32;
 }
+// staged tile of the real-part-only (preconditioner) kernels: one value per entry
+static size_t stage_lds_bytes_ro(int TR) {
+    const size_t ecap = (size_t)TR * kStageNnzPerRow;
+    return ecap * (8 + 4) + 4 * (size_t)(TR + 1) + 32;
+}
 static size_t stage_lds_bytes_bre(int TR, bool per_gdir) {
     return stage_lds_bytes(TR, per_gdir) + (size_t)TR * kStageNnzPerRow * 8;
 }
@@ -3038,9 +3103,23 @@ void fit_spmv_grid(SolveGeom &g, bool per_gdir) {
     const void *fn = nullptr;
     size_t lds = 0;
     if (g.mapping == kMapStaged || g.mapping == kMapSL) {
+        // the least occupancy of the t-kernels (operator and preconditioner) with a full tile
         const int TR = g.mapping == kMapStaged ? stage_rows(g, per_gdir) : 0;
         lds = TR > 0 ? stage_lds_bytes(TR, per_gdir) : 0;
-        DMRI_DISPATCH_G(g.G, per_gdir, fn = reinterpret_cast<const void *>(&spmv_t_kernel<GG, PP, kOpPre>));
+        DMRI_DISPATCH_G(g.G, per_gdir, fn = reinterpret_cast<const void *>(&spmv_t_kernel<GG, PP, kOpTP>));
+        const void *fpre = nullptr;
+        DMRI_DISPATCH_G(g.G, per_gdir, fpre = reinterpret_cast<const void *>(&spmv_t_kernel<GG, PP, kOpPre>));
+        int nb_pre = 0;  // with the full tile (the preconditioner may take the imaginary part too)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_pre, fpre, g.spmv_threads, lds) != hipSuccess)
+            throw Error(-2, "occupancy query of the SpMV kernels failed");
+        int dev = 0, cus = 0, nb = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, g.spmv_threads, lds) != hipSuccess)
+            throw Error(-2, "occupancy query of the SpMV kernels failed");
+        const int64_t cap = (int64_t)std::min(nb, nb_pre) * cus / 8 * 8;
+        if (cap >= 8 && cap < g.grid) g.grid = (int)cap;
+        return;
     } else if (g.mapping == kMapWave) {
         fn = per_gdir ? reinterpret_cast<const void *>(&spmv_t_wave_kernel<true, kOpPre>)
                       : reinterpret_cast<const void *>(&spmv_t_wave_kernel<false, kOpPre>);
@@ -3101,7 +3180,9 @@ static void spmv_v_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipS
                                   0, a, a.rowptr, a.col, a.vals, g.S, it);
     } else {
         const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
-        const uint32_t lds = TR > 0 ? (uint32_t)stage_lds_bytes(TR, a.per_gdir) : 0u;
+        const uint32_t lds =
+            TR > 0 ? (uint32_t)(OP == kOpPre && a.areal ? stage_lds_bytes_ro(TR) : stage_lds_bytes(TR, a.per_gdir))
+                   : 0u;
         DMRI_DISPATCH_G(g.G, a.per_gdir,
                         hipExtLaunchKernelGGL((spmv_v_kernel<GG, PP, OP>), dim3(g.grid), dim3(g.spmv_threads), lds, s,
                                               e0, e1, 0, a, g.S, g.rpb, TR, it));
@@ -3119,7 +3200,9 @@ static void spmv_t_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipS
                                   0, a, a.rowptr, a.col, a.vals, g.S, (int)g.grid, it);
     } else {
         const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
-        const uint32_t lds = TR > 0 ? (uint32_t)stage_lds_bytes(TR, a.per_gdir) : 0u;
+        const uint32_t lds =
+            TR > 0 ? (uint32_t)(OP == kOpPre && a.areal ? stage_lds_bytes_ro(TR) : stage_lds_bytes(TR, a.per_gdir))
+                   : 0u;
         DMRI_DISPATCH_G(g.G, a.per_gdir,
                         hipExtLaunchKernelGGL((spmv_t_kernel<GG, PP, OP>), dim3(g.grid), dim3(g.spmv_threads), lds, s,
                                               e0, e1, 0, a, g.S, g.rpb, TR, (int)g.grid, it));
@@ -3188,6 +3271,18 @@ void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStrea
         check_launch();
     }
 }
+// areal[e] = vals[vw e]: the scaled real part alone, for the preconditioner SpMVs
+__global__ void __launch_bounds__(256) extract_real_kernel(int64_t nnz, const double *vals, int vw, double *areal) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
+        areal[e] = vals[vw * e];
+}
+void launch_extract_real(int64_t nnz, const double *vals, int vw, double *areal, hipStream_t s) {
+    unsigned g = grid_for(nnz, 256);
+    if (g > 4096u) g = 4096u;
+    extract_real_kernel<<<g, 256, 0, s>>>(nnz, vals, vw, areal);
+    check_launch();
+}
+
 // Zero the Krylov states of a step and the all-done flag (one dispatch instead of two fills)
 __global__ void __launch_bounds__(256) reset_state_kernel(int *st, int nwords, int *flag) {
     for (int i = threadIdx.x; i < nwords; i += blockDim.x) st[i] = 0;

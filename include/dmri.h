@@ -174,7 +174,9 @@ typedef struct {
                               5 = mapping 1 over tiles whose distinct columns are staged once
                               in LDS (lanes * signals <= 64) */
     int32_t preconditioner; /* 0 = Jacobi (the symmetric diagonal scaling) with the degree-1
-                               polynomial c (2I - c A) of the scaled system, c = 1.8 / b where b
+                               polynomial c (2I - c A) of the scaled system (of its real part alone
+                               when the gradient term's bound is below 5 % of the real part's),
+                               c = 1.8 / b where b
                                bounds the largest eigenvalue of its real part (1.05 x a 30-step
                                power iteration estimate, capped by the Gershgorin bound),
                                applied from the right (the residual the
