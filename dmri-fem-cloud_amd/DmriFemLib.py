@@ -23,7 +23,7 @@ Reference behaviour kept (file:line in DmriFemLib.py of the reference):
     q = sqrt(b / int_0^T F^2) from the sympy profile (826-842); bvalue or gvalue (847-858).
   * MRI_simulation: theta 0.5, nskip 5 (860-863); time loop while t < T + k with t += k and
     the right-hand side profile lagged one step (878-911); u0 = 1 on both compartments (865-876).
-  * PostProcessing: signal = int (1-p) u0r + p u1r / int 1 and the result lines and log.txt
+  * PostProcessing: signal = int (1-p) u0r + p u1r / int u(0) and the result lines and log.txt
     append of 917-988.
 The PDE is solved by libdmri; this module only prepares inputs and formats outputs.
 Not mirrored: dolfin plotting, .pvd output, and the strong pseudo-periodic function spaces.
@@ -254,9 +254,24 @@ class MRI_simulation:
             t += self.k
         return np.asarray(fA), np.asarray(fB), (np.asarray(FB) if integral else None)
 
+    def InitialCondition(self, mydomain, ic):
+        """Initial state per mesh vertex (MRI_simulation.InitialCondition, DmriFemLib.py:865-876): the
+        real part of every compartment takes it, the imaginary part is 0.  ic: None (1 everywhere),
+        an array of one value per vertex, or a callable of the (nv, 3) vertex coordinates."""
+        xyz = mydomain.mymesh.xyz
+        if ic is None:
+            return None
+        if callable(ic):
+            u0 = np.asarray(ic(xyz), dtype=np.float64).ravel()
+        else:
+            u0 = np.asarray(ic, dtype=np.float64).ravel()
+        if u0.shape[0] != xyz.shape[0]:
+            raise ValueError("the initial condition needs one value per mesh vertex (%d), got %d"
+                             % (xyz.shape[0], u0.shape[0]))
+        return u0
+
     def solve(self, mydomain, mri_para, linsolver, ic=None):
-        if ic is not None:
-            raise NotImplementedError("a custom initial condition is not available in this build (u0 = 1)")
+        self.u0_vertex = self.InitialCondition(mydomain, ic)
         mesh = mydomain.mymesh
         phase = None
         if mydomain.IsDomainMultiple:
@@ -284,7 +299,7 @@ class MRI_simulation:
         self.signal_c, self.stats = ctx.solve(fA, fB, self.k, mri_para.qvalue, g, theta=self.theta,
                                               rtol=p["relative_tolerance"], atol=p["absolute_tolerance"],
                                               maxit=min(int(p["maximum_iterations"]), 2 ** 31 - 1),
-                                              F_explicit=FB)
+                                              F_explicit=FB, u0=self.u0_vertex)
         self.elapsed_time = time.time() - start
         self.t = len(fA) * self.k
         self.nsteps = len(fA)
@@ -296,16 +311,22 @@ def PostProcessing(mydomain, mri_para, mri_simu, plt, ms=""):
     ctx = mri_simu.ctx
     info = ctx.info()
     whole_vol = info["volume"]
-    voi = whole_vol  # int Dirac_Delta dx with Dirac_Delta = 1 on um-scale meshes (DmriFemLib.py:865-871)
     signal = mri_simu.signal_c.real * whole_vol
+    rowptr, _, dof_of = ctx.export_pattern()
+    w = np.add.reduceat(ctx.export_values("m"), rowptr[:-1])  # int phi_d over the DOF's compartment
+    nv = info["nv"]
+    u0 = getattr(mri_simu, "u0_vertex", None)
+    u0d = np.ones(w.shape[0])  # the initial state per DOF
+    if u0 is not None:
+        for c in range(dof_of.shape[0] // nv):
+            d = dof_of[c * nv:(c + 1) * nv]
+            u0d[d[d >= 0]] = u0[d >= 0]
+    voi = float((w * u0d).sum())  # int Dirac_Delta dx (DmriFemLib.py:924)
     if mydomain.IsDomainMultiple:
         u = ctx.solution()
-        rowptr, _, dof_of = ctx.export_pattern()
-        w = np.add.reduceat(ctx.export_values("m"), rowptr[:-1])
-        nv = info["nv"]
         live0 = dof_of[:nv]
         n0 = int((live0 >= 0).sum())
-        initial0, initial1 = float(w[:n0].sum()), float(w[n0:].sum())
+        initial0, initial1 = float((w * u0d)[:n0].sum()), float((w * u0d)[n0:].sum())
         signal0, signal1 = float((w[:n0] * u[:n0].real).sum()), float((w[n0:] * u[n0:].real).sum())
         if np.isscalar(mydomain.kappa):
             out_text = ("b: %.3f, g: %.3f, q: %.3e, Signal: %.3e, Normalized signal: %.6e, kappa: %.3e, dt: %.3f, "

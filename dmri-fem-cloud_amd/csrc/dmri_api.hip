@@ -33,7 +33,7 @@ constexpr int kEventPool = 512;
 struct SolverWork {
     DevArray<double2> x, r, rh, p, v, t, b, xprev, xprev2, ybc, bper, ph, sh;
     DevArray<double2> yh[kMaxHist], hp[kMaxHist], hq[kMaxHist];  // projection history
-    DevArray<double> sc, vals, bre, areal, part, tot, coef, coefP, sig, spart, ppart, ptot, pgam, gsig;
+    DevArray<double> sc, vals, bre, areal, part, tot, coef, coefP, sig, spart, ppart, ptot, pgam, gsig, u0v;
     DevArray<KrylovState> st;
     DevArray<int> all_done, mask;
     KrylovState *h_st = nullptr, *d_h_st = nullptr;  // pinned host copy and its device address
@@ -723,7 +723,13 @@ struct Stepper {
                 }
             }
         DMRI_HIP(hipMemcpyAsync(w.coef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
-        launch_init_x(n, S, w.sc.p, w.x.p, s);
+        if (seq->u0_vertex) {
+            if (w.u0v.n != (size_t)c->nv) w.u0v.alloc(c->nv);
+            DMRI_HIP(hipMemcpyAsync(w.u0v.p, seq->u0_vertex, sizeof(double) * c->nv, hipMemcpyHostToDevice, s));
+            launch_init_x_vertex(n, S, w.sc.p, c->perm.p, c->dof_vert.p, w.u0v.p, w.x.p, s);
+        } else {
+            launch_init_x(n, S, w.sc.p, w.x.p, s);
+        }
         // Finalize kernels reduce the partials unless one signal's few partials can be summed by
         // every consumer block directly (mixed first/later iterations need per-signal counts).
         const bool use_fin = S > 1 || std::max(geo.grid, geo.vgrid) > 512;
@@ -1461,6 +1467,11 @@ void validate_seq(const dmri_seq *seq, const dmri_solver_opts *opts) {
         require(std::isfinite(seq->f_implicit[i]) && std::isfinite(seq->f_explicit[i]), DMRI_ERR_ARG,
                 "time profile has a non-finite value");
 }
+void validate_u0(const dmri_ctx *c, const dmri_seq *seq) {
+    if (!seq->u0_vertex) return;
+    for (int64_t v = 0; v < c->nv; ++v)
+        require(std::isfinite(seq->u0_vertex[v]), DMRI_ERR_ARG, "u0_vertex has a non-finite value");
+}
 
 // nsig signals with per-signal q and gradient direction, in launch groups of <= 64.
 void solve_batched_impl(dmri_ctx *c, int nsig, const double *q, const double *gdir, const dmri_seq *seq,
@@ -1468,6 +1479,7 @@ void solve_batched_impl(dmri_ctx *c, int nsig, const double *q, const double *gd
     require(c->assembled, DMRI_ERR_STATE, "solve: call dmri_assemble first");
     require(nsig >= 1 && q && gdir && sig, DMRI_ERR_ARG, "solve: need nsig >= 1 and non-null q, gdir, sig_out");
     validate_seq(seq, opts);
+    validate_u0(c, seq);
     if (c->periodic && seq->nsteps > 0) {
         require(seq->F_explicit != nullptr, DMRI_ERR_ARG,
                 "periodic directions are set: dmri_seq.F_explicit (int_0^t f per step) is required");
@@ -1848,6 +1860,7 @@ int dmri_adjoint(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts,
         require(c->assembled, DMRI_ERR_STATE, "dmri_adjoint: call dmri_assemble first");
         require(!c->periodic, DMRI_ERR_ARG, "dmri_adjoint: the weak pseudo-periodic condition is not supported");
         validate_seq(seq, opts);
+        validate_u0(c, seq);
         set_device(c);
         adjoint_impl(c, seq, opts, aopts, sig_out, grads, stats);
     });

@@ -229,6 +229,9 @@ void launch_renumber(int64_t n, const int *iperm, const int *src, int *dst, hipS
 void launch_unscale_perm(int64_t n, int S, int sig, const int *perm, const double *sc, const double2 *x, double2 *u,
                          hipStream_t s);
 void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s);
+// x[i][s] = u0[dof_vert[perm[i]]] / sc[i] (u0 per mesh vertex)
+void launch_init_x_vertex(int64_t n, int S, const double *sc, const int *perm, const int *dof_vert, const double *u0,
+                          double2 *x, hipStream_t s);
 // Gershgorin bound max_i sum_j |a_ij| of the real implicit part of the scaled operator (the
 // first value of every vals entry, vw values per entry) into out[0]; part >= kMaxGrid doubles
 void launch_gershgorin(int64_t n, const int *rowptr, const double *vals, int vw, double *part, double *out,

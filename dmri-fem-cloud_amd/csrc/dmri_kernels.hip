@@ -3047,6 +3047,19 @@ void launch_gershgorin(int64_t n, const int *rowptr, const double *vals, int vw,
     max_final_kernel<<<1, kThreads, 0, s>>>(part, (int)g, out);
     check_launch();
 }
+// x[i][s] = u0(vertex of solver DOF i) / sc_i: a custom initial state in the scaled variables
+__global__ void init_x_vertex_kernel(int64_t n, int S, const double *sc, const int *perm, const int *dof_vert,
+                                     const double *u0, double2 *x) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n * S) return;
+    const int64_t i = k / S;
+    x[k] = make_double2(u0[dof_vert[perm[i]]] / sc[i], 0.0);
+}
+void launch_init_x_vertex(int64_t n, int S, const double *sc, const int *perm, const int *dof_vert, const double *u0,
+                          double2 *x, hipStream_t s) {
+    init_x_vertex_kernel<<<grid_for(n * S, 256), 256, 0, s>>>(n, S, sc, perm, dof_vert, u0, x);
+    check_launch();
+}
 void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s) {
     init_x_kernel<<<grid_for(n * S, 256), 256, 0, s>>>(n, S, sc, x);
     check_launch();

@@ -41,7 +41,7 @@ class _Seq(ctypes.Structure):
     _fields_ = [("nsteps", ctypes.c_int32), ("f_implicit", ctypes.POINTER(ctypes.c_double)),
                 ("f_explicit", ctypes.POINTER(ctypes.c_double)), ("dt", ctypes.c_double),
                 ("theta", ctypes.c_double), ("q", ctypes.c_double), ("gdir", ctypes.c_double * 3),
-                ("F_explicit", ctypes.POINTER(ctypes.c_double))]
+                ("F_explicit", ctypes.POINTER(ctypes.c_double)), ("u0_vertex", ctypes.POINTER(ctypes.c_double))]
 
 
 class _Opts(ctypes.Structure):
@@ -220,7 +220,7 @@ class Context:
     # -- solve --------------------------------------------------------------------------
     @staticmethod
     def _seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes, guess=0,
-                  mapping=0, F_explicit=None, precond=0):
+                  mapping=0, F_explicit=None, precond=0, u0=None, nv=None):
         fA = np.ascontiguousarray(f_implicit, dtype=np.float64)
         fB = np.ascontiguousarray(f_explicit, dtype=np.float64)
         if fA.shape != fB.shape:
@@ -233,6 +233,10 @@ class Context:
         s.f_implicit = _ptr(fA, ctypes.c_double)
         s.f_explicit = _ptr(fB, ctypes.c_double)
         s.F_explicit = _ptr(FB, ctypes.c_double) if FB is not None else None
+        U0 = None if u0 is None else np.ascontiguousarray(u0, dtype=np.float64).ravel()
+        if U0 is not None and nv is not None and U0.shape[0] != nv:
+            raise ValueError(f"u0 needs one value per mesh vertex ({nv}), got {U0.shape[0]}")
+        s.u0_vertex = _ptr(U0, ctypes.c_double) if U0 is not None else None
         s.dt, s.theta, s.q = float(dt), float(theta), float(q)
         for i in range(3):
             s.gdir[i] = float(gdir[i])
@@ -242,15 +246,19 @@ class Context:
         o.initial_guess = int(guess)
         o.mapping = int(mapping)
         o.preconditioner = int(precond)
-        return s, o, (fA, fB, FB)
+        return s, o, (fA, fB, FB, U0)
+
+    def _nv(self, u0):
+        return None if u0 is None else int(self.info()["nv"])
 
     def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0):
+              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None):
         """precond: 0 = Jacobi + degree-1 polynomial (default), 1 = Jacobi only.  guess: Krylov start
         of each step, 0 = residual-minimising projection on the last states (default), 1 = previous
-        state, 2 = linear, 3 = quadratic extrapolation."""
+        state, 2 = linear, 3 = quadratic extrapolation.  u0: initial state per mesh vertex (real part
+        of every compartment's DOF there; None = 1)."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes,
-                                    guess, mapping, F_explicit, precond)
+                                    guess, mapping, F_explicit, precond, u0, self._nv(u0))
         sig = np.zeros(2)
         st = _Stats()
         _check(LIB.dmri_solve(self._h, ctypes.byref(s), ctypes.byref(o), _ptr(sig, ctypes.c_double),
@@ -258,13 +266,13 @@ class Context:
         return complex(sig[0], sig[1]), {k: getattr(st, k) for k, _ in _Stats._fields_}
 
     def solve_batched(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-                      profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0):
+                      profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None):
         """Signals for every (q[s], gdir[s]) on the shared pattern; returns complex array (nsig,)."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(gdirs, dtype=np.float64).reshape(-1, 3),
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
-                                    lanes, guess, mapping, F_explicit, precond)
+                                    lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0))
         sig = np.zeros(2 * qs.shape[0])
         st = _Stats()
         _check(LIB.dmri_solve_batched(self._h, qs.shape[0], _ptr(qs, ctypes.c_double), _ptr(g, ctypes.c_double),
@@ -288,10 +296,10 @@ class Context:
         _check(LIB.dmri_set_tuning(self._h, int(nsig), 1 if per_gdir else 0, int(lanes), int(mapping)))
 
     def adjoint(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-                checkpoint_stride=0, lanes=0, guess=0, mapping=0, precond=0):
+                checkpoint_stride=0, lanes=0, guess=0, mapping=0, precond=0, u0=None):
         """Signal and its adjoint gradients (see dmri_adjoint): returns (signal, grads dict, stats)."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, False, lanes,
-                                    guess, mapping, None, precond)
+                                    guess, mapping, None, precond, u0, self._nv(u0))
         ao = _AdjOpts()
         ao.checkpoint_stride = int(checkpoint_stride)
         n = s.nsteps

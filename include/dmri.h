@@ -147,6 +147,13 @@ typedef struct {
                                  (GCloud: t_{n-1}, MRI_parameters.itime_profile, DmriFemLib.py:834-835,
                                  901-902): the phase of the pseudo-periodic mirror values; required
                                  when periodic directions are set, ignored otherwise (may be NULL) */
+    const double *u0_vertex;  /* initial state: nv real values per mesh vertex, taken by the real
+                                 part of every compartment's DOF at that vertex (the imaginary part
+                                 is 0), as MRI_simulation.InitialCondition assigns a P1 field to
+                                 u_0.sub(0) and u_0.sub(2) (DmriFemLib.py:865-876); NULL = 1
+                                 everywhere.  The signal stays sum_i w_i Re u_i / |Omega|: a caller
+                                 with a custom state divides by int u0 / |Omega| (PostProcessing's
+                                 vol_of_interest, DmriFemLib.py:924) */
 } dmri_seq;
 
 typedef struct {

@@ -260,6 +260,46 @@ def test_projection_guess_batched_matches_quadratic(pg):
     assert abs(one - got[3]) <= 1e-11 * abs(got[3])
 
 
+def _u0_vertex(mesh):
+    """A discontinuous initial state: 1 on one half of the mesh, 0.25 on the other, 2 near the axis."""
+    x = mesh.xyz
+    u = np.where(x[:, 0] < np.median(x[:, 0]), 1.0, 0.25)
+    u[np.hypot(x[:, 0], x[:, 1]) < 1.5] = 2.0
+    return u
+
+
+def _u0_dofs(S, mesh, u0v):
+    nv = mesh.xyz.shape[0]
+    u = np.zeros(S.ndof, dtype=np.complex128)
+    for c in range(S.dof_of.shape[0] // nv):
+        d = S.dof_of[c * nv:(c + 1) * nv]
+        u[d[d >= 0]] = u0v[d >= 0]
+    return u
+
+
+@pytest.mark.parametrize("name,b", [("cyl1", 1000.0), ("cyl2", 2500.0)])
+def test_custom_initial_state_matches_oracle(name, b):
+    """u0 per vertex (the reference's InitialCondition with a P1 field): plain and batched solves
+    against the oracle started from the same state."""
+    mesh, kw = MESHES[name]
+    fA, fB = _pgse_gcloud(10600.0, 43100.0, 400.0)
+    q = O.q_from_b_pgse(b, 10600.0, 43100.0)
+    u0v = _u0_vertex(mesh)
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    S = O.System(mesh.xyz, mesh.tets, mesh.phase, D=kw["D"], kappa=kw["kappa"])
+    ref, _, _ = S.solve((0, 1, 0), q, 400.0, fA, fB, rtol=1e-13, u0=_u0_dofs(S, mesh, u0v))
+    got, _ = ctx.solve(fA, fB, 400.0, q, (0, 1, 0), rtol=1e-13, u0=u0v)
+    assert abs(got.real - ref) <= 1e-9 * abs(ref)
+    sig, _ = ctx.solve_batched(fA, fB, 400.0, np.array([q, 0.0]), np.array([[0, 1, 0], [1, 0, 0]]), rtol=1e-13,
+                               u0=u0v)
+    assert abs(sig[0] - got) <= 1e-11 * abs(got)
+    # b = 0: the state only relaxes, so the signal is int u0 / |Omega| up to the T2 decay
+    w = np.add.reduceat(S.mats["m"], S.rowptr[:-1])
+    assert abs(sig[1].real - (w * _u0_dofs(S, mesh, u0v).real).sum() / w.sum()) <= 1e-10
+    with pytest.raises(ValueError):
+        ctx.solve(fA, fB, 400.0, q, (0, 1, 0), u0=u0v[:-1])
+
+
 def test_sampled_event_profile_and_spmv_replay():
     """profile=k times the SpMV pair of the first k Krylov iterations of each step only, leaves the
     result unchanged, and the back-to-back replay reports positive per-launch times."""
@@ -328,3 +368,48 @@ def test_polynomial_preconditioner_matches_jacobi_and_oracle(mapping, lanes):
     for j in (1, 2):
         ref, _, _ = S.solve(gd[j], qs[j], 200.0, fA, fB, rtol=1e-13)
         assert abs(poly[j].real - ref) <= 1e-9 * abs(ref)
+
+
+def test_dmrifemlib_initial_condition_normalises_by_its_integral(tmp_path, monkeypatch):
+    """MRI_simulation.solve(..., ic) with a discontinuous P1 initial state; PostProcessing divides
+    by vol_of_interest = int ic dx as the reference does (DmriFemLib.py:865-876, 924)."""
+    import re
+
+    import DmriFemLib as L
+
+    monkeypatch.chdir(tmp_path)
+    mesh = L.BoxMesh(L.Point(0, 0, 0), L.Point(10, 8, 6), 6, 5, 4)
+    p = L.MRI_parameters()
+    p.bvalue = 1000
+    p.delta, p.Delta = 10600.0, 43100.0
+    p.set_gradient_dir(mesh, 1, 0, 0)
+    p.T = p.Delta + p.delta
+    p.fs_sym = L.sp.Piecewise((1., p.s < p.delta), (0., p.s < p.Delta), (-1., p.s < p.T), (0., True))
+    p.Apply()
+    dom = L.MyDomain(mesh, p)
+    dom.PeriodicDir = [0, 0, 0]
+    dom.IsDomainPeriodic = False
+    dom.IsDomainMultiple = False
+    dom.D0 = 3e-3
+    dom.Apply()
+    simu = L.MRI_simulation()
+    simu.k = 400
+    solver = L.KrylovSolver("bicgstab", "jacobi")
+    solver.parameters["relative_tolerance"] = 1e-13
+    solver.parameters["absolute_tolerance"] = 0.0
+
+    def ic(x):
+        return np.where(x[:, 0] < 5.0, 1.0, 0.5)
+
+    simu.solve(dom, p, solver, ic=ic)
+    out = L.PostProcessing(dom, p, simu, None)
+    got = float(re.search(r"Normalized signal: ([0-9.e+-]+)", out).group(1))
+    voi_got = float(re.search(r"vol_of_interest: ([0-9.e+-]+)", out).group(1))
+    fA, fB = _pgse_gcloud(10600.0, 43100.0, 400.0)
+    S = O.System(mesh.xyz, mesh.tets, None, D=3e-3)
+    u0 = _u0_dofs(S, mesh, ic(mesh.xyz))
+    ref, _, _ = S.solve((1, 0, 0), p.qvalue, 400.0, fA, fB, rtol=1e-13, u0=u0)
+    w = np.add.reduceat(S.mats["m"], S.rowptr[:-1])
+    voi = float((w * u0.real).sum())
+    assert abs(voi_got - voi) <= 1e-3 * voi
+    assert abs(got - ref * w.sum() / voi) <= 2e-6 * abs(got)
