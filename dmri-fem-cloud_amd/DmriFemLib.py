@@ -89,10 +89,15 @@ class Point:
 
 
 def Mesh(path_or_mesh):
-    """Mesh from a DOLFIN XML file (Mesh(fmesh), comri main.cpp:200) or a TetMesh."""
+    """Mesh from a DOLFIN XML file (Mesh(fmesh), comri main.cpp:200), a TetMesh, or an ASCII
+    gmsh 2.x file (``.msh``, ``.msh.zip``) taken as dolfin-convert would convert it to XML
+    (comri/meshes/neuron_download/dolfin-convert.py:357-597)."""
     if isinstance(path_or_mesh, dmri_mesh.TetMesh):
         return path_or_mesh
-    return dmri_mesh.read_dolfin_xml(str(path_or_mesh))
+    path = str(path_or_mesh)
+    if path.endswith((".msh", ".gmsh", ".msh.zip", ".gmsh.zip")):
+        return dmri_mesh.read_gmsh(path)
+    return dmri_mesh.read_dolfin_xml(path)
 
 
 def BoxMesh(p0, p1, nx, ny, nz):

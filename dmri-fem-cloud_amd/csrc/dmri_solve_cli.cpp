@@ -1,6 +1,7 @@
 // dmri-solve: command-line driver over the libdmri C ABI.
 //
 // Accepts the options of the reference drivers and prints their result lines:
+// Meshes: DOLFIN XML, or ASCII gmsh 2.x (.msh/.gmsh) read as dolfin-convert would convert it.
 //   GCloud style (GCloudDmriSolver.py:61-133):  -f mesh.xml -M 0|1 -b B -q Q -p kappa -d delta -D Delta
 //                                               -k dt -K D0 -T2 T2 -gdir gx gy gz
 //   comri style (comri/two-comp/hpc-fenics-cpp/main.cpp:581-648): -m mesh.xml -N steps -b B -q G -d delta
@@ -8,12 +9,14 @@
 // Extra: -phase file (per-cell phase, one integer per line, or a DOLFIN mesh_function /
 // mesh_value_collection file whose values are partition markers taken modulo 2, as
 // CreatePhaseFunc does, DmriFemLib.py:758-765), -variant gcloud|comri|comri-until-T,
-// -theta, -rtol, -atol, -maxit, -device.
+// -theta, -rtol, -atol, -maxit, -device, -convert out.xml (write the mesh as DOLFIN XML and stop,
+// as dolfin-convert mesh.msh mesh.xml does; no device is used).
 //
 // Sequences: gcloud = loop t < T + k, t += k, explicit side lagged one step, PGSE profile
 // exclusive at T (DmriFemLib.py:897-910, GCloudDmriSolver.py:188-193); comri = same f on
 // both sides, loop t < T + dt, FT inclusive at Delta + delta (main.cpp:87-92,296-318);
 // comri-until-T = the multilayer loop t < T (comri/multilayer/hpc-fenics-cpp/main.cpp:301).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -74,6 +77,116 @@ bool read_dolfin_xml(const std::string &path, Mesh &m) {
     return !m.xyz.empty() && !m.tets.empty();
 }
 
+// ASCII gmsh 2.x, read as dolfin-convert's gmsh2xml turns it into the DOLFIN XML mesh the
+// reference solvers load (comri/meshes/neuron_download/dolfin-convert.py:357-597): the tetrahedra
+// (element type 4) in file order are the cells; the nodes they use are the vertices, numbered in
+// $Nodes order; each coordinate passes through the converter's "%g" (write_vertex, 889-892).
+// The four nodes after an element's tags are taken for any tag count (the converter needs two).
+bool read_gmsh(const std::string &path, Mesh &m) {
+    std::ifstream in(path);
+    if (!in) return false;
+    std::string line;
+    auto next_line = [&]() -> bool { return (bool)std::getline(in, line); };
+    auto trim = [](std::string &s) {
+        while (!s.empty() && (s.back() == '\r' || s.back() == ' ')) s.pop_back();
+    };
+    std::vector<long> node_id;
+    std::vector<double> xyz;
+    std::vector<long> tet_nodes;
+    bool fmt_ok = false, have_nodes = false;
+    while (next_line()) {
+        trim(line);
+        if (line == "$MeshFormat") {
+            if (!next_line()) return false;
+            std::istringstream ss(line);
+            std::string ver;
+            int ftype = -1;
+            ss >> ver >> ftype;
+            fmt_ok = !ver.empty() && ver[0] == '2' && ftype == 0;
+            if (!fmt_ok) return false;
+        } else if (line == "$Nodes") {
+            long n = 0;
+            if (!next_line() || (n = std::atol(line.c_str())) <= 0) return false;
+            node_id.resize(n);
+            xyz.resize(3 * n);
+            char buf[64];
+            for (long i = 0; i < n; ++i) {
+                if (!next_line()) return false;
+                std::istringstream ss(line);
+                double c[3];
+                if (!(ss >> node_id[i] >> c[0] >> c[1] >> c[2])) return false;
+                for (int d = 0; d < 3; ++d) {
+                    std::snprintf(buf, sizeof(buf), "%g", c[d]);
+                    xyz[3 * i + d] = std::strtod(buf, nullptr);
+                }
+            }
+            have_nodes = true;
+        } else if (line == "$Elements") {
+            long n = 0;
+            if (!next_line() || (n = std::atol(line.c_str())) <= 0) return false;
+            for (long i = 0; i < n; ++i) {
+                if (!next_line()) return false;
+                std::istringstream ss(line);
+                long id, type, ntags, t;
+                if (!(ss >> id >> type >> ntags)) return false;
+                for (long k = 0; k < ntags; ++k) ss >> t;
+                if (type != 4) continue;
+                for (int k = 0; k < 4; ++k) {
+                    if (!(ss >> t)) return false;
+                    tet_nodes.push_back(t);
+                }
+            }
+        }
+    }
+    if (!fmt_ok || !have_nodes || tet_nodes.empty()) return false;
+    long maxid = 0;
+    for (long v : node_id) maxid = std::max(maxid, v);
+    for (long v : tet_nodes) maxid = std::max(maxid, v);
+    std::vector<long> pos(maxid + 1, -1);
+    for (size_t i = 0; i < node_id.size(); ++i) {
+        if (node_id[i] < 0) return false;
+        pos[node_id[i]] = (long)i;
+    }
+    std::vector<char> used(node_id.size(), 0);
+    for (long v : tet_nodes) {
+        if (v < 0 || pos[v] < 0) return false;
+        used[pos[v]] = 1;
+    }
+    std::vector<int32_t> renum(node_id.size(), -1);
+    int32_t nv = 0;
+    for (size_t i = 0; i < node_id.size(); ++i)
+        if (used[i]) {
+            renum[i] = nv++;
+            m.xyz.insert(m.xyz.end(), xyz.begin() + 3 * i, xyz.begin() + 3 * i + 3);
+        }
+    m.tets.resize(tet_nodes.size());
+    for (size_t k = 0; k < tet_nodes.size(); ++k) m.tets[k] = renum[pos[tet_nodes[k]]];
+    return true;
+}
+
+// DOLFIN XML with every coordinate printed to 17 significant digits (exact round trip)
+bool write_dolfin_xml(const std::string &path, const Mesh &m) {
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    const size_t nv = m.xyz.size() / 3, nc = m.tets.size() / 4;
+    std::fprintf(f, "<?xml version=\"1.0\"?>\n<dolfin xmlns:dolfin=\"http://fenicsproject.org\">\n"
+                    "  <mesh celltype=\"tetrahedron\" dim=\"3\">\n    <vertices size=\"%zu\">\n", nv);
+    for (size_t i = 0; i < nv; ++i)
+        std::fprintf(f, "      <vertex index=\"%zu\" x=\"%.17g\" y=\"%.17g\" z=\"%.17g\" />\n", i, m.xyz[3 * i],
+                     m.xyz[3 * i + 1], m.xyz[3 * i + 2]);
+    std::fprintf(f, "    </vertices>\n    <cells size=\"%zu\">\n", nc);
+    for (size_t i = 0; i < nc; ++i)
+        std::fprintf(f, "      <tetrahedron index=\"%zu\" v0=\"%d\" v1=\"%d\" v2=\"%d\" v3=\"%d\" />\n", i,
+                     m.tets[4 * i], m.tets[4 * i + 1], m.tets[4 * i + 2], m.tets[4 * i + 3]);
+    std::fprintf(f, "    </cells>\n  </mesh>\n</dolfin>\n");
+    return std::fclose(f) == 0;
+}
+
+bool ends_with(const std::string &s, const char *suffix) {
+    const size_t n = std::strlen(suffix);
+    return s.size() >= n && s.compare(s.size() - n, n, suffix) == 0;
+}
+
 // per-cell phase: plain integers, or <value cell_index="i" ... value="m" /> entries (marker % 2)
 bool read_phase(const std::string &path, size_t nc, std::vector<int32_t> &phase) {
     std::ifstream in(path);
@@ -104,17 +217,17 @@ bool read_phase(const std::string &path, size_t nc, std::vector<int32_t> &phase)
 
 void usage() {
     std::fprintf(stderr,
-                 "usage: dmri-solve (-f|-m) mesh.xml [-M 0|1] [-phase file] [-b B | -q Q] [-d delta] [-D Delta]\n"
+                 "usage: dmri-solve (-f|-m) mesh.xml|mesh.msh [-M 0|1] [-phase file] [-b B | -q Q] [-d delta] [-D Delta]\n"
                  "                  [-k dt | -N steps] [-K D0] [-p kappa] [-T2 T2] [-gdir|-v gx gy gz]\n"
                  "                  [-variant gcloud|comri|comri-until-T] [-theta t] [-rtol r] [-atol a]\n"
-                 "                  [-maxit n] [-device d]\n");
+                 "                  [-maxit n] [-device d] [-convert out.xml]\n");
 }
 
 }  // namespace
 
 int main(int argc, char **argv) {
     // defaults of GCloudDmriSolver.py:52-55
-    std::string fmesh, fphase, variant = "gcloud";
+    std::string fmesh, fphase, fconvert, variant = "gcloud";
     int multiple = 0, nsteps = 100, maxit = 100000, device = 0;
     double b = 1000, qin = -1, kappa = 1e-5, delta = 10600, Delta = 43100, dt = 200, D0 = 3e-3, T2 = 1e16;
     double theta = 0.5, rtol = 1e-9, atol = 1e-10;
@@ -150,6 +263,7 @@ int main(int argc, char **argv) {
         else if (a == "-atol") { atol = std::atof(next()); ++i; }
         else if (a == "-maxit") { maxit = std::atoi(next()); ++i; }
         else if (a == "-device") { device = std::atoi(next()); ++i; }
+        else if (a == "-convert") { fconvert = next(); ++i; }
         else if (a == "-j" || a == "-r" || a == "-s" || a == "-IsPeriodic" || a == "-isperiodic") { ++i; }
         else if (a == "-h" || a == "--help") { usage(); return 0; }
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return 2; }
@@ -205,9 +319,21 @@ int main(int argc, char **argv) {
     }
 
     Mesh m;
-    if (!read_dolfin_xml(fmesh, m)) {
-        std::fprintf(stderr, "cannot read DOLFIN XML tetrahedral mesh %s\n", fmesh.c_str());
+    const bool gmsh = ends_with(fmesh, ".msh") || ends_with(fmesh, ".gmsh");
+    if (gmsh ? !read_gmsh(fmesh, m) : !read_dolfin_xml(fmesh, m)) {
+        std::fprintf(stderr, "cannot read %s tetrahedral mesh %s\n", gmsh ? "ASCII gmsh 2.x" : "DOLFIN XML",
+                     fmesh.c_str());
         return 1;
+    }
+    if (!fconvert.empty()) {
+        // mesh conversion only (dolfin-convert mesh.msh mesh.xml); no device is touched
+        if (!write_dolfin_xml(fconvert, m)) {
+            std::fprintf(stderr, "cannot write %s\n", fconvert.c_str());
+            return 1;
+        }
+        std::printf("converted %s: %zu vertices, %zu tetrahedra -> %s\n", fmesh.c_str(), m.xyz.size() / 3,
+                    m.tets.size() / 4, fconvert.c_str());
+        return 0;
     }
     const int64_t nv = (int64_t)m.xyz.size() / 3, nc = (int64_t)m.tets.size() / 4;
     std::vector<int32_t> phase;

@@ -41,6 +41,7 @@ __all__ = [
     "extrude_triangles",
     "cell_volumes",
     "read_dolfin_xml",
+    "read_gmsh",
     "write_dolfin_xml",
 ]
 
@@ -480,6 +481,90 @@ def read_dolfin_xml(path: str) -> TetMesh:
     for i, a, b, c, d in cx:
         tets[int(i)] = (int(a), int(b), int(c), int(d))
     return TetMesh(xyz, tets, None, {"kind": "dolfin_xml", "path": path})
+
+
+def _gmsh_text(path: str) -> str:
+    if path.endswith(".zip"):
+        import zipfile
+
+        with zipfile.ZipFile(path) as zf:
+            names = [n for n in zf.namelist() if n.endswith((".msh", ".gmsh")) and not n.startswith("__MACOSX")]
+            if len(names) != 1:
+                raise ValueError(f"{path}: expected one .msh file in the archive, found {names}")
+            return zf.read(names[0]).decode("ascii")
+    with open(path, "r") as fh:
+        return fh.read()
+
+
+def read_gmsh(path: str, dolfin_convert: bool = True) -> TetMesh:
+    """Read an ASCII gmsh 2.x tetrahedral mesh (``.msh``, or a ``.zip`` holding one) as the
+    reference's converter turns it into the DOLFIN XML mesh its solvers load
+    (comri/meshes/neuron_download/dolfin-convert.py:357-597, gmsh2xml):
+
+    * the cells are the tetrahedra (element type 4) in file order; other elements are skipped;
+    * the vertices are the nodes that some tetrahedron uses, numbered in their ``$Nodes`` order
+      (the converter's set-ordered ``vertex_dict`` is undone by its ``nodelist``, 449-453, 529, 560-565,
+      so the final numbering does not depend on the set's iteration order);
+    * each coordinate passes through the XML writer's ``"%g"`` (6 significant digits,
+      write_vertex 889-892), so the mesh equals the one DOLFIN reads back from the converted file.
+      ``dolfin_convert=False`` keeps the file's digits instead.
+
+    The converter takes a tetrahedron's nodes as ``element[3 + ntags:9]``, i.e. it needs the two
+    tags gmsh writes by default; here the four nodes after the tags are taken for any tag count.
+    ``info["physical"]`` holds each cell's first tag (the gmsh physical group; the converter
+    drops it).  Binary files and meshes without tetrahedra raise ValueError.
+    """
+    lines = _gmsh_text(path).splitlines()
+
+    def block(name):
+        try:
+            i = lines.index("$" + name)
+            j = lines.index("$End" + name, i)
+        except ValueError:
+            raise ValueError(f"{path}: no ${name} block") from None
+        return i, j
+
+    i, _ = block("MeshFormat")
+    fmt = lines[i + 1].split()
+    if len(fmt) != 3 or not fmt[0].startswith("2") or fmt[1] != "0":
+        raise ValueError(f"{path}: not an ASCII gmsh 2.x file (format line {lines[i + 1]!r})")
+    i, j = block("Nodes")
+    nn = int(lines[i + 1])
+    if j - i - 2 != nn:
+        raise ValueError(f"{path}: $Nodes declares {nn} nodes but holds {j - i - 2} lines")
+    tok = np.array(" ".join(lines[i + 2:j]).split(), dtype=object).reshape(nn, 4)
+    node_id = tok[:, 0].astype(np.int64)
+    if dolfin_convert:
+        xyz_all = np.array([[float("%g" % float(v)) for v in row] for row in tok[:, 1:]], dtype=np.float64)
+    else:
+        xyz_all = tok[:, 1:].astype(np.float64)
+    i, j = block("Elements")
+    ne = int(lines[i + 1])
+    if j - i - 2 != ne:
+        raise ValueError(f"{path}: $Elements declares {ne} elements but holds {j - i - 2} lines")
+    tets, phys = [], []
+    for line in lines[i + 2:j]:
+        e = line.split()
+        if int(e[1]) != 4:
+            continue
+        nt = int(e[2])
+        tets.append([int(v) for v in e[3 + nt:7 + nt]])
+        phys.append(int(e[3]) if nt > 0 else 0)
+    if not tets:
+        raise ValueError(f"{path}: no tetrahedra (element type 4)")
+    tets = np.asarray(tets, dtype=np.int64)
+    # used nodes, renumbered in $Nodes order
+    pos = np.full(int(node_id.max()) + 1, -1, dtype=np.int64)
+    pos[node_id] = np.arange(nn)
+    if tets.max() >= pos.size or (pos[tets] < 0).any():
+        raise ValueError(f"{path}: a tetrahedron names a node that $Nodes does not define")
+    used = np.zeros(nn, dtype=bool)
+    used[pos[tets].ravel()] = True
+    newidx = np.cumsum(used) - 1
+    xyz = np.ascontiguousarray(xyz_all[used])
+    tv = newidx[pos[tets]].astype(np.int32)
+    return TetMesh(xyz, tv, None, {"kind": "gmsh", "path": path, "physical": np.asarray(phys, dtype=np.int32),
+                                   "dolfin_convert": bool(dolfin_convert)})
 
 
 def write_dolfin_xml(mesh: TetMesh, path: str) -> None:

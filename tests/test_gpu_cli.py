@@ -66,3 +66,33 @@ def test_cli_rejects_zero_gradient(tmp_path):
     dm.write_dolfin_xml(m, str(tmp_path / "b.xml"))
     r = subprocess.run([CLI, "-f", "b.xml", "-gdir", "0", "0", "0"], capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode != 0 and "|g|=0" in r.stdout
+
+
+def test_gmsh_mesh_of_the_reference(tmp_path):
+    """The reference's undulating-axon gmsh mesh (comri/meshes/undyAxon_Ax_1E_6_vol.msh.zip, copied
+    under tests/golden/), read by the CLI as dolfin-convert would convert it, against the oracle on
+    the same mesh; the CLI's converted XML gives the same signal bit for bit."""
+    import zipfile
+
+    golden = os.path.join(ROOT, "tests", "golden", "undyAxon_Ax_1E_6_vol.msh.zip")
+    with zipfile.ZipFile(golden) as zf:
+        zf.extractall(tmp_path)
+    msh = [f for f in os.listdir(tmp_path) if f.endswith(".msh")][0]
+    args = ["-M", 0, "-b", 1000, "-d", 10600, "-D", 43100, "-k", 200, "-gdir", 1, 0, 0, "-rtol", 1e-13, "-atol", 0]
+    sig = []
+    for mesh in (msh, "axon.xml"):
+        if mesh == "axon.xml":
+            _run(["-f", msh, "-convert", "axon.xml"], tmp_path)
+        r = subprocess.run([CLI, "-f", mesh] + [str(a) for a in args], capture_output=True, text=True,
+                           cwd=tmp_path, timeout=120)
+        assert r.returncode == 0, r.stderr
+        mm = re.search(r"nv=4687 nc=22172 .* signal=([0-9.e+-]+)", r.stderr)
+        assert mm, r.stderr
+        sig.append(float(mm.group(1)))
+    assert sig[0] == sig[1]
+    m = dm.read_gmsh(golden)
+    S = O.System(m.xyz, m.tets, None, D=3e-3)
+    fA, fB = O.gcloud_sequence(O.pgse_piecewise(10600.0, 43100.0), 53700.0, 200.0)
+    ref, _, _ = S.solve((1, 0, 0), O.q_from_b_pgse(1000.0, 10600.0, 43100.0), 200.0, fA, fB, rtol=1e-13)
+    print(f"axon mesh: gpu {sig[0]:.15e} oracle {ref:.15e}")
+    assert abs(sig[0] - ref) <= 1e-9 * abs(ref)
