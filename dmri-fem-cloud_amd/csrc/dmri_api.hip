@@ -1134,11 +1134,25 @@ struct Stepper {
     }
 };
 
-void solve_group(dmri_ctx *c, int S, const double *q, const double *g, bool per_gdir, const dmri_seq *seq,
-                 const dmri_solver_opts *opts, double *sig, dmri_stats &st) {
+// Signals j0 .. j0+S-1 of nsig; snapshots (seq->snap_every > 0) go to their slots of seq->snap_out.
+void solve_group(dmri_ctx *c, int S, int j0, int nsig, const double *q, const double *g, bool per_gdir,
+                 const dmri_seq *seq, const dmri_solver_opts *opts, double *sig, dmri_stats &st) {
     Stepper run;
     run.setup(c, S, q, g, per_gdir, seq, opts, &st);
-    for (int k = 0; k < run.N; ++k) run.step(k);
+    const int every = seq->snap_every;
+    const int64_t n = c->ndof;
+    DevArray<double2> snap;
+    if (every > 0) snap.alloc((size_t)S * n);
+    for (int k = 0; k < run.N; ++k) {
+        run.step(k);
+        if (every > 0 && k % every == 0) {
+            for (int s = 0; s < S; ++s)
+                launch_unscale_perm(n, S, s, c->perm.p, c->work.sc.p, run.xc, snap.p + (size_t)s * n, c->stream);
+            double *dst = seq->snap_out + ((size_t)(k / every) * nsig + j0) * 2 * (size_t)n;
+            DMRI_HIP(hipMemcpyAsync(dst, snap.p, sizeof(double2) * (size_t)S * n, hipMemcpyDeviceToHost, c->stream));
+            DMRI_HIP(hipStreamSynchronize(c->stream));
+        }
+    }
     run.signal(sig);
 }
 
@@ -1480,6 +1494,8 @@ void solve_batched_impl(dmri_ctx *c, int nsig, const double *q, const double *gd
     require(nsig >= 1 && q && gdir && sig, DMRI_ERR_ARG, "solve: need nsig >= 1 and non-null q, gdir, sig_out");
     validate_seq(seq, opts);
     validate_u0(c, seq);
+    require(seq->snap_every >= 0, DMRI_ERR_ARG, "snap_every must be >= 0");
+    require(seq->snap_every == 0 || seq->snap_out != nullptr, DMRI_ERR_ARG, "snap_every > 0 needs snap_out");
     if (c->periodic && seq->nsteps > 0) {
         require(seq->F_explicit != nullptr, DMRI_ERR_ARG,
                 "periodic directions are set: dmri_seq.F_explicit (int_0^t f per step) is required");
@@ -1503,7 +1519,7 @@ void solve_batched_impl(dmri_ctx *c, int nsig, const double *q, const double *gd
         bool same = true;
         for (int j = 1; j < S && same; ++j)
             for (int k = 0; k < 3; ++k) same = same && g[3 * (size_t)(j0 + j) + k] == g[3 * (size_t)j0 + k];
-        solve_group(c, S, q + j0, g.data() + 3 * (size_t)j0, !same, seq, opts, sig + 2 * (size_t)j0, st);
+        solve_group(c, S, j0, nsig, q + j0, g.data() + 3 * (size_t)j0, !same, seq, opts, sig + 2 * (size_t)j0, st);
     }
     const auto t1 = std::chrono::steady_clock::now();
     st.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();

@@ -9,6 +9,9 @@
 // Extra: -phase file (per-cell phase, one integer per line, or a DOLFIN mesh_function /
 // mesh_value_collection file whose values are partition markers taken modulo 2, as
 // CreatePhaseFunc does, DmriFemLib.py:758-765), -variant gcloud|comri|comri-until-T,
+// -s 1 -j nskip: write u after every step whose 0-based counter is a multiple of nskip as legacy
+// VTK files dir/u000000.vtk, ... (comri's u.bin snapshots, main.cpp:851,886-890; dir = -o, or -f
+// when the mesh comes with -m as in comri, default "results").
 // -theta, -rtol, -atol, -maxit, -device, -convert out.xml (write the mesh as DOLFIN XML and stop,
 // as dolfin-convert mesh.msh mesh.xml does; no device is used).
 //
@@ -25,6 +28,8 @@
 #include <sstream>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "dmri.h"
 
@@ -182,6 +187,49 @@ bool write_dolfin_xml(const std::string &path, const Mesh &m) {
     return std::fclose(f) == 0;
 }
 
+// Legacy VTK unstructured grid (binary, big-endian) of the state u after one time step: the mesh
+// and, per compartment c, the point fields u<c>_re and u<c>_im (0 where c has no DOF).
+void put_be(FILE *f, const void *p, size_t bytes, size_t count) {
+    std::vector<unsigned char> b(bytes * count);
+    const unsigned char *s = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < count; ++i)
+        for (size_t k = 0; k < bytes; ++k) b[i * bytes + k] = s[i * bytes + bytes - 1 - k];
+    std::fwrite(b.data(), 1, b.size(), f);
+}
+
+bool write_vtk(const std::string &path, const Mesh &m, int ncomp, const std::vector<int32_t> &dof_of,
+               const double *u, int step) {
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    const size_t nv = m.xyz.size() / 3, nc = m.tets.size() / 4;
+    std::fprintf(f, "# vtk DataFile Version 3.0\ndmri-solve u after time step %d\nBINARY\n"
+                    "DATASET UNSTRUCTURED_GRID\nPOINTS %zu double\n", step, nv);
+    put_be(f, m.xyz.data(), 8, 3 * nv);
+    std::fprintf(f, "\nCELLS %zu %zu\n", nc, 5 * nc);
+    std::vector<int32_t> cells(5 * nc);
+    for (size_t i = 0; i < nc; ++i) {
+        cells[5 * i] = 4;
+        for (int k = 0; k < 4; ++k) cells[5 * i + 1 + k] = m.tets[4 * i + k];
+    }
+    put_be(f, cells.data(), 4, cells.size());
+    std::fprintf(f, "\nCELL_TYPES %zu\n", nc);
+    std::vector<int32_t> types(nc, 10);  // VTK_TETRA
+    put_be(f, types.data(), 4, nc);
+    std::fprintf(f, "\nPOINT_DATA %zu\n", nv);
+    std::vector<double> field(nv);
+    for (int c = 0; c < ncomp; ++c)
+        for (int part = 0; part < 2; ++part) {
+            for (size_t v = 0; v < nv; ++v) {
+                const int32_t d = dof_of[(size_t)c * nv + v];
+                field[v] = d < 0 ? 0.0 : u[2 * (size_t)d + part];
+            }
+            std::fprintf(f, "SCALARS u%d_%s double 1\nLOOKUP_TABLE default\n", c, part ? "im" : "re");
+            put_be(f, field.data(), 8, nv);
+            std::fprintf(f, "\n");
+        }
+    return std::fclose(f) == 0;
+}
+
 bool ends_with(const std::string &s, const char *suffix) {
     const size_t n = std::strlen(suffix);
     return s.size() >= n && s.compare(s.size() - n, n, suffix) == 0;
@@ -220,14 +268,15 @@ void usage() {
                  "usage: dmri-solve (-f|-m) mesh.xml|mesh.msh [-M 0|1] [-phase file] [-b B | -q Q] [-d delta] [-D Delta]\n"
                  "                  [-k dt | -N steps] [-K D0] [-p kappa] [-T2 T2] [-gdir|-v gx gy gz]\n"
                  "                  [-variant gcloud|comri|comri-until-T] [-theta t] [-rtol r] [-atol a]\n"
-                 "                  [-maxit n] [-device d] [-convert out.xml]\n");
+                 "                  [-maxit n] [-device d] [-convert out.xml] [-s 0|1 [-j nskip] [-o dir]]\n");
 }
 
 }  // namespace
 
 int main(int argc, char **argv) {
     // defaults of GCloudDmriSolver.py:52-55
-    std::string fmesh, fphase, fconvert, variant = "gcloud";
+    std::string fmesh, fmesh_m, farg, fphase, fconvert, dir = "results", variant = "gcloud";
+    int nskip = 5, is_save = 0;
     int multiple = 0, nsteps = 100, maxit = 100000, device = 0;
     double b = 1000, qin = -1, kappa = 1e-5, delta = 10600, Delta = 43100, dt = 200, D0 = 3e-3, T2 = 1e16;
     double theta = 0.5, rtol = 1e-9, atol = 1e-10;
@@ -242,7 +291,11 @@ int main(int argc, char **argv) {
             }
             return argv[i + k];
         };
-        if (a == "-f" || a == "-m") { fmesh = next(); ++i; }
+        if (a == "-f") { farg = next(); ++i; }
+        else if (a == "-m") { fmesh_m = next(); ++i; }
+        else if (a == "-j") { nskip = std::atoi(next()); ++i; }
+        else if (a == "-s") { is_save = std::atoi(next()); ++i; }
+        else if (a == "-o") { dir = next(); ++i; }
         else if (a == "-M") { multiple = std::atoi(next()); ++i; }
         else if (a == "-phase" || a == "-c") { fphase = next(); ++i; multiple = 1; }
         else if (a == "-b") { b = std::atof(next()); have_b = true; ++i; }
@@ -264,11 +317,22 @@ int main(int argc, char **argv) {
         else if (a == "-maxit") { maxit = std::atoi(next()); ++i; }
         else if (a == "-device") { device = std::atoi(next()); ++i; }
         else if (a == "-convert") { fconvert = next(); ++i; }
-        else if (a == "-j" || a == "-r" || a == "-s" || a == "-IsPeriodic" || a == "-isperiodic") { ++i; }
+        else if (a == "-r" || a == "-IsPeriodic" || a == "-isperiodic") { ++i; }
         else if (a == "-h" || a == "--help") { usage(); return 0; }
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return 2; }
     }
+    // comri: -m mesh -f output directory (main.cpp:599-615); GCloud: -f mesh
+    if (!fmesh_m.empty()) {
+        fmesh = fmesh_m;
+        if (!farg.empty()) dir = farg;
+    } else {
+        fmesh = farg;
+    }
     if (fmesh.empty()) { usage(); return 2; }
+    if (is_save && nskip <= 0) {
+        std::fprintf(stderr, "-j nskip must be positive\n");
+        return 2;
+    }
     const bool comri = variant != "gcloud";
     if (variant != "gcloud" && variant != "comri" && variant != "comri-until-T") {
         std::fprintf(stderr, "unknown variant %s\n", variant.c_str());
@@ -370,6 +434,15 @@ int main(int argc, char **argv) {
     seq.gdir[0] = g[0];
     seq.gdir[1] = g[1];
     seq.gdir[2] = g[2];
+    dmri_info info0{};
+    check(dmri_get_info(ctx, &info0), "dmri_get_info");
+    std::vector<double> snaps;
+    if (is_save) {
+        const size_t nsnap = ((size_t)seq.nsteps + nskip - 1) / nskip;
+        snaps.assign(nsnap * 2 * (size_t)info0.ndof, 0.0);
+        seq.snap_every = nskip;
+        seq.snap_out = snaps.data();
+    }
     dmri_solver_opts op{};
     op.rtol = rtol;
     op.atol = atol;
@@ -377,6 +450,27 @@ int main(int argc, char **argv) {
     double sig[2];
     dmri_stats st{};
     check(dmri_solve(ctx, &seq, &op, sig, &st), "dmri_solve");
+    if (is_save) {
+        // u after every step with step_counter % nskip == 0, numbered by write as File does
+        // (main.cpp:851,886-890): dir/u000000.vtk, dir/u000001.vtk, ...
+        ::mkdir(dir.c_str(), 0755);
+        const int ncomp = (int)info0.ncomp;
+        std::vector<int32_t> dof_of((size_t)ncomp * nv);
+        std::vector<int64_t> rp(info0.ndof + 1);
+        std::vector<int32_t> cl(info0.nnz);
+        check(dmri_export_pattern(ctx, rp.data(), cl.data(), dof_of.data()), "dmri_export_pattern");
+        const size_t nsnap = snaps.size() / (2 * (size_t)info0.ndof);
+        for (size_t j = 0; j < nsnap; ++j) {
+            char name[32];
+            std::snprintf(name, sizeof(name), "/u%06zu.vtk", j);
+            if (!write_vtk(dir + name, m, ncomp, dof_of, snaps.data() + j * 2 * (size_t)info0.ndof,
+                           (int)(j * nskip))) {
+                std::fprintf(stderr, "cannot write %s%s\n", dir.c_str(), name);
+                dmri_destroy(ctx);
+                return 1;
+            }
+        }
+    }
     dmri_info info{};
     check(dmri_get_info(ctx, &info), "dmri_get_info");
     const double whole = info.volume, elapsed = st.solve_ms * 1e-3;

@@ -41,7 +41,8 @@ class _Seq(ctypes.Structure):
     _fields_ = [("nsteps", ctypes.c_int32), ("f_implicit", ctypes.POINTER(ctypes.c_double)),
                 ("f_explicit", ctypes.POINTER(ctypes.c_double)), ("dt", ctypes.c_double),
                 ("theta", ctypes.c_double), ("q", ctypes.c_double), ("gdir", ctypes.c_double * 3),
-                ("F_explicit", ctypes.POINTER(ctypes.c_double)), ("u0_vertex", ctypes.POINTER(ctypes.c_double))]
+                ("F_explicit", ctypes.POINTER(ctypes.c_double)), ("u0_vertex", ctypes.POINTER(ctypes.c_double)),
+                ("snap_every", ctypes.c_int32), ("snap_out", ctypes.POINTER(ctypes.c_double))]
 
 
 class _Opts(ctypes.Structure):
@@ -251,34 +252,59 @@ class Context:
     def _nv(self, u0):
         return None if u0 is None else int(self.info()["nv"])
 
+    def _snapshots(self, s, snap_every, nsig):
+        """Attach a snapshot buffer to the sequence: (nsnap, nsig, ndof) complex, or None."""
+        snap_every = int(snap_every)
+        if snap_every < 0:
+            raise ValueError("snap_every must be >= 0")
+        if snap_every == 0:
+            return None
+        nsnap = (s.nsteps + snap_every - 1) // snap_every
+        buf = np.zeros((nsnap, nsig, int(self.info()["ndof"])), dtype=np.complex128)
+        s.snap_every = snap_every
+        s.snap_out = _ptr(buf.view(np.float64), ctypes.c_double)
+        return buf
+
     def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None):
+              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None, snap_every=0):
         """precond: 0 = Jacobi + degree-1 polynomial (default), 1 = Jacobi only.  guess: Krylov start
         of each step, 0 = residual-minimising projection on the last states (default), 1 = previous
         state, 2 = linear, 3 = quadratic extrapolation.  u0: initial state per mesh vertex (real part
-        of every compartment's DOF there; None = 1)."""
+        of every compartment's DOF there; None = 1).  snap_every > 0: stats["snapshots"] holds u
+        (complex, DOF order) after every step whose 0-based index is a multiple of it, shape
+        (nsnap, ndof)."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes,
                                     guess, mapping, F_explicit, precond, u0, self._nv(u0))
+        snaps = self._snapshots(s, snap_every, 1)
         sig = np.zeros(2)
         st = _Stats()
         _check(LIB.dmri_solve(self._h, ctypes.byref(s), ctypes.byref(o), _ptr(sig, ctypes.c_double),
                               ctypes.byref(st)))
-        return complex(sig[0], sig[1]), {k: getattr(st, k) for k, _ in _Stats._fields_}
+        stats = {k: getattr(st, k) for k, _ in _Stats._fields_}
+        if snaps is not None:
+            stats["snapshots"] = snaps[:, 0, :]
+        return complex(sig[0], sig[1]), stats
 
     def solve_batched(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-                      profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None):
-        """Signals for every (q[s], gdir[s]) on the shared pattern; returns complex array (nsig,)."""
+                      profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None,
+                      snap_every=0):
+        """Signals for every (q[s], gdir[s]) on the shared pattern; returns complex array (nsig,).
+        snap_every > 0: stats["snapshots"] has shape (nsnap, nsig, ndof), as in solve."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(gdirs, dtype=np.float64).reshape(-1, 3),
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
                                     lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0))
+        snaps = self._snapshots(s, snap_every, qs.shape[0])
         sig = np.zeros(2 * qs.shape[0])
         st = _Stats()
         _check(LIB.dmri_solve_batched(self._h, qs.shape[0], _ptr(qs, ctypes.c_double), _ptr(g, ctypes.c_double),
                                       ctypes.byref(s), ctypes.byref(o), _ptr(sig, ctypes.c_double),
                                       ctypes.byref(st)))
-        return sig.view(np.complex128), {k: getattr(st, k) for k, _ in _Stats._fields_}
+        stats = {k: getattr(st, k) for k, _ in _Stats._fields_}
+        if snaps is not None:
+            stats["snapshots"] = snaps
+        return sig.view(np.complex128), stats
 
     MAPPINGS = {1: "spread", 2: "spread-staged", 3: "registers", 4: "wave-per-row", 5: "tile-compressed"}
 

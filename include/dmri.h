@@ -154,6 +154,16 @@ typedef struct {
                                  everywhere.  The signal stays sum_i w_i Re u_i / |Omega|: a caller
                                  with a custom state divides by int u0 / |Omega| (PostProcessing's
                                  vol_of_interest, DmriFemLib.py:924) */
+    int32_t snap_every;       /* > 0: copy the state to snap_out after every step whose 0-based index
+                                 is a multiple of snap_every, as comri's time loop writes u after the
+                                 solve of every step with step_counter % nskip == 0 (-j nskip, -s 1;
+                                 comri/two-comp/hpc-fenics-cpp/main.cpp:886-890); 0 = no snapshots.
+                                 dmri_solve and dmri_solve_batched only (dmri_adjoint ignores it). */
+    double *snap_out;         /* with snap_every > 0: ceil(nsteps / snap_every) * nsig * 2 * ndof
+                                 doubles; snapshot j of signal s starts at (j * nsig + s) * 2 * ndof
+                                 and holds u in dmri_get_solution's layout (interleaved re/im in DOF
+                                 order; nsig = 1 for dmri_solve).  A device-to-host copy (and a wait
+                                 for it) per snapshot, so only as often as needed */
 } dmri_seq;
 
 typedef struct {

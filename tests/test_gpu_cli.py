@@ -96,3 +96,74 @@ def test_gmsh_mesh_of_the_reference(tmp_path):
     ref, _, _ = S.solve((1, 0, 0), O.q_from_b_pgse(1000.0, 10600.0, 43100.0), 200.0, fA, fB, rtol=1e-13)
     print(f"axon mesh: gpu {sig[0]:.15e} oracle {ref:.15e}")
     assert abs(sig[0] - ref) <= 1e-9 * abs(ref)
+
+
+def _read_vtk(path):
+    """Legacy binary VTK unstructured grid: points, tets and the point fields (big-endian)."""
+    data = open(path, "rb").read()
+    pos = 0
+
+    def line():
+        nonlocal pos
+        e = data.index(b"\n", pos)
+        s = data[pos:e].decode()
+        pos = e + 1
+        return s
+
+    def arr(dt, count):
+        nonlocal pos
+        a = np.frombuffer(data, dtype=dt, count=count, offset=pos)
+        pos += a.nbytes
+        return a
+
+    assert line().startswith("# vtk DataFile")
+    title = line()
+    assert line() == "BINARY" and line() == "DATASET UNSTRUCTURED_GRID"
+    nv = int(line().split()[1])
+    xyz = arr(">f8", 3 * nv).reshape(nv, 3)
+    line()
+    nc = int(line().split()[1])
+    cells = arr(">i4", 5 * nc).reshape(nc, 5)
+    line()
+    line()
+    types = arr(">i4", nc)
+    line()
+    assert line() == f"POINT_DATA {nv}"
+    fields = {}
+    while pos < len(data):
+        hdr = line().split()
+        assert hdr[0] == "SCALARS" and line() == "LOOKUP_TABLE default"
+        fields[hdr[1]] = arr(">f8", nv).copy()
+        line()
+    return title, xyz, cells, types, fields
+
+
+def test_comri_snapshots_written_as_vtk(tmp_path):
+    """comri -s 1 -j nskip -f dir: u after every step whose counter is a multiple of nskip
+    (main.cpp:851,886-890), one VTK file per write, against the oracle's states."""
+    m = dm.cylinder_mesh(8.0, 4.0, 3000, seed=11, interface_radii=(4.0,))
+    dm.write_dolfin_xml(m, str(tmp_path / "cyl.xml"))
+    (tmp_path / "phase.txt").write_text("\n".join(str(int(p)) for p in m.phase) + "\n")
+    _run(["-m", "cyl.xml", "-c", "phase.txt", "-variant", "comri", "-b", 2000, "-d", 10600, "-D", 43100,
+          "-k", 2000, "-K", 3e-3, "-p", 1e-5, "-v", 0, 1, 0, "-rtol", 1e-13, "-atol", 0, "-s", 1, "-j", 5,
+          "-f", "out"], tmp_path)
+    fA, fB = O.comri_sequence(O.pgse_comri(10600.0, 43100.0), 53700.0, 2000.0)
+    files = sorted(os.listdir(tmp_path / "out"))
+    assert files == [f"u{j:06d}.vtk" for j in range((len(fA) + 4) // 5)]
+    S = O.System(m.xyz, m.tets, m.phase, D=3e-3, kappa=1e-5)
+    q = O.q_from_b_pgse(2000.0, 10600.0, 43100.0)
+    nv = m.nv
+    for j in (0, 2, len(files) - 1):
+        title, xyz, cells, types, fields = _read_vtk(str(tmp_path / "out" / files[j]))
+        assert title.endswith(f"step {5 * j}")
+        assert np.array_equal(xyz, m.xyz) and np.array_equal(cells[:, 1:], m.tets)
+        assert (cells[:, 0] == 4).all() and (types == 10).all()
+        _, u_ref, _ = S.solve((0, 1, 0), q, 2000.0, fA[:5 * j + 1], fB[:5 * j + 1], rtol=1e-13)
+        ref = np.zeros((2, nv), dtype=np.complex128)
+        for c in range(2):
+            d = S.dof_of[c * nv:(c + 1) * nv]
+            ref[c, d >= 0] = u_ref[d[d >= 0]]
+        got = np.stack([fields[f"u{c}_re"] + 1j * fields[f"u{c}_im"] for c in range(2)])
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        print(f"  snapshot {j}: rel err {err:.2e}", flush=True)
+        assert err <= 1e-10

@@ -413,3 +413,48 @@ def test_dmrifemlib_initial_condition_normalises_by_its_integral(tmp_path, monke
     voi = float((w * u0.real).sum())
     assert abs(voi_got - voi) <= 1e-3 * voi
     assert abs(got - ref * w.sum() / voi) <= 2e-6 * abs(got)
+
+
+def test_snapshots_match_oracle_states():
+    """u after the steps whose 0-based index is a multiple of snap_every (comri's -s 1 -j nskip,
+    comri/two-comp/hpc-fenics-cpp/main.cpp:886-890), in DOF order, against the oracle's state after
+    the same number of steps; the last one equals dmri_get_solution when it is the final step."""
+    mesh, kw = MESHES["cyl2"]
+    delta, Delta, k = 10600.0, 43100.0, 200.0
+    q = O.q_from_b_pgse(2000.0, delta, Delta)
+    fA, fB = _pgse_gcloud(delta, Delta, k)
+    fA, fB = fA[:61], fB[:61]  # steps through the first gradient pulse and into the gap
+    every = 15
+    print(f"[snapshots] cyl2 steps={len(fA)} every={every}", flush=True)
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    s_gpu, st = ctx.solve(fA, fB, k, q, (0, 1, 0), rtol=1e-13, snap_every=every)
+    snaps = st["snapshots"]
+    assert snaps.shape == (5, ctx.info()["ndof"])
+    S = O.System(mesh.xyz, mesh.tets, mesh.phase, D=kw["D"], kappa=kw["kappa"])
+    for j in range(snaps.shape[0]):
+        m = j * every + 1
+        _, u_ref, _ = S.solve((0, 1, 0), q, k, fA[:m], fB[:m], rtol=1e-13)
+        err = np.linalg.norm(snaps[j] - u_ref) / np.linalg.norm(u_ref)
+        print(f"  after {m} steps: rel err {err:.2e}", flush=True)
+        assert err <= 1e-10
+    assert np.array_equal(snaps[-1], ctx.solution(0))
+
+
+def test_batched_snapshots_land_in_their_signal_slots():
+    """70 signals (two launch groups): snapshot j of signal s sits at [j, s] and equals the plain
+    solve's snapshot of that signal to the solver tolerance."""
+    mesh, kw = MESHES["box4"]
+    delta, Delta, k = 1000.0, 10000.0, 200.0
+    fA, fB = O.gcloud_sequence(O.pgse_piecewise(delta, Delta), delta + Delta, k)
+    rng = np.random.default_rng(9)
+    nsig = 70
+    gd = rng.normal(size=(nsig, 3))
+    qs = np.sqrt(rng.uniform(0, 3000, nsig)) / np.sqrt(delta * delta * (Delta - delta / 3))
+    ctx = _gpu_system(mesh, 2e-3, 0.0)
+    sig, st = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13, snap_every=20)
+    snaps = st["snapshots"]
+    assert snaps.shape == (3, nsig, ctx.info()["ndof"])
+    for j in (0, 63, 64, 69):
+        s1, st1 = ctx.solve(fA, fB, k, qs[j], gd[j], rtol=1e-13, snap_every=20)
+        err = np.abs(snaps[:, j] - st1["snapshots"]).max() / np.abs(st1["snapshots"]).max()
+        assert err <= 1e-11, (j, err)
