@@ -458,3 +458,13 @@ def test_batched_snapshots_land_in_their_signal_slots():
         s1, st1 = ctx.solve(fA, fB, k, qs[j], gd[j], rtol=1e-13, snap_every=20)
         err = np.abs(snaps[:, j] - st1["snapshots"]).max() / np.abs(st1["snapshots"]).max()
         assert err <= 1e-11, (j, err)
+
+
+def test_copy_bandwidth_reference_runs():
+    """The copy kernel behind bench.py's measured-bandwidth reference (one element per thread over a
+    covering grid) runs at sizes that are and are not a multiple of the block size."""
+    mesh, kw = MESHES["box4"]
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    for nbytes in (1 << 26, (1 << 26) + 16 * 77):
+        gbs = ctx.copy_bandwidth(nbytes, 3)
+        assert 100.0 < gbs < 50000.0, gbs
