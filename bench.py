@@ -270,7 +270,7 @@ def run_forward(args, name):
         for key in merged:
             if key in ("iters_max", "relres_max"):
                 merged[key] = max(s_[key] for s_ in stats)
-            elif key != "spmv_bytes":
+            elif key not in ("spmv_bytes", "precond_used", "reserved"):
                 merged[key] = sum(s_[key] for s_ in stats)
         return np.concatenate(sigs), merged
 
@@ -340,6 +340,8 @@ def run_forward(args, name):
         "config": cfg,
         "dof_steps_per_s": float(info["ndof"]) * len(fA) * njob / elapsed,
         "krylov_iters_per_theta_step": iters / (len(fA) * njob),
+        "preconditioner_used": {0: "jacobi", 1: "polynomial", 2: "polynomial of the real part",
+                                3: "polynomial, complex diagonal"}.get(all_stats[-1].get("precond_used")),
         "setup_ms": info["setup_ms"],
         "mesh_gen_s": w["mesh_gen_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -437,7 +439,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rtol", type=float, default=1e-12)
     ap.add_argument("--precond", type=int, default=0,
-                    help="0 = Jacobi + degree-1 polynomial (library default), 1 = Jacobi only")
+                    help="0 = Jacobi + degree-1 polynomial, chosen by the library (default), 1 = Jacobi only, "
+                         "2 = the polynomial scaled by the complex diagonal")
     ap.add_argument("--group", type=int, default=32,
                     help="signals per launch group (1..64); 32 was the fastest for the 512- and 128-signal "
                          "workloads (64: 2-4 %% slower, 16: 5-8 %% slower)")

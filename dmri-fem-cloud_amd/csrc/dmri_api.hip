@@ -33,7 +33,7 @@ constexpr int kEventPool = 512;
 struct SolverWork {
     DevArray<double2> x, r, rh, p, v, t, b, xprev, xprev2, ybc, bper, ph, sh;
     DevArray<double2> yh[kMaxHist], hp[kMaxHist], hq[kMaxHist];  // projection history
-    DevArray<double> sc, vals, bre, areal, part, tot, coef, coefP, sig, spart, ppart, ptot, pgam, gsig, u0v;
+    DevArray<double> sc, vals, bre, areal, pdiag, part, tot, coef, coefP, sig, spart, ppart, ptot, pgam, gsig, u0v;
     DevArray<KrylovState> st;
     DevArray<int> all_done, mask;
     KrylovState *h_st = nullptr, *d_h_st = nullptr;  // pinned host copy and its device address
@@ -673,7 +673,7 @@ struct Stepper {
             if (opts->mapping > 0) mapping = opts->mapping;
             if (opts->lanes_per_row > 0) lanes = opts->lanes_per_row;
         }
-        poly = opts->preconditioner == 0;
+        poly = opts->preconditioner == 0 || opts->preconditioner == 2;
         if (poly && (mapping == kMapSR || mapping == kMapTile)) {  // kernels without the polynomial ops
             mapping = kMapStaged;
             while (lanes > 1 && lanes * S > 64) lanes >>= 1;
@@ -790,11 +790,34 @@ struct Stepper {
             double fmax = 0.0;
             for (int k = 0; k < N; ++k) fmax = std::max(fmax, std::fabs(seq->f_implicit[k]));
             const double imag = theta * qf * fmax * gs * jb;
-            real_pre = imag <= kRealPreconditioner * (1.8 / a.pc);
+            const double breal = 1.8 / a.pc;
+            real_pre = opts->preconditioner == 0 && imag <= kRealPreconditioner * breal;
+            st->precond_used = 1;
             if (real_pre) {
                 if (w.areal.n != (size_t)c->nnz) w.areal.alloc(c->nnz);
                 launch_extract_real(c->nnz, w.vals.p, vw, w.areal.p, s);
                 a.areal = w.areal.p;
+                st->precond_used = 2;
+            } else if (opts->preconditioner == 2 || imag >= kComplexDiagonal * breal) {
+                // The gradient term dominates the diagonal of rows far along g: the polynomial then
+                // acts on the operator scaled by its complex diagonal, whose spectrum stays near
+                // the real part's; c = 1.8 / (a bound of that spectrum over every step and signal).
+                std::vector<double> gc((size_t)4 * S);
+                for (int j = 0; j < S; ++j) {
+                    for (int k = 0; k < 3; ++k) gc[4 * (size_t)j + k] = per_gdir ? g[3 * j + k] : (k == 0 ? 1.0 : 0.0);
+                    gc[4 * (size_t)j + 3] = theta * std::fabs(q[j]) * fmax;
+                }
+                DevArray<double> gcd, part, out;
+                gcd.alloc(gc.size());
+                part.alloc(kMaxGrid);
+                out.alloc(1);
+                DMRI_HIP(hipMemcpyAsync(gcd.p, gc.data(), sizeof(double) * gc.size(), hipMemcpyHostToDevice, s));
+                if (w.pdiag.n != (size_t)n * vw) w.pdiag.alloc((size_t)n * vw);
+                launch_cdiag_prep(n, c->srow.p, c->scol.p, w.vals.p, vw, S, gcd.p, breal, w.pdiag.p, part.p, out.p, s);
+                const double hb = read_scalar(out.p, s);
+                a.pc = 1.8 / std::max(breal, hb);
+                a.pdiag = w.pdiag.p;
+                st->precond_used = 3;
             }
         }
         a.rtol = opts->rtol;
@@ -1477,6 +1500,7 @@ void validate_seq(const dmri_seq *seq, const dmri_solver_opts *opts) {
             "need rtol > 0 or atol > 0");
     require(opts->lanes_per_row >= 0, DMRI_ERR_ARG, "lanes_per_row must be >= 0");
     require(opts->initial_guess >= 0 && opts->initial_guess <= 3, DMRI_ERR_ARG, "initial_guess must lie in [0, 3]");
+    require(opts->preconditioner >= 0 && opts->preconditioner <= 2, DMRI_ERR_ARG, "preconditioner must lie in [0, 2]");
     for (int i = 0; i < seq->nsteps; ++i)
         require(std::isfinite(seq->f_implicit[i]) && std::isfinite(seq->f_explicit[i]), DMRI_ERR_ARG,
                 "time profile has a non-finite value");

@@ -92,6 +92,10 @@ constexpr int kOpV = 0, kOpT = 0, kOpPre = 1, kOpTP = 2;
 // the preconditioner polynomial takes the real part alone when the imaginary part's bound is at
 // most this fraction of the real part's (measured: 0.002 -> same iterations, 3.3 -> twice as many)
 constexpr double kRealPreconditioner = 0.05;
+// the polynomial acts on the operator scaled by its complex diagonal when that bound of the gradient
+// term reaches this fraction of the real part's (below it the iterations it saves do not pay for the
+// extra work of its SpMVs)
+constexpr double kComplexDiagonal = 1.0;
 constexpr int kTileMaxEntries = 4096;  // entries of a tile the compression build sorts in LDS
 constexpr int kTileLdsBytes = 32768;   // LDS cap of the staged x values of one tile (distinct columns x S)
 constexpr int kMaxSRSignals = 8;
@@ -234,6 +238,13 @@ void launch_init_x_vertex(int64_t n, int S, const double *sc, const int *perm, c
                           double2 *x, hipStream_t s);
 // Gershgorin bound max_i sum_j |a_ij| of the real implicit part of the scaled operator (the
 // first value of every vals entry, vw values per entry) into out[0]; part >= kMaxGrid doubles
+// Diagonal values of the scaled operator per row into pdiag[vw*i ..] and, into out, a bound of the
+// spectrum of the operator scaled by its complex diagonal over every signal and step:
+// max over rows i and signals s of max over 0 <= c <= cmax_s of (min(R_i, breal) + c Q_is) / |a_ii + i c j_is|,
+// with R_i = sum_j |a_ij|, Q_is = sum_k |g_sk| sum_j |jk_ij|, j_is = |g_s . j_ii| (one direction: the
+// stored j, g = 1).  gc = [S][4]: g_s and cmax_s.
+void launch_cdiag_prep(int64_t n, const int *rowptr, const int *col, const double *vals, int vw, int S,
+                       const double *gc, double breal, double *pdiag, double *part, double *out, hipStream_t s);
 void launch_gershgorin(int64_t n, const int *rowptr, const double *vals, int vw, double *part, double *out,
                        hipStream_t s);
 // y = Re(A) x for the real implicit part (vw values per entry), and xy = x.*y, yy = y.*y
@@ -262,6 +273,9 @@ struct SolveArgs {
     double2 *ph, *sh;               // preconditioned p and s (polynomial preconditioner), or null
     double pc;                      // polynomial scale c: z -> c (2 z - c Are z)
     const double *areal;            // [nnz] the scaled real part alone (preconditioner SpMVs)
+    const double *pdiag;            // [n*vw] diagonal values of the scaled operator per row (a, j or
+                                    // a, jx, jy, jz): the polynomial preconditioner in the variables
+                                    // scaled by the complex diagonal a_ii + i (c_s . j_ii); null = off
     double2 *hp, *hq;               // mode 4 writes Are x and Jg x here (projection history)
     const double *gsig;             // [S*3] unit gradient directions (per-direction values only)
     double *partU, *partV, *partT;  // [grid][S][kNV] each

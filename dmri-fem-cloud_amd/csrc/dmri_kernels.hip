@@ -185,6 +185,51 @@ __global__ void __launch_bounds__(kThreads) row_absmax_kernel(int64_t n, const i
     }
     if (threadIdx.x == 0) parts[blockIdx.x] = sm[0];
 }
+// Diagonal values per row and the bound of the complex-diagonal-scaled operator (launch_cdiag_prep).
+// h(c) = (R + c Q) / sqrt(a^2 + c^2 j^2) rises up to c* = Q a^2 / (R j^2) and falls after it, so its
+// largest value on [0, cmax] is at min(c*, cmax).
+__global__ void __launch_bounds__(kThreads) cdiag_prep_kernel(int64_t n, const int *rowptr, const int *col,
+                                                              const double *vals, int vw, int S, const double *gc,
+                                                              double breal, double *pdiag, double *parts) {
+    __shared__ double sm[kThreads];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double R = 0.0, Q[3] = {0.0, 0.0, 0.0}, dg[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+            const double *v = vals + (int64_t)vw * e;
+            R += fabs(v[0]);
+            for (int k = 1; k < vw; ++k) Q[k - 1] += fabs(v[k]);
+            if (col[e] == (int)i)
+                for (int k = 0; k < vw; ++k) dg[k] = v[k];
+        }
+        for (int k = 0; k < vw; ++k) pdiag[(int64_t)vw * i + k] = dg[k];
+        const double a = dg[0], Rb = fmin(R, breal);
+        for (int s = 0; s < S; ++s) {
+            const double *g = gc + 4 * s;
+            double q, j;
+            if (vw == 4) {
+                q = fabs(g[0]) * Q[0] + fabs(g[1]) * Q[1] + fabs(g[2]) * Q[2];
+                j = fabs(g[0] * dg[1] + g[1] * dg[2] + g[2] * dg[3]);
+            } else {
+                q = Q[0];
+                j = fabs(dg[1]);
+            }
+            const double cmax = g[3];
+            double c = cmax;
+            if (Rb > 0.0 && j > 0.0) c = fmin(cmax, q * a * a / (Rb * j * j));
+            const double h = (Rb + c * q) / sqrt(a * a + c * c * j * j);
+            const double h1 = (Rb + cmax * q) / sqrt(a * a + cmax * cmax * j * j);
+            m = fmax(m, fmax(fmax(h, h1), Rb / fabs(a)));
+        }
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x] = sm[0];
+}
 __global__ void __launch_bounds__(kThreads) max_final_kernel(const double *parts, int np, double *out) {
     __shared__ double sm[kThreads];
     double m = 0.0;
@@ -1184,6 +1229,25 @@ __device__ __forceinline__ double2 imag_coef_apply(const Acc<PG> &acc, const dou
     return make_double2(acc.a.x - cj.y, acc.a.y + cj.x);
 }
 
+// Inverse of the complex diagonal d = a_ii + i (c_s . j_ii) of the scaled operator at row i for
+// signal s: conj(d) times the hardware reciprocal of |d|^2 (an approximate inverse, but the same
+// one at every application, so the preconditioner stays one fixed linear operator).
+template <bool PG>
+__device__ __forceinline__ double2 cdiag_inv(const double *pdiag, const double *coef, int64_t i, int s) {
+    double re, im;
+    if (PG) {
+        const double4 d = *reinterpret_cast<const double4 *>(pdiag + 4 * i);
+        re = d.x;
+        im = coef[3 * s] * d.y + coef[3 * s + 1] * d.z + coef[3 * s + 2] * d.w;
+    } else {
+        const double2 d = *reinterpret_cast<const double2 *>(pdiag + 2 * i);
+        re = d.x;
+        im = coef[3 * s] * d.y;
+    }
+    const double r = __builtin_amdgcn_rcp(re * re + im * im);
+    return make_double2(re * r, -im * r);
+}
+
 template <int G>
 __device__ __forceinline__ void group_reduce(double2 &v) {
 #pragma unroll
@@ -1767,14 +1831,20 @@ __global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, in
     const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     const double2 *p = a.p;
-    auto gather = [&](int c) { return p[(int64_t)c * S_ + s]; };
+    // with the complex diagonal d: ph = c D^-1 (2 p - c A D^-1 p), A D^-1 p gathering p / d
+    const bool cd = OP == kOpPre && a.pdiag != nullptr;
+    auto gather = [&](int c) {
+        const double2 z = p[(int64_t)c * S_ + s];
+        return cd ? c_mul(z, cdiag_inv<PG>(a.pdiag, a.coefA, c, s)) : z;
+    };
     auto emit = [&](int64_t row, const Acc<PG> &acc) {
         if (L.g != 0) return;
         const int64_t idx = row * S_ + s;
         const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
         if (OP == kOpPre) {
             const double2 z = p[idx];
-            a.ph[idx] = make_double2(a.pc * (2.0 * z.x - a.pc * y.x), a.pc * (2.0 * z.y - a.pc * y.y));
+            const double2 w = make_double2(a.pc * (2.0 * z.x - a.pc * y.x), a.pc * (2.0 * z.y - a.pc * y.y));
+            a.ph[idx] = cd ? c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s)) : w;
             return;
         }
         a.v[idx] = y;
@@ -1866,10 +1936,12 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
     const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     const double2 *r = a.r, *v = a.v, *sh = a.sh;
+    const bool cd = OP == kOpPre && a.pdiag != nullptr;  // complex diagonal, as in spmv_v_kernel
     auto gather = [&](int c) {
         const int64_t j = (int64_t)c * S_ + s;
         if (OP == kOpTP) return sh[j];
-        return c_sub(r[j], c_mul(alpha, v[j]));
+        const double2 z = c_sub(r[j], c_mul(alpha, v[j]));
+        return cd ? c_mul(z, cdiag_inv<PG>(a.pdiag, a.coefA, c, s)) : z;
     };
     auto emit = [&](int64_t row, const Acc<PG> &acc) {
         if (L.g != 0) return;
@@ -1877,7 +1949,8 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
         const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
         const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
         if (OP == kOpPre) {
-            a.sh[idx] = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
+            const double2 w = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
+            a.sh[idx] = cd ? c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s)) : w;
             return;
         }
         a.t[idx] = y;
@@ -3045,6 +3118,17 @@ void launch_gershgorin(int64_t n, const int *rowptr, const double *vals, int vw,
     if (g > (unsigned)kMaxGrid) g = kMaxGrid;
     if (g == 0) g = 1;
     row_absmax_kernel<<<g, kThreads, 0, s>>>(n, rowptr, vals, vw, part);
+    check_launch();
+    max_final_kernel<<<1, kThreads, 0, s>>>(part, (int)g, out);
+    check_launch();
+}
+void launch_cdiag_prep(int64_t n, const int *rowptr, const int *col, const double *vals, int vw, int S,
+                       const double *gc, double breal, double *pdiag, double *part, double *out, hipStream_t s) {
+    if (vw != 2 && vw != 4) throw Error(-1, "cdiag_prep: 2 or 4 values per entry");
+    unsigned g = grid_for(n, kThreads);
+    if (g > (unsigned)kMaxGrid) g = kMaxGrid;
+    if (g == 0) g = 1;
+    cdiag_prep_kernel<<<g, kThreads, 0, s>>>(n, rowptr, col, vals, vw, S, gc, breal, pdiag, part);
     check_launch();
     max_final_kernel<<<1, kThreads, 0, s>>>(part, (int)g, out);
     check_launch();

@@ -54,7 +54,8 @@ class _Opts(ctypes.Structure):
 class _Stats(ctypes.Structure):
     _fields_ = [("iters_total", ctypes.c_int64), ("iters_max", ctypes.c_int32), ("restarts", ctypes.c_int32),
                 ("relres_max", ctypes.c_double), ("solve_ms", ctypes.c_double), ("spmv_ms", ctypes.c_double),
-                ("spmv_launches", ctypes.c_int64), ("spmv_bytes", ctypes.c_double)]
+                ("spmv_launches", ctypes.c_int64), ("spmv_bytes", ctypes.c_double),
+                ("precond_used", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class _AdjOpts(ctypes.Structure):

@@ -191,17 +191,23 @@ typedef struct {
                               5 = mapping 1 over tiles whose distinct columns are staged once
                               in LDS (lanes * signals <= 64) */
     int32_t preconditioner; /* 0 = Jacobi (the symmetric diagonal scaling) with the degree-1
-                               polynomial c (2I - c A) of the scaled system (of its real part alone
-                               when the gradient term's bound is below 5 % of the real part's),
-                               c = 1.8 / b where b
-                               bounds the largest eigenvalue of its real part (1.05 x a 30-step
-                               power iteration estimate, capped by the Gershgorin bound),
-                               applied from the right (the residual the
-                               tolerance tests stays the true one; the default), 1 = Jacobi only,
-                               as the reference's
-                               KrylovSolver("bicgstab", "jacobi").  Both converge to the same
-                               solution; the polynomial needs SpMV mapping 1, 2 or 4 (others fall
-                               back to mapping 2). */
+                               polynomial c (2I - c A) of the scaled system, applied from the right
+                               (the residual the tolerance tests stays the true one; the default):
+                               - of its real part alone when the gradient term's bound is below 5 %
+                                 of the real part's;
+                               - in the variables scaled by the complex diagonal D = diag(A)
+                                 (c D^-1 (2I - c A D^-1)) when that bound reaches the real part's
+                                 (strong gradients over a large domain);
+                               - of the whole operator otherwise.
+                               c = 1.8 / b where b bounds the largest eigenvalue of the real part
+                               (1.05 x a 30-step power iteration estimate, capped by the Gershgorin
+                               bound), or, with the complex diagonal, the larger of that and a
+                               Gershgorin-type bound of D^-1 A over every step and signal.
+                               1 = Jacobi only, as the reference's KrylovSolver("bicgstab",
+                               "jacobi").  2 = the complex-diagonal polynomial whatever the bounds.
+                               Every choice converges to the same solution; the polynomial needs
+                               SpMV mapping 1, 2 or 4 (others fall back to mapping 2).
+                               dmri_stats.precond_used reports the choice. */
 } dmri_solver_opts;
 
 typedef struct {
@@ -213,6 +219,10 @@ typedef struct {
     double spmv_ms;         /* summed device time of the timed SpMV launches (profile > 0) */
     int64_t spmv_launches;  /* number of timed SpMV launches */
     double spmv_bytes;      /* algorithmic bytes of one SpMV launch */
+    int32_t precond_used;   /* preconditioner of the last launch group: 0 = Jacobi only, 1 = the
+                               polynomial of the scaled operator, 2 = the polynomial of its real part,
+                               3 = the polynomial in the variables scaled by the complex diagonal */
+    int32_t reserved;
 } dmri_stats;
 
 /* Run the theta scheme from u0 = 1 (real part, every live DOF) and return the signal
