@@ -1211,6 +1211,7 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     const bool proj = opts->initial_guess == 0;
     run.setup(c, 1, &seq->q, g, false, seq, opts, &st);
     const int nck = (N + K - 1) / K;
+    const int klast = keep_all ? 0 : (nck - 1) * K;  // first step of the last segment
     // checkpoints (x^k, x^(k-1), x^(k-2)) at k = 0, K, 2K, ...; a segment holds x^k0 .. x^k1
     DevArray<double2> ck, seg, mu, lam, lam2;
     ck.alloc((size_t)std::max(nck, 1) * 3 * n);
@@ -1230,9 +1231,11 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
             DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j + 1) * n, run.state_back(k > 0 ? 1 : 0), vb,
                                     hipMemcpyDeviceToDevice, s));
         }
-        if (keep_all && k == 0) DMRI_HIP(hipMemcpyAsync(segp(0), run.xc, vb, hipMemcpyDeviceToDevice, s));
+        // the last segment's states stay in the segment buffer, so the backward sweep starts
+        // without recomputing it
+        if (k == klast) DMRI_HIP(hipMemcpyAsync(segp(0), run.xc, vb, hipMemcpyDeviceToDevice, s));
         run.step(k);
-        if (keep_all) DMRI_HIP(hipMemcpyAsync(segp(k + 1), run.xc, vb, hipMemcpyDeviceToDevice, s));
+        if (k >= klast) DMRI_HIP(hipMemcpyAsync(segp(k - klast + 1), run.xc, vb, hipMemcpyDeviceToDevice, s));
     }
     run.signal(sig);
     const auto t1 = std::chrono::steady_clock::now();
@@ -1265,7 +1268,7 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     double2 *mu_cur = mu.p;
     for (int j = nck - 1; j >= 0; --j) {
         const int k0 = j * K, k1 = std::min(N, k0 + K);
-        if (!keep_all) {
+        if (!keep_all && j != nck - 1) {
             if (run.fh.m > 0) {
                 const double2 *states[3] = {ck.p + (size_t)(3 * j) * n, ck.p + (size_t)(3 * j + 1) * n,
                                             ck.p + (size_t)(3 * j + 2) * n};

@@ -272,8 +272,11 @@ int dmri_copy_bandwidth(dmri_ctx *ctx, int64_t bytes, int32_t reps, double *gbps
  *   lambda^N = w/|Omega|,  A_n mu_n = lambda^(n+1),  lambda^n = B_n mu_n,
  *   dS/dp = Re sum_n mu_n^T (dB_n/dp u^n - dA_n/dp u^(n+1)),
  * so the backward sweep runs the forward kernels.  States are kept at checkpoints every
- * checkpoint_stride steps as (u^k, u^(k-1)) and recomputed per segment, which reproduces the
- * forward states bit for bit; checkpoint_stride >= N keeps every state instead.  0 = automatic:
+ * checkpoint_stride steps as (u^k, u^(k-1), u^(k-2)) and recomputed per segment (the last segment's
+ * states are kept from the forward sweep itself).  With the extrapolated initial guesses the
+ * recomputed states equal the forward ones bit for bit; with the projection guess (the default),
+ * whose history restarts at each checkpoint, they agree to the solver tolerance.
+ * checkpoint_stride >= N keeps every state instead.  0 = automatic:
  * every state when the N+1 states fit in a quarter of the free device memory (then no step is
  * recomputed), else checkpoints every ceil(sqrt(N)) steps.
  * No reference counterpart: the reference computes signals only. */

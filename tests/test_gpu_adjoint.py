@@ -70,7 +70,10 @@ def test_automatic_checkpointing_keeps_every_state_when_it_fits():
 def test_adjoint_signal_equals_forward(adj):
     s, g, s_all, g_all = adj
     assert abs(s.real - _fwd()) <= 1e-13
-    assert g["checkpoints"] > 1 and g["recomputed_steps"] == len(FA)
+    stride = int(np.ceil(np.sqrt(len(FA))))
+    nck = -(-len(FA) // stride)
+    # every segment but the last is recomputed; the last keeps the forward sweep's states
+    assert g["checkpoints"] > 1 and g["recomputed_steps"] == (nck - 1) * stride
     assert g_all["checkpoints"] == 0 and g_all["recomputed_steps"] == 0
     # recomputation from checkpoints reproduces the stored-states gradients exactly
     for k in ("D", "invT2"):
