@@ -798,7 +798,8 @@ struct Stepper {
                 launch_extract_real(c->nnz, w.vals.p, vw, w.areal.p, s);
                 a.areal = w.areal.p;
                 st->precond_used = 2;
-            } else if (opts->preconditioner == 2 || imag >= kComplexDiagonal * breal) {
+            } else if ((geo.mapping == kMapSL || geo.mapping == kMapStaged) &&
+                       (opts->preconditioner == 2 || imag >= kComplexDiagonal * breal)) {
                 // The gradient term dominates the diagonal of rows far along g: the polynomial then
                 // acts on the operator scaled by its complex diagonal, whose spectrum stays near
                 // the real part's; c = 1.8 / (a bound of that spectrum over every step and signal).

@@ -89,6 +89,9 @@ constexpr int kMapTile = 5;  // tiles of rpb rows: the tile's distinct columns s
 // SpMV kernel operations: v = A p / t = A s (plain BiCGStab), the degree-1 polynomial
 // preconditioner z -> 2z - A z applied to p or s, and t = A sh (preconditioned BiCGStab)
 constexpr int kOpV = 0, kOpT = 0, kOpPre = 1, kOpTP = 2;
+// kernel-side variant of kOpPre with the complex diagonal (SolveArgs::pdiag), compiled apart so
+// that the plain preconditioner kernels keep their registers
+constexpr int kOpPreCD = 3;
 // the preconditioner polynomial takes the real part alone when the imaginary part's bound is at
 // most this fraction of the real part's (measured: 0.002 -> same iterations, 3.3 -> twice as many)
 constexpr double kRealPreconditioner = 0.05;

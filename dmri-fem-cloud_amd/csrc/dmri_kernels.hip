@@ -1831,20 +1831,25 @@ __global__ void __launch_bounds__(kThreads) spmv_v_kernel(SolveArgs a, int S, in
     const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     const double2 *p = a.p;
-    // with the complex diagonal d: ph = c D^-1 (2 p - c A D^-1 p), A D^-1 p gathering p / d
-    const bool cd = OP == kOpPre && a.pdiag != nullptr;
+    // kOpPreCD, with the complex diagonal d: ph = c D^-1 (2 p - c A D^-1 p), A D^-1 p gathering p / d
+    constexpr bool cd = OP == kOpPreCD;
+    constexpr bool pre = OP == kOpPre || OP == kOpPreCD;
     auto gather = [&](int c) {
         const double2 z = p[(int64_t)c * S_ + s];
-        return cd ? c_mul(z, cdiag_inv<PG>(a.pdiag, a.coefA, c, s)) : z;
+        if constexpr (cd) return c_mul(z, cdiag_inv<PG>(a.pdiag, a.coefA, c, s));
+        return z;
     };
     auto emit = [&](int64_t row, const Acc<PG> &acc) {
         if (L.g != 0) return;
         const int64_t idx = row * S_ + s;
         const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
-        if (OP == kOpPre) {
+        if (pre) {
             const double2 z = p[idx];
             const double2 w = make_double2(a.pc * (2.0 * z.x - a.pc * y.x), a.pc * (2.0 * z.y - a.pc * y.y));
-            a.ph[idx] = cd ? c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s)) : w;
+            if constexpr (cd)
+                a.ph[idx] = c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s));
+            else
+                a.ph[idx] = w;
             return;
         }
         a.v[idx] = y;
@@ -1936,21 +1941,26 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
     const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     const double2 *r = a.r, *v = a.v, *sh = a.sh;
-    const bool cd = OP == kOpPre && a.pdiag != nullptr;  // complex diagonal, as in spmv_v_kernel
+    constexpr bool cd = OP == kOpPreCD;  // complex diagonal, as in spmv_v_kernel
+    constexpr bool pre = OP == kOpPre || OP == kOpPreCD;
     auto gather = [&](int c) {
         const int64_t j = (int64_t)c * S_ + s;
         if (OP == kOpTP) return sh[j];
         const double2 z = c_sub(r[j], c_mul(alpha, v[j]));
-        return cd ? c_mul(z, cdiag_inv<PG>(a.pdiag, a.coefA, c, s)) : z;
+        if constexpr (cd) return c_mul(z, cdiag_inv<PG>(a.pdiag, a.coefA, c, s));
+        return z;
     };
     auto emit = [&](int64_t row, const Acc<PG> &acc) {
         if (L.g != 0) return;
         const int64_t idx = row * S_ + s;
         const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
         const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
-        if (OP == kOpPre) {
+        if (pre) {
             const double2 w = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
-            a.sh[idx] = cd ? c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s)) : w;
+            if constexpr (cd)
+                a.sh[idx] = c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s));
+            else
+                a.sh[idx] = w;
             return;
         }
         a.t[idx] = y;
@@ -1963,7 +1973,7 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
         staged_spmv<G, PG, true>(a, rpb, TR, L, act, gather, emit);
     else
         staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit);
-    if (OP != kOpPre) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
+    if (!pre) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
 }
 
 // ---- tile-compressed rows: a tile is the TR rows of one pass of the (row, signal) lane groups.
@@ -3274,7 +3284,8 @@ template <int OP>
 static void spmv_v_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                             hipEvent_t e1) {
     if (g.mapping == kMapWave) {
-        if (a.per_gdir)
+        if constexpr (OP == kOpPreCD) throw Error(-1, "the complex-diagonal preconditioner needs SpMV mapping 1 or 2");
+        else if (a.per_gdir)
             hipExtLaunchKernelGGL((spmv_v_wave_kernel<true, OP>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1,
                                   0, a, a.rowptr, a.col, a.vals, g.S, it);
         else
@@ -3294,7 +3305,8 @@ template <int OP>
 static void spmv_t_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                             hipEvent_t e1) {
     if (g.mapping == kMapWave) {
-        if (a.per_gdir)
+        if constexpr (OP == kOpPreCD) throw Error(-1, "the complex-diagonal preconditioner needs SpMV mapping 1 or 2");
+        else if (a.per_gdir)
             hipExtLaunchKernelGGL((spmv_t_wave_kernel<true, OP>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1,
                                   0, a, a.rowptr, a.col, a.vals, g.S, (int)g.grid, it);
         else
@@ -3327,7 +3339,10 @@ void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
                         hipExtLaunchKernelGGL((spmv_v_sr_kernel<GG, PP>), dim3(g.grid), dim3(g.sr_threads), 0, s, e0,
                                               e1, 0, a, g.S, g.sr_rpb, it));
     } else if (op == kOpPre) {
-        spmv_v_dispatch<kOpPre>(g, a, it, s, e0, e1);
+        if (a.pdiag && g.mapping != kMapWave)
+            spmv_v_dispatch<kOpPreCD>(g, a, it, s, e0, e1);
+        else
+            spmv_v_dispatch<kOpPre>(g, a, it, s, e0, e1);
     } else {
         spmv_v_dispatch<kOpV>(g, a, it, s, e0, e1);
     }
@@ -3353,7 +3368,10 @@ void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
                         hipExtLaunchKernelGGL((spmv_t_sr_kernel<GG, PP>), dim3(g.grid), dim3(g.sr_threads), 0, s, e0,
                                               e1, 0, a, g.S, g.sr_rpb, (int)g.grid, it));
     } else if (op == kOpPre) {
-        spmv_t_dispatch<kOpPre>(g, a, it, s, e0, e1);
+        if (a.pdiag && g.mapping != kMapWave)
+            spmv_t_dispatch<kOpPreCD>(g, a, it, s, e0, e1);
+        else
+            spmv_t_dispatch<kOpPre>(g, a, it, s, e0, e1);
     } else if (op == kOpTP) {
         spmv_t_dispatch<kOpTP>(g, a, it, s, e0, e1);
     } else {
