@@ -3458,6 +3458,9 @@ void launch_proj_guess(const SolveGeom &g, const ProjArgs &p, int mh, hipStream_
     // signal window of the Gram kernel: at most 2 (pair, signal) combinations per thread
     const int Sw = std::max(1, std::min(p.S, 2 * kThreads / (nv / 2)));
     const dim3 ggrid(grid, (p.S + Sw - 1) / Sw);
+    // the combination streams the history once: one element per thread over a covering grid
+    const int64_t cblocks = (p.n * p.S + (int64_t)g.vec_rpb * p.S - 1) / ((int64_t)g.vec_rpb * p.S);
+    const unsigned cgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cblocks, 1 << 30));
 #define DMRI_PROJ(M)                                                                   \
     proj_gram_kernel<M><<<ggrid, kThreads, lds, s>>>(p, Sw);                            \
     check_launch();                                                                    \
@@ -3465,7 +3468,7 @@ void launch_proj_guess(const SolveGeom &g, const ProjArgs &p, int mh, hipStream_
     check_launch();                                                                    \
     proj_solve_kernel<M><<<p.S, 64, 0, s>>>(p);                                        \
     check_launch();                                                                    \
-    proj_combine_kernel<M><<<g.vgrid, g.vec_threads, 0, s>>>(p, g.vec_rpb);            \
+    proj_combine_kernel<M><<<cgrid, g.vec_threads, 0, s>>>(p, g.vec_rpb);              \
     check_launch();
     if (mh == 4) {
         DMRI_PROJ(4)
