@@ -26,7 +26,8 @@ Reference behaviour kept (file:line in DmriFemLib.py of the reference):
   * PostProcessing: signal = int (1-p) u0r + p u1r / int u(0) and the result lines and log.txt
     append of 917-988.
 The PDE is solved by libdmri; this module only prepares inputs and formats outputs.
-Not mirrored: dolfin plotting, .pvd output, and the strong pseudo-periodic function spaces.
+Not mirrored: dolfin plotting, .pvd output (MRI_simulation.save_snapshots keeps u every nskip
+steps as arrays instead), and the strong pseudo-periodic function spaces.
 """
 from __future__ import annotations
 
@@ -244,6 +245,11 @@ class MRI_simulation:
         self.nskip = 5
         self.theta = 0.5
         self.device = 0
+        # True: keep u after every step whose counter is a multiple of nskip (the reference prints
+        # its progress at those steps, DmriFemLib.py:898) in self.snapshots, shape
+        # (nsnap, ncomp, nv) complex per compartment and vertex (0 where the compartment has no DOF),
+        # with the 0-based step indices in self.snapshot_steps
+        self.save_snapshots = False
 
     def _sequence(self, mri_para, integral=False):
         """f(t_n), f(t_{n-1}) and, for the periodic terms, F(t_{n-1}) = itime_profile(tp)
@@ -304,8 +310,20 @@ class MRI_simulation:
         self.signal_c, self.stats = ctx.solve(fA, fB, self.k, mri_para.qvalue, g, theta=self.theta,
                                               rtol=p["relative_tolerance"], atol=p["absolute_tolerance"],
                                               maxit=min(int(p["maximum_iterations"]), 2 ** 31 - 1),
-                                              F_explicit=FB, u0=self.u0_vertex)
+                                              F_explicit=FB, u0=self.u0_vertex,
+                                              snap_every=self.nskip if self.save_snapshots else 0)
         self.elapsed_time = time.time() - start
+        if self.save_snapshots:
+            snaps = self.stats.pop("snapshots")
+            _, _, dof_of = ctx.export_pattern()
+            nv = mesh.nv
+            ncomp = dof_of.shape[0] // nv
+            out = np.zeros((snaps.shape[0], ncomp, nv), dtype=np.complex128)
+            for c in range(ncomp):
+                d = dof_of[c * nv:(c + 1) * nv]
+                out[:, c, d >= 0] = snaps[:, d[d >= 0]]
+            self.snapshots = out
+            self.snapshot_steps = list(range(0, len(fA), self.nskip))
         self.t = len(fA) * self.k
         self.nsteps = len(fA)
         self.ctx = ctx

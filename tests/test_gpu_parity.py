@@ -401,9 +401,16 @@ def test_dmrifemlib_initial_condition_normalises_by_its_integral(tmp_path, monke
     def ic(x):
         return np.where(x[:, 0] < 5.0, 1.0, 0.5)
 
+    simu.save_snapshots = True
+    simu.nskip = 20
     simu.solve(dom, p, solver, ic=ic)
     out = L.PostProcessing(dom, p, simu, None)
     got = float(re.search(r"Normalized signal: ([0-9.e+-]+)", out).group(1))
+    # u every nskip steps per compartment and vertex; the last one is u^N when N - 1 is a multiple
+    assert simu.snapshots.shape == (len(simu.snapshot_steps), 1, mesh.nv)
+    if simu.snapshot_steps[-1] == simu.nsteps - 1:
+        _, _, dof_of = simu.ctx.export_pattern()
+        assert np.array_equal(simu.snapshots[-1, 0], simu.ctx.solution(0)[dof_of])
     voi_got = float(re.search(r"vol_of_interest: ([0-9.e+-]+)", out).group(1))
     fA, fB = _pgse_gcloud(10600.0, 43100.0, 400.0)
     S = O.System(mesh.xyz, mesh.tets, None, D=3e-3)
