@@ -482,7 +482,9 @@ void ensure_group_workspace(dmri_ctx *c, const SolveGeom &g, int nsteps, bool pe
     const size_t np = (size_t)std::max(g.grid, g.vgrid) * g.S * kNV;
     if (w.part.n < 3 * np) w.part.alloc(3 * np);
     if (w.tot.n < (size_t)3 * g.S * kNV) w.tot.alloc(3 * (size_t)g.S * kNV);
-    if (w.coef.n < (size_t)2 * nsteps * g.S * 3) w.coef.alloc(std::max<size_t>(1, (size_t)2 * nsteps * g.S * 3));
+    // at least one step's worth: the host always uploads max(N, 1) steps of coefficients
+    const size_t ncoef = (size_t)2 * std::max(nsteps, 1) * g.S * 3;
+    if (w.coef.n < ncoef) w.coef.alloc(ncoef);
     if (w.st.n < (size_t)g.S) w.st.alloc(kMaxSignals);
     if (!w.all_done.p) w.all_done.alloc(1);
     if (!w.mask.p) w.mask.alloc(kMaxSignals);
@@ -701,8 +703,10 @@ struct Stepper {
                         const double gd = g[3 * j + d];
                         hP[((size_t)k * S + j) * 3 + d] = q[j] * seq->F_explicit[k] * gd;
                     }
-            if (w.coefP.n < hP.size()) w.coefP.alloc(hP.size());
-            DMRI_HIP(hipMemcpyAsync(w.coefP.p, hP.data(), sizeof(double) * hP.size(), hipMemcpyHostToDevice, s));
+            if (!hP.empty()) {
+                if (w.coefP.n < hP.size()) w.coefP.alloc(hP.size());
+                DMRI_HIP(hipMemcpyAsync(w.coefP.p, hP.data(), sizeof(double) * hP.size(), hipMemcpyHostToDevice, s));
+            }
             const size_t nS = (size_t)n * S;
             if (w.ybc.n != nS) w.ybc.alloc(nS);
             if (w.bper.n != nS) w.bper.alloc(nS);
@@ -1523,7 +1527,8 @@ void solve_batched_impl(dmri_ctx *c, int nsig, const double *q, const double *gd
     validate_seq(seq, opts);
     validate_u0(c, seq);
     require(seq->snap_every >= 0, DMRI_ERR_ARG, "snap_every must be >= 0");
-    require(seq->snap_every == 0 || seq->snap_out != nullptr, DMRI_ERR_ARG, "snap_every > 0 needs snap_out");
+    require(seq->snap_every == 0 || seq->nsteps == 0 || seq->snap_out != nullptr, DMRI_ERR_ARG,
+            "snap_every > 0 needs snap_out");
     if (c->periodic && seq->nsteps > 0) {
         require(seq->F_explicit != nullptr, DMRI_ERR_ARG,
                 "periodic directions are set: dmri_seq.F_explicit (int_0^t f per step) is required");

@@ -263,7 +263,8 @@ class Context:
         nsnap = (s.nsteps + snap_every - 1) // snap_every
         buf = np.zeros((nsnap, nsig, int(self.info()["ndof"])), dtype=np.complex128)
         s.snap_every = snap_every
-        s.snap_out = _ptr(buf.view(np.float64), ctypes.c_double)
+        if buf.size:
+            s.snap_out = _ptr(buf.view(np.float64), ctypes.c_double)
         return buf
 
     def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,

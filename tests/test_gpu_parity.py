@@ -517,3 +517,20 @@ def test_automatic_complex_diagonal_on_a_large_domain():
           f"{st['iters_total']} (jacobi {st1['iters_total']})", flush=True)
     assert st["precond_used"] == 3
     assert abs(s_gpu.real - s_ref) <= 1e-9 * abs(s_ref)
+
+
+def test_snapshot_edge_cases():
+    """No step: no snapshot and the initial signal 1; a negative interval is rejected."""
+    mesh, kw = MESHES["box4"]
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    s0, st = ctx.solve(np.zeros(0), np.zeros(0), 200.0, 1e-5, (1, 0, 0), snap_every=3)
+    assert st["snapshots"].shape == (0, ctx.info()["ndof"])
+    assert abs(s0.real - 1.0) <= 1e-13
+    with pytest.raises(ValueError):
+        ctx.solve(np.zeros(4), np.zeros(4), 200.0, 1e-5, (1, 0, 0), snap_every=-1)
+    # one snapshot per step with an interval of 1; the last is the final state
+    fA, fB = _pgse_gcloud(1000.0, 3000.0, 200.0)
+    _, st = ctx.solve(fA, fB, 200.0, O.q_from_b_pgse(1000.0, 1000.0, 3000.0), (1, 0, 0), rtol=1e-13,
+                      snap_every=1)
+    assert st["snapshots"].shape[0] == len(fA)
+    assert np.array_equal(st["snapshots"][-1], ctx.solution(0))
