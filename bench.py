@@ -346,7 +346,7 @@ def run_forward(args, name):
         "mesh_gen_s": w["mesh_gen_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": "profiles/spmv_traffic_cylinders200k_5sig.json (rocprofv3 FETCH_SIZE x1.9998 + "
+                     "traffic_source": "profiles/spmv_traffic_cylinders200k_5sig.json (rocprofv3 FETCH_SIZE x1.9999 + "
                                        "WRITE_SIZE x1.0, copy-kernel calibrated)" if traffic else None,
                      "kernel": f"spmv_v/spmv_t (batched complex value-split CSR SpMV, {S_group} signals per launch, "
                                "fused reductions)",
