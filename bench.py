@@ -1,35 +1,41 @@
 #!/usr/bin/env python3
 """Benchmark: batched Bloch-Torrey FEM solves on one MI355X.
 
-Default workload (--workload c1, the headline configuration): concentric cylinders r_in=4 um,
-r_out=8 um, L=20 um, ~200k-vertex P1 tetrahedral mesh (seeded, 2-D Delaunay + extrusion),
-D_in=D_out=3e-3 um^2/us, kappa=1e-5 um/us, T2=inf (reference default 1e16), PGSE delta=10600 us,
-Delta=43100 us, dt=200 us (270 theta steps, GCloud loop t < T+k, lagged right-hand side),
-theta=1/2, gdir=(0,1,0), b in {0, 500, 1000, 2000, 4000} s/mm^2, FP64, Krylov rtol 1e-12.
+Default workload (--workload c4, the largest configuration): neuron-like surrogate mesh, soma
+sphere r=8 um + 6 seeded branching dendrites (r 0.5-2 um, 3 bifurcation levels, lengths 20-60 um),
+voxelised into ~2M vertices / 9.4M P1 tetrahedra, one compartment, D=3e-3 um^2/us, T2=inf
+(reference default 1e16); cos-OGSE with 4 periods (delta = Delta = 19900 us, t0 = seq_ext = 100 us,
+T = 40 ms), dt = 100 us (401 theta steps, GCloud loop t < T+k, lagged right-hand side), theta=1/2,
+32 Fibonacci-sphere gradient directions x b in {1000, 2000, 3000, 4000} s/mm^2 = 128 signals in
+launch groups of 32, FP64, Krylov rtol 1e-12.
 
-One step = one complete job on mesh data already resident in HBM: device assembly
-(pattern, colouring, M/K/R/I/Jx/Jy/Jz) + every signal of the workload.  value = signals/s.
+One bench step = one theta step of all 128 signals (a resumable batch keeps every launch group's
+states, Krylov workspace and initial-guess history in HBM between steps); the mesh is assembled
+on the device before the timed window.  value = complex DOF-steps/s (ndof x signals x steps /
+time); signals_per_s_equiv divides it by the DOF-steps of one full 401-step signal.
 
-Other workloads (each prints the same kind of line; --theta-steps cuts the sequence to its
-first K steps for a shorter measurement, stated in the line):
+Other workloads (whole jobs per bench step: device assembly + every signal's full sequence;
+--theta-steps cuts the sequence to its first K steps, stated in the line):
+  c1  two-compartment concentric cylinders r_in=4, r_out=8, L=20 um, ~200k vertices, kappa=1e-5,
+      PGSE delta=10600 us, Delta=43100 us, dt=200 us (270 steps), gdir (0,1,0),
+      b in {0, 500, 1000, 2000, 4000}: value = signals/s.
   c2  periodic box 30x30x10 um with 20 seeded cylinders (r uniform 2-4 um), ~500k vertices,
       weak pseudo-periodic faces in x and y, kappa=1e-5, 64 Fibonacci-sphere directions x
       8 b-values (linspace 0..4000) = 512 signals in launch groups of 32 (--group).
-  c4  neuron-like surrogate (soma r=8 um + 6 seeded dendrite trees, 3 bifurcation levels),
-      ~2M vertices, cos-OGSE 4 periods, T=40 ms, dt=100 us (401 steps), 32 directions x
-      4 b-values = 128 signals.
+  c4 --whole-jobs  the default workload timed as whole jobs.
   c3  the c1 geometry at ~300k vertices, b=2000, T2=80 ms: the adjoint (forward + checkpointed
       reverse sweep with every gradient) against the forward solve alone.
 
-Also reported: DOF-steps/s, the SpMV roofline and the CPU baseline.  The SpMV launches of the
-first --event-iters Krylov iterations of every theta step inside the timed region are bracketed
-by HIP events on the solver's stream (sampled so that the events, which cost GPU time of their
-own, barely perturb the timed work); achieved = algorithmic bytes per launch / mean launch time.
+Also reported: the SpMV roofline and the CPU baseline.  The SpMV launches of the first
+--event-iters Krylov iterations of every theta step inside the timed region are bracketed by HIP
+events on the solver's stream (sampled so that the events, which cost GPU time of their own,
+barely perturb the timed work); achieved = algorithmic bytes per launch / mean launch time.
 After the timed region the same two kernels are replayed back to back on the solver's buffers
-(no per-launch events) as a second figure that compares directly with a kernel trace.  CPU baseline:
-the FP64 CPU oracle on the host cores, bounded sample: the first --cpu-steps theta steps of one
-signal, scaled per signal.  The GPU result of the same truncated sequence is compared with the
-CPU one (full-size parity on a prefix).
+(no per-launch events) as a second figure that compares directly with a kernel trace.  CPU
+baseline: the FP64 CPU oracle on the host cores, bounded sample (c4: theta steps 1..--cpu-steps-1
+of one signal after an untimed step 0, and step 1 again on one core; c1/c2: the first --cpu-steps
+theta steps of one signal, scaled per signal).  The GPU's plain solve of the same prefix is
+compared with the CPU one (full-size parity on a prefix).
 """
 from __future__ import annotations
 
@@ -45,83 +51,52 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dmri-fem-cloud_amd"))
 
+import dmri_sequences as DS  # noqa: E402
+
 METRIC = "signals/sec & DOF-steps/sec at <=1e-8 rel err vs FP64 CPU; HBM GB/s vs peak"
 HBM_PEAK_GBS = 8000.0
 
 
 def gcloud_loop(f, T, k):
-    """GCloud loop (t < T + k, t += k): (f(t_n), f(t_{n-1})) with t_{-1} = 0, and the list of t_{n-1}."""
-    fA, fB, tB = [], [], []
-    t, tp = 0.0, 0.0
-    while t < T + k:
-        fA.append(f(t))
-        fB.append(f(tp))
-        tB.append(tp)
-        tp = t
-        t += k
-    return np.asarray(fA), np.asarray(fB), np.asarray(tB)
+    """(f(t_n), f(t_{n-1}), t_{n-1}) over the GCloud loop (t < T + k, t_{-1} = 0)."""
+    fA, fB, tB = DS.gcloud_loop(f, T, k, F=lambda t: t)
+    return fA, fB, tB
 
 
 def pgse_sequence(delta, Delta, k):
     """PGSE of GCloudDmriSolver.py:188-193 (exclusive at T) on the GCloud loop."""
-    T = delta + Delta
-
-    def f(s):
-        return 1.0 if s < delta else (0.0 if s < Delta else (-1.0 if s < T else 0.0))
-
-    fA, fB, _ = gcloud_loop(f, T, k)
+    fA, fB, _ = DS.gcloud_loop(DS.pgse(delta, Delta), delta + Delta, k)
     return fA, fB
 
 
 def pgse_integral_explicit(delta, Delta, k):
     """F(t_{n-1}) = int_0^t f of the PGSE on the GCloud loop (the weak-periodic mirror phase)."""
-    T = delta + Delta
-
-    def F(t):
-        return t if t < delta else (delta if t < Delta else (delta - (t - Delta) if t < T else 0.0))
-
-    _, FB, _ = gcloud_loop(F, T, k)
-    return FB
+    return DS.gcloud_loop(DS.pgse(delta, Delta), delta + Delta, k, F=DS.pgse_integral(delta, Delta))[2]
 
 
 def q_from_b(b, delta, Delta):
-    return float(np.sqrt(b) / np.sqrt(delta * delta * (Delta - delta / 3.0)))
+    return DS.q_from_b(b, DS.pgse_int_F2(delta, Delta))
 
 
 def cos_ogse(nperiod, delta, Delta, t0=100.0, ext=100.0):
-    """cos-OGSE of ArbitraryTimeSequence.ipynb (profile 3): omega = 2 pi nperiod / delta,
-    tau = (Delta + delta)/2, T = Delta + delta + t0 + ext; returns (f, T, int_0^T F^2)."""
-    om = 2.0 * math.pi * nperiod / delta
-    tau = 0.5 * (Delta + delta)
-    T = Delta + delta + t0 + ext
-
-    def f(s):
-        if s < t0:
-            return 0.0
-        if s <= delta + t0:
-            return math.cos(om * (s - t0))
-        if s <= tau + t0:
-            return 0.0
-        if s <= delta + tau + t0:
-            return -math.cos(om * (s - t0 - tau))
-        return 0.0
-
-    # F = sin(om (s - t0))/om on the first lobe, its negative shifted by tau on the second;
-    # with nperiod whole periods F returns to 0 after each lobe: int F^2 = 2 (delta/2)/om^2
-    return f, T, delta / (om * om)
+    """cos-OGSE of ArbitraryTimeSequence.ipynb (profile 3): returns (f, T, int_0^T F^2)."""
+    f, T = DS.cos_ogse(nperiod, delta, Delta, t0, ext)
+    return f, T, DS.cos_ogse_int_F2(nperiod, delta, Delta)
 
 
-def fibonacci_directions(n):
-    k = np.arange(n) + 0.5
-    phi = np.arccos(1 - 2 * k / n)
-    th = math.pi * (1 + 5 ** 0.5) * k
-    return np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], axis=1)
+fibonacci_directions = DS.fibonacci_directions
 
 
-def measured_traffic(info, mapping_lanes):
+TRAFFIC_FILES = {"c1": "spmv_traffic_cylinders200k_5sig.json", "c4": "spmv_traffic_neuron2m_32sig.json"}
+
+
+def measured_traffic(info, mapping_lanes, name="c1"):
     """HBM bytes per SpMV launch from the committed FETCH_SIZE/WRITE_SIZE counter runs of this
     workload (copy-kernel calibrated), if the mesh and SpMV mapping are the ones measured."""
-    path = os.path.join(ROOT, "profiles", "spmv_traffic_cylinders200k_5sig.json")
+    fname = TRAFFIC_FILES.get(name)
+    if fname is None:
+        return None
+    path = os.path.join(ROOT, "profiles", fname)
     try:
         with open(path) as fh:
             d = json.load(fh)
@@ -130,7 +105,8 @@ def measured_traffic(info, mapping_lanes):
     if (d.get("nnz"), d.get("ndof")) != (info["nnz"], info["ndof"]) or \
             list(d.get("mapping_lanes", [])) != list(mapping_lanes or []):
         return None
-    return float(d["traffic_per_launch"])
+    return {"traffic_per_launch": float(d["traffic_per_launch"]),
+            "source": f"profiles/{fname} ({d.get('calibration', 'rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-kernel calibrated')})"}
 
 
 # ------------------------------------------------------------------------------------------
@@ -299,7 +275,7 @@ def run_forward(args, name):
     iters = sum(s["iters_total"] for s in all_stats)
     us_v, us_t, replay_bytes = ctx.time_spmv(S_group, per_gdir, 50)
     used = tuned if tuned is not None else ([args.mapping, args.lanes] if args.mapping and args.lanes else None)
-    traffic = measured_traffic(info, used) if name == "c1" else None
+    traffic = measured_traffic(info, used, name)
     replay_achieved = replay_bytes / (0.5 * (us_v + us_t) * 1e-6) / 1e9
     b0 = [abs(s.real - 1.0) for s, q in zip(sig, qs) if q == 0.0]
 
@@ -345,9 +321,8 @@ def run_forward(args, name):
         "setup_ms": info["setup_ms"],
         "mesh_gen_s": w["mesh_gen_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": "profiles/spmv_traffic_cylinders200k_5sig.json (rocprofv3 FETCH_SIZE x1.9999 + "
-                                       "WRITE_SIZE x1.0, copy-kernel calibrated)" if traffic else None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["traffic_per_launch"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "kernel": f"spmv_v/spmv_t (batched complex value-split CSR SpMV, {S_group} signals per launch, "
                                "fused reductions)",
                      "avg_launch_us": avg_us, "bytes_per_launch": spmv_bytes, "launches": spmv_n,
@@ -360,6 +335,175 @@ def run_forward(args, name):
         "parity": parity_block(w, b0, prefix_parity, sig),
     }
     print(json.dumps(out))
+
+
+def run_stepwise(args, name):
+    """Theta step by theta step: every signal of the workload advances one theta step per bench step
+    through a resumable batch (dmri_batch_advance) that keeps the states, Krylov workspaces and
+    initial-guess histories of all launch groups in HBM.  value = complex DOF-steps/s."""
+    import dmri_native as N
+
+    w = workload(name, args)
+    mesh, fA, fB, FB, k = w["mesh"], w["fA"], w["fB"], w["FB"], w["k"]
+    qs, gd = w["qs"], w["gdirs"]
+    nsig = len(qs)
+    per_gdir = bool(np.any(np.abs(gd - gd[0]) > 0))
+    S_group = min(args.group, nsig)
+    nfull = len(fA)
+    if args.warmup + args.steps > nfull:
+        raise SystemExit(f"warmup + steps ({args.warmup + args.steps}) exceed the {nfull} theta steps of the sequence")
+    ctx = N.Context(0)
+    ctx.set_mesh(mesh.xyz, mesh.tets, mesh.phase)
+    ctx.set_coeffs(**w["coeffs"])
+    if w["periodic"]:
+        ctx.set_periodic(w["periodic"])
+    ctx.assemble()
+    info = ctx.info()
+    copy_gbs = ctx.copy_bandwidth(args.copy_bytes, 10) if args.copy_bytes > 0 else None
+    tuned = None
+    if not args.no_tune and args.lanes == 0 and args.mapping == 0:
+        tuned = ctx.tune(S_group, per_gdir)[0]
+    print(f"  mesh nv={info['nv']} ndof={info['ndof']} nnz={info['nnz']}, {nsig} signals in groups of {S_group}, "
+          f"SpMV mapping/lanes {tuned}", file=sys.stderr, flush=True)
+    tb = time.perf_counter()
+    batch = ctx.batch(fA, fB, k, qs, gd, rtol=args.rtol, profile=args.event_iters, lanes=args.lanes,
+                      guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond, group=S_group)
+    t_begin = time.perf_counter() - tb
+    for i in range(args.warmup):
+        batch.advance(1)
+        print(f"  warmup theta step {i} done", file=sys.stderr, flush=True)
+    all_stats = []
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        all_stats.append(batch.advance(1))
+        print(f"  theta step {args.warmup + i}: {all_stats[-1]['solve_ms']:.1f} ms, "
+              f"{all_stats[-1]['iters_total'] / nsig:.1f} Krylov iterations per signal", file=sys.stderr, flush=True)
+    t2 = time.perf_counter()
+    elapsed = t2 - t1
+    sig, done = batch.signals()
+    nd = float(info["ndof"])
+    value = nd * nsig * args.steps / elapsed
+    spmv_ms = sum(s["spmv_ms"] for s in all_stats)
+    spmv_n = sum(s["spmv_launches"] for s in all_stats)
+    spmv_bytes = all_stats[0]["spmv_bytes"]
+    avg_us = 1e3 * spmv_ms / max(spmv_n, 1)
+    achieved = spmv_bytes / (avg_us * 1e-6) / 1e9 if spmv_n else 0.0
+    iters = sum(s["iters_total"] for s in all_stats)
+    precond = all_stats[-1].get("precond_used")
+    batch.close()
+    us_v, us_t, replay_bytes = ctx.time_spmv(S_group, per_gdir, 20)
+    replay_achieved = replay_bytes / (0.5 * (us_v + us_t) * 1e-6) / 1e9
+    traffic = measured_traffic(info, tuned, name)
+
+    # one signal at a time on the GPU over the same theta steps (a launch group of one)
+    plain = None
+    if not args.no_plain:
+        j = w["cpu_signal"]
+        pb = ctx.batch(fA, fB, k, qs[j:j + 1], gd[j:j + 1], rtol=args.rtol, lanes=0, guess=args.guess,
+                       F_explicit=FB, precond=args.precond, group=1)
+        pb.advance(args.warmup)
+        tp0 = time.perf_counter()
+        pst = pb.advance(args.steps)
+        tp = time.perf_counter() - tp0
+        psig, _ = pb.signals()
+        pb.close()
+        plain_rate = nd * args.steps / tp
+        plain = {"dof_steps_per_s": plain_rate, "signals_per_s_equiv": plain_rate / (nd * nfull),
+                 "sample": f"signal {j} alone (a launch group of one), theta steps {args.warmup}..."
+                           f"{args.warmup + args.steps - 1}", "batched_over_plain": value / plain_rate,
+                 "krylov_iters_per_theta_step": pst["iters_total"] / args.steps,
+                 "rel_diff_vs_batched": abs(psig[0] - sig[j]) / max(abs(sig[j]), 1e-300)}
+
+    cpu, parity = (None, None)
+    if not args.no_cpu:
+        cpu, parity = cpu_baseline_steps(args, w, ctx, info)
+
+    cfg = {"workload": w["desc"].replace("step = device assembly + all signals",
+                                         "step = one theta step of every signal"),
+           "nv": info["nv"], "nc": info["nc"], "ndof": info["ndof"], "nnz": info["nnz"], "n_iface": info["n_iface"],
+           "theta_steps_full_sequence": nfull, "signals": nsig, "launch_group": S_group, "rtol": args.rtol,
+           "timed_window": f"theta steps {args.warmup}..{args.warmup + args.steps - 1} of {nfull} (after "
+                           f"{args.warmup} warm-up steps from u0 = 1), every signal",
+           "spmv_mapping_lanes": tuned}
+    cfg.update(w["extra"])
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": cfg,
+        "signals_per_s_equiv": value / (nd * nfull),
+        "krylov_iters_per_theta_step": iters / (args.steps * nsig),
+        "preconditioner_used": {0: "jacobi", 1: "polynomial", 2: "polynomial of the real part",
+                                3: "polynomial, complex diagonal"}.get(precond),
+        "setup_ms": info["setup_ms"],
+        "batch_begin_s": t_begin,
+        "mesh_gen_s": w["mesh_gen_s"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["traffic_per_launch"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "kernel": f"spmv_v/spmv_t (batched complex value-split CSR SpMV, {S_group} signals per launch, "
+                               "fused reductions)",
+                     "avg_launch_us": avg_us, "bytes_per_launch": spmv_bytes, "launches": spmv_n,
+                     "events": f"first {args.event_iters} Krylov iterations of every theta step",
+                     "replay_us": {"spmv_v": us_v, "spmv_t": us_t}, "replay_achieved": replay_achieved,
+                     "copy_kernel_gbs": copy_gbs,
+                     "frac_of_copy": (achieved / copy_gbs) if copy_gbs else None},
+        "cpu_baseline": cpu,
+        "plain_one_signal_at_a_time": plain,
+        "parity": parity,
+        "signals_first": [complex(x).real for x in sig[:4]],
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline_steps(args, w, ctx, info):
+    """The FP64 CPU oracle on one signal: theta step 0 untimed, then steps 1..n-1 timed on every host
+    core (and step 1 again on one core); the GPU's plain solve of the same n steps is the parity
+    check at full size."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    m = w["mesh"]
+    fA, fB, FB = w["fA"], w["fB"], w["FB"]
+    n = max(2, min(args.cpu_steps, len(fA)))
+    j = w["cpu_signal"]
+    q, g = w["qs"][j], w["gdirs"][j]
+    c = w["coeffs"]
+    S = O.System(m.xyz, m.tets, m.phase, D=c["D"], invT2=c["invT2"], kappa=c["kappa"], periodic=w["periodic"])
+    FBn = None if FB is None else FB
+    _, u1, _ = S.solve(g, q, w["k"], fA[:1], fB[:1], rtol=args.rtol, FB=None if FBn is None else FBn[:1])
+    cores = O.num_threads()
+    tc = time.perf_counter()
+    s_cpu, _, st = S.solve(g, q, w["k"], fA[1:n], fB[1:n], rtol=args.rtol, u0=u1,
+                           FB=None if FBn is None else FBn[1:n])
+    tcpu = time.perf_counter() - tc
+    one = None
+    if not args.no_cpu_1core:
+        O.set_threads(1)
+        t1c = time.perf_counter()
+        _, _, st1 = S.solve(g, q, w["k"], fA[1:2], fB[1:2], rtol=args.rtol, u0=u1,
+                            FB=None if FBn is None else FBn[1:2])
+        t1 = time.perf_counter() - t1c
+        O.set_threads(cores)
+        one = {"value": float(info["ndof"]) / t1, "unit": "DOF-steps/s", "cores": 1,
+               "sample": f"theta step 1 of signal {j}, {int(st1['iters'])} BiCGStab iterations, {t1:.2f} s"}
+    s_gpu, _ = ctx.solve(fA[:n], fB[:n], w["k"], q, g, rtol=args.rtol, F_explicit=None if FB is None else FB[:n])
+    cpu = {"value": float(info["ndof"]) * (n - 1) / tcpu, "unit": "DOF-steps/s", "cores": cores, "kind": "port",
+           "sample": f"FP64 CPU oracle (C/OpenMP BiCGStab+Jacobi, same mesh, rtol {args.rtol:g}): theta steps "
+                     f"1..{n - 1} of signal {j} (q={q:.4g}), {int(st['iters'])} BiCGStab iterations, {tcpu:.2f} s",
+           "krylov_iters_per_theta_step": st["iters"] / (n - 1), "one_core": one}
+    parity = {"theta_steps": n, "signal_gpu": s_gpu.real, "signal_cpu": s_cpu,
+              "rel_err": abs(s_gpu.real - s_cpu) / abs(s_cpu)}
+    return cpu, parity
 
 
 def run_adjoint(args):
@@ -432,11 +576,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4"],
+                    help="c4 (default): the neuron-like surrogate, one theta step of all 128 signals per bench step; "
+                         "c1, c2: whole jobs per bench step; c3: the adjoint")
+    ap.add_argument("--whole-jobs", action="store_true",
+                    help="c4 only: time whole jobs (assembly + every signal's full sequence) instead of theta steps")
     ap.add_argument("--nv", type=int, default=0, help="target vertex count (0 = the workload's own)")
     ap.add_argument("--theta-steps", type=int, default=0, help="cut the sequence to its first K theta steps")
-    ap.add_argument("--cpu-steps", type=int, default=60, help="theta steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="theta steps of the CPU baseline sample (0 = the workload's default: 60 for c1/c2, 3 for c4)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-1core", action="store_true", help="skip the one-core CPU figure (c4)")
     ap.add_argument("--rtol", type=float, default=1e-12)
     ap.add_argument("--precond", type=int, default=0,
                     help="0 = Jacobi + degree-1 polynomial, chosen by the library (default), 1 = Jacobi only, "
@@ -461,10 +611,14 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus N>1 is not supported "
-                                   "(signals shard with no exchange; see the sharding section of DESIGN.md)"}))
+                                   "(signals are independent: each GPU would take its own launch groups)"}))
         sys.exit(2)
+    if args.cpu_steps <= 0:
+        args.cpu_steps = 3 if args.workload == "c4" else 60
     if args.workload == "c3":
         run_adjoint(args)
+    elif args.workload == "c4" and not args.whole_jobs:
+        run_stepwise(args, args.workload)
     else:
         run_forward(args, args.workload)
 

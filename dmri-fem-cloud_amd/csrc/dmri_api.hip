@@ -66,6 +66,7 @@ struct dmri_ctx {
     double kappa = 0.0;
     DevArray<double> Dc, iT2c, Dcell, iT2cell, kappa_f;
     bool has_Dcell = false, has_iT2cell = false, has_kappa_f = false;
+    bool Dcell_symmetric = true;  // every D_cell tensor symmetric (the adjoint needs it)
     // assembled
     bool assembled = false;
     int64_t ndof = 0, nnz = 0, n_iface = 0, n_bfacet = 0;
@@ -92,6 +93,7 @@ struct dmri_ctx {
     int last_S = 1;
     std::vector<std::pair<long long, int>> tuned;  // (S*2 + per_gdir) -> lanes per row
     bool solved = false;
+    uint64_t generation = 0;  // bumped by every dmri_assemble: batches made before it are stale
     hipEvent_t ev0[kEventPool], ev1[kEventPool];
     bool events_made = false;
 
@@ -274,8 +276,24 @@ void assemble_periodic(dmri_ctx *c) {
     }
 }
 
+// Interface matrix kappa int_Gamma (u0-u1)(v0-v1) into gi (zeroed first), one Jones-Plassmann
+// colour of facets per launch; returns the number of colours.
+int assemble_interface(dmri_ctx *c, double kappa, const double *kappa_f, double *gi, int *miss) {
+    hipStream_t s = c->stream;
+    DMRI_HIP(hipMemsetAsync(gi, 0, sizeof(double) * c->nnz, s));
+    DevArray<int> perm;
+    std::vector<int64_t> offs = colour_elements(3, c->iface_v.p, c->n_iface, c->fptr.p, c->flist.p, perm, s);
+    const int ncol = (int)offs.size() - 1;
+    for (int k = 0; k < ncol; ++k)
+        launch_assemble_facets(offs[k], offs[k + 1] - offs[k], perm.p, c->iface_v.p, c->xyz.p, c->nv, c->dof_of.p,
+                               c->rowptr.p, c->col.p, kappa, kappa_f, gi, miss, s);
+    DMRI_HIP(hipStreamSynchronize(s));
+    return ncol;
+}
+
 void assemble_impl(dmri_ctx *c) {
     require(c->has_mesh, DMRI_ERR_STATE, "dmri_assemble: no mesh (call dmri_mesh_set first)");
+    ++c->generation;
     c->tiles.clear();
     set_device(c);
     hipStream_t s = c->stream;
@@ -346,6 +364,14 @@ void assemble_impl(dmri_ctx *c) {
                        flist, c->iface_v.p, c->dof_of.p, cnt.p, nullptr, nullptr, overflow.p, s);
         require(read_scalar(overflow.p, s) == 0, DMRI_ERR_LIMIT,
                 "a vertex touches more than 512 cells; pattern build limit exceeded");
+        // the CSR offsets are 32-bit: check the entry count in 64 bits before the 32-bit scan
+        DevArray<unsigned long long> total;
+        total.alloc(1);
+        total.zero(s);
+        launch_count_sum64(cnt.p, c->ndof, total.p, s);
+        const unsigned long long nnz64 = read_scalar(total.p, s);
+        require(nnz64 < (1ULL << 31), DMRI_ERR_LIMIT,
+                "the sparsity pattern has " + std::to_string(nnz64) + " entries; the limit is 2^31 - 1");
         exclusive_scan(cnt.p, c->rowptr.p, c->ndof, s);
         c->nnz = read_scalar(c->rowptr.p + c->ndof, s);
         require(c->nnz > 0, DMRI_ERR_MESH, "empty pattern");
@@ -366,16 +392,9 @@ void assemble_impl(dmri_ctx *c) {
                          c->has_iT2cell ? c->iT2cell.p : nullptr, c->val[DMRI_MAT_M].p, c->val[DMRI_MAT_K].p,
                          c->val[DMRI_MAT_R].p, c->val[DMRI_MAT_JX].p, c->val[DMRI_MAT_JY].p, c->val[DMRI_MAT_JZ].p, s);
     c->ncol_iface = 0;
-    if (have_iface) {
-        DevArray<int> perm;
-        std::vector<int64_t> offs = colour_elements(3, c->iface_v.p, c->n_iface, c->fptr.p, c->flist.p, perm, s);
-        c->ncol_iface = (int)offs.size() - 1;
-        for (int k = 0; k < c->ncol_iface; ++k)
-            launch_assemble_facets(offs[k], offs[k + 1] - offs[k], perm.p, c->iface_v.p, c->xyz.p, nv, c->dof_of.p,
-                                   c->rowptr.p, c->col.p, c->kappa, c->has_kappa_f ? c->kappa_f.p : nullptr,
-                                   c->val[DMRI_MAT_I].p, miss.p, s);
-        DMRI_HIP(hipStreamSynchronize(s));
-    }
+    if (have_iface)
+        c->ncol_iface = assemble_interface(c, c->kappa, c->has_kappa_f ? c->kappa_f.p : nullptr,
+                                           c->val[DMRI_MAT_I].p, miss.p);
     require(read_scalar(miss.p, s) == 0, DMRI_ERR_MESH, "element entry outside the CSR pattern");
 
     // 5b. weak pseudo-periodic boundary mass and mirror entries
@@ -460,10 +479,9 @@ int auto_lanes(int S) {
     return l;
 }
 
-void ensure_group_workspace(dmri_ctx *c, const SolveGeom &g, int nsteps, bool per_gdir) {
+void ensure_group_workspace(dmri_ctx *c, SolverWork &w, const SolveGeom &g, int nsteps, bool per_gdir) {
     hipStream_t s = c->stream;
     const size_t nS = (size_t)c->ndof * g.S;
-    SolverWork &w = c->work;
     if (w.x.n != nS) {
         w.x.alloc(nS);
         w.r.alloc(nS);
@@ -635,6 +653,7 @@ int proj_capacity(size_t nS, size_t held_bytes) {
 
 struct Stepper {
     dmri_ctx *c = nullptr;
+    SolverWork *wk = nullptr;  // the launch group's buffers (the context's own, or a batch group's)
     hipStream_t s = nullptr;
     int S = 1, N = 0;
     int64_t n = 0;
@@ -654,8 +673,9 @@ struct Stepper {
 
     // q[s], g[3*s] (unit).  Scales the system, uploads the per-step coefficients, sets u0 = 1.
     void setup(dmri_ctx *ctx, int S_, const double *q, const double *g, bool pg, const dmri_seq *seq,
-               const dmri_solver_opts *o, dmri_stats *stats) {
+               const dmri_solver_opts *o, dmri_stats *stats, SolverWork *work = nullptr) {
         c = ctx;
+        wk = work ? work : &c->work;
         s = c->stream;
         S = S_;
         n = c->ndof;
@@ -683,8 +703,8 @@ struct Stepper {
         geo = make_geom(n, lanes, S, mapping);
         fit_spmv_grid(geo, per_gdir);
         tiles = tiles_for(c, geo);
-        ensure_group_workspace(c, geo, N, per_gdir);
-        SolverWork &w = c->work;
+        ensure_group_workspace(c, *wk, geo, N, per_gdir);
+        SolverWork &w = *wk;
         auto &V = c->val;
         const double theta = seq->theta, dt = seq->dt;
         const double *pv = c->periodic ? c->pval.p : nullptr;
@@ -840,7 +860,7 @@ struct Stepper {
     // Residual-minimising guess over the last fh.m states (see proj_capacity); the quadratic
     // extrapolation stays in use when the history does not fit.
     void setup_projection(const double *g) {
-        SolverWork &w = c->work;
+        SolverWork &w = *wk;
         const size_t nS = (size_t)n * S;
         size_t held = 0;  // history buffers this context already holds count as free
         for (int j = 0; j < kMaxHist; ++j) held += (w.yh[j].n + w.hp[j].n + w.hq[j].n) * sizeof(double2);
@@ -866,7 +886,7 @@ struct Stepper {
     }
 
     void ensure_proj_scratch(int m, const double *g) {
-        SolverWork &w = c->work;
+        SolverWork &w = *wk;
         const size_t npart = (size_t)proj_grid(geo) * S * proj_nv(m);
         if (w.ppart.n < npart) w.ppart.alloc(npart);
         if (w.ptot.n < (size_t)S * proj_nv(m)) w.ptot.alloc((size_t)S * proj_nv(m));
@@ -881,7 +901,7 @@ struct Stepper {
     // Guess of the current step (coefficients set by begin_step, b in work.b) from history h
     // into a free buffer of h, which it returns.
     double2 *projected_guess(ProjHistory &h) {
-        SolverWork &w = c->work;
+        SolverWork &w = *wk;
         double2 *y0 = h.take_free();
         ProjArgs pa{};
         pa.n = n;
@@ -935,7 +955,7 @@ struct Stepper {
     // Restart the projection history from count stored states (newest first): the older ones get
     // their history products, the newest becomes xc (its products come with the next step).
     void reseed_projection(const double2 *const *states, int count) {
-        SolverWork &w = c->work;
+        SolverWork &w = *wk;
         fh.reset(fh.m, w.yh, w.hp, w.hq);
         fh.yfree.push_back(w.x.p);
         const size_t vb = sizeof(double2) * (size_t)n * S;
@@ -962,7 +982,7 @@ struct Stepper {
     // empty); returns the buffer holding the solution.
     double2 *solve_projected(int k, const double2 *rhs, ProjHistory &h, double2 *start) {
         begin_step(k);
-        DMRI_HIP(hipMemcpyAsync(c->work.b.p, rhs, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        DMRI_HIP(hipMemcpyAsync(wk->b.p, rhs, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
         double2 *y = h.nh() > 0 ? projected_guess(h) : start;
         a.x = y;
         launch_rhs_residual(geo, a, 1, nullptr, s);
@@ -980,13 +1000,13 @@ struct Stepper {
         a.hq = slot.second;
         launch_rhs_residual(geo, a, 4, nullptr, s);
         h.push(y, slot);
-        DMRI_HIP(hipMemcpyAsync(out, c->work.b.p, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        DMRI_HIP(hipMemcpyAsync(out, wk->b.p, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
         a.x = xc;
         a.hp = a.hq = nullptr;
     }
 
     void begin_step(int step) {
-        SolverWork &w = c->work;
+        SolverWork &w = *wk;
         a.coefA = w.coef.p + (size_t)step * S * 3;
         a.coefB = w.coef.p + (size_t)std::max(N, 1) * S * 3 + (size_t)step * S * 3;
         launch_reset_state(w.st.p, S, w.all_done.p, s);
@@ -994,7 +1014,7 @@ struct Stepper {
 
     // BiCGStab iterations on a.x with the residual already formed; restarts on breakdown.
     void krylov(int step) {
-        SolverWork &w = c->work;
+        SolverWork &w = *wk;
         int it = 0, it_before = 0, restarts = 0;
         for (;;) {
             const int nl = std::min(chunk, kEventPool / 2);
@@ -1095,7 +1115,7 @@ struct Stepper {
         a.xprev = xp;
         a.hp = a.hq = nullptr;
         if (c->periodic) {
-            SolverWork &w = c->work;
+            SolverWork &w = *wk;
             launch_periodic_step(c->n_per, c->n_prow, S, c->sper_dof.p, c->sper_src.p, c->per_w.p, c->per_ell.p,
                                  c->sprow.p, c->srow.p, c->scol.p, c->spval.p, w.sc.p, xc,
                                  w.coefP.p + (size_t)k * S * 3, omt, w.ybc.p, w.bper.p, s);
@@ -1130,7 +1150,7 @@ struct Stepper {
     // Solve A_k y = rhs (complex-symmetric A_k) starting from y; one signal.
     void solve_given(int k, const double2 *rhs, double2 *y) {
         begin_step(k);
-        DMRI_HIP(hipMemcpyAsync(c->work.b.p, rhs, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        DMRI_HIP(hipMemcpyAsync(wk->b.p, rhs, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
         a.x = y;
         launch_rhs_residual(geo, a, 1, nullptr, s);
         krylov(k);
@@ -1142,15 +1162,14 @@ struct Stepper {
         begin_step(k);
         a.x = y;
         launch_rhs_residual(geo, a, 3, nullptr, s);
-        DMRI_HIP(hipMemcpyAsync(out, c->work.b.p, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        DMRI_HIP(hipMemcpyAsync(out, wk->b.p, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
         a.x = xc;
     }
 
+    // Signals of the current state u^n (the state buffers are left as they are, so stepping may go on)
     void signal(double *sig) {
-        SolverWork &w = c->work;
-        if (xc != w.x.p)
-            DMRI_HIP(hipMemcpyAsync(w.x.p, xc, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
-        launch_signal(n, S, c->sw.p, w.sc.p, w.x.p, w.spart.p, w.sig.p, s);
+        SolverWork &w = *wk;
+        launch_signal(n, S, c->sw.p, w.sc.p, xc, w.spart.p, w.sig.p, s);
         std::vector<double> hs((size_t)S * kNV);
         DMRI_HIP(hipMemcpyAsync(hs.data(), w.sig.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
         DMRI_HIP(hipStreamSynchronize(s));
@@ -1158,7 +1177,14 @@ struct Stepper {
             sig[2 * j] = hs[(size_t)j * kNV] / c->volume;
             sig[2 * j + 1] = hs[(size_t)j * kNV + 1] / c->volume;
         }
-        c->last_S = S;
+    }
+
+    // Move the final state into the workspace's x buffer (dmri_get_solution reads it there).
+    void park() {
+        SolverWork &w = *wk;
+        if (xc != w.x.p)
+            DMRI_HIP(hipMemcpyAsync(w.x.p, xc, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        xc = w.x.p;
     }
 };
 
@@ -1182,6 +1208,8 @@ void solve_group(dmri_ctx *c, int S, int j0, int nsig, const double *q, const do
         }
     }
     run.signal(sig);
+    run.park();
+    c->last_S = S;
 }
 
 // Discrete adjoint of the theta scheme for one signal (see the C ABI comment of dmri_adjoint).
@@ -1247,6 +1275,30 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     // backward sweep
     SolverWork &w = c->work;
     const double theta = seq->theta;
+    // What each parameter multiplies in the operator.  A scalar coefficient multiplies a matrix of
+    // unit coefficient, assembled here so that the gradient does not depend on the coefficient's
+    // value (D = 0 and kappa = 0 included); a field's multiplier multiplies the assembled matrix.
+    DevArray<double> unitK, unitI, junk, ones;
+    const double *kdiff = c->val[DMRI_MAT_K].p, *idiff = c->val[DMRI_MAT_I].p;
+    const double *mdiff = c->has_iT2cell ? c->val[DMRI_MAT_R].p : c->val[DMRI_MAT_M].p;
+    if (!c->has_Dcell) {
+        const double one[2] = {1.0, 1.0};
+        ones.upload(one, 2, s);
+        unitK.alloc(c->nnz);
+        junk.alloc(c->nnz);
+        launch_assemble_rows(c->ndof, c->dof_vert.p, c->dof_comp.p, c->rowptr.p, c->col.p, c->tets.p,
+                             c->ncomp == 2 ? c->phase.p : nullptr, c->xyz.p, c->vptr.p, c->vlist.p, ones.p, nullptr,
+                             c->iT2c.p, nullptr, junk.p, unitK.p, junk.p, junk.p, junk.p, junk.p, s);
+        kdiff = unitK.p;
+    }
+    if (c->ncomp == 2 && c->n_iface > 0 && !c->has_kappa_f) {
+        DevArray<int> miss;
+        miss.alloc(1);
+        miss.zero(s);
+        unitI.alloc(c->nnz);
+        assemble_interface(c, 1.0, nullptr, unitI.p, miss.p);
+        idiff = unitI.p;
+    }
     launch_adj_init_lambda(n, c->sw.p, w.sc.p, 1.0 / c->volume, lam.p, s);
     DMRI_HIP(hipMemsetAsync(mu.p, 0, vb, s));
     // per-step contraction partials stay on the device and are summed on the host at the end
@@ -1299,8 +1351,8 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
             else
                 run.solve_given(k, lam.p, mu_cur);
             nparts_of[k] = launch_adj_contract(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p,
-                                               c->ncomp == 2 ? c->scomp.p : nullptr, c->val[DMRI_MAT_M].p,
-                                               c->val[DMRI_MAT_K].p, c->val[DMRI_MAT_I].p, w.vals.p, mu_cur,
+                                               c->ncomp == 2 ? c->scomp.p : nullptr, mdiff, kdiff, idiff, w.vals.p,
+                                               mu_cur,
                                                segp(k - base), segp(k - base + 1), theta,
                                                parts.p + (size_t)k * kMaxGrid * 14, s);
             if (bh.m > 0)
@@ -1334,20 +1386,12 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
         }
     }
     const auto t2 = std::chrono::steady_clock::now();
-    const double nan = std::numeric_limits<double>::quiet_NaN();
     for (int cc = 0; cc < 2; ++cc) {
-        if (cc >= c->ncomp) {
-            grads->dS_dD[cc] = 0.0;
-            grads->dS_dinvT2[cc] = 0.0;
-            continue;
-        }
-        // K = D_c Khat_c for a scalar D (a multiplier of the tensor field otherwise)
-        const double scale = c->has_Dcell ? 1.0 : c->D[cc];
-        grads->dS_dD[cc] = scale > 0 ? gK[cc] / scale : nan;
-        grads->dS_dinvT2[cc] = c->has_iT2cell ? nan : gM[cc];
+        const bool live = cc < c->ncomp;
+        grads->dS_dD[cc] = live ? gK[cc] : 0.0;
+        grads->dS_dinvT2[cc] = live ? gM[cc] : 0.0;
     }
-    const double kscale = c->has_kappa_f ? 1.0 : c->kappa;
-    grads->dS_dkappa = c->ncomp == 2 ? (kscale > 0 ? gI / kscale : nan) : 0.0;
+    grads->dS_dkappa = c->ncomp == 2 ? gI : 0.0;
     grads->dS_dq = gq;
     if (grads->dS_df_implicit)
         for (int k = 0; k < N; ++k) grads->dS_df_implicit[k] = dfa[k];
@@ -1437,7 +1481,7 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
     const int64_t n = c->ndof;
     const int cand[6] = {1, 2, 4, 8, 16, 32};
     SolveGeom g0 = make_geom(n, 1, S, kMapSL);
-    ensure_group_workspace(c, g0, 1, per_gdir);
+    ensure_group_workspace(c, c->work, g0, 1, per_gdir);
     SolverWork &w = c->work;
     auto &V = c->val;
     launch_prep_scale(n, c->sdiag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, nullptr,
@@ -1519,11 +1563,11 @@ void validate_u0(const dmri_ctx *c, const dmri_seq *seq) {
         require(std::isfinite(seq->u0_vertex[v]), DMRI_ERR_ARG, "u0_vertex has a non-finite value");
 }
 
-// nsig signals with per-signal q and gradient direction, in launch groups of <= 64.
-void solve_batched_impl(dmri_ctx *c, int nsig, const double *q, const double *gdir, const dmri_seq *seq,
-                        const dmri_solver_opts *opts, double *sig, dmri_stats *stats) {
+// Checks of a batched solve's inputs; returns the unit gradient directions (3 per signal).
+std::vector<double> checked_signals(dmri_ctx *c, int nsig, const double *q, const double *gdir, const dmri_seq *seq,
+                                    const dmri_solver_opts *opts) {
     require(c->assembled, DMRI_ERR_STATE, "solve: call dmri_assemble first");
-    require(nsig >= 1 && q && gdir && sig, DMRI_ERR_ARG, "solve: need nsig >= 1 and non-null q, gdir, sig_out");
+    require(nsig >= 1 && q && gdir, DMRI_ERR_ARG, "solve: need nsig >= 1 and non-null q, gdir");
     validate_seq(seq, opts);
     validate_u0(c, seq);
     require(seq->snap_every >= 0, DMRI_ERR_ARG, "snap_every must be >= 0");
@@ -1544,20 +1588,117 @@ void solve_batched_impl(dmri_ctx *c, int nsig, const double *q, const double *gd
         require(gn > 1e-10 && std::isfinite(gn), DMRI_ERR_ARG, "|g|=0: gradient direction must be non-zero");
         for (int k = 0; k < 3; ++k) g[3 * (size_t)j + k] = d[k] / gn;
     }
+    return g;
+}
+
+// true when signals j0 .. j0+S-1 do not all share one gradient direction
+bool group_per_gdir(const std::vector<double> &g, int j0, int S) {
+    bool same = true;
+    for (int j = 1; j < S && same; ++j)
+        for (int k = 0; k < 3; ++k) same = same && g[3 * (size_t)(j0 + j) + k] == g[3 * (size_t)j0 + k];
+    return !same;
+}
+
+// nsig signals with per-signal q and gradient direction, in launch groups of <= 64.
+void solve_batched_impl(dmri_ctx *c, int nsig, const double *q, const double *gdir, const dmri_seq *seq,
+                        const dmri_solver_opts *opts, double *sig, dmri_stats *stats) {
+    require(sig != nullptr, DMRI_ERR_ARG, "solve: sig_out is null");
+    const std::vector<double> g = checked_signals(c, nsig, q, gdir, seq, opts);
     set_device(c);
     dmri_stats st{};
     const auto t0 = std::chrono::steady_clock::now();
     for (int j0 = 0; j0 < nsig; j0 += kMaxSignals) {
         const int S = std::min(kMaxSignals, nsig - j0);
-        bool same = true;
-        for (int j = 1; j < S && same; ++j)
-            for (int k = 0; k < 3; ++k) same = same && g[3 * (size_t)(j0 + j) + k] == g[3 * (size_t)j0 + k];
-        solve_group(c, S, j0, nsig, q + j0, g.data() + 3 * (size_t)j0, !same, seq, opts, sig + 2 * (size_t)j0, st);
+        solve_group(c, S, j0, nsig, q + j0, g.data() + 3 * (size_t)j0, group_per_gdir(g, j0, S), seq, opts,
+                    sig + 2 * (size_t)j0, st);
     }
     const auto t1 = std::chrono::steady_clock::now();
     st.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     if (stats) *stats = st;
     c->solved = true;
+}
+
+}  // namespace
+
+// A batched solve that advances on request: every launch group keeps its state, its Krylov
+// workspace and its projection history in HBM between calls.
+struct dmri_batch {
+    struct Group {
+        SolverWork work;
+        Stepper run;
+        dmri_stats st{};
+        int j0 = 0, S = 0;
+    };
+    dmri_ctx *c = nullptr;
+    uint64_t generation = 0;
+    int nsig = 0, N = 0, done = 0;
+    dmri_solver_opts opts{};
+    std::vector<std::unique_ptr<Group>> groups;
+};
+
+namespace {
+
+void batch_begin_impl(dmri_ctx *c, int nsig, const double *q, const double *gdir, const dmri_seq *seq,
+                      const dmri_solver_opts *opts, int group, dmri_batch **out) {
+    require(out != nullptr, DMRI_ERR_ARG, "dmri_batch_begin: out is null");
+    *out = nullptr;
+    require(group >= 0 && group <= kMaxSignals, DMRI_ERR_ARG, "dmri_batch_begin: group must lie in [0, 64]");
+    require(seq != nullptr && seq->snap_every == 0, DMRI_ERR_ARG,
+            "dmri_batch_begin: snapshots are not taken by a resumable batch (snap_every must be 0)");
+    const std::vector<double> g = checked_signals(c, nsig, q, gdir, seq, opts);
+    set_device(c);
+    auto b = std::make_unique<dmri_batch>();
+    b->c = c;
+    b->generation = c->generation;
+    b->nsig = nsig;
+    b->N = seq->nsteps;
+    b->opts = *opts;
+    const int G = group > 0 ? group : kMaxSignals;
+    for (int j0 = 0; j0 < nsig; j0 += G) {
+        const int S = std::min(G, nsig - j0);
+        auto gr = std::make_unique<dmri_batch::Group>();
+        gr->j0 = j0;
+        gr->S = S;
+        gr->run.setup(c, S, q + j0, g.data() + 3 * (size_t)j0, group_per_gdir(g, j0, S), seq, &b->opts, &gr->st,
+                      &gr->work);
+        b->groups.push_back(std::move(gr));
+    }
+    DMRI_HIP(hipStreamSynchronize(c->stream));
+    *out = b.release();
+}
+
+void batch_check(dmri_batch *b) {
+    require(b != nullptr, DMRI_ERR_ARG, "null batch");
+    require(b->c->assembled && b->generation == b->c->generation, DMRI_ERR_STATE,
+            "the context was assembled again (or its inputs changed) after dmri_batch_begin");
+}
+
+void batch_advance_impl(dmri_batch *b, int nsteps, dmri_stats *stats) {
+    batch_check(b);
+    require(nsteps >= 0, DMRI_ERR_ARG, "nsteps < 0");
+    set_device(b->c);
+    for (auto &gr : b->groups) gr->st = dmri_stats{};
+    const auto t0 = std::chrono::steady_clock::now();
+    const int end = std::min(b->N, b->done + nsteps);
+    for (; b->done < end; ++b->done)
+        for (auto &gr : b->groups) gr->run.step(b->done);
+    DMRI_HIP(hipStreamSynchronize(b->c->stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (!stats) return;
+    dmri_stats st{};
+    for (auto &gr : b->groups) {
+        const dmri_stats &x = gr->st;
+        st.iters_total += x.iters_total;
+        st.iters_max = std::max(st.iters_max, x.iters_max);
+        st.restarts += x.restarts;
+        st.relres_max = std::max(st.relres_max, x.relres_max);
+        st.spmv_ms += x.spmv_ms;
+        st.spmv_launches += x.spmv_launches;
+        st.spmv_bytes = x.spmv_bytes;
+        st.precond_used = x.precond_used;
+    }
+    st.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    *stats = st;
 }
 
 }  // namespace
@@ -1671,7 +1812,21 @@ int dmri_coeffs_set(dmri_ctx *c, const dmri_coeffs *k) {
         c->Dc.upload(c->D, 2, c->stream);
         c->iT2c.upload(c->invT2, 2, c->stream);
         c->has_Dcell = k->D_cell != nullptr;
-        if (c->has_Dcell) c->Dcell.upload(k->D_cell, 9 * c->nc, c->stream);
+        c->Dcell_symmetric = true;
+        if (c->has_Dcell) {
+            for (int64_t t = 0; t < c->nc; ++t) {
+                const double *d = k->D_cell + 9 * t;
+                double mx = 0.0;
+                for (int i = 0; i < 9; ++i) {
+                    require(std::isfinite(d[i]), DMRI_ERR_ARG, "D_cell has a non-finite value");
+                    mx = std::max(mx, std::fabs(d[i]));
+                }
+                const double tol = 1e-12 * mx;
+                if (std::fabs(d[1] - d[3]) > tol || std::fabs(d[2] - d[6]) > tol || std::fabs(d[5] - d[7]) > tol)
+                    c->Dcell_symmetric = false;
+            }
+            c->Dcell.upload(k->D_cell, 9 * c->nc, c->stream);
+        }
         c->has_iT2cell = k->invT2_cell != nullptr;
         if (c->has_iT2cell) c->iT2cell.upload(k->invT2_cell, c->nc, c->stream);
         c->has_kappa_f = k->kappa_facet != nullptr;
@@ -1838,7 +1993,7 @@ int dmri_time_spmv(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t reps, do
         for (auto &x : st0) x.rho[0] = make_double2(1.0, 0.0);
         // the buffers keep the last solve's scaled values and vectors when it had this shape
         if (c->work.x.n != (size_t)c->ndof * nsig || c->work.vals.n != (size_t)c->nnz * (pg ? 4 : 2)) {
-            ensure_group_workspace(c, geo, 1, pg);
+            ensure_group_workspace(c, c->work, geo, 1, pg);
             auto &V = c->val;
             SolverWork &w = c->work;
             launch_prep_scale(c->ndof, c->sdiag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
@@ -1908,6 +2063,9 @@ int dmri_adjoint(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts,
         require(c != nullptr && grads != nullptr && sig_out != nullptr, DMRI_ERR_ARG, "null argument");
         require(c->assembled, DMRI_ERR_STATE, "dmri_adjoint: call dmri_assemble first");
         require(!c->periodic, DMRI_ERR_ARG, "dmri_adjoint: the weak pseudo-periodic condition is not supported");
+        require(!c->has_Dcell || c->Dcell_symmetric, DMRI_ERR_ARG,
+                "dmri_adjoint: a non-symmetric D_cell tensor makes the operator non-symmetric; the adjoint "
+                "sweep needs a complex-symmetric operator");
         validate_seq(seq, opts);
         validate_u0(c, seq);
         set_device(c);
@@ -1926,6 +2084,37 @@ int dmri_get_solution(dmri_ctx *c, int32_t sig, double *u) {
         launch_unscale_perm(c->ndof, c->last_S, sig, c->perm.p, c->work.sc.p, c->work.x.p, tmp.p, c->stream);
         DMRI_HIP(hipMemcpyAsync(u, tmp.p, sizeof(double2) * c->ndof, hipMemcpyDeviceToHost, c->stream));
         DMRI_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int dmri_batch_begin(dmri_ctx *c, int32_t nsig, const double *q, const double *gdir, const dmri_seq *seq,
+                     const dmri_solver_opts *opts, int32_t group, dmri_batch **out) {
+    return guarded([&] {
+        require(c != nullptr, DMRI_ERR_ARG, "null context");
+        batch_begin_impl(c, nsig, q, gdir, seq, opts, group, out);
+    });
+}
+
+int dmri_batch_advance(dmri_batch *b, int32_t nsteps, dmri_stats *stats) {
+    return guarded([&] { batch_advance_impl(b, nsteps, stats); });
+}
+
+int dmri_batch_signals(dmri_batch *b, double *sig_out, int32_t *steps_done) {
+    return guarded([&] {
+        batch_check(b);
+        require(sig_out != nullptr, DMRI_ERR_ARG, "null argument");
+        set_device(b->c);
+        for (auto &gr : b->groups) gr->run.signal(sig_out + 2 * (size_t)gr->j0);
+        if (steps_done) *steps_done = b->done;
+    });
+}
+
+int dmri_batch_end(dmri_batch *b) {
+    return guarded([&] {
+        if (!b) return;
+        (void)hipSetDevice(b->c->device);
+        (void)hipStreamSynchronize(b->c->stream);
+        delete b;
     });
 }
 

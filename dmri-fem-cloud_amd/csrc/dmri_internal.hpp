@@ -160,6 +160,8 @@ void launch_assemble_rows(int64_t n, const int *dof_vert, const int *dof_comp, c
 void launch_assemble_facets(int64_t start, int64_t count, const int *perm, const int *iface_v, const double *xyz,
                             int64_t nv, const int *dof_of, const int *rowptr, const int *col, double kappa,
                             const double *kappa_f, double *gi, int *miss, hipStream_t s);
+// out[0] += sum of cnt[0..n) in 64 bits (out zeroed by the caller)
+void launch_count_sum64(const int *cnt, int64_t n, unsigned long long *out, hipStream_t s);
 void launch_diag_and_weights(int64_t n, const int *rowptr, const int *col, const double *m, int *diag, double *w,
                              int *missing_diag, hipStream_t s);
 // deterministic sum of x[0..n) into out[0] (two kernels, no atomics); part >= kMaxGrid doubles

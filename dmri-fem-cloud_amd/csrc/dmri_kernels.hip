@@ -668,9 +668,11 @@ __global__ void __launch_bounds__(256) assemble_rows_kernel(int64_t n, const int
                 const double mab = vol * dab / 20.0;
                 m += mab;
                 rr += it2 * mab;
+                // row i = test function (vertex a), column j = trial function (vertex b):
+                // inner(D grad(u), grad(v)) gives K_ij = grad(phi_i) . D grad(phi_j)
                 double kab = 0.0;
 #pragma unroll
-                for (int d = 0; d < 3; ++d) kab += Gb[d] * (Dm[3 * d] * Ga[0] + Dm[3 * d + 1] * Ga[1] + Dm[3 * d + 2] * Ga[2]);
+                for (int d = 0; d < 3; ++d) kab += Ga[d] * (Dm[3 * d] * Gb[0] + Dm[3 * d + 1] * Gb[1] + Dm[3 * d + 2] * Gb[2]);
                 kk += vol * kab;
                 const double w = vol * dab / 120.0;
                 const double S0 = X[0][0] + X[1][0] + X[2][0] + X[3][0];
@@ -732,6 +734,15 @@ __global__ void assemble_facets_kernel(int64_t start, int64_t count, const int *
             gi[p01] -= val;
             gi[p10] -= val;
         }
+}
+
+// sum of the row counts in 64 bits (the CSR offsets are 32-bit: the caller checks the total fits)
+__global__ void count_sum64_kernel(const int *cnt, int64_t n, unsigned long long *out) {
+    unsigned long long v = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v += (unsigned long long)(unsigned)cnt[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, v);
 }
 
 __global__ void diag_weights_kernel(int64_t n, const int *rowptr, const int *col, const double *m, int *diag,
@@ -2876,6 +2887,14 @@ void launch_assemble_facets(int64_t start, int64_t count, const int *perm, const
                                                                 col, kappa, kappa_f, gi, miss);
     check_launch();
 }
+void launch_count_sum64(const int *cnt, int64_t n, unsigned long long *out, hipStream_t s) {
+    unsigned g = grid_for(n, 256);
+    if (g > 1024) g = 1024;
+    if (g == 0) g = 1;
+    count_sum64_kernel<<<g, 256, 0, s>>>(cnt, n, out);
+    check_launch();
+}
+
 void launch_diag_and_weights(int64_t n, const int *rowptr, const int *col, const double *m, int *diag, double *w,
                              int *missing_diag, hipStream_t s) {
     diag_weights_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rowptr, col, m, diag, w, missing_diag);

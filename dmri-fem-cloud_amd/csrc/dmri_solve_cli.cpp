@@ -3,7 +3,10 @@
 // Accepts the options of the reference drivers and prints their result lines:
 // Meshes: DOLFIN XML, or ASCII gmsh 2.x (.msh/.gmsh) read as dolfin-convert would convert it.
 //   GCloud style (GCloudDmriSolver.py:61-133):  -f mesh.xml -M 0|1 -b B -q Q -p kappa -d delta -D Delta
-//                                               -k dt -K D0 -T2 T2 -gdir gx gy gz
+//                                               -k dt -K D0 -T2 T2 -gdir gx gy gz -pdir px py pz
+//                                               (-pdir: weak pseudo-periodic faces normal to the axes
+//                                               with p != 0; -IsPeriodic 1, the strong form, exits
+//                                               with an error)
 //   comri style (comri/two-comp/hpc-fenics-cpp/main.cpp:581-648): -m mesh.xml -N steps -b B -q G -d delta
 //                                               -D Delta -k dt -K D0 -p kappa -v gx gy gz -j nskip
 // Extra: -phase file (per-cell phase, one integer per line, or a DOLFIN mesh_function /
@@ -267,6 +270,7 @@ void usage() {
     std::fprintf(stderr,
                  "usage: dmri-solve (-f|-m) mesh.xml|mesh.msh [-M 0|1] [-phase file] [-b B | -q Q] [-d delta] [-D Delta]\n"
                  "                  [-k dt | -N steps] [-K D0] [-p kappa] [-T2 T2] [-gdir|-v gx gy gz]\n"
+                 "                  [-pdir px py pz] [-IsPeriodic 0] [-r 0]\n"
                  "                  [-variant gcloud|comri|comri-until-T] [-theta t] [-rtol r] [-atol a]\n"
                  "                  [-maxit n] [-device d] [-convert out.xml] [-s 0|1 [-j nskip] [-o dir]]\n");
 }
@@ -281,6 +285,8 @@ int main(int argc, char **argv) {
     double b = 1000, qin = -1, kappa = 1e-5, delta = 10600, Delta = 43100, dt = 200, D0 = 3e-3, T2 = 1e16;
     double theta = 0.5, rtol = 1e-9, atol = 1e-10;
     double g[3] = {0, 1, 0};
+    int32_t pdir[3] = {0, 0, 0};  // -pdir (GCloudDmriSolver.py:126-131)
+    int is_periodic = 0, nrefine = 0;
     bool have_dt = false, have_b = false, have_q = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -317,7 +323,11 @@ int main(int argc, char **argv) {
         else if (a == "-maxit") { maxit = std::atoi(next()); ++i; }
         else if (a == "-device") { device = std::atoi(next()); ++i; }
         else if (a == "-convert") { fconvert = next(); ++i; }
-        else if (a == "-r" || a == "-IsPeriodic" || a == "-isperiodic") { ++i; }
+        else if (a == "-pdir") {
+            pdir[0] = std::atoi(next(1)); pdir[1] = std::atoi(next(2)); pdir[2] = std::atoi(next(3)); i += 3;
+        }
+        else if (a == "-IsPeriodic" || a == "-isperiodic") { is_periodic = std::atoi(next()); ++i; }
+        else if (a == "-r") { nrefine = std::atoi(next()); ++i; }
         else if (a == "-h" || a == "--help") { usage(); return 0; }
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return 2; }
     }
@@ -331,6 +341,19 @@ int main(int argc, char **argv) {
     if (fmesh.empty()) { usage(); return 2; }
     if (is_save && nskip <= 0) {
         std::fprintf(stderr, "-j nskip must be positive\n");
+        return 2;
+    }
+    if (nrefine != 0) {
+        // comri's -r refines the mesh uniformly before the solve (two-comp main.cpp:615); meshes are
+        // taken as given here
+        std::fprintf(stderr, "-r %d: mesh refinement is not supported (refine the mesh beforehand)\n", nrefine);
+        return 2;
+    }
+    const bool periodic = pdir[0] || pdir[1] || pdir[2];
+    if (is_periodic && periodic) {
+        // IsDomainPeriodic = 1 selects the strong pseudo-periodic form (PeriodicBD, DmriFemLib.py:327-375)
+        std::fprintf(stderr, "-IsPeriodic 1: the strong pseudo-periodic formulation is not available; "
+                             "-IsPeriodic 0 with -pdir imposes the weak one\n");
         return 2;
     }
     const bool comri = variant != "gcloud";
@@ -360,15 +383,21 @@ int main(int argc, char **argv) {
     }
     if (comri && !have_dt) dt = T / nsteps;
 
-    // time profile per step
-    std::vector<double> fA, fB;
+    // time profile per step, and F = int_0^t f at the explicit side's time (the weak periodic terms'
+    // phase: itime_profile(tp), DmriFemLib.py:901-902)
+    std::vector<double> fA, fB, FB;
     auto f_py = [&](double s) { return s < delta ? 1.0 : (s < Delta ? 0.0 : (s < T ? -1.0 : 0.0)); };
+    auto F_py = [&](double s) { return s < delta ? s : (s < Delta ? delta : (s < T ? delta - (s - Delta) : 0.0)); };
     auto f_c = [&](double t) { return 1.0 * (t >= 0 && t < delta) - 1.0 * (t >= Delta && t <= Delta + delta); };
+    auto F_c = [&](double t) {
+        return t < 0 ? 0.0 : (t < delta ? t : (t < Delta ? delta : (t <= Delta + delta ? delta - (t - Delta) : 0.0)));
+    };
     if (!comri) {
         double t = 0, tp = 0;
         while (t < T + dt) {
             fA.push_back(f_py(t));
             fB.push_back(f_py(tp));
+            FB.push_back(F_py(tp));
             tp = t;
             t += dt;
         }
@@ -377,6 +406,7 @@ int main(int argc, char **argv) {
         const double end = variant == "comri-until-T" ? T : T + dt;
         while (t < end) {
             fA.push_back(f_c(t));
+            FB.push_back(F_c(t));
             t += dt;
         }
         fB = fA;
@@ -423,6 +453,7 @@ int main(int argc, char **argv) {
     co.invT2[0] = co.invT2[1] = 1.0 / T2;
     co.kappa = kappa;
     check(dmri_coeffs_set(ctx, &co), "dmri_coeffs_set");
+    if (periodic) check(dmri_periodic_set(ctx, pdir), "dmri_periodic_set");
     check(dmri_assemble(ctx), "dmri_assemble");
     dmri_seq seq{};
     seq.nsteps = (int32_t)fA.size();
@@ -434,6 +465,7 @@ int main(int argc, char **argv) {
     seq.gdir[0] = g[0];
     seq.gdir[1] = g[1];
     seq.gdir[2] = g[2];
+    if (periodic) seq.F_explicit = FB.data();
     dmri_info info0{};
     check(dmri_get_info(ctx, &info0), "dmri_get_info");
     std::vector<double> snaps;

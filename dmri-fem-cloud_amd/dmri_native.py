@@ -74,7 +74,8 @@ EXPORTED = ["dmri_create", "dmri_destroy", "dmri_last_error", "dmri_version", "d
             "dmri_assemble", "dmri_get_info", "dmri_export_pattern", "dmri_export_values", "dmri_export_facets",
             "dmri_periodic_set", "dmri_export_periodic",
             "dmri_solve", "dmri_solve_batched", "dmri_tune", "dmri_set_tuning", "dmri_time_spmv", "dmri_copy_bandwidth",
-            "dmri_adjoint", "dmri_get_solution"]
+            "dmri_adjoint", "dmri_get_solution", "dmri_batch_begin", "dmri_batch_advance", "dmri_batch_signals",
+            "dmri_batch_end"]
 
 
 def _load():
@@ -105,6 +106,10 @@ def _load():
     L.dmri_time_spmv.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, dp, dp, dp]
     L.dmri_copy_bandwidth.argtypes = [vp, i64, ctypes.c_int32, dp]
     L.dmri_adjoint.argtypes = [vp, P(_Seq), P(_Opts), P(_AdjOpts), dp, P(_Grads), P(_Stats)]
+    L.dmri_batch_begin.argtypes = [vp, ctypes.c_int32, dp, dp, P(_Seq), P(_Opts), ctypes.c_int32, P(vp)]
+    L.dmri_batch_advance.argtypes = [vp, ctypes.c_int32, P(_Stats)]
+    L.dmri_batch_signals.argtypes = [vp, dp, i32p]
+    L.dmri_batch_end.argtypes = [vp]
     for name in EXPORTED:
         if name not in ("dmri_last_error", "dmri_version"):
             getattr(L, name).restype = ctypes.c_int
@@ -132,6 +137,7 @@ class Context:
 
     def __init__(self, device: int = 0):
         self._h = ctypes.c_void_p()
+        self.has_fields = False
         _check(LIB.dmri_create(ctypes.byref(self._h), int(device)))
 
     def close(self):
@@ -167,6 +173,7 @@ class Context:
         c.invT2[0], c.invT2[1] = float(Tv[0]), float(Tv[1])
         c.kappa = float(kappa)
         self._keep = []
+        self.has_fields = any(a is not None for a in (D_cell, invT2_cell, kappa_facet))
         for name, arr in (("D_cell", D_cell), ("invT2_cell", invT2_cell), ("kappa_facet", kappa_facet)):
             if arr is not None:
                 a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
@@ -308,6 +315,20 @@ class Context:
             stats["snapshots"] = snaps
         return sig.view(np.complex128), stats
 
+    def batch(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
+              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None, group=0):
+        """Resumable batched solve (dmri_batch_begin): returns a Batch whose advance(k) runs k more
+        theta steps of every signal and whose signals() gives the signals of the current states."""
+        qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(gdirs, dtype=np.float64).reshape(-1, 3),
+                                                 (qs.shape[0], 3)))
+        s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
+                                    lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0))
+        h = ctypes.c_void_p()
+        _check(LIB.dmri_batch_begin(self._h, qs.shape[0], _ptr(qs, ctypes.c_double), _ptr(g, ctypes.c_double),
+                                    ctypes.byref(s), ctypes.byref(o), int(group), ctypes.byref(h)))
+        return Batch(self, h, qs.shape[0], s.nsteps)
+
     MAPPINGS = {1: "spread", 2: "spread-staged", 3: "registers", 4: "wave-per-row", 5: "tile-compressed"}
 
     def tune(self, nsig: int, per_gdir: bool = False):
@@ -363,3 +384,39 @@ class Context:
         u = np.zeros(2 * n)
         _check(LIB.dmri_get_solution(self._h, int(sig), _ptr(u, ctypes.c_double)))
         return u.view(np.complex128)
+
+
+class Batch:
+    """A resumable batched solve of a Context (see dmri_batch_begin in include/dmri.h)."""
+
+    def __init__(self, ctx: Context, handle, nsig: int, nsteps: int):
+        self.ctx, self._h, self.nsig, self.nsteps = ctx, handle, nsig, nsteps
+
+    def advance(self, nsteps: int = 1) -> dict:
+        st = _Stats()
+        _check(LIB.dmri_batch_advance(self._h, int(nsteps), ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    def signals(self):
+        """(complex signals of the current states, theta steps done)."""
+        sig = np.zeros(2 * self.nsig)
+        done = ctypes.c_int32()
+        _check(LIB.dmri_batch_signals(self._h, _ptr(sig, ctypes.c_double), ctypes.byref(done)))
+        return sig.view(np.complex128), done.value
+
+    def close(self):
+        if self._h:
+            LIB.dmri_batch_end(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -23,6 +23,12 @@ def _np(t):
 class BlochTorreySignal(torch.autograd.Function):
     @staticmethod
     def forward(ctx, D, invT2, kappa, q, f_implicit, f_explicit, solver, dt, gdir, theta, rtol, stride):
+        if getattr(solver, "has_fields", False):
+            # the scalar inputs below would replace the context's D_cell / invT2_cell / kappa_facet
+            # fields and change the physics; the field multipliers' gradients are returned by
+            # Context.adjoint directly
+            raise ValueError("bloch_torrey_signal takes scalar coefficients; this context holds per-cell or "
+                             "per-facet coefficient fields (use Context.adjoint for their multiplier gradients)")
         Dv, Tv = _np(D).reshape(2), _np(invT2).reshape(2)
         solver.set_coeffs(D=tuple(Dv), invT2=tuple(Tv), kappa=float(_np(kappa)))
         solver.assemble()

@@ -74,7 +74,9 @@ int dmri_mesh_set(dmri_ctx *ctx, int64_t nv, const double *xyz, int64_t nc, cons
  * comri kcoeff/kappa coefficients (comri/two-comp/hpc-fenics-cpp/main.cpp:741). */
 typedef struct {
     double D[2];                /* scalar diffusivity per compartment */
-    const double *D_cell;       /* optional nc*9 row-major tensor per cell (overrides D) */
+    const double *D_cell;       /* optional nc*9 row-major tensor per cell (overrides D); K_ij =
+                                   int grad(phi_i) . D grad(phi_j), row i the test function, as
+                                   inner(D*grad(u), grad(v)) of FuncF_wBC (DmriFemLib.py:43-44) */
     double invT2[2];            /* 1/T2 per compartment (reference default T2 = 1e16) */
     const double *invT2_cell;   /* optional nc values (overrides invT2) */
     double kappa;               /* interface permeability */
@@ -242,6 +244,27 @@ int dmri_solve(dmri_ctx *ctx, const dmri_seq *seq, const dmri_solver_opts *opts,
 int dmri_solve_batched(dmri_ctx *ctx, int32_t nsig, const double *q, const double *gdir, const dmri_seq *seq,
                        const dmri_solver_opts *opts, double *sig_out, dmri_stats *stats);
 
+/* Resumable batched solve: the signals of dmri_solve_batched advanced a given number of theta
+ * steps per call, every launch group (up to `group` signals, 0 = 64) keeping its state, its
+ * Krylov workspace and its initial-guess history in HBM between calls.  dmri_batch_begin checks
+ * the inputs, scales the system and sets u0 (the time profile is copied to the device, so the
+ * caller's arrays may be freed after it returns; seq->snap_every must be 0); dmri_batch_advance
+ * runs up to nsteps more theta steps of every signal (fewer at the end of the sequence; stats as
+ * in dmri_solve, solve_ms = this call); dmri_batch_signals gives the signals of the current states
+ * (2*nsig doubles, as sig_out of dmri_solve_batched; after the last step they equal its result)
+ * and the number of steps done; dmri_batch_end frees the batch.  A batch stays bound to its
+ * context, which must not be assembled again (or given new inputs) while the batch lives
+ * (DMRI_ERR_STATE), and every batch must be ended before its context is destroyed.  Several
+ * batches of one context run one after another on its stream.
+ * No reference counterpart: the reference runs each signal's time loop to its end in one call
+ * (MRI_simulation.solve, DmriFemLib.py:878-915). */
+typedef struct dmri_batch dmri_batch;
+int dmri_batch_begin(dmri_ctx *ctx, int32_t nsig, const double *q, const double *gdir, const dmri_seq *seq,
+                     const dmri_solver_opts *opts, int32_t group, dmri_batch **out);
+int dmri_batch_advance(dmri_batch *batch, int32_t nsteps, dmri_stats *stats);
+int dmri_batch_signals(dmri_batch *batch, double *sig_out, int32_t *steps_done);
+int dmri_batch_end(dmri_batch *batch);
+
 /* Autotuner: time the two SpMV kernels of a Krylov iteration for every valid (mapping 1..5,
  * lanes per row 1..32) on the assembled pattern (us[30] = [mapping-1][log2 lanes] microseconds,
  * -1 = not valid) and keep the fastest for later solves of this context whose options leave
@@ -279,6 +302,8 @@ int dmri_copy_bandwidth(dmri_ctx *ctx, int64_t bytes, int32_t reps, double *gbps
  * checkpoint_stride >= N keeps every state instead.  0 = automatic:
  * every state when the N+1 states fit in a quarter of the free device memory (then no step is
  * recomputed), else checkpoints every ceil(sqrt(N)) steps.
+ * A D_cell field must hold symmetric tensors (DMRI_ERR_ARG otherwise: the operator is then not
+ * complex symmetric).
  * No reference counterpart: the reference computes signals only. */
 typedef struct {
     int32_t checkpoint_stride;
@@ -286,9 +311,12 @@ typedef struct {
 } dmri_adjoint_opts;
 
 typedef struct {
-    double dS_dD[2];        /* per compartment: d/dD_c for scalar D, d/d(multiplier) for a D_cell field */
-    double dS_dinvT2[2];    /* per compartment d/d(1/T2_c) (NaN with invT2_cell) */
-    double dS_dkappa;       /* d/dkappa (scalar kappa) or d/d(multiplier) of kappa_facet; 0 for one compartment */
+    double dS_dD[2];        /* per compartment: d/dD_c for scalar D (also at D_c = 0), or d/dm_c where the
+                               D_cell tensors of compartment c's cells are scaled by m_c (at m_c = 1) */
+    double dS_dinvT2[2];    /* per compartment: d/d(1/T2_c) for scalar 1/T2, or d/dm_c of the invT2_cell
+                               values of compartment c's cells scaled by m_c (at m_c = 1) */
+    double dS_dkappa;       /* d/dkappa for scalar kappa (also at kappa = 0), or d/dm of kappa_facet scaled
+                               by m (at m = 1); 0 for one compartment */
     double dS_dq;           /* at fixed time profile */
     double *dS_df_implicit; /* optional out, nsteps: d/d f_implicit[n] at fixed q */
     double *dS_df_explicit; /* optional out, nsteps: d/d f_explicit[n] at fixed q */

@@ -423,16 +423,18 @@ int orc_assemble(int64_t nv, const double *xyz, int64_t nc, const int32_t *tets,
         for (int a = 0; a < 4; ++a)
             for (int q = 0; q < 3; ++q) S[q] += X[a][q];
         for (int a = 0; a < 4; ++a) {
-            double DG[3];
-            for (int q = 0; q < 3; ++q) DG[q] = D[3 * q] * G[a][0] + D[3 * q + 1] * G[a][1] + D[3 * q + 2] * G[a][2];
             for (int b = 0; b < 4; ++b) {
+                /* row = test function phi_a, column = trial phi_b: inner(D grad(u), grad(v)) gives
+                   K_ab = grad(phi_a) . D grad(phi_b) (FuncF_wBC, DmriFemLib.py:43-44) */
+                double DG[3];
+                for (int q = 0; q < 3; ++q) DG[q] = D[3 * q] * G[b][0] + D[3 * q + 1] * G[b][1] + D[3 * q + 2] * G[b][2];
                 int64_t pos = csr_find(rowptr, col, d[a], d[b]);
                 if (pos < 0) return -3;
                 double dab = (a == b) ? 2.0 : 1.0;
                 double mab = vol * dab / 20.0;
                 m[pos] += mab;
                 r[pos] += it2 * mab;
-                k[pos] += vol * (G[b][0] * DG[0] + G[b][1] * DG[1] + G[b][2] * DG[2]);
+                k[pos] += vol * (G[a][0] * DG[0] + G[a][1] * DG[1] + G[a][2] * DG[2]);
                 double w = vol * dab / 120.0;
                 jx[pos] += w * (S[0] + X[a][0] + X[b][0]);
                 jy[pos] += w * (S[1] + X[a][1] + X[b][1]);

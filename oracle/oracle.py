@@ -137,6 +137,59 @@ def q_from_b_pgse(b, delta, Delta):
     return math.sqrt(b) / math.sqrt(delta * delta * (Delta - delta / 3.0))
 
 
+def cos_ogse(nperiod, delta, Delta, t0=100.0, seq_ext=100.0):
+    """cos-OGSE, profile 3 of ArbitraryTimeSequence.ipynb:380-392, evaluated in floats:
+    omega = 2 pi nperiod / delta, tau = (Delta + delta)/2, T = Delta + delta + t0 + seq_ext and the
+    Piecewise branches (0, s < t0), (cos(omega (s - t0)), s <= delta + t0), (0, s <= tau + t0),
+    (-cos(omega (s - t0 - tau)), s <= delta + tau + t0), (0, True).  Returns (f, T)."""
+    om = 2.0 * nperiod * math.pi / delta
+    tau = 0.5 * (Delta + delta)
+    T = Delta + delta + t0 + seq_ext
+
+    def f(s):
+        if s < t0:
+            return 0.0
+        if s <= delta + t0:
+            return math.cos(om * (s - t0))
+        if s <= tau + t0:
+            return 0.0
+        if s <= delta + tau + t0:
+            return -math.cos(om * (s - t0 - tau))
+        return 0.0
+
+    return f, T
+
+
+def cos_ogse_integral(nperiod, delta, Delta, t0=100.0):
+    """F(t) = int_0^t f of cos_ogse (MRI_parameters.itime_profile_sym, DmriFemLib.py:829-832, for
+    that profile): sin(omega (t - t0))/omega on the first lobe, held at its end value until the
+    second lobe, which subtracts sin(omega (t - t0 - tau))/omega."""
+    om = 2.0 * nperiod * math.pi / delta
+    tau = 0.5 * (Delta + delta)
+    F1 = math.sin(om * delta) / om
+
+    def F(t):
+        if t < t0:
+            return 0.0
+        if t <= delta + t0:
+            return math.sin(om * (t - t0)) / om
+        if t <= tau + t0:
+            return F1
+        if t <= delta + tau + t0:
+            return F1 - math.sin(om * (t - t0 - tau)) / om
+        return F1 - math.sin(om * delta) / om
+
+    return F
+
+
+def int_F2_cos_ogse(nperiod, delta, Delta):
+    """int_0^T F(s)^2 ds of cos_ogse (MRI_parameters.integral_term_for_gb, DmriFemLib.py:826-827) for
+    whole periods (F returns to 0 after each lobe and the lobes do not overlap, tau >= delta):
+    2 int_0^delta sin^2(omega u)/omega^2 du = delta / omega^2."""
+    om = 2.0 * nperiod * math.pi / delta
+    return delta / (om * om)
+
+
 def gcloud_sequence(f, T, k):
     """(fA, fB) of MRI_simulation.solve: loop while t < T + k, t += k, RHS lagged
     (tp starts at 0), DmriFemLib.py:886-911."""

@@ -167,3 +167,30 @@ def test_comri_snapshots_written_as_vtk(tmp_path):
         err = np.abs(got - ref).max() / np.abs(ref).max()
         print(f"  snapshot {j}: rel err {err:.2e}", flush=True)
         assert err <= 1e-10
+
+
+def test_gcloud_periodic_box_matches_oracle(tmp_path):
+    """-pdir px py pz (GCloudDmriSolver.py:126-131) imposes the weak pseudo-periodic condition; the
+    normalised signal equals the oracle's periodic solve with F(t_{n-1}) of the PGSE."""
+    m = dm.box_mesh((0.0, 0.0, 0.0), (8.0, 5.0, 5.0), 8, 5, 5)
+    dm.write_dolfin_xml(m, str(tmp_path / "box.xml"))
+    out = _run(["-f", "box.xml", "-M", 0, "-K", 2e-3, "-b", 1500, "-d", 1000, "-D", 6000, "-k", 100,
+                "-gdir", 1, 0.5, 0, "-pdir", 1, 1, 0, "-rtol", 1e-13, "-atol", 0], tmp_path)
+    mm = re.search(r"Normalized signal: ([0-9.e+-]+)", out)
+    assert mm, out
+    S = O.System(m.xyz, m.tets, None, D=2e-3, periodic=(1, 1, 0))
+    f = O.pgse_piecewise(1000.0, 6000.0)
+    fA, fB = O.gcloud_sequence(f, 7000.0, 100.0)
+    F = O.pgse_integral(1000.0, 6000.0)
+    _, FBv = O.gcloud_sequence(F, 7000.0, 100.0)
+    ref, _, _ = S.solve((1, 0.5, 0), O.q_from_b_pgse(1500.0, 1000.0, 6000.0), 100.0, fA, fB, rtol=1e-13, FB=FBv)
+    assert mm.group(1) == f"{ref:.6e}"
+
+
+def test_cli_rejects_strong_periodic_and_refinement(tmp_path):
+    m = dm.box_mesh((0.0,) * 3, (2.0,) * 3, 2, 2, 2)
+    dm.write_dolfin_xml(m, str(tmp_path / "box.xml"))
+    for extra in (["-pdir", 1, 0, 0, "-IsPeriodic", 1], ["-r", 1]):
+        r = subprocess.run([CLI, "-f", "box.xml", "-K", 2e-3] + [str(a) for a in extra], capture_output=True,
+                           text=True, cwd=tmp_path, timeout=60)
+        assert r.returncode != 0 and ("strong" in r.stderr or "refinement" in r.stderr), r.stderr

@@ -440,3 +440,138 @@ def test_periodic_physics_known_answers():
     s_neu = O.System(m.xyz, m.tets, None, D=3e-3).solve((1, 0, 0), q, k, fA, fB, rtol=1e-13)[0]
     free = math.exp(-500.0 * 3e-3)
     assert abs(s_per - free) < 0.1 * abs(s_neu - free)
+
+
+def _notebook_cos_ogse(nperiod, delta, Delta):
+    """Profile 3 of ArbitraryTimeSequence.ipynb:380-392 as the notebook builds it (sympy Piecewise,
+    t0 = seq_ext = 100), with MRI_parameters' symbolic integrals (DmriFemLib.py:826-832)."""
+    import sympy as sp
+
+    s, u = sp.Symbol("s"), sp.Symbol("u")
+    t0, seq_ext = 100, 100
+    Dd = Delta + delta
+    T = Dd + t0 + seq_ext
+    omega = 2.0 * nperiod * math.pi / delta
+    tau = Dd / 2.0
+    fs = sp.Piecewise((0.0, s < t0), (sp.cos(omega * (s - t0)), s <= delta + t0), (0.0, s <= tau + t0),
+                      (-sp.cos(omega * (s - t0 - tau)), s <= delta + tau + t0), (0.0, True))
+    ifs = sp.integrate(fs.subs(s, u), (u, 0, s))
+    i4 = float(sp.integrate(ifs * ifs, (s, 0, T)))
+    return s, fs, ifs, i4, T
+
+
+@pytest.mark.parametrize("nperiod,delta,Delta", [(4, 19900.0, 19900.0), (1, 10000.0, 30000.0)])
+def test_cos_ogse_matches_sympy_profile_and_integrals(nperiod, delta, Delta):
+    import dmri_sequences as DS
+
+    s, fs, ifs, i4, T = _notebook_cos_ogse(nperiod, delta, Delta)
+    f, T_o = O.cos_ogse(nperiod, delta, Delta)
+    F = O.cos_ogse_integral(nperiod, delta, Delta)
+    fl, T_l = DS.cos_ogse(nperiod, delta, Delta)
+    Fl = DS.cos_ogse_integral(nperiod, delta, Delta)
+    assert T_o == T and T_l == T
+    # every sample of the GCloud loop at dt = 100 us, plus the branch ends (inclusive there)
+    tau = 0.5 * (Delta + delta)
+    ts = list(np.arange(0.0, T + 100.0, 100.0)) + [100.0, delta + 100.0, tau + 100.0, delta + tau + 100.0,
+                                                   99.999, delta + 100.001]
+    for t in ts:
+        ref_f = float(fs.subs(s, t))
+        ref_F = float(ifs.subs(s, t))
+        assert abs(f(t) - ref_f) <= 1e-12 and abs(fl(t) - ref_f) <= 1e-12, t
+        assert abs(F(t) - ref_F) <= 1e-9 * (1 + abs(ref_F)) and abs(Fl(t) - ref_F) <= 1e-9 * (1 + abs(ref_F)), t
+    # q = sqrt(b / int_0^T F^2): the closed form against sympy's integral
+    assert abs(O.int_F2_cos_ogse(nperiod, delta, Delta) - i4) <= 1e-10 * i4
+    assert abs(DS.cos_ogse_int_F2(nperiod, delta, Delta) - i4) <= 1e-10 * i4
+    # the 4-period workload: T = 40 ms and 401 theta steps at dt = 100 us (t < T + k)
+    if nperiod == 4:
+        fA, fB = O.gcloud_sequence(f, T, 100.0)
+        assert T == 40000.0 and len(fA) == 401 and fA[0] == 0.0 and fA[1] == 1.0
+        gA, gB, GB = DS.gcloud_loop(fl, T, 100.0, Fl)
+        assert np.array_equal(gA, fA) and np.array_equal(gB, fB) and GB[0] == 0.0
+
+
+def test_waveform_module_pgse_matches_oracle():
+    import dmri_sequences as DS
+
+    f_o, f_l = O.pgse_piecewise(10600.0, 43100.0), DS.pgse(10600.0, 43100.0)
+    F_o, F_l = O.pgse_integral(10600.0, 43100.0), DS.pgse_integral(10600.0, 43100.0)
+    for t in np.arange(0.0, 54000.0, 50.0):
+        assert f_o(t) == f_l(t) and F_o(t) == F_l(t)
+    assert abs(DS.q_from_b(1000.0, DS.pgse_int_F2(10600.0, 43100.0)) - O.q_from_b_pgse(1000.0, 10600.0, 43100.0)) \
+        <= 1e-18
+
+
+def _cell_gradients(X):
+    E = (X[1:] - X[0]).T
+    Einv = np.linalg.inv(E)
+    G = np.zeros((4, 3))
+    G[1:] = Einv
+    G[0] = -Einv.sum(axis=0)
+    return G
+
+
+def test_cell_fields_nonsymmetric_tensor_and_facet_kappa_match_quadrature():
+    """D_cell (non-symmetric tensors), invT2_cell and kappa_facet branches of orc_assemble against a
+    per-cell quadrature assembly in Python: K_ab = int grad(phi_a) . D_t grad(phi_b) (row a the test
+    function, inner(D*grad(u), grad(v)) of FuncF_wBC), R = sum_t invT2_t M_t, I per facet."""
+    m, phase = _tiny_two_comp()
+    rng = np.random.default_rng(5)
+    nc = m.nc
+    Dcell = rng.normal(size=(nc, 3, 3)) * 1e-3 + np.eye(3)[None] * 3e-3
+    assert np.abs(Dcell - Dcell.transpose(0, 2, 1)).max() > 1e-4  # genuinely non-symmetric
+    it2 = rng.uniform(1e-5, 1e-4, size=nc)
+    S0 = O.System(m.xyz, m.tets, phase)
+    kf = rng.uniform(1e-6, 1e-4, size=S0.fv.shape[0])
+    S = O.System(m.xyz, m.tets, phase, Dcell=Dcell.reshape(nc, 9), invT2cell=it2, kappa_facet=kf)
+    n = S.ndof
+    K = np.zeros((n, n))
+    R = np.zeros((n, n))
+    for t in range(nc):
+        X = m.xyz[m.tets[t]]
+        pts, w, lam, _ = tet_quadrature(X, n=3)
+        Mt = np.einsum("q,qa,qb->ab", w, lam, lam)
+        G = _cell_gradients(X)
+        Kt = w.sum() * G @ Dcell[t] @ G.T
+        d = [S.dof_of[phase[t] * S.nv + v] for v in m.tets[t]]
+        K[np.ix_(d, d)] += Kt
+        R[np.ix_(d, d)] += it2[t] * Mt
+    I = np.zeros((n, n))
+    for f, F in enumerate(S.fv):
+        wq, lamq = tri_quadrature(m.xyz[F], n=3)
+        Fm = kf[f] * np.einsum("q,qa,qb->ab", wq, lamq, lamq)
+        d0 = [S.dof_of[v] for v in F]
+        d1 = [S.dof_of[S.nv + v] for v in F]
+        I[np.ix_(d0, d0)] += Fm
+        I[np.ix_(d1, d1)] += Fm
+        I[np.ix_(d0, d1)] -= Fm
+        I[np.ix_(d1, d0)] -= Fm
+    assert np.abs(S.dense("k") - K).max() <= 1e-13 * np.abs(K).max()
+    assert np.abs(S.dense("k") - S.dense("k").T).max() > 1e-6  # the transpose would fail the line above
+    assert np.abs(S.dense("r") - R).max() <= 1e-13 * np.abs(R).max()
+    assert np.abs(S.dense("gi") - I).max() <= 1e-13 * np.abs(I).max()
+
+
+def _brute_interface_facets(tets, phase):
+    """Every face of every cell keyed by its sorted vertex triple; a face shared by two cells of
+    different phase is an interface facet (abs(jump(phase)) dS in icondition_wBC)."""
+    faces = {}
+    for t, T in enumerate(tets):
+        for skip in range(4):
+            key = tuple(sorted(int(T[i]) for i in range(4) if i != skip))
+            faces.setdefault(key, []).append(t)
+    out = [k for k, cells in faces.items() if len(cells) == 2 and phase[cells[0]] != phase[cells[1]]]
+    assert all(len(c) <= 2 for c in faces.values())
+    return np.array(sorted(out), dtype=np.int32).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("maker", ["box", "cyl"])
+def test_interface_facets_match_brute_force_enumeration(maker):
+    if maker == "box":
+        m, phase = _tiny_two_comp()
+    else:
+        m = dm.cylinder_mesh(6.0, 3.0, 1500, seed=17, interface_radii=(3.0,))
+        phase = m.phase
+    S = O.System(m.xyz, m.tets, phase)
+    ref = _brute_interface_facets(m.tets, phase)
+    assert ref.shape[0] > 10
+    assert np.array_equal(S.fv, ref)
