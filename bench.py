@@ -26,11 +26,12 @@ Other workloads (whole jobs per bench step: device assembly + every signal's ful
   c3  the c1 geometry at ~300k vertices, b=2000, T2=80 ms: the adjoint (forward + checkpointed
       reverse sweep with every gradient) against the forward solve alone.
 
-Also reported: the SpMV roofline and the CPU baseline.  The SpMV launches of the first
---event-iters Krylov iterations of every theta step inside the timed region are bracketed by HIP
+Also reported: the roofline and the CPU baseline.  Every kernel of the first --event-iters
+Krylov iterations of every theta step inside the timed region is bracketed by kernel-bound HIP
 events on the solver's stream (sampled so that the events, which cost GPU time of their own,
-barely perturb the timed work); achieved = algorithmic bytes per launch / mean launch time.
-After the timed region the same two kernels are replayed back to back on the solver's buffers
+barely perturb the timed work); each kernel's achieved bandwidth = its algorithmic bytes per
+launch / its mean launch time, and "roofline" is that of the kernel with the most device time.
+After the timed region the two operator SpMVs are replayed back to back on the solver's buffers
 (no per-launch events) as a second figure that compares directly with a kernel trace.  CPU
 baseline: the FP64 CPU oracle on the host cores, bounded sample (c4: theta steps 1..--cpu-steps-1
 of one signal after an untimed step 0, and step 1 again on one core; c1/c2: the first --cpu-steps
@@ -86,13 +87,71 @@ def cos_ogse(nperiod, delta, Delta, t0=100.0, ext=100.0):
 
 fibonacci_directions = DS.fibonacci_directions
 
+KERNEL_NAMES = ("bicg_p", "spmv_v_pre", "spmv_v", "spmv_t_pre", "spmv_t", "bicg_update")
+KERNEL_DESC = {"bicg_p": "p = r + beta (p - omega v)", "spmv_v_pre": "ph = c (2 p - c A p)", "spmv_v": "v = A ph",
+               "spmv_t_pre": "s = r - alpha v, sh = c (2 s - c A s)", "spmv_t": "t = A sh",
+               "bicg_update": "x += alpha ph + omega sh, r = s - omega t"}
+
+
+def merge_stats(stats):
+    """Sum the per-call solver statistics (max for the per-step maxima, element-wise for the per-kernel
+    arrays)."""
+    out = dict(stats[0])
+    for key in out:
+        vals = [s_[key] for s_ in stats]
+        if key in ("iters_max", "relres_max"):
+            out[key] = max(vals)
+        elif key.startswith("kern_"):
+            out[key] = [sum(v[j] for v in vals) for j in range(len(vals[0]))]
+        elif key not in ("spmv_bytes", "precond_used", "reserved"):
+            out[key] = sum(vals)
+    return out
+
+
+def kernel_table(st):
+    """Per-kernel device time and algorithmic bandwidth of the event-timed Krylov iterations."""
+    total = sum(st["kern_ms"]) or 1.0
+    out = {}
+    for j, name in enumerate(KERNEL_NAMES):
+        nl = st["kern_launches"][j]
+        if nl == 0:
+            continue
+        us = 1e3 * st["kern_ms"][j] / nl
+        b = st["kern_bytes"][j] / nl
+        out[name] = {"what": KERNEL_DESC[name], "launches": nl, "avg_us": us, "bytes_per_launch": b,
+                     "achieved_gbs": b / (us * 1e-6) / 1e9, "frac_of_peak": b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                     "share_of_iteration_time": st["kern_ms"][j] / total}
+    return out
+
+
+def roofline_block(st, copy_gbs, traffic, S_group, event_iters):
+    """The roofline of the dominant kernel (most device time over the timed iterations) and the table
+    of every kernel of an iteration."""
+    table = kernel_table(st)
+    if not table:
+        return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None}
+    dom = max(table, key=lambda k: table[k]["avg_us"] * table[k]["launches"])
+    d = table[dom]
+    t = traffic.get(dom) if traffic else None
+    return {"bound": "hbm", "achieved": d["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": d["achieved_gbs"] / HBM_PEAK_GBS, "traffic": t["bytes"] if t else None,
+            "traffic_source": t["source"] if t else None,
+            "kernel": f"{dom}: {KERNEL_DESC[dom]} (batched complex CSR, {S_group} signals per launch)",
+            "avg_launch_us": d["avg_us"], "bytes_per_launch": d["bytes_per_launch"], "launches": d["launches"],
+            "events": f"every kernel of the first {event_iters} Krylov iteration(s) of every theta step, "
+                      "kernel-bound HIP events on the solver's stream",
+            "copy_kernel_gbs": copy_gbs, "frac_of_copy": (d["achieved_gbs"] / copy_gbs) if copy_gbs else None,
+            "kernels": table}
+
 
 TRAFFIC_FILES = {"c1": "spmv_traffic_cylinders200k_5sig.json", "c4": "spmv_traffic_neuron2m_32sig.json"}
 
 
 def measured_traffic(info, mapping_lanes, name="c1"):
-    """HBM bytes per SpMV launch from the committed FETCH_SIZE/WRITE_SIZE counter runs of this
-    workload (copy-kernel calibrated), if the mesh and SpMV mapping are the ones measured."""
+    """HBM bytes per launch of each iteration kernel from the committed FETCH_SIZE/WRITE_SIZE counter
+    runs of this workload (copy-kernel calibrated), if the mesh and SpMV mapping are the ones measured:
+    {kernel name: {"bytes": ..., "source": ...}}."""
     fname = TRAFFIC_FILES.get(name)
     if fname is None:
         return None
@@ -105,8 +164,8 @@ def measured_traffic(info, mapping_lanes, name="c1"):
     if (d.get("nnz"), d.get("ndof")) != (info["nnz"], info["ndof"]) or \
             list(d.get("mapping_lanes", [])) != list(mapping_lanes or []):
         return None
-    return {"traffic_per_launch": float(d["traffic_per_launch"]),
-            "source": f"profiles/{fname} ({d.get('calibration', 'rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-kernel calibrated')})"}
+    src = f"profiles/{fname} ({d.get('calibration', 'rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-kernel calibrated')})"
+    return {k: {"bytes": float(v), "source": src} for k, v in d.get("traffic_per_launch_by_kernel", {}).items()}
 
 
 # ------------------------------------------------------------------------------------------
@@ -242,13 +301,7 @@ def run_forward(args, name):
             sigs.append(s_)
             stats.append(st_)
             print(f"  signals {sl.stop}/{nsig} done, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-        merged = dict(stats[0])
-        for key in merged:
-            if key in ("iters_max", "relres_max"):
-                merged[key] = max(s_[key] for s_ in stats)
-            elif key not in ("spmv_bytes", "precond_used", "reserved"):
-                merged[key] = sum(s_[key] for s_ in stats)
-        return np.concatenate(sigs), merged
+        return np.concatenate(sigs), merge_stats(stats)
 
     copy_gbs = ctx.copy_bandwidth(args.copy_bytes, 10) if args.copy_bytes > 0 else None
     tuned = None
@@ -267,16 +320,15 @@ def run_forward(args, name):
     info = ctx.info()
     elapsed = t2 - t1
     njob = nsig * args.steps
-    spmv_ms = sum(s["spmv_ms"] for s in all_stats)
-    spmv_n = sum(s["spmv_launches"] for s in all_stats)
-    spmv_bytes = all_stats[0]["spmv_bytes"]
-    avg_us = 1e3 * spmv_ms / max(spmv_n, 1)
-    achieved = spmv_bytes / (avg_us * 1e-6) / 1e9 if spmv_n else 0.0
-    iters = sum(s["iters_total"] for s in all_stats)
+    merged = merge_stats(all_stats)
+    iters = merged["iters_total"]
     us_v, us_t, replay_bytes = ctx.time_spmv(S_group, per_gdir, 50)
     used = tuned if tuned is not None else ([args.mapping, args.lanes] if args.mapping and args.lanes else None)
     traffic = measured_traffic(info, used, name)
     replay_achieved = replay_bytes / (0.5 * (us_v + us_t) * 1e-6) / 1e9
+    roof = roofline_block(merged, copy_gbs, traffic, S_group, args.event_iters)
+    roof["replay_us"] = {"spmv_v": us_v, "spmv_t": us_t}
+    roof["replay_achieved"] = replay_achieved
     b0 = [abs(s.real - 1.0) for s, q in zip(sig, qs) if q == 0.0]
 
     plain = None
@@ -320,16 +372,7 @@ def run_forward(args, name):
                                 3: "polynomial, complex diagonal"}.get(all_stats[-1].get("precond_used")),
         "setup_ms": info["setup_ms"],
         "mesh_gen_s": w["mesh_gen_s"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["traffic_per_launch"] if traffic else None,
-                     "traffic_source": traffic["source"] if traffic else None,
-                     "kernel": f"spmv_v/spmv_t (batched complex value-split CSR SpMV, {S_group} signals per launch, "
-                               "fused reductions)",
-                     "avg_launch_us": avg_us, "bytes_per_launch": spmv_bytes, "launches": spmv_n,
-                     "events": f"first {args.event_iters} Krylov iterations of every theta step",
-                     "replay_us": {"spmv_v": us_v, "spmv_t": us_t}, "replay_achieved": replay_achieved,
-                     "copy_kernel_gbs": copy_gbs,
-                     "frac_of_copy": (achieved / copy_gbs) if copy_gbs else None},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "plain_one_signal_at_a_time": plain,
         "parity": parity_block(w, b0, prefix_parity, sig),
@@ -383,17 +426,16 @@ def run_stepwise(args, name):
     sig, done = batch.signals()
     nd = float(info["ndof"])
     value = nd * nsig * args.steps / elapsed
-    spmv_ms = sum(s["spmv_ms"] for s in all_stats)
-    spmv_n = sum(s["spmv_launches"] for s in all_stats)
-    spmv_bytes = all_stats[0]["spmv_bytes"]
-    avg_us = 1e3 * spmv_ms / max(spmv_n, 1)
-    achieved = spmv_bytes / (avg_us * 1e-6) / 1e9 if spmv_n else 0.0
-    iters = sum(s["iters_total"] for s in all_stats)
+    merged = merge_stats(all_stats)
+    iters = merged["iters_total"]
     precond = all_stats[-1].get("precond_used")
     batch.close()
     us_v, us_t, replay_bytes = ctx.time_spmv(S_group, per_gdir, 20)
     replay_achieved = replay_bytes / (0.5 * (us_v + us_t) * 1e-6) / 1e9
     traffic = measured_traffic(info, tuned, name)
+    roof = roofline_block(merged, copy_gbs, traffic, S_group, args.event_iters)
+    roof["replay_us"] = {"spmv_v": us_v, "spmv_t": us_t}
+    roof["replay_achieved"] = replay_achieved
 
     # one signal at a time on the GPU over the same theta steps (a launch group of one)
     plain = None
@@ -447,16 +489,7 @@ def run_stepwise(args, name):
         "setup_ms": info["setup_ms"],
         "batch_begin_s": t_begin,
         "mesh_gen_s": w["mesh_gen_s"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["traffic_per_launch"] if traffic else None,
-                     "traffic_source": traffic["source"] if traffic else None,
-                     "kernel": f"spmv_v/spmv_t (batched complex value-split CSR SpMV, {S_group} signals per launch, "
-                               "fused reductions)",
-                     "avg_launch_us": avg_us, "bytes_per_launch": spmv_bytes, "launches": spmv_n,
-                     "events": f"first {args.event_iters} Krylov iterations of every theta step",
-                     "replay_us": {"spmv_v": us_v, "spmv_t": us_t}, "replay_achieved": replay_achieved,
-                     "copy_kernel_gbs": copy_gbs,
-                     "frac_of_copy": (achieved / copy_gbs) if copy_gbs else None},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "plain_one_signal_at_a_time": plain,
         "parity": parity,

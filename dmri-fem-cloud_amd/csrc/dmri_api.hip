@@ -31,7 +31,7 @@ T read_scalar(const T *d, hipStream_t s) {
 constexpr int kEventPool = 512;
 
 struct SolverWork {
-    DevArray<double2> x, r, rh, p, v, t, b, xprev, xprev2, ybc, bper, ph, sh;
+    DevArray<double2> x, r, rh, p, v, t, b, xprev, xprev2, ybc, bper, ph, sh, sv;
     DevArray<double2> yh[kMaxHist], hp[kMaxHist], hq[kMaxHist];  // projection history
     DevArray<double> sc, vals, bre, areal, pdiag, part, tot, coef, coefP, sig, spart, ppart, ptot, pgam, gsig, u0v;
     DevArray<KrylovState> st;
@@ -668,6 +668,9 @@ struct Stepper {
     ProjHistory fh;  // projection guess of the forward steps (fh.m = 0: off); fh.ys[0] = xc
     bool poly = true;       // degree-1 polynomial preconditioner
     bool real_pre = false;  // the polynomial of the real part alone
+    double kbytes[6] = {0, 0, 0, 0, 0, 0};  // algorithmic bytes per launch of the six iteration kernels
+    int precond = 0;                         // dmri_stats.precond_used of this launch group
+    bool timed_first[kEventPool / 6] = {};   // timed iteration k of a chunk is the solve's first
     double omt = 0.5;
     int chunk = 8;
 
@@ -796,8 +799,10 @@ struct Stepper {
             const size_t nS = (size_t)n * S;
             if (w.ph.n != nS) w.ph.alloc(nS);
             if (w.sh.n != nS) w.sh.alloc(nS);
+            if (w.sv.n != nS) w.sv.alloc(nS);
             a.ph = w.ph.p;
             a.sh = w.sh.p;
+            a.s = w.sv.p;
             const int vw = per_gdir ? 4 : 2;
             a.pc = poly_scale(c, w.vals.p, vw);
             // The polynomial takes the real part alone when the imaginary part of the scaled
@@ -848,11 +853,23 @@ struct Stepper {
         a.rtol = opts->rtol;
         a.atol = opts->atol;
         a.maxit = opts->maxit;
-        // timed launches: v = A p (p gathered, r^ read, v written: 48 B per row and signal) and
-        // t = A s (plain: r, v gathered, t written: 48 B; preconditioned: sh gathered, r, v read at
-        // the own row, t written: 64 B), so 48 or 56 B on average
-        st->spmv_bytes = (double)c->nnz * (4.0 + 8.0 * (per_gdir ? 4 : 2)) + 4.0 * (double)(n + 1) +
-                         (poly ? 56.0 : 48.0) * (double)n * S;
+        // algorithmic bytes per launch of each iteration kernel (dmri_stats.kern_bytes): the operator
+        // SpMVs v = A ph and t = A sh read their matrix entries and 48 B per row and signal (one vector
+        // gathered, one read at the own row, one written)
+        {
+            const int vw = per_gdir ? 4 : 2, npre = real_pre ? 1 : vw;
+            const double nS = (double)n * S;
+            auto E = [&](int k) { return (double)c->nnz * (4.0 + 8.0 * k) + 4.0 * (double)(n + 1); };
+            const double cdx = a.pdiag ? 8.0 * vw * (double)n : 0.0;  // the complex diagonal per row
+            kbytes[0] = 64.0 * nS;
+            kbytes[1] = poly ? E(npre) + 32.0 * nS + cdx : 0.0;
+            kbytes[2] = E(vw) + 48.0 * nS;
+            kbytes[3] = poly ? E(npre) + 64.0 * nS + cdx : 0.0;
+            kbytes[4] = E(vw) + 48.0 * nS;
+            kbytes[5] = 128.0 * nS;
+        }
+        st->spmv_bytes = 0.5 * (kbytes[2] + kbytes[4]);
+        precond = poly ? st->precond_used : 0;
         prof = opts->profile != 0;
         chunk = 8;
     }
@@ -1016,30 +1033,31 @@ struct Stepper {
     void krylov(int step) {
         SolverWork &w = *wk;
         int it = 0, it_before = 0, restarts = 0;
+        constexpr int kK = 6;  // timed kernels per iteration (dmri_stats.kern_*)
         for (;;) {
-            const int nl = std::min(chunk, kEventPool / 2);
+            const int nl = chunk;
             int ntimed = 0;
             for (int k = 0; k < nl; ++k, ++it) {
-                // the SpMV launches of iterations it < profile carry start/end events that take
-                // the timestamps of their own dispatch (no marker packets between kernels)
-                const bool timed = prof && it < opts->profile;
-                launch_bicg_p(geo, a, it, s);
+                // every kernel of the iterations it < profile carries start/end events that take the
+                // timestamps of its own dispatch (no marker packets between kernels)
+                const bool timed = prof && it < opts->profile && ntimed < kEventPool / kK;
+                auto e0 = [&](int j) { return timed ? c->ev0[kK * ntimed + j] : nullptr; };
+                auto e1 = [&](int j) { return timed ? c->ev1[kK * ntimed + j] : nullptr; };
+                launch_bicg_p(geo, a, it, s, e0(0), e1(0));
                 if (poly) {
                     // ph = 2p - Ap, v = A ph; sh = 2s - As, t = A sh (right preconditioning)
-                    launch_spmv_v(geo, a, it, s, nullptr, nullptr, kOpPre);
+                    launch_spmv_v(geo, a, it, s, e0(1), e1(1), kOpPre);
                     SolveArgs av = a;
                     av.p = a.ph;
-                    launch_spmv_v(geo, av, it, s, timed ? c->ev0[2 * k] : nullptr, timed ? c->ev1[2 * k] : nullptr);
-                    launch_spmv_t(geo, a, it, s, nullptr, nullptr, kOpPre);
-                    launch_spmv_t(geo, a, it, s, timed ? c->ev0[2 * k + 1] : nullptr,
-                                  timed ? c->ev1[2 * k + 1] : nullptr, kOpTP);
+                    launch_spmv_v(geo, av, it, s, e0(2), e1(2));
+                    launch_spmv_t(geo, a, it, s, e0(3), e1(3), kOpPre);
+                    launch_spmv_t(geo, a, it, s, e0(4), e1(4), kOpTP);
                 } else {
-                    launch_spmv_v(geo, a, it, s, timed ? c->ev0[2 * k] : nullptr, timed ? c->ev1[2 * k] : nullptr);
-                    launch_spmv_t(geo, a, it, s, timed ? c->ev0[2 * k + 1] : nullptr,
-                                  timed ? c->ev1[2 * k + 1] : nullptr);
+                    launch_spmv_v(geo, a, it, s, e0(2), e1(2));
+                    launch_spmv_t(geo, a, it, s, e0(4), e1(4));
                 }
-                ntimed += timed ? 1 : 0;
-                launch_bicg_update(geo, a, it, s);
+                launch_bicg_update(geo, a, it, s, e0(5), e1(5));
+                if (timed) timed_first[ntimed++] = it == 0;
             }
             launch_publish_state(w.st.p, S, w.all_done.p, w.d_h_st, w.d_h_flag, s);
             DMRI_HIP(hipStreamSynchronize(s));
@@ -1052,11 +1070,18 @@ struct Stepper {
             if (prof) {
                 const int real = std::max(0, std::min(ntimed, it_last - it_before));
                 for (int k = 0; k < real; ++k)
-                    for (int j = 0; j < 2; ++j) {
+                    for (int j = 0; j < kK; ++j) {
+                        if (!poly && (j == 1 || j == 3)) continue;
                         float ms = 0;
-                        DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[2 * k + j], c->ev1[2 * k + j]));
-                        st->spmv_ms += ms;
-                        st->spmv_launches += 1;
+                        DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[kK * k + j], c->ev1[kK * k + j]));
+                        st->kern_ms[j] += ms;
+                        st->kern_launches[j] += 1;
+                        // the first iteration's p = r reads one vector and writes one
+                        st->kern_bytes[j] += kbytes[j] - (j == 0 && timed_first[k] ? 32.0 * (double)n * S : 0.0);
+                        if (j == 2 || j == 4) {
+                            st->spmv_ms += ms;
+                            st->spmv_launches += 1;
+                        }
                     }
             }
             it_before = it;
@@ -1694,8 +1719,13 @@ void batch_advance_impl(dmri_batch *b, int nsteps, dmri_stats *stats) {
         st.relres_max = std::max(st.relres_max, x.relres_max);
         st.spmv_ms += x.spmv_ms;
         st.spmv_launches += x.spmv_launches;
-        st.spmv_bytes = x.spmv_bytes;
-        st.precond_used = x.precond_used;
+        st.spmv_bytes = 0.5 * (gr->run.kbytes[2] + gr->run.kbytes[4]);
+        st.precond_used = gr->run.precond;
+        for (int j = 0; j < 6; ++j) {
+            st.kern_ms[j] += x.kern_ms[j];
+            st.kern_launches[j] += x.kern_launches[j];
+            st.kern_bytes[j] += x.kern_bytes[j];
+        }
     }
     st.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     *stats = st;

@@ -156,7 +156,22 @@ void launch_color_scatter(const int *color, int64_t ne, const int *offs, int *cu
 void launch_assemble_rows(int64_t n, const int *dof_vert, const int *dof_comp, const int *rowptr, const int *col,
                           const int *tets, const int *phase, const double *xyz, const int *vptr, const int *vlist,
                           const double *Dc, const double *Dcell, const double *iT2c, const double *iT2cell, double *m,
-                          double *k, double *r, double *jx, double *jy, double *jz, hipStream_t s);
+                          double *k, double *r, double *jx, double *jy, double *jz, hipStream_t s,
+                          double *cx = nullptr, double *cy = nullptr, double *cz = nullptr);
+// Strong pseudo-periodic constraint: g = (c - c^T)/2; per DOF the DOF of its master vertex (key),
+// the constrained ids (mid, from the scan of is_master); the constrained pattern P^T X P of the CSR
+// (rows, cols of the merged runs of equal (mid row, mid col) keys, sorted stably) and the run sums.
+void launch_antisym(int64_t n, const int *rowptr, const int *col, const double *c, double *g, int *miss,
+                    hipStream_t s);
+void launch_master_dofs(int64_t n, int64_t nv, const int *dof_vert, const int *dof_comp, const int *master,
+                        const int *dof_of, int *key, int *is_master, int *bad, hipStream_t s);
+void launch_merged_ids(int64_t n, const int *key, const int *scan, int *mid, hipStream_t s);
+// returns the constrained entry count; rstart[u], sidx: run starts and the sorted entry positions
+int64_t build_merge(int64_t n, int64_t nnz, const int *rowptr, const int *col, const int *mid, int64_t nm,
+                    DevArray<int> &mrowptr, DevArray<int> &mcol, DevArray<int> &rstart, DevArray<int> &sidx,
+                    hipStream_t s);
+void launch_run_sum(int64_t nu, int64_t nnz, const int *rstart, const int *sidx, const double *val, double *merged,
+                    hipStream_t s);
 void launch_assemble_facets(int64_t start, int64_t count, const int *perm, const int *iface_v, const double *xyz,
                             int64_t nv, const int *dof_of, const int *rowptr, const int *col, double kappa,
                             const double *kappa_f, double *gi, int *miss, hipStream_t s);
@@ -211,7 +226,17 @@ void launch_prep_scale(int64_t n, const int *diag, const double *m, const double
 void launch_prep_values(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
                         const double *m, const double *k, const double *r, const double *gi, const double *pv,
                         const double *jx, const double *jy, const double *jz, bool per_gdir, double gx, double gy,
-                        double gz, double dt, double theta, double *vals, double *bre, hipStream_t s);
+                        double gz, double dt, double theta, double *vals, double *bre, hipStream_t s,
+                        double theta_b = -1.0);  // explicit-side weight of L (default 1 - theta)
+// strong periodic form: mgg = S (D_c M) S in the solver numbering (scomp null: one compartment), and
+// per step vals.a = afix + s2a mgg (and areal), bre = bfix + s2b mgg
+void launch_sbc_mgg(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc, const double *m,
+                    const int *scomp, double D0, double D1, double *mgg, hipStream_t s);
+void launch_sbc_step(int64_t nnz, int vw, const double *afix, const double *bfix, const double *mgg, double s2a,
+                     double s2b, double *vals, double *bre, double *areal, hipStream_t s);
+// the DOF map on the constrained space (is_master, mid from launch_master_dofs / launch_merged_ids)
+void launch_constrained_dofs(int64_t n, int64_t nl, const int *is_master, const int *mid, const int *dof_vert,
+                             const int *dof_comp, int *dof_of, int *nvert, int *ncomp, hipStream_t s);
 // Solver numbering along a Morton curve of the DOF vertices: perm (new -> old), iperm, the
 // renumbered CSR (srow, scol) with emap[e'] = old entry index, and per new row the old diagonal
 // position sdiag, the signal weight sw and (two compartments) the compartment scomp.
@@ -276,6 +301,9 @@ struct SolveArgs {
     double gc0, gc1, gc2;
     const double2 *bper;            // weak pseudo-periodic right-hand side of this step, or null
     double2 *ph, *sh;               // preconditioned p and s (polynomial preconditioner), or null
+    double2 *s;                     // s = r - alpha v, written at the own rows by the preconditioner's
+                                    // t-kernel and read by the next t-kernel and the update (null: the
+                                    // consumers form it from r and v)
     double pc;                      // polynomial scale c: z -> c (2 z - c Are z)
     const double *areal;            // [nnz] the scaled real part alone (preconditioner SpMVs)
     const double *pdiag;            // [n*vw] diagonal values of the scaled operator per row (a, j or
@@ -323,7 +351,8 @@ constexpr int proj_nv(int mh) { return 2 * (mh * (mh + 1) / 2 + mh); }
 // mh: history capacity (4, 6, 8, 10 or 12) >= p.nh; part >= proj_grid() * S * proj_nv(mh) doubles
 int proj_grid(const SolveGeom &g);
 void launch_proj_guess(const SolveGeom &g, const ProjArgs &p, int mh, hipStream_t s);
-void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
+void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr);
 // op: kOpV / kOpPre for the v-kernel (kOpPre writes ph from p), kOpT / kOpPre / kOpTP for the
 // t-kernel (kOpPre writes sh from s = r - alpha v; kOpTP gathers sh).  Mappings 1, 2 and 4 take
 // every op, mappings 3 and 5 only kOpV / kOpT.
@@ -331,7 +360,8 @@ void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
                    hipEvent_t e1 = nullptr, int op = kOpV);
 void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
                    hipEvent_t e1 = nullptr, int op = kOpT);
-void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s);
+void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr);
 // areal[e] = vals[vw e]
 void launch_extract_real(int64_t nnz, const double *vals, int vw, double *areal, hipStream_t s);
 // zero st[0..S) and *flag

@@ -587,7 +587,7 @@ __global__ void __launch_bounds__(256) assemble_rows_kernel(int64_t n, const int
                                                             const int *vlist, const double *Dc, const double *Dcell,
                                                             const double *iT2c, const double *iT2cell, double *mv,
                                                             double *kv, double *rv, double *jxv, double *jyv,
-                                                            double *jzv) {
+                                                            double *jzv, double *cxv, double *cyv, double *czv) {
     const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes;
     const int lane = threadIdx.x % kRowLanes;
     if (r >= n) return;
@@ -597,7 +597,7 @@ __global__ void __launch_bounds__(256) assemble_rows_kernel(int64_t n, const int
     for (int e = e0 + lane; e < e1; e += kRowLanes) {
         const int cl = col[e];
         const int vj = dof_vert[cl];
-        double m = 0.0, kk = 0.0, rr = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+        double m = 0.0, kk = 0.0, rr = 0.0, jx = 0.0, jy = 0.0, jz = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
         if (dof_comp[cl] == cr) {
             for (int q = q0; q < q1; ++q) {
                 const int t = vlist[q];
@@ -681,7 +681,17 @@ __global__ void __launch_bounds__(256) assemble_rows_kernel(int64_t n, const int
                 jx += w * (S0 + Xa[0] + Xb[0]);
                 jy += w * (S1 + Xa[1] + Xb[1]);
                 jz += w * (S2 + Xa[2] + Xb[2]);
+                // strong periodic form: int 2 D (d phi_j / d x_d) phi_i = 2 D grad(phi_j)_d |T| / 4
+                const double cd = 0.5 * vol * Dc[cr];
+                cx += cd * Gb[0];
+                cy += cd * Gb[1];
+                cz += cd * Gb[2];
             }
+        }
+        if (cxv) {
+            cxv[e] = cx;
+            cyv[e] = cy;
+            czv[e] = cz;
         }
         mv[e] = m;
         kv[e] = kk;
@@ -1162,7 +1172,7 @@ __global__ void prep_values_kernel(int64_t n, const int *rowptr, const int *col,
                                    const double *m, const double *k, const double *r, const double *gi,
                                    const double *pv, const double *jx, const double *jy, const double *jz,
                                    int per_gdir, double gx, double gy, double gz, double dt, double theta,
-                                   double *vals, double *bre) {
+                                   double theta_b, double *vals, double *bre) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t i = t / kPrepLanes;
     if (i >= n) return;
@@ -1172,7 +1182,7 @@ __global__ void prep_values_kernel(int64_t n, const int *rowptr, const int *col,
         const int o = emap[e];
         const double l = k[o] + r[o] + gi[o];
         const double mk = m[o] / dt;
-        bre[e] = s * (mk - (1.0 - theta) * l);
+        bre[e] = s * (mk - theta_b * l);
         const double impl = s * (mk + theta * (l + (pv ? pv[o] : 0.0)));
         if (per_gdir) {
             vals[4 * (int64_t)e + 0] = impl;
@@ -1965,13 +1975,14 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
         if (L.g != 0) return;
         const int64_t idx = row * S_ + s;
         const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
-        const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
+        const double2 sv = (OP == kOpTP && a.s) ? a.s[idx] : c_sub(r[idx], c_mul(alpha, v[idx]));
         if (pre) {
             const double2 w = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
             if constexpr (cd)
                 a.sh[idx] = c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s));
             else
                 a.sh[idx] = w;
+            if (a.s) a.s[idx] = sv;
             return;
         }
         a.t[idx] = y;
@@ -2191,9 +2202,10 @@ __global__ void __launch_bounds__(kThreads) spmv_t_wave_kernel(SolveArgs a, cons
             if (!act) return;
             const int64_t idx = row * S + lane;
             const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
-            const double2 sv = c_sub(r[idx], c_mul(alpha, v[idx]));
+            const double2 sv = (OP == kOpTP && a.s) ? a.s[idx] : c_sub(r[idx], c_mul(alpha, v[idx]));
             if (OP == kOpPre) {
                 a.sh[idx] = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
+                if (a.s) a.s[idx] = sv;
                 return;
             }
             a.t[idx] = y;
@@ -2384,7 +2396,7 @@ __global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int 
     if (act) {
         for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total;
              i += (int64_t)gridDim.x * vrpb * S) {
-            const double2 sv = c_sub(a.r[i], c_mul(alpha, a.v[i]));
+            const double2 sv = a.s ? a.s[i] : c_sub(a.r[i], c_mul(alpha, a.v[i]));
             // right preconditioning: the iterate moves along the preconditioned directions
             if (a.ph)
                 a.x[i] = c_add(a.x[i], c_add(c_mul(alpha, a.ph[i]), c_mul(omega, a.sh[i])));
@@ -2872,11 +2884,150 @@ void launch_color_scatter(const int *color, int64_t ne, const int *offs, int *cu
 void launch_assemble_rows(int64_t n, const int *dof_vert, const int *dof_comp, const int *rowptr, const int *col,
                           const int *tets, const int *phase, const double *xyz, const int *vptr, const int *vlist,
                           const double *Dc, const double *Dcell, const double *iT2c, const double *iT2cell, double *m,
-                          double *k, double *r, double *jx, double *jy, double *jz, hipStream_t s) {
+                          double *k, double *r, double *jx, double *jy, double *jz, hipStream_t s, double *cx,
+                          double *cy, double *cz) {
     if (n <= 0) return;
     assemble_rows_kernel<<<grid_for(n * kRowLanes, 256), 256, 0, s>>>(n, dof_vert, dof_comp, rowptr, col, tets, phase,
                                                                        xyz, vptr, vlist, Dc, Dcell, iT2c, iT2cell, m,
-                                                                       k, r, jx, jy, jz);
+                                                                       k, r, jx, jy, jz, cx, cy, cz);
+    check_launch();
+}
+
+// --------------------------------------------------------------------------------------
+// Strong pseudo-periodic constraint (set-up).
+// --------------------------------------------------------------------------------------
+// g[e] = (c[e] - c[e^T]) / 2 on a structurally symmetric pattern (one thread per row)
+__global__ void antisym_kernel(int64_t n, const int *rowptr, const int *col, const double *c, double *g, int *miss) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+        const int pt = csr_find(rowptr, col, col[e], (int)i);
+        if (pt < 0) {
+            atomicOr(miss, 1);
+            continue;
+        }
+        g[e] = 0.5 * (c[e] - c[pt]);
+    }
+}
+
+// key[d] = DOF of (compartment of d, master vertex of d's vertex); bad when that DOF is dead
+__global__ void master_dof_kernel(int64_t n, int64_t nv, const int *dof_vert, const int *dof_comp, const int *master,
+                                  const int *dof_of, int *key, int *is_master, int *bad) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n) return;
+    const int v = dof_vert[d], c = dof_comp[d];
+    const int k = dof_of[(int64_t)c * nv + master[v]];
+    if (k < 0) atomicOr(bad, 1);
+    key[d] = k < 0 ? (int)d : k;
+    is_master[d] = key[d] == (int)d;
+}
+
+// mid[d] = constrained id of d (the scan position of its master)
+__global__ void merged_id_kernel(int64_t n, const int *key, const int *scan, int *mid) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n) return;
+    mid[d] = scan[key[d]];
+}
+
+// entry keys (constrained row * nm + constrained column) and their original positions
+__global__ void merge_key_kernel(int64_t n, const int *rowptr, const int *col, const int *mid, int64_t nm,
+                                 uint64_t *ekey, int *eidx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t r = (uint64_t)mid[i];
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+        ekey[e] = r * (uint64_t)nm + (uint64_t)mid[col[e]];
+        eidx[e] = e;
+    }
+}
+
+// head[p] = 1 where a sorted key starts a run
+__global__ void run_head_kernel(int64_t nnz, const uint64_t *skey, int *head) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    head[p] = (p == 0 || skey[p] != skey[p - 1]) ? 1 : 0;
+}
+
+// run starts rstart[u] (u = run index = hscan[p] at a head) and the constrained CSR column and row counts
+__global__ void run_fill_kernel(int64_t nnz, int64_t nm, const uint64_t *skey, const int *head, const int *hscan,
+                                int *rstart, int *mcol, int *rowcnt) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nnz || !head[p]) return;
+    const int u = hscan[p];
+    rstart[u] = (int)p;
+    mcol[u] = (int)(skey[p] % (uint64_t)nm);
+    atomicAdd(&rowcnt[skey[p] / (uint64_t)nm], 1);
+}
+
+// merged[u] = sum of val over the run's entries in sorted (= original, the sort is stable) order
+__global__ void run_sum_kernel(int64_t nu, int64_t nnz, const int *rstart, const int *sidx, const double *val,
+                               double *merged) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const int p1 = u + 1 < nu ? rstart[u + 1] : (int)nnz;
+    double acc = 0.0;
+    for (int p = rstart[u]; p < p1; ++p) acc += val[sidx[p]];
+    merged[u] = acc;
+}
+
+void launch_antisym(int64_t n, const int *rowptr, const int *col, const double *c, double *g, int *miss,
+                    hipStream_t s) {
+    antisym_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rowptr, col, c, g, miss);
+    check_launch();
+}
+
+void launch_master_dofs(int64_t n, int64_t nv, const int *dof_vert, const int *dof_comp, const int *master,
+                        const int *dof_of, int *key, int *is_master, int *bad, hipStream_t s) {
+    master_dof_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, nv, dof_vert, dof_comp, master, dof_of, key, is_master, bad);
+    check_launch();
+}
+
+void launch_merged_ids(int64_t n, const int *key, const int *scan, int *mid, hipStream_t s) {
+    merged_id_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, key, scan, mid);
+    check_launch();
+}
+
+int64_t build_merge(int64_t n, int64_t nnz, const int *rowptr, const int *col, const int *mid, int64_t nm,
+                    DevArray<int> &mrowptr, DevArray<int> &mcol, DevArray<int> &rstart, DevArray<int> &sidx,
+                    hipStream_t s) {
+    DevArray<uint64_t> key, skey;
+    DevArray<int> idx, head, hscan, cnt;
+    key.alloc(nnz);
+    skey.alloc(nnz);
+    idx.alloc(nnz);
+    sidx.alloc(nnz);
+    merge_key_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rowptr, col, mid, nm, key.p, idx.p);
+    check_launch();
+    int bits = 1;
+    while (bits < 64 && ((uint64_t)1 << bits) < (uint64_t)nm * (uint64_t)nm) ++bits;
+    size_t tmp_bytes = 0;
+    DMRI_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key.p, skey.p, idx.p, sidx.p, (int)nnz, 0, bits, s));
+    DevArray<char> tmp;
+    tmp.alloc(std::max<size_t>(tmp_bytes, 1));
+    DMRI_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, key.p, skey.p, idx.p, sidx.p, (int)nnz, 0, bits, s));
+    head.alloc(nnz);
+    hscan.alloc(nnz + 1);
+    run_head_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, skey.p, head.p);
+    check_launch();
+    exclusive_scan(head.p, hscan.p, nnz, s);
+    int nu = 0;
+    DMRI_HIP(hipMemcpyAsync(&nu, hscan.p + nnz, sizeof(int), hipMemcpyDeviceToHost, s));
+    DMRI_HIP(hipStreamSynchronize(s));
+    rstart.alloc(std::max(nu, 1));
+    mcol.alloc(std::max(nu, 1));
+    cnt.alloc(nm);
+    cnt.zero(s);
+    run_fill_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, nm, skey.p, head.p, hscan.p, rstart.p, mcol.p, cnt.p);
+    check_launch();
+    mrowptr.alloc(nm + 1);
+    exclusive_scan(cnt.p, mrowptr.p, nm, s);
+    DMRI_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+    return nu;
+}
+
+void launch_run_sum(int64_t nu, int64_t nnz, const int *rstart, const int *sidx, const double *val, double *merged,
+                    hipStream_t s) {
+    run_sum_kernel<<<grid_for(nu, 256), 256, 0, s>>>(nu, nnz, rstart, sidx, val, merged);
     check_launch();
 }
 void launch_assemble_facets(int64_t start, int64_t count, const int *perm, const int *iface_v, const double *xyz,
@@ -2976,10 +3127,69 @@ void launch_prep_scale(int64_t n, const int *diag, const double *m, const double
 void launch_prep_values(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
                         const double *m, const double *k, const double *r, const double *gi, const double *pv,
                         const double *jx, const double *jy, const double *jz, bool per_gdir, double gx, double gy,
-                        double gz, double dt, double theta, double *vals, double *bre, hipStream_t s) {
+                        double gz, double dt, double theta, double *vals, double *bre, hipStream_t s,
+                        double theta_b) {
+    if (theta_b < 0.0) theta_b = 1.0 - theta;
     prep_values_kernel<<<grid_for(n * kPrepLanes, 256), 256, 0, s>>>(n, rowptr, col, emap, sc, m, k, r, gi, pv, jx,
                                                                       jy, jz, per_gdir ? 1 : 0, gx, gy, gz, dt, theta,
-                                                                      vals, bre);
+                                                                      theta_b, vals, bre);
+    check_launch();
+}
+
+// strong periodic form: mgg[e] = sc_i sc_j D_c M (the (g.D g) q^2 F^2 mass term per unit q^2 F^2)
+__global__ void sbc_mgg_kernel(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
+                               const double *m, const int *scomp, double D0, double D1, double *mgg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double Dc = (scomp && scomp[i] == 1) ? D1 : D0;
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) mgg[e] = sc[i] * sc[col[e]] * Dc * m[emap[e]];
+}
+
+// per step: the real implicit part afix + s2a mgg into vals (vw values per entry; and into areal when
+// set), the explicit real part bfix + s2b mgg into bre
+__global__ void sbc_step_kernel(int64_t nnz, int vw, const double *afix, const double *bfix, const double *mgg,
+                                double s2a, double s2b, double *vals, double *bre, double *areal) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const double mg = mgg[e];
+    const double a = afix[e] + s2a * mg;
+    vals[(int64_t)vw * e] = a;
+    if (areal) areal[e] = a;
+    bre[e] = bfix[e] + s2b * mg;
+}
+
+void launch_sbc_mgg(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc, const double *m,
+                    const int *scomp, double D0, double D1, double *mgg, hipStream_t s) {
+    sbc_mgg_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rowptr, col, emap, sc, m, scomp, D0, D1, mgg);
+    check_launch();
+}
+
+void launch_sbc_step(int64_t nnz, int vw, const double *afix, const double *bfix, const double *mgg, double s2a,
+                     double s2b, double *vals, double *bre, double *areal, hipStream_t s) {
+    sbc_step_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, vw, afix, bfix, mgg, s2a, s2b, vals, bre, areal);
+    check_launch();
+}
+
+// constrained DOF map: dof_of[c nv + v] <- mid[dof_of] (-1 stays), and per kept (master) DOF its
+// vertex and compartment at its constrained id
+__global__ void remap_dof_of_kernel(int64_t nl, const int *mid, int *dof_of) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    const int d = dof_of[i];
+    if (d >= 0) dof_of[i] = mid[d];
+}
+__global__ void compact_dofs_kernel(int64_t n, const int *is_master, const int *mid, const int *dof_vert,
+                                    const int *dof_comp, int *nvert, int *ncomp) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n || !is_master[d]) return;
+    nvert[mid[d]] = dof_vert[d];
+    ncomp[mid[d]] = dof_comp[d];
+}
+void launch_constrained_dofs(int64_t n, int64_t nl, const int *is_master, const int *mid, const int *dof_vert,
+                             const int *dof_comp, int *dof_of, int *nvert, int *ncomp, hipStream_t s) {
+    remap_dof_of_kernel<<<grid_for(nl, 256), 256, 0, s>>>(nl, mid, dof_of);
+    check_launch();
+    compact_dofs_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, is_master, mid, dof_vert, dof_comp, nvert, ncomp);
     check_launch();
 }
 
@@ -3287,8 +3497,9 @@ void launch_rhs_residual(const SolveGeom &g, const SolveArgs &a, int mode, const
         check_launch();
     }
 }
-void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s) {
-    bicg_p_kernel<<<g.vgrid, g.vec_threads, 0, s>>>(a, g.S, g.vec_rpb, g.grid, g.vgrid, it);
+void launch_bicg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(bicg_p_kernel, dim3(g.vgrid), dim3(g.vec_threads), 0, s, e0, e1, 0, a, g.S, g.vec_rpb,
+                          g.grid, g.vgrid, it);
     check_launch();
 }
 // The SpMV kernels go through hipExtLaunchKernelGGL so that e0/e1 (when not null) take the start
@@ -3402,8 +3613,10 @@ void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
         check_launch();
     }
 }
-void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s) {
-    bicg_update_kernel<<<g.vgrid, g.vec_threads, 0, s>>>(a, g.S, g.vec_rpb, g.grid, it);
+void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
+                        hipEvent_t e1) {
+    hipExtLaunchKernelGGL(bicg_update_kernel, dim3(g.vgrid), dim3(g.vec_threads), 0, s, e0, e1, 0, a, g.S,
+                          g.vec_rpb, g.grid, it);
     check_launch();
     if (a.totU) {
         finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partU, g.vgrid, g.S, a.totU);

@@ -55,7 +55,19 @@ class _Stats(ctypes.Structure):
     _fields_ = [("iters_total", ctypes.c_int64), ("iters_max", ctypes.c_int32), ("restarts", ctypes.c_int32),
                 ("relres_max", ctypes.c_double), ("solve_ms", ctypes.c_double), ("spmv_ms", ctypes.c_double),
                 ("spmv_launches", ctypes.c_int64), ("spmv_bytes", ctypes.c_double),
-                ("precond_used", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("precond_used", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("kern_ms", ctypes.c_double * 6), ("kern_launches", ctypes.c_int64 * 6),
+                ("kern_bytes", ctypes.c_double * 6)]
+
+KERNELS = ("bicg_p", "spmv_v_pre", "spmv_v", "spmv_t_pre", "spmv_t", "bicg_update")
+
+
+def _stats_dict(st) -> dict:
+    out = {}
+    for k, _ in _Stats._fields_:
+        v = getattr(st, k)
+        out[k] = list(v) if k.startswith("kern_") else v
+    return out
 
 
 class _AdjOpts(ctypes.Structure):
@@ -289,7 +301,7 @@ class Context:
         st = _Stats()
         _check(LIB.dmri_solve(self._h, ctypes.byref(s), ctypes.byref(o), _ptr(sig, ctypes.c_double),
                               ctypes.byref(st)))
-        stats = {k: getattr(st, k) for k, _ in _Stats._fields_}
+        stats = _stats_dict(st)
         if snaps is not None:
             stats["snapshots"] = snaps[:, 0, :]
         return complex(sig[0], sig[1]), stats
@@ -310,7 +322,7 @@ class Context:
         _check(LIB.dmri_solve_batched(self._h, qs.shape[0], _ptr(qs, ctypes.c_double), _ptr(g, ctypes.c_double),
                                       ctypes.byref(s), ctypes.byref(o), _ptr(sig, ctypes.c_double),
                                       ctypes.byref(st)))
-        stats = {k: getattr(st, k) for k, _ in _Stats._fields_}
+        stats = _stats_dict(st)
         if snaps is not None:
             stats["snapshots"] = snaps
         return sig.view(np.complex128), stats
@@ -363,7 +375,7 @@ class Context:
         grads = {"D": np.array(g.dS_dD[:]), "invT2": np.array(g.dS_dinvT2[:]), "kappa": g.dS_dkappa, "q": g.dS_dq,
                  "f_implicit": dfa[:n].copy(), "f_explicit": dfb[:n].copy(), "ms_forward": g.ms_forward,
                  "ms_adjoint": g.ms_adjoint, "checkpoints": g.checkpoints, "recomputed_steps": g.recomputed_steps}
-        return complex(sig[0], sig[1]), grads, {k: getattr(st, k) for k, _ in _Stats._fields_}
+        return complex(sig[0], sig[1]), grads, _stats_dict(st)
 
     def time_spmv(self, nsig: int, per_gdir: bool = False, reps: int = 50):
         """(us per spmv_v launch, us per spmv_t launch, algorithmic bytes per launch) of back-to-back
@@ -395,7 +407,7 @@ class Batch:
     def advance(self, nsteps: int = 1) -> dict:
         st = _Stats()
         _check(LIB.dmri_batch_advance(self._h, int(nsteps), ctypes.byref(st)))
-        return {k: getattr(st, k) for k, _ in _Stats._fields_}
+        return _stats_dict(st)
 
     def signals(self):
         """(complex signals of the current states, theta steps done)."""

@@ -225,6 +225,18 @@ typedef struct {
                                polynomial of the scaled operator, 2 = the polynomial of its real part,
                                3 = the polynomial in the variables scaled by the complex diagonal */
     int32_t reserved;
+    /* Every kernel of the timed Krylov iterations (profile > 0), by its own dispatch timestamps:
+     * [0] bicg_p (p = r + beta (p - omega v)), [1] preconditioner SpMV on p (ph), [2] v = A ph (or A p),
+     * [3] preconditioner SpMV on s = r - alpha v (sh; s written), [4] t = A sh (or A s), [5] the update of
+     * x and r.  [1] and [3] stay 0 without the polynomial preconditioner.  kern_ms: summed device time;
+     * kern_bytes: summed algorithmic HBM bytes of those launches (matrix entries, column indices and row
+     * offsets once, every vector read or written once; per launch: [0] 64 n S (32 n S in a solve's first
+     * iteration), [1] E(npre) + 32 n S, [2] E(nval) + 48 n S, [3] E(npre) + 64 n S, [4] E(nval) + 48 n S,
+     * [5] 128 n S, with E(k) = nnz (4 + 8k) + 4 (n+1), nval = 2 (4 with per-signal directions) and npre = 1
+     * for the real-part polynomial, else nval). */
+    double kern_ms[6];
+    int64_t kern_launches[6];
+    double kern_bytes[6];
 } dmri_stats;
 
 /* Run the theta scheme from u0 = 1 (real part, every live DOF) and return the signal

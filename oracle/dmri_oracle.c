@@ -474,6 +474,117 @@ int orc_assemble(int64_t nv, const double *xyz, int64_t nc, const int32_t *tets,
 }
 
 /* ------------------------------------------------------------------------- */
+/* Strong pseudo-periodic formulation (IsDomainPeriodic = True): the terms of   */
+/* FuncF_sBC, outer_interface and inner_interface (DmriFemLib.py:148-165) that   */
+/* carry the gradient direction, per axis d on the unconstrained DOF pattern,   */
+/* for a scalar D per compartment:                                              */
+/*   c_d[ij] = int 2 D_c (d phi_j / d x_d) phi_i   (inner(g, D grad u) +         */
+/*             inner(grad u, D g) tested with v, FuncF_sBC 162-165);             */
+/*   b_d[ij] = int D_c n_d phi_i phi_j over every exterior facet of a cell of     */
+/*             compartment c (outer_interface 157-159, n the outward normal) and, */
+/*             for two compartments, every interface facet with n the outward     */
+/*             normal of compartment c's cell: inner_interface (149-155) reduces  */
+/*             to -i q F D_0 (g.n0) u0 v0 + i q F D_1 (g.n0) u1 v1 with n0 the    */
+/*             outward normal of the phase-0 cell, the outer term of each side.   */
+/* The reference integrates each compartment's outer term over every exterior   */
+/* facet, also those of the other compartment's cells, where that field has     */
+/* only dead DOFs but for the interface vertices; here it is taken over the      */
+/* compartment's own facets.                                                    */
+/* In the scheme the operator adds i theta q F (c - b) . g.                     */
+/* Returns 0, or a negative code when an entry falls outside the pattern.       */
+/* ------------------------------------------------------------------------- */
+static void outward_normal(const double X[4][3], int o, double n[3], double *area) {
+    /* facet opposite local vertex o of a cell with vertices X */
+    int k = 0, idx[3];
+    for (int a = 0; a < 4; ++a)
+        if (a != o) idx[k++] = a;
+    double e1[3], e2[3];
+    for (int q = 0; q < 3; ++q) {
+        e1[q] = X[idx[1]][q] - X[idx[0]][q];
+        e2[q] = X[idx[2]][q] - X[idx[0]][q];
+    }
+    n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    *area = 0.5 * len;
+    double dir = 0;
+    for (int q = 0; q < 3; ++q) dir += n[q] * (X[o][q] - X[idx[0]][q]);
+    double sg = dir > 0 ? -1.0 : 1.0; /* away from the opposite vertex */
+    for (int q = 0; q < 3; ++q) n[q] *= sg / len;
+}
+
+static int add_facet_normal_mass(const int64_t *rowptr, const int32_t *col, const int32_t dd[3], double Dc,
+                                 const double n[3], double area, double *bx, double *by, double *bz) {
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            int64_t pos = csr_find(rowptr, col, dd[a], dd[b]);
+            if (pos < 0) return -1;
+            double w = Dc * area * ((a == b) ? 2.0 : 1.0) / 12.0;
+            bx[pos] += w * n[0];
+            by[pos] += w * n[1];
+            bz[pos] += w * n[2];
+        }
+    return 0;
+}
+
+int orc_assemble_sbc(int64_t nv, const double *xyz, int64_t nc, const int32_t *tets, const int32_t *phase,
+                     const int32_t *dof_of, const double *Dc, const int64_t *rowptr, const int32_t *col, int64_t nnz,
+                     double *cx, double *cy, double *cz, double *bx, double *by, double *bz) {
+    double *outs[6] = {cx, cy, cz, bx, by, bz};
+    for (int o = 0; o < 6; ++o) memset(outs[o], 0, sizeof(double) * (size_t)nnz);
+    for (int64_t t = 0; t < nc; ++t) {
+        int c = phase ? phase[t] : 0;
+        double X[4][3], G[4][3];
+        int32_t d[4];
+        for (int a = 0; a < 4; ++a) {
+            int32_t v = tets[4 * t + a];
+            d[a] = dof_of[c * nv + v];
+            for (int q = 0; q < 3; ++q) X[a][q] = xyz[3 * v + q];
+        }
+        double vol = tet_geom(X, G);
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                int64_t pos = csr_find(rowptr, col, d[a], d[b]);
+                if (pos < 0) return -3;
+                /* int phi_a = vol / 4, grad phi_b constant on the cell */
+                cx[pos] += 2.0 * Dc[c] * G[b][0] * vol / 4.0;
+                cy[pos] += 2.0 * Dc[c] * G[b][1] * vol / 4.0;
+                cz[pos] += 2.0 * Dc[c] * G[b][2] * vol / 4.0;
+            }
+    }
+    /* exterior facets (each belongs to one cell) and interface facets (both sides) */
+    face_rec *f = collect_faces(nc, tets);
+    if (!f) return -1;
+    int64_t m = 4 * nc;
+    int rc = 0;
+    for (int64_t i = 0; i < m && !rc;) {
+        int64_t j = i + 1;
+        while (j < m && f[j].v[0] == f[i].v[0] && f[j].v[1] == f[i].v[1] && f[j].v[2] == f[i].v[2]) ++j;
+        int take = (j - i == 1) || (j - i == 2 && phase && phase[f[i].cell] != phase[f[i + 1].cell]);
+        for (int64_t k = i; take && k < j && !rc; ++k) {
+            int64_t t = f[k].cell;
+            int c = phase ? phase[t] : 0;
+            double X[4][3];
+            int o = -1;
+            for (int a = 0; a < 4; ++a) {
+                int32_t v = tets[4 * t + a];
+                for (int q = 0; q < 3; ++q) X[a][q] = xyz[3 * v + q];
+                if (v != f[k].v[0] && v != f[k].v[1] && v != f[k].v[2]) o = a;
+            }
+            double n[3], area;
+            outward_normal(X, o, n, &area);
+            int32_t dd[3];
+            for (int a = 0; a < 3; ++a) dd[a] = dof_of[c * nv + f[k].v[a]];
+            if (add_facet_normal_mass(rowptr, col, dd, Dc[c], n, area, bx, by, bz)) rc = -4;
+        }
+        i = j;
+    }
+    free(f);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Linear algebra helpers (complex vectors, real CSR value arrays).            */
 /* ------------------------------------------------------------------------- */
 

@@ -66,6 +66,8 @@ def lib():
                                    i32p, dp, dp, i64]
         L.orc_spmv.restype = None
         L.orc_spmv.argtypes = [i64, i64p, i32p, dp, dp, ctypes.c_double, dp, dp]
+        L.orc_assemble_sbc.restype = ctypes.c_int
+        L.orc_assemble_sbc.argtypes = [i64, dp, i64, i32p, i32p, i32p, dp, i64p, i32p, i64, dp, dp, dp, dp, dp, dp]
         L.orc_signal.restype = None
         L.orc_signal.argtypes = [i64, dp, dp, dp]
         L.orc_num_threads.restype = ctypes.c_int
@@ -373,3 +375,103 @@ class System:
         L.orc_signal(self.ndof, _p(self.w, ctypes.c_double), _p(uv, ctypes.c_double), _p(sig, ctypes.c_double))
         return sig[0] / self.volume, u, {"iters": stats[0], "max_iters": stats[1], "max_relres": stats[2],
                                          "restarts": stats[3], "signal_raw": sig.copy()}
+
+
+# ------------------------------------------------------------------------------------
+# strong pseudo-periodic formulation (IsDomainPeriodic = True)
+# ------------------------------------------------------------------------------------
+def periodic_masters(xyz, pdir, tol):
+    """Master vertex of every vertex under the periodic constraint of PeriodicBD (DmriFemLib.py:
+    327-375): a vertex within tol of the max face of a periodic axis is the image of the vertex at
+    the same point moved to the min face (map() moves x by -L along that axis); the chain of maps is
+    followed to its end, so a vertex on several max faces takes the vertex at the corner of the min
+    faces.  Brute force over the vertices; tol = 1e-2 hmin as in PeriodicBD."""
+    xyz = np.asarray(xyz, dtype=np.float64)
+    lo, hi = xyz.min(axis=0), xyz.max(axis=0)
+    canon = xyz.copy()
+    for d in range(3):
+        if pdir[d]:
+            canon[np.abs(xyz[:, d] - hi[d]) < tol, d] = lo[d]
+    master = np.arange(xyz.shape[0])
+    for v in np.nonzero(np.any(canon != xyz, axis=1))[0]:
+        cand = np.nonzero(np.abs(xyz - canon[v]).max(axis=1) < tol)[0]
+        if cand.shape[0] != 1:
+            raise ValueError(f"vertex {v}: {cand.shape[0]} vertices at its periodic image (the mesh is not periodic)")
+        master[v] = cand[0]
+    return master
+
+
+class StrongPeriodicSystem:
+    """The strong pseudo-periodic scheme on the constrained (periodic) P1 space, built from the
+    unconstrained oracle matrices by P^T X P (P maps the constrained DOFs onto the vertices they
+    stand for), with scalar D per compartment.  Per step (ThetaMethodF/L_sBC1c/2c, DmriFemLib.py:
+    167-238, theta on both sides as there, F = int_0^t f, FA = F(t_n), FB = F(t_{n-1})):
+      A_n = M/k + theta (K + R + kappa I) + theta q^2 FA^2 D_c M + i theta q FA (C - B).g
+      B_n = M/k - theta (K + R + kappa I) - theta q^2 FB^2 D_c M - i theta q FB (C - B).g
+    with C, B the orc_assemble_sbc terms (the 1e-16 the reference adds to q F in outer_interface is
+    left out: it moves the signal by about 1e-17).  Solved by a sparse LU per step (exact discrete
+    scheme); signal = sum (M 1) Re u / |Omega|."""
+
+    def __init__(self, xyz, tets, phase=None, D=3e-3, invT2=1e-16, kappa=0.0, pdir=(1, 1, 1)):
+        import scipy.sparse as sps
+
+        self.base = System(xyz, tets, phase, D=D, invT2=invT2, kappa=kappa)
+        S = self.base
+        X = S.xyz[S.tets]
+        ii, jj = np.triu_indices(4, 1)
+        hmin = float(np.linalg.norm(X[:, ii] - X[:, jj], axis=2).max(axis=1).min())
+        self.master = periodic_masters(S.xyz, pdir, 1e-2 * hmin)
+        L = lib()
+        Dc = np.ascontiguousarray(np.broadcast_to(np.asarray(D, dtype=np.float64), (2,)))
+        ext = {k: np.zeros(S.nnz) for k in ("cx", "cy", "cz", "bx", "by", "bz")}
+        rc = L.orc_assemble_sbc(S.nv, _p(S.xyz, ctypes.c_double), S.nc, _p(S.tets, ctypes.c_int32),
+                                _p(S.phase, ctypes.c_int32), _p(S.dof_of, ctypes.c_int32), _p(Dc, ctypes.c_double),
+                                _p(S.rowptr, ctypes.c_int64), _p(S.col, ctypes.c_int32), S.nnz,
+                                *[_p(ext[k], ctypes.c_double) for k in ("cx", "cy", "cz", "bx", "by", "bz")])
+        if rc != 0:
+            raise RuntimeError(f"oracle sBC assembly failed ({rc})")
+        self.ext = ext
+        # constrained DOFs: the DOF of (compartment, master vertex)
+        nv, ncomp = S.nv, S.ncomp
+        key = np.full(S.ndof, -1, dtype=np.int64)
+        for c in range(ncomp):
+            d = S.dof_of[c * nv:(c + 1) * nv]
+            live = np.nonzero(d >= 0)[0]
+            dm = S.dof_of[c * nv + self.master[live]]
+            if np.any(dm < 0):
+                raise ValueError("a periodic image vertex has no DOF in the compartment of its master")
+            key[d[live]] = dm
+        uniq, self.merged_of = np.unique(key, return_inverse=True)
+        self.n = uniq.shape[0]
+        rows = np.repeat(np.arange(S.ndof), np.diff(S.rowptr))
+        P = sps.csr_matrix((np.ones(S.ndof), (np.arange(S.ndof), self.merged_of)), shape=(S.ndof, self.n))
+
+        def merged(vals):
+            A = sps.csr_matrix((vals, (rows, S.col)), shape=(S.ndof, S.ndof))
+            return (P.T @ A @ P).tocsr()
+
+        self.M = merged(S.mats["m"])
+        self.L = merged(S.mats["k"] + S.mats["r"] + S.mats["gi"])
+        comp = np.zeros(S.ndof)
+        if ncomp == 2:
+            comp[S.dof_of[nv:][S.dof_of[nv:] >= 0]] = 1.0
+        dcomp = np.where(comp == 1.0, Dc[1], Dc[0])
+        self.Mgg = merged(S.mats["m"] * dcomp[rows])
+        self.G = [merged(ext["c" + a] - ext["b" + a]) for a in "xyz"]
+        self.w = np.asarray(self.M.sum(axis=1)).ravel()
+        self.volume = float(self.w.sum())
+
+    def solve(self, gdir, q, dt, FA, FB, theta=0.5, u0=None):
+        import scipy.sparse.linalg as spl
+
+        g = np.asarray(gdir, dtype=np.float64)
+        g = g / np.linalg.norm(g)
+        Gg = g[0] * self.G[0] + g[1] * self.G[1] + g[2] * self.G[2]
+        u = np.ones(self.n, dtype=np.complex128) if u0 is None else np.asarray(u0, dtype=np.complex128)
+        base_a = (self.M / dt + theta * self.L).astype(np.complex128)
+        base_b = (self.M / dt - theta * self.L).astype(np.complex128)
+        for fa, fb in zip(FA, FB):
+            A = base_a + (theta * q * q * fa * fa) * self.Mgg + (1j * theta * q * fa) * Gg
+            B = base_b - (theta * q * q * fb * fb) * self.Mgg - (1j * theta * q * fb) * Gg
+            u = spl.splu(A.tocsc()).solve(B @ u)
+        return float((self.w * u.real).sum() / self.volume), u

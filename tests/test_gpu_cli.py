@@ -191,6 +191,6 @@ def test_cli_rejects_strong_periodic_and_refinement(tmp_path):
     m = dm.box_mesh((0.0,) * 3, (2.0,) * 3, 2, 2, 2)
     dm.write_dolfin_xml(m, str(tmp_path / "box.xml"))
     for extra in (["-pdir", 1, 0, 0, "-IsPeriodic", 1], ["-r", 1]):
-        r = subprocess.run([CLI, "-f", "box.xml", "-K", 2e-3] + [str(a) for a in extra], capture_output=True,
+        r = subprocess.run([CLI, "-f", "box.xml", "-K", "2e-3"] + [str(a) for a in extra], capture_output=True,
                            text=True, cwd=tmp_path, timeout=60)
         assert r.returncode != 0 and ("strong" in r.stderr or "refinement" in r.stderr), r.stderr

@@ -312,8 +312,16 @@ def test_sampled_event_profile_and_spmv_replay():
     assert s0 == s1
     assert st0["spmv_launches"] == 0
     assert len(fA) <= st1["spmv_launches"] <= 2 * len(fA) and st1["spmv_ms"] > 0
-    us_v, us_t, nb = ctx.time_spmv(1, False, 10)
     info = ctx.info()
+    # every kernel of the timed iterations: the six of a preconditioned iteration, with their own bytes
+    nl = st1["kern_launches"]
+    assert nl[0] == nl[1] == nl[2] == nl[3] == nl[4] == nl[5] == st1["spmv_launches"] // 2 > 0
+    assert all(m > 0 for m in st1["kern_ms"])
+    n, nnz = info["ndof"], info["nnz"]
+    E2 = nnz * 20.0 + 4.0 * (n + 1)
+    assert abs(st1["kern_bytes"][2] / nl[2] - (E2 + 48.0 * n)) <= 1e-9 * E2
+    assert abs(st1["kern_bytes"][5] / nl[5] - 128.0 * n) <= 1e-9 * n
+    us_v, us_t, nb = ctx.time_spmv(1, False, 10)
     assert us_v > 0 and us_t > 0
     assert nb == info["nnz"] * 20.0 + 4.0 * (info["ndof"] + 1) + 48.0 * info["ndof"]
 
