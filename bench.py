@@ -87,10 +87,10 @@ def cos_ogse(nperiod, delta, Delta, t0=100.0, ext=100.0):
 
 fibonacci_directions = DS.fibonacci_directions
 
-KERNEL_NAMES = ("bicg_p", "spmv_v_pre", "spmv_v", "spmv_t_pre", "spmv_t", "bicg_update")
+KERNEL_NAMES = ("bicg_p", "spmv_v_pre", "spmv_v", "spass", "spmv_t_pre", "spmv_t", "bicg_update")
 KERNEL_DESC = {"bicg_p": "p = r + beta (p - omega v)", "spmv_v_pre": "ph = c (2 p - c A p)", "spmv_v": "v = A ph",
-               "spmv_t_pre": "s = r - alpha v, sh = c (2 s - c A s)", "spmv_t": "t = A sh",
-               "bicg_update": "x += alpha ph + omega sh, r = s - omega t"}
+               "spass": "alpha = rho / <r^, v>, s = r - alpha v", "spmv_t_pre": "sh = c (2 s - c A s)",
+               "spmv_t": "t = A sh", "bicg_update": "x += alpha ph + omega sh, r = s - omega t"}
 
 
 def merge_stats(stats):

@@ -26,8 +26,8 @@ Reference behaviour kept (file:line in DmriFemLib.py of the reference):
   * PostProcessing: signal = int (1-p) u0r + p u1r / int u(0) and the result lines and log.txt
     append of 917-988.
 The PDE is solved by libdmri; this module only prepares inputs and formats outputs.
-Not mirrored: dolfin plotting, .pvd output (MRI_simulation.save_snapshots keeps u every nskip
-steps as arrays instead), and the strong pseudo-periodic function spaces.
+Not mirrored: dolfin plotting and .pvd output (MRI_simulation.save_snapshots keeps u every nskip
+steps as arrays instead).
 """
 from __future__ import annotations
 
@@ -114,9 +114,10 @@ def convert_q2g(qvalue):
 
 
 class KrylovSolver:
-    """Solver settings holder (KrylovSolver("bicgstab", "jacobi") in GCloudDmriSolver.py:219-222).
-    libdmri runs BiCGStab with Jacobi; other method names are accepted for compatibility and run
-    the same converged-solution solver."""
+    """Solver settings holder (KrylovSolver("bicgstab", "jacobi") in GCloudDmriSolver.py:219-222;
+    KrylovSolver("gmres", ...) in the notebooks).  "gmres" runs libdmri's restarted GMRES, every
+    other method its BiCGStab; both with libdmri's Jacobi-scaled preconditioner and converged to the
+    relative tolerance (the preconditioner name is kept for compatibility)."""
 
     def __init__(self, method="bicgstab", pc="jacobi"):
         self.method, self.pc = method, pc
@@ -226,12 +227,12 @@ class MyDomain:
         self.D = np.stack(comps, axis=1)
 
     def Apply(self):
-        # PeriodicDir with IsDomainPeriodic = False: the weak pseudo-periodic condition
-        # (MyDomain.Apply, DmriFemLib.py:618-637) runs in libdmri; the strong form
-        # (IsDomainPeriodic = True, PeriodicBD 327-375) is not available
-        if sum(self.PeriodicDir) > 0 and self.IsDomainPeriodic:
-            raise NotImplementedError("the strong pseudo-periodic formulation (IsDomainPeriodic=True) is not "
-                                      "available in this build; set IsDomainPeriodic=False for the weak one")
+        # PeriodicDir with IsDomainPeriodic = False: the weak pseudo-periodic condition; with
+        # IsDomainPeriodic = True the strong one on the periodic (constrained) space (MyDomain.Apply,
+        # DmriFemLib.py:618-637; PeriodicBD 327-375); both run in libdmri
+        if sum(self.PeriodicDir) > 0:
+            print("The pseudo-periodic BCS are strongly imposed." if self.IsDomainPeriodic else
+                  "The pseudo-periodic BCS are weakly imposed.")
         if self.IsDomainMultiple:
             if self.phase is None:
                 raise ValueError("IsDomainMultiple=True needs mydomain.phase (per-cell 0/1)")
@@ -252,18 +253,21 @@ class MRI_simulation:
         self.save_snapshots = False
 
     def _sequence(self, mri_para, integral=False):
-        """f(t_n), f(t_{n-1}) and, for the periodic terms, F(t_{n-1}) = itime_profile(tp)
-        (MRI_simulation.solve, DmriFemLib.py:897-910)."""
-        fA, fB, FB = [], [], []
+        """f(t_n), f(t_{n-1}) and, for the periodic terms, F(t_n) = itime_profile(t) and F(t_{n-1}) =
+        itime_profile(tp) (MRI_simulation.solve, DmriFemLib.py:897-910)."""
+        fA, fB, FA, FB = [], [], [], []
         t, tp = 0.0, 0.0
         while t < mri_para.T + self.k:
             fA.append(mri_para.time_profile(t))
             fB.append(mri_para.time_profile(tp))
             if integral:
+                FA.append(mri_para.itime_profile(t))
                 FB.append(mri_para.itime_profile(tp))
             tp = t
             t += self.k
-        return np.asarray(fA), np.asarray(fB), (np.asarray(FB) if integral else None)
+        if not integral:
+            return np.asarray(fA), np.asarray(fB), None, None
+        return np.asarray(fA), np.asarray(fB), np.asarray(FB), np.asarray(FA)
 
     def InitialCondition(self, mydomain, ic):
         """Initial state per mesh vertex (MRI_simulation.InitialCondition, DmriFemLib.py:865-876): the
@@ -301,16 +305,20 @@ class MRI_simulation:
             ctx.set_coeffs(D=0.0, D_cell=np.asarray(D), invT2=invT2 if invT2 is not None else 0.0,
                            kappa=float(mydomain.kappa), invT2_cell=invT2_cell)
         periodic = sum(mydomain.PeriodicDir) > 0
-        ctx.set_periodic(mydomain.PeriodicDir if periodic else (0, 0, 0))
+        if periodic and mydomain.IsDomainPeriodic:
+            ctx.set_periodic_strong(mydomain.PeriodicDir)
+        else:
+            ctx.set_periodic(mydomain.PeriodicDir if periodic else (0, 0, 0))
         ctx.assemble()
-        fA, fB, FB = self._sequence(mri_para, integral=periodic)
+        fA, fB, FB, FA = self._sequence(mri_para, integral=periodic)
         p = linsolver.parameters
         start = time.time()
         g = mri_para.gdir.v if isinstance(mri_para.gdir, Point) else np.asarray(mri_para.gdir, dtype=np.float64)
         self.signal_c, self.stats = ctx.solve(fA, fB, self.k, mri_para.qvalue, g, theta=self.theta,
                                               rtol=p["relative_tolerance"], atol=p["absolute_tolerance"],
                                               maxit=min(int(p["maximum_iterations"]), 2 ** 31 - 1),
-                                              F_explicit=FB, u0=self.u0_vertex,
+                                              F_explicit=FB, F_implicit=FA, u0=self.u0_vertex,
+                                              krylov=1 if str(linsolver.method).lower() == "gmres" else 0,
                                               snap_every=self.nskip if self.save_snapshots else 0)
         self.elapsed_time = time.time() - start
         if self.save_snapshots:

@@ -8,6 +8,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -34,6 +35,11 @@ struct SolverWork {
     DevArray<double2> x, r, rh, p, v, t, b, xprev, xprev2, ybc, bper, ph, sh, sv;
     DevArray<double2> yh[kMaxHist], hp[kMaxHist], hq[kMaxHist];  // projection history
     DevArray<double> sc, vals, bre, areal, pdiag, part, tot, coef, coefP, sig, spart, ppart, ptot, pgam, gsig, u0v;
+    DevArray<double> afix, bfix, mgg;  // strong periodic form: fixed real parts and the q^2 F^2 mass term
+    // restarted GMRES: basis, new vector, partials, coefficients, Hessenberg, rotations, totals
+    DevArray<double2> gV, gw, gH, gcs, gg, gy;
+    DevArray<double> gpart, gh, ghsum, gcc, ginv, gtot, gwn;
+    DevArray<int> gnk;
     DevArray<KrylovState> st;
     DevArray<int> all_done, mask;
     KrylovState *h_st = nullptr, *d_h_st = nullptr;  // pinned host copy and its device address
@@ -78,6 +84,10 @@ struct dmri_ctx {
     DevArray<double> w;
     // weak pseudo-periodic boundary condition
     int pdir[3] = {0, 0, 0};
+    bool strong = false;                   // the periodic directions are imposed by the strong form
+    bool strong_active = false;            // assembled with the strong periodic constraint
+    std::vector<double> h_xyz;             // host copy of the coordinates (periodic vertex matching)
+    DevArray<double> gval[3];              // strong form: (C - B)_d = (C_d - C_d^T)/2 on the constrained space
     bool periodic = false;                 // assembled with periodic directions
     int64_t n_per = 0, n_prow = 0;
     DevArray<double> pval, per_w, per_ell;
@@ -276,6 +286,131 @@ void assemble_periodic(dmri_ctx *c) {
     }
 }
 
+// Master vertex of every vertex under the strong periodic constraint (PeriodicBD, DmriFemLib.py:
+// 327-375): a vertex within tol of the max face of a periodic axis stands for the vertex at the same
+// point moved to the min face (map() moves x by -L along that axis); the chain of maps is followed to
+// its end, so a vertex on several max faces takes the vertex at the corner of the min faces.  The
+// image is matched within tol (max norm) among the vertices near a min face; it must be unique.
+std::vector<int> periodic_masters(const dmri_ctx *c, double tol, int64_t &moved) {
+    const int64_t nv = c->nv;
+    const double *x = c->h_xyz.data();
+    std::vector<int> master(nv);
+    auto cell = [&](const double *p, int d) { return (int64_t)std::floor((p[d] - c->bb_lo[d]) / tol); };
+    auto hkey = [](int64_t i, int64_t j, int64_t k) {
+        return ((uint64_t)(i & 0x1FFFFF) << 42) | ((uint64_t)(j & 0x1FFFFF) << 21) | (uint64_t)(k & 0x1FFFFF);
+    };
+    std::unordered_multimap<uint64_t, int> grid;
+    for (int64_t v = 0; v < nv; ++v) {
+        master[v] = (int)v;
+        const double *p = x + 3 * v;
+        bool near_min = false;
+        for (int d = 0; d < 3; ++d) near_min = near_min || (c->pdir[d] && std::fabs(p[d] - c->bb_lo[d]) < tol);
+        if (near_min) grid.emplace(hkey(cell(p, 0), cell(p, 1), cell(p, 2)), (int)v);
+    }
+    moved = 0;
+    for (int64_t v = 0; v < nv; ++v) {
+        const double *p = x + 3 * v;
+        double q[3] = {p[0], p[1], p[2]};
+        bool mv = false;
+        for (int d = 0; d < 3; ++d)
+            if (c->pdir[d] && std::fabs(p[d] - c->bb_hi[d]) < tol) {
+                q[d] = c->bb_lo[d];
+                mv = true;
+            }
+        if (!mv) continue;
+        int found = -1, count = 0;
+        const int64_t ci = cell(q, 0), cj = cell(q, 1), ck = cell(q, 2);
+        for (int64_t di = -1; di <= 1; ++di)
+            for (int64_t dj = -1; dj <= 1; ++dj)
+                for (int64_t dk = -1; dk <= 1; ++dk) {
+                    auto rg = grid.equal_range(hkey(ci + di, cj + dj, ck + dk));
+                    for (auto it = rg.first; it != rg.second; ++it) {
+                        const double *o = x + 3 * (int64_t)it->second;
+                        if (std::fabs(o[0] - q[0]) < tol && std::fabs(o[1] - q[1]) < tol && std::fabs(o[2] - q[2]) < tol) {
+                            found = it->second;
+                            ++count;
+                        }
+                    }
+                }
+        require(count == 1, DMRI_ERR_MESH,
+                "strong periodic condition: vertex " + std::to_string(v) + " has " + std::to_string(count) +
+                    " vertices at its periodic image (the mesh is not periodic within 1e-2 hmin)");
+        master[v] = found;
+        ++moved;
+    }
+    return master;
+}
+
+template <class T>
+void take(DevArray<T> &dst, DevArray<T> &src) {
+    std::swap(dst.p, src.p);
+    std::swap(dst.n, src.n);
+}
+
+// The strong periodic constraint on the assembled unconstrained system: the direction terms
+// (C - B)_d = (C_d - C_d^T)/2 (C + C^T = 2B, the boundary integral of every compartment's outer
+// normal, for a scalar D per compartment), then every value array becomes P^T X P on the space
+// where a DOF on a max face is its master's DOF (DOLFIN's constrained function space), with the DOF
+// map, pattern and counts replaced by the constrained ones.
+void apply_periodic_constraint(dmri_ctx *c, DevArray<double> *cvals) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->ndof, nv = c->nv, nnz = c->nnz;
+    int64_t moved = 0;
+    const std::vector<int> master = periodic_masters(c, 1e-2 * c->hmin, moved);
+    DevArray<int> dmaster, key, ism, scan, mid, bad;
+    dmaster.upload(master.data(), nv, s);
+    key.alloc(n);
+    ism.alloc(n);
+    scan.alloc(n + 1);
+    mid.alloc(n);
+    bad.alloc(1);
+    bad.zero(s);
+    launch_master_dofs(n, nv, c->dof_vert.p, c->dof_comp.p, dmaster.p, c->dof_of.p, key.p, ism.p, bad.p, s);
+    require(read_scalar(bad.p, s) == 0, DMRI_ERR_MESH,
+            "strong periodic condition: a periodic image has no DOF in its master's compartment");
+    exclusive_scan(ism.p, scan.p, n, s);
+    const int64_t nm = read_scalar(scan.p + n, s);
+    launch_merged_ids(n, key.p, scan.p, mid.p, s);
+    DevArray<double> g[3];
+    {
+        DevArray<int> miss;
+        miss.alloc(1);
+        miss.zero(s);
+        for (int d = 0; d < 3; ++d) {
+            g[d].alloc(nnz);
+            launch_antisym(n, c->rowptr.p, c->col.p, cvals[d].p, g[d].p, miss.p, s);
+        }
+        require(read_scalar(miss.p, s) == 0, DMRI_ERR_MESH, "the sparsity pattern is not symmetric");
+    }
+    DevArray<int> mrow, mcol, rstart, sidx;
+    const int64_t nu = build_merge(n, nnz, c->rowptr.p, c->col.p, mid.p, nm, mrow, mcol, rstart, sidx, s);
+    auto merge = [&](DevArray<double> &a) {
+        DevArray<double> t;
+        t.alloc(nu);
+        launch_run_sum(nu, nnz, rstart.p, sidx.p, a.p, t.p, s);
+        take(a, t);
+    };
+    for (auto &a : c->val) merge(a);
+    for (int d = 0; d < 3; ++d) {
+        merge(g[d]);
+        take(c->gval[d], g[d]);
+    }
+    DevArray<int> nvert, ncomp;
+    nvert.alloc(nm);
+    ncomp.alloc(nm);
+    launch_constrained_dofs(n, (int64_t)c->ncomp * nv, ism.p, mid.p, c->dof_vert.p, c->dof_comp.p, c->dof_of.p,
+                            nvert.p, ncomp.p, s);
+    take(c->dof_vert, nvert);
+    take(c->dof_comp, ncomp);
+    take(c->rowptr, mrow);
+    take(c->col, mcol);
+    DMRI_HIP(hipStreamSynchronize(s));
+    c->ndof = nm;
+    c->nnz = nu;
+    c->n_per = n - nm;
+    (void)moved;
+}
+
 // Interface matrix kappa int_Gamma (u0-u1)(v0-v1) into gi (zeroed first), one Jones-Plassmann
 // colour of facets per launch; returns the number of colours.
 int assemble_interface(dmri_ctx *c, double kappa, const double *kappa_f, double *gi, int *miss) {
@@ -381,8 +516,14 @@ void assemble_impl(dmri_ctx *c) {
     }
 
     // 5. cell matrices by row gather (every entry written once); interface facets by colour
+    const bool strong = c->strong && (c->pdir[0] || c->pdir[1] || c->pdir[2]);
+    require(!strong || !c->has_Dcell, DMRI_ERR_ARG,
+            "the strong periodic condition needs a scalar D per compartment (no D_cell field)");
     for (auto &a : c->val) a.alloc(c->nnz);
     c->val[DMRI_MAT_I].zero(s);
+    DevArray<double> cvals[3];  // strong form: int 2 D (d phi_j / d x_d) phi_i per axis
+    if (strong)
+        for (auto &a : cvals) a.alloc(c->nnz);
     DevArray<int> miss;
     miss.alloc(1);
     miss.zero(s);
@@ -390,17 +531,21 @@ void assemble_impl(dmri_ctx *c) {
     launch_assemble_rows(c->ndof, c->dof_vert.p, c->dof_comp.p, c->rowptr.p, c->col.p, c->tets.p, phase, c->xyz.p,
                          c->vptr.p, c->vlist.p, c->Dc.p, c->has_Dcell ? c->Dcell.p : nullptr, c->iT2c.p,
                          c->has_iT2cell ? c->iT2cell.p : nullptr, c->val[DMRI_MAT_M].p, c->val[DMRI_MAT_K].p,
-                         c->val[DMRI_MAT_R].p, c->val[DMRI_MAT_JX].p, c->val[DMRI_MAT_JY].p, c->val[DMRI_MAT_JZ].p, s);
+                         c->val[DMRI_MAT_R].p, c->val[DMRI_MAT_JX].p, c->val[DMRI_MAT_JY].p, c->val[DMRI_MAT_JZ].p, s,
+                         strong ? cvals[0].p : nullptr, strong ? cvals[1].p : nullptr, strong ? cvals[2].p : nullptr);
     c->ncol_iface = 0;
     if (have_iface)
         c->ncol_iface = assemble_interface(c, c->kappa, c->has_kappa_f ? c->kappa_f.p : nullptr,
                                            c->val[DMRI_MAT_I].p, miss.p);
     require(read_scalar(miss.p, s) == 0, DMRI_ERR_MESH, "element entry outside the CSR pattern");
 
-    // 5b. weak pseudo-periodic boundary mass and mirror entries
-    c->periodic = (c->pdir[0] || c->pdir[1] || c->pdir[2]) && c->n_bfacet > 0;
+    // 5b. weak pseudo-periodic boundary mass and mirror entries, or the strong periodic constraint
+    c->periodic = !strong && (c->pdir[0] || c->pdir[1] || c->pdir[2]) && c->n_bfacet > 0;
     c->n_per = c->n_prow = 0;
+    c->strong_active = strong;
+    for (auto &a : c->gval) a.release();
     if (c->periodic) assemble_periodic(c);
+    if (strong) apply_periodic_constraint(c, cvals);
 
     // 6. diagonal positions, signal weights w = M 1, volume
     {
@@ -668,9 +813,13 @@ struct Stepper {
     ProjHistory fh;  // projection guess of the forward steps (fh.m = 0: off); fh.ys[0] = xc
     bool poly = true;       // degree-1 polynomial preconditioner
     bool real_pre = false;  // the polynomial of the real part alone
-    double kbytes[6] = {0, 0, 0, 0, 0, 0};  // algorithmic bytes per launch of the six iteration kernels
+    double kbytes[7] = {0, 0, 0, 0, 0, 0, 0};  // algorithmic bytes per launch of the iteration kernels
     int precond = 0;                         // dmri_stats.precond_used of this launch group
-    bool timed_first[kEventPool / 6] = {};   // timed iteration k of a chunk is the solve's first
+    bool strong = false;                     // the strong periodic form (per-step real parts)
+    bool gmres_on = false;                   // restarted GMRES instead of BiCGStab
+    GmArgs gm{};
+    std::vector<double> s2A, s2B;            // strong form: theta q^2 F^2 per step, implicit and explicit
+    bool timed_first[kEventPool / 7] = {};   // timed iteration k of a chunk is the solve's first
     double omt = 0.5;
     int chunk = 8;
 
@@ -713,10 +862,40 @@ struct Stepper {
         const double *pv = c->periodic ? c->pval.p : nullptr;
         launch_prep_scale(n, c->sdiag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, pv, dt,
                           theta, w.sc.p, s);
+        // the strong periodic form (ThetaMethodF/L_sBC1c/2c, DmriFemLib.py:167-238) takes theta on both
+        // sides, the direction terms (C - B).g in place of the gradient moments and F = int f in place of f
+        strong = c->strong_active;
+        const double *j3[3] = {V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p};
+        if (strong)
+            for (int d = 0; d < 3; ++d) j3[d] = c->gval[d].p;
         launch_prep_values(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p,
-                           V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, pv, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p,
-                           per_gdir, g[0], g[1], g[2], dt, theta, w.vals.p, w.bre.p, s);
+                           V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, pv, j3[0], j3[1], j3[2], per_gdir, g[0], g[1], g[2], dt,
+                           theta, w.vals.p, w.bre.p, s, strong ? theta : 1.0 - theta);
         omt = 1.0 - theta;
+        const int vw0 = per_gdir ? 4 : 2;
+        const double *fimp = strong ? seq->F_implicit : seq->f_implicit;
+        const double *fexp = strong ? seq->F_explicit : seq->f_explicit;
+        double s2max = 0.0;
+        if (strong) {
+            for (int j = 1; j < S; ++j)
+                require(q[j] == q[0], DMRI_ERR_ARG, "strong periodic form: the signals of a launch group share q");
+            s2A.assign(std::max(N, 1), 0.0);
+            s2B.assign(std::max(N, 1), 0.0);
+            for (int k = 0; k < N; ++k) {
+                s2A[k] = theta * q[0] * q[0] * fimp[k] * fimp[k];
+                s2B[k] = -theta * q[0] * q[0] * fexp[k] * fexp[k];
+                s2max = std::max(s2max, s2A[k]);
+            }
+            if (w.afix.n != (size_t)c->nnz) w.afix.alloc(c->nnz);
+            if (w.bfix.n != (size_t)c->nnz) w.bfix.alloc(c->nnz);
+            if (w.mgg.n != (size_t)c->nnz) w.mgg.alloc(c->nnz);
+            launch_extract_real(c->nnz, w.vals.p, vw0, w.afix.p, s);
+            DMRI_HIP(hipMemcpyAsync(w.bfix.p, w.bre.p, sizeof(double) * c->nnz, hipMemcpyDeviceToDevice, s));
+            launch_sbc_mgg(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p, V[DMRI_MAT_M].p,
+                           c->ncomp == 2 ? c->scomp.p : nullptr, c->D[0], c->D[1], w.mgg.p, s);
+            // the preconditioner's bounds are taken of the largest real part of the sequence
+            launch_sbc_step(c->nnz, vw0, w.afix.p, w.bfix.p, w.mgg.p, s2max, 0.0, w.vals.p, w.bre.p, nullptr, s);
+        }
         if (c->periodic) {
             // mirror phase per step and signal: q_s F(t_{n-1}) g_s
             std::vector<double> hP((size_t)std::max(N, 1) * S * 3, 0.0);
@@ -741,8 +920,8 @@ struct Stepper {
         double *cA = coef.data(), *cB = coef.data() + (size_t)std::max(N, 1) * S * 3;
         for (int k = 0; k < N; ++k)
             for (int j = 0; j < S; ++j) {
-                const double av = theta * q[j] * seq->f_implicit[k];
-                const double bv = -(1.0 - theta) * q[j] * seq->f_explicit[k];
+                const double av = theta * q[j] * fimp[k];
+                const double bv = -(strong ? theta : 1.0 - theta) * q[j] * fexp[k];
                 for (int d = 0; d < 3; ++d) {
                     const double gd = per_gdir ? g[3 * j + d] : (d == 0 ? 1.0 : 0.0);
                     cA[((size_t)k * S + j) * 3 + d] = av * gd;
@@ -776,7 +955,9 @@ struct Stepper {
         // 2u^n - u^(n-1), 3 = quadratic extrapolation 3u^n - 3u^(n-1) + u^(n-2)
         guess_order = opts->initial_guess == 1 ? 0 : (opts->initial_guess == 2 ? 1 : 2);
         extrap = guess_order > 0;
-        if (opts->initial_guess == 0) setup_projection(g);
+        // the projection keeps products with the real part, which the strong form changes every step:
+        // it takes the quadratic extrapolation instead
+        if (opts->initial_guess == 0 && !strong) setup_projection(g);
         a.x = xc;
         a.xprev = xp;
         a.r = w.r.p;
@@ -817,7 +998,7 @@ struct Stepper {
                 if (per_gdir) gs = std::max(gs, std::fabs(g[3 * j]) + std::fabs(g[3 * j + 1]) + std::fabs(g[3 * j + 2]));
             }
             double fmax = 0.0;
-            for (int k = 0; k < N; ++k) fmax = std::max(fmax, std::fabs(seq->f_implicit[k]));
+            for (int k = 0; k < N; ++k) fmax = std::max(fmax, std::fabs(fimp[k]));
             const double imag = theta * qf * fmax * gs * jb;
             const double breal = 1.8 / a.pc;
             real_pre = opts->preconditioner == 0 && imag <= kRealPreconditioner * breal;
@@ -827,7 +1008,7 @@ struct Stepper {
                 launch_extract_real(c->nnz, w.vals.p, vw, w.areal.p, s);
                 a.areal = w.areal.p;
                 st->precond_used = 2;
-            } else if ((geo.mapping == kMapSL || geo.mapping == kMapStaged) &&
+            } else if ((geo.mapping == kMapSL || geo.mapping == kMapStaged) && !strong &&
                        (opts->preconditioner == 2 || imag >= kComplexDiagonal * breal)) {
                 // The gradient term dominates the diagonal of rows far along g: the polynomial then
                 // acts on the operator scaled by its complex diagonal, whose spectrum stays near
@@ -853,6 +1034,8 @@ struct Stepper {
         a.rtol = opts->rtol;
         a.atol = opts->atol;
         a.maxit = opts->maxit;
+        gmres_on = opts->krylov == 1;
+        if (gmres_on) setup_gmres();
         // algorithmic bytes per launch of each iteration kernel (dmri_stats.kern_bytes): the operator
         // SpMVs v = A ph and t = A sh read their matrix entries and 48 B per row and signal (one vector
         // gathered, one read at the own row, one written)
@@ -864,14 +1047,138 @@ struct Stepper {
             kbytes[0] = 64.0 * nS;
             kbytes[1] = poly ? E(npre) + 32.0 * nS + cdx : 0.0;
             kbytes[2] = E(vw) + 48.0 * nS;
-            kbytes[3] = poly ? E(npre) + 64.0 * nS + cdx : 0.0;
-            kbytes[4] = E(vw) + 48.0 * nS;
-            kbytes[5] = 128.0 * nS;
+            kbytes[3] = poly ? 48.0 * nS : 0.0;
+            kbytes[4] = poly ? E(npre) + 32.0 * nS + cdx : 0.0;
+            kbytes[5] = E(vw) + 48.0 * nS;
+            kbytes[6] = 128.0 * nS;
         }
-        st->spmv_bytes = 0.5 * (kbytes[2] + kbytes[4]);
+        st->spmv_bytes = 0.5 * (kbytes[2] + kbytes[5]);
         precond = poly ? st->precond_used : 0;
         prof = opts->profile != 0;
         chunk = 8;
+    }
+
+    void setup_gmres() {
+        SolverWork &w = *wk;
+        const int m = opts->restart > 0 ? opts->restart : 30;
+        require(m >= 1 && m <= kGmMaxRestart, DMRI_ERR_ARG, "GMRES restart length must lie in [1, 64]");
+        const size_t nS = (size_t)n * S;
+        auto need = [](auto &arr, size_t cnt) {
+            if (arr.n != cnt) arr.alloc(cnt);
+        };
+        need(w.gV, (size_t)(m + 1) * nS);
+        need(w.gw, nS);
+        need(w.gH, (size_t)S * m * (m + 1));
+        need(w.gcs, (size_t)S * m);
+        need(w.gg, (size_t)S * (m + 1));
+        need(w.gy, (size_t)S * m);
+        need(w.gpart, (size_t)std::max(geo.vgrid, 1) * S * 2 * kGmChunk);
+        need(w.gh, (size_t)S * 2 * (m + 1));
+        need(w.ghsum, (size_t)S * 2 * (m + 1));
+        need(w.gcc, (size_t)S * m);
+        need(w.ginv, (size_t)S);
+        need(w.gtot, (size_t)S * kNV);
+        need(w.gwn, (size_t)S * kNV);
+        need(w.gnk, (size_t)S);
+        gm = GmArgs{};
+        gm.n = n;
+        gm.S = S;
+        gm.m = m;
+        gm.V = w.gV.p;
+        gm.w = w.gw.p;
+        gm.part = w.gpart.p;
+        gm.h = w.gh.p;
+        gm.hsum = w.ghsum.p;
+        gm.H = w.gH.p;
+        gm.cc = w.gcc.p;
+        gm.cs = w.gcs.p;
+        gm.g = w.gg.p;
+        gm.y = w.gy.p;
+        gm.inv = w.ginv.p;
+        gm.nk = w.gnk.p;
+        gm.st = w.st.p;
+        gm.all_done = w.all_done.p;
+        gm.rtol = opts->rtol;
+        gm.atol = opts->atol;
+        gm.maxit = opts->maxit;
+    }
+
+    // Restarted GMRES on a.x with the residual already formed (r and the partials of |r|^2, |b|^2):
+    // cycles of up to m Arnoldi steps queued at once (steps past a signal's end exit at once), then
+    // x += M (V y), the true residual and the next cycle's start; one host check per cycle.
+    void gmres(int step) {
+        SolverWork &w = *wk;
+        const int m = gm.m;
+        const size_t nS = (size_t)n * S;
+        launch_gm_fin(a.partU, geo.grid, S, kNV, kNV, w.gtot.p, kNV, 0, s);
+        launch_gm_start(gm, w.gtot.p, true, s);
+        launch_gm_v0(geo, gm, a.r, s);
+        for (int cycle = 0;; ++cycle) {
+            for (int j = 0; j < m; ++j) {
+                SolveArgs av = a;
+                double2 *vj = w.gV.p + (size_t)j * nS;
+                if (poly) {
+                    av.p = vj;
+                    launch_spmv_v(geo, av, 0, s, nullptr, nullptr, kOpPre);  // ph = M v_j
+                    av.p = a.ph;
+                } else {
+                    av.p = vj;
+                }
+                av.v = w.gw.p;
+                av.rh = w.gV.p;
+                launch_spmv_v(geo, av, 0, s);  // w = A M v_j
+                for (int pass = 0; pass < 2; ++pass) {
+                    for (int i0 = 0; i0 <= j; i0 += kGmChunk) {
+                        const int cnt = std::min(kGmChunk, j + 1 - i0);
+                        launch_gm_dots(geo, gm, i0, cnt, s);
+                        launch_gm_fin(w.gpart.p, geo.vgrid, S, 2 * kGmChunk, 2 * cnt, w.gh.p, 2 * (m + 1), 2 * i0, s);
+                    }
+                    launch_gm_orth(geo, gm, j, pass == 1, s);
+                }
+                launch_gm_fin(w.gpart.p, geo.vgrid, S, kNV, kNV, w.gwn.p, kNV, 0, s);
+                launch_gm_hess(gm, j, w.gwn.p, s);
+                launch_gm_scale(geo, gm, j, s);
+            }
+            // x += M (V y) for every signal that took steps this cycle
+            launch_reset_state_flags(w.st.p, S, w.all_done.p, s);
+            launch_gm_combine(geo, gm, a.p, s);
+            if (poly) {
+                SolveArgs av = a;
+                av.p = a.p;
+                av.ph = a.sh;
+                launch_spmv_v(geo, av, 0, s, nullptr, nullptr, kOpPre);
+                launch_gm_axpy((int64_t)nS, a.sh, a.x, s);
+            } else {
+                launch_gm_axpy((int64_t)nS, a.p, a.x, s);
+            }
+            launch_rhs_residual(geo, a, 1, nullptr, s);
+            launch_gm_fin(a.partU, geo.grid, S, kNV, kNV, w.gtot.p, kNV, 0, s);
+            launch_gm_start(gm, w.gtot.p, false, s);
+            launch_gm_v0(geo, gm, a.r, s);
+            launch_publish_state(w.st.p, S, w.all_done.p, w.d_h_st, w.d_h_flag, s);
+            DMRI_HIP(hipStreamSynchronize(s));
+            if (w.h_flag[0] == 0) continue;  // signals left that have not converged: another cycle
+            for (int j = 0; j < S; ++j)
+                if (!w.h_st[j].conv)
+                    throw Error(DMRI_ERR_NOCONV, "GMRES did not converge in " + std::to_string(opts->maxit) +
+                                                     " iterations at step " + std::to_string(step) + " (signal " +
+                                                     std::to_string(j) + ")");
+            break;
+        }
+        int step_max = 0;
+        for (int j = 0; j < S; ++j) {
+            const KrylovState &h = w.h_st[j];
+            st->iters_total += h.iters;
+            step_max = std::max(step_max, h.iters);
+            const double rel = h.bnorm2 > 0 ? std::sqrt(h.rnorm2 / h.bnorm2) : std::sqrt(h.rnorm2);
+            st->relres_max = std::max(st->relres_max, rel);
+        }
+        st->iters_max = std::max(st->iters_max, step_max);
+    }
+
+    void solve_krylov(int step) {
+        if (gmres_on) gmres(step);
+        else krylov(step);
     }
 
     // Residual-minimising guess over the last fh.m states (see proj_capacity); the quadratic
@@ -953,7 +1260,7 @@ struct Stepper {
         double2 *y0 = projected_guess(fh);
         a.x = y0;
         launch_rhs_residual(geo, a, 1, nullptr, s);
-        krylov(k);
+        solve_krylov(k);
         xc = y0;
         a.x = xc;
         fh.ys.push_front(y0);
@@ -1003,7 +1310,7 @@ struct Stepper {
         double2 *y = h.nh() > 0 ? projected_guess(h) : start;
         a.x = y;
         launch_rhs_residual(geo, a, 1, nullptr, s);
-        krylov(k);
+        solve_krylov(k);
         a.x = xc;
         return y;
     }
@@ -1024,6 +1331,9 @@ struct Stepper {
 
     void begin_step(int step) {
         SolverWork &w = *wk;
+        if (strong)
+            launch_sbc_step(c->nnz, per_gdir ? 4 : 2, w.afix.p, w.bfix.p, w.mgg.p, s2A[step], s2B[step], w.vals.p,
+                            w.bre.p, real_pre ? w.areal.p : nullptr, s);
         a.coefA = w.coef.p + (size_t)step * S * 3;
         a.coefB = w.coef.p + (size_t)std::max(N, 1) * S * 3 + (size_t)step * S * 3;
         launch_reset_state(w.st.p, S, w.all_done.p, s);
@@ -1033,7 +1343,7 @@ struct Stepper {
     void krylov(int step) {
         SolverWork &w = *wk;
         int it = 0, it_before = 0, restarts = 0;
-        constexpr int kK = 6;  // timed kernels per iteration (dmri_stats.kern_*)
+        constexpr int kK = 7;  // timed kernels per iteration (dmri_stats.kern_*)
         for (;;) {
             const int nl = chunk;
             int ntimed = 0;
@@ -1050,13 +1360,14 @@ struct Stepper {
                     SolveArgs av = a;
                     av.p = a.ph;
                     launch_spmv_v(geo, av, it, s, e0(2), e1(2));
-                    launch_spmv_t(geo, a, it, s, e0(3), e1(3), kOpPre);
-                    launch_spmv_t(geo, a, it, s, e0(4), e1(4), kOpTP);
+                    launch_spass(geo, a, it, s, e0(3), e1(3));
+                    launch_spmv_t(geo, a, it, s, e0(4), e1(4), kOpPre);
+                    launch_spmv_t(geo, a, it, s, e0(5), e1(5), kOpTP);
                 } else {
                     launch_spmv_v(geo, a, it, s, e0(2), e1(2));
-                    launch_spmv_t(geo, a, it, s, e0(4), e1(4));
+                    launch_spmv_t(geo, a, it, s, e0(5), e1(5));
                 }
-                launch_bicg_update(geo, a, it, s, e0(5), e1(5));
+                launch_bicg_update(geo, a, it, s, e0(6), e1(6));
                 if (timed) timed_first[ntimed++] = it == 0;
             }
             launch_publish_state(w.st.p, S, w.all_done.p, w.d_h_st, w.d_h_flag, s);
@@ -1071,14 +1382,14 @@ struct Stepper {
                 const int real = std::max(0, std::min(ntimed, it_last - it_before));
                 for (int k = 0; k < real; ++k)
                     for (int j = 0; j < kK; ++j) {
-                        if (!poly && (j == 1 || j == 3)) continue;
+                        if (!poly && (j == 1 || j == 3 || j == 4)) continue;
                         float ms = 0;
                         DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[kK * k + j], c->ev1[kK * k + j]));
                         st->kern_ms[j] += ms;
                         st->kern_launches[j] += 1;
                         // the first iteration's p = r reads one vector and writes one
                         st->kern_bytes[j] += kbytes[j] - (j == 0 && timed_first[k] ? 32.0 * (double)n * S : 0.0);
-                        if (j == 2 || j == 4) {
+                        if (j == 2 || j == 5) {
                             st->spmv_ms += ms;
                             st->spmv_launches += 1;
                         }
@@ -1169,7 +1480,7 @@ struct Stepper {
             if (extrap)
                 DMRI_HIP(hipMemcpyAsync(xp, xc, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
         }
-        krylov(k);
+        solve_krylov(k);
     }
 
     // Solve A_k y = rhs (complex-symmetric A_k) starting from y; one signal.
@@ -1178,7 +1489,7 @@ struct Stepper {
         DMRI_HIP(hipMemcpyAsync(wk->b.p, rhs, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
         a.x = y;
         launch_rhs_residual(geo, a, 1, nullptr, s);
-        krylov(k);
+        solve_krylov(k);
         a.x = xc;
     }
 
@@ -1430,17 +1741,28 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     c->solved = false;  // work.x no longer holds u^N of a plain solve
 }
 
-// Launch spmv_v and spmv_t `reps` times each (each followed by its finalize kernel, as in a
-// solve) on the prepared solver buffers; us_v, us_t = mean duration of one SpMV kernel from the
-// start and end timestamps of its own dispatch (events bound to the kernel, no marker packets).
-void replay_spmv(dmri_ctx *c, SolveGeom geo, int S, bool per_gdir, const std::vector<KrylovState> &st0,
-                 int reps, double &us_v, double &us_t) {
+// Replay the SpMV kernels of one preconditioned BiCGStab iteration as a solve launches them
+// (preconditioner SpMV on p, v = A ph, preconditioner SpMV on s = r - alpha v writing s, t = A sh),
+// `reps` times each, back to back, on the context's solver buffers; us[k] = mean duration of kernel
+// k from the start and end timestamps of its own dispatch (kernel-bound events, no marker packets).
+// poly = false replays the plain pair v = A p, t = A s instead (us[1] = us[3] = 0).
+void replay_spmv(dmri_ctx *c, SolveGeom geo, int S, bool per_gdir, const std::vector<KrylovState> &st0, int reps,
+                 bool poly, double us[4]) {
     hipStream_t s = c->stream;
     const TileCompression *tiles = tiles_for(c, geo);
     SolverWork &w = c->work;
     const int64_t n = c->ndof;
     const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
     if (w.part.n < 3 * np) w.part.alloc(3 * np);
+    const size_t nS = (size_t)n * S;
+    if (poly) {
+        if (w.ph.n != nS) w.ph.alloc(nS);
+        if (w.sh.n != nS) w.sh.alloc(nS);
+        if (w.sv.n != nS) w.sv.alloc(nS);
+        launch_init_x(n, S, w.sc.p, w.ph.p, s);
+        launch_init_x(n, S, w.sc.p, w.sh.p, s);
+        launch_init_x(n, S, w.sc.p, w.sv.p, s);
+    }
     SolveArgs a{};
     a.n = n;
     a.rowptr = c->srow.p;
@@ -1461,6 +1783,12 @@ void replay_spmv(dmri_ctx *c, SolveGeom geo, int S, bool per_gdir, const std::ve
     a.partU = w.part.p;
     a.partV = w.part.p + np;
     a.partT = w.part.p + 2 * np;
+    if (poly) {
+        a.ph = w.ph.p;
+        a.sh = w.sh.p;
+        a.s = w.sv.p;
+        a.pc = 0.9;
+    }
     const bool use_fin = S > 1 || std::max(geo.grid, geo.vgrid) > 512;  // as in a solve
     if (w.tot.n < (size_t)3 * S * kNV) w.tot.alloc(3 * (size_t)S * kNV);
     a.totU = use_fin ? w.tot.p : nullptr;
@@ -1473,59 +1801,95 @@ void replay_spmv(dmri_ctx *c, SolveGeom geo, int S, bool per_gdir, const std::ve
     attach_tiles(a, tiles);
     DMRI_HIP(hipMemcpyAsync(w.st.p, st0.data(), sizeof(KrylovState) * S, hipMemcpyHostToDevice, s));
     DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
-    launch_spmv_v(geo, a, 0, s);  // warm-up
-    launch_spmv_t(geo, a, 0, s);
-    reps = std::min(reps, kEventPool / 2);
-    for (int k = 0; k < reps; ++k) {
-        launch_spmv_v(geo, a, 0, s, c->ev0[2 * k], c->ev1[2 * k]);
-        launch_spmv_t(geo, a, 0, s, c->ev0[2 * k + 1], c->ev1[2 * k + 1]);
-    }
+    SolveArgs av = a;
+    if (poly) av.p = a.ph;
+    auto run = [&](int k, hipEvent_t e0, hipEvent_t e1) {
+        switch (k) {
+            case 0: launch_spmv_v(geo, a, 0, s, e0, e1, kOpPre); break;
+            case 1: launch_spmv_v(geo, av, 0, s, e0, e1); break;
+            case 2: launch_spmv_t(geo, a, 0, s, e0, e1, kOpPre); break;
+            default: launch_spmv_t(geo, a, 0, s, e0, e1, poly ? kOpTP : kOpT); break;
+        }
+    };
+    const int k0 = poly ? 0 : 1;
+    for (int k = k0; k < 4; k += poly ? 1 : 2) run(k, nullptr, nullptr);  // warm-up
+    reps = std::min(reps, kEventPool / 4);
+    for (int r = 0; r < reps; ++r)
+        for (int k = k0; k < 4; k += poly ? 1 : 2) run(k, c->ev0[4 * r + k], c->ev1[4 * r + k]);
     DMRI_HIP(hipStreamSynchronize(s));
-    double sv = 0, stt = 0;
-    for (int k = 0; k < reps; ++k) {
-        float ms = 0;
-        DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[2 * k], c->ev1[2 * k]));
-        sv += ms;
-        DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[2 * k + 1], c->ev1[2 * k + 1]));
-        stt += ms;
+    for (int k = 0; k < 4; ++k) {
+        us[k] = 0.0;
+        if (!poly && (k == 0 || k == 2)) continue;
+        double sum = 0;
+        for (int r = 0; r < reps; ++r) {
+            float ms = 0;
+            DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[4 * r + k], c->ev1[4 * r + k]));
+            sum += ms;
+        }
+        us[k] = 1e3 * sum / reps;
     }
-    us_v = 1e3 * sv / reps;
-    us_t = 1e3 * stt / reps;
 }
 
-// Time the SpMV kernel of every lanes-per-row candidate for S signals on the assembled pattern
-// and remember the fastest for later solves of this context.
-// Time spmv_v + spmv_t for every (mapping, lanes per row) candidate for S signals on the
-// assembled pattern and remember the fastest for later solves of this context.
-// us[5*6]: [mapping-1][log2 lanes] microseconds per (spmv_v + spmv_t), -1 where not valid.
-void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_mapping, double *us) {
-    require(c->assembled, DMRI_ERR_STATE, "dmri_tune: call dmri_assemble first");
-    require(S >= 1 && S <= kMaxSignals, DMRI_ERR_ARG, "dmri_tune: nsig must lie in [1, 64]");
-    set_device(c);
+// Scaled values and vectors of a tuning or replay run on the context's buffers (dt = 1, theta = 1/2,
+// the x direction, a small imaginary coefficient)
+void prepare_replay(dmri_ctx *c, const SolveGeom &g0, int S, bool per_gdir) {
     hipStream_t s = c->stream;
     const int64_t n = c->ndof;
-    const int cand[6] = {1, 2, 4, 8, 16, 32};
-    SolveGeom g0 = make_geom(n, 1, S, kMapSL);
     ensure_group_workspace(c, c->work, g0, 1, per_gdir);
     SolverWork &w = c->work;
     auto &V = c->val;
-    launch_prep_scale(n, c->sdiag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, nullptr,
-                      1.0, 0.5, w.sc.p, s);
-    launch_prep_values(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p,
-                       V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, nullptr, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p,
-                       V[DMRI_MAT_JZ].p, per_gdir, 1.0, 0.0, 0.0, 1.0, 0.5, w.vals.p, w.bre.p, s);
+    launch_prep_scale(n, c->sdiag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, nullptr, 1.0,
+                      0.5, w.sc.p, s);
+    launch_prep_values(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
+                       V[DMRI_MAT_I].p, nullptr, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p, V[DMRI_MAT_JZ].p, per_gdir, 1.0, 0.0,
+                       0.0, 1.0, 0.5, w.vals.p, w.bre.p, s);
     launch_init_x(n, S, w.sc.p, w.p.p, s);
     launch_init_x(n, S, w.sc.p, w.r.p, s);
     launch_init_x(n, S, w.sc.p, w.v.p, s);
     launch_init_x(n, S, w.sc.p, w.rh.p, s);
     std::vector<double> coef((size_t)S * 3, 1e-4);
     DMRI_HIP(hipMemcpyAsync(w.coef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
+}
+
+std::vector<KrylovState> replay_states(int S) {
     std::vector<KrylovState> st0(S);
     std::memset(st0.data(), 0, sizeof(KrylovState) * S);
-    for (auto &x : st0) x.rho[0] = make_double2(1.0, 0.0);
+    for (auto &x : st0) {
+        x.rho[0] = make_double2(1.0, 0.0);
+        x.alpha[0] = make_double2(0.5, 0.0);
+    }
+    return st0;
+}
+
+void install_tuning(dmri_ctx *c, int S, bool per_gdir, int mapping, int lanes) {
+    const long long key = (long long)S * 2 + (per_gdir ? 1 : 0);
+    const int val = mapping * 64 + lanes;
+    for (auto &t : c->tuned)
+        if (t.first == key) {
+            t.second = val;
+            return;
+        }
+    c->tuned.emplace_back(key, val);
+}
+
+// Time the SpMV kernels of a polynomial-preconditioned BiCGStab iteration (the library's default
+// solve) for every (mapping, lanes per row) candidate that runs it (mappings 1, 2 and 4; the
+// signals-in-registers and tile-compressed mappings have no preconditioner kernels) for S signals on
+// the assembled pattern, and remember the fastest for later solves of this context.
+// us[5*6]: [mapping-1][log2 lanes] microseconds per iteration's four SpMVs, -1 where not valid.
+void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_mapping, double *us) {
+    require(c->assembled, DMRI_ERR_STATE, "dmri_tune: call dmri_assemble first");
+    require(S >= 1 && S <= kMaxSignals, DMRI_ERR_ARG, "dmri_tune: nsig must lie in [1, 64]");
+    set_device(c);
+    const int64_t n = c->ndof;
+    const int cand[6] = {1, 2, 4, 8, 16, 32};
+    prepare_replay(c, make_geom(n, 1, S, kMapSL), S, per_gdir);
+    SolverWork &w = c->work;
+    const std::vector<KrylovState> st0 = replay_states(S);
     double best_us = 1e300;
     int def_mapping, def_lanes;
     default_mapping(S, def_mapping, def_lanes);
+    if (def_mapping == kMapSR || def_mapping == kMapTile) def_mapping = kMapStaged;
     *best_mapping = def_mapping;
     *best_lanes = def_lanes;
     double def_us = -1.0;
@@ -1536,14 +1900,15 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
             const int G = cand[i];
             double &slot = us[(m - 1) * 6 + i];
             slot = -1.0;
-            if (m == kMapWave ? G != 1 : (m == kMapSR ? S > kMaxSRSignals : G * S > 64)) continue;
+            if (m == kMapSR || m == kMapTile) continue;
+            if (m == kMapWave ? G != 1 : G * S > 64) continue;
             SolveGeom geo = make_geom(n, G, S, m);
             fit_spmv_grid(geo, per_gdir);
             const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
             if (w.part.n < 3 * np) w.part.alloc(3 * np);
-            double uv = 0, ut = 0;
-            replay_spmv(c, geo, S, per_gdir, st0, 20, uv, ut);
-            slot = uv + ut;
+            double u4[4];
+            replay_spmv(c, geo, S, per_gdir, st0, 20, true, u4);
+            slot = u4[0] + u4[1] + u4[2] + u4[3];
             if (m == def_mapping && G == def_lanes) def_us = slot;
             if (slot < best_us) {
                 best_us = slot;
@@ -1555,15 +1920,57 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
         *best_mapping = def_mapping;
         *best_lanes = def_lanes;
     }
-    const long long key = (long long)S * 2 + (per_gdir ? 1 : 0);
-    const int val = *best_mapping * 64 + *best_lanes;
-    bool found = false;
-    for (auto &t : c->tuned)
-        if (t.first == key) {
-            t.second = val;
-            found = true;
+    install_tuning(c, S, per_gdir, *best_mapping, *best_lanes);
+}
+
+// Tuning database: a text file of "device ndof nnz nsig per_gdir mapping lanes us" lines
+std::string device_name(const dmri_ctx *c) {
+    hipDeviceProp_t prop{};
+    DMRI_HIP(hipGetDeviceProperties(&prop, c->device));
+    std::string name = prop.gcnArchName;
+    for (auto &ch : name)
+        if (ch == ' ' || ch == ':') ch = '_';
+    return name.empty() ? "unknown" : name;
+}
+
+void tune_db_impl(dmri_ctx *c, const char *path, int S, bool per_gdir, int *best_lanes, int *best_mapping, int *hit) {
+    require(c->assembled, DMRI_ERR_STATE, "dmri_tune_db: call dmri_assemble first");
+    require(S >= 1 && S <= kMaxSignals, DMRI_ERR_ARG, "dmri_tune_db: nsig must lie in [1, 64]");
+    const std::string dev = device_name(c);
+    *hit = 0;
+    if (path && *path) {
+        if (FILE *f = std::fopen(path, "r")) {
+            char d[128];
+            long long nd = 0, nz = 0;
+            int ns = 0, pg = 0, mp = 0, ln = 0;
+            double u = 0;
+            int found_m = 0, found_l = 0;
+            while (std::fscanf(f, "%127s %lld %lld %d %d %d %d %lf", d, &nd, &nz, &ns, &pg, &mp, &ln, &u) == 8)
+                if (dev == d && nd == c->ndof && nz == c->nnz && ns == S && pg == (per_gdir ? 1 : 0)) {
+                    found_m = mp;  // the last matching line wins
+                    found_l = ln;
+                }
+            std::fclose(f);
+            if (found_m > 0) {
+                (void)make_geom(c->ndof, found_l, S, found_m);  // validates the stored choice
+                install_tuning(c, S, per_gdir, found_m, found_l);
+                *best_mapping = found_m;
+                *best_lanes = found_l;
+                *hit = 1;
+                return;
+            }
         }
-    if (!found) c->tuned.emplace_back(key, val);
+    }
+    std::vector<double> us(30);
+    tune_impl(c, S, per_gdir, best_lanes, best_mapping, us.data());
+    if (path && *path) {
+        const double u = us[(*best_mapping - 1) * 6 + (int)std::log2((double)*best_lanes)];
+        FILE *f = std::fopen(path, "a");
+        require(f != nullptr, DMRI_ERR_ARG, std::string("dmri_tune_db: cannot append to ") + path);
+        std::fprintf(f, "%s %lld %lld %d %d %d %d %.3f\n", dev.c_str(), (long long)c->ndof, (long long)c->nnz, S,
+                     per_gdir ? 1 : 0, *best_mapping, *best_lanes, u);
+        std::fclose(f);
+    }
 }
 
 void validate_seq(const dmri_seq *seq, const dmri_solver_opts *opts) {
@@ -1598,11 +2005,17 @@ std::vector<double> checked_signals(dmri_ctx *c, int nsig, const double *q, cons
     require(seq->snap_every >= 0, DMRI_ERR_ARG, "snap_every must be >= 0");
     require(seq->snap_every == 0 || seq->nsteps == 0 || seq->snap_out != nullptr, DMRI_ERR_ARG,
             "snap_every > 0 needs snap_out");
-    if (c->periodic && seq->nsteps > 0) {
+    if ((c->periodic || c->strong_active) && seq->nsteps > 0) {
         require(seq->F_explicit != nullptr, DMRI_ERR_ARG,
                 "periodic directions are set: dmri_seq.F_explicit (int_0^t f per step) is required");
         for (int i = 0; i < seq->nsteps; ++i)
             require(std::isfinite(seq->F_explicit[i]), DMRI_ERR_ARG, "F_explicit has a non-finite value");
+    }
+    if (c->strong_active && seq->nsteps > 0) {
+        require(seq->F_implicit != nullptr, DMRI_ERR_ARG,
+                "the strong periodic condition needs dmri_seq.F_implicit (int_0^t f at each step's time)");
+        for (int i = 0; i < seq->nsteps; ++i)
+            require(std::isfinite(seq->F_implicit[i]), DMRI_ERR_ARG, "F_implicit has a non-finite value");
     }
     std::vector<double> g(3 * (size_t)nsig);
     for (int j = 0; j < nsig; ++j) {
@@ -1614,6 +2027,19 @@ std::vector<double> checked_signals(dmri_ctx *c, int nsig, const double *q, cons
         for (int k = 0; k < 3; ++k) g[3 * (size_t)j + k] = d[k] / gn;
     }
     return g;
+}
+
+// Launch groups of up to G consecutive signals; with the strong periodic form a group also ends where
+// q changes (its real part depends on q).  (first signal, count) pairs.
+std::vector<std::pair<int, int>> launch_groups(const dmri_ctx *c, int nsig, const double *q, int G) {
+    std::vector<std::pair<int, int>> out;
+    for (int j0 = 0; j0 < nsig;) {
+        int S = 1;
+        while (j0 + S < nsig && S < G && (!c->strong_active || q[j0 + S] == q[j0])) ++S;
+        out.emplace_back(j0, S);
+        j0 += S;
+    }
+    return out;
 }
 
 // true when signals j0 .. j0+S-1 do not all share one gradient direction
@@ -1632,8 +2058,8 @@ void solve_batched_impl(dmri_ctx *c, int nsig, const double *q, const double *gd
     set_device(c);
     dmri_stats st{};
     const auto t0 = std::chrono::steady_clock::now();
-    for (int j0 = 0; j0 < nsig; j0 += kMaxSignals) {
-        const int S = std::min(kMaxSignals, nsig - j0);
+    for (const auto &gr : launch_groups(c, nsig, q, kMaxSignals)) {
+        const int j0 = gr.first, S = gr.second;
         solve_group(c, S, j0, nsig, q + j0, g.data() + 3 * (size_t)j0, group_per_gdir(g, j0, S), seq, opts,
                     sig + 2 * (size_t)j0, st);
     }
@@ -1679,8 +2105,8 @@ void batch_begin_impl(dmri_ctx *c, int nsig, const double *q, const double *gdir
     b->N = seq->nsteps;
     b->opts = *opts;
     const int G = group > 0 ? group : kMaxSignals;
-    for (int j0 = 0; j0 < nsig; j0 += G) {
-        const int S = std::min(G, nsig - j0);
+    for (const auto &rg : launch_groups(c, nsig, q, G)) {
+        const int j0 = rg.first, S = rg.second;
         auto gr = std::make_unique<dmri_batch::Group>();
         gr->j0 = j0;
         gr->S = S;
@@ -1719,9 +2145,9 @@ void batch_advance_impl(dmri_batch *b, int nsteps, dmri_stats *stats) {
         st.relres_max = std::max(st.relres_max, x.relres_max);
         st.spmv_ms += x.spmv_ms;
         st.spmv_launches += x.spmv_launches;
-        st.spmv_bytes = 0.5 * (gr->run.kbytes[2] + gr->run.kbytes[4]);
+        st.spmv_bytes = 0.5 * (gr->run.kbytes[2] + gr->run.kbytes[5]);
         st.precond_used = gr->run.precond;
-        for (int j = 0; j < 6; ++j) {
+        for (int j = 0; j < 7; ++j) {
             st.kern_ms[j] += x.kern_ms[j];
             st.kern_launches[j] += x.kern_launches[j];
             st.kern_bytes[j] += x.kern_bytes[j];
@@ -1795,6 +2221,7 @@ int dmri_mesh_set(dmri_ctx *c, int64_t nv, const double *xyz, int64_t nc, const 
         }
         set_device(c);
         c->xyz.upload(xyz, 3 * nv, c->stream);
+        c->h_xyz.assign(xyz, xyz + 3 * nv);
         c->tets.upload(tets, 4 * nc, c->stream);
         if (cell_phase) c->phase.upload(cell_phase, nc, c->stream);
         else c->phase.release();
@@ -1896,7 +2323,7 @@ int dmri_get_info(dmri_ctx *c, dmri_info *info) {
             info->ncolors_iface = c->ncol_iface;
             info->volume = c->volume;
             info->setup_ms = c->setup_ms;
-            info->n_periodic = c->periodic ? c->n_per : 0;
+            info->n_periodic = (c->periodic || c->strong_active) ? c->n_per : 0;
             info->kappa_e = c->periodic ? 3e-3 / c->hmin : 0.0;
         }
     });
@@ -1924,14 +2351,16 @@ int dmri_export_pattern(dmri_ctx *c, int64_t *rowptr, int32_t *col, int32_t *dof
 int dmri_export_values(dmri_ctx *c, int which, double *val) {
     return guarded([&] {
         require(c != nullptr && val != nullptr, DMRI_ERR_ARG, "null argument");
-        require(which >= 0 && which <= DMRI_MAT_P, DMRI_ERR_ARG, "unknown matrix id");
+        require(which >= 0 && which <= DMRI_MAT_GZ, DMRI_ERR_ARG, "unknown matrix id");
         require(c->assembled, DMRI_ERR_STATE, "not assembled");
         set_device(c);
-        if (which == DMRI_MAT_P && !c->periodic) {
+        const bool gdir_term = which >= DMRI_MAT_GX;
+        if ((which == DMRI_MAT_P && !c->periodic) || (gdir_term && !c->strong_active)) {
             std::memset(val, 0, sizeof(double) * c->nnz);
             return;
         }
-        const double *src = which == DMRI_MAT_P ? c->pval.p : c->val[which].p;
+        const double *src = which == DMRI_MAT_P ? c->pval.p : (gdir_term ? c->gval[which - DMRI_MAT_GX].p
+                                                                           : c->val[which].p);
         DMRI_HIP(hipMemcpyAsync(val, src, sizeof(double) * c->nnz, hipMemcpyDeviceToHost, c->stream));
         DMRI_HIP(hipStreamSynchronize(c->stream));
     });
@@ -1952,6 +2381,17 @@ int dmri_periodic_set(dmri_ctx *c, const int32_t dir[3]) {
     return guarded([&] {
         require(c != nullptr && dir != nullptr, DMRI_ERR_ARG, "null argument");
         for (int d = 0; d < 3; ++d) c->pdir[d] = dir[d] != 0 ? 1 : 0;
+        c->strong = false;
+        c->assembled = false;
+        c->solved = false;
+    });
+}
+
+int dmri_periodic_strong_set(dmri_ctx *c, const int32_t dir[3]) {
+    return guarded([&] {
+        require(c != nullptr && dir != nullptr, DMRI_ERR_ARG, "null argument");
+        for (int d = 0; d < 3; ++d) c->pdir[d] = dir[d] != 0 ? 1 : 0;
+        c->strong = true;
         c->assembled = false;
         c->solved = false;
     });
@@ -2016,33 +2456,37 @@ int dmri_time_spmv(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t reps, do
                 mapping = t.second / 64;
                 lanes = t.second % 64;
             }
+        // the preconditioned solve runs mappings 1, 2 and 4 (others fall back to 2)
+        if (mapping == kMapSR || mapping == kMapTile) {
+            mapping = kMapStaged;
+            while (lanes > 1 && lanes * nsig > 64) lanes >>= 1;
+        }
         SolveGeom geo = make_geom(c->ndof, lanes, nsig, mapping);
         fit_spmv_grid(geo, pg);
-        std::vector<KrylovState> st0(nsig);
-        std::memset(st0.data(), 0, sizeof(KrylovState) * nsig);
-        for (auto &x : st0) x.rho[0] = make_double2(1.0, 0.0);
         // the buffers keep the last solve's scaled values and vectors when it had this shape
-        if (c->work.x.n != (size_t)c->ndof * nsig || c->work.vals.n != (size_t)c->nnz * (pg ? 4 : 2)) {
-            ensure_group_workspace(c, c->work, geo, 1, pg);
-            auto &V = c->val;
-            SolverWork &w = c->work;
-            launch_prep_scale(c->ndof, c->sdiag.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p, V[DMRI_MAT_R].p,
-                              V[DMRI_MAT_I].p, nullptr, 1.0, 0.5, w.sc.p, c->stream);
-            launch_prep_values(c->ndof, c->srow.p, c->scol.p, c->emap.p, w.sc.p, V[DMRI_MAT_M].p, V[DMRI_MAT_K].p,
-                               V[DMRI_MAT_R].p, V[DMRI_MAT_I].p, nullptr, V[DMRI_MAT_JX].p, V[DMRI_MAT_JY].p,
-                               V[DMRI_MAT_JZ].p, pg, 1.0, 0.0, 0.0, 1.0, 0.5, w.vals.p, w.bre.p, c->stream);
-            launch_init_x(c->ndof, nsig, w.sc.p, w.p.p, c->stream);
-            launch_init_x(c->ndof, nsig, w.sc.p, w.r.p, c->stream);
-            launch_init_x(c->ndof, nsig, w.sc.p, w.v.p, c->stream);
-            launch_init_x(c->ndof, nsig, w.sc.p, w.rh.p, c->stream);
-            std::vector<double> coef((size_t)nsig * 3, 1e-4);
-            DMRI_HIP(hipMemcpyAsync(w.coef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice,
-                                    c->stream));
-        }
-        replay_spmv(c, geo, nsig, pg, st0, reps, *us_v, *us_t);
+        if (c->work.x.n != (size_t)c->ndof * nsig || c->work.vals.n != (size_t)c->nnz * (pg ? 4 : 2))
+            prepare_replay(c, geo, nsig, pg);
+        double u4[4];
+        replay_spmv(c, geo, nsig, pg, replay_states(nsig), reps, true, u4);
+        *us_v = u4[1];
+        *us_t = u4[3];
         if (bytes_per_launch)
             *bytes_per_launch = (double)c->nnz * (4.0 + 8.0 * (pg ? 4 : 2)) + 4.0 * (double)(c->ndof + 1) +
                                 48.0 * (double)c->ndof * nsig;
+    });
+}
+
+int dmri_tune_db(dmri_ctx *c, const char *path, int32_t nsig, int32_t per_gdir, int32_t *best_lanes,
+                 int32_t *best_mapping, int32_t *hit) {
+    return guarded([&] {
+        require(c != nullptr && best_lanes != nullptr && best_mapping != nullptr && hit != nullptr, DMRI_ERR_ARG,
+                "null argument");
+        set_device(c);
+        int bl = 0, bm = 0, h = 0;
+        tune_db_impl(c, path, nsig, per_gdir != 0, &bl, &bm, &h);
+        *best_lanes = bl;
+        *best_mapping = bm;
+        *hit = h;
     });
 }
 
@@ -2092,7 +2536,8 @@ int dmri_adjoint(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts,
     return guarded([&] {
         require(c != nullptr && grads != nullptr && sig_out != nullptr, DMRI_ERR_ARG, "null argument");
         require(c->assembled, DMRI_ERR_STATE, "dmri_adjoint: call dmri_assemble first");
-        require(!c->periodic, DMRI_ERR_ARG, "dmri_adjoint: the weak pseudo-periodic condition is not supported");
+        require(!c->periodic && !c->strong_active, DMRI_ERR_ARG,
+                "dmri_adjoint: the pseudo-periodic conditions are not supported");
         require(!c->has_Dcell || c->Dcell_symmetric, DMRI_ERR_ARG,
                 "dmri_adjoint: a non-symmetric D_cell tensor makes the operator non-symmetric; the adjoint "
                 "sweep needs a complex-symmetric operator");

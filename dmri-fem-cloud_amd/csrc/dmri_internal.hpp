@@ -301,9 +301,9 @@ struct SolveArgs {
     double gc0, gc1, gc2;
     const double2 *bper;            // weak pseudo-periodic right-hand side of this step, or null
     double2 *ph, *sh;               // preconditioned p and s (polynomial preconditioner), or null
-    double2 *s;                     // s = r - alpha v, written at the own rows by the preconditioner's
-                                    // t-kernel and read by the next t-kernel and the update (null: the
-                                    // consumers form it from r and v)
+    double2 *s;                     // s = r - alpha v, written by the s-pass before the preconditioner's
+                                    // t-kernel, which gathers it; read by the next t-kernel and the
+                                    // update (null: the consumers form it from r and v)
     double pc;                      // polynomial scale c: z -> c (2 z - c Are z)
     const double *areal;            // [nnz] the scaled real part alone (preconditioner SpMVs)
     const double *pdiag;            // [n*vw] diagonal values of the scaled operator per row (a, j or
@@ -362,6 +362,10 @@ void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
                    hipEvent_t e1 = nullptr, int op = kOpT);
 void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);
+// alpha = rho / <r^, v> per signal and s = r - alpha v into a.s (the polynomial preconditioner's t-kernel
+// then gathers one vector instead of forming s from two)
+void launch_spass(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr);
 // areal[e] = vals[vw e]
 void launch_extract_real(int64_t nnz, const double *vals, int vw, double *areal, hipStream_t s);
 // zero st[0..S) and *flag
@@ -373,6 +377,42 @@ void launch_signal(int64_t n, int S, const double *w, const double *sc, const do
                    hipStream_t s);
 void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *x, double2 *u, hipStream_t s);
 void launch_copy(const double2 *src, double2 *dst, int64_t n, hipStream_t s);
+// ---------------------------------------------------------------- restarted GMRES
+constexpr int kGmChunk = 4;        // basis vectors per dot-product launch
+constexpr int kGmMaxRestart = 64;  // largest restart length
+struct GmArgs {
+    int64_t n;
+    int S, m;                // signals, restart length
+    double2 *V;              // m+1 basis vectors of n*S, vector i at V + i*n*S
+    double2 *w;              // the new Arnoldi vector
+    double *part;            // block partials
+    double *h;               // [S][2(m+1)] this pass's Gram-Schmidt coefficients (complex)
+    double *hsum;            // [S][2(m+1)] the sum of both passes
+    double2 *H;              // [S][m][m+1] Hessenberg columns (upper triangular after the rotations)
+    double *cc;              // [S][m] Givens cosines
+    double2 *cs;             // [S][m] Givens sines
+    double2 *g;              // [S][m+1] rotated right-hand side
+    double2 *y;              // [S][m] the cycle's coefficients
+    double *inv;             // [S] 1/|w| (1/|r| at a cycle start)
+    int *nk;                 // [S] Arnoldi steps of the cycle
+    KrylovState *st;         // tol2, rnorm2, done (for this cycle), conv, iters
+    int *all_done;
+    double rtol, atol;
+    int maxit;
+};
+void launch_gm_dots(const SolveGeom &g, const GmArgs &a, int i0, int cnt, hipStream_t s);
+void launch_gm_fin(const double *parts, int nparts, int S, int pstride, int nv, double *out, int ostride, int o,
+                   hipStream_t s);
+void launch_gm_orth(const SolveGeom &g, const GmArgs &a, int j, bool norm, hipStream_t s);
+void launch_gm_hess(const GmArgs &a, int j, const double *wn2, hipStream_t s);
+void launch_gm_scale(const SolveGeom &g, const GmArgs &a, int j, hipStream_t s);
+void launch_gm_start(const GmArgs &a, const double *tot, bool first, hipStream_t s);
+void launch_gm_v0(const SolveGeom &g, const GmArgs &a, const double2 *r, hipStream_t s);
+void launch_gm_combine(const SolveGeom &g, const GmArgs &a, double2 *out, hipStream_t s);
+void launch_gm_axpy(int64_t total, const double2 *z, double2 *x, hipStream_t s);
+// st[s].done = 0 for every signal and *flag = 0, the rest of the states kept
+void launch_reset_state_flags(KrylovState *st, int S, int *flag, hipStream_t s);
+
 // adjoint contraction of one step; returns the number of block partials (14 doubles each)
 int launch_adj_contract(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
                         const int *dof_comp, const double *mm, const double *kk, const double *gi, const double *vals,

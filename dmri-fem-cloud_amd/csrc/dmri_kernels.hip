@@ -1964,10 +1964,11 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
     const double2 *r = a.r, *v = a.v, *sh = a.sh;
     constexpr bool cd = OP == kOpPreCD;  // complex diagonal, as in spmv_v_kernel
     constexpr bool pre = OP == kOpPre || OP == kOpPreCD;
+    const double2 *sm = a.s;  // s materialised by the s-pass (null: formed from r and v here)
     auto gather = [&](int c) {
         const int64_t j = (int64_t)c * S_ + s;
         if (OP == kOpTP) return sh[j];
-        const double2 z = c_sub(r[j], c_mul(alpha, v[j]));
+        const double2 z = sm ? sm[j] : c_sub(r[j], c_mul(alpha, v[j]));
         if constexpr (cd) return c_mul(z, cdiag_inv<PG>(a.pdiag, a.coefA, c, s));
         return z;
     };
@@ -1975,14 +1976,13 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
         if (L.g != 0) return;
         const int64_t idx = row * S_ + s;
         const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
-        const double2 sv = (OP == kOpTP && a.s) ? a.s[idx] : c_sub(r[idx], c_mul(alpha, v[idx]));
+        const double2 sv = sm ? sm[idx] : c_sub(r[idx], c_mul(alpha, v[idx]));
         if (pre) {
             const double2 w = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
             if constexpr (cd)
                 a.sh[idx] = c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s));
             else
                 a.sh[idx] = w;
-            if (a.s) a.s[idx] = sv;
             return;
         }
         a.t[idx] = y;
@@ -2190,22 +2190,21 @@ __global__ void __launch_bounds__(kThreads) spmv_t_wave_kernel(SolveArgs a, cons
     const bool act = lane < S && sact[lane];
     const double2 alpha = act ? salpha[lane] : make_double2(0.0, 0.0);
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
-    const double2 *r = a.r, *v = a.v, *sh = a.sh;
+    const double2 *r = a.r, *v = a.v, *sh = a.sh, *sm = a.s;
     wave_rows<PG>(
         a.n, rowptr, col, vals, act,
         [&](int c) {
             const int64_t j = (int64_t)c * S + lane;
             if (OP == kOpTP) return sh[j];
-            return c_sub(r[j], c_mul(alpha, v[j]));
+            return sm ? sm[j] : c_sub(r[j], c_mul(alpha, v[j]));
         },
         [&](int64_t row, const Acc<PG> &acc) {
             if (!act) return;
             const int64_t idx = row * S + lane;
             const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
-            const double2 sv = (OP == kOpTP && a.s) ? a.s[idx] : c_sub(r[idx], c_mul(alpha, v[idx]));
+            const double2 sv = sm ? sm[idx] : c_sub(r[idx], c_mul(alpha, v[idx]));
             if (OP == kOpPre) {
                 a.sh[idx] = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
-                if (a.s) a.s[idx] = sv;
                 return;
             }
             a.t[idx] = y;
@@ -2219,6 +2218,288 @@ __global__ void __launch_bounds__(kThreads) spmv_t_wave_kernel(SolveArgs a, cons
                               a.partT);
 }
 
+// --------------------------------------------------------------------------------------
+// Restarted GMRES for S signals in lockstep (right-preconditioned like BiCGStab): Arnoldi with
+// classical Gram-Schmidt applied twice, complex Givens rotations per signal on the device, and the
+// update x += M (V y) at the end of a cycle.  The per-signal KrylovState marks a signal done for the
+// rest of the cycle (converged, the cycle full, or a lucky breakdown), which the SpMV kernels honour.
+// --------------------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_signal_partials_n(const double (&v)[NV], bool writer, int slot, int nslots,
+                                                        int S, double *parts) {
+    __shared__ double buf[NV][kThreads];
+    if (writer) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) buf[k][slot] = v[k];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < S) {
+        const int s = threadIdx.x;
+        double acc[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+        for (int j = s; j < nslots; j += S) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) acc[k] += buf[k][j];
+        }
+#pragma unroll
+        for (int k = 0; k < NV; ++k) parts[((size_t)blockIdx.x * S + s) * NV + k] = acc[k];
+    }
+}
+
+// conj(v_i) . w for basis vectors i0 .. i0+cnt-1 (cnt <= kGmChunk): block partials [grid][S][2 kGmChunk]
+__global__ void __launch_bounds__(kThreads) gm_dots_kernel(GmArgs a, int i0, int cnt, int vrpb) {
+    if (*a.all_done) return;
+    constexpr int NV = 2 * kGmChunk;
+    const int S = a.S;
+    const int64_t total = a.n * S, nS = a.n * S;
+    const int s = threadIdx.x % S;
+    const bool valid = (int)threadIdx.x < vrpb * S;
+    const bool act = valid && !a.st[s].done;
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    if (act)
+        for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total; i += (int64_t)gridDim.x * vrpb * S) {
+            const double2 wv = a.w[i];
+#pragma unroll
+            for (int k = 0; k < kGmChunk; ++k)
+                if (k < cnt) {
+                    const double2 v = a.V[(int64_t)(i0 + k) * nS + i];
+                    acc[2 * k] += v.x * wv.x + v.y * wv.y;
+                    acc[2 * k + 1] += v.x * wv.y - v.y * wv.x;
+                }
+        }
+    block_signal_partials_n<NV>(acc, valid, threadIdx.x, vrpb * S, S, a.part);
+}
+
+// One block per signal: out[s][o + k] = sum over blocks of parts[b][s][k] (pstride doubles per block and
+// signal), k < nv, in a fixed order
+__global__ void __launch_bounds__(kThreads) gm_fin_kernel(const double *parts, int nparts, int S, int pstride, int nv,
+                                                          double *out, int ostride, int o) {
+    const int s = blockIdx.x;
+    for (int k = 0; k < nv; ++k) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nparts; b += blockDim.x) v += parts[((size_t)b * S + s) * pstride + k];
+        double r[1] = {v};
+        block_reduce<1>(r);
+        if (threadIdx.x == 0) out[(size_t)s * ostride + o + k] = r[0];
+    }
+}
+
+// w -= sum_{i <= j} h_i v_i (h from a.h); norm: partials [|w|^2, 0, 0, 0] (kNV layout); the pass's
+// coefficients are added to a.hsum
+__global__ void __launch_bounds__(kThreads) gm_orth_kernel(GmArgs a, int j, int norm, int vrpb) {
+    if (*a.all_done) return;
+    const int S = a.S;
+    const int64_t total = a.n * S, nS = a.n * S;
+    const int s = threadIdx.x % S;
+    const bool valid = (int)threadIdx.x < vrpb * S;
+    const bool act = valid && !a.st[s].done;
+    const int hst = 2 * (a.m + 1);
+    if (blockIdx.x == 0 && (int)threadIdx.x < S && !a.st[threadIdx.x].done)
+        for (int i = 0; i <= j; ++i) {
+            a.hsum[(size_t)threadIdx.x * hst + 2 * i] += a.h[(size_t)threadIdx.x * hst + 2 * i];
+            a.hsum[(size_t)threadIdx.x * hst + 2 * i + 1] += a.h[(size_t)threadIdx.x * hst + 2 * i + 1];
+        }
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    if (act)
+        for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total; i += (int64_t)gridDim.x * vrpb * S) {
+            double2 wv = a.w[i];
+            for (int q = 0; q <= j; ++q) {
+                const double hr = a.h[(size_t)s * hst + 2 * q], hi = a.h[(size_t)s * hst + 2 * q + 1];
+                const double2 v = a.V[(int64_t)q * nS + i];
+                wv.x -= hr * v.x - hi * v.y;
+                wv.y -= hr * v.y + hi * v.x;
+            }
+            a.w[i] = wv;
+            red[0] += wv.x * wv.x + wv.y * wv.y;
+        }
+    if (norm) block_signal_partials(red, valid, threadIdx.x, vrpb * S, S, a.part);
+}
+
+// One block, thread = signal: column j of the Hessenberg matrix from hsum and |w|, the earlier Givens
+// rotations applied to it, a new rotation zeroing its subdiagonal, the residual estimate |g_{j+1}|;
+// a signal ends its cycle when that estimate meets its tolerance, at a lucky breakdown (|w| = 0) or
+// when the cycle is full.  wn2 = [S][kNV] totals of the norm pass.
+__global__ void gm_hess_kernel(GmArgs a, int j, const double *wn2) {
+    if (*a.all_done) return;
+    const int S = a.S, m = a.m, hst = 2 * (m + 1);
+    const int s = threadIdx.x;
+    __shared__ int sact[kMaxSignals];
+    if (s < S) {
+        KrylovState &st = a.st[s];
+        int act = !st.done;
+        if (act) {
+            double2 *H = a.H + (size_t)s * m * (m + 1) + (size_t)j * (m + 1);
+            for (int i = 0; i <= j; ++i)
+                H[i] = make_double2(a.hsum[(size_t)s * hst + 2 * i], a.hsum[(size_t)s * hst + 2 * i + 1]);
+            const double hn = sqrt(wn2[(size_t)s * kNV]);
+            double2 *cs = a.cs + (size_t)s * m;
+            double *cc = a.cc + (size_t)s * m;
+            for (int i = 0; i < j; ++i) {
+                const double2 x0 = H[i], x1 = H[i + 1];
+                H[i] = c_add(make_double2(cc[i] * x0.x, cc[i] * x0.y), c_mul(cs[i], x1));
+                H[i + 1] = c_sub(make_double2(cc[i] * x1.x, cc[i] * x1.y), c_mul(make_double2(cs[i].x, -cs[i].y), x0));
+            }
+            const double2 hj = H[j];
+            const double ah = sqrt(hj.x * hj.x + hj.y * hj.y);
+            const double t = sqrt(ah * ah + hn * hn);
+            double c;
+            double2 sn, r;
+            if (t == 0.0) {
+                c = 1.0;
+                sn = make_double2(0.0, 0.0);
+                r = make_double2(0.0, 0.0);
+            } else if (ah == 0.0) {
+                c = 0.0;
+                sn = make_double2(1.0, 0.0);
+                r = make_double2(hn, 0.0);
+            } else {
+                const double2 ph = make_double2(hj.x / ah, hj.y / ah);
+                c = ah / t;
+                sn = make_double2(ph.x * hn / t, ph.y * hn / t);
+                r = make_double2(ph.x * t, ph.y * t);
+            }
+            H[j] = r;
+            cc[j] = c;
+            cs[j] = sn;
+            double2 *g = a.g + (size_t)s * (m + 1);
+            const double2 gj = g[j];
+            g[j] = make_double2(c * gj.x, c * gj.y);
+            g[j + 1] = c_mul(make_double2(-sn.x, sn.y), gj);
+            const double res2 = g[j + 1].x * g[j + 1].x + g[j + 1].y * g[j + 1].y;
+            st.rnorm2 = res2;
+            st.iters += 1;
+            a.nk[s] = j + 1;
+            a.inv[s] = hn > 0.0 ? 1.0 / hn : 0.0;
+            if (res2 <= st.tol2 || hn == 0.0 || j + 1 >= m || st.iters >= a.maxit) {
+                st.done = 1;
+                act = 0;
+            }
+        }
+        sact[s] = act;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int q = 0; q < S; ++q)
+            if (sact[q]) all = 0;
+        if (all) *a.all_done = 1;
+    }
+}
+
+// v_{j+1} = w / |w| for the signals still in the cycle
+__global__ void __launch_bounds__(kThreads) gm_scale_kernel(GmArgs a, int j, int vrpb) {
+    if (*a.all_done) return;
+    const int S = a.S;
+    const int64_t total = a.n * S, nS = a.n * S;
+    const int s = threadIdx.x % S;
+    if ((int)threadIdx.x >= vrpb * S || a.st[s].done) return;
+    const double iv = a.inv[s];
+    for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total; i += (int64_t)gridDim.x * vrpb * S) {
+        const double2 wv = a.w[i];
+        a.V[(int64_t)(j + 1) * nS + i] = make_double2(wv.x * iv, wv.y * iv);
+    }
+}
+
+// Start of a cycle from the residual totals tot[s] = [|r|^2, 0, |r|^2, |b|^2] (first: also the
+// tolerance): g = (|r|, 0, ...), hsum = 0, signals within tolerance converge; the others restart
+// their Arnoldi process (st.done = 0).  One block, thread = signal.
+__global__ void gm_start_kernel(GmArgs a, const double *tot, int first) {
+    const int S = a.S, m = a.m;
+    const int s = threadIdx.x;
+    __shared__ int sact[kMaxSignals];
+    if (s < S) {
+        KrylovState &st = a.st[s];
+        const double rn2 = tot[(size_t)s * kNV], bn2 = tot[(size_t)s * kNV + 3];
+        if (first) {
+            const double t1 = a.rtol * a.rtol * bn2, t2 = a.atol * a.atol;
+            st.tol2 = t1 > t2 ? t1 : t2;
+            st.bnorm2 = bn2;
+            st.iters = 0;
+            st.conv = 0;
+            st.breakdown = 0;
+        }
+        st.rnorm2 = rn2;
+        const bool conv = rn2 <= st.tol2;
+        const bool lim = !conv && st.iters >= a.maxit;
+        st.conv = conv ? 1 : 0;
+        st.done = (conv || lim) ? 1 : 0;
+        a.nk[s] = 0;
+        const double beta = sqrt(rn2);
+        a.inv[s] = beta > 0.0 ? 1.0 / beta : 0.0;
+        double2 *g = a.g + (size_t)s * (m + 1);
+        g[0] = make_double2(beta, 0.0);
+        for (int i = 1; i <= m; ++i) g[i] = make_double2(0.0, 0.0);
+        for (int i = 0; i < 2 * (m + 1); ++i) a.hsum[(size_t)s * 2 * (m + 1) + i] = 0.0;
+        sact[s] = !st.done;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int q = 0; q < S; ++q) any |= sact[q];
+        *a.all_done = any ? 0 : 1;
+    }
+}
+
+// v_0 = r / |r| for the signals starting a cycle
+__global__ void __launch_bounds__(kThreads) gm_v0_kernel(GmArgs a, const double2 *r, int vrpb) {
+    const int S = a.S;
+    const int64_t total = a.n * S;
+    const int s = threadIdx.x % S;
+    if ((int)threadIdx.x >= vrpb * S || a.st[s].done) return;
+    const double iv = a.inv[s];
+    for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total; i += (int64_t)gridDim.x * vrpb * S) {
+        const double2 rv = r[i];
+        a.V[i] = make_double2(rv.x * iv, rv.y * iv);
+    }
+}
+
+// End of a cycle: y from R y = g by back substitution over the nk steps of each signal (one block,
+// thread = signal) ...
+__global__ void gm_solve_kernel(GmArgs a) {
+    const int S = a.S, m = a.m;
+    const int s = threadIdx.x;
+    if (s >= S) return;
+    const int k = a.nk[s];
+    const double2 *H = a.H + (size_t)s * m * (m + 1);
+    const double2 *g = a.g + (size_t)s * (m + 1);
+    double2 *y = a.y + (size_t)s * m;
+    for (int i = k - 1; i >= 0; --i) {
+        double2 acc = g[i];
+        for (int q = i + 1; q < k; ++q) acc = c_sub(acc, c_mul(H[(size_t)q * (m + 1) + i], y[q]));
+        y[i] = c_div(acc, H[(size_t)i * (m + 1) + i]);
+    }
+}
+
+// ... then out = sum_i y_i v_i (0 for a signal that took no step)
+__global__ void __launch_bounds__(kThreads) gm_combine_kernel(GmArgs a, double2 *out, int vrpb) {
+    const int S = a.S, m = a.m;
+    const int64_t total = a.n * S, nS = a.n * S;
+    const int s = threadIdx.x % S;
+    if ((int)threadIdx.x >= vrpb * S) return;
+    const int k = a.nk[s];
+    const double2 *y = a.y + (size_t)s * m;
+    for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total; i += (int64_t)gridDim.x * vrpb * S) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int q = 0; q < k; ++q) acc = c_add(acc, c_mul(y[q], a.V[(int64_t)q * nS + i]));
+        out[i] = acc;
+    }
+}
+
+// x += z
+__global__ void __launch_bounds__(kThreads) gm_axpy_kernel(int64_t total, const double2 *z, double2 *x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = c_add(x[i], z[i]);
+}
+
+// every signal active again and the all-done flag cleared (the other state is kept)
+__global__ void reopen_kernel(KrylovState *st, int S, int *flag) {
+    const int s = threadIdx.x;
+    if (s < S) st[s].done = 0;
+    if (s == 0) *flag = 0;
+}
 // ---- signals in registers: each lane takes entries g, g+G, ... of a row for all S <= 8
 // signals, so column indices and values are loaded once per entry instead of once per signal.
 constexpr int kMaxSR = 8;
@@ -2364,6 +2645,21 @@ __global__ void __launch_bounds__(kThreads) spmv_t_sr_kernel(SolveArgs a, int S,
 
 // omega = <t, s> / <t, t> (0 when |s| already meets the tolerance);
 // x += alpha p + omega s ; r = s - omega t ; partials U: [<r^, r>.re, .im, |r|^2, 0]
+// s-pass: alpha = rho / <r^, v> (the head every t-kernel runs too) and s = r - alpha v for the active
+// signals, so the preconditioner's t-kernel gathers one vector instead of two
+__global__ void __launch_bounds__(kThreads) spass_kernel(SolveArgs a, int S, int vrpb, int nparts, int it) {
+    if (*a.all_done) return;
+    __shared__ double2 salpha[kMaxSignals];
+    __shared__ int sact[kMaxSignals];
+    spmv_t_head(a, S, nparts, it, salpha, sact);
+    const int64_t total = a.n * S;
+    const int s = threadIdx.x % S;
+    if ((int)threadIdx.x >= vrpb * S || !sact[s]) return;
+    const double2 alpha = salpha[s];
+    for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total; i += (int64_t)gridDim.x * vrpb * S)
+        a.s[i] = c_sub(a.r[i], c_mul(alpha, a.v[i]));
+}
+
 __global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int S, int vrpb, int nparts, int it) {
     if (*a.all_done) return;
     __shared__ double tot[kMaxSignals][kNV];
@@ -3613,6 +3909,12 @@ void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
         check_launch();
     }
 }
+void launch_spass(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(spass_kernel, dim3(g.vgrid), dim3(g.vec_threads), 0, s, e0, e1, 0, a, g.S, g.vec_rpb,
+                          g.grid, it);
+    check_launch();
+}
+
 void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                         hipEvent_t e1) {
     hipExtLaunchKernelGGL(bicg_update_kernel, dim3(g.vgrid), dim3(g.vec_threads), 0, s, e0, e1, 0, a, g.S,
@@ -3726,5 +4028,54 @@ void launch_signal(int64_t n, int S, const double *w, const double *sc, const do
     finalize_kernel<<<S, kThreads, 0, s>>>(part, grid, S, out);
     check_launch();
 }
+
+void launch_gm_dots(const SolveGeom &g, const GmArgs &a, int i0, int cnt, hipStream_t s) {
+    gm_dots_kernel<<<g.vgrid, g.vec_threads, 0, s>>>(a, i0, cnt, g.vec_rpb);
+    check_launch();
+}
+void launch_gm_fin(const double *parts, int nparts, int S, int pstride, int nv, double *out, int ostride, int o,
+                   hipStream_t s) {
+    gm_fin_kernel<<<S, kThreads, 0, s>>>(parts, nparts, S, pstride, nv, out, ostride, o);
+    check_launch();
+}
+void launch_gm_orth(const SolveGeom &g, const GmArgs &a, int j, bool norm, hipStream_t s) {
+    gm_orth_kernel<<<g.vgrid, g.vec_threads, 0, s>>>(a, j, norm ? 1 : 0, g.vec_rpb);
+    check_launch();
+}
+void launch_gm_hess(const GmArgs &a, int j, const double *wn2, hipStream_t s) {
+    gm_hess_kernel<<<1, 64, 0, s>>>(a, j, wn2);
+    check_launch();
+}
+void launch_gm_scale(const SolveGeom &g, const GmArgs &a, int j, hipStream_t s) {
+    gm_scale_kernel<<<g.vgrid, g.vec_threads, 0, s>>>(a, j, g.vec_rpb);
+    check_launch();
+}
+void launch_gm_start(const GmArgs &a, const double *tot, bool first, hipStream_t s) {
+    gm_start_kernel<<<1, 64, 0, s>>>(a, tot, first ? 1 : 0);
+    check_launch();
+}
+void launch_gm_v0(const SolveGeom &g, const GmArgs &a, const double2 *r, hipStream_t s) {
+    gm_v0_kernel<<<g.vgrid, g.vec_threads, 0, s>>>(a, r, g.vec_rpb);
+    check_launch();
+}
+void launch_gm_combine(const SolveGeom &g, const GmArgs &a, double2 *out, hipStream_t s) {
+    gm_solve_kernel<<<1, 64, 0, s>>>(a);
+    check_launch();
+    gm_combine_kernel<<<g.vgrid, g.vec_threads, 0, s>>>(a, out, g.vec_rpb);
+    check_launch();
+}
+void launch_reset_state_flags(KrylovState *st, int S, int *flag, hipStream_t s) {
+    reopen_kernel<<<1, 64, 0, s>>>(st, S, flag);
+    check_launch();
+}
+
+void launch_gm_axpy(int64_t total, const double2 *z, double2 *x, hipStream_t s) {
+    unsigned gr = grid_for(total, kThreads);
+    if (gr > (unsigned)kMaxGrid) gr = kMaxGrid;
+    if (gr == 0) gr = 1;
+    gm_axpy_kernel<<<gr, kThreads, 0, s>>>(total, z, x);
+    check_launch();
+}
+
 
 }  // namespace dmri

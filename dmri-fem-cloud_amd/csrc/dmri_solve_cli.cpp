@@ -4,9 +4,9 @@
 // Meshes: DOLFIN XML, or ASCII gmsh 2.x (.msh/.gmsh) read as dolfin-convert would convert it.
 //   GCloud style (GCloudDmriSolver.py:61-133):  -f mesh.xml -M 0|1 -b B -q Q -p kappa -d delta -D Delta
 //                                               -k dt -K D0 -T2 T2 -gdir gx gy gz -pdir px py pz
-//                                               (-pdir: weak pseudo-periodic faces normal to the axes
-//                                               with p != 0; -IsPeriodic 1, the strong form, exits
-//                                               with an error)
+//                                               -IsPeriodic 0|1 (-pdir: pseudo-periodic faces normal to
+//                                               the axes with p != 0, weakly imposed, or strongly with
+//                                               -IsPeriodic 1 on a periodic mesh)
 //   comri style (comri/two-comp/hpc-fenics-cpp/main.cpp:581-648): -m mesh.xml -N steps -b B -q G -d delta
 //                                               -D Delta -k dt -K D0 -p kappa -v gx gy gz -j nskip
 // Extra: -phase file (per-cell phase, one integer per line, or a DOLFIN mesh_function /
@@ -270,7 +270,7 @@ void usage() {
     std::fprintf(stderr,
                  "usage: dmri-solve (-f|-m) mesh.xml|mesh.msh [-M 0|1] [-phase file] [-b B | -q Q] [-d delta] [-D Delta]\n"
                  "                  [-k dt | -N steps] [-K D0] [-p kappa] [-T2 T2] [-gdir|-v gx gy gz]\n"
-                 "                  [-pdir px py pz] [-IsPeriodic 0] [-r 0]\n"
+                 "                  [-pdir px py pz] [-IsPeriodic 0|1] [-r 0]\n"
                  "                  [-variant gcloud|comri|comri-until-T] [-theta t] [-rtol r] [-atol a]\n"
                  "                  [-maxit n] [-device d] [-convert out.xml] [-s 0|1 [-j nskip] [-o dir]]\n");
 }
@@ -349,13 +349,8 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "-r %d: mesh refinement is not supported (refine the mesh beforehand)\n", nrefine);
         return 2;
     }
+    // -IsPeriodic 1 with -pdir: the strong pseudo-periodic form (PeriodicBD, DmriFemLib.py:327-375)
     const bool periodic = pdir[0] || pdir[1] || pdir[2];
-    if (is_periodic && periodic) {
-        // IsDomainPeriodic = 1 selects the strong pseudo-periodic form (PeriodicBD, DmriFemLib.py:327-375)
-        std::fprintf(stderr, "-IsPeriodic 1: the strong pseudo-periodic formulation is not available; "
-                             "-IsPeriodic 0 with -pdir imposes the weak one\n");
-        return 2;
-    }
     const bool comri = variant != "gcloud";
     if (variant != "gcloud" && variant != "comri" && variant != "comri-until-T") {
         std::fprintf(stderr, "unknown variant %s\n", variant.c_str());
@@ -385,7 +380,7 @@ int main(int argc, char **argv) {
 
     // time profile per step, and F = int_0^t f at the explicit side's time (the weak periodic terms'
     // phase: itime_profile(tp), DmriFemLib.py:901-902)
-    std::vector<double> fA, fB, FB;
+    std::vector<double> fA, fB, FA, FB;
     auto f_py = [&](double s) { return s < delta ? 1.0 : (s < Delta ? 0.0 : (s < T ? -1.0 : 0.0)); };
     auto F_py = [&](double s) { return s < delta ? s : (s < Delta ? delta : (s < T ? delta - (s - Delta) : 0.0)); };
     auto f_c = [&](double t) { return 1.0 * (t >= 0 && t < delta) - 1.0 * (t >= Delta && t <= Delta + delta); };
@@ -397,6 +392,7 @@ int main(int argc, char **argv) {
         while (t < T + dt) {
             fA.push_back(f_py(t));
             fB.push_back(f_py(tp));
+            FA.push_back(F_py(t));
             FB.push_back(F_py(tp));
             tp = t;
             t += dt;
@@ -406,6 +402,7 @@ int main(int argc, char **argv) {
         const double end = variant == "comri-until-T" ? T : T + dt;
         while (t < end) {
             fA.push_back(f_c(t));
+            FA.push_back(F_c(t));
             FB.push_back(F_c(t));
             t += dt;
         }
@@ -453,7 +450,8 @@ int main(int argc, char **argv) {
     co.invT2[0] = co.invT2[1] = 1.0 / T2;
     co.kappa = kappa;
     check(dmri_coeffs_set(ctx, &co), "dmri_coeffs_set");
-    if (periodic) check(dmri_periodic_set(ctx, pdir), "dmri_periodic_set");
+    if (periodic && is_periodic) check(dmri_periodic_strong_set(ctx, pdir), "dmri_periodic_strong_set");
+    else if (periodic) check(dmri_periodic_set(ctx, pdir), "dmri_periodic_set");
     check(dmri_assemble(ctx), "dmri_assemble");
     dmri_seq seq{};
     seq.nsteps = (int32_t)fA.size();
@@ -465,7 +463,10 @@ int main(int argc, char **argv) {
     seq.gdir[0] = g[0];
     seq.gdir[1] = g[1];
     seq.gdir[2] = g[2];
-    if (periodic) seq.F_explicit = FB.data();
+    if (periodic) {
+        seq.F_explicit = FB.data();
+        seq.F_implicit = FA.data();
+    }
     dmri_info info0{};
     check(dmri_get_info(ctx, &info0), "dmri_get_info");
     std::vector<double> snaps;

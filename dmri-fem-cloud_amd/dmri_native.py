@@ -13,7 +13,7 @@ import numpy as np
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DMRI_LIB", os.path.join(_PKG, "lib", "libdmri.so"))
 
-DMRI_MAT = {"m": 0, "k": 1, "r": 2, "gi": 3, "jx": 4, "jy": 5, "jz": 6, "p": 7}
+DMRI_MAT = {"m": 0, "k": 1, "r": 2, "gi": 3, "jx": 4, "jy": 5, "jz": 6, "p": 7, "gx": 8, "gy": 9, "gz": 10}
 ERRORS = {-1: "ARG", -2: "HIP", -3: "STATE", -4: "NOCONV", -5: "MESH", -6: "ALLOC", -7: "LIMIT"}
 
 
@@ -42,13 +42,15 @@ class _Seq(ctypes.Structure):
                 ("f_explicit", ctypes.POINTER(ctypes.c_double)), ("dt", ctypes.c_double),
                 ("theta", ctypes.c_double), ("q", ctypes.c_double), ("gdir", ctypes.c_double * 3),
                 ("F_explicit", ctypes.POINTER(ctypes.c_double)), ("u0_vertex", ctypes.POINTER(ctypes.c_double)),
-                ("snap_every", ctypes.c_int32), ("snap_out", ctypes.POINTER(ctypes.c_double))]
+                ("snap_every", ctypes.c_int32), ("snap_out", ctypes.POINTER(ctypes.c_double)),
+                ("F_implicit", ctypes.POINTER(ctypes.c_double))]
 
 
 class _Opts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxit", ctypes.c_int32),
                 ("profile", ctypes.c_int32), ("lanes_per_row", ctypes.c_int32), ("initial_guess", ctypes.c_int32),
-                ("mapping", ctypes.c_int32), ("preconditioner", ctypes.c_int32)]
+                ("mapping", ctypes.c_int32), ("preconditioner", ctypes.c_int32), ("krylov", ctypes.c_int32),
+                ("restart", ctypes.c_int32)]
 
 
 class _Stats(ctypes.Structure):
@@ -56,10 +58,10 @@ class _Stats(ctypes.Structure):
                 ("relres_max", ctypes.c_double), ("solve_ms", ctypes.c_double), ("spmv_ms", ctypes.c_double),
                 ("spmv_launches", ctypes.c_int64), ("spmv_bytes", ctypes.c_double),
                 ("precond_used", ctypes.c_int32), ("reserved", ctypes.c_int32),
-                ("kern_ms", ctypes.c_double * 6), ("kern_launches", ctypes.c_int64 * 6),
-                ("kern_bytes", ctypes.c_double * 6)]
+                ("kern_ms", ctypes.c_double * 7), ("kern_launches", ctypes.c_int64 * 7),
+                ("kern_bytes", ctypes.c_double * 7)]
 
-KERNELS = ("bicg_p", "spmv_v_pre", "spmv_v", "spmv_t_pre", "spmv_t", "bicg_update")
+KERNELS = ("bicg_p", "spmv_v_pre", "spmv_v", "spass", "spmv_t_pre", "spmv_t", "bicg_update")
 
 
 def _stats_dict(st) -> dict:
@@ -84,9 +86,9 @@ class _Grads(ctypes.Structure):
 
 EXPORTED = ["dmri_create", "dmri_destroy", "dmri_last_error", "dmri_version", "dmri_mesh_set", "dmri_coeffs_set",
             "dmri_assemble", "dmri_get_info", "dmri_export_pattern", "dmri_export_values", "dmri_export_facets",
-            "dmri_periodic_set", "dmri_export_periodic",
+            "dmri_periodic_set", "dmri_periodic_strong_set", "dmri_export_periodic",
             "dmri_solve", "dmri_solve_batched", "dmri_tune", "dmri_set_tuning", "dmri_time_spmv", "dmri_copy_bandwidth",
-            "dmri_adjoint", "dmri_get_solution", "dmri_batch_begin", "dmri_batch_advance", "dmri_batch_signals",
+            "dmri_tune_db", "dmri_adjoint", "dmri_get_solution", "dmri_batch_begin", "dmri_batch_advance", "dmri_batch_signals",
             "dmri_batch_end"]
 
 
@@ -109,12 +111,14 @@ def _load():
     L.dmri_export_values.argtypes = [vp, ctypes.c_int, dp]
     L.dmri_export_facets.argtypes = [vp, i32p]
     L.dmri_periodic_set.argtypes = [vp, i32p]
+    L.dmri_periodic_strong_set.argtypes = [vp, i32p]
     L.dmri_export_periodic.argtypes = [vp, i32p, i32p, dp, dp]
     L.dmri_solve.argtypes = [vp, P(_Seq), P(_Opts), dp, P(_Stats)]
     L.dmri_solve_batched.argtypes = [vp, ctypes.c_int32, dp, dp, P(_Seq), P(_Opts), dp, P(_Stats)]
     L.dmri_get_solution.argtypes = [vp, ctypes.c_int32, dp]
     L.dmri_tune.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, i32p, i32p, dp]
     L.dmri_set_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.dmri_tune_db.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, i32p, i32p, i32p]
     L.dmri_time_spmv.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, dp, dp, dp]
     L.dmri_copy_bandwidth.argtypes = [vp, i64, ctypes.c_int32, dp]
     L.dmri_adjoint.argtypes = [vp, P(_Seq), P(_Opts), P(_AdjOpts), dp, P(_Grads), P(_Stats)]
@@ -146,6 +150,8 @@ def version() -> str:
 
 class Context:
     """One solver context on one GPU (see include/dmri.h)."""
+
+    KRYLOV = {"bicgstab": 0, "gmres": 1}
 
     def __init__(self, device: int = 0):
         self._h = ctypes.c_void_p()
@@ -227,6 +233,12 @@ class Context:
         d = np.ascontiguousarray([1 if x else 0 for x in dirs], dtype=np.int32)
         _check(LIB.dmri_periodic_set(self._h, _ptr(d, ctypes.c_int32)))
 
+    def set_periodic_strong(self, dirs):
+        """Strong pseudo-periodic condition on the faces normal to the axes with dirs[d] != 0 (next
+        assemble): max-face vertices share their min-face image's DOF (dmri_periodic_strong_set)."""
+        d = np.ascontiguousarray([1 if x else 0 for x in dirs], dtype=np.int32)
+        _check(LIB.dmri_periodic_strong_set(self._h, _ptr(d, ctypes.c_int32)))
+
     def export_periodic(self):
         """Mirror entries of the last assemble: dict(dof, src (ne,3), w (ne,3), ell (ne,3))."""
         ne = self.info()["n_periodic"]
@@ -241,7 +253,7 @@ class Context:
     # -- solve --------------------------------------------------------------------------
     @staticmethod
     def _seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes, guess=0,
-                  mapping=0, F_explicit=None, precond=0, u0=None, nv=None):
+                  mapping=0, F_explicit=None, precond=0, u0=None, nv=None, F_implicit=None, krylov=0, restart=0):
         fA = np.ascontiguousarray(f_implicit, dtype=np.float64)
         fB = np.ascontiguousarray(f_explicit, dtype=np.float64)
         if fA.shape != fB.shape:
@@ -254,6 +266,10 @@ class Context:
         s.f_implicit = _ptr(fA, ctypes.c_double)
         s.f_explicit = _ptr(fB, ctypes.c_double)
         s.F_explicit = _ptr(FB, ctypes.c_double) if FB is not None else None
+        FA = None if F_implicit is None else np.ascontiguousarray(F_implicit, dtype=np.float64)
+        if FA is not None and FA.shape != fA.shape:
+            raise ValueError("F_implicit and the time profile differ in length")
+        s.F_implicit = _ptr(FA, ctypes.c_double) if FA is not None else None
         U0 = None if u0 is None else np.ascontiguousarray(u0, dtype=np.float64).ravel()
         if U0 is not None and nv is not None and U0.shape[0] != nv:
             raise ValueError(f"u0 needs one value per mesh vertex ({nv}), got {U0.shape[0]}")
@@ -267,7 +283,9 @@ class Context:
         o.initial_guess = int(guess)
         o.mapping = int(mapping)
         o.preconditioner = int(precond)
-        return s, o, (fA, fB, FB, U0)
+        o.krylov = Context.KRYLOV[krylov] if isinstance(krylov, str) else int(krylov)
+        o.restart = int(restart)
+        return s, o, (fA, fB, FB, U0, FA)
 
     def _nv(self, u0):
         return None if u0 is None else int(self.info()["nv"])
@@ -287,7 +305,8 @@ class Context:
         return buf
 
     def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None, snap_every=0):
+              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None, snap_every=0,
+              F_implicit=None, krylov=0, restart=0):
         """precond: 0 = Jacobi + degree-1 polynomial (default), 1 = Jacobi only.  guess: Krylov start
         of each step, 0 = residual-minimising projection on the last states (default), 1 = previous
         state, 2 = linear, 3 = quadratic extrapolation.  u0: initial state per mesh vertex (real part
@@ -295,7 +314,7 @@ class Context:
         (complex, DOF order) after every step whose 0-based index is a multiple of it, shape
         (nsnap, ndof)."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes,
-                                    guess, mapping, F_explicit, precond, u0, self._nv(u0))
+                                    guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov, restart)
         snaps = self._snapshots(s, snap_every, 1)
         sig = np.zeros(2)
         st = _Stats()
@@ -308,14 +327,14 @@ class Context:
 
     def solve_batched(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
                       profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None,
-                      snap_every=0):
+                      snap_every=0, F_implicit=None, krylov=0, restart=0):
         """Signals for every (q[s], gdir[s]) on the shared pattern; returns complex array (nsig,).
         snap_every > 0: stats["snapshots"] has shape (nsnap, nsig, ndof), as in solve."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(gdirs, dtype=np.float64).reshape(-1, 3),
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
-                                    lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0))
+                                    lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov, restart)
         snaps = self._snapshots(s, snap_every, qs.shape[0])
         sig = np.zeros(2 * qs.shape[0])
         st = _Stats()
@@ -328,14 +347,15 @@ class Context:
         return sig.view(np.complex128), stats
 
     def batch(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None, group=0):
+              profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None, group=0,
+              F_implicit=None, krylov=0, restart=0):
         """Resumable batched solve (dmri_batch_begin): returns a Batch whose advance(k) runs k more
         theta steps of every signal and whose signals() gives the signals of the current states."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(gdirs, dtype=np.float64).reshape(-1, 3),
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
-                                    lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0))
+                                    lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov, restart)
         h = ctypes.c_void_p()
         _check(LIB.dmri_batch_begin(self._h, qs.shape[0], _ptr(qs, ctypes.c_double), _ptr(g, ctypes.c_double),
                                     ctypes.byref(s), ctypes.byref(o), int(group), ctypes.byref(h)))
@@ -353,14 +373,21 @@ class Context:
                  if us[(m - 1) * 6 + i] >= 0}
         return (bm.value, bl.value), times
 
+    def tune_db(self, path, nsig: int, per_gdir: bool = False):
+        """Stored choice from the tuning database at path, or tune and append; ((mapping, lanes), hit)."""
+        bl, bm, hit = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(LIB.dmri_tune_db(self._h, None if path is None else str(path).encode(), int(nsig),
+                                1 if per_gdir else 0, ctypes.byref(bl), ctypes.byref(bm), ctypes.byref(hit)))
+        return (bm.value, bl.value), bool(hit.value)
+
     def set_tuning(self, nsig: int, per_gdir: bool, lanes: int, mapping: int):
         _check(LIB.dmri_set_tuning(self._h, int(nsig), 1 if per_gdir else 0, int(lanes), int(mapping)))
 
     def adjoint(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-                checkpoint_stride=0, lanes=0, guess=0, mapping=0, precond=0, u0=None):
+                checkpoint_stride=0, lanes=0, guess=0, mapping=0, precond=0, u0=None, krylov=0, restart=0):
         """Signal and its adjoint gradients (see dmri_adjoint): returns (signal, grads dict, stats)."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, False, lanes,
-                                    guess, mapping, None, precond, u0, self._nv(u0))
+                                    guess, mapping, None, precond, u0, self._nv(u0), None, krylov, restart)
         ao = _AdjOpts()
         ao.checkpoint_stride = int(checkpoint_stride)
         n = s.nsteps

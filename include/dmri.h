@@ -43,9 +43,12 @@ enum dmri_matrix {
     DMRI_MAT_JX = 4, /* int x phi_i phi_j  (GdotX, DmriFemLib.py:33-39; Jg = gx Jx + gy Jy + gz Jz) */
     DMRI_MAT_JY = 5,
     DMRI_MAT_JZ = 6,
-    DMRI_MAT_P = 7   /* weak pseudo-periodic boundary mass  int_{dOmega} kappa_e phi_i phi_j per compartment,
+    DMRI_MAT_P = 7,  /* weak pseudo-periodic boundary mass  int_{dOmega} kappa_e phi_i phi_j per compartment,
                         kappa_e = (3e-3/hmin) * P1 indicator of the periodic faces (ThetaMethodF_wBC1c/2c,
                         DmriFemLib.py:92,143; WeakPseudoPeridicMarker 601-610); zero without periodic dirs */
+    DMRI_MAT_GX = 8, /* strong periodic form: (C - B)_x (dmri_periodic_strong_set); zero otherwise */
+    DMRI_MAT_GY = 9,
+    DMRI_MAT_GZ = 10
 };
 
 /* Create a context on HIP device `device`.
@@ -101,7 +104,8 @@ typedef struct {
     double volume;        /* sum of M (|Omega|; whole_vol of PostProcessing, DmriFemLib.py:923) */
     double hmin, hmax;    /* min/max greatest vertex distance of a cell (mymesh.hmin/hmax) */
     double setup_ms;      /* device time of dmri_assemble */
-    int64_t n_periodic;   /* DOFs on a periodic face that take a mirrored value (dmri_export_periodic) */
+    int64_t n_periodic;   /* weak periodic condition: DOFs on a periodic face that take a mirrored value
+                             (dmri_export_periodic); strong: DOFs identified with their master's */
     double kappa_e;       /* 3e-3/hmin when periodic directions are set (MyDomain.kappa_e_scalar), else 0 */
 } dmri_info;
 int dmri_get_info(dmri_ctx *ctx, dmri_info *info);
@@ -127,6 +131,29 @@ int dmri_export_facets(dmri_ctx *ctx, int32_t *fv);
  * (dmri_seq.F_explicit).  dir = {0,0,0} switches the condition off.  The adjoint does not support
  * it (DMRI_ERR_ARG). */
 int dmri_periodic_set(dmri_ctx *ctx, const int32_t dir[3]);
+
+/* Strong pseudo-periodic condition (MyDomain.PeriodicDir with IsDomainPeriodic = True: PeriodicBD,
+ * DmriFemLib.py:327-375, on the function spaces of MyFunctionSpaces 453-486; the forms
+ * ThetaMethodF/L_sBC1c/2c 167-238 with FuncF_sBC, outer_interface and inner_interface 148-165).
+ * dir[d] != 0 identifies the two faces of the bounding box normal to axis d: every vertex within
+ * 1e-2 hmin of the max face shares the DOF of the vertex at its image on the min face (found within
+ * the same distance; the mesh must be periodic there, DMRI_ERR_MESH otherwise; a vertex on several
+ * max faces takes the min corner).  The unknown is the magnetisation times exp(i q F(t) g.x), and
+ * each step solves (theta on both sides, as the reference's forms)
+ *   A_n = M/k + theta (K + R + I) + theta q^2 F_n^2 D_c M + i theta q F_n (C - B).g
+ *   B_n = M/k - theta (K + R + I) - theta q^2 F_{n-1}^2 D_c M - i theta q F_{n-1} (C - B).g
+ * on the constrained space, with C_d = int 2 D_c (d phi_j/d x_d) phi_i and B_d the integral of D_c n_d
+ * phi_i phi_j over each compartment's boundary (outer facets and the interface, n outward of that
+ * compartment).  Each compartment's outer term is taken over its own facets; the reference takes each
+ * compartment's over every exterior facet, which differs only where the other compartment's cells
+ * reach a non-periodic face with g.n != 0.  The reference's 1e-16 added to q F in outer_interface is
+ * left out (below FP64 resolution of the signal).  Needs a scalar D per compartment (no D_cell),
+ * dmri_seq.F_implicit and F_explicit, and one q per launch group (batched signals are grouped by q).
+ * The initial guess is the quadratic extrapolation when the projection is asked for.  The exports
+ * then describe the constrained space (dof_of maps every vertex of a max face to its master's DOF;
+ * DMRI_MAT_GX..GZ hold (C - B)_d).  dir = {0,0,0}, or dmri_periodic_set, switches it off.  The
+ * adjoint does not support it (DMRI_ERR_ARG).  Takes effect at the next dmri_assemble. */
+int dmri_periodic_strong_set(dmri_ctx *ctx, const int32_t dir[3]);
 
 /* Parity hook: the n_periodic mirrored DOFs of the last dmri_assemble.  dof[e] is the DOF that takes
  * the mirrored value; src[3e..3e+2] the DOFs (same compartment, -1 = dead) of the facet vertices
@@ -166,6 +193,9 @@ typedef struct {
                                  and holds u in dmri_get_solution's layout (interleaved re/im in DOF
                                  order; nsig = 1 for dmri_solve).  A device-to-host copy (and a wait
                                  for it) per snapshot, so only as often as needed */
+    const double *F_implicit; /* nsteps values of F(t) = int_0^t f on the implicit side's time (t_n;
+                                 MRI_parameters.itime_profile at the step's t, DmriFemLib.py:901): required
+                                 by the strong periodic condition, ignored otherwise (may be NULL) */
 } dmri_seq;
 
 typedef struct {
@@ -210,6 +240,15 @@ typedef struct {
                                Every choice converges to the same solution; the polynomial needs
                                SpMV mapping 1, 2 or 4 (others fall back to mapping 2).
                                dmri_stats.precond_used reports the choice. */
+    int32_t krylov;         /* 0 = BiCGStab (KrylovSolver("bicgstab", ...), GCloudDmriSolver.py:219;
+                               the default), 1 = restarted GMRES (KrylovSolver("gmres", ...),
+                               comri/one-comp/fenics-cpp/main.cpp:180-183) with the same right
+                               preconditioner: classical Gram-Schmidt applied twice, Givens rotations
+                               on the device, the true residual checked at the end of every cycle.
+                               Both stop at ||b - A x|| <= max(rtol ||b||, atol).  dmri_stats.iters_*
+                               count Arnoldi steps for GMRES. */
+    int32_t restart;        /* GMRES restart length, 0 = 30, at most 64 (m + 1 basis vectors per signal
+                               are kept in HBM) */
 } dmri_solver_opts;
 
 typedef struct {
@@ -227,16 +266,17 @@ typedef struct {
     int32_t reserved;
     /* Every kernel of the timed Krylov iterations (profile > 0), by its own dispatch timestamps:
      * [0] bicg_p (p = r + beta (p - omega v)), [1] preconditioner SpMV on p (ph), [2] v = A ph (or A p),
-     * [3] preconditioner SpMV on s = r - alpha v (sh; s written), [4] t = A sh (or A s), [5] the update of
-     * x and r.  [1] and [3] stay 0 without the polynomial preconditioner.  kern_ms: summed device time;
-     * kern_bytes: summed algorithmic HBM bytes of those launches (matrix entries, column indices and row
-     * offsets once, every vector read or written once; per launch: [0] 64 n S (32 n S in a solve's first
-     * iteration), [1] E(npre) + 32 n S, [2] E(nval) + 48 n S, [3] E(npre) + 64 n S, [4] E(nval) + 48 n S,
-     * [5] 128 n S, with E(k) = nnz (4 + 8k) + 4 (n+1), nval = 2 (4 with per-signal directions) and npre = 1
-     * for the real-part polynomial, else nval). */
-    double kern_ms[6];
-    int64_t kern_launches[6];
-    double kern_bytes[6];
+     * [3] the s-pass (alpha, s = r - alpha v), [4] preconditioner SpMV on s (sh), [5] t = A sh (or A s),
+     * [6] the update of x and r.  [1], [3] and [4] stay 0 without the polynomial preconditioner.
+     * kern_ms: summed device time; kern_bytes: summed algorithmic HBM bytes of those launches (matrix
+     * entries, column indices and row offsets once, every vector read or written once; per launch:
+     * [0] 64 n S (32 n S in a solve's first iteration), [1] E(npre) + 32 n S, [2] E(nval) + 48 n S,
+     * [3] 48 n S, [4] E(npre) + 32 n S, [5] E(nval) + 48 n S, [6] 128 n S, with E(k) = nnz (4 + 8k) +
+     * 4 (n+1), nval = 2 (4 with per-signal directions) and npre = 1 for the real-part polynomial, else
+     * nval). */
+    double kern_ms[7];
+    int64_t kern_launches[7];
+    double kern_bytes[7];
 } dmri_stats;
 
 /* Run the theta scheme from u0 = 1 (real part, every live DOF) and return the signal
@@ -277,23 +317,32 @@ int dmri_batch_advance(dmri_batch *batch, int32_t nsteps, dmri_stats *stats);
 int dmri_batch_signals(dmri_batch *batch, double *sig_out, int32_t *steps_done);
 int dmri_batch_end(dmri_batch *batch);
 
-/* Autotuner: time the two SpMV kernels of a Krylov iteration for every valid (mapping 1..5,
- * lanes per row 1..32) on the assembled pattern (us[30] = [mapping-1][log2 lanes] microseconds,
- * -1 = not valid) and keep the fastest for later solves of this context whose options leave
- * mapping and lanes_per_row at 0.  dmri_set_tuning installs a stored choice (e.g. from a tuning
- * database) without timing.  No reference counterpart (the reference has no GPU kernels);
- * results do not depend on the choice beyond rounding. */
+/* Autotuner: time the four SpMV kernels of a polynomial-preconditioned BiCGStab iteration (the default
+ * solve: the preconditioner SpMV on p, v = A ph, the preconditioner SpMV on s, t = A sh) for every
+ * (mapping, lanes per row 1..32) that runs them (mappings 1, 2 and 4) on the assembled pattern
+ * (us[30] = [mapping-1][log2 lanes] microseconds per iteration's four SpMVs, -1 = not valid) and keep
+ * the fastest for later solves of this context whose options leave mapping and lanes_per_row at 0.
+ * dmri_set_tuning installs a stored choice without timing.  No reference counterpart (the reference
+ * has no GPU kernels); results do not depend on the choice beyond rounding. */
 int dmri_tune(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t *best_lanes, int32_t *best_mapping,
               double *us);
 int dmri_set_tuning(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t lanes, int32_t mapping);
 
-/* Replay the two SpMV kernels of a BiCGStab iteration (A p and A s, s = r - alpha v formed on the
- * fly) `reps` times each, back to back, on this context's solver buffers for a launch group of
- * nsig signals, with the mapping a solve of that group would use.  us_v, us_t = microseconds per
- * launch (HIP events around each batch of reps, so no per-launch event gap is included);
- * bytes_per_launch (may be NULL) = algorithmic HBM bytes of one launch: nnz*(4 + 8*nv) column
- * indices and values (nv = 2, or 4 with per-signal gradient directions) + 4*(n+1) row pointers +
- * 48*n*nsig for the complex vectors read and written.  No reference counterpart. */
+/* Tuning database: a text file with one "device ndof nnz nsig per_gdir mapping lanes us" line per
+ * tuned (device, pattern size, launch group) key.  dmri_tune_db installs the stored choice for this
+ * context's pattern when the file holds one (*hit = 1, nothing is timed; the last matching line
+ * wins); otherwise it runs dmri_tune and appends the choice (*hit = 0).  path NULL or "" = tune
+ * without a database.  No reference counterpart. */
+int dmri_tune_db(dmri_ctx *ctx, const char *path, int32_t nsig, int32_t per_gdir, int32_t *best_lanes,
+                 int32_t *best_mapping, int32_t *hit);
+
+/* Replay the two operator SpMV kernels of a polynomial-preconditioned BiCGStab iteration (v = A ph and
+ * t = A sh with s = r - alpha v read at the own row) `reps` times each, back to back, on this context's
+ * solver buffers for a launch group of nsig signals, with the mapping a solve of that group would use.
+ * us_v, us_t = microseconds per launch (kernel-bound HIP events, no marker packets between launches);
+ * bytes_per_launch (may be NULL) = algorithmic HBM bytes of one launch: nnz*(4 + 8*nv) column indices
+ * and values (nv = 2, or 4 with per-signal gradient directions) + 4*(n+1) row pointers + 48*n*nsig for
+ * the complex vectors (one gathered, one read at the own row, one written).  No reference counterpart. */
 int dmri_time_spmv(dmri_ctx *ctx, int32_t nsig, int32_t per_gdir, int32_t reps, double *us_v, double *us_t,
                    double *bytes_per_launch);
 

@@ -450,6 +450,7 @@ class StrongPeriodicSystem:
             A = sps.csr_matrix((vals, (rows, S.col)), shape=(S.ndof, S.ndof))
             return (P.T @ A @ P).tocsr()
 
+        self.pattern = merged(np.ones(S.nnz))  # the constrained sparsity (every product entry)
         self.M = merged(S.mats["m"])
         self.L = merged(S.mats["k"] + S.mats["r"] + S.mats["gi"])
         comp = np.zeros(S.ndof)

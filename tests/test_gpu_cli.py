@@ -187,10 +187,24 @@ def test_gcloud_periodic_box_matches_oracle(tmp_path):
     assert mm.group(1) == f"{ref:.6e}"
 
 
-def test_cli_rejects_strong_periodic_and_refinement(tmp_path):
+def test_cli_rejects_refinement(tmp_path):
     m = dm.box_mesh((0.0,) * 3, (2.0,) * 3, 2, 2, 2)
     dm.write_dolfin_xml(m, str(tmp_path / "box.xml"))
-    for extra in (["-pdir", 1, 0, 0, "-IsPeriodic", 1], ["-r", 1]):
-        r = subprocess.run([CLI, "-f", "box.xml", "-K", "2e-3"] + [str(a) for a in extra], capture_output=True,
-                           text=True, cwd=tmp_path, timeout=60)
-        assert r.returncode != 0 and ("strong" in r.stderr or "refinement" in r.stderr), r.stderr
+    r = subprocess.run([CLI, "-f", "box.xml", "-K", "2e-3", "-r", "1"], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=60)
+    assert r.returncode != 0 and "refinement" in r.stderr, r.stderr
+
+
+def test_gcloud_strong_periodic_box_matches_oracle(tmp_path):
+    """-IsPeriodic 1 -pdir px py pz: the strong pseudo-periodic form on a periodic box mesh."""
+    m = dm.box_mesh((0.0, 0.0, 0.0), (4.0, 3.0, 3.0), 8, 6, 6)
+    dm.write_dolfin_xml(m, str(tmp_path / "box.xml"))
+    out = _run(["-f", "box.xml", "-M", 0, "-K", 2e-3, "-b", 1500, "-d", 1000, "-D", 6000, "-k", 100,
+                "-gdir", 1, 0.5, 0, "-pdir", 1, 1, 0, "-IsPeriodic", 1, "-rtol", 1e-13, "-atol", 0], tmp_path)
+    mm = re.search(r"Normalized signal: ([0-9.e+-]+)", out)
+    assert mm, out
+    SP = O.StrongPeriodicSystem(m.xyz, m.tets, None, D=2e-3, pdir=(1, 1, 0))
+    F = O.pgse_integral(1000.0, 6000.0)
+    FA, FB = O.gcloud_sequence(F, 7000.0, 100.0)
+    ref, _ = SP.solve((1, 0.5, 0), O.q_from_b_pgse(1500.0, 1000.0, 6000.0), 100.0, FA, FB)
+    assert mm.group(1) == f"{ref:.6e}"

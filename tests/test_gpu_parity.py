@@ -315,12 +315,13 @@ def test_sampled_event_profile_and_spmv_replay():
     info = ctx.info()
     # every kernel of the timed iterations: the six of a preconditioned iteration, with their own bytes
     nl = st1["kern_launches"]
-    assert nl[0] == nl[1] == nl[2] == nl[3] == nl[4] == nl[5] == st1["spmv_launches"] // 2 > 0
+    assert len(set(nl)) == 1 and nl[0] == st1["spmv_launches"] // 2 > 0
     assert all(m > 0 for m in st1["kern_ms"])
     n, nnz = info["ndof"], info["nnz"]
     E2 = nnz * 20.0 + 4.0 * (n + 1)
     assert abs(st1["kern_bytes"][2] / nl[2] - (E2 + 48.0 * n)) <= 1e-9 * E2
-    assert abs(st1["kern_bytes"][5] / nl[5] - 128.0 * n) <= 1e-9 * n
+    assert abs(st1["kern_bytes"][3] / nl[3] - 48.0 * n) <= 1e-9 * n
+    assert abs(st1["kern_bytes"][6] / nl[6] - 128.0 * n) <= 1e-9 * n
     us_v, us_t, nb = ctx.time_spmv(1, False, 10)
     assert us_v > 0 and us_t > 0
     assert nb == info["nnz"] * 20.0 + 4.0 * (info["ndof"] + 1) + 48.0 * info["ndof"]
@@ -542,3 +543,28 @@ def test_snapshot_edge_cases():
                       snap_every=1)
     assert st["snapshots"].shape[0] == len(fA)
     assert np.array_equal(st["snapshots"][-1], ctx.solution(0))
+
+
+def test_tuning_database_round_trip(tmp_path):
+    """dmri_tune_db: a miss tunes and appends one line, a hit installs the stored choice without
+    timing; another pattern misses; the stored choice gives the same signals."""
+    mesh, kw = MESHES["cyl2"]
+    db = tmp_path / "tune.db"
+    ctx = _gpu_system(mesh, kw["D"], kw["kappa"])
+    (m1, l1), hit1 = ctx.tune_db(db, 4, True)
+    assert not hit1 and m1 in (1, 2, 4)
+    lines = db.read_text().split("\n")
+    assert len([x for x in lines if x.strip()]) == 1 and lines[0].split()[5:7] == [str(m1), str(l1)]
+    ctx2 = _gpu_system(mesh, kw["D"], kw["kappa"])
+    (m2, l2), hit2 = ctx2.tune_db(db, 4, True)
+    assert hit2 and (m2, l2) == (m1, l1)
+    other, kwo = MESHES["cyl1"]
+    ctx3 = _gpu_system(other, kwo["D"], kwo["kappa"])
+    (_, _), hit3 = ctx3.tune_db(db, 4, True)
+    assert not hit3 and len([x for x in db.read_text().split("\n") if x.strip()]) == 2
+    fA, fB = _pgse_gcloud(10600.0, 43100.0, 800.0)
+    qs = np.full(4, O.q_from_b_pgse(1500.0, 10600.0, 43100.0))
+    gd = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1]], dtype=float)
+    s_tuned, _ = ctx2.solve_batched(fA, fB, 800.0, qs, gd, rtol=1e-13)
+    s_def, _ = _gpu_system(mesh, kw["D"], kw["kappa"]).solve_batched(fA, fB, 800.0, qs, gd, rtol=1e-13)
+    assert np.abs(s_tuned - s_def).max() <= 1e-11

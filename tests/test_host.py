@@ -113,11 +113,11 @@ def test_dmrifemlib_parameters_match_closed_forms():
     assert abs(p.gdir.x() - 1.0) < 1e-15
     simu = L.MRI_simulation()
     simu.k = 200
-    fA, fB, FB = simu._sequence(p, integral=True)
+    fA, fB, FB, FA = simu._sequence(p, integral=True)
     rA, rB = O.gcloud_sequence(O.pgse_piecewise(10600.0, 43100.0), 53700.0, 200.0)
     assert np.array_equal(fA, rA) and np.array_equal(fB, rB)
-    _, rF = O.gcloud_sequence(O.pgse_integral(10600.0, 43100.0), 53700.0, 200.0)
-    assert np.abs(FB - rF).max() <= 1e-9
+    rFA, rF = O.gcloud_sequence(O.pgse_integral(10600.0, 43100.0), 53700.0, 200.0)
+    assert np.abs(FB - rF).max() <= 1e-9 and np.abs(FA - rFA).max() <= 1e-9
     dom = L.MyDomain(mesh, p)
     assert f"{dom.hmin:.3e}" == f"{np.sqrt(3) * 5 / 4:.3e}"
 

@@ -575,3 +575,52 @@ def test_interface_facets_match_brute_force_enumeration(maker):
     ref = _brute_interface_facets(m.tets, phase)
     assert ref.shape[0] > 10
     assert np.array_equal(S.fv, ref)
+
+
+def test_strong_periodic_direction_terms_match_quadrature_and_integration_by_parts():
+    """orc_assemble_sbc: C_d = int 2 D (d phi_j/d x_d) phi_i against quadrature on one tet; over a
+    two-compartment mesh, C_d + C_d^T = 2 B_d (int 2D g.grad(phi_i phi_j) = boundary integral of D g.n
+    over each compartment's outer and interface facets), which pins the facet normals' signs."""
+    X = np.array([[0.1, 0.0, 0.2], [1.4, 0.2, 0.1], [0.3, 1.2, -0.2], [0.2, 0.4, 1.1]])
+    SP = O.StrongPeriodicSystem(X, np.array([[0, 1, 2, 3]]), None, D=1.7e-3, pdir=(0, 0, 0))
+    pts, w, lam, E = tet_quadrature(X)
+    G = _cell_gradients(X)
+    S = SP.base
+    for d, k in enumerate(("cx", "cy", "cz")):
+        ref = np.einsum("q,qa->a", w, lam)[:, None] * (2 * 1.7e-3 * G[:, d])[None, :]  # [i, j]
+        assert np.abs(S.dense(SP.ext[k]) - ref).max() <= 1e-13 * np.abs(ref).max()
+    m = dm.cylinder_mesh(5.0, 3.0, 1200, seed=21, interface_radii=(2.5,))
+    SP2 = O.StrongPeriodicSystem(m.xyz, m.tets, m.phase, D=(2e-3, 1e-3), pdir=(0, 0, 0))
+    S2 = SP2.base
+    for a in "xyz":
+        C = S2.dense(SP2.ext["c" + a])
+        B = S2.dense(SP2.ext["b" + a])
+        assert np.abs(B).max() > 0
+        assert np.abs(C + C.T - 2 * B).max() <= 1e-13 * np.abs(B).max()
+
+
+def test_periodic_masters_brute_force_on_a_box():
+    m = dm.box_mesh((0.0, -1.0, 0.0), (4.0, 2.0, 3.0), 4, 3, 3)
+    master = O.periodic_masters(m.xyz, (1, 1, 0), 1e-3)
+    for v, x in enumerate(m.xyz):
+        t = x.copy()
+        if abs(x[0] - 4.0) < 1e-9:
+            t[0] = 0.0
+        if abs(x[1] - 2.0) < 1e-9:
+            t[1] = -1.0
+        w_ = np.nonzero(np.all(np.abs(m.xyz - t) < 1e-9, axis=1))[0]
+        assert w_.shape == (1,) and master[v] == w_[0]
+    assert master[np.argmin(np.abs(m.xyz - [4.0, 2.0, 3.0]).sum(axis=1))] == \
+        np.argmin(np.abs(m.xyz - [0.0, -1.0, 3.0]).sum(axis=1))
+
+
+def test_strong_periodic_fully_periodic_box_closed_form():
+    m = dm.box_mesh((0.0, 0.0, 0.0), (3.0, 2.0, 2.0), 6, 4, 4)
+    SP = O.StrongPeriodicSystem(m.xyz, m.tets, None, D=2e-3, invT2=1 / 5e4, pdir=(1, 1, 1))
+    k, theta, q, D, iT2 = 100.0, 0.5, 3e-4, 2e-3, 1 / 5e4
+    F = O.pgse_integral(1000.0, 4000.0)
+    FA, FB = O.gcloud_sequence(F, 5000.0, k)
+    s, u = SP.solve((0.3, 1.0, 0.4), q, k, FA, FB)
+    r = np.prod([(1 / k - theta * (q * q * D * fb * fb + iT2)) / (1 / k + theta * (q * q * D * fa * fa + iT2))
+                 for fa, fb in zip(FA, FB)])
+    assert abs(s - r) <= 1e-12 * r and np.abs(u - u[0]).max() <= 1e-12
