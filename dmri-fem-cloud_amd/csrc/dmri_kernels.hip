@@ -2287,8 +2287,9 @@ __global__ void __launch_bounds__(kThreads) gm_fin_kernel(const double *parts, i
     }
 }
 
-// w -= sum_{i <= j} h_i v_i (h from a.h); norm: partials [|w|^2, 0, 0, 0] (kNV layout); the pass's
-// coefficients are added to a.hsum
+// w -= sum_{i <= j} h_i v_i (h from a.h); norm (the second pass): partials [|w|^2, 0, 0, 0] (kNV
+// layout).  Column j of the Hessenberg matrix is the sum of both passes' coefficients: the first pass
+// stores its h in a.hsum, the second adds to it.
 __global__ void __launch_bounds__(kThreads) gm_orth_kernel(GmArgs a, int j, int norm, int vrpb) {
     if (*a.all_done) return;
     const int S = a.S;
@@ -2299,8 +2300,10 @@ __global__ void __launch_bounds__(kThreads) gm_orth_kernel(GmArgs a, int j, int 
     const int hst = 2 * (a.m + 1);
     if (blockIdx.x == 0 && (int)threadIdx.x < S && !a.st[threadIdx.x].done)
         for (int i = 0; i <= j; ++i) {
-            a.hsum[(size_t)threadIdx.x * hst + 2 * i] += a.h[(size_t)threadIdx.x * hst + 2 * i];
-            a.hsum[(size_t)threadIdx.x * hst + 2 * i + 1] += a.h[(size_t)threadIdx.x * hst + 2 * i + 1];
+            double *hs = a.hsum + (size_t)threadIdx.x * hst + 2 * i;
+            const double *h = a.h + (size_t)threadIdx.x * hst + 2 * i;
+            hs[0] = norm ? hs[0] + h[0] : h[0];
+            hs[1] = norm ? hs[1] + h[1] : h[1];
         }
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     if (act)
