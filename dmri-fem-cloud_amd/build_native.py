@@ -2,6 +2,7 @@
 
     python dmri-fem-cloud_amd/build_native.py            # build everything
     python dmri-fem-cloud_amd/build_native.py --check    # print the output paths
+    python dmri-fem-cloud_amd/build_native.py --sanitized OUT  # the CLI under ASan + UBSan (host code)
 
 Outputs: dmri-fem-cloud_amd/lib/libdmri.so and dmri-fem-cloud_amd/lib/dmri-solve.
 Object files go to dmri-fem-cloud_amd/_build/.
@@ -76,8 +77,21 @@ def build(verbose: bool = False) -> str:
     return libdmri_path()
 
 
+def build_cli_sanitized(out: str) -> str:
+    """The dmri-solve CLI built with AddressSanitizer and UndefinedBehaviorSanitizer (host code only:
+    the CLI is plain C++ built by g++, linked against the normal libdmri.so), for running its
+    argument parsing and mesh/phase readers under the sanitizers; a finding aborts the program."""
+    cli_src = os.path.join(CSRC, CLI_SOURCE)
+    _run(["g++", "-O1", "-g", "-std=c++17", "-Wall", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+          "-fno-omit-frame-pointer", "-I", INCLUDE, cli_src, "-o", out, "-L", LIBDIR, "-ldmri",
+          f"-Wl,-rpath,{LIBDIR}", "-lm"])
+    return out
+
+
 if __name__ == "__main__":
-    if "--check" in sys.argv:
+    if "--sanitized" in sys.argv:
+        print(build_cli_sanitized(sys.argv[sys.argv.index("--sanitized") + 1]))
+    elif "--check" in sys.argv:
         print(libdmri_path(), cli_path())
     else:
         print(build(verbose=True))

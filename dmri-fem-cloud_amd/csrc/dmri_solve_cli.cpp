@@ -23,6 +23,7 @@
 // both sides, loop t < T + dt, FT inclusive at Delta + delta (main.cpp:87-92,296-318);
 // comri-until-T = the multilayer loop t < T (comri/multilayer/hpc-fenics-cpp/main.cpp:301).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -30,6 +31,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include <sys/stat.h>
@@ -55,34 +57,45 @@ bool attr(const std::string &line, const char *name, std::string &out) {
     return true;
 }
 
+// Vertices and cells are placed by their index attribute; an index must lie below the number of
+// entries of its kind (as the Python reader, which sizes the arrays by that count).  A malformed
+// number makes the file unreadable (false), never an exception.
 bool read_dolfin_xml(const std::string &path, Mesh &m) {
     std::ifstream in(path);
     if (!in) return false;
-    std::string line, v;
-    while (std::getline(in, line)) {
-        if (line.find("<vertex ") != std::string::npos) {
-            std::string si, sx, sy, sz;
-            if (!attr(line, "index", si) || !attr(line, "x", sx) || !attr(line, "y", sy) || !attr(line, "z", sz))
-                return false;
-            size_t i = std::stoul(si);
-            if (m.xyz.size() < 3 * (i + 1)) m.xyz.resize(3 * (i + 1));
-            m.xyz[3 * i] = std::stod(sx);
-            m.xyz[3 * i + 1] = std::stod(sy);
-            m.xyz[3 * i + 2] = std::stod(sz);
-        } else if (line.find("<tetrahedron ") != std::string::npos) {
-            std::string si, a, b, c, d;
-            if (!attr(line, "index", si) || !attr(line, "v0", a) || !attr(line, "v1", b) || !attr(line, "v2", c) ||
-                !attr(line, "v3", d))
-                return false;
-            size_t i = std::stoul(si);
-            if (m.tets.size() < 4 * (i + 1)) m.tets.resize(4 * (i + 1));
-            m.tets[4 * i] = std::stoi(a);
-            m.tets[4 * i + 1] = std::stoi(b);
-            m.tets[4 * i + 2] = std::stoi(c);
-            m.tets[4 * i + 3] = std::stoi(d);
+    std::string line;
+    std::vector<std::pair<size_t, std::array<double, 3>>> verts;
+    std::vector<std::pair<size_t, std::array<int32_t, 4>>> cells;
+    try {
+        while (std::getline(in, line)) {
+            if (line.find("<vertex ") != std::string::npos) {
+                std::string si, sx, sy, sz;
+                if (!attr(line, "index", si) || !attr(line, "x", sx) || !attr(line, "y", sy) || !attr(line, "z", sz))
+                    return false;
+                verts.push_back({std::stoul(si), {std::stod(sx), std::stod(sy), std::stod(sz)}});
+            } else if (line.find("<tetrahedron ") != std::string::npos) {
+                std::string si, a, b, c, d;
+                if (!attr(line, "index", si) || !attr(line, "v0", a) || !attr(line, "v1", b) ||
+                    !attr(line, "v2", c) || !attr(line, "v3", d))
+                    return false;
+                cells.push_back({std::stoul(si), {std::stoi(a), std::stoi(b), std::stoi(c), std::stoi(d)}});
+            }
         }
+    } catch (const std::exception &) {  // std::invalid_argument / std::out_of_range from stoul, stod, stoi
+        return false;
     }
-    return !m.xyz.empty() && !m.tets.empty();
+    if (verts.empty() || cells.empty()) return false;
+    m.xyz.assign(3 * verts.size(), 0.0);
+    m.tets.assign(4 * cells.size(), 0);
+    for (const auto &v : verts) {
+        if (v.first >= verts.size()) return false;
+        for (int d = 0; d < 3; ++d) m.xyz[3 * v.first + d] = v.second[d];
+    }
+    for (const auto &c : cells) {
+        if (c.first >= cells.size()) return false;
+        for (int k = 0; k < 4; ++k) m.tets[4 * c.first + k] = c.second[k];
+    }
+    return true;
 }
 
 // ASCII gmsh 2.x, read as dolfin-convert's gmsh2xml turns it into the DOLFIN XML mesh the
@@ -115,17 +128,17 @@ bool read_gmsh(const std::string &path, Mesh &m) {
         } else if (line == "$Nodes") {
             long n = 0;
             if (!next_line() || (n = std::atol(line.c_str())) <= 0) return false;
-            node_id.resize(n);
-            xyz.resize(3 * n);
             char buf[64];
-            for (long i = 0; i < n; ++i) {
+            for (long i = 0; i < n; ++i) {  // grown line by line: a count past the end fails at EOF
                 if (!next_line()) return false;
                 std::istringstream ss(line);
+                long id;
                 double c[3];
-                if (!(ss >> node_id[i] >> c[0] >> c[1] >> c[2])) return false;
+                if (!(ss >> id >> c[0] >> c[1] >> c[2]) || id < 0) return false;
+                node_id.push_back(id);
                 for (int d = 0; d < 3; ++d) {
                     std::snprintf(buf, sizeof(buf), "%g", c[d]);
-                    xyz[3 * i + d] = std::strtod(buf, nullptr);
+                    xyz.push_back(std::strtod(buf, nullptr));
                 }
             }
             have_nodes = true;
@@ -137,7 +150,8 @@ bool read_gmsh(const std::string &path, Mesh &m) {
                 std::istringstream ss(line);
                 long id, type, ntags, t;
                 if (!(ss >> id >> type >> ntags)) return false;
-                for (long k = 0; k < ntags; ++k) ss >> t;
+                for (long k = 0; k < ntags; ++k)
+                    if (!(ss >> t)) return false;
                 if (type != 4) continue;
                 for (int k = 0; k < 4; ++k) {
                     if (!(ss >> t)) return false;
@@ -147,18 +161,29 @@ bool read_gmsh(const std::string &path, Mesh &m) {
         }
     }
     if (!fmt_ok || !have_nodes || tet_nodes.empty()) return false;
+    // node id -> position in $Nodes: a table over the ids when they are dense (the usual 1..n),
+    // a hash map when they are sparse
     long maxid = 0;
     for (long v : node_id) maxid = std::max(maxid, v);
-    for (long v : tet_nodes) maxid = std::max(maxid, v);
-    std::vector<long> pos(maxid + 1, -1);
+    const bool dense = maxid <= 4 * (long)node_id.size() + 1024;
+    std::vector<long> table(dense ? maxid + 1 : 0, -1);
+    std::unordered_map<long, long> hashed;
     for (size_t i = 0; i < node_id.size(); ++i) {
-        if (node_id[i] < 0) return false;
-        pos[node_id[i]] = (long)i;
+        if (dense) table[node_id[i]] = (long)i;
+        else hashed[node_id[i]] = (long)i;
     }
+    auto where = [&](long id) -> long {
+        if (dense) return id >= 0 && id <= maxid ? table[id] : -1;
+        auto it = hashed.find(id);
+        return it == hashed.end() ? -1 : it->second;
+    };
+    std::vector<long> pos_of(tet_nodes.size());
     std::vector<char> used(node_id.size(), 0);
-    for (long v : tet_nodes) {
-        if (v < 0 || pos[v] < 0) return false;
-        used[pos[v]] = 1;
+    for (size_t k = 0; k < tet_nodes.size(); ++k) {
+        const long p = where(tet_nodes[k]);
+        if (p < 0) return false;
+        pos_of[k] = p;
+        used[p] = 1;
     }
     std::vector<int32_t> renum(node_id.size(), -1);
     int32_t nv = 0;
@@ -168,7 +193,7 @@ bool read_gmsh(const std::string &path, Mesh &m) {
             m.xyz.insert(m.xyz.end(), xyz.begin() + 3 * i, xyz.begin() + 3 * i + 3);
         }
     m.tets.resize(tet_nodes.size());
-    for (size_t k = 0; k < tet_nodes.size(); ++k) m.tets[k] = renum[pos[tet_nodes[k]]];
+    for (size_t k = 0; k < tet_nodes.size(); ++k) m.tets[k] = renum[pos_of[k]];
     return true;
 }
 
@@ -246,22 +271,26 @@ bool read_phase(const std::string &path, size_t nc, std::vector<int32_t> &phase)
     std::string line;
     size_t plain = 0;
     bool xml = false;
-    while (std::getline(in, line)) {
-        if (line.find("<value ") != std::string::npos) {
-            xml = true;
-            std::string ci, val;
-            if (!attr(line, "cell_index", ci) || !attr(line, "value", val)) return false;
-            size_t i = std::stoul(ci);
-            if (i >= nc) return false;
-            phase[i] = std::stoi(val) % 2;
-        } else if (!xml) {
-            std::istringstream ss(line);
-            long v;
-            while (ss >> v) {
-                if (plain >= nc) return false;
-                phase[plain++] = (int32_t)(v % 2);
+    try {
+        while (std::getline(in, line)) {
+            if (line.find("<value ") != std::string::npos) {
+                xml = true;
+                std::string ci, val;
+                if (!attr(line, "cell_index", ci) || !attr(line, "value", val)) return false;
+                size_t i = std::stoul(ci);
+                if (i >= nc) return false;
+                phase[i] = (std::stoi(val) % 2 + 2) % 2;  // marker mod 2, non-negative
+            } else if (!xml) {
+                std::istringstream ss(line);
+                long v;
+                while (ss >> v) {
+                    if (plain >= nc) return false;
+                    phase[plain++] = (int32_t)((v % 2 + 2) % 2);
+                }
             }
         }
+    } catch (const std::exception &) {  // a malformed cell_index or value
+        return false;
     }
     return xml || plain == nc;
 }
