@@ -8,6 +8,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <unordered_map>
 #include <string>
 #include <vector>
@@ -142,6 +143,33 @@ void require(bool cond, int code, const std::string &msg) {
 }
 
 void set_device(dmri_ctx *c) { DMRI_HIP(hipSetDevice(c->device)); }
+
+// ---- staging buffer of dmri_get_solution_noalloc: DMRI_NOALLOC_MAX_DOF complex doubles (32 MiB)
+// in every GPU's copy of this code object, held until the process ends.  On the host, `scratch`
+// names a stand-in rather than device memory, so host code never hands it to a kernel or a copy:
+// each device's address is looked up once, with that device current, and kept in g_scratch.
+__device__ __align__(256) double2 scratch[DMRI_NOALLOC_MAX_DOF];
+
+struct ScratchSlot {
+    std::mutex mu;  // held for the whole of a call that uses this device's buffer
+    double2 *ptr = nullptr;
+};
+std::mutex g_scratch_mu;
+std::map<int, std::unique_ptr<ScratchSlot>> g_scratch;  // device -> its buffer
+
+// The slot of the current device `device`, registered on first use (dmri_create does that).
+ScratchSlot &scratch_slot(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it != g_scratch.end()) return *it->second;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+        throw Error(DMRI_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    auto slot = std::make_unique<ScratchSlot>();
+    slot->ptr = static_cast<double2 *>(p);
+    return *(g_scratch[device] = std::move(slot));
+}
 
 // vertex -> element incidence lists (sorted)
 void build_incidence(int nvert, const int *ev, int64_t ne, int64_t nv, DevArray<int> &ptr, DevArray<int> &list,
@@ -2173,6 +2201,7 @@ int dmri_create(dmri_ctx **out, int device) {
         DMRI_HIP(hipGetDeviceCount(&ndev));
         require(device >= 0 && device < ndev, DMRI_ERR_ARG, "dmri_create: no HIP device " + std::to_string(device));
         DMRI_HIP(hipSetDevice(device));
+        (void)scratch_slot(device);
         auto *c = new dmri_ctx();
         c->device = device;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -2558,6 +2587,23 @@ int dmri_get_solution(dmri_ctx *c, int32_t sig, double *u) {
         tmp.alloc(c->ndof);
         launch_unscale_perm(c->ndof, c->last_S, sig, c->perm.p, c->work.sc.p, c->work.x.p, tmp.p, c->stream);
         DMRI_HIP(hipMemcpyAsync(u, tmp.p, sizeof(double2) * c->ndof, hipMemcpyDeviceToHost, c->stream));
+        DMRI_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int dmri_get_solution_noalloc(dmri_ctx *c, int32_t sig, double *u) {
+    return guarded([&] {
+        require(c != nullptr && u != nullptr, DMRI_ERR_ARG, "null argument");
+        require(c->solved, DMRI_ERR_STATE, "no solution (call dmri_solve first)");
+        require(sig >= 0 && sig < c->last_S, DMRI_ERR_ARG, "signal index outside the last launch group");
+        require(c->ndof <= DMRI_NOALLOC_MAX_DOF, DMRI_ERR_LIMIT,
+                "dmri_get_solution_noalloc: " + std::to_string(c->ndof) + " DOFs exceed its buffer of " +
+                    std::to_string(DMRI_NOALLOC_MAX_DOF) + " (use dmri_get_solution)");
+        set_device(c);
+        ScratchSlot &slot = scratch_slot(c->device);
+        std::lock_guard<std::mutex> lk(slot.mu);
+        launch_unscale_perm(c->ndof, c->last_S, sig, c->perm.p, c->work.sc.p, c->work.x.p, slot.ptr, c->stream);
+        DMRI_HIP(hipMemcpyAsync(u, slot.ptr, sizeof(double2) * c->ndof, hipMemcpyDeviceToHost, c->stream));
         DMRI_HIP(hipStreamSynchronize(c->stream));
     });
 }

@@ -88,8 +88,8 @@ EXPORTED = ["dmri_create", "dmri_destroy", "dmri_last_error", "dmri_version", "d
             "dmri_assemble", "dmri_get_info", "dmri_export_pattern", "dmri_export_values", "dmri_export_facets",
             "dmri_periodic_set", "dmri_periodic_strong_set", "dmri_export_periodic",
             "dmri_solve", "dmri_solve_batched", "dmri_tune", "dmri_set_tuning", "dmri_time_spmv", "dmri_copy_bandwidth",
-            "dmri_tune_db", "dmri_adjoint", "dmri_get_solution", "dmri_batch_begin", "dmri_batch_advance", "dmri_batch_signals",
-            "dmri_batch_end"]
+            "dmri_tune_db", "dmri_adjoint", "dmri_get_solution", "dmri_get_solution_noalloc", "dmri_batch_begin",
+            "dmri_batch_advance", "dmri_batch_signals", "dmri_batch_end"]
 
 
 def _load():
@@ -116,6 +116,7 @@ def _load():
     L.dmri_solve.argtypes = [vp, P(_Seq), P(_Opts), dp, P(_Stats)]
     L.dmri_solve_batched.argtypes = [vp, ctypes.c_int32, dp, dp, P(_Seq), P(_Opts), dp, P(_Stats)]
     L.dmri_get_solution.argtypes = [vp, ctypes.c_int32, dp]
+    L.dmri_get_solution_noalloc.argtypes = [vp, ctypes.c_int32, dp]
     L.dmri_tune.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, i32p, i32p, dp]
     L.dmri_set_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     L.dmri_tune_db.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, i32p, i32p, i32p]

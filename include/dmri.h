@@ -391,6 +391,19 @@ int dmri_adjoint(dmri_ctx *ctx, const dmri_seq *seq, const dmri_solver_opts *opt
  * re/im, DOF order; the DOF of (compartment c, vertex v) is dof_of[c*nv+v]). */
 int dmri_get_solution(dmri_ctx *ctx, int32_t sig, double *u);
 
+/* Largest ndof that dmri_get_solution_noalloc takes: the size of its device staging buffer (2^21
+ * complex doubles, 32 MiB on every GPU the library runs on, held until the process ends). */
+#define DMRI_NOALLOC_MAX_DOF 2097152
+
+/* dmri_get_solution without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same result as dmri_get_solution, bit for bit; the rescaled state in DOF order
+ * is staged in a file-scope device buffer instead of a temporary allocation, then copied to u.
+ * ndof > DMRI_NOALLOC_MAX_DOF: DMRI_ERR_LIMIT.  Every call on one GPU shares that buffer, which
+ * keeps the last call's contents; overlapping calls on one GPU would race on it, so the library
+ * holds a per-device lock for the length of a call and they run one after another. */
+int dmri_get_solution_noalloc(dmri_ctx *ctx, int32_t sig, double *u);
+
 #ifdef __cplusplus
 }
 #endif
