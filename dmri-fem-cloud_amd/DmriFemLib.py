@@ -42,7 +42,8 @@ import dmri_mesh
 import dmri_native
 
 __all__ = ["sp", "Point", "Mesh", "BoxMesh", "MRI_parameters", "MRI_simulation", "MyDomain", "KrylovSolver",
-           "PETScLUSolver", "PostProcessing", "convert_g2q", "convert_q2g", "CreatePhaseFunc", "MPI"]
+           "PETScLUSolver", "PostProcessing", "convert_g2q", "convert_q2g", "CreatePhaseFunc", "MPI",
+           "InterfaceKappa"]
 
 G_RATIO = 2.675e8
 
@@ -142,6 +143,71 @@ def CreatePhaseFunc(mymesh, evengroup, oddgroup, partition_marker):
     pm = np.asarray(partition_marker)
     phase = (pm % 2).astype(np.int32)
     return phase, sorted(set(int(v) for v in pm))
+
+
+class InterfaceKappa:
+    """A permeability per interface facet, in place of the CR function of the reference's variable-
+    permeability example (MultilayeredDiskVariablePermeability.ipynb, cell 4: facet_f[facet] =
+    kappa_tensor[marker[cell 0]][marker[cell 1]] over the facets whose two cells carry different
+    partition markers, copied into a CR Function's facet dofs and set as mydomain.kappa).
+
+    InterfaceKappa(kappa_tensor, partition_marker): that rule, with the two cells of every interface
+    facet found from the mesh.  mydomain.kappa may also be a callable of the (n, 3) facet midpoints
+    (a CR interpolant's dofs are its values there) or an array with one value per interface facet in
+    dmri_export_facets order.  Each facet's value is taken as constant over the facet, which is the
+    CR function's restriction when no cell beside it has a second interface facet of another value.
+    """
+
+    def __init__(self, kappa_tensor, partition_marker):
+        self.kappa_tensor = np.asarray(kappa_tensor, dtype=np.float64)
+        self.partition_marker = np.asarray(partition_marker).ravel()
+
+
+def facet_cells(tets, fv):
+    """The two cells of each facet fv[i] (vertex triples) of the tetrahedral mesh tets: (n, 2) cell
+    indices, lower index first; -1 for a facet on the boundary; ValueError for a facet not in the mesh."""
+    tets = np.asarray(tets, dtype=np.int64)
+    nc = tets.shape[0]
+    faces = np.concatenate([tets[:, [1, 2, 3]], tets[:, [0, 2, 3]], tets[:, [0, 1, 3]], tets[:, [0, 1, 2]]])
+    owner = np.tile(np.arange(nc), 4)
+    faces.sort(axis=1)
+    nv = int(tets.max()) + 1
+    key = (faces[:, 0] * nv + faces[:, 1]) * nv + faces[:, 2]
+    order = np.lexsort((owner, key))
+    key, owner = key[order], owner[order]
+    q = np.sort(np.asarray(fv, dtype=np.int64), axis=1)
+    qk = (q[:, 0] * nv + q[:, 1]) * nv + q[:, 2]
+    lo = np.searchsorted(key, qk, side="left")
+    hi = np.searchsorted(key, qk, side="right")
+    if np.any(hi == lo):
+        raise ValueError("a facet is not a face of the mesh")
+    out = np.full((q.shape[0], 2), -1, dtype=np.int64)
+    out[:, 0] = owner[lo]
+    two = hi - lo >= 2
+    out[two, 1] = owner[lo[two] + 1]
+    return out
+
+
+def interface_kappa_values(kappa, xyz, tets, fv):
+    """mydomain.kappa (InterfaceKappa, callable or per-facet array) as one value per interface facet
+    fv (dmri_export_facets order)."""
+    fv = np.asarray(fv)
+    if isinstance(kappa, InterfaceKappa):
+        cells = facet_cells(tets, fv)
+        if np.any(cells[:, 1] < 0):
+            raise ValueError("an interface facet with a single cell")
+        m = kappa.partition_marker
+        if m.shape[0] != np.asarray(tets).shape[0]:
+            raise ValueError("InterfaceKappa needs one partition marker per cell")
+        return kappa.kappa_tensor[m[cells[:, 0]], m[cells[:, 1]]].astype(np.float64)
+    if callable(kappa):
+        vals = np.asarray(kappa(np.asarray(xyz)[fv].mean(axis=1)), dtype=np.float64).ravel()
+    else:
+        vals = np.asarray(kappa, dtype=np.float64).ravel()
+    if vals.shape[0] != fv.shape[0]:
+        raise ValueError("mydomain.kappa needs one value per interface facet (%d), got %d"
+                         % (fv.shape[0], vals.shape[0]))
+    return vals
 
 
 class MRI_parameters:
@@ -298,18 +364,25 @@ class MRI_simulation:
         invT2_cell = None if invT2 is not None else 1.0 / np.asarray(mri_para.T2, dtype=np.float64)
         ctx = dmri_native.Context(self.device)
         ctx.set_mesh(mesh.xyz, mesh.tets, phase)
+        kappa_field = not np.isscalar(mydomain.kappa)
+        coeffs = dict(invT2=invT2 if invT2 is not None else 0.0, invT2_cell=invT2_cell,
+                      kappa=0.0 if kappa_field else float(mydomain.kappa))
         if np.isscalar(D):
-            ctx.set_coeffs(D=float(D), invT2=invT2 if invT2 is not None else 0.0, kappa=float(mydomain.kappa),
-                           invT2_cell=invT2_cell)
+            coeffs.update(D=float(D))
         else:
-            ctx.set_coeffs(D=0.0, D_cell=np.asarray(D), invT2=invT2 if invT2 is not None else 0.0,
-                           kappa=float(mydomain.kappa), invT2_cell=invT2_cell)
+            coeffs.update(D=0.0, D_cell=np.asarray(D))
+        ctx.set_coeffs(**coeffs)
         periodic = sum(mydomain.PeriodicDir) > 0
         if periodic and mydomain.IsDomainPeriodic:
             ctx.set_periodic_strong(mydomain.PeriodicDir)
         else:
             ctx.set_periodic(mydomain.PeriodicDir if periodic else (0, 0, 0))
         ctx.assemble()
+        if kappa_field:
+            # the interface facets are known after the first assembly; the second takes their values
+            kf = interface_kappa_values(mydomain.kappa, mesh.xyz, mesh.tets, ctx.export_facets())
+            ctx.set_coeffs(kappa_facet=kf, **coeffs)
+            ctx.assemble()
         fA, fB, FB, FA = self._sequence(mri_para, integral=periodic)
         p = linsolver.parameters
         start = time.time()

@@ -153,3 +153,58 @@ def test_scalar_gradients_at_zero_coefficient(which):
     print(f"  adjoint {got:.10e} fd {fd:.10e}", flush=True)
     assert np.isfinite(got) and got != 0.0
     assert abs(got - fd) <= 1e-4 * max(abs(got), abs(fd))
+
+
+def test_dmrifemlib_interface_kappa_by_marker():
+    """mydomain.kappa = InterfaceKappa(kappa_tensor, partition_marker) (the CR-field permeability of
+    MultilayeredDiskVariablePermeability.ipynb, cell 4) on three layers with markers 0, 1, 2: the
+    drop-in's signal equals the oracle's with kappa_tensor[marker[a]][marker[b]] on every interface
+    facet (cells found by brute force), and a uniform tensor gives the scalar-kappa signal."""
+    import DmriFemLib as L
+
+    print("[fields] DmriFemLib InterfaceKappa", flush=True)
+    m = dm.cylinder_mesh(6.0, 2.0, 1500, seed=5, interface_radii=(2.5, 4.5))
+    c = m.xyz[m.tets].mean(axis=1)
+    r = np.hypot(c[:, 0], c[:, 1])
+    marker = (r > 2.5).astype(np.int64) + (r > 4.5)
+    phase = (marker % 2).astype(np.int32)
+    K = np.array([[0.0, 1e-4, 0.0], [1e-4, 0.0, 1e-5], [0.0, 1e-5, 0.0]])
+
+    def run(kappa):
+        p = L.MRI_parameters()
+        p.bvalue = 1000
+        p.delta, p.Delta = 20000.0, 20000.0
+        p.set_gradient_dir(m, 1, 0, 0)
+        p.T = p.Delta + p.delta
+        p.fs_sym = L.sp.Piecewise((1., p.s < p.delta), (0., p.s < p.Delta), (-1., p.s < p.T), (0., True))
+        p.Apply()
+        dom = L.MyDomain(m, p)
+        dom.phase = phase
+        dom.IsDomainMultiple = True
+        dom.kappa = kappa
+        dom.Apply()
+        dom.D = 3e-3
+        simu = L.MRI_simulation()
+        simu.k = 1000
+        solver = L.KrylovSolver("bicgstab", "jacobi")
+        solver.parameters["relative_tolerance"] = 1e-13
+        solver.parameters["absolute_tolerance"] = 0.0
+        simu.solve(dom, p, solver)
+        return simu, p
+
+    simu, p = run(L.InterfaceKappa(K, marker))
+    S0 = O.System(m.xyz, m.tets, phase)
+    faces = {}
+    for ci, t in enumerate(m.tets):
+        for k in range(4):
+            faces.setdefault(tuple(sorted(np.delete(t, k))), []).append(ci)
+    kf = np.array([K[marker[faces[tuple(sorted(f))][0]], marker[faces[tuple(sorted(f))][1]]] for f in S0.fv])
+    assert set(np.unique(kf)) == {1e-4, 1e-5}
+    fA, fB = O.gcloud_sequence(O.pgse_piecewise(20000.0, 20000.0), 40000.0, 1000.0)
+    S = O.System(m.xyz, m.tets, phase, D=3e-3, kappa_facet=kf)
+    ref, _, _ = S.solve((1, 0, 0), p.qvalue, 1000.0, fA, fB, rtol=1e-13)
+    print(f"  drop-in {simu.signal_c.real:.14e} oracle {ref:.14e}", flush=True)
+    assert abs(simu.signal_c.real - ref) <= 1e-9 * abs(ref)
+    uni, _ = run(L.InterfaceKappa(np.full((3, 3), 3e-5), marker))
+    sca, _ = run(3e-5)
+    assert abs(uni.signal_c.real - sca.signal_c.real) <= 1e-12 * abs(sca.signal_c.real)

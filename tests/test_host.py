@@ -153,3 +153,49 @@ def test_profile_summary_samples_first_iterations_of_every_step(tmp_path):
     assert len(picked) == 3 * 4
     assert picked.count("spmv_v_kernel<1, false, 0>") == 3 and picked.count("spmv_t_kernel<1, false, 2>") == 3
     assert len(ps.sampled(rows, 2)) == 3 * 4 * 2
+
+
+def _layered_cylinder():
+    """Three concentric layers (partition markers 0, 1, 2 by cell-centroid radius), phase = marker % 2."""
+    m = dm.cylinder_mesh(6.0, 2.0, 700, seed=5, interface_radii=(2.5, 4.5))
+    c = m.xyz[m.tets].mean(axis=1)
+    r = np.hypot(c[:, 0], c[:, 1])
+    marker = (r > 2.5).astype(np.int64) + (r > 4.5)
+    return m, marker
+
+
+def test_facet_cells_and_interface_kappa_against_brute_force():
+    """DmriFemLib.facet_cells / interface_kappa_values: the two cells of every facet between
+    compartments, found by testing every cell, and kappa_tensor[marker[a]][marker[b]] on them (the rule
+    of the reference's variable-permeability example); the midpoint callable and the array forms."""
+    import DmriFemLib as L
+
+    m, marker = _layered_cylinder()
+    phase = marker % 2
+    tets = m.tets.astype(np.int64)
+    faces = {}
+    for ci, t in enumerate(tets):
+        for k in range(4):
+            f = tuple(sorted(np.delete(t, k)))
+            faces.setdefault(f, []).append(ci)
+    iface = [f for f, cs in faces.items() if len(cs) == 2 and phase[cs[0]] != phase[cs[1]]]
+    assert len(iface) > 50
+    rng = np.random.default_rng(0)
+    fv = np.array([rng.permutation(f) for f in iface])  # vertex order inside a facet does not matter
+    cells = L.facet_cells(m.tets, fv)
+    for f, (a, b) in zip(iface, cells):
+        assert [a, b] == sorted(faces[f])
+    K = np.array([[0.0, 1e-4, 0.0], [1e-4, 0.0, 1e-5], [0.0, 1e-5, 0.0]])
+    vals = L.interface_kappa_values(L.InterfaceKappa(K, marker), m.xyz, m.tets, fv)
+    expect = np.array([K[marker[faces[f][0]], marker[faces[f][1]]] for f in iface])
+    np.testing.assert_array_equal(vals, expect)
+    assert set(np.unique(vals)) == {1e-4, 1e-5}
+    mid = m.xyz[fv].mean(axis=1)
+    np.testing.assert_array_equal(L.interface_kappa_values(lambda x: 2e-5 * x[:, 2], m.xyz, m.tets, fv), 2e-5 * mid[:, 2])
+    np.testing.assert_array_equal(L.interface_kappa_values(expect, m.xyz, m.tets, fv), expect)
+    with pytest.raises(ValueError):
+        L.interface_kappa_values(expect[:-1], m.xyz, m.tets, fv)
+    with pytest.raises(ValueError):
+        L.facet_cells(m.tets, np.array([[0, 1, 2 + m.nv - 3]]))
+    boundary = [f for f, cs in faces.items() if len(cs) == 1][:5]
+    assert np.all(L.facet_cells(m.tets, np.array(boundary))[:, 1] == -1)
