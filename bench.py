@@ -405,7 +405,10 @@ def run_stepwise(args, name):
     copy_gbs = ctx.copy_bandwidth(args.copy_bytes, 10) if args.copy_bytes > 0 else None
     tuned = None
     if not args.no_tune and args.lanes == 0 and args.mapping == 0:
-        tuned = ctx.tune(S_group, per_gdir)[0]
+        tuned, times = ctx.tune(S_group, per_gdir)
+        print("  SpMV tuning, us per iteration's four SpMVs (mapping/lanes): " +
+              ", ".join(f"{m}/{l} {t:.0f}" for (m, l), t in sorted(times.items(), key=lambda kv: kv[1])[:8]),
+              file=sys.stderr, flush=True)
     print(f"  mesh nv={info['nv']} ndof={info['ndof']} nnz={info['nnz']}, {nsig} signals in groups of {S_group}, "
           f"SpMV mapping/lanes {tuned}", file=sys.stderr, flush=True)
     tb = time.perf_counter()

@@ -1036,7 +1036,7 @@ struct Stepper {
                 launch_extract_real(c->nnz, w.vals.p, vw, w.areal.p, s);
                 a.areal = w.areal.p;
                 st->precond_used = 2;
-            } else if ((geo.mapping == kMapSL || geo.mapping == kMapStaged) && !strong &&
+            } else if ((geo.mapping == kMapSL || geo.mapping == kMapStaged || geo.mapping == kMapPair) && !strong &&
                        (opts->preconditioner == 2 || imag >= kComplexDiagonal * breal)) {
                 // The gradient term dominates the diagonal of rows far along g: the polynomial then
                 // acts on the operator scaled by its complex diagonal, whose spectrum stays near
@@ -1901,10 +1901,10 @@ void install_tuning(dmri_ctx *c, int S, bool per_gdir, int mapping, int lanes) {
 }
 
 // Time the SpMV kernels of a polynomial-preconditioned BiCGStab iteration (the library's default
-// solve) for every (mapping, lanes per row) candidate that runs it (mappings 1, 2 and 4; the
+// solve) for every (mapping, lanes per row) candidate that runs it (mappings 1, 2, 4 and 6; the
 // signals-in-registers and tile-compressed mappings have no preconditioner kernels) for S signals on
 // the assembled pattern, and remember the fastest for later solves of this context.
-// us[5*6]: [mapping-1][log2 lanes] microseconds per iteration's four SpMVs, -1 where not valid.
+// us[6*6]: [mapping-1][log2 lanes] microseconds per iteration's four SpMVs, -1 where not valid.
 void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_mapping, double *us) {
     require(c->assembled, DMRI_ERR_STATE, "dmri_tune: call dmri_assemble first");
     require(S >= 1 && S <= kMaxSignals, DMRI_ERR_ARG, "dmri_tune: nsig must lie in [1, 64]");
@@ -1923,13 +1923,17 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
     double def_us = -1.0;
     // 20 replays per candidate, then the default is kept unless a candidate beats it by 3 %
     // (timing noise must not flip the choice between runs)
-    for (int m = kMapSL; m <= kMapTile; ++m)
+    int pair_sp = 1;
+    while (2 * pair_sp < S) pair_sp *= 2;
+    for (int m = kMapSL; m <= kMapPair; ++m)
         for (int i = 0; i < 6; ++i) {
             const int G = cand[i];
             double &slot = us[(m - 1) * 6 + i];
             slot = -1.0;
             if (m == kMapSR || m == kMapTile) continue;
-            if (m == kMapWave ? G != 1 : G * S > 64) continue;
+            if (m == kMapWave && G != 1) continue;
+            if (m == kMapPair && (S < 2 || G > 4 || G * pair_sp > 64)) continue;
+            if (m != kMapWave && m != kMapPair && G * S > 64) continue;
             SolveGeom geo = make_geom(n, G, S, m);
             fit_spmv_grid(geo, per_gdir);
             const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
@@ -2485,7 +2489,7 @@ int dmri_time_spmv(dmri_ctx *c, int32_t nsig, int32_t per_gdir, int32_t reps, do
                 mapping = t.second / 64;
                 lanes = t.second % 64;
             }
-        // the preconditioned solve runs mappings 1, 2 and 4 (others fall back to 2)
+        // the preconditioned solve runs mappings 1, 2, 4 and 6 (others fall back to 2)
         if (mapping == kMapSR || mapping == kMapTile) {
             mapping = kMapStaged;
             while (lanes > 1 && lanes * nsig > 64) lanes >>= 1;

@@ -86,6 +86,7 @@ constexpr int kMapStaged = 2;
 constexpr int kMapSR = 3;
 constexpr int kMapWave = 4;  // one wave per row, lane = signal; row data through the scalar cache
 constexpr int kMapTile = 5;  // tiles of rpb rows: the tile's distinct columns staged once in LDS
+constexpr int kMapPair = 6;  // mapping 2 with two signals per lane (G in 1, 2, 4; at least 2 signals)
 // SpMV kernel operations: v = A p / t = A s (plain BiCGStab), the degree-1 polynomial
 // preconditioner z -> 2z - A z applied to p or s, and t = A sh (preconditioned BiCGStab)
 constexpr int kOpV = 0, kOpT = 0, kOpPre = 1, kOpTP = 2;
@@ -116,6 +117,7 @@ struct SolveGeom {
     int grid = 8;            // blocks of every SpMV-type kernel (a multiple of 8)
     int tile_rows = 0;       // kMapTile: rows of a compressed tile (= rpb)
     int tile_ucap = 0;       // kMapTile: distinct columns staged per tile (LDS), set from the build
+    int pair_sp = 0;         // kMapPair: signal pairs per row, ceil(S / 2) rounded up to a power of two
     int vgrid = 1;           // blocks of the vector kernels
     int pgrid = 1;           // blocks of the projection Gram kernel
 };

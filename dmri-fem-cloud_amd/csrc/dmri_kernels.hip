@@ -31,6 +31,22 @@ __device__ __forceinline__ double2 c_axpj(double2 a, double s, double2 j) {
     return make_double2(a.x - s * j.y, a.y + s * j.x);
 }
 
+// Loads and stores with the non-temporal hint, for the streaming vector kernels (s-pass and the
+// BiCGStab update), where each element is touched once.  In the SpMV kernels the same hint on the
+// matrix entries, own-row values or results measured slower on the 2M-vertex neuron workload (up to
+// +15 % per launch for the stores), so they keep plain accesses.
+typedef double dmri_d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_once(const double2 *p) {
+    const dmri_d2v v = __builtin_nontemporal_load(reinterpret_cast<const dmri_d2v *>(p));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_once(double2 *p, double2 v) {
+    dmri_d2v w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<dmri_d2v *>(p));
+}
+
 
 // --------------------------------------------------------------------------------------
 // deterministic reductions without atomics or device-scope fences
@@ -1405,6 +1421,67 @@ __device__ __forceinline__ void real_row_accumulate(const int *col, const double
     }
 }
 
+// Copy rows t0 .. t0+nr-1 of the pattern into LDS (the caller holds a barrier before and after):
+// their ne entries' column indices (scol) and values (svals: VW per entry, or RO: one per entry
+// from a.areal), and the row offsets relative to the tile's first entry (srp).  Loads go through
+// registers in batches of kStageBatch independent loads per thread, so a thread waits on memory
+// once per batch rather than once per element.
+template <bool RO, int VW>
+__device__ __forceinline__ void stage_tile(const SolveArgs &a, int64_t t0, int nr, int E0, int ne, double *svals,
+                                           int *scol, int *srp) {
+    const int nt = blockDim.x;
+    const int rp = threadIdx.x <= nr ? a.rowptr[t0 + threadIdx.x] : 0;
+    if (RO) {
+        const double *gv = a.areal + E0;
+        for (int base = 0; base < ne; base += kStageBatch * nt) {
+            int cv[kStageBatch];
+            double vv[kStageBatch];
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) {
+                const int i = base + u * nt + threadIdx.x;
+                const bool ok = i < ne;
+                cv[u] = ok ? a.col[E0 + i] : 0;
+                vv[u] = ok ? gv[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) {
+                const int i = base + u * nt + threadIdx.x;
+                if (i < ne) {
+                    scol[i] = cv[u];
+                    svals[i] = vv[u];
+                }
+            }
+        }
+    } else {
+        const double2 *gv = reinterpret_cast<const double2 *>(a.vals + (int64_t)E0 * VW);
+        double2 *lv = reinterpret_cast<double2 *>(svals);
+        for (int base = 0; base < ne; base += kStageBatch * nt) {
+            int cv[kStageBatch];
+            double2 vv[kStageBatch][VW / 2 > 0 ? VW / 2 : 1];
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) {
+                const int i = base + u * nt + threadIdx.x;
+                const bool ok = i < ne;
+                cv[u] = ok ? a.col[E0 + i] : 0;
+#pragma unroll
+                for (int h = 0; h < VW / 2; ++h)
+                    vv[u][h] = ok ? gv[(int64_t)i * (VW / 2) + h] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) {
+                const int i = base + u * nt + threadIdx.x;
+                if (i < ne) {
+                    scol[i] = cv[u];
+#pragma unroll
+                    for (int h = 0; h < VW / 2; ++h) lv[i * (VW / 2) + h] = vv[u][h];
+                }
+            }
+        }
+    }
+    if ((int)threadIdx.x <= nr) srp[threadIdx.x] = rp - E0;
+    for (int i = threadIdx.x + nt; i <= nr; i += nt) srp[i] = a.rowptr[t0 + i] - E0;
+}
+
 template <int G, bool PG, bool RO = false, class Gather, class Emit>
 __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR, const Lane &L, bool act, Gather X,
                                             Emit emit) {
@@ -1437,61 +1514,7 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
         const int ne = a.rowptr[t0 + nr] - E0;
         const bool fits = ne <= ECAP;
         __syncthreads();
-        if (fits) {
-            // Copy through registers in batches of kStageBatch independent loads per thread, so
-            // a thread waits on memory once per batch rather than once per element.
-            const int nt = blockDim.x;
-            const int rp = threadIdx.x <= nr ? a.rowptr[t0 + threadIdx.x] : 0;
-            if (RO) {
-                const double *gv = a.areal + E0;
-                for (int base = 0; base < ne; base += kStageBatch * nt) {
-                    int cv[kStageBatch];
-                    double vv[kStageBatch];
-#pragma unroll
-                    for (int u = 0; u < kStageBatch; ++u) {
-                        const int i = base + u * nt + threadIdx.x;
-                        const bool ok = i < ne;
-                        cv[u] = ok ? a.col[E0 + i] : 0;
-                        vv[u] = ok ? gv[i] : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < kStageBatch; ++u) {
-                        const int i = base + u * nt + threadIdx.x;
-                        if (i < ne) {
-                            scol[i] = cv[u];
-                            svals[i] = vv[u];
-                        }
-                    }
-                }
-            } else {
-                const double2 *gv = reinterpret_cast<const double2 *>(a.vals + (int64_t)E0 * VW);
-                double2 *lv = reinterpret_cast<double2 *>(svals);
-                for (int base = 0; base < ne; base += kStageBatch * nt) {
-                    int cv[kStageBatch];
-                    double2 vv[kStageBatch][VW / 2 > 0 ? VW / 2 : 1];
-#pragma unroll
-                    for (int u = 0; u < kStageBatch; ++u) {
-                        const int i = base + u * nt + threadIdx.x;
-                        const bool ok = i < ne;
-                        cv[u] = ok ? a.col[E0 + i] : 0;
-#pragma unroll
-                        for (int h = 0; h < VW / 2; ++h)
-                            vv[u][h] = ok ? gv[(int64_t)i * (VW / 2) + h] : make_double2(0.0, 0.0);
-                    }
-#pragma unroll
-                    for (int u = 0; u < kStageBatch; ++u) {
-                        const int i = base + u * nt + threadIdx.x;
-                        if (i < ne) {
-                            scol[i] = cv[u];
-#pragma unroll
-                            for (int h = 0; h < VW / 2; ++h) lv[i * (VW / 2) + h] = vv[u][h];
-                        }
-                    }
-                }
-            }
-            if ((int)threadIdx.x <= nr) srp[threadIdx.x] = rp - E0;
-            for (int i = threadIdx.x + nt; i <= nr; i += nt) srp[i] = a.rowptr[t0 + i] - E0;
-        }
+        if (fits) stage_tile<RO, VW>(a, t0, nr, E0, ne, svals, scol, srp);
         __syncthreads();
         if (!act) continue;
         for (int lr = L.lrow; lr < nr; lr += rpb) {
@@ -1996,6 +2019,279 @@ __global__ void __launch_bounds__(kThreads) spmv_t_kernel(SolveArgs a, int S, in
     else
         staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit);
     if (!pre) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
+}
+
+// ---- two signals per lane (kMapPair): LDS-staged tiles as in staged_spmv, but a lane serves
+// signals 2k and 2k+1 of its row, so every staged column index and value it reads, and every
+// address it forms, serves two signals, and it keeps two gathers in flight per entry.  A row takes
+// LP = G x SP lanes (SP: signal pairs rounded up to a power of two; G lanes split its entries), a
+// wave holds 64 / LP whole rows.  Each entry's imaginary coefficient is folded in before the
+// multiply (a + i c_s.j), so a signal needs one complex accumulator.
+struct PairLane {
+    int lrow, sp, g;
+};
+template <int G>
+__device__ __forceinline__ PairLane pair_lane(int SP) {
+    const int LP = G * SP;
+    const int rem = threadIdx.x % LP;
+    return PairLane{(int)threadIdx.x / LP, rem / G, rem % G};
+}
+
+// Row entries e0 + g, e0 + g + G, ... < e1 (column indices col[], values vals[] with VW per entry,
+// or RO: one real value per entry) against the two gathered signals X(c, x0, x1): four entries per
+// step, the last step masked, so a lane waits on memory ceil(entries / 4G) times per row.
+// cf[k]: the imaginary coefficients of the lane's signal k (per direction when PG).
+template <int G, bool PG, bool RO, class Gather>
+__device__ __forceinline__ void pair_row(const int *col, const double *vals, int e0, int e1, int g,
+                                         const double (&cf)[2][3], Gather X, double2 (&y)[2]) {
+    constexpr int VW = RO ? 1 : (PG ? 4 : 2);
+    y[0] = make_double2(0.0, 0.0);
+    y[1] = make_double2(0.0, 0.0);
+    for (int e = e0 + g; e < e1; e += 4 * G) {
+        int c[4];
+        double2 x0[4], x1[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[k] = e + k * G < e1 ? col[e + k * G] : -1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x0[k] = make_double2(0.0, 0.0);
+            x1[k] = make_double2(0.0, 0.0);
+            if (c[k] >= 0) X(c[k], x0[k], x1[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (c[k] < 0) continue;
+            const int64_t ek = (int64_t)(e + k * G) * VW;
+            double av, j0, j1;
+            if (RO) {
+                av = vals[ek];
+                j0 = j1 = 0.0;
+            } else if (PG) {
+                const double4 v = *reinterpret_cast<const double4 *>(vals + ek);
+                av = v.x;
+                j0 = cf[0][0] * v.y + cf[0][1] * v.z + cf[0][2] * v.w;
+                j1 = cf[1][0] * v.y + cf[1][1] * v.z + cf[1][2] * v.w;
+            } else {
+                const double2 v = *reinterpret_cast<const double2 *>(vals + ek);
+                av = v.x;
+                j0 = cf[0][0] * v.y;
+                j1 = cf[1][0] * v.y;
+            }
+            // (a + i j) x
+            y[0].x += av * x0[k].x - j0 * x0[k].y;
+            y[0].y += av * x0[k].y + j0 * x0[k].x;
+            y[1].x += av * x1[k].x - j1 * x1[k].y;
+            y[1].y += av * x1[k].y + j1 * x1[k].x;
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        y[0].x += __shfl_xor(y[0].x, off, 64);
+        y[0].y += __shfl_xor(y[0].y, off, 64);
+        y[1].x += __shfl_xor(y[1].x, off, 64);
+        y[1].y += __shfl_xor(y[1].y, off, 64);
+    }
+}
+
+// The tiles of the block (XCD-aware walk of xcd_tiles) staged in LDS, then every row of a tile for
+// the lane's two signals; emit(row, y) acts on lanes g == 0 and receives both signals' sums.  A tile
+// whose entries exceed the LDS budget is read straight from global memory.
+template <int G, bool PG, bool RO, class Gather, class Emit>
+__device__ __forceinline__ void pair_spmv(const SolveArgs &a, int SP, int TR, const PairLane &L, bool act,
+                                          const double (&cf)[2][3], Gather X, Emit emit) {
+    constexpr int VW = RO ? 1 : (PG ? 4 : 2);
+    const int rpb = (int)blockDim.x / (G * SP);
+    const int ECAP = TR * kStageNnzPerRow;
+    extern __shared__ double4 dmri_stage_lds[];
+    double *svals = reinterpret_cast<double *>(dmri_stage_lds);
+    int *scol = reinterpret_cast<int *>(svals + (size_t)ECAP * VW);
+    int *srp = scol + ECAP;
+    const double *gvals = RO ? a.areal : a.vals;
+    const TileWalk W = xcd_tiles(a.n, TR);
+    for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
+        const int nr = (int)((W.end - t0) < TR ? (W.end - t0) : TR);
+        const int E0 = a.rowptr[t0];
+        const int ne = a.rowptr[t0 + nr] - E0;
+        const bool fits = ne <= ECAP;
+        __syncthreads();
+        if (fits) stage_tile<RO, VW>(a, t0, nr, E0, ne, svals, scol, srp);
+        __syncthreads();
+        if (!act) continue;
+        for (int lr = L.lrow; lr < nr; lr += rpb) {
+            double2 y[2];
+            if (fits)
+                pair_row<G, PG, RO>(scol, svals, srp[lr], srp[lr + 1], L.g, cf, X, y);
+            else
+                pair_row<G, PG, RO>(a.col, gvals, a.rowptr[t0 + lr], a.rowptr[t0 + lr + 1], L.g, cf, X, y);
+            if (L.g == 0) emit(t0 + lr, y);
+        }
+    }
+}
+
+// Per-signal block partials of a pair kernel (rk[k]: lane's sums for its signal 2 sp + k, zero on
+// lanes g != 0): the rows of a wave sit LP lanes apart and are added with shuffles, then lane sp
+// of every wave writes LDS and thread s < S adds the waves in order.
+__device__ __forceinline__ void pair_partials(double (&r0)[kNV], double (&r1)[kNV], int LP, int sp, int g, int S,
+                                              double *parts) {
+#pragma unroll
+    for (int k = 0; k < kNV; ++k)
+        for (int off = LP; off < 64; off <<= 1) {
+            r0[k] += __shfl_xor(r0[k], off, 64);
+            r1[k] += __shfl_xor(r1[k], off, 64);
+        }
+    double(&buf)[kNV][kThreads] = reduce_scratch();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = (int)blockDim.x >> 6;
+    if (lane < LP && g == 0) {
+        const int s0 = 2 * sp;
+#pragma unroll
+        for (int k = 0; k < kNV; ++k) {
+            if (s0 < S) buf[k][wave * S + s0] = r0[k];
+            if (s0 + 1 < S) buf[k][wave * S + s0 + 1] = r1[k];
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < S) {
+        double acc[kNV] = {0.0, 0.0, 0.0, 0.0};
+        for (int w = 0; w < nw; ++w) {
+#pragma unroll
+            for (int k = 0; k < kNV; ++k) acc[k] += buf[k][w * S + threadIdx.x];
+        }
+#pragma unroll
+        for (int k = 0; k < kNV; ++k) parts[((size_t)blockIdx.x * S + threadIdx.x) * kNV + k] = acc[k];
+    }
+}
+
+// The imaginary coefficients of the lane's two signals (zero past S)
+template <bool PG>
+__device__ __forceinline__ void pair_coefs(const double *coef, int s0, int S, double (&cf)[2][3]) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cf[k][d] = s0 + k < S && (PG || d == 0) ? coef[3 * (s0 + k) + d] : 0.0;
+}
+
+// spmv_v_kernel's operations (kOpV, kOpPre, kOpPreCD) with two signals per lane
+template <int G, bool PG, int OP>
+__global__ void __launch_bounds__(kThreads) spmv_v_pair_kernel(SolveArgs a, int S, int SP, int TR, int it) {
+    if (*a.all_done) return;
+    const PairLane L = pair_lane<G>(SP);
+    const int s0 = 2 * L.sp, s1 = s0 + 1;
+    const bool on0 = s0 < S && !a.st[s0].done;
+    const bool on1 = s1 < S && !a.st[s1].done;
+    constexpr bool cd = OP == kOpPreCD;
+    constexpr bool pre = OP == kOpPre || OP == kOpPreCD;
+    double cf[2][3];
+    pair_coefs<PG>(a.coefA, s0, S, cf);
+    double r0[kNV] = {0.0, 0.0, 0.0, 0.0}, r1[kNV] = {0.0, 0.0, 0.0, 0.0};
+    const double2 *p = a.p;
+    const int64_t S_ = S;
+    auto gather = [&](int c, double2 &x0, double2 &x1) {
+        const int64_t j = (int64_t)c * S_ + s0;
+        if (on0) x0 = p[j];
+        if (on1) x1 = p[j + 1];
+        if constexpr (cd) {
+            if (on0) x0 = c_mul(x0, cdiag_inv<PG>(a.pdiag, a.coefA, c, s0));
+            if (on1) x1 = c_mul(x1, cdiag_inv<PG>(a.pdiag, a.coefA, c, s1));
+        }
+    };
+    auto emit = [&](int64_t row, const double2 (&y)[2]) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (!(k == 0 ? on0 : on1)) continue;
+            const int s = s0 + k;
+            const int64_t idx = row * S_ + s;
+            if (pre) {
+                const double2 z = p[idx];
+                const double2 w = make_double2(a.pc * (2.0 * z.x - a.pc * y[k].x), a.pc * (2.0 * z.y - a.pc * y[k].y));
+                if constexpr (cd)
+                    a.ph[idx] = c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s));
+                else
+                    a.ph[idx] = w;
+                continue;
+            }
+            a.v[idx] = y[k];
+            const double2 h = a.rh[idx];
+            double(&r)[kNV] = k == 0 ? r0 : r1;
+            r[0] += h.x * y[k].x + h.y * y[k].y;
+            r[1] += h.x * y[k].y - h.y * y[k].x;
+        }
+    };
+    const bool act = on0 || on1;
+    if (OP == kOpPre && a.areal)
+        pair_spmv<G, PG, true>(a, SP, TR, L, act, cf, gather, emit);
+    else
+        pair_spmv<G, PG, false>(a, SP, TR, L, act, cf, gather, emit);
+    if (OP == kOpV) pair_partials(r0, r1, G * SP, L.sp, L.g, S, a.partV);
+}
+
+// spmv_t_kernel's operations (kOpT, kOpTP, kOpPre, kOpPreCD) with two signals per lane
+template <int G, bool PG, int OP>
+__global__ void __launch_bounds__(kThreads) spmv_t_pair_kernel(SolveArgs a, int S, int SP, int TR, int nparts,
+                                                               int it) {
+    if (*a.all_done) return;
+    __shared__ double2 salpha[kMaxSignals];
+    __shared__ int sact[kMaxSignals];
+    spmv_t_head(a, S, nparts, it, salpha, sact);
+    const PairLane L = pair_lane<G>(SP);
+    const int s0 = 2 * L.sp, s1 = s0 + 1;
+    const bool on0 = s0 < S && sact[s0];
+    const bool on1 = s1 < S && sact[s1];
+    const double2 al0 = on0 ? salpha[s0] : make_double2(0.0, 0.0);
+    const double2 al1 = on1 ? salpha[s1] : make_double2(0.0, 0.0);
+    constexpr bool cd = OP == kOpPreCD;
+    constexpr bool pre = OP == kOpPre || OP == kOpPreCD;
+    double cf[2][3];
+    pair_coefs<PG>(a.coefA, s0, S, cf);
+    double r0[kNV] = {0.0, 0.0, 0.0, 0.0}, r1[kNV] = {0.0, 0.0, 0.0, 0.0};
+    const double2 *r = a.r, *v = a.v, *sh = a.sh, *sm = a.s;
+    const int64_t S_ = S;
+    // s at entry j of signal k: materialised by the s-pass, or formed from r and v
+    auto sval = [&](int64_t j, const double2 &al) { return sm ? sm[j] : c_sub(r[j], c_mul(al, v[j])); };
+    auto gather = [&](int c, double2 &x0, double2 &x1) {
+        const int64_t j = (int64_t)c * S_ + s0;
+        if (OP == kOpTP) {
+            if (on0) x0 = sh[j];
+            if (on1) x1 = sh[j + 1];
+            return;
+        }
+        if (on0) x0 = sval(j, al0);
+        if (on1) x1 = sval(j + 1, al1);
+        if constexpr (cd) {
+            if (on0) x0 = c_mul(x0, cdiag_inv<PG>(a.pdiag, a.coefA, c, s0));
+            if (on1) x1 = c_mul(x1, cdiag_inv<PG>(a.pdiag, a.coefA, c, s1));
+        }
+    };
+    auto emit = [&](int64_t row, const double2 (&y)[2]) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (!(k == 0 ? on0 : on1)) continue;
+            const int s = s0 + k;
+            const int64_t idx = row * S_ + s;
+            const double2 sv = sval(idx, k == 0 ? al0 : al1);
+            if (pre) {
+                const double2 w =
+                    make_double2(a.pc * (2.0 * sv.x - a.pc * y[k].x), a.pc * (2.0 * sv.y - a.pc * y[k].y));
+                if constexpr (cd)
+                    a.sh[idx] = c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s));
+                else
+                    a.sh[idx] = w;
+                continue;
+            }
+            a.t[idx] = y[k];
+            double(&rr)[kNV] = k == 0 ? r0 : r1;
+            rr[0] += y[k].x * sv.x + y[k].y * sv.y;
+            rr[1] += y[k].x * sv.y - y[k].y * sv.x;
+            rr[2] += y[k].x * y[k].x + y[k].y * y[k].y;
+            rr[3] += sv.x * sv.x + sv.y * sv.y;
+        }
+    };
+    const bool act = on0 || on1;
+    if (OP == kOpPre && a.areal)
+        pair_spmv<G, PG, true>(a, SP, TR, L, act, cf, gather, emit);
+    else
+        pair_spmv<G, PG, false>(a, SP, TR, L, act, cf, gather, emit);
+    if (!pre) pair_partials(r0, r1, G * SP, L.sp, L.g, S, a.partT);
 }
 
 // ---- tile-compressed rows: a tile is the TR rows of one pass of the (row, signal) lane groups.
@@ -2660,7 +2956,7 @@ __global__ void __launch_bounds__(kThreads) spass_kernel(SolveArgs a, int S, int
     if ((int)threadIdx.x >= vrpb * S || !sact[s]) return;
     const double2 alpha = salpha[s];
     for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total; i += (int64_t)gridDim.x * vrpb * S)
-        a.s[i] = c_sub(a.r[i], c_mul(alpha, a.v[i]));
+        st_once(a.s + i, c_sub(ld_once(a.r + i), c_mul(alpha, ld_once(a.v + i))));
 }
 
 __global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int S, int vrpb, int nparts, int it) {
@@ -2695,15 +2991,16 @@ __global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int 
     if (act) {
         for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total;
              i += (int64_t)gridDim.x * vrpb * S) {
-            const double2 sv = a.s ? a.s[i] : c_sub(a.r[i], c_mul(alpha, a.v[i]));
+            const double2 sv = a.s ? ld_once(a.s + i) : c_sub(ld_once(a.r + i), c_mul(alpha, ld_once(a.v + i)));
             // right preconditioning: the iterate moves along the preconditioned directions
+            const double2 xi = ld_once(a.x + i);
             if (a.ph)
-                a.x[i] = c_add(a.x[i], c_add(c_mul(alpha, a.ph[i]), c_mul(omega, a.sh[i])));
+                st_once(a.x + i, c_add(xi, c_add(c_mul(alpha, ld_once(a.ph + i)), c_mul(omega, ld_once(a.sh + i)))));
             else
-                a.x[i] = c_add(a.x[i], c_add(c_mul(alpha, a.p[i]), c_mul(omega, sv)));
-            const double2 rn = c_sub(sv, c_mul(omega, a.t[i]));
-            a.r[i] = rn;
-            const double2 h = a.rh[i];
+                st_once(a.x + i, c_add(xi, c_add(c_mul(alpha, ld_once(a.p + i)), c_mul(omega, sv))));
+            const double2 rn = c_sub(sv, c_mul(omega, ld_once(a.t + i)));
+            st_once(a.r + i, rn);
+            const double2 h = ld_once(a.rh + i);
             red[0] += h.x * rn.x + h.y * rn.y;
             red[1] += h.x * rn.y - h.y * rn.x;
             red[2] += rn.x * rn.x + rn.y * rn.y;
@@ -3354,18 +3651,25 @@ void launch_diag_and_weights(int64_t n, const int *rowptr, const int *col, const
 SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
     if (S < 1 || S > kMaxSignals) throw Error(-1, "signals per launch must lie in [1, 64]");
     if (G < 1 || G > 32 || (G & (G - 1))) throw Error(-1, "lanes per row must be a power of two in [1, 32]");
-    if (mapping != kMapSL && mapping != kMapStaged && mapping != kMapSR && mapping != kMapWave && mapping != kMapTile)
+    if (mapping != kMapSL && mapping != kMapStaged && mapping != kMapSR && mapping != kMapWave && mapping != kMapTile &&
+        mapping != kMapPair)
         throw Error(-1, "unknown SpMV mapping");
+    int pair_sp = 1;
+    while (2 * pair_sp < S) pair_sp *= 2;
+    if (mapping == kMapPair && (S < 2 || G > 4 || G * pair_sp > 64))
+        throw Error(-1, "the two-signals-per-lane mapping needs at least 2 signals, lanes 1, 2 or 4 and "
+                        "lanes x signal pairs <= 64");
     if (mapping == kMapWave && G != 1) throw Error(-1, "the wave-per-row mapping takes one lane per signal (lanes 1)");
     if (mapping == kMapSR && S > kMaxSRSignals)
         throw Error(-1, "signals-in-registers mapping needs at most 8 signals per launch group");
-    if (mapping != kMapSR && G * S > 64) throw Error(-1, "lanes per row times signals must not exceed 64");
+    if (mapping != kMapSR && mapping != kMapPair && G * S > 64)
+        throw Error(-1, "lanes per row times signals must not exceed 64");
     SolveGeom g;
     g.G = G;
     g.S = S;
     g.mapping = mapping;
     // the residual kernel always spreads signals over lanes
-    g.G_rhs = mapping == kMapSR ? std::max(1, std::min(G, 64 / S)) : G;
+    g.G_rhs = mapping == kMapSR || mapping == kMapPair ? std::max(1, std::min(G, 64 / S)) : G;
     while (g.G_rhs & (g.G_rhs - 1)) g.G_rhs &= g.G_rhs - 1;
     g.rhs_rpb = std::max(1, kThreads / (g.G_rhs * S));
     g.rhs_threads = g.rhs_rpb * g.G_rhs * S;
@@ -3376,6 +3680,11 @@ SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
         g.spmv_threads = kThreads;
     }
     if (mapping == kMapTile) g.tile_rows = g.rpb;
+    if (mapping == kMapPair) {  // G x pair_sp lanes per row
+        g.pair_sp = pair_sp;
+        g.rpb = kThreads / (G * pair_sp);
+        g.spmv_threads = kThreads;
+    }
     g.sr_rpb = std::max(1, kThreads / G);
     g.sr_threads = g.sr_rpb * G;
     g.vec_rpb = std::max(1, kThreads / S);
@@ -3739,9 +4048,38 @@ void launch_unscale(int64_t n, int S, int sig, const double *sc, const double2 *
         }                                                                                   \
     } while (0)
 
+#define DMRI_DISPATCH_GP(G_, PG_, CALL)                                                     \
+    do {                                                                                    \
+        switch (G_) {                                                                       \
+            case 1: { constexpr int GG = 1; if (PG_) { constexpr bool PP = true; CALL; }   \
+                      else { constexpr bool PP = false; CALL; } } break;                   \
+            case 2: { constexpr int GG = 2; if (PG_) { constexpr bool PP = true; CALL; }   \
+                      else { constexpr bool PP = false; CALL; } } break;                   \
+            case 4: { constexpr int GG = 4; if (PG_) { constexpr bool PP = true; CALL; }   \
+                      else { constexpr bool PP = false; CALL; } } break;                   \
+            default: throw Error(-1, "unsupported lanes per row for two signals per lane"); \
+        }                                                                                   \
+    } while (0)
+
 void fit_spmv_grid(SolveGeom &g, bool per_gdir) {
     const void *fn = nullptr;
     size_t lds = 0;
+    if (g.mapping == kMapPair) {
+        // the least occupancy of the two t-kernels with a full tile, as for mapping 2
+        lds = stage_lds_bytes(stage_rows(g, per_gdir), per_gdir);
+        const void *fpre = nullptr;
+        DMRI_DISPATCH_GP(g.G, per_gdir, fn = reinterpret_cast<const void *>(&spmv_t_pair_kernel<GG, PP, kOpTP>));
+        DMRI_DISPATCH_GP(g.G, per_gdir, fpre = reinterpret_cast<const void *>(&spmv_t_pair_kernel<GG, PP, kOpPre>));
+        int dev = 0, cus = 0, nb = 0, nb_pre = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, g.spmv_threads, lds) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_pre, fpre, g.spmv_threads, lds) != hipSuccess)
+            throw Error(-2, "occupancy query of the SpMV kernels failed");
+        const int64_t cap = (int64_t)std::min(nb, nb_pre) * cus / 8 * 8;
+        if (cap >= 8 && cap < g.grid) g.grid = (int)cap;
+        return;
+    }
     if (g.mapping == kMapStaged || g.mapping == kMapSL) {
         // the least occupancy of the t-kernels (operator and preconditioner) with a full tile
         const int TR = g.mapping == kMapStaged ? stage_rows(g, per_gdir) : 0;
@@ -3812,7 +4150,14 @@ static void tile_lds(const SolveGeom &g, int &ucap, uint32_t &bytes) {
 template <int OP>
 static void spmv_v_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                             hipEvent_t e1) {
-    if (g.mapping == kMapWave) {
+    if (g.mapping == kMapPair) {
+        const int TR = stage_rows(g, a.per_gdir);
+        const uint32_t lds =
+            (uint32_t)(OP == kOpPre && a.areal ? stage_lds_bytes_ro(TR) : stage_lds_bytes(TR, a.per_gdir));
+        DMRI_DISPATCH_GP(g.G, a.per_gdir,
+                         hipExtLaunchKernelGGL((spmv_v_pair_kernel<GG, PP, OP>), dim3(g.grid), dim3(g.spmv_threads),
+                                               lds, s, e0, e1, 0, a, g.S, g.pair_sp, TR, it));
+    } else if (g.mapping == kMapWave) {
         if constexpr (OP == kOpPreCD) throw Error(-1, "the complex-diagonal preconditioner needs SpMV mapping 1 or 2");
         else if (a.per_gdir)
             hipExtLaunchKernelGGL((spmv_v_wave_kernel<true, OP>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1,
@@ -3833,7 +4178,14 @@ static void spmv_v_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipS
 template <int OP>
 static void spmv_t_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                             hipEvent_t e1) {
-    if (g.mapping == kMapWave) {
+    if (g.mapping == kMapPair) {
+        const int TR = stage_rows(g, a.per_gdir);
+        const uint32_t lds =
+            (uint32_t)(OP == kOpPre && a.areal ? stage_lds_bytes_ro(TR) : stage_lds_bytes(TR, a.per_gdir));
+        DMRI_DISPATCH_GP(g.G, a.per_gdir,
+                         hipExtLaunchKernelGGL((spmv_t_pair_kernel<GG, PP, OP>), dim3(g.grid), dim3(g.spmv_threads),
+                                               lds, s, e0, e1, 0, a, g.S, g.pair_sp, TR, (int)g.grid, it));
+    } else if (g.mapping == kMapWave) {
         if constexpr (OP == kOpPreCD) throw Error(-1, "the complex-diagonal preconditioner needs SpMV mapping 1 or 2");
         else if (a.per_gdir)
             hipExtLaunchKernelGGL((spmv_t_wave_kernel<true, OP>), dim3(g.grid), dim3(g.spmv_threads), 0, s, e0, e1,

@@ -367,10 +367,10 @@ class Context:
     def tune(self, nsig: int, per_gdir: bool = False):
         """Time every (mapping, lanes) candidate; returns ((mapping, lanes), {(mapping, lanes): us})."""
         bl, bm = ctypes.c_int32(), ctypes.c_int32()
-        us = np.zeros(30)
+        us = np.zeros(36)
         _check(LIB.dmri_tune(self._h, int(nsig), 1 if per_gdir else 0, ctypes.byref(bl), ctypes.byref(bm),
                              _ptr(us, ctypes.c_double)))
-        times = {(m, l): us[(m - 1) * 6 + i] for m in (1, 2, 3, 4, 5) for i, l in enumerate((1, 2, 4, 8, 16, 32))
+        times = {(m, l): us[(m - 1) * 6 + i] for m in (1, 2, 3, 4, 5, 6) for i, l in enumerate((1, 2, 4, 8, 16, 32))
                  if us[(m - 1) * 6 + i] >= 0}
         return (bm.value, bl.value), times
 

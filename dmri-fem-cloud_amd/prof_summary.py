@@ -3,17 +3,27 @@
 
   python3 prof_summary.py trace    <kernel_trace.csv>      [--sample K]
   python3 prof_summary.py counters <counter_collection.csv> [--sample K] [--copy-bytes B]
+  python3 prof_summary.py traffic  <FETCH_SIZE csv> <WRITE_SIZE csv> --copy-bytes B
+                                   --nnz NNZ --ndof N --mapping M L --workload TEXT --command CMD ... --out F
 
 trace: per kernel the number of dispatches, mean and median duration, and the mean idle time
-between the end of the previous dispatch and its start.  For the SpMV kernels it also prints the
-mean over the first K dispatches after every residual kernel (the first K Krylov iterations
-of each theta step, where every signal of the launch group is still active): the same launches
-that the solver's sampled kernel events time (dmri_solver_opts.profile = K).
+between the end of the previous dispatch and its start.  For the Krylov iteration kernels (SpMVs,
+bicg_p, s-pass, update) it also prints the mean over the first K dispatches after every residual
+kernel (the first K Krylov iterations of each theta step, where every signal of the launch group is
+still active): the same launches that the solver's sampled kernel events time
+(dmri_solver_opts.profile = K).
 
 counters: per kernel the mean counter value per dispatch (KiB), for all dispatches and for the
 same first-K sample.  With --copy-bytes B (the size given to dmri_copy_bandwidth) the copy kernel
 calibrates the counter: scale = expected KiB per copy dispatch (B/1024 read, B/1024 written) /
 measured mean, and scaled values are printed beside the raw ones.
+
+traffic: the HBM bytes per launch of every kernel of a Krylov iteration (calibrated FETCH_SIZE +
+WRITE_SIZE, mean over the full dispatches: those above half the kernel's largest value, which
+leaves out the launches queued past convergence that exit at once) as the JSON file bench.py reads
+for the roofline's "traffic", keyed by the names of the iteration kernels in dmri_stats (bicg_p,
+spmv_v_pre, spmv_v, spass, spmv_t_pre, spmv_t, bicg_update).  bicg_p's figure is the general
+iteration's (p = r + beta (p - omega v)), not the first one's (p = r).
 """
 from __future__ import annotations
 
@@ -23,7 +33,8 @@ import csv
 import statistics
 
 SPMV = ("spmv_v_kernel", "spmv_t_kernel", "spmv_v_sr_kernel", "spmv_t_sr_kernel", "spmv_v_wave_kernel",
-        "spmv_t_wave_kernel", "spmv_v_tile_kernel", "spmv_t_tile_kernel")
+        "spmv_t_wave_kernel", "spmv_v_tile_kernel", "spmv_t_tile_kernel", "spmv_v_pair_kernel",
+        "spmv_t_pair_kernel", "bicg_p_kernel", "spass_kernel", "bicg_update_kernel")
 
 
 # kernels that form a theta step's residual just before its Krylov iterations
@@ -116,17 +127,94 @@ def counters(path, k, copy_bytes):
                   f"{(f'{statistics.mean(s):16.1f}' if s else f'{chr(32) * 16}')} {scaled}")
 
 
+# SpMV kernels of each mapping (dmri_solver_opts.mapping) with the polynomial preconditioner
+MAPPING_KERNELS = {1: ("spmv_v_kernel", "spmv_t_kernel"), 2: ("spmv_v_kernel", "spmv_t_kernel"),
+                   4: ("spmv_v_wave_kernel", "spmv_t_wave_kernel"), 6: ("spmv_v_pair_kernel", "spmv_t_pair_kernel")}
+
+
+def iteration_kernel(name: str, mapping: int):
+    """dmri_stats name of an iteration kernel from its template instance (SpMV kernels of the given
+    mapping only), or None.  The last template argument of the SpMV kernels is the operation: 0 v =
+    A ph (or t = A s), 1 the polynomial preconditioner, 2 t = A sh, 3 the preconditioner with the
+    complex diagonal."""
+    b = base(name)
+    if b in ("bicg_p_kernel", "spass_kernel", "bicg_update_kernel"):
+        return b[:-len("_kernel")]
+    if b not in MAPPING_KERNELS.get(mapping, ()):
+        return None
+    side = "spmv_v" if b.startswith("spmv_v") else "spmv_t"
+    args = short(name).split("<")[1].rstrip(">").split(",") if "<" in short(name) else []
+    op = int(args[-1]) if args and args[-1].strip().lstrip("-").isdigit() else 0
+    return side + ("_pre" if op in (1, 3) else "")
+
+
+def per_dispatch(path):
+    """(rows, values) per dispatch of a counter CSV (rows of one dispatch summed)."""
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"]))
+    per = collections.OrderedDict()
+    for r in rows:
+        key = int(r["Dispatch_Id"])
+        if key not in per:
+            per[key] = [r, 0.0]
+        per[key][1] += float(r["Counter_Value"])
+    return [v[0] for v in per.values()], [v[1] for v in per.values()]
+
+
+def traffic(fetch_csv, write_csv, copy_bytes, out, meta, mapping):
+    import json
+
+    res, scales = {}, {}
+    for which, path in (("fetch", fetch_csv), ("write", write_csv)):
+        disp, vals = per_dispatch(path)
+        cp = [v for r, v in zip(disp, vals) if short(r["Kernel_Name"]) == "copy_kernel"]
+        scale = (copy_bytes / 1024.0) / statistics.mean(cp)
+        scales[which] = scale
+        acc = collections.defaultdict(list)
+        for r, v in zip(disp, vals):
+            nm = iteration_kernel(r["Kernel_Name"], mapping)
+            if nm:
+                acc[nm].append(v * scale * 1024.0)
+        for nm, v in acc.items():
+            full = [x for x in v if x > 0.5 * max(v)]
+            res.setdefault(nm, {})[which] = statistics.mean(full)
+            res[nm][which + "_launches"] = len(full)
+    doc = dict(meta)
+    doc["fetch_scale"] = scales["fetch"]
+    doc["write_scale"] = scales["write"]
+    doc["read_bytes_per_launch_by_kernel"] = {nm: v.get("fetch", 0.0) for nm, v in res.items()}
+    doc["write_bytes_per_launch_by_kernel"] = {nm: v.get("write", 0.0) for nm, v in res.items()}
+    doc["traffic_per_launch_by_kernel"] = {nm: v.get("fetch", 0.0) + v.get("write", 0.0) for nm, v in res.items()}
+    doc["launches_by_kernel"] = {nm: v.get("fetch_launches", 0) for nm, v in res.items()}
+    doc["calibration"] = (f"rocprofv3 FETCH_SIZE x {scales['fetch']:.4f} + WRITE_SIZE x {scales['write']:.4f} "
+                          f"(copy kernel of {copy_bytes:.0f} B), mean over each kernel's full dispatches")
+    with open(out, "w") as fh:
+        json.dump(doc, fh, indent=1)
+    for nm, v in sorted(doc["traffic_per_launch_by_kernel"].items()):
+        print(f"{nm:12s} {v / 1e9:8.3f} GB per launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "counters"])
+    ap.add_argument("mode", choices=["trace", "counters", "traffic"])
     ap.add_argument("csv")
+    ap.add_argument("csv2", nargs="?")
     ap.add_argument("--sample", type=int, default=2)
     ap.add_argument("--copy-bytes", type=float, default=0.0)
+    ap.add_argument("--nnz", type=int)
+    ap.add_argument("--ndof", type=int)
+    ap.add_argument("--mapping", type=int, nargs=2)
+    ap.add_argument("--workload", default="")
+    ap.add_argument("--command", action="append", default=[])
+    ap.add_argument("--out")
     a = ap.parse_args()
     if a.mode == "trace":
         trace(a.csv, a.sample)
-    else:
+    elif a.mode == "counters":
         counters(a.csv, a.sample, a.copy_bytes)
+    else:
+        meta = {"workload": a.workload, "nnz": a.nnz, "ndof": a.ndof, "mapping_lanes": a.mapping,
+                "commands": a.command}
+        traffic(a.csv, a.csv2, a.copy_bytes, a.out, meta, a.mapping[0])
 
 
 if __name__ == "__main__":

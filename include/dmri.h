@@ -221,7 +221,8 @@ typedef struct {
                               4 = one wave per row, lane = signal, row data through the scalar
                               cache (lanes_per_row 1; for large launch groups),
                               5 = mapping 1 over tiles whose distinct columns are staged once
-                              in LDS (lanes * signals <= 64) */
+                              in LDS (lanes * signals <= 64), 6 = mapping 2 with two signals per
+                              lane (at least 2 signals; lanes_per_row 1, 2 or 4) */
     int32_t preconditioner; /* 0 = Jacobi (the symmetric diagonal scaling) with the degree-1
                                polynomial c (2I - c A) of the scaled system, applied from the right
                                (the residual the tolerance tests stays the true one; the default):
@@ -319,8 +320,8 @@ int dmri_batch_end(dmri_batch *batch);
 
 /* Autotuner: time the four SpMV kernels of a polynomial-preconditioned BiCGStab iteration (the default
  * solve: the preconditioner SpMV on p, v = A ph, the preconditioner SpMV on s, t = A sh) for every
- * (mapping, lanes per row 1..32) that runs them (mappings 1, 2 and 4) on the assembled pattern
- * (us[30] = [mapping-1][log2 lanes] microseconds per iteration's four SpMVs, -1 = not valid) and keep
+ * (mapping, lanes per row 1..32) that runs them (mappings 1, 2, 4 and 6) on the assembled pattern
+ * (us[36] = [mapping-1][log2 lanes] microseconds per iteration's four SpMVs, -1 = not valid) and keep
  * the fastest for later solves of this context whose options leave mapping and lanes_per_row at 0.
  * dmri_set_tuning installs a stored choice without timing.  No reference counterpart (the reference
  * has no GPU kernels); results do not depend on the choice beyond rounding. */

@@ -195,7 +195,9 @@ def test_autotuner_picks_valid_choice_and_results_agree():
         best, times = ctx.tune(nsig, pg)
         assert best in times and all(t > 0 for t in times.values())
         m, l = best
-        assert (m == 3 and nsig <= 8) or (m in (1, 2, 5) and l * nsig <= 64) or (m == 4 and l == 1)
+        assert ((m == 3 and nsig <= 8) or (m in (1, 2, 5) and l * nsig <= 64) or (m == 4 and l == 1)
+                or (m == 6 and nsig >= 2 and l <= 4))
+    assert {(6, 1), (6, 2), (6, 4)} <= set(times)  # 16 signals: two signals per lane with 1, 2 and 4 lanes
     fA, fB = _pgse_gcloud(10600.0, 43100.0, 400.0)
     q = O.q_from_b_pgse(1000.0, 10600.0, 43100.0)
     s_tuned, _ = ctx.solve(fA, fB, 400.0, q, (0, 1, 0), rtol=1e-13)
@@ -205,7 +207,7 @@ def test_autotuner_picks_valid_choice_and_results_agree():
 
 
 @pytest.mark.parametrize("mapping,lanes", [(1, 1), (1, 4), (2, 1), (2, 4), (3, 1), (3, 2), (3, 8), (4, 1), (5, 1),
-                                           (5, 2), (5, 8)])
+                                           (5, 2), (5, 8), (6, 1), (6, 2), (6, 4)])
 @pytest.mark.parametrize("pg", [False, True])
 def test_spmv_mappings_agree_batched(mapping, lanes, pg):
     mesh, kw = MESHES["cyl2"]
@@ -341,6 +343,8 @@ def test_wave_mapping_full_launch_group_matches_oracle():
     assert np.all(np.abs(got - ref) <= 1e-11 * np.abs(ref))
     til, _ = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13, mapping=5, lanes=1)
     assert np.all(np.abs(til - ref) <= 1e-11 * np.abs(ref))
+    pr, _ = ctx.solve_batched(fA, fB, k, qs, gd, rtol=1e-13, mapping=6, lanes=2)  # 32 signal pairs x 2 lanes
+    assert np.all(np.abs(pr - ref) <= 1e-11 * np.abs(ref))
     S = O.System(mesh.xyz, mesh.tets, mesh.phase, D=kw["D"], kappa=kw["kappa"])
     for j in (0, 17, 63):
         s, _, _ = S.solve(gd[j], qs[j], k, fA, fB, rtol=1e-13)
@@ -359,7 +363,7 @@ def test_tile_mapping_single_signal_matches_oracle():
     assert abs(got.real - ref) <= 1e-9 * abs(ref)
 
 
-@pytest.mark.parametrize("mapping,lanes", [(2, 1), (1, 2), (4, 1)])
+@pytest.mark.parametrize("mapping,lanes", [(2, 1), (1, 2), (4, 1), (6, 1), (6, 2)])
 def test_polynomial_preconditioner_matches_jacobi_and_oracle(mapping, lanes):
     """Right preconditioning by 2I - A (scaled system) converges to the same signals as plain
     Jacobi-scaled BiCGStab, in fewer iterations."""
@@ -507,6 +511,10 @@ def test_complex_diagonal_polynomial_matches_oracle(name, b, g):
     assert abs(sig[0].real - s_ref) <= 1e-9 * abs(s_ref)
     s2, _, _ = S.solve(gd[2], 0.5 * q, k, fA, fB, rtol=1e-13)
     assert abs(sig[2].real - s2) <= 1e-9 * abs(s2)
+    # the same through the two-signals-per-lane kernels (their complex-diagonal variants)
+    sp, stp = ctx.solve_batched(fA, fB, k, [q, q, 0.5 * q], gd, rtol=1e-13, precond=2, mapping=6, lanes=1)
+    assert stp["precond_used"] == 3
+    assert np.all(np.abs(sp - sig) <= 1e-10 * np.abs(sig))
 
 
 def test_automatic_complex_diagonal_on_a_large_domain():

@@ -138,8 +138,9 @@ def test_bench_sequence_matches_oracle():
 
 
 def test_profile_summary_samples_first_iterations_of_every_step(tmp_path):
-    """prof_summary's sample: the first k dispatches of every SpMV kernel after each step's residual
-    kernel (the launches bench.py times with events), from a synthetic kernel trace."""
+    """prof_summary's sample: the first k dispatches of every Krylov iteration kernel after each
+    step's residual kernel (the launches bench.py times with events), from a synthetic kernel trace;
+    the names the traffic summary gives them."""
     import prof_summary as ps
 
     names = []
@@ -150,9 +151,14 @@ def test_profile_summary_samples_first_iterations_of_every_step(tmp_path):
                       "spmv_t_kernel<1, false, 1>", "spmv_t_kernel<1, false, 2>", "bicg_update_kernel"]
     rows = [{"Kernel_Name": f"void dmri::(anonymous namespace)::{n}(dmri::SolveArgs, int)"} for n in names]
     picked = [ps.short(rows[i]["Kernel_Name"]) for i in sorted(ps.sampled(rows, 1))]
-    assert len(picked) == 3 * 4
+    assert len(picked) == 3 * 6
     assert picked.count("spmv_v_kernel<1, false, 0>") == 3 and picked.count("spmv_t_kernel<1, false, 2>") == 3
-    assert len(ps.sampled(rows, 2)) == 3 * 4 * 2
+    assert picked.count("bicg_update_kernel") == 3
+    assert len(ps.sampled(rows, 2)) == 3 * 6 * 2
+    named = [ps.iteration_kernel(f"void dmri::{n}(dmri::SolveArgs, int)", 2) for n in names[3:9]]
+    assert named == ["bicg_p", "spmv_v_pre", "spmv_v", "spmv_t_pre", "spmv_t", "bicg_update"]
+    assert ps.iteration_kernel("void dmri::spmv_v_pair_kernel<1, true, 3>(dmri::SolveArgs, int)", 6) == "spmv_v_pre"
+    assert ps.iteration_kernel("void dmri::spmv_v_pair_kernel<1, true, 0>(dmri::SolveArgs, int)", 2) is None
 
 
 def _layered_cylinder():
