@@ -543,22 +543,38 @@ def cpu_baseline_steps(args, w, ctx, info):
 
 
 def run_adjoint(args):
-    """c3: forward solve alone vs adjoint (forward + checkpointed reverse sweep, all gradients)."""
+    """c3: forward solve alone vs adjoint (forward + checkpointed reverse sweep, all gradients) on the
+    300k-vertex cylinders; c4adj: the same on the 2M-vertex neuron surrogate under cos-OGSE (one
+    signal: b = 4000, the first of the 32 directions)."""
     import dmri_mesh as dm
     import dmri_native as N
 
-    delta, Delta, k, b = 10600.0, 43100.0, 200.0, 2000.0
     t0 = time.time()
-    mesh = dm.cylinder_mesh(8.0, 20.0, args.nv or 300000, seed=1003, interface_radii=(4.0,))
+    if args.workload == "c4adj":
+        f, T, int4gb = cos_ogse(4, 19900.0, 19900.0)
+        k = 100.0
+        mesh = dm.neuron_mesh(target_nv=args.nv or 2000000, seed=1004)
+        fA, fB, _ = gcloud_loop(f, T, k)
+        q = math.sqrt(4000.0 / int4gb)
+        g = tuple(float(x) for x in fibonacci_directions(32)[0])
+        coeffs = dict(D=3e-3, invT2=1e-16, kappa=0.0)
+        desc = ("adjoint: neuron-like surrogate (~2M vertices, seed 1004), D=3e-3, cos-OGSE 4 periods "
+                "delta=Delta=19900 us, dt=100 us, b=4000, the first of 32 Fibonacci directions")
+    else:
+        delta, Delta, k, b = 10600.0, 43100.0, 200.0, 2000.0
+        mesh = dm.cylinder_mesh(8.0, 20.0, args.nv or 300000, seed=1003, interface_radii=(4.0,))
+        fA, fB = pgse_sequence(delta, Delta, k)
+        q = q_from_b(b, delta, Delta)
+        g = (0.0, 1.0, 0.0)
+        coeffs = dict(D=3e-3, invT2=1.0 / 80000.0, kappa=1e-5)
+        desc = ("adjoint: concentric cylinders r_in=4 r_out=8 L=20 um (~300k vertices, seed 1003), "
+                "kappa=1e-5, D=3e-3, T2=80 ms, PGSE 10600/43100 us, dt=200 us, b=2000, gdir (0,1,0)")
     t_mesh = time.time() - t0
-    fA, fB = pgse_sequence(delta, Delta, k)
     if args.theta_steps:
         fA, fB = fA[:args.theta_steps], fB[:args.theta_steps]
-    q = q_from_b(b, delta, Delta)
-    g = (0.0, 1.0, 0.0)
     ctx = N.Context(0)
     ctx.set_mesh(mesh.xyz, mesh.tets, mesh.phase)
-    ctx.set_coeffs(D=3e-3, invT2=1.0 / 80000.0, kappa=1e-5)
+    ctx.set_coeffs(**coeffs)
     ctx.assemble()
     sqrt_stride = int(math.ceil(math.sqrt(len(fA))))
     for _ in range(args.warmup):
@@ -586,16 +602,18 @@ def run_adjoint(args):
         "value": ratio, "unit": "x forward", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * (tf + ta) / args.steps, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "adjoint: concentric cylinders r_in=4 r_out=8 L=20 um (~300k vertices, seed 1003), "
-                               "kappa=1e-5, D=3e-3, T2=80 ms, PGSE 10600/43100 us, dt=200 us, b=2000, gdir (0,1,0)",
+        "config": {"workload": desc,
                    "nv": info["nv"], "ndof": info["ndof"], "nnz": info["nnz"], "theta_steps": len(fA),
                    "rtol": args.rtol, "checkpoints": grads["checkpoints"],
                    "recomputed_steps": grads["recomputed_steps"]},
         "forward_ms": 1e3 * tf / args.steps, "adjoint_ms": 1e3 * ta / args.steps,
         "policy": "automatic: every state kept in HBM (they fit), no recompute",
+        "krylov_iterations": {k_: grads["iters_" + k_] for k_ in ("forward", "recompute", "backward")},
         "sqrt_checkpointing": {"stride": sqrt_stride, "checkpoints": gq["checkpoints"],
                                "recomputed_steps": gq["recomputed_steps"], "adjoint_ms": 1e3 * tq / args.steps,
                                "ratio": tq / tf,
+                               "krylov_iterations": {k_: gq["iters_" + k_] for k_ in ("forward", "recompute",
+                                                                                      "backward")},
                                "max_rel_diff_vs_all_states": max(
                                    abs(gq[key_] - grads[key_]) / max(abs(grads[key_]), 1e-300)
                                    for key_ in ("kappa", "q"))},
@@ -612,9 +630,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4", "c4adj"],
                     help="c4 (default): the neuron-like surrogate, one theta step of all 128 signals per bench step; "
-                         "c1, c2: whole jobs per bench step; c3: the adjoint")
+                         "c1, c2: whole jobs per bench step; c3, c4adj: the adjoint on the cylinders / the neuron")
     ap.add_argument("--whole-jobs", action="store_true",
                     help="c4 only: time whole jobs (assembly + every signal's full sequence) instead of theta steps")
     ap.add_argument("--nv", type=int, default=0, help="target vertex count (0 = the workload's own)")
@@ -651,7 +669,7 @@ def main():
         sys.exit(2)
     if args.cpu_steps <= 0:
         args.cpu_steps = 3 if args.workload == "c4" else 60
-    if args.workload == "c3":
+    if args.workload in ("c3", "c4adj"):
         run_adjoint(args)
     elif args.workload == "c4" and not args.whole_jobs:
         run_stepwise(args, args.workload)

@@ -122,8 +122,9 @@ class KrylovSolver:
 
     def __init__(self, method="bicgstab", pc="jacobi"):
         self.method, self.pc = method, pc
+        # "restart": Arnoldi steps per GMRES cycle (PETSc's default 30), unused by BiCGStab
         self.parameters = {"relative_tolerance": 1e-9, "absolute_tolerance": 1e-10, "maximum_iterations": 100000,
-                           "nonzero_initial_guess": False}
+                           "nonzero_initial_guess": False, "restart": 30}
 
 
 class PETScLUSolver(KrylovSolver):
@@ -392,6 +393,7 @@ class MRI_simulation:
                                               maxit=min(int(p["maximum_iterations"]), 2 ** 31 - 1),
                                               F_explicit=FB, F_implicit=FA, u0=self.u0_vertex,
                                               krylov=1 if str(linsolver.method).lower() == "gmres" else 0,
+                                              restart=int(p.get("restart", 30)),
                                               snap_every=self.nskip if self.save_snapshots else 0)
         self.elapsed_time = time.time() - start
         if self.save_snapshots:

@@ -1265,7 +1265,7 @@ struct Stepper {
             pa.P[j] = h.pq[j].first;
             pa.Q[j] = h.pq[j].second;
         }
-        pa.b = w.b.p;
+        pa.b = a.b;
         pa.coefA = a.coefA;
         pa.gsig = per_gdir ? w.gsig.p : nullptr;
         pa.part = w.ppart.p;
@@ -1334,12 +1334,13 @@ struct Stepper {
     // empty); returns the buffer holding the solution.
     double2 *solve_projected(int k, const double2 *rhs, ProjHistory &h, double2 *start) {
         begin_step(k);
-        DMRI_HIP(hipMemcpyAsync(wk->b.p, rhs, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        a.b = const_cast<double2 *>(rhs);  // read in place (the residual passes only read b)
         double2 *y = h.nh() > 0 ? projected_guess(h) : start;
         a.x = y;
         launch_rhs_residual(geo, a, 1, nullptr, s);
         solve_krylov(k);
         a.x = xc;
+        a.b = wk->b.p;
         return y;
     }
 
@@ -1350,10 +1351,11 @@ struct Stepper {
         a.x = y;
         a.hp = slot.first;
         a.hq = slot.second;
+        a.b = out;  // the right-hand-side pass writes B_k y straight into out
         launch_rhs_residual(geo, a, 4, nullptr, s);
         h.push(y, slot);
-        DMRI_HIP(hipMemcpyAsync(out, wk->b.p, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
         a.x = xc;
+        a.b = wk->b.p;
         a.hp = a.hq = nullptr;
     }
 
@@ -1514,20 +1516,22 @@ struct Stepper {
     // Solve A_k y = rhs (complex-symmetric A_k) starting from y; one signal.
     void solve_given(int k, const double2 *rhs, double2 *y) {
         begin_step(k);
-        DMRI_HIP(hipMemcpyAsync(wk->b.p, rhs, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        a.b = const_cast<double2 *>(rhs);
         a.x = y;
         launch_rhs_residual(geo, a, 1, nullptr, s);
         solve_krylov(k);
         a.x = xc;
+        a.b = wk->b.p;
     }
 
     // out = B_k y
     void apply_B(int k, double2 *y, double2 *out) {
         begin_step(k);
         a.x = y;
+        a.b = out;
         launch_rhs_residual(geo, a, 3, nullptr, s);
-        DMRI_HIP(hipMemcpyAsync(out, wk->b.p, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
         a.x = xc;
+        a.b = wk->b.p;
     }
 
     // Signals of the current state u^n (the state buffers are left as they are, so stepping may go on)
@@ -1636,6 +1640,8 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     }
     run.signal(sig);
     const auto t1 = std::chrono::steady_clock::now();
+    const int64_t it_fwd = st.iters_total;
+    int64_t it_rec = 0;
     // backward sweep
     SolverWork &w = c->work;
     const double theta = seq->theta;
@@ -1700,11 +1706,13 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
                 DMRI_HIP(hipMemcpyAsync(run.xq, ck.p + (size_t)(3 * j + 2) * n, vb, hipMemcpyDeviceToDevice, s));
             }
             DMRI_HIP(hipMemcpyAsync(segp(0), run.xc, vb, hipMemcpyDeviceToDevice, s));
+            const int64_t before = st.iters_total;
             for (int k = k0; k < k1; ++k) {
                 run.step(k);
                 DMRI_HIP(hipMemcpyAsync(segp(k - k0 + 1), run.xc, vb, hipMemcpyDeviceToDevice, s));
                 ++recomputed;
             }
+            it_rec += st.iters_total - before;
         }
         for (int k = k1 - 1; k >= k0; --k) {
             const int base = keep_all ? 0 : k0;
@@ -1765,6 +1773,9 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     grads->ms_adjoint = std::chrono::duration<double, std::milli>(t2 - t1).count();
     grads->checkpoints = keep_all ? 0 : nck;
     grads->recomputed_steps = recomputed;
+    grads->iters_forward = it_fwd;
+    grads->iters_recompute = it_rec;
+    grads->iters_backward = st.iters_total - it_fwd - it_rec;
     if (stats) *stats = st;
     c->solved = false;  // work.x no longer holds u^N of a plain solve
 }
@@ -1932,7 +1943,8 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
             slot = -1.0;
             if (m == kMapSR || m == kMapTile) continue;
             if (m == kMapWave && G != 1) continue;
-            if (m == kMapPair && (S < 2 || G > 4 || G * pair_sp > 64)) continue;
+            if (m == kMapPair && (S < 2 || G > 4 || G * pair_sp > 64 || (uint64_t)n * S * 16u >= (1ull << 32)))
+                continue;
             if (m != kMapWave && m != kMapPair && G * S > 64) continue;
             SolveGeom geo = make_geom(n, G, S, m);
             fit_spmv_grid(geo, per_gdir);

@@ -1343,7 +1343,8 @@ __device__ __forceinline__ void row_accumulate(const int *rowptr, const int *col
 }
 
 // Same accumulation with the row's column indices and values already staged in LDS;
-// four entries per step keep four independent gathers in flight per lane.
+// four entries per step keep four independent gathers in flight per lane, and the up to three
+// entries left go in one masked step (their gathers in flight together, not one after another).
 template <int G, bool PG, class Gather>
 __device__ __forceinline__ void lds_row_accumulate(const int *srp, const int *scol, const double *svals, int lr,
                                                    int g, Gather X, Acc<PG> &acc) {
@@ -1360,7 +1361,18 @@ __device__ __forceinline__ void lds_row_accumulate(const int *srp, const int *sc
         acc_entry<PG>(svals, e + 2 * G, x2, acc);
         acc_entry<PG>(svals, e + 3 * G, x3, acc);
     }
-    for (; e < e1; e += G) acc_entry<PG>(svals, e, X(scol[e]), acc);
+    if (e < e1) {
+        int c[3];
+        double2 x[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c[k] = e + k * G < e1 ? scol[e + k * G] : -1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (c[k] >= 0) x[k] = X(c[k]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (c[k] >= 0) acc_entry<PG>(svals, e + k * G, x[k], acc);
+    }
 }
 
 // Rows of this block.  Workgroups are dealt to the 8 XCDs round-robin, so block b runs on XCD
@@ -1414,10 +1426,20 @@ __device__ __forceinline__ void real_row_accumulate(const int *col, const double
         acc.a.x += v0 * x0.x + v1 * x1.x + v2 * x2.x + v3 * x3.x;
         acc.a.y += v0 * x0.y + v1 * x1.y + v2 * x2.y + v3 * x3.y;
     }
-    for (; e < e1; e += G) {
-        const double2 x0 = X(col[e]);
-        acc.a.x += av[e] * x0.x;
-        acc.a.y += av[e] * x0.y;
+    if (e < e1) {
+        int c[3];
+        double2 x[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c[k] = e + k * G < e1 ? col[e + k * G] : -1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (c[k] >= 0) x[k] = X(c[k]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (c[k] >= 0) {
+                acc.a.x += av[e + k * G] * x[k].x;
+                acc.a.y += av[e + k * G] * x[k].y;
+            }
     }
 }
 
@@ -2037,52 +2059,70 @@ __device__ __forceinline__ PairLane pair_lane(int SP) {
     return PairLane{(int)threadIdx.x / LP, rem / G, rem % G};
 }
 
+// One entry's contribution (a + i j) x to both signals' sums, four FMAs per signal.
+template <bool PG, bool RO>
+__device__ __forceinline__ void pair_entry(const double *vals, int64_t e, const double (&cf)[2][3], const double2 &x0,
+                                           const double2 &x1, double2 (&y)[2]) {
+    constexpr int VW = RO ? 1 : (PG ? 4 : 2);
+    const int64_t ek = e * VW;
+    double av, j0 = 0.0, j1 = 0.0;
+    if (RO) {
+        av = vals[ek];
+    } else if (PG) {
+        const double4 v = *reinterpret_cast<const double4 *>(vals + ek);
+        av = v.x;
+        j0 = fma(cf[0][2], v.w, fma(cf[0][1], v.z, cf[0][0] * v.y));
+        j1 = fma(cf[1][2], v.w, fma(cf[1][1], v.z, cf[1][0] * v.y));
+    } else {
+        const double2 v = *reinterpret_cast<const double2 *>(vals + ek);
+        av = v.x;
+        j0 = cf[0][0] * v.y;
+        j1 = cf[1][0] * v.y;
+    }
+    y[0].x = fma(av, x0.x, y[0].x);
+    y[0].y = fma(av, x0.y, y[0].y);
+    y[1].x = fma(av, x1.x, y[1].x);
+    y[1].y = fma(av, x1.y, y[1].y);
+    if (!RO) {
+        y[0].x = fma(-j0, x0.y, y[0].x);
+        y[0].y = fma(j0, x0.x, y[0].y);
+        y[1].x = fma(-j1, x1.y, y[1].x);
+        y[1].y = fma(j1, x1.x, y[1].y);
+    }
+}
+
 // Row entries e0 + g, e0 + g + G, ... < e1 (column indices col[], values vals[] with VW per entry,
-// or RO: one real value per entry) against the two gathered signals X(c, x0, x1): four entries per
-// step, the last step masked, so a lane waits on memory ceil(entries / 4G) times per row.
-// cf[k]: the imaginary coefficients of the lane's signal k (per direction when PG).
+// or RO: one real value per entry) against the two gathered signals X(c, x0, x1): full steps of
+// four entries without branches (every step's eight gathers are in flight together), then the up
+// to three entries left in one masked step.  cf[k]: the imaginary coefficients of the lane's
+// signal k (per direction when PG).
 template <int G, bool PG, bool RO, class Gather>
 __device__ __forceinline__ void pair_row(const int *col, const double *vals, int e0, int e1, int g,
                                          const double (&cf)[2][3], Gather X, double2 (&y)[2]) {
-    constexpr int VW = RO ? 1 : (PG ? 4 : 2);
     y[0] = make_double2(0.0, 0.0);
     y[1] = make_double2(0.0, 0.0);
-    for (int e = e0 + g; e < e1; e += 4 * G) {
+    int e = e0 + g;
+    for (; e + 3 * G < e1; e += 4 * G) {
         int c[4];
         double2 x0[4], x1[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) c[k] = e + k * G < e1 ? col[e + k * G] : -1;
+        for (int k = 0; k < 4; ++k) c[k] = col[e + k * G];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            x0[k] = make_double2(0.0, 0.0);
-            x1[k] = make_double2(0.0, 0.0);
+        for (int k = 0; k < 4; ++k) X(c[k], x0[k], x1[k]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pair_entry<PG, RO>(vals, e + k * G, cf, x0[k], x1[k], y);
+    }
+    if (e < e1) {
+        int c[3];
+        double2 x0[3], x1[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c[k] = e + k * G < e1 ? col[e + k * G] : -1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
             if (c[k] >= 0) X(c[k], x0[k], x1[k]);
-        }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (c[k] < 0) continue;
-            const int64_t ek = (int64_t)(e + k * G) * VW;
-            double av, j0, j1;
-            if (RO) {
-                av = vals[ek];
-                j0 = j1 = 0.0;
-            } else if (PG) {
-                const double4 v = *reinterpret_cast<const double4 *>(vals + ek);
-                av = v.x;
-                j0 = cf[0][0] * v.y + cf[0][1] * v.z + cf[0][2] * v.w;
-                j1 = cf[1][0] * v.y + cf[1][1] * v.z + cf[1][2] * v.w;
-            } else {
-                const double2 v = *reinterpret_cast<const double2 *>(vals + ek);
-                av = v.x;
-                j0 = cf[0][0] * v.y;
-                j1 = cf[1][0] * v.y;
-            }
-            // (a + i j) x
-            y[0].x += av * x0[k].x - j0 * x0[k].y;
-            y[0].y += av * x0[k].y + j0 * x0[k].x;
-            y[1].x += av * x1[k].x - j1 * x1[k].y;
-            y[1].y += av * x1[k].y + j1 * x1[k].x;
-        }
+        for (int k = 0; k < 3; ++k)
+            if (c[k] >= 0) pair_entry<PG, RO>(vals, e + k * G, cf, x0[k], x1[k], y);
     }
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) {
@@ -2091,6 +2131,18 @@ __device__ __forceinline__ void pair_row(const int *col, const double *vals, int
         y[1].x += __shfl_xor(y[1].x, off, 64);
         y[1].y += __shfl_xor(y[1].y, off, 64);
     }
+}
+
+// Both signals of a lane at column c of a signal-interleaved vector: element c S + s0 and the
+// next, through a 32-bit byte offset from the kernel argument's base (the load then takes the
+// base from scalar registers and a single 32-bit offset from the lane; make_geom keeps n S 16 B of
+// every vector of this mapping below 4 GiB).  Signal s0 + 1 past the last signal reads s0 again.
+__device__ __forceinline__ void pair_load(const double2 *base, uint32_t row_bytes, uint32_t lane_bytes,
+                                          uint32_t next_bytes, int c, double2 &x0, double2 &x1) {
+    const char *p = reinterpret_cast<const char *>(base);
+    const uint32_t o0 = (uint32_t)c * row_bytes + lane_bytes, o1 = o0 + next_bytes;
+    x0 = *reinterpret_cast<const double2 *>(p + o0);
+    x1 = *reinterpret_cast<const double2 *>(p + o1);
 }
 
 // The tiles of the block (XCD-aware walk of xcd_tiles) staged in LDS, then every row of a tile for
@@ -2186,13 +2238,14 @@ __global__ void __launch_bounds__(kThreads) spmv_v_pair_kernel(SolveArgs a, int 
     double r0[kNV] = {0.0, 0.0, 0.0, 0.0}, r1[kNV] = {0.0, 0.0, 0.0, 0.0};
     const double2 *p = a.p;
     const int64_t S_ = S;
+    // signals of finished or absent lanes are gathered and multiplied too, but never written
+    const uint32_t rb = 16u * (uint32_t)S, lb = 16u * (uint32_t)s0, nb = s1 < S ? 16u : 0u;
+    const int s1c = s1 < S ? s1 : s0;
     auto gather = [&](int c, double2 &x0, double2 &x1) {
-        const int64_t j = (int64_t)c * S_ + s0;
-        if (on0) x0 = p[j];
-        if (on1) x1 = p[j + 1];
+        pair_load(p, rb, lb, nb, c, x0, x1);
         if constexpr (cd) {
-            if (on0) x0 = c_mul(x0, cdiag_inv<PG>(a.pdiag, a.coefA, c, s0));
-            if (on1) x1 = c_mul(x1, cdiag_inv<PG>(a.pdiag, a.coefA, c, s1));
+            x0 = c_mul(x0, cdiag_inv<PG>(a.pdiag, a.coefA, c, s0));
+            x1 = c_mul(x1, cdiag_inv<PG>(a.pdiag, a.coefA, c, s1c));
         }
     };
     auto emit = [&](int64_t row, const double2 (&y)[2]) {
@@ -2248,18 +2301,20 @@ __global__ void __launch_bounds__(kThreads) spmv_t_pair_kernel(SolveArgs a, int 
     const int64_t S_ = S;
     // s at entry j of signal k: materialised by the s-pass, or formed from r and v
     auto sval = [&](int64_t j, const double2 &al) { return sm ? sm[j] : c_sub(r[j], c_mul(al, v[j])); };
+    // signals of finished or absent lanes are gathered and multiplied too, but never written
+    const uint32_t rb = 16u * (uint32_t)S, lb = 16u * (uint32_t)s0, nb = s1 < S ? 16u : 0u;
+    const int s1c = s1 < S ? s1 : s0;
     auto gather = [&](int c, double2 &x0, double2 &x1) {
-        const int64_t j = (int64_t)c * S_ + s0;
-        if (OP == kOpTP) {
-            if (on0) x0 = sh[j];
-            if (on1) x1 = sh[j + 1];
-            return;
+        if (OP == kOpTP || sm) {
+            pair_load(OP == kOpTP ? sh : sm, rb, lb, nb, c, x0, x1);
+        } else {
+            const int64_t j = (int64_t)c * S_ + s0;
+            x0 = c_sub(r[j], c_mul(al0, v[j]));
+            x1 = c_sub(r[j + nb / 16], c_mul(al1, v[j + nb / 16]));
         }
-        if (on0) x0 = sval(j, al0);
-        if (on1) x1 = sval(j + 1, al1);
         if constexpr (cd) {
-            if (on0) x0 = c_mul(x0, cdiag_inv<PG>(a.pdiag, a.coefA, c, s0));
-            if (on1) x1 = c_mul(x1, cdiag_inv<PG>(a.pdiag, a.coefA, c, s1));
+            x0 = c_mul(x0, cdiag_inv<PG>(a.pdiag, a.coefA, c, s0));
+            x1 = c_mul(x1, cdiag_inv<PG>(a.pdiag, a.coefA, c, s1c));
         }
     };
     auto emit = [&](int64_t row, const double2 (&y)[2]) {
@@ -3659,6 +3714,8 @@ SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
     if (mapping == kMapPair && (S < 2 || G > 4 || G * pair_sp > 64))
         throw Error(-1, "the two-signals-per-lane mapping needs at least 2 signals, lanes 1, 2 or 4 and "
                         "lanes x signal pairs <= 64");
+    if (mapping == kMapPair && (uint64_t)n * (uint64_t)S * 16u >= (1ull << 32))
+        throw Error(-1, "the two-signals-per-lane mapping addresses vectors of at most 4 GiB (n x signals x 16 B)");
     if (mapping == kMapWave && G != 1) throw Error(-1, "the wave-per-row mapping takes one lane per signal (lanes 1)");
     if (mapping == kMapSR && S > kMaxSRSignals)
         throw Error(-1, "signals-in-registers mapping needs at most 8 signals per launch group");

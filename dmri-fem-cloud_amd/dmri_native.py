@@ -81,7 +81,8 @@ class _Grads(ctypes.Structure):
                 ("dS_dq", ctypes.c_double), ("dS_df_implicit", ctypes.POINTER(ctypes.c_double)),
                 ("dS_df_explicit", ctypes.POINTER(ctypes.c_double)), ("ms_forward", ctypes.c_double),
                 ("ms_adjoint", ctypes.c_double), ("checkpoints", ctypes.c_int32),
-                ("recomputed_steps", ctypes.c_int32)]
+                ("recomputed_steps", ctypes.c_int32), ("iters_forward", ctypes.c_int64),
+                ("iters_recompute", ctypes.c_int64), ("iters_backward", ctypes.c_int64)]
 
 
 EXPORTED = ["dmri_create", "dmri_destroy", "dmri_last_error", "dmri_version", "dmri_mesh_set", "dmri_coeffs_set",
@@ -402,7 +403,9 @@ class Context:
                                 ctypes.byref(g), ctypes.byref(st)))
         grads = {"D": np.array(g.dS_dD[:]), "invT2": np.array(g.dS_dinvT2[:]), "kappa": g.dS_dkappa, "q": g.dS_dq,
                  "f_implicit": dfa[:n].copy(), "f_explicit": dfb[:n].copy(), "ms_forward": g.ms_forward,
-                 "ms_adjoint": g.ms_adjoint, "checkpoints": g.checkpoints, "recomputed_steps": g.recomputed_steps}
+                 "ms_adjoint": g.ms_adjoint, "checkpoints": g.checkpoints, "recomputed_steps": g.recomputed_steps,
+                 "iters_forward": g.iters_forward, "iters_recompute": g.iters_recompute,
+                 "iters_backward": g.iters_backward}
         return complex(sig[0], sig[1]), grads, _stats_dict(st)
 
     def time_spmv(self, nsig: int, per_gdir: bool = False, reps: int = 50):

@@ -384,6 +384,9 @@ typedef struct {
     double *dS_df_explicit; /* optional out, nsteps: d/d f_explicit[n] at fixed q */
     double ms_forward, ms_adjoint;
     int32_t checkpoints, recomputed_steps;
+    /* Krylov iterations of the forward sweep, of the steps recomputed from checkpoints, and of the
+       backward (adjoint) solves */
+    int64_t iters_forward, iters_recompute, iters_backward;
 } dmri_grads;
 int dmri_adjoint(dmri_ctx *ctx, const dmri_seq *seq, const dmri_solver_opts *opts, const dmri_adjoint_opts *aopts,
                  double sig_out[2], dmri_grads *grads, dmri_stats *stats);
