@@ -103,7 +103,7 @@ def merge_stats(stats):
             out[key] = max(vals)
         elif key.startswith("kern_"):
             out[key] = [sum(v[j] for v in vals) for j in range(len(vals[0]))]
-        elif key not in ("spmv_bytes", "precond_used", "reserved"):
+        elif key not in ("spmv_bytes", "precond_used"):
             out[key] = sum(vals)
     return out
 
@@ -291,13 +291,15 @@ def run_forward(args, name):
         ctx.assemble()
         if nsig <= S_group:
             return ctx.solve_batched(fA, fB, k, qs, gd, rtol=args.rtol, profile=profile, lanes=args.lanes,
-                                     guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond)
+                                     guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond,
+                                     graphs=args.graphs)
         sigs, stats = [], []
         t0 = time.perf_counter()
         for g0 in range(0, nsig, S_group):
             sl = slice(g0, min(nsig, g0 + S_group))
             s_, st_ = ctx.solve_batched(fA, fB, k, qs[sl], gd[sl], rtol=args.rtol, profile=profile, lanes=args.lanes,
-                                        guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond)
+                                        guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond,
+                                        graphs=args.graphs)
             sigs.append(s_)
             stats.append(st_)
             print(f"  signals {sl.stop}/{nsig} done, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -348,7 +350,7 @@ def run_forward(args, name):
 
     cfg = {"workload": w["desc"], "nv": info["nv"], "nc": info["nc"], "ndof": info["ndof"], "nnz": info["nnz"],
            "n_iface": info["n_iface"], "n_periodic": info["n_periodic"], "theta_steps": len(fA),
-           "signals": nsig, "launch_group": S_group, "rtol": args.rtol, "spmv_mapping_lanes": tuned}
+           "signals": nsig, "launch_group": S_group, "rtol": args.rtol, "spmv_mapping_lanes": tuned, "graphs": args.graphs}
     cfg.update(w["extra"])
     if "full_steps" in w:
         cfg["theta_steps_of_full_sequence"] = f"first {len(fA)} of {w['full_steps']}"
@@ -413,7 +415,8 @@ def run_stepwise(args, name):
           f"SpMV mapping/lanes {tuned}", file=sys.stderr, flush=True)
     tb = time.perf_counter()
     batch = ctx.batch(fA, fB, k, qs, gd, rtol=args.rtol, profile=args.event_iters, lanes=args.lanes,
-                      guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond, group=S_group)
+                      guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond, group=S_group,
+                      graphs=args.graphs)
     t_begin = time.perf_counter() - tb
     for i in range(args.warmup):
         batch.advance(1)
@@ -469,7 +472,7 @@ def run_stepwise(args, name):
            "theta_steps_full_sequence": nfull, "signals": nsig, "launch_group": S_group, "rtol": args.rtol,
            "timed_window": f"theta steps {args.warmup}..{args.warmup + args.steps - 1} of {nfull} (after "
                            f"{args.warmup} warm-up steps from u0 = 1), every signal",
-           "spmv_mapping_lanes": tuned}
+           "spmv_mapping_lanes": tuned, "graphs": args.graphs}
     cfg.update(w["extra"])
     out = {
         "metric": METRIC,
@@ -658,6 +661,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="counter runs only: truncate to the first K theta steps (result line not comparable)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel reference")
+    ap.add_argument("--graphs", type=int, default=0, choices=[0, 1],
+                    help="1: replay the Krylov iteration runs as captured hipGraphs")
     ap.add_argument("--event-iters", type=int, default=1,
                     help="Krylov iterations per theta step whose SpMV launches are event-timed")
     args = ap.parse_args()

@@ -37,6 +37,8 @@ struct SolverWork {
     DevArray<double2> yh[kMaxHist], hp[kMaxHist], hq[kMaxHist];  // projection history
     DevArray<double> sc, vals, bre, areal, pdiag, part, tot, coef, coefP, sig, spart, ppart, ptot, pgam, gsig, u0v;
     DevArray<double> afix, bfix, mgg;  // strong periodic form: fixed real parts and the q^2 F^2 mass term
+    DevArray<double> coefcur;          // graph mode: the current step's coefficients (A then B)
+    DevArray<double2> xk;              // graph mode: the projection guess's Krylov iterate
     // restarted GMRES: basis, new vector, partials, coefficients, Hessenberg, rotations, totals
     DevArray<double2> gV, gw, gH, gcs, gg, gy;
     DevArray<double> gpart, gh, ghsum, gcc, ginv, gtot, gwn;
@@ -850,6 +852,28 @@ struct Stepper {
     bool timed_first[kEventPool / 7] = {};   // timed iteration k of a chunk is the solve's first
     double omt = 0.5;
     int chunk = 8;
+    // hipGraph replay of BiCGStab iterations (opts->graphs): the captured runs, keyed by the kernel
+    // arguments, the first iteration and the count
+    struct ChainGraph {
+        SolveArgs args;
+        int it0, count;
+        hipGraph_t g;
+        hipGraphExec_t e;
+    };
+    bool graphs_on = false;
+    std::vector<ChainGraph> graphs;
+    Stepper() = default;
+    Stepper(const Stepper &) = delete;
+    Stepper &operator=(const Stepper &) = delete;
+    ~Stepper() { drop_graphs(); }
+    void drop_graphs() {
+        if (!graphs.empty() && s) (void)hipStreamSynchronize(s);  // no replay still running
+        for (auto &cg : graphs) {
+            (void)hipGraphExecDestroy(cg.e);
+            (void)hipGraphDestroy(cg.g);
+        }
+        graphs.clear();
+    }
 
     // q[s], g[3*s] (unit).  Scales the system, uploads the per-step coefficients, sets u0 = 1.
     void setup(dmri_ctx *ctx, int S_, const double *q, const double *g, bool pg, const dmri_seq *seq,
@@ -1084,6 +1108,12 @@ struct Stepper {
         precond = poly ? st->precond_used : 0;
         prof = opts->profile != 0;
         chunk = 8;
+        drop_graphs();
+        graphs_on = opts->graphs != 0 && !gmres_on;
+        if (graphs_on) {
+            if (w.coefcur.n < (size_t)6 * S) w.coefcur.alloc((size_t)6 * S);
+            if (fh.m > 0 && w.xk.n != (size_t)n * S) w.xk.alloc((size_t)n * S);
+        }
     }
 
     void setup_gmres() {
@@ -1252,9 +1282,9 @@ struct Stepper {
 
     // Guess of the current step (coefficients set by begin_step, b in work.b) from history h
     // into a free buffer of h, which it returns.
-    double2 *projected_guess(ProjHistory &h) {
+    double2 *projected_guess(ProjHistory &h, double2 *into = nullptr) {
         SolverWork &w = *wk;
-        double2 *y0 = h.take_free();
+        double2 *y0 = into ? into : h.take_free();
         ProjArgs pa{};
         pa.n = n;
         pa.S = S;
@@ -1284,11 +1314,23 @@ struct Stepper {
         a.hp = slot.first;
         a.hq = slot.second;
         launch_rhs_residual(geo, a, 4, nullptr, s);
+        a.hp = a.hq = nullptr;
         fh.pq.push_front(slot);
-        double2 *y0 = projected_guess(fh);
-        a.x = y0;
-        launch_rhs_residual(geo, a, 1, nullptr, s);
-        solve_krylov(k);
+        double2 *y0;
+        if (graphs_on) {
+            // the Krylov iterate stays in one buffer (the captured kernels' argument), then moves to
+            // the history's free buffer
+            y0 = fh.take_free();
+            a.x = projected_guess(fh, wk->xk.p);
+            launch_rhs_residual(geo, a, 1, nullptr, s);
+            solve_krylov(k);
+            DMRI_HIP(hipMemcpyAsync(y0, wk->xk.p, sizeof(double2) * (size_t)n * S, hipMemcpyDeviceToDevice, s));
+        } else {
+            y0 = projected_guess(fh);
+            a.x = y0;
+            launch_rhs_residual(geo, a, 1, nullptr, s);
+            solve_krylov(k);
+        }
         xc = y0;
         a.x = xc;
         fh.ys.push_front(y0);
@@ -1364,9 +1406,101 @@ struct Stepper {
         if (strong)
             launch_sbc_step(c->nnz, per_gdir ? 4 : 2, w.afix.p, w.bfix.p, w.mgg.p, s2A[step], s2B[step], w.vals.p,
                             w.bre.p, real_pre ? w.areal.p : nullptr, s);
-        a.coefA = w.coef.p + (size_t)step * S * 3;
-        a.coefB = w.coef.p + (size_t)std::max(N, 1) * S * 3 + (size_t)step * S * 3;
+        const double *cA = w.coef.p + (size_t)step * S * 3;
+        const double *cB = w.coef.p + (size_t)std::max(N, 1) * S * 3 + (size_t)step * S * 3;
+        if (graphs_on) {  // a fixed slot, so that the captured kernels' arguments repeat from step to step
+            DMRI_HIP(hipMemcpyAsync(w.coefcur.p, cA, sizeof(double) * 3 * S, hipMemcpyDeviceToDevice, s));
+            DMRI_HIP(hipMemcpyAsync(w.coefcur.p + 3 * S, cB, sizeof(double) * 3 * S, hipMemcpyDeviceToDevice, s));
+            a.coefA = w.coefcur.p;
+            a.coefB = w.coefcur.p + 3 * S;
+        } else {
+            a.coefA = cA;
+            a.coefB = cB;
+        }
         launch_reset_state(w.st.p, S, w.all_done.p, s);
+    }
+
+    // The seven kernels of BiCGStab iteration it (e0(j), e1(j): the events of kernel j, or null)
+    template <class E0, class E1>
+    void launch_iteration(int it, E0 e0, E1 e1) {
+        launch_bicg_p(geo, a, it, s, e0(0), e1(0));
+        if (poly) {
+            // ph = 2p - Ap, v = A ph; sh = 2s - As, t = A sh (right preconditioning)
+            launch_spmv_v(geo, a, it, s, e0(1), e1(1), kOpPre);
+            SolveArgs av = a;
+            av.p = a.ph;
+            launch_spmv_v(geo, av, it, s, e0(2), e1(2));
+            launch_spass(geo, a, it, s, e0(3), e1(3));
+            launch_spmv_t(geo, a, it, s, e0(4), e1(4), kOpPre);
+            launch_spmv_t(geo, a, it, s, e0(5), e1(5), kOpTP);
+        } else {
+            launch_spmv_v(geo, a, it, s, e0(2), e1(2));
+            launch_spmv_t(geo, a, it, s, e0(5), e1(5));
+        }
+        launch_bicg_update(geo, a, it, s, e0(6), e1(6));
+    }
+
+    // Iterations it0 .. it0+count-1 from captured graphs of 2^k iterations each (at most 64 at the
+    // largest), so that the few run lengths a solve needs recur from step to step.  A graph is keyed
+    // by the bytes of the kernel arguments, its first iteration and its length: a replay launches
+    // exactly what the plain launches would.  A capture the runtime refuses turns graphs off and
+    // launches the iterations one by one.
+    void launch_chain(int it0, int count) {
+        auto none = [](int) -> hipEvent_t { return nullptr; };
+        while (count > 0) {
+            int len = 64;
+            while (len > count) len >>= 1;
+            if (!graphs_on) {
+                for (int i = 0; i < len; ++i) launch_iteration(it0 + i, none, none);
+            } else {
+                int hit = -1;
+                for (size_t q = 0; q < graphs.size() && hit < 0; ++q)
+                    if (graphs[q].it0 == it0 && graphs[q].count == len &&
+                        std::memcmp(&graphs[q].args, &a, sizeof(SolveArgs)) == 0)
+                        hit = (int)q;
+                if (hit < 0) hit = capture_chain(it0, len);
+                if (hit >= 0) {
+                    DMRI_HIP(hipGraphLaunch(graphs[hit].e, s));
+                    if (st) st->graph_launches += 1;
+                } else {
+                    for (int i = 0; i < len; ++i) launch_iteration(it0 + i, none, none);
+                }
+            }
+            it0 += len;
+            count -= len;
+        }
+    }
+    // Captures iterations it0 .. it0+len-1 into the cache; returns its index, or -1 (graphs off)
+    int capture_chain(int it0, int len) {
+        auto none = [](int) -> hipEvent_t { return nullptr; };
+        if (graphs.size() >= 256) drop_graphs();
+        ChainGraph cg{};
+        cg.args = a;
+        cg.it0 = it0;
+        cg.count = len;
+        bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            try {
+                for (int i = 0; i < len; ++i) launch_iteration(it0 + i, none, none);
+            } catch (...) {
+                ok = false;
+            }
+            hipGraph_t g = nullptr;
+            const bool ended = hipStreamEndCapture(s, &g) == hipSuccess;
+            ok = ok && ended && g != nullptr && hipGraphInstantiate(&cg.e, g, nullptr, nullptr, 0) == hipSuccess;
+            cg.g = g;
+            if (!ok) {
+                if (cg.e) (void)hipGraphExecDestroy(cg.e);
+                if (g) (void)hipGraphDestroy(g);
+            }
+        }
+        (void)hipGetLastError();
+        if (!ok) {
+            graphs_on = false;
+            return -1;
+        }
+        graphs.push_back(cg);
+        return (int)graphs.size() - 1;
     }
 
     // BiCGStab iterations on a.x with the residual already formed; restarts on breakdown.
@@ -1377,28 +1511,20 @@ struct Stepper {
         for (;;) {
             const int nl = chunk;
             int ntimed = 0;
-            for (int k = 0; k < nl; ++k, ++it) {
+            int k = 0;
+            for (; k < nl; ++k, ++it) {
                 // every kernel of the iterations it < profile carries start/end events that take the
                 // timestamps of its own dispatch (no marker packets between kernels)
                 const bool timed = prof && it < opts->profile && ntimed < kEventPool / kK;
+                if (!timed && graphs_on) break;
                 auto e0 = [&](int j) { return timed ? c->ev0[kK * ntimed + j] : nullptr; };
                 auto e1 = [&](int j) { return timed ? c->ev1[kK * ntimed + j] : nullptr; };
-                launch_bicg_p(geo, a, it, s, e0(0), e1(0));
-                if (poly) {
-                    // ph = 2p - Ap, v = A ph; sh = 2s - As, t = A sh (right preconditioning)
-                    launch_spmv_v(geo, a, it, s, e0(1), e1(1), kOpPre);
-                    SolveArgs av = a;
-                    av.p = a.ph;
-                    launch_spmv_v(geo, av, it, s, e0(2), e1(2));
-                    launch_spass(geo, a, it, s, e0(3), e1(3));
-                    launch_spmv_t(geo, a, it, s, e0(4), e1(4), kOpPre);
-                    launch_spmv_t(geo, a, it, s, e0(5), e1(5), kOpTP);
-                } else {
-                    launch_spmv_v(geo, a, it, s, e0(2), e1(2));
-                    launch_spmv_t(geo, a, it, s, e0(5), e1(5));
-                }
-                launch_bicg_update(geo, a, it, s, e0(6), e1(6));
+                launch_iteration(it, e0, e1);
                 if (timed) timed_first[ntimed++] = it == 0;
+            }
+            if (k < nl) {  // the untimed rest of the run as one graph
+                launch_chain(it, nl - k);
+                it += nl - k;
             }
             launch_publish_state(w.st.p, S, w.all_done.p, w.d_h_st, w.d_h_flag, s);
             DMRI_HIP(hipStreamSynchronize(s));
@@ -1604,33 +1730,50 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     }
     K = std::max(1, std::min(K, std::max(N, 1)));
     const bool keep_all = K >= N;
-    // A checkpoint holds three states.  With the extrapolated guesses (initial_guess 1-3) they are
-    // the whole history of the next step, so a recomputed segment repeats the forward sweep bit
-    // for bit.  The projection guess (the default) restarts its history from them, so recomputed
-    // states agree with the first sweep's to the solver tolerance.  The backward sweep keeps its
-    // own projection history of adjoint states.
+    // A checkpoint holds the state the next step restarts from: with the extrapolated guesses
+    // (initial_guess 1-3) three states, with the projection guess (the default) the whole history
+    // of up to 12 states (their Are / Jg products are formed again when a segment is recomputed).
+    // Either way a recomputed segment repeats the forward sweep bit for bit, so it costs what the
+    // forward sweep cost.  When the full histories of every checkpoint would take more than a quarter
+    // of the free device memory, a checkpoint keeps the newest three states of the history and the
+    // recomputed states agree with the first sweep's to the solver tolerance.  The backward sweep
+    // keeps its own projection history of adjoint states.
     const bool proj = opts->initial_guess == 0;
     run.setup(c, 1, &seq->q, g, false, seq, opts, &st);
     const int nck = (N + K - 1) / K;
     const int klast = keep_all ? 0 : (nck - 1) * K;  // first step of the last segment
-    // checkpoints (x^k, x^(k-1), x^(k-2)) at k = 0, K, 2K, ...; a segment holds x^k0 .. x^k1
+    const size_t vb = sizeof(double2) * (size_t)n;
+    int ckm = 3;  // states per checkpoint
+    if (!keep_all && run.fh.m > 0) {
+        size_t fr = 0, tot = 0;
+        DMRI_HIP(hipMemGetInfo(&fr, &tot));
+        if ((double)nck * run.fh.m * (double)vb <= 0.25 * (double)fr) ckm = run.fh.m;
+    }
+    // checkpoints at k = 0, K, 2K, ...: the history newest first (x^k, x^(k-1), ...); a segment
+    // holds x^k0 .. x^k1
     DevArray<double2> ck, seg, mu, lam, lam2;
-    ck.alloc((size_t)std::max(nck, 1) * 3 * n);
+    std::vector<int> ck_count(std::max(nck, 1), 0);
+    ck.alloc((size_t)std::max(nck, 1) * ckm * n);
     seg.alloc((size_t)(std::min(K, std::max(N, 1)) + 1) * n);
     mu.alloc(n);
     lam.alloc(n);
     lam2.alloc(n);
-    const size_t vb = sizeof(double2) * (size_t)n;
     auto segp = [&](int j) { return seg.p + (size_t)j * n; };
+    auto ckp = [&](int j, int b) { return ck.p + ((size_t)j * ckm + b) * n; };
     // forward sweep
     for (int k = 0; k < N; ++k) {
         if (k % K == 0 && !keep_all) {
             const int j = k / K;
-            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j) * n, run.xc, vb, hipMemcpyDeviceToDevice, s));
-            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j + 2) * n, run.state_back(k > 1 ? 2 : 0), vb,
-                                    hipMemcpyDeviceToDevice, s));
-            DMRI_HIP(hipMemcpyAsync(ck.p + (size_t)(3 * j + 1) * n, run.state_back(k > 0 ? 1 : 0), vb,
-                                    hipMemcpyDeviceToDevice, s));
+            if (run.fh.m > 0) {
+                ck_count[j] = std::min(ckm, (int)run.fh.ys.size());
+                for (int b = 0; b < ck_count[j]; ++b)
+                    DMRI_HIP(hipMemcpyAsync(ckp(j, b), run.state_back(b), vb, hipMemcpyDeviceToDevice, s));
+            } else {
+                ck_count[j] = 3;
+                DMRI_HIP(hipMemcpyAsync(ckp(j, 0), run.xc, vb, hipMemcpyDeviceToDevice, s));
+                DMRI_HIP(hipMemcpyAsync(ckp(j, 2), run.state_back(k > 1 ? 2 : 0), vb, hipMemcpyDeviceToDevice, s));
+                DMRI_HIP(hipMemcpyAsync(ckp(j, 1), run.state_back(k > 0 ? 1 : 0), vb, hipMemcpyDeviceToDevice, s));
+            }
         }
         // the last segment's states stay in the segment buffer, so the backward sweep starts
         // without recomputing it
@@ -1669,6 +1812,10 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
         assemble_interface(c, 1.0, nullptr, unitI.p, miss.p);
         idiff = unitI.p;
     }
+    // the contraction's four values per entry, scaled and in solver order, read contiguously each step
+    DevArray<double4> adjv;
+    adjv.alloc(c->nnz);
+    launch_adj_values(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p, mdiff, kdiff, idiff, w.vals.p, adjv.p, s);
     launch_adj_init_lambda(n, c->sw.p, w.sc.p, 1.0 / c->volume, lam.p, s);
     DMRI_HIP(hipMemsetAsync(mu.p, 0, vb, s));
     // per-step contraction partials stay on the device and are summed on the host at the end
@@ -1693,26 +1840,34 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
         }
     }
     double2 *mu_cur = mu.p;
+    // Segment j's states x^k0 .. x^k1 into dst, recomputed by stepper r from checkpoint j; returns
+    // the Krylov iterations taken.
+    auto recompute = [&](Stepper &r, int j, double2 *dst) -> int64_t {
+        const int k0 = j * K, k1 = std::min(N, k0 + K);
+        const hipStream_t rs = r.s;
+        if (r.fh.m > 0) {
+            const double2 *states[kMaxHist];
+            for (int b = 0; b < ck_count[j]; ++b) states[b] = ckp(j, b);
+            r.reseed_projection(states, ck_count[j]);
+        } else {
+            DMRI_HIP(hipMemcpyAsync(r.xc, ckp(j, 0), vb, hipMemcpyDeviceToDevice, rs));
+            DMRI_HIP(hipMemcpyAsync(r.xp, ckp(j, 1), vb, hipMemcpyDeviceToDevice, rs));
+            DMRI_HIP(hipMemcpyAsync(r.xq, ckp(j, 2), vb, hipMemcpyDeviceToDevice, rs));
+        }
+        DMRI_HIP(hipMemcpyAsync(dst, r.xc, vb, hipMemcpyDeviceToDevice, rs));
+        const int64_t before = r.st->iters_total;
+        for (int k = k0; k < k1; ++k) {
+            r.step(k);
+            DMRI_HIP(hipMemcpyAsync(dst + (size_t)(k - k0 + 1) * n, r.xc, vb, hipMemcpyDeviceToDevice, rs));
+        }
+        DMRI_HIP(hipStreamSynchronize(rs));
+        return r.st->iters_total - before;
+    };
     for (int j = nck - 1; j >= 0; --j) {
         const int k0 = j * K, k1 = std::min(N, k0 + K);
         if (!keep_all && j != nck - 1) {
-            if (run.fh.m > 0) {
-                const double2 *states[3] = {ck.p + (size_t)(3 * j) * n, ck.p + (size_t)(3 * j + 1) * n,
-                                            ck.p + (size_t)(3 * j + 2) * n};
-                run.reseed_projection(states, std::min(3, k0 + 1));
-            } else {
-                DMRI_HIP(hipMemcpyAsync(run.xc, ck.p + (size_t)(3 * j) * n, vb, hipMemcpyDeviceToDevice, s));
-                DMRI_HIP(hipMemcpyAsync(run.xp, ck.p + (size_t)(3 * j + 1) * n, vb, hipMemcpyDeviceToDevice, s));
-                DMRI_HIP(hipMemcpyAsync(run.xq, ck.p + (size_t)(3 * j + 2) * n, vb, hipMemcpyDeviceToDevice, s));
-            }
-            DMRI_HIP(hipMemcpyAsync(segp(0), run.xc, vb, hipMemcpyDeviceToDevice, s));
-            const int64_t before = st.iters_total;
-            for (int k = k0; k < k1; ++k) {
-                run.step(k);
-                DMRI_HIP(hipMemcpyAsync(segp(k - k0 + 1), run.xc, vb, hipMemcpyDeviceToDevice, s));
-                ++recomputed;
-            }
-            it_rec += st.iters_total - before;
+            it_rec += recompute(run, j, seg.p);
+            recomputed += k1 - k0;
         }
         for (int k = k1 - 1; k >= k0; --k) {
             const int base = keep_all ? 0 : k0;
@@ -1722,11 +1877,9 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
                 mu_cur = run.solve_projected(k, lam.p, bh, mu_cur);
             else
                 run.solve_given(k, lam.p, mu_cur);
-            nparts_of[k] = launch_adj_contract(n, c->srow.p, c->scol.p, c->emap.p, w.sc.p,
-                                               c->ncomp == 2 ? c->scomp.p : nullptr, mdiff, kdiff, idiff, w.vals.p,
-                                               mu_cur,
-                                               segp(k - base), segp(k - base + 1), theta,
-                                               parts.p + (size_t)k * kMaxGrid * 14, s);
+            nparts_of[k] = launch_adj_contract_rows(n, c->srow.p, c->scol.p, c->ncomp == 2 ? c->scomp.p : nullptr,
+                                                    adjv.p, mu_cur, segp(k - base), segp(k - base + 1), theta,
+                                                    parts.p + (size_t)k * kMaxGrid * 14, s);
             if (bh.m > 0)
                 run.apply_B_record(k, mu_cur, lam2.p, bh);  // lambda^k = B_k mu_k, mu_k into the history
             else
@@ -1943,9 +2096,10 @@ void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_map
             slot = -1.0;
             if (m == kMapSR || m == kMapTile) continue;
             if (m == kMapWave && G != 1) continue;
-            if (m == kMapPair && (S < 2 || G > 4 || G * pair_sp > 64 || (uint64_t)n * S * 16u >= (1ull << 32)))
-                continue;
-            if (m != kMapWave && m != kMapPair && G * S > 64) continue;
+            const bool pairs = m == kMapPair;
+            if (pairs && (S < 2 || G > 4 || G * pair_sp > 64)) continue;
+            if (m == kMapPair && (uint64_t)n * S * 16u >= (1ull << 32)) continue;
+            if (m != kMapWave && !pairs && G * S > 64) continue;
             SolveGeom geo = make_geom(n, G, S, m);
             fit_spmv_grid(geo, per_gdir);
             const size_t np = (size_t)std::max(geo.grid, geo.vgrid) * S * kNV;
@@ -2186,6 +2340,7 @@ void batch_advance_impl(dmri_batch *b, int nsteps, dmri_stats *stats) {
         st.iters_total += x.iters_total;
         st.iters_max = std::max(st.iters_max, x.iters_max);
         st.restarts += x.restarts;
+        st.graph_launches += x.graph_launches;
         st.relres_max = std::max(st.relres_max, x.relres_max);
         st.spmv_ms += x.spmv_ms;
         st.spmv_launches += x.spmv_launches;

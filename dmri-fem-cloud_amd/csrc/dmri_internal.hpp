@@ -415,11 +415,14 @@ void launch_gm_axpy(int64_t total, const double2 *z, double2 *x, hipStream_t s);
 // st[s].done = 0 for every signal and *flag = 0, the rest of the states kept
 void launch_reset_state_flags(KrylovState *st, int S, int *flag, hipStream_t s);
 
-// adjoint contraction of one step; returns the number of block partials (14 doubles each)
-int launch_adj_contract(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
-                        const int *dof_comp, const double *mm, const double *kk, const double *gi, const double *vals,
-                        const double2 *mu, const double2 *xn, const double2 *xn1, double theta, double *parts,
-                        hipStream_t s);
+// the contraction's values in solver entry order (s_i s_j K, s_i s_j M, s_i s_j I, Jg), once per sweep,
+// and the per-step contraction over them (G lanes per row); returns the number of block partials
+void launch_adj_values(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
+                       const double *mm, const double *kk, const double *gi, const double *vals, double4 *adjv,
+                       hipStream_t s);
+int launch_adj_contract_rows(int64_t n, const int *rowptr, const int *col, const int *dof_comp, const double4 *adjv,
+                             const double2 *mu, const double2 *xn, const double2 *xn1, double theta, double *parts,
+                             hipStream_t s);
 void launch_adj_init_lambda(int64_t n, const double *w, const double *sc, double inv_vol, double2 *lam,
                             hipStream_t s);
 

@@ -3306,42 +3306,67 @@ __global__ void __launch_bounds__(kThreads) proj_combine_kernel(ProjArgs p, int 
     }
 }
 
+// The contraction's value arrays in the solver's entry order, each entry's four values together and
+// scaled like the solver's operator: adjv[e] = (s_i s_j K, s_i s_j M, s_i s_j I, Jg) (Jg already
+// scaled in vals).  Built once per adjoint sweep; one thread per row.
+__global__ void __launch_bounds__(kThreads) adj_values_kernel(int64_t n, const int *rowptr, const int *col,
+                                                              const int *emap, const double *sc, const double *mm,
+                                                              const double *kk, const double *gi,
+                                                              const double *vals, double4 *adjv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double si = sc[i];
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+        const double sij = si * sc[col[e]];
+        const int o = emap[e];
+        adjv[e] = make_double4(sij * kk[o], sij * mm[o], sij * gi[o], vals[2 * (int64_t)e + 1]);
+    }
+}
+
 // Adjoint contraction of one theta step (one signal).  With the scaled states x_n, x_n1, the
 // scaled adjoint mu and y = (1-theta) x_n + theta x_n1, every row i adds (complex products,
-// no conjugation; s_ij = sc_i sc_j):
+// no conjugation; s_ij = sc_i sc_j, the values from adjv):
 //   k[0..3]  mu_i sum_j s_ij K_ij y_j  into compartment 0 / 1 of row i
 //   k[4..7]  mu_i sum_j s_ij M_ij y_j  into compartment 0 / 1
 //   k[8,9]   mu_i sum_j s_ij I_ij y_j
-//   k[10,11] mu_i sum_j Jg_ij x_n,j    (Jg: the scaled g.J of vals[2e+1])
+//   k[10,11] mu_i sum_j Jg_ij x_n,j    (Jg: the scaled g.J)
 //   k[12,13] mu_i sum_j Jg_ij x_n1,j
+// G lanes per row take every G-th entry (coalesced entry reads) and add their sums with shuffles.
 // Block partials go to parts[block * 14 + k] and are summed on the host in block order.
-__global__ void __launch_bounds__(kThreads) adj_contract_kernel(int64_t n, const int *rowptr, const int *col,
-                                                                const int *emap, const double *sc,
-                                                                const int *dof_comp, const double *mm,
-                                                                const double *kk,
-                                                                const double *gi, const double *vals,
-                                                                const double2 *mu, const double2 *xn,
-                                                                const double2 *xn1, double theta, double *parts) {
+template <int G>
+__global__ void __launch_bounds__(kThreads) adj_contract_rows_kernel(int64_t n, const int *rowptr, const int *col,
+                                                                     const int *dof_comp, const double4 *adjv,
+                                                                     const double2 *mu, const double2 *xn,
+                                                                     const double2 *xn1, double theta, double *parts) {
     double red[14];
 #pragma unroll
     for (int k = 0; k < 14; ++k) red[k] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double si = sc[i];
+    const int g = threadIdx.x % G;
+    const int rpb = blockDim.x / G;
+    for (int64_t i = (int64_t)blockIdx.x * rpb + threadIdx.x / G; i < n; i += (int64_t)gridDim.x * rpb) {
         double2 aK = make_double2(0, 0), aM = make_double2(0, 0), aI = make_double2(0, 0);
         double2 aJ0 = make_double2(0, 0), aJ1 = make_double2(0, 0);
-        for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+        const int e1 = rowptr[i + 1];
+        for (int e = rowptr[i] + g; e < e1; e += G) {
             const int j = col[e];
-            const double sij = si * sc[j];
+            const double4 v = adjv[e];
             const double2 x0 = xn[j], x1 = xn1[j];
             const double2 y = make_double2((1.0 - theta) * x0.x + theta * x1.x, (1.0 - theta) * x0.y + theta * x1.y);
-            const int o = emap[e];
-            const double vk = sij * kk[o], vm = sij * mm[o], vi = sij * gi[o], vj = vals[2 * (int64_t)e + 1];
-            aK.x += vk * y.x; aK.y += vk * y.y;
-            aM.x += vm * y.x; aM.y += vm * y.y;
-            aI.x += vi * y.x; aI.y += vi * y.y;
-            aJ0.x += vj * x0.x; aJ0.y += vj * x0.y;
-            aJ1.x += vj * x1.x; aJ1.y += vj * x1.y;
+            aK.x += v.x * y.x; aK.y += v.x * y.y;
+            aM.x += v.y * y.x; aM.y += v.y * y.y;
+            aI.x += v.z * y.x; aI.y += v.z * y.y;
+            aJ0.x += v.w * x0.x; aJ0.y += v.w * x0.y;
+            aJ1.x += v.w * x1.x; aJ1.y += v.w * x1.y;
         }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            aK.x += __shfl_xor(aK.x, off, 64); aK.y += __shfl_xor(aK.y, off, 64);
+            aM.x += __shfl_xor(aM.x, off, 64); aM.y += __shfl_xor(aM.y, off, 64);
+            aI.x += __shfl_xor(aI.x, off, 64); aI.y += __shfl_xor(aI.y, off, 64);
+            aJ0.x += __shfl_xor(aJ0.x, off, 64); aJ0.y += __shfl_xor(aJ0.y, off, 64);
+            aJ1.x += __shfl_xor(aJ1.x, off, 64); aJ1.y += __shfl_xor(aJ1.y, off, 64);
+        }
+        if (g != 0) continue;
         const double2 m = mu[i];
         const int c = dof_comp ? dof_comp[i] : 0;
         const double2 pK = c_mul(m, aK), pM = c_mul(m, aM), pI = c_mul(m, aI);
@@ -3711,7 +3736,8 @@ SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
         throw Error(-1, "unknown SpMV mapping");
     int pair_sp = 1;
     while (2 * pair_sp < S) pair_sp *= 2;
-    if (mapping == kMapPair && (S < 2 || G > 4 || G * pair_sp > 64))
+    const bool pairs = mapping == kMapPair;
+    if (pairs && (S < 2 || G > 4 || G * pair_sp > 64))
         throw Error(-1, "the two-signals-per-lane mapping needs at least 2 signals, lanes 1, 2 or 4 and "
                         "lanes x signal pairs <= 64");
     if (mapping == kMapPair && (uint64_t)n * (uint64_t)S * 16u >= (1ull << 32))
@@ -3719,14 +3745,13 @@ SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
     if (mapping == kMapWave && G != 1) throw Error(-1, "the wave-per-row mapping takes one lane per signal (lanes 1)");
     if (mapping == kMapSR && S > kMaxSRSignals)
         throw Error(-1, "signals-in-registers mapping needs at most 8 signals per launch group");
-    if (mapping != kMapSR && mapping != kMapPair && G * S > 64)
-        throw Error(-1, "lanes per row times signals must not exceed 64");
+    if (mapping != kMapSR && !pairs && G * S > 64) throw Error(-1, "lanes per row times signals must not exceed 64");
     SolveGeom g;
     g.G = G;
     g.S = S;
     g.mapping = mapping;
     // the residual kernel always spreads signals over lanes
-    g.G_rhs = mapping == kMapSR || mapping == kMapPair ? std::max(1, std::min(G, 64 / S)) : G;
+    g.G_rhs = mapping == kMapSR || pairs ? std::max(1, std::min(G, 64 / S)) : G;
     while (g.G_rhs & (g.G_rhs - 1)) g.G_rhs &= g.G_rhs - 1;
     g.rhs_rpb = std::max(1, kThreads / (g.G_rhs * S));
     g.rhs_threads = g.rhs_rpb * g.G_rhs * S;
@@ -3737,7 +3762,7 @@ SolveGeom make_geom(int64_t n, int G, int S, int mapping) {
         g.spmv_threads = kThreads;
     }
     if (mapping == kMapTile) g.tile_rows = g.rpb;
-    if (mapping == kMapPair) {  // G x pair_sp lanes per row
+    if (pairs) {  // G x pair_sp lanes per row
         g.pair_sp = pair_sp;
         g.rpb = kThreads / (G * pair_sp);
         g.spmv_threads = kThreads;
@@ -4054,15 +4079,19 @@ void launch_init_x(int64_t n, int S, const double *sc, double2 *x, hipStream_t s
     init_x_kernel<<<grid_for(n * S, 256), 256, 0, s>>>(n, S, sc, x);
     check_launch();
 }
-int launch_adj_contract(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
-                        const int *dof_comp, const double *mm, const double *kk, const double *gi, const double *vals,
-                        const double2 *mu, const double2 *xn, const double2 *xn1, double theta, double *parts,
-                        hipStream_t s) {
-    unsigned g = grid_for(n, kThreads);
-    if (g > (unsigned)kMaxGrid) g = kMaxGrid;
+void launch_adj_values(int64_t n, const int *rowptr, const int *col, const int *emap, const double *sc,
+                       const double *mm, const double *kk, const double *gi, const double *vals, double4 *adjv,
+                       hipStream_t s) {
+    adj_values_kernel<<<grid_for(n, kThreads), kThreads, 0, s>>>(n, rowptr, col, emap, sc, mm, kk, gi, vals, adjv);
+    check_launch();
+}
+int launch_adj_contract_rows(int64_t n, const int *rowptr, const int *col, const int *dof_comp, const double4 *adjv,
+                             const double2 *mu, const double2 *xn, const double2 *xn1, double theta, double *parts,
+                             hipStream_t s) {
+    constexpr int G = 8;  // lanes per row
+    unsigned g = (unsigned)std::min<int64_t>((n + kThreads / G - 1) / (kThreads / G), kMaxGrid);
     if (g == 0) g = 1;
-    adj_contract_kernel<<<g, kThreads, 0, s>>>(n, rowptr, col, emap, sc, dof_comp, mm, kk, gi, vals, mu, xn, xn1,
-                                                theta, parts);
+    adj_contract_rows_kernel<G><<<g, kThreads, 0, s>>>(n, rowptr, col, dof_comp, adjv, mu, xn, xn1, theta, parts);
     check_launch();
     return (int)g;
 }

@@ -50,14 +50,14 @@ class _Opts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxit", ctypes.c_int32),
                 ("profile", ctypes.c_int32), ("lanes_per_row", ctypes.c_int32), ("initial_guess", ctypes.c_int32),
                 ("mapping", ctypes.c_int32), ("preconditioner", ctypes.c_int32), ("krylov", ctypes.c_int32),
-                ("restart", ctypes.c_int32)]
+                ("restart", ctypes.c_int32), ("graphs", ctypes.c_int32)]
 
 
 class _Stats(ctypes.Structure):
     _fields_ = [("iters_total", ctypes.c_int64), ("iters_max", ctypes.c_int32), ("restarts", ctypes.c_int32),
                 ("relres_max", ctypes.c_double), ("solve_ms", ctypes.c_double), ("spmv_ms", ctypes.c_double),
                 ("spmv_launches", ctypes.c_int64), ("spmv_bytes", ctypes.c_double),
-                ("precond_used", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("precond_used", ctypes.c_int32), ("graph_launches", ctypes.c_int32),
                 ("kern_ms", ctypes.c_double * 7), ("kern_launches", ctypes.c_int64 * 7),
                 ("kern_bytes", ctypes.c_double * 7)]
 
@@ -255,7 +255,8 @@ class Context:
     # -- solve --------------------------------------------------------------------------
     @staticmethod
     def _seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes, guess=0,
-                  mapping=0, F_explicit=None, precond=0, u0=None, nv=None, F_implicit=None, krylov=0, restart=0):
+                  mapping=0, F_explicit=None, precond=0, u0=None, nv=None, F_implicit=None, krylov=0, restart=0,
+                  graphs=0):
         fA = np.ascontiguousarray(f_implicit, dtype=np.float64)
         fB = np.ascontiguousarray(f_explicit, dtype=np.float64)
         if fA.shape != fB.shape:
@@ -287,6 +288,7 @@ class Context:
         o.preconditioner = int(precond)
         o.krylov = Context.KRYLOV[krylov] if isinstance(krylov, str) else int(krylov)
         o.restart = int(restart)
+        o.graphs = int(graphs)
         return s, o, (fA, fB, FB, U0, FA)
 
     def _nv(self, u0):
@@ -308,7 +310,7 @@ class Context:
 
     def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
               profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None, snap_every=0,
-              F_implicit=None, krylov=0, restart=0):
+              F_implicit=None, krylov=0, restart=0, graphs=0):
         """precond: 0 = Jacobi + degree-1 polynomial (default), 1 = Jacobi only.  guess: Krylov start
         of each step, 0 = residual-minimising projection on the last states (default), 1 = previous
         state, 2 = linear, 3 = quadratic extrapolation.  u0: initial state per mesh vertex (real part
@@ -316,7 +318,8 @@ class Context:
         (complex, DOF order) after every step whose 0-based index is a multiple of it, shape
         (nsnap, ndof)."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes,
-                                    guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov, restart)
+                                    guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov, restart,
+                                    graphs)
         snaps = self._snapshots(s, snap_every, 1)
         sig = np.zeros(2)
         st = _Stats()
@@ -329,14 +332,15 @@ class Context:
 
     def solve_batched(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
                       profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None,
-                      snap_every=0, F_implicit=None, krylov=0, restart=0):
+                      snap_every=0, F_implicit=None, krylov=0, restart=0, graphs=0):
         """Signals for every (q[s], gdir[s]) on the shared pattern; returns complex array (nsig,).
         snap_every > 0: stats["snapshots"] has shape (nsnap, nsig, ndof), as in solve."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(gdirs, dtype=np.float64).reshape(-1, 3),
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
-                                    lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov, restart)
+                                    lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov,
+                                    restart, graphs)
         snaps = self._snapshots(s, snap_every, qs.shape[0])
         sig = np.zeros(2 * qs.shape[0])
         st = _Stats()
@@ -350,14 +354,15 @@ class Context:
 
     def batch(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
               profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None, group=0,
-              F_implicit=None, krylov=0, restart=0):
+              F_implicit=None, krylov=0, restart=0, graphs=0):
         """Resumable batched solve (dmri_batch_begin): returns a Batch whose advance(k) runs k more
         theta steps of every signal and whose signals() gives the signals of the current states."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(gdirs, dtype=np.float64).reshape(-1, 3),
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
-                                    lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov, restart)
+                                    lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov,
+                                    restart, graphs)
         h = ctypes.c_void_p()
         _check(LIB.dmri_batch_begin(self._h, qs.shape[0], _ptr(qs, ctypes.c_double), _ptr(g, ctypes.c_double),
                                     ctypes.byref(s), ctypes.byref(o), int(group), ctypes.byref(h)))
@@ -371,7 +376,7 @@ class Context:
         us = np.zeros(36)
         _check(LIB.dmri_tune(self._h, int(nsig), 1 if per_gdir else 0, ctypes.byref(bl), ctypes.byref(bm),
                              _ptr(us, ctypes.c_double)))
-        times = {(m, l): us[(m - 1) * 6 + i] for m in (1, 2, 3, 4, 5, 6) for i, l in enumerate((1, 2, 4, 8, 16, 32))
+        times = {(m, l): us[(m - 1) * 6 + i] for m in range(1, 7) for i, l in enumerate((1, 2, 4, 8, 16, 32))
                  if us[(m - 1) * 6 + i] >= 0}
         return (bm.value, bl.value), times
 
@@ -386,10 +391,11 @@ class Context:
         _check(LIB.dmri_set_tuning(self._h, int(nsig), 1 if per_gdir else 0, int(lanes), int(mapping)))
 
     def adjoint(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
-                checkpoint_stride=0, lanes=0, guess=0, mapping=0, precond=0, u0=None, krylov=0, restart=0):
+                checkpoint_stride=0, lanes=0, guess=0, mapping=0, precond=0, u0=None, krylov=0, restart=0,
+                graphs=0):
         """Signal and its adjoint gradients (see dmri_adjoint): returns (signal, grads dict, stats)."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, False, lanes,
-                                    guess, mapping, None, precond, u0, self._nv(u0), None, krylov, restart)
+                                    guess, mapping, None, precond, u0, self._nv(u0), None, krylov, restart, graphs)
         ao = _AdjOpts()
         ao.checkpoint_stride = int(checkpoint_stride)
         n = s.nsteps

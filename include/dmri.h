@@ -250,6 +250,12 @@ typedef struct {
                                count Arnoldi steps for GMRES. */
     int32_t restart;        /* GMRES restart length, 0 = 30, at most 64 (m + 1 basis vectors per signal
                                are kept in HBM) */
+    int32_t graphs;         /* 1: BiCGStab launches each run of iterations between two convergence
+                               checks as hipGraphs of 2^k iterations, each captured once per (kernel
+                               arguments, first iteration, length) and replayed (the step's
+                               coefficients go to a fixed device slot and the projection guess iterates
+                               in a fixed buffer so that the arguments repeat); results are identical
+                               to 0 (plain launches); timed iterations (profile) go out one by one */
 } dmri_solver_opts;
 
 typedef struct {
@@ -264,7 +270,8 @@ typedef struct {
     int32_t precond_used;   /* preconditioner of the last launch group: 0 = Jacobi only, 1 = the
                                polynomial of the scaled operator, 2 = the polynomial of its real part,
                                3 = the polynomial in the variables scaled by the complex diagonal */
-    int32_t reserved;
+    int32_t graph_launches; /* hipGraph launches of captured iteration runs (graphs = 1; 0 when the
+                               runtime refused the capture and the iterations went out one by one) */
     /* Every kernel of the timed Krylov iterations (profile > 0), by its own dispatch timestamps:
      * [0] bicg_p (p = r + beta (p - omega v)), [1] preconditioner SpMV on p (ph), [2] v = A ph (or A p),
      * [3] the s-pass (alpha, s = r - alpha v), [4] preconditioner SpMV on s (sh), [5] t = A sh (or A s),

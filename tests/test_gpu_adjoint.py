@@ -198,3 +198,19 @@ def test_adjoint_with_custom_initial_state_matches_fd():
     fd = (_ctx(kappa=COEF["kappa"] + d).solve(FA, FB, K, Q, G, rtol=1e-14, guess=3, u0=u0)[0].real -
           _ctx(kappa=COEF["kappa"] - d).solve(FA, FB, K, Q, G, rtol=1e-14, guess=3, u0=u0)[0].real) / (2 * d)
     assert _close(g["kappa"], fd), (g["kappa"], fd)
+
+
+def test_projection_checkpoints_recompute_bit_for_bit():
+    """With the projection guess a checkpoint holds the solver's whole history, so the recomputed
+    segments repeat the forward sweep: the gradients equal the stored-states run's bit for bit."""
+    stride = int(np.ceil(np.sqrt(len(FA))))
+    print(f"[adjoint] projection checkpoints, stride {stride}", flush=True)
+    s1, g1, _ = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14, checkpoint_stride=stride)
+    s0, g0, _ = _ctx().adjoint(FA, FB, K, Q, G, rtol=1e-14, checkpoint_stride=10 ** 6)
+    assert g1["recomputed_steps"] > 0 and g0["recomputed_steps"] == 0
+    assert s1 == s0
+    for k in ("D", "invT2", "f_implicit", "f_explicit"):
+        np.testing.assert_array_equal(g1[k], g0[k])
+    assert g1["kappa"] == g0["kappa"] and g1["q"] == g0["q"]
+    assert g1["iters_forward"] == g0["iters_forward"] and g1["iters_backward"] == g0["iters_backward"]
+    assert g1["iters_recompute"] <= g1["iters_forward"]
