@@ -2,6 +2,7 @@
 """Text summaries of rocprofv3 CSV output for libdmri runs.
 
   python3 prof_summary.py trace    <kernel_trace.csv>      [--sample K]
+  python3 prof_summary.py steps    <kernel_trace.csv>      [--first KERNEL]
   python3 prof_summary.py counters <counter_collection.csv> [--sample K] [--copy-bytes B]
   python3 prof_summary.py traffic  <FETCH_SIZE csv> <WRITE_SIZE csv> --copy-bytes B
                                    --nnz NNZ --ndof N --mapping M L --workload TEXT --command CMD ... --out F
@@ -12,6 +13,9 @@ bicg_p, s-pass, update) it also prints the mean over the first K dispatches afte
 kernel (the first K Krylov iterations of each theta step, where every signal of the launch group is
 still active): the same launches that the solver's sampled kernel events time
 (dmri_solver_opts.profile = K).
+
+steps: per theta step (from one dispatch of its first kernel to the next) the span, the time the
+kernels cover and the idle rest, as medians over the steps.
 
 counters: per kernel the mean counter value per dispatch (KiB), for all dispatches and for the
 same first-K sample.  With --copy-bytes B (the size given to dmri_copy_bandwidth) the copy kernel
@@ -88,6 +92,39 @@ def trace(path, k):
         sm = f"{statistics.mean(sv):14.2f}" if sv else f"{'':14s}"
         print(f"{name[:44]:44s} {len(v):7d} {statistics.mean(v):9.2f} {statistics.median(v):9.2f} {g:7.2f} "
               f"{len(sv) if sv else '':>12} {sm}")
+
+
+def steps(path, first):
+    """Per theta step, from one dispatch of the step's first kernel to the next: the span, the time
+    covered by kernels (the union of their intervals), idle = span - covered, and the dispatches.
+    Spans over 5x the median (the gap between two solves) are left out; prints the medians."""
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    starts = [i for i, r in enumerate(rows) if base(r["Kernel_Name"]) == first]
+    rec = []
+    for a, b in zip(starts, starts[1:]):
+        t0, t1 = int(rows[a]["Start_Timestamp"]), int(rows[b]["Start_Timestamp"])
+        covered, cs, ce = 0, None, None
+        for r in rows[a:b]:
+            s, e = int(r["Start_Timestamp"]), min(int(r["End_Timestamp"]), t1)
+            if ce is None or s > ce:
+                covered += 0 if ce is None else ce - cs
+                cs, ce = s, e
+            else:
+                ce = max(ce, e)
+        covered += ce - cs
+        rec.append(((t1 - t0) / 1e3, covered / 1e3, (t1 - t0 - covered) / 1e3, b - a))
+    if not rec:
+        print(f"no two dispatches of {first}")
+        return None
+    med_span = statistics.median(r[0] for r in rec)
+    rec = [r for r in rec if r[0] <= 5 * med_span]
+    out = {"steps": len(rec), "span_us": statistics.median(r[0] for r in rec),
+           "busy_us": statistics.median(r[1] for r in rec), "idle_us": statistics.median(r[2] for r in rec),
+           "idle_mean_us": statistics.mean(r[2] for r in rec), "dispatches": statistics.median(r[3] for r in rec)}
+    print(f"theta steps {out['steps']} (from {first}): median span {out['span_us']:.1f} us, kernels "
+          f"{out['busy_us']:.1f} us, idle {out['idle_us']:.1f} us (mean {out['idle_mean_us']:.1f}), "
+          f"{out['dispatches']:.0f} dispatches")
+    return out
 
 
 def counters(path, k, copy_bytes):
@@ -195,7 +232,7 @@ def traffic(fetch_csv, write_csv, copy_bytes, out, meta, mapping):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "counters", "traffic"])
+    ap.add_argument("mode", choices=["trace", "steps", "counters", "traffic"])
     ap.add_argument("csv")
     ap.add_argument("csv2", nargs="?")
     ap.add_argument("--sample", type=int, default=2)
@@ -206,9 +243,12 @@ def main():
     ap.add_argument("--workload", default="")
     ap.add_argument("--command", action="append", default=[])
     ap.add_argument("--out")
+    ap.add_argument("--first", default="rhs_history_kernel", help="steps: the kernel that opens a theta step")
     a = ap.parse_args()
     if a.mode == "trace":
         trace(a.csv, a.sample)
+    elif a.mode == "steps":
+        steps(a.csv, a.first)
     elif a.mode == "counters":
         counters(a.csv, a.sample, a.copy_bytes)
     else:

@@ -205,3 +205,23 @@ def test_facet_cells_and_interface_kappa_against_brute_force():
         L.facet_cells(m.tets, np.array([[0, 1, 2 + m.nv - 3]]))
     boundary = [f for f, cs in faces.items() if len(cs) == 1][:5]
     assert np.all(L.facet_cells(m.tets, np.array(boundary))[:, 1] == -1)
+
+
+def test_profile_step_summary_idle_from_synthetic_trace(tmp_path):
+    """prof_summary steps: span, kernel-covered time and idle of each theta step from a synthetic
+    kernel trace (two steps of 3 kernels with known gaps, then a long pause that is left out)."""
+    import prof_summary as ps
+
+    rows = [("rhs_history_kernel<1, false>", 0, 10), ("residual_kernel<1, false>", 12, 20),
+            ("bicg_p_kernel", 25, 40), ("rhs_history_kernel<1, false>", 50, 60),
+            ("residual_kernel<1, false>", 61, 70), ("bicg_p_kernel", 70, 95),
+            ("rhs_history_kernel<1, false>", 100, 110), ("rhs_history_kernel<1, false>", 900_000, 900_010)]
+    p = tmp_path / "kt.csv"
+    with open(p, "w") as fh:
+        fh.write("Kernel_Name,Start_Timestamp,End_Timestamp\n")
+        for n, s, e in rows:
+            fh.write(f"\"void dmri::{n}(dmri::SolveArgs, int)\",{s * 1000},{e * 1000}\n")
+    out = ps.steps(str(p), "rhs_history_kernel")
+    # steps: [0, 50): covered 10 + 8 + 15 = 33, idle 17; [50, 100): covered 10 + 34 = 44, idle 6
+    assert out["steps"] == 2
+    assert out["span_us"] == 50.0 and out["busy_us"] == 38.5 and out["idle_us"] == 11.5
