@@ -2069,6 +2069,9 @@ void install_tuning(dmri_ctx *c, int S, bool per_gdir, int mapping, int lanes) {
 // signals-in-registers and tile-compressed mappings have no preconditioner kernels) for S signals on
 // the assembled pattern, and remember the fastest for later solves of this context.
 // us[6*6]: [mapping-1][log2 lanes] microseconds per iteration's four SpMVs, -1 where not valid.
+// Tuner timings: us[(mapping - 1) * 6 + log2(lanes)] for mappings 1..kMapPair and lanes 1..32
+constexpr int kTuneSlots = kMapPair * 6;
+
 void tune_impl(dmri_ctx *c, int S, bool per_gdir, int *best_lanes, int *best_mapping, double *us) {
     require(c->assembled, DMRI_ERR_STATE, "dmri_tune: call dmri_assemble first");
     require(S >= 1 && S <= kMaxSignals, DMRI_ERR_ARG, "dmri_tune: nsig must lie in [1, 64]");
@@ -2159,7 +2162,7 @@ void tune_db_impl(dmri_ctx *c, const char *path, int S, bool per_gdir, int *best
             }
         }
     }
-    std::vector<double> us(30);
+    std::vector<double> us(kTuneSlots);
     tune_impl(c, S, per_gdir, best_lanes, best_mapping, us.data());
     if (path && *path) {
         const double u = us[(*best_mapping - 1) * 6 + (int)std::log2((double)*best_lanes)];
