@@ -1266,6 +1266,28 @@ __device__ __forceinline__ double2 imag_coef_apply(const Acc<PG> &acc, const dou
     return make_double2(acc.a.x - cj.y, acc.a.y + cj.x);
 }
 
+// A row's (a + i c_s.J) x with the signal's coefficients folded into every entry (per-signal
+// directions): one complex sum instead of Acc<true>'s four, which frees registers in the SpMV
+// kernels that apply one coefficient set (the right-hand-side passes, which apply two, keep Acc).
+struct FoldAcc {
+    double2 a;
+    double cx, cy, cz;
+};
+__device__ __forceinline__ FoldAcc fold_acc(const double *coef, int s) {
+    return FoldAcc{make_double2(0.0, 0.0), coef[3 * s], coef[3 * s + 1], coef[3 * s + 2]};
+}
+template <int G, bool PG>
+__device__ __forceinline__ double2 imag_coef_apply(const FoldAcc &acc, const double *, int) {
+    return acc.a;
+}
+template <bool PG>
+__device__ __forceinline__ void acc_zero(Acc<PG> &acc) {
+    acc.a = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < (PG ? 3 : 1); ++d) acc.j[d] = make_double2(0.0, 0.0);
+}
+__device__ __forceinline__ void acc_zero(FoldAcc &acc) { acc.a = make_double2(0.0, 0.0); }
+
 // Inverse of the complex diagonal d = a_ii + i (c_s . j_ii) of the scaled operator at row i for
 // signal s: conj(d) times the hardware reciprocal of |d|^2 (an approximate inverse, but the same
 // one at every application, so the preconditioner stays one fixed linear operator).
@@ -1300,6 +1322,10 @@ __device__ __forceinline__ void group_reduce_acc(Acc<PG> &acc) {
 #pragma unroll
     for (int d = 0; d < (PG ? 3 : 1); ++d) group_reduce<G>(acc.j[d]);
 }
+template <int G, bool PG>
+__device__ __forceinline__ void group_reduce_acc(FoldAcc &acc) {
+    group_reduce<G>(acc.a);
+}
 
 // Accumulate row entries g, g+G, ... of x = X(col) (X(c) returns the gathered value).
 template <bool PG>
@@ -1323,14 +1349,21 @@ __device__ __forceinline__ void acc_entry(const double *vals, int e, double2 xv,
     }
 }
 
+// FoldAcc (per-signal directions): w = c_s.J of the entry, then (a + i w) x, six FMAs
+template <bool PG>
+__device__ __forceinline__ void acc_entry(const double *vals, int e, double2 xv, FoldAcc &acc) {
+    const double4 v = *reinterpret_cast<const double4 *>(vals + 4 * (int64_t)e);
+    const double w = fma(acc.cz, v.w, fma(acc.cy, v.z, acc.cx * v.y));
+    acc.a.x = fma(v.x, xv.x, fma(-w, xv.y, acc.a.x));
+    acc.a.y = fma(v.x, xv.y, fma(w, xv.x, acc.a.y));
+}
+
 // Accumulate row entries g, g+G, ... of x = X(col) (X(c) returns the gathered value).
 // Two entries per step, so each lane keeps two independent gathers in flight.
-template <int G, bool PG, class Gather>
+template <int G, bool PG, class Gather, class AccT>
 __device__ __forceinline__ void row_accumulate(const int *rowptr, const int *col, const double *vals, int64_t row,
-                                               int g, Gather X, Acc<PG> &acc) {
-    acc.a = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int d = 0; d < (PG ? 3 : 1); ++d) acc.j[d] = make_double2(0.0, 0.0);
+                                               int g, Gather X, AccT &acc) {
+    acc_zero(acc);
     const int e1 = rowptr[row + 1];
     int e = rowptr[row] + g;
     for (; e + G < e1; e += 2 * G) {
@@ -1345,12 +1378,10 @@ __device__ __forceinline__ void row_accumulate(const int *rowptr, const int *col
 // Same accumulation with the row's column indices and values already staged in LDS;
 // four entries per step keep four independent gathers in flight per lane, and the up to three
 // entries left go in one masked step (their gathers in flight together, not one after another).
-template <int G, bool PG, class Gather>
+template <int G, bool PG, class Gather, class AccT>
 __device__ __forceinline__ void lds_row_accumulate(const int *srp, const int *scol, const double *svals, int lr,
-                                                   int g, Gather X, Acc<PG> &acc) {
-    acc.a = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int d = 0; d < (PG ? 3 : 1); ++d) acc.j[d] = make_double2(0.0, 0.0);
+                                                   int g, Gather X, AccT &acc) {
+    acc_zero(acc);
     const int e1 = srp[lr + 1];
     int e = srp[lr] + g;
     for (; e + 3 * G < e1; e += 4 * G) {
@@ -1412,12 +1443,10 @@ __device__ __forceinline__ Lane spmv_lane(int S) {
 // emit(row, acc) receives the group-reduced accumulators (acted on by lane g == 0).
 // Real part only (RO): y = sum a_e x_col(e) from the separate array a.areal, one value per entry;
 // acc.a receives it and acc.j stays zero.
-template <int G, bool PG, class Gather>
+template <int G, bool PG, class Gather, class AccT>
 __device__ __forceinline__ void real_row_accumulate(const int *col, const double *av, int e0, int e1, int g, Gather X,
-                                                    Acc<PG> &acc) {
-    acc.a = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int d = 0; d < (PG ? 3 : 1); ++d) acc.j[d] = make_double2(0.0, 0.0);
+                                                    AccT &acc) {
+    acc_zero(acc);
     int e = e0 + g;
     for (; e + 3 * G < e1; e += 4 * G) {
         const int c0 = col[e], c1 = col[e + G], c2 = col[e + 2 * G], c3 = col[e + 3 * G];
@@ -1504,9 +1533,9 @@ __device__ __forceinline__ void stage_tile(const SolveArgs &a, int64_t t0, int n
     for (int i = threadIdx.x + nt; i <= nr; i += nt) srp[i] = a.rowptr[t0 + i] - E0;
 }
 
-template <int G, bool PG, bool RO = false, class Gather, class Emit>
+template <int G, bool PG, bool RO = false, class Gather, class Emit, class AccT = Acc<PG>>
 __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR, const Lane &L, bool act, Gather X,
-                                            Emit emit) {
+                                            Emit emit, AccT proto = AccT{}) {
     constexpr int VW = RO ? 1 : (PG ? 4 : 2);
     if (TR <= 0) {  // direct: entries straight from global memory
         const TileWalk W = xcd_tiles(a.n, rpb);
@@ -1514,7 +1543,7 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
         for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
             const int64_t row = t0 + L.lrow;
             if (row >= W.end) continue;
-            Acc<PG> acc;
+            AccT acc = proto;
             if (RO)
                 real_row_accumulate<G, PG>(a.col, a.areal, a.rowptr[row], a.rowptr[row + 1], L.g, X, acc);
             else
@@ -1540,7 +1569,7 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
         __syncthreads();
         if (!act) continue;
         for (int lr = L.lrow; lr < nr; lr += rpb) {
-            Acc<PG> acc;
+            AccT acc = proto;
             if (RO) {
                 if (fits) real_row_accumulate<G, PG>(scol, svals, srp[lr], srp[lr + 1], L.g, X, acc);
                 else real_row_accumulate<G, PG>(a.col, a.areal, a.rowptr[t0 + lr], a.rowptr[t0 + lr + 1], L.g, X, acc);
@@ -1910,7 +1939,7 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
         if constexpr (cd) return c_mul(z, cdiag_inv<PG>(a.pdiag, a.coefA, c, s));
         return z;
     };
-    auto emit = [&](int64_t row, const Acc<PG> &acc) {
+    auto emit = [&](int64_t row, const auto &acc) {
         if (L.g != 0) return;
         const int64_t idx = row * S_ + s;
         const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
@@ -1928,9 +1957,12 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
         red[0] += h.x * y.x + h.y * y.y;
         red[1] += h.x * y.y - h.y * y.x;
     };
-    // the preconditioner takes the real part alone when a.areal is set
+    // the preconditioner takes the real part alone when a.areal is set; with per-signal directions
+    // the signal's coefficients are folded into every entry
     if (OP == kOpPre && a.areal)
         staged_spmv<G, PG, true>(a, rpb, TR, L, act, gather, emit);
+    else if constexpr (PG)
+        staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit, fold_acc(a.coefA, s));
     else
         staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit);
     if (OP == kOpV) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
@@ -2008,26 +2040,29 @@ spmv_t_kernel(SolveArgs a, int S, int rpb, int TR, int nparts, int it) {
     spmv_t_head(a, S, nparts, it, salpha, sact);
     const Lane L = spmv_lane<G>(S);
     const bool act = sact[L.s];
-    const double2 alpha = salpha[L.s];
+    // The preconditioned path (every OP but kOpT) reads s as the s-pass materialised it (the
+    // launcher checks that it is there); plain BiCGStab forms s = r - alpha v where it reads it.
+    constexpr bool given_s = OP != kOpT;
+    const double2 alpha = given_s ? make_double2(0.0, 0.0) : salpha[L.s];
     const int S_ = S;
     const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     const double2 *r = a.r, *v = a.v, *sh = a.sh;
     constexpr bool cd = OP == kOpPreCD;  // complex diagonal, as in spmv_v_kernel
     constexpr bool pre = OP == kOpPre || OP == kOpPreCD;
-    const double2 *sm = a.s;  // s materialised by the s-pass (null: formed from r and v here)
+    const double2 *sm = a.s;
     auto gather = [&](int c) {
         const int64_t j = (int64_t)c * S_ + s;
         if (OP == kOpTP) return sh[j];
-        const double2 z = sm ? sm[j] : c_sub(r[j], c_mul(alpha, v[j]));
+        const double2 z = given_s ? sm[j] : c_sub(r[j], c_mul(alpha, v[j]));
         if constexpr (cd) return c_mul(z, cdiag_inv<PG>(a.pdiag, a.coefA, c, s));
         return z;
     };
-    auto emit = [&](int64_t row, const Acc<PG> &acc) {
+    auto emit = [&](int64_t row, const auto &acc) {
         if (L.g != 0) return;
         const int64_t idx = row * S_ + s;
         const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
-        const double2 sv = sm ? sm[idx] : c_sub(r[idx], c_mul(alpha, v[idx]));
+        const double2 sv = given_s ? sm[idx] : c_sub(r[idx], c_mul(alpha, v[idx]));
         if (pre) {
             const double2 w = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
             if constexpr (cd)
@@ -2044,6 +2079,8 @@ spmv_t_kernel(SolveArgs a, int S, int rpb, int TR, int nparts, int it) {
     };
     if (OP == kOpPre && a.areal)
         staged_spmv<G, PG, true>(a, rpb, TR, L, act, gather, emit);
+    else if constexpr (PG)
+        staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit, fold_acc(a.coefA, s));
     else
         staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit);
     if (!pre) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
@@ -2471,9 +2508,10 @@ __global__ void __launch_bounds__(kThreads) spmv_t_tile_kernel(SolveArgs a, int 
 // the same for every lane: with the row index made wave-uniform they load through the scalar
 // cache once per wave (s_load), the vector registers hold only the gathered x and the
 // accumulators, and each lane gathers its own signal's 16 B of x (64 signals: one 1 KB run).
-template <bool PG, class Gather, class Emit>
+template <bool PG, class Gather, class Emit, class AccT = Acc<PG>>
 __device__ __forceinline__ void wave_rows(int64_t n, const int *__restrict__ rowptr, const int *__restrict__ col,
-                                          const double *__restrict__ vals, bool act, Gather X, Emit emit) {
+                                          const double *__restrict__ vals, bool act, Gather X, Emit emit,
+                                          AccT proto = AccT{}) {
     const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
     const TileWalk W = xcd_tiles(n, nw);
     for (int64_t t0 = W.first; t0 < W.end; t0 += W.stride) {
@@ -2481,10 +2519,8 @@ __device__ __forceinline__ void wave_rows(int64_t n, const int *__restrict__ row
         if (row >= W.end) continue;
         const int r = __builtin_amdgcn_readfirstlane((int)row);
         const int e0 = rowptr[r], e1 = rowptr[r + 1];
-        Acc<PG> acc;
-        acc.a = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int d = 0; d < (PG ? 3 : 1); ++d) acc.j[d] = make_double2(0.0, 0.0);
+        AccT acc = proto;
+        acc_zero(acc);
         int e = e0;
         for (; e + 4 <= e1; e += 4) {
             int c[4];
@@ -2513,22 +2549,25 @@ __global__ void __launch_bounds__(kThreads) spmv_v_wave_kernel(SolveArgs a, cons
     const bool act = lane < S && !a.st[lane].done;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     const double2 *p = a.p;
-    wave_rows<PG>(
-        a.n, rowptr, col, vals, act, [&](int c) { return p[(int64_t)c * S + lane]; },
-        [&](int64_t row, const Acc<PG> &acc) {
-            if (!act) return;
-            const int64_t idx = row * S + lane;
-            const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
-            if (OP == kOpPre) {
-                const double2 z = p[idx];
-                a.ph[idx] = make_double2(a.pc * (2.0 * z.x - a.pc * y.x), a.pc * (2.0 * z.y - a.pc * y.y));
-                return;
-            }
-            a.v[idx] = y;
-            const double2 h = a.rh[idx];
-            red[0] += h.x * y.x + h.y * y.y;
-            red[1] += h.x * y.y - h.y * y.x;
-        });
+    auto emit = [&](int64_t row, const auto &acc) {
+        if (!act) return;
+        const int64_t idx = row * S + lane;
+        const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
+        if (OP == kOpPre) {
+            const double2 z = p[idx];
+            a.ph[idx] = make_double2(a.pc * (2.0 * z.x - a.pc * y.x), a.pc * (2.0 * z.y - a.pc * y.y));
+            return;
+        }
+        a.v[idx] = y;
+        const double2 h = a.rh[idx];
+        red[0] += h.x * y.x + h.y * y.y;
+        red[1] += h.x * y.y - h.y * y.x;
+    };
+    auto gather = [&](int c) { return p[(int64_t)c * S + lane]; };
+    if constexpr (PG)  // lanes beyond the signals read no coefficients
+        wave_rows<PG>(a.n, rowptr, col, vals, act, gather, emit, lane < S ? fold_acc(a.coefA, lane) : FoldAcc{});
+    else
+        wave_rows<PG>(a.n, rowptr, col, vals, act, gather, emit);
     if (OP == kOpV)
         block_signal_partials(red, lane < S, (int)(threadIdx.x >> 6) * S + lane, (int)(blockDim.x >> 6) * S, S,
                               a.partV);
@@ -2548,28 +2587,30 @@ __global__ void __launch_bounds__(kThreads) spmv_t_wave_kernel(SolveArgs a, cons
     const double2 alpha = act ? salpha[lane] : make_double2(0.0, 0.0);
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     const double2 *r = a.r, *v = a.v, *sh = a.sh, *sm = a.s;
-    wave_rows<PG>(
-        a.n, rowptr, col, vals, act,
-        [&](int c) {
-            const int64_t j = (int64_t)c * S + lane;
-            if (OP == kOpTP) return sh[j];
-            return sm ? sm[j] : c_sub(r[j], c_mul(alpha, v[j]));
-        },
-        [&](int64_t row, const Acc<PG> &acc) {
-            if (!act) return;
-            const int64_t idx = row * S + lane;
-            const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
-            const double2 sv = sm ? sm[idx] : c_sub(r[idx], c_mul(alpha, v[idx]));
-            if (OP == kOpPre) {
-                a.sh[idx] = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
-                return;
-            }
-            a.t[idx] = y;
-            red[0] += y.x * sv.x + y.y * sv.y;
-            red[1] += y.x * sv.y - y.y * sv.x;
-            red[2] += y.x * y.x + y.y * y.y;
-            red[3] += sv.x * sv.x + sv.y * sv.y;
-        });
+    auto gather = [&](int c) {
+        const int64_t j = (int64_t)c * S + lane;
+        if (OP == kOpTP) return sh[j];
+        return sm ? sm[j] : c_sub(r[j], c_mul(alpha, v[j]));
+    };
+    auto emit = [&](int64_t row, const auto &acc) {
+        if (!act) return;
+        const int64_t idx = row * S + lane;
+        const double2 y = imag_coef_apply<1, PG>(acc, a.coefA, lane);
+        const double2 sv = sm ? sm[idx] : c_sub(r[idx], c_mul(alpha, v[idx]));
+        if (OP == kOpPre) {
+            a.sh[idx] = make_double2(a.pc * (2.0 * sv.x - a.pc * y.x), a.pc * (2.0 * sv.y - a.pc * y.y));
+            return;
+        }
+        a.t[idx] = y;
+        red[0] += y.x * sv.x + y.y * sv.y;
+        red[1] += y.x * sv.y - y.y * sv.x;
+        red[2] += y.x * y.x + y.y * y.y;
+        red[3] += sv.x * sv.x + sv.y * sv.y;
+    };
+    if constexpr (PG)
+        wave_rows<PG>(a.n, rowptr, col, vals, act, gather, emit, lane < S ? fold_acc(a.coefA, lane) : FoldAcc{});
+    else
+        wave_rows<PG>(a.n, rowptr, col, vals, act, gather, emit);
     if (OP != kOpPre)
         block_signal_partials(red, lane < S, (int)(threadIdx.x >> 6) * S + lane, (int)(blockDim.x >> 6) * S, S,
                               a.partT);
@@ -4331,6 +4372,7 @@ void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
                    int op) {
     if (op != kOpT && (g.mapping == kMapSR || g.mapping == kMapTile))
         throw Error(-1, "the polynomial preconditioner needs SpMV mapping 1, 2 or 4");
+    if (op != kOpT && !a.s) throw Error(-1, "the preconditioned t-kernels read s from the s-pass (a.s is null)");
     if (g.mapping == kMapTile) {
         int ucap;
         uint32_t lds;
