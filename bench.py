@@ -145,7 +145,8 @@ def roofline_block(st, copy_gbs, traffic, S_group, event_iters):
             "kernels": table}
 
 
-TRAFFIC_FILES = {"c1": "spmv_traffic_cylinders200k_5sig.json", "c4": "spmv_traffic_neuron2m_32sig.json"}
+TRAFFIC_FILES = {"c1": "spmv_traffic_cylinders200k_5sig.json", "c2": "spmv_traffic_box500k_32sig.json",
+                 "c4": "spmv_traffic_neuron2m_32sig.json"}
 
 
 def measured_traffic(info, mapping_lanes, name="c1"):

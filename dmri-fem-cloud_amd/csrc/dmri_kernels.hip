@@ -1376,32 +1376,36 @@ __device__ __forceinline__ void row_accumulate(const int *rowptr, const int *col
 }
 
 // Same accumulation with the row's column indices and values already staged in LDS;
-// four entries per step keep four independent gathers in flight per lane, and the up to three
-// entries left go in one masked step (their gathers in flight together, not one after another).
+// kRowStep entries per step keep that many independent gathers in flight per lane, and the entries
+// left go in one masked step (their gathers in flight together, not one after another).
+constexpr int kRowStep = 4;
 template <int G, bool PG, class Gather, class AccT>
 __device__ __forceinline__ void lds_row_accumulate(const int *srp, const int *scol, const double *svals, int lr,
                                                    int g, Gather X, AccT &acc) {
     acc_zero(acc);
     const int e1 = srp[lr + 1];
     int e = srp[lr] + g;
-    for (; e + 3 * G < e1; e += 4 * G) {
-        const int c0 = scol[e], c1 = scol[e + G], c2 = scol[e + 2 * G], c3 = scol[e + 3 * G];
-        const double2 x0 = X(c0), x1 = X(c1), x2 = X(c2), x3 = X(c3);
-        acc_entry<PG>(svals, e, x0, acc);
-        acc_entry<PG>(svals, e + G, x1, acc);
-        acc_entry<PG>(svals, e + 2 * G, x2, acc);
-        acc_entry<PG>(svals, e + 3 * G, x3, acc);
+    constexpr int B = kRowStep;
+    for (; e + (B - 1) * G < e1; e += B * G) {
+        int c[B];
+        double2 x[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) c[k] = scol[e + k * G];
+#pragma unroll
+        for (int k = 0; k < B; ++k) x[k] = X(c[k]);
+#pragma unroll
+        for (int k = 0; k < B; ++k) acc_entry<PG>(svals, e + k * G, x[k], acc);
     }
     if (e < e1) {
-        int c[3];
-        double2 x[3];
+        int c[B - 1];
+        double2 x[B - 1];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) c[k] = e + k * G < e1 ? scol[e + k * G] : -1;
+        for (int k = 0; k < B - 1; ++k) c[k] = e + k * G < e1 ? scol[e + k * G] : -1;
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < B - 1; ++k)
             if (c[k] >= 0) x[k] = X(c[k]);
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < B - 1; ++k)
             if (c[k] >= 0) acc_entry<PG>(svals, e + k * G, x[k], acc);
     }
 }
