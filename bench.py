@@ -670,7 +670,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus N>1 is not supported "
+        print(json.dumps({"error": "this library runs on one GPU only; --gpus N>1 is not supported "
                                    "(signals are independent: each GPU would take its own launch groups)"}))
         sys.exit(2)
     if args.cpu_steps <= 0:

@@ -2,7 +2,7 @@
 
 Bars: CSR pattern and DOF map bit-exact; assembled values within 1e-13 relative to the
 largest entry of their row; signals within 1e-9 relative of the oracle at Krylov rtol 1e-13
-(the north-star bar is 1e-8); the BoxMesh golden of the reference notebook to its 7 printed digits.
+(the parity bar is 1e-8); the BoxMesh golden of the reference notebook to its 7 printed digits.
 """
 import numpy as np
 import pytest
