@@ -88,9 +88,13 @@ def cos_ogse(nperiod, delta, Delta, t0=100.0, ext=100.0):
 fibonacci_directions = DS.fibonacci_directions
 
 KERNEL_NAMES = ("bicg_p", "spmv_v_pre", "spmv_v", "spass", "spmv_t_pre", "spmv_t", "bicg_update")
+# the same dmri_stats slots under COCG (slots 3-5 unused)
+COCG_NAMES = ("cocg_p", "spmv_pre", "spmv_q", None, None, None, "cocg_update")
 KERNEL_DESC = {"bicg_p": "p = r + beta (p - omega v)", "spmv_v_pre": "ph = c (2 p - c A p)", "spmv_v": "v = A ph",
                "spass": "alpha = rho / <r^, v>, s = r - alpha v", "spmv_t_pre": "sh = c (2 s - c A s)",
-               "spmv_t": "t = A sh", "bicg_update": "x += alpha ph + omega sh, r = s - omega t"}
+               "spmv_t": "t = A sh", "bicg_update": "x += alpha ph + omega sh, r = s - omega t",
+               "cocg_p": "x += alpha p (previous iteration), p = z + beta p", "spmv_pre": "z = c (2 r - c A r), r^T z",
+               "spmv_q": "q = A p, p^T q", "cocg_update": "alpha = rho / p^T q, r -= alpha q"}
 
 
 def merge_stats(stats):
@@ -108,13 +112,13 @@ def merge_stats(stats):
     return out
 
 
-def kernel_table(st):
+def kernel_table(st, krylov="bicgstab"):
     """Per-kernel device time and algorithmic bandwidth of the event-timed Krylov iterations."""
     total = sum(st["kern_ms"]) or 1.0
     out = {}
-    for j, name in enumerate(KERNEL_NAMES):
+    for j, name in enumerate(COCG_NAMES if krylov == "cocg" else KERNEL_NAMES):
         nl = st["kern_launches"][j]
-        if nl == 0:
+        if nl == 0 or name is None:
             continue
         us = 1e3 * st["kern_ms"][j] / nl
         b = st["kern_bytes"][j] / nl
@@ -124,10 +128,10 @@ def kernel_table(st):
     return out
 
 
-def roofline_block(st, copy_gbs, traffic, S_group, event_iters):
+def roofline_block(st, copy_gbs, traffic, S_group, event_iters, krylov="bicgstab"):
     """The roofline of the dominant kernel (most device time over the timed iterations) and the table
     of every kernel of an iteration."""
-    table = kernel_table(st)
+    table = kernel_table(st, krylov)
     if not table:
         return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None}
@@ -244,7 +248,7 @@ def cpu_baseline(args, w, ctx, info, gdir, q):
                           FB=None if FB is None else FB[:nst])
     tcpu = time.perf_counter() - tc
     s_gpu, _ = ctx.solve(fA[:nst], fB[:nst], w["k"], q, gdir, rtol=args.rtol,
-                         F_explicit=None if FB is None else FB[:nst])
+                         F_explicit=None if FB is None else FB[:nst], krylov=args.krylov)
     parity = abs(s_gpu.real - s_cpu) / abs(s_cpu)
     per_signal = tcpu * len(fA) / nst
     cpu = {"value": 1.0 / per_signal, "unit": "signals/s", "cores": O.num_threads(), "kind": "port",
@@ -293,14 +297,14 @@ def run_forward(args, name):
         if nsig <= S_group:
             return ctx.solve_batched(fA, fB, k, qs, gd, rtol=args.rtol, profile=profile, lanes=args.lanes,
                                      guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond,
-                                     graphs=args.graphs)
+                                     graphs=args.graphs, krylov=args.krylov)
         sigs, stats = [], []
         t0 = time.perf_counter()
         for g0 in range(0, nsig, S_group):
             sl = slice(g0, min(nsig, g0 + S_group))
             s_, st_ = ctx.solve_batched(fA, fB, k, qs[sl], gd[sl], rtol=args.rtol, profile=profile, lanes=args.lanes,
                                         guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond,
-                                        graphs=args.graphs)
+                                        graphs=args.graphs, krylov=args.krylov)
             sigs.append(s_)
             stats.append(st_)
             print(f"  signals {sl.stop}/{nsig} done, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -329,7 +333,7 @@ def run_forward(args, name):
     used = tuned if tuned is not None else ([args.mapping, args.lanes] if args.mapping and args.lanes else None)
     traffic = measured_traffic(info, used, name)
     replay_achieved = replay_bytes / (0.5 * (us_v + us_t) * 1e-6) / 1e9
-    roof = roofline_block(merged, copy_gbs, traffic, S_group, args.event_iters)
+    roof = roofline_block(merged, copy_gbs, traffic, S_group, args.event_iters, args.krylov)
     roof["replay_us"] = {"spmv_v": us_v, "spmv_t": us_t}
     roof["replay_achieved"] = replay_achieved
     b0 = [abs(s.real - 1.0) for s, q in zip(sig, qs) if q == 0.0]
@@ -338,7 +342,7 @@ def run_forward(args, name):
     if not args.no_plain:
         idx = list(range(nsig)) if nsig <= 8 else [w["cpu_signal"], nsig - 1]
         tp0 = time.perf_counter()
-        ps = [ctx.solve(fA, fB, k, qs[j], gd[j], rtol=args.rtol, F_explicit=FB)[0] for j in idx]
+        ps = [ctx.solve(fA, fB, k, qs[j], gd[j], rtol=args.rtol, F_explicit=FB, krylov=args.krylov)[0] for j in idx]
         tp = time.perf_counter() - tp0
         plain = {"signals_per_s": len(idx) / tp, "sample": f"{len(idx)} of {nsig} signals, one launch group each",
                  "batched_over_plain": (njob / elapsed) / (len(idx) / tp),
@@ -351,7 +355,7 @@ def run_forward(args, name):
 
     cfg = {"workload": w["desc"], "nv": info["nv"], "nc": info["nc"], "ndof": info["ndof"], "nnz": info["nnz"],
            "n_iface": info["n_iface"], "n_periodic": info["n_periodic"], "theta_steps": len(fA),
-           "signals": nsig, "launch_group": S_group, "rtol": args.rtol, "spmv_mapping_lanes": tuned, "graphs": args.graphs}
+           "signals": nsig, "launch_group": S_group, "rtol": args.rtol, "spmv_mapping_lanes": tuned, "graphs": args.graphs, "krylov": args.krylov}
     cfg.update(w["extra"])
     if "full_steps" in w:
         cfg["theta_steps_of_full_sequence"] = f"first {len(fA)} of {w['full_steps']}"
@@ -417,7 +421,7 @@ def run_stepwise(args, name):
     tb = time.perf_counter()
     batch = ctx.batch(fA, fB, k, qs, gd, rtol=args.rtol, profile=args.event_iters, lanes=args.lanes,
                       guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond, group=S_group,
-                      graphs=args.graphs)
+                      graphs=args.graphs, krylov=args.krylov)
     t_begin = time.perf_counter() - tb
     for i in range(args.warmup):
         batch.advance(1)
@@ -440,7 +444,7 @@ def run_stepwise(args, name):
     us_v, us_t, replay_bytes = ctx.time_spmv(S_group, per_gdir, 20)
     replay_achieved = replay_bytes / (0.5 * (us_v + us_t) * 1e-6) / 1e9
     traffic = measured_traffic(info, tuned, name)
-    roof = roofline_block(merged, copy_gbs, traffic, S_group, args.event_iters)
+    roof = roofline_block(merged, copy_gbs, traffic, S_group, args.event_iters, args.krylov)
     roof["replay_us"] = {"spmv_v": us_v, "spmv_t": us_t}
     roof["replay_achieved"] = replay_achieved
 
@@ -449,7 +453,7 @@ def run_stepwise(args, name):
     if not args.no_plain:
         j = w["cpu_signal"]
         pb = ctx.batch(fA, fB, k, qs[j:j + 1], gd[j:j + 1], rtol=args.rtol, lanes=0, guess=args.guess,
-                       F_explicit=FB, precond=args.precond, group=1)
+                       F_explicit=FB, precond=args.precond, group=1, krylov=args.krylov)
         pb.advance(args.warmup)
         tp0 = time.perf_counter()
         pst = pb.advance(args.steps)
@@ -473,7 +477,7 @@ def run_stepwise(args, name):
            "theta_steps_full_sequence": nfull, "signals": nsig, "launch_group": S_group, "rtol": args.rtol,
            "timed_window": f"theta steps {args.warmup}..{args.warmup + args.steps - 1} of {nfull} (after "
                            f"{args.warmup} warm-up steps from u0 = 1), every signal",
-           "spmv_mapping_lanes": tuned, "graphs": args.graphs}
+           "spmv_mapping_lanes": tuned, "graphs": args.graphs, "krylov": args.krylov}
     cfg.update(w["extra"])
     out = {
         "metric": METRIC,
@@ -536,7 +540,8 @@ def cpu_baseline_steps(args, w, ctx, info):
         O.set_threads(cores)
         one = {"value": float(info["ndof"]) / t1, "unit": "DOF-steps/s", "cores": 1,
                "sample": f"theta step 1 of signal {j}, {int(st1['iters'])} BiCGStab iterations, {t1:.2f} s"}
-    s_gpu, _ = ctx.solve(fA[:n], fB[:n], w["k"], q, g, rtol=args.rtol, F_explicit=None if FB is None else FB[:n])
+    s_gpu, _ = ctx.solve(fA[:n], fB[:n], w["k"], q, g, rtol=args.rtol, F_explicit=None if FB is None else FB[:n],
+                         krylov=args.krylov)
     cpu = {"value": float(info["ndof"]) * (n - 1) / tcpu, "unit": "DOF-steps/s", "cores": cores, "kind": "port",
            "sample": f"FP64 CPU oracle (C/OpenMP BiCGStab+Jacobi, same mesh, rtol {args.rtol:g}): theta steps "
                      f"1..{n - 1} of signal {j} (q={q:.4g}), {int(st['iters'])} BiCGStab iterations, {tcpu:.2f} s",
@@ -582,18 +587,19 @@ def run_adjoint(args):
     ctx.assemble()
     sqrt_stride = int(math.ceil(math.sqrt(len(fA))))
     for _ in range(args.warmup):
-        ctx.solve(fA, fB, k, q, g, rtol=args.rtol)
-        ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol)
-        ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, checkpoint_stride=sqrt_stride)
+        ctx.solve(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov)
+        ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov)
+        ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, checkpoint_stride=sqrt_stride, krylov=args.krylov)
     tf = ta = tq = 0.0
     grads = gq = None
     for _ in range(args.steps):
         a0 = time.perf_counter()
-        s_f, _ = ctx.solve(fA, fB, k, q, g, rtol=args.rtol)
+        s_f, _ = ctx.solve(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov)
         a1 = time.perf_counter()
-        s_a, grads, _ = ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol)
+        s_a, grads, _ = ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov)
         a2 = time.perf_counter()
-        _, gq, _ = ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, checkpoint_stride=sqrt_stride)
+        _, gq, _ = ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, checkpoint_stride=sqrt_stride,
+                                krylov=args.krylov)
         a3 = time.perf_counter()
         tf += a1 - a0
         ta += a2 - a1
@@ -608,7 +614,7 @@ def run_adjoint(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": desc,
                    "nv": info["nv"], "ndof": info["ndof"], "nnz": info["nnz"], "theta_steps": len(fA),
-                   "rtol": args.rtol, "checkpoints": grads["checkpoints"],
+                   "rtol": args.rtol, "krylov": args.krylov, "checkpoints": grads["checkpoints"],
                    "recomputed_steps": grads["recomputed_steps"]},
         "forward_ms": 1e3 * tf / args.steps, "adjoint_ms": 1e3 * ta / args.steps,
         "policy": "automatic: every state kept in HBM (they fit), no recompute",
@@ -662,6 +668,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="counter runs only: truncate to the first K theta steps (result line not comparable)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel reference")
+    ap.add_argument("--krylov", default="cocg", choices=["bicgstab", "cocg", "gmres"],
+                    help="Krylov method of every theta step (cocg, the default here: the workloads' operators are "
+                         "complex symmetric, one preconditioned product per iteration against BiCGStab's two)")
     ap.add_argument("--graphs", type=int, default=0, choices=[0, 1],
                     help="1: replay the Krylov iteration runs as captured hipGraphs")
     ap.add_argument("--event-iters", type=int, default=1,

@@ -116,9 +116,10 @@ def convert_q2g(qvalue):
 
 class KrylovSolver:
     """Solver settings holder (KrylovSolver("bicgstab", "jacobi") in GCloudDmriSolver.py:219-222;
-    KrylovSolver("gmres", ...) in the notebooks).  "gmres" runs libdmri's restarted GMRES, every
-    other method its BiCGStab; both with libdmri's Jacobi-scaled preconditioner and converged to the
-    relative tolerance (the preconditioner name is kept for compatibility)."""
+    KrylovSolver("gmres", ...) in the notebooks).  "gmres" runs libdmri's restarted GMRES, "cocg" its
+    COCG (complex-symmetric operators only), every other method its BiCGStab; all with libdmri's
+    Jacobi-scaled preconditioner and converged to the relative tolerance (the preconditioner name is
+    kept for compatibility)."""
 
     def __init__(self, method="bicgstab", pc="jacobi"):
         self.method, self.pc = method, pc
@@ -392,7 +393,7 @@ class MRI_simulation:
                                               rtol=p["relative_tolerance"], atol=p["absolute_tolerance"],
                                               maxit=min(int(p["maximum_iterations"]), 2 ** 31 - 1),
                                               F_explicit=FB, F_implicit=FA, u0=self.u0_vertex,
-                                              krylov=1 if str(linsolver.method).lower() == "gmres" else 0,
+                                              krylov={"gmres": 1, "cocg": 2}.get(str(linsolver.method).lower(), 0),
                                               restart=int(p.get("restart", 30)),
                                               snap_every=self.nskip if self.save_snapshots else 0)
         self.elapsed_time = time.time() - start

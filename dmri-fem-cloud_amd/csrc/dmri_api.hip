@@ -847,6 +847,7 @@ struct Stepper {
     int precond = 0;                         // dmri_stats.precond_used of this launch group
     bool strong = false;                     // the strong periodic form (per-step real parts)
     bool gmres_on = false;                   // restarted GMRES instead of BiCGStab
+    bool cocg_on = false;                    // COCG instead of BiCGStab (complex-symmetric operators)
     GmArgs gm{};
     std::vector<double> s2A, s2B;            // strong form: theta q^2 F^2 per step, implicit and explicit
     bool timed_first[kEventPool / 7] = {};   // timed iteration k of a chunk is the solve's first
@@ -900,7 +901,8 @@ struct Stepper {
             if (opts->lanes_per_row > 0) lanes = opts->lanes_per_row;
         }
         poly = opts->preconditioner == 0 || opts->preconditioner == 2;
-        if (poly && (mapping == kMapSR || mapping == kMapTile)) {  // kernels without the polynomial ops
+        if ((poly && (mapping == kMapSR || mapping == kMapTile)) ||  // kernels without the polynomial ops
+            (opts->krylov == 2 && mapping != kMapSL && mapping != kMapStaged)) {  // COCG's SpMVs: mappings 1, 2
             mapping = kMapStaged;
             while (lanes > 1 && lanes * S > 64) lanes >>= 1;
         }
@@ -1087,6 +1089,11 @@ struct Stepper {
         a.atol = opts->atol;
         a.maxit = opts->maxit;
         gmres_on = opts->krylov == 1;
+        cocg_on = opts->krylov == 2;
+        // COCG needs A^T = A: every form but the strong periodic one (its direction terms are
+        // antisymmetric)
+        require(!(cocg_on && strong), DMRI_ERR_ARG,
+                "COCG needs a complex-symmetric operator: not available with the strong periodic form");
         if (gmres_on) setup_gmres();
         // algorithmic bytes per launch of each iteration kernel (dmri_stats.kern_bytes): the operator
         // SpMVs v = A ph and t = A sh read their matrix entries and 48 B per row and signal (one vector
@@ -1103,8 +1110,17 @@ struct Stepper {
             kbytes[4] = poly ? E(npre) + 32.0 * nS + cdx : 0.0;
             kbytes[5] = E(vw) + 48.0 * nS;
             kbytes[6] = 128.0 * nS;
+            if (cocg_on) {
+                // [0] x += alpha_prev p, p = z + beta p (x, p, z read, x, p written), [1] z = M r,
+                // [2] q = A p (one vector gathered, one written, the own row of the gathered vector
+                // counted once), [6] r -= alpha q (r, q read, r written)
+                kbytes[0] = 80.0 * nS;
+                kbytes[2] = E(vw) + 32.0 * nS;
+                kbytes[3] = kbytes[4] = kbytes[5] = 0.0;
+                kbytes[6] = 48.0 * nS;
+            }
         }
-        st->spmv_bytes = 0.5 * (kbytes[2] + kbytes[5]);
+        st->spmv_bytes = cocg_on ? kbytes[2] : 0.5 * (kbytes[2] + kbytes[5]);
         precond = poly ? st->precond_used : 0;
         prof = opts->profile != 0;
         chunk = 8;
@@ -1423,6 +1439,15 @@ struct Stepper {
     // The seven kernels of BiCGStab iteration it (e0(j), e1(j): the events of kernel j, or null)
     template <class E0, class E1>
     void launch_iteration(int it, E0 e0, E1 e1) {
+        if (cocg_on) {
+            SolveArgs ar = a;  // z = M r (into a.ph) with r^T z
+            ar.p = a.r;
+            launch_cocg_spmv(geo, ar, it, s, e0(1), e1(1), true);
+            launch_cocg_p(geo, a, it, s, e0(0), e1(0));      // p = z + beta p
+            launch_cocg_spmv(geo, a, it, s, e0(2), e1(2), false);  // q = A p (into a.v) with p^T q
+            launch_cocg_update(geo, a, it, s, e0(6), e1(6));
+            return;
+        }
         launch_bicg_p(geo, a, it, s, e0(0), e1(0));
         if (poly) {
             // ph = 2p - Ap, v = A ph; sh = 2s - As, t = A sh (right preconditioning)
@@ -1539,12 +1564,14 @@ struct Stepper {
                 for (int k = 0; k < real; ++k)
                     for (int j = 0; j < kK; ++j) {
                         if (!poly && (j == 1 || j == 3 || j == 4)) continue;
+                        if (cocg_on && (j == 3 || j == 4 || j == 5)) continue;
                         float ms = 0;
                         DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[kK * k + j], c->ev1[kK * k + j]));
                         st->kern_ms[j] += ms;
                         st->kern_launches[j] += 1;
-                        // the first iteration's p = r reads one vector and writes one
-                        st->kern_bytes[j] += kbytes[j] - (j == 0 && timed_first[k] ? 32.0 * (double)n * S : 0.0);
+                        // the first iteration's p = r (COCG: p = z) reads one vector and writes one
+                        const double first_less = cocg_on ? 48.0 : 32.0;
+                        st->kern_bytes[j] += kbytes[j] - (j == 0 && timed_first[k] ? first_less * (double)n * S : 0.0);
                         if (j == 2 || j == 5) {
                             st->spmv_ms += ms;
                             st->spmv_launches += 1;
@@ -1566,6 +1593,7 @@ struct Stepper {
                     mask[j] = 1;
                     any_restart = true;
                     h.done = 0;
+                    h.stop = 0;
                     h.breakdown = 0;
                     h.it0 = it;
                     continue;
@@ -2186,6 +2214,9 @@ void validate_seq(const dmri_seq *seq, const dmri_solver_opts *opts) {
     require(opts->lanes_per_row >= 0, DMRI_ERR_ARG, "lanes_per_row must be >= 0");
     require(opts->initial_guess >= 0 && opts->initial_guess <= 3, DMRI_ERR_ARG, "initial_guess must lie in [0, 3]");
     require(opts->preconditioner >= 0 && opts->preconditioner <= 2, DMRI_ERR_ARG, "preconditioner must lie in [0, 2]");
+    require(opts->krylov >= 0 && opts->krylov <= 2, DMRI_ERR_ARG, "krylov must be 0 (BiCGStab), 1 (GMRES) or 2 (COCG)");
+    require(opts->krylov != 2 || opts->preconditioner != 1, DMRI_ERR_ARG,
+            "COCG runs with the polynomial preconditioner (preconditioner 0 or 2)");
     for (int i = 0; i < seq->nsteps; ++i)
         require(std::isfinite(seq->f_implicit[i]) && std::isfinite(seq->f_explicit[i]), DMRI_ERR_ARG,
                 "time profile has a non-finite value");
@@ -2347,7 +2378,7 @@ void batch_advance_impl(dmri_batch *b, int nsteps, dmri_stats *stats) {
         st.relres_max = std::max(st.relres_max, x.relres_max);
         st.spmv_ms += x.spmv_ms;
         st.spmv_launches += x.spmv_launches;
-        st.spmv_bytes = 0.5 * (gr->run.kbytes[2] + gr->run.kbytes[5]);
+        st.spmv_bytes = gr->run.cocg_on ? gr->run.kbytes[2] : 0.5 * (gr->run.kbytes[2] + gr->run.kbytes[5]);
         st.precond_used = gr->run.precond;
         for (int j = 0; j < 7; ++j) {
             st.kern_ms[j] += x.kern_ms[j];

@@ -67,7 +67,10 @@ struct KrylovState {
     double2 rho[2], alpha[2], omega[2];
     double tol2, bnorm2, rnorm2, pad;
     int done, conv, breakdown, iters;
-    int it0, restarts, pad1, pad2;
+    int it0, restarts;
+    int stop;  // COCG: the p-kernel stopped the signal (converged, breakdown or maxit); the update
+               // kernel of the same iteration then marks it done (see cocg_p_kernel)
+    int pad2;
 };
 
 constexpr int kThreads = 256;   // target threads per block of every solver kernel
@@ -93,6 +96,10 @@ constexpr int kOpV = 0, kOpT = 0, kOpPre = 1, kOpTP = 2;
 // kernel-side variant of kOpPre with the complex diagonal (SolveArgs::pdiag), compiled apart so
 // that the plain preconditioner kernels keep their registers
 constexpr int kOpPreCD = 3;
+// COCG's SpMVs (mappings 1 and 2): q = A p with the partials of p^T q, and z = M p (the polynomial,
+// plain or with the complex diagonal) with the partials of p^T z; unconjugated products, since
+// COCG works with the complex-symmetric bilinear form
+constexpr int kOpVDot = 4, kOpPreDot = 5, kOpPreCDDot = 6;
 // the preconditioner polynomial takes the real part alone when the imaginary part's bound is at
 // most this fraction of the real part's (measured: 0.002 -> same iterations, 3.3 -> twice as many)
 constexpr double kRealPreconditioner = 0.05;
@@ -363,6 +370,18 @@ void launch_spmv_v(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
 void launch_spmv_t(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
                    hipEvent_t e1 = nullptr, int op = kOpT);
 void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr);
+// COCG (conjugate orthogonal conjugate gradients) for the complex-symmetric operator, right-
+// preconditioned by the same polynomial M (also complex symmetric).  One iteration: z = M r with
+// rho = r^T z (launch_cocg_spmv, pre), the convergence test, the previous iteration's
+// x += alpha p and p = z + (rho / rho_prev) p (launch_cocg_p), q = A p with p^T q
+// (launch_cocg_spmv), then alpha = rho / p^T q and r -= alpha q with |r|^2 (launch_cocg_update).
+// a.ph holds z, a.v holds q.
+void launch_cocg_spmv(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                      bool pre);
+void launch_cocg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr);
+void launch_cocg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);
 // alpha = rho / <r^, v> per signal and s = r - alpha v into a.s (the polynomial preconditioner's t-kernel
 // then gathers one vector instead of forming s from two)

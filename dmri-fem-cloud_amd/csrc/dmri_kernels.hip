@@ -1936,8 +1936,10 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
     const double2 *p = a.p;
     // kOpPreCD, with the complex diagonal d: ph = c D^-1 (2 p - c A D^-1 p), A D^-1 p gathering p / d
-    constexpr bool cd = OP == kOpPreCD;
-    constexpr bool pre = OP == kOpPre || OP == kOpPreCD;
+    constexpr bool cd = OP == kOpPreCD || OP == kOpPreCDDot;
+    constexpr bool pre = OP == kOpPre || OP == kOpPreCD || OP == kOpPreDot || OP == kOpPreCDDot;
+    // COCG's operations: the unconjugated product of the input's own row with the output
+    constexpr bool pdot = OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot;
     auto gather = [&](int c) {
         const double2 z = p[(int64_t)c * S_ + s];
         if constexpr (cd) return c_mul(z, cdiag_inv<PG>(a.pdiag, a.coefA, c, s));
@@ -1950,26 +1952,38 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
         if (pre) {
             const double2 z = p[idx];
             const double2 w = make_double2(a.pc * (2.0 * z.x - a.pc * y.x), a.pc * (2.0 * z.y - a.pc * y.y));
-            if constexpr (cd)
-                a.ph[idx] = c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s));
-            else
-                a.ph[idx] = w;
+            double2 out = w;
+            if constexpr (cd) out = c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s));
+            a.ph[idx] = out;
+            if constexpr (pdot) {
+                const double2 zo = c_mul(z, out);
+                red[0] += zo.x;
+                red[1] += zo.y;
+            }
             return;
         }
         a.v[idx] = y;
-        const double2 h = a.rh[idx];
-        red[0] += h.x * y.x + h.y * y.y;
-        red[1] += h.x * y.y - h.y * y.x;
+        if constexpr (pdot) {
+            const double2 py = c_mul(p[idx], y);
+            red[0] += py.x;
+            red[1] += py.y;
+        } else {
+            const double2 h = a.rh[idx];
+            red[0] += h.x * y.x + h.y * y.y;
+            red[1] += h.x * y.y - h.y * y.x;
+        }
     };
     // the preconditioner takes the real part alone when a.areal is set; with per-signal directions
     // the signal's coefficients are folded into every entry
-    if (OP == kOpPre && a.areal)
+    if ((OP == kOpPre || OP == kOpPreDot) && a.areal)
         staged_spmv<G, PG, true>(a, rpb, TR, L, act, gather, emit);
     else if constexpr (PG)
         staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit, fold_acc(a.coefA, s));
     else
         staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit);
-    if (OP == kOpV) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
+    if (OP == kOpV || OP == kOpPreDot || OP == kOpPreCDDot)
+        block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
+    if (OP == kOpVDot) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
 }
 
 // r = b - (Are + i cA.J) x with the stored b, r^ = r, in the SpMV kernels' staged row mapping
@@ -3109,6 +3123,144 @@ __global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int 
             const double2 h = ld_once(a.rh + i);
             red[0] += h.x * rn.x + h.y * rn.y;
             red[1] += h.x * rn.y - h.y * rn.x;
+            red[2] += rn.x * rn.x + rn.y * rn.y;
+        }
+    }
+    block_signal_partials(red, valid, threadIdx.x, vrpb * S, S, a.partU);
+}
+
+// ---- COCG (see launch_cocg_spmv in dmri_internal.hpp).  Per-signal scalars in KrylovState: rho
+// (r^T z) and alpha in the slot of the iteration's parity, as for BiCGStab.  The iterate's update
+// x += alpha p of an iteration is applied by the next iteration's p-kernel, which reads that p
+// anyway before it overwrites it, so the update kernel streams only r and q.
+// Head: the convergence test on |r|^2 (partials U, from the residual kernel at a solve's first
+// iteration, else from the update), rho = r^T z (partials V, from the preconditioner SpMV of this
+// iteration), beta = rho / rho_prev; body: x += alpha_prev p (every iteration but a solve's first,
+// also for a signal that stops here), then p = z (first iteration) or z + beta p.
+__global__ void __launch_bounds__(kThreads) cocg_p_kernel(SolveArgs a, int S, int vrpb, int nparts_rhs,
+                                                          int nparts_upd, int nparts_spmv, int it) {
+    if (*a.all_done) return;
+    __shared__ double totU[kMaxSignals][kNV], totV[kMaxSignals][kNV];
+    __shared__ double2 sbeta[kMaxSignals], salpha[kMaxSignals];
+    __shared__ int sact[kMaxSignals], sfirst[kMaxSignals], sx[kMaxSignals];
+    signal_totals(a.partU, a.totU, (it - a.st[0].it0) == 0 ? nparts_rhs : nparts_upd, S, totU);
+    signal_totals(a.partV, a.totV, nparts_spmv, S, totV);
+    if ((int)threadIdx.x < S) {
+        const int s = threadIdx.x;
+        KrylovState &st = a.st[s];
+        // done and stop change only in other kernels, so every block of this one reads the same
+        // values and agrees on which signals still owe their pending update of x
+        int act = !st.done && !st.stop;
+        const int local = it - st.it0;
+        sfirst[s] = local == 0;
+        sx[s] = 0;
+        if (act) {
+            const double rn2 = totU[s][2];
+            double tol2;
+            if (local == 0) {
+                const double t1 = a.rtol * a.rtol * totU[s][3], t2 = a.atol * a.atol;
+                tol2 = t1 > t2 ? t1 : t2;
+            } else {
+                tol2 = st.tol2;
+            }
+            const double2 rho = make_double2(totV[s][0], totV[s][1]);
+            const bool conv = rn2 <= tol2;
+            const bool brk = !conv && rho.x == 0.0 && rho.y == 0.0;
+            const bool lim = !conv && local >= a.maxit;
+            const bool newly = conv || brk || lim;
+            // the previous iteration's x += alpha p is still to be applied
+            sx[s] = local > 0;
+            salpha[s] = local > 0 ? st.alpha[(local - 1) & 1] : make_double2(0.0, 0.0);
+            if (blockIdx.x == 0) {
+                if (local == 0) {
+                    st.tol2 = tol2;
+                    st.bnorm2 = totU[s][3];
+                }
+                st.rnorm2 = rn2;
+                st.iters = it;
+                if (newly) {  // done is set by the update kernel (see stop)
+                    st.stop = 1;
+                    st.conv = conv ? 1 : 0;
+                    st.breakdown = brk ? 1 : 0;
+                } else {
+                    st.rho[local & 1] = rho;
+                }
+            }
+            if (newly) act = 0;
+            else if (local > 0) sbeta[s] = c_div(rho, st.rho[(local - 1) & 1]);
+        }
+        sact[s] = act;
+    }
+    // (the all-done flag is raised by the update kernel: raised here, it would stop this kernel's
+    // later blocks before they apply the pending updates of the signals that stop now)
+    __syncthreads();
+    const int64_t total = a.n * S;
+    const int s = threadIdx.x % S;
+    if ((int)threadIdx.x >= vrpb * S) return;
+    const bool act = sact[s], upd_x = sx[s];
+    if (!act && !upd_x) return;
+    const bool first = sfirst[s];
+    const double2 beta = sbeta[s], alpha = salpha[s];
+    for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total; i += (int64_t)gridDim.x * vrpb * S) {
+        if (first) {  // a solve's first iteration: no update pending, p = z
+            a.p[i] = a.ph[i];
+            continue;
+        }
+        const double2 pi = a.p[i];
+        st_once(a.x + i, c_add(ld_once(a.x + i), c_mul(alpha, pi)));
+        if (act) a.p[i] = c_add(a.ph[i], c_mul(beta, pi));
+    }
+}
+
+// alpha = rho / p^T q (partials T, from q = A p) and r -= alpha q; partials U: [0, 0, |r|^2, 0].
+// A zero p^T q before convergence is a breakdown (the host restarts the signal from its x).
+__global__ void __launch_bounds__(kThreads) cocg_update_kernel(SolveArgs a, int S, int vrpb, int nparts, int it) {
+    if (*a.all_done) return;
+    __shared__ double tot[kMaxSignals][kNV];
+    __shared__ double2 salpha[kMaxSignals];
+    __shared__ int sact[kMaxSignals];
+    signal_totals(a.partT, a.totT, nparts, S, tot);
+    if ((int)threadIdx.x < S) {
+        const int s = threadIdx.x;
+        KrylovState &st = a.st[s];
+        int act = !st.done && !st.stop;
+        const int local = it - st.it0;
+        if (st.stop && blockIdx.x == 0) st.done = 1;  // stopped by this iteration's p-kernel
+        if (act) {
+            const double2 pq = make_double2(tot[s][0], tot[s][1]);
+            if (pq.x == 0.0 && pq.y == 0.0) {
+                if (blockIdx.x == 0) {
+                    st.done = 1;
+                    st.breakdown = 1;
+                    st.conv = 0;
+                }
+                act = 0;
+            } else {
+                const double2 al = c_div(st.rho[local & 1], pq);
+                salpha[s] = al;
+                if (blockIdx.x == 0) st.alpha[local & 1] = al;
+            }
+        }
+        sact[s] = act;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // every signal stopped: later launches exit at once
+        int all = 1;
+        for (int s = 0; s < S; ++s)
+            if (sact[s]) all = 0;
+        if (all) *a.all_done = 1;
+    }
+    const int64_t total = a.n * S;
+    const int s = threadIdx.x % S;
+    const bool valid = (int)threadIdx.x < vrpb * S;
+    const bool act = valid && sact[s];
+    const double2 alpha = salpha[s];
+    double red[kNV] = {0.0, 0.0, 0.0, 0.0};
+    if (act) {
+        for (int64_t i = (int64_t)blockIdx.x * vrpb * S + threadIdx.x; i < total;
+             i += (int64_t)gridDim.x * vrpb * S) {
+            const double2 rn = c_sub(ld_once(a.r + i), c_mul(alpha, ld_once(a.v + i)));
+            st_once(a.r + i, rn);
             red[2] += rn.x * rn.x + rn.y * rn.y;
         }
     }
@@ -4289,7 +4441,16 @@ static void tile_lds(const SolveGeom &g, int &ucap, uint32_t &bytes) {
 template <int OP>
 static void spmv_v_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                             hipEvent_t e1) {
-    if (g.mapping == kMapPair) {
+    if constexpr (OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot) {
+        if (g.mapping != kMapSL && g.mapping != kMapStaged) throw Error(-1, "COCG needs SpMV mapping 1 or 2");
+        const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
+        const uint32_t lds =
+            TR > 0 ? (uint32_t)(OP == kOpPreDot && a.areal ? stage_lds_bytes_ro(TR) : stage_lds_bytes(TR, a.per_gdir))
+                   : 0u;
+        DMRI_DISPATCH_G(g.G, a.per_gdir,
+                        hipExtLaunchKernelGGL((spmv_v_kernel<GG, PP, OP>), dim3(g.grid), dim3(g.spmv_threads), lds, s,
+                                              e0, e1, 0, a, g.S, g.rpb, TR, it));
+    } else if (g.mapping == kMapPair) {
         const int TR = stage_rows(g, a.per_gdir);
         const uint32_t lds =
             (uint32_t)(OP == kOpPre && a.areal ? stage_lds_bytes_ro(TR) : stage_lds_bytes(TR, a.per_gdir));
@@ -4413,6 +4574,36 @@ void launch_spass(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s,
 void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                         hipEvent_t e1) {
     hipExtLaunchKernelGGL(bicg_update_kernel, dim3(g.vgrid), dim3(g.vec_threads), 0, s, e0, e1, 0, a, g.S,
+                          g.vec_rpb, g.grid, it);
+    check_launch();
+    if (a.totU) {
+        finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partU, g.vgrid, g.S, a.totU);
+        check_launch();
+    }
+}
+void launch_cocg_spmv(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                      bool pre) {
+    if (pre) {
+        if (a.pdiag) spmv_v_dispatch<kOpPreCDDot>(g, a, it, s, e0, e1);
+        else spmv_v_dispatch<kOpPreDot>(g, a, it, s, e0, e1);
+    } else {
+        spmv_v_dispatch<kOpVDot>(g, a, it, s, e0, e1);
+    }
+    check_launch();
+    double *part = pre ? a.partV : a.partT, *tot = pre ? a.totV : a.totT;
+    if (tot) {
+        finalize_kernel<<<g.S, kThreads, 0, s>>>(part, g.grid, g.S, tot);
+        check_launch();
+    }
+}
+void launch_cocg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(cocg_p_kernel, dim3(g.vgrid), dim3(g.vec_threads), 0, s, e0, e1, 0, a, g.S, g.vec_rpb,
+                          g.grid, g.vgrid, g.grid, it);
+    check_launch();
+}
+void launch_cocg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
+                        hipEvent_t e1) {
+    hipExtLaunchKernelGGL(cocg_update_kernel, dim3(g.vgrid), dim3(g.vec_threads), 0, s, e0, e1, 0, a, g.S,
                           g.vec_rpb, g.grid, it);
     check_launch();
     if (a.totU) {

@@ -153,7 +153,7 @@ def version() -> str:
 class Context:
     """One solver context on one GPU (see include/dmri.h)."""
 
-    KRYLOV = {"bicgstab": 0, "gmres": 1}
+    KRYLOV = {"bicgstab": 0, "gmres": 1, "cocg": 2}
 
     def __init__(self, device: int = 0):
         self._h = ctypes.c_void_p()

@@ -173,15 +173,19 @@ def iteration_kernel(name: str, mapping: int):
     """dmri_stats name of an iteration kernel from its template instance (SpMV kernels of the given
     mapping only), or None.  The last template argument of the SpMV kernels is the operation: 0 v =
     A ph (or t = A s), 1 the polynomial preconditioner, 2 t = A sh, 3 the preconditioner with the
-    complex diagonal."""
+    complex diagonal; COCG's 4 q = A p (spmv_q) and 5, 6 z = M r (spmv_pre)."""
     b = base(name)
-    if b in ("bicg_p_kernel", "spass_kernel", "bicg_update_kernel"):
+    if b in ("bicg_p_kernel", "spass_kernel", "bicg_update_kernel", "cocg_p_kernel", "cocg_update_kernel"):
         return b[:-len("_kernel")]
     if b not in MAPPING_KERNELS.get(mapping, ()):
         return None
     side = "spmv_v" if b.startswith("spmv_v") else "spmv_t"
     args = short(name).split("<")[1].rstrip(">").split(",") if "<" in short(name) else []
     op = int(args[-1]) if args and args[-1].strip().lstrip("-").isdigit() else 0
+    if op == 4:  # COCG: q = A p
+        return "spmv_q"
+    if op in (5, 6):  # COCG: z = M r
+        return "spmv_pre"
     return side + ("_pre" if op in (1, 3) else "")
 
 

@@ -246,8 +246,14 @@ typedef struct {
                                comri/one-comp/fenics-cpp/main.cpp:180-183) with the same right
                                preconditioner: classical Gram-Schmidt applied twice, Givens rotations
                                on the device, the true residual checked at the end of every cycle.
-                               Both stop at ||b - A x|| <= max(rtol ||b||, atol).  dmri_stats.iters_*
-                               count Arnoldi steps for GMRES. */
+                               2 = COCG (conjugate orthogonal CG) for the complex-symmetric operator
+                               (every form but the strong periodic one; needs the polynomial
+                               preconditioner, preconditioner 0 or 2): one preconditioned product
+                               per iteration where BiCGStab takes two, with unconjugated products
+                               r^T z, p^T A p.  All stop at ||r|| <= max(rtol ||b||, atol).
+                               dmri_stats.iters_* count Arnoldi steps for GMRES; for COCG the timed
+                               kernels are [0] p = z + beta p, [1] z = M r, [2] q = A p, [6] the
+                               update of x and r. */
     int32_t restart;        /* GMRES restart length, 0 = 30, at most 64 (m + 1 basis vectors per signal
                                are kept in HBM) */
     int32_t graphs;         /* 1: BiCGStab launches each run of iterations between two convergence

@@ -159,6 +159,10 @@ def test_profile_summary_samples_first_iterations_of_every_step(tmp_path):
     assert named == ["bicg_p", "spmv_v_pre", "spmv_v", "spmv_t_pre", "spmv_t", "bicg_update"]
     assert ps.iteration_kernel("void dmri::spmv_v_pair_kernel<1, true, 3>(dmri::SolveArgs, int)", 6) == "spmv_v_pre"
     assert ps.iteration_kernel("void dmri::spmv_v_pair_kernel<1, true, 0>(dmri::SolveArgs, int)", 2) is None
+    # COCG's kernels: the p- and update kernels by name, the SpMVs by their operation
+    assert [ps.iteration_kernel(f"void dmri::{k}(dmri::SolveArgs, int)", 2) for k in
+            ("cocg_p_kernel", "spmv_v_kernel<1, true, 5>", "spmv_v_kernel<1, false, 6>", "spmv_v_kernel<1, true, 4>",
+             "cocg_update_kernel")] == ["cocg_p", "spmv_pre", "spmv_pre", "spmv_q", "cocg_update"]
 
 
 def _layered_cylinder():
