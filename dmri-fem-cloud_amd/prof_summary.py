@@ -38,7 +38,8 @@ import statistics
 
 SPMV = ("spmv_v_kernel", "spmv_t_kernel", "spmv_v_sr_kernel", "spmv_t_sr_kernel", "spmv_v_wave_kernel",
         "spmv_t_wave_kernel", "spmv_v_tile_kernel", "spmv_t_tile_kernel", "spmv_v_pair_kernel",
-        "spmv_t_pair_kernel", "bicg_p_kernel", "spass_kernel", "bicg_update_kernel")
+        "spmv_t_pair_kernel", "bicg_p_kernel", "spass_kernel", "bicg_update_kernel", "cocg_p_kernel",
+        "cocg_update_kernel")
 
 
 # kernels that form a theta step's residual just before its Krylov iterations

@@ -256,7 +256,7 @@ typedef struct {
                                update of x and r. */
     int32_t restart;        /* GMRES restart length, 0 = 30, at most 64 (m + 1 basis vectors per signal
                                are kept in HBM) */
-    int32_t graphs;         /* 1: BiCGStab launches each run of iterations between two convergence
+    int32_t graphs;         /* 1: BiCGStab and COCG launch each run of iterations between two convergence
                                checks as hipGraphs of 2^k iterations, each captured once per (kernel
                                arguments, first iteration, length) and replayed (the step's
                                coefficients go to a fixed device slot and the projection guess iterates
@@ -287,7 +287,8 @@ typedef struct {
      * [0] 64 n S (32 n S in a solve's first iteration), [1] E(npre) + 32 n S, [2] E(nval) + 48 n S,
      * [3] 48 n S, [4] E(npre) + 32 n S, [5] E(nval) + 48 n S, [6] 128 n S, with E(k) = nnz (4 + 8k) +
      * 4 (n+1), nval = 2 (4 with per-signal directions) and npre = 1 for the real-part polynomial, else
-     * nval). */
+     * nval; with COCG: [0] 80 n S (32 n S in a solve's first iteration), [1] E(npre) + 32 n S,
+     * [2] E(nval) + 32 n S, [6] 48 n S). */
     double kern_ms[7];
     int64_t kern_launches[7];
     double kern_bytes[7];
