@@ -248,7 +248,7 @@ def cpu_baseline(args, w, ctx, info, gdir, q):
                           FB=None if FB is None else FB[:nst])
     tcpu = time.perf_counter() - tc
     s_gpu, _ = ctx.solve(fA[:nst], fB[:nst], w["k"], q, gdir, rtol=args.rtol,
-                         F_explicit=None if FB is None else FB[:nst], krylov=args.krylov)
+                         F_explicit=None if FB is None else FB[:nst], krylov=args.krylov, pre_fp32=args.pre_fp32)
     parity = abs(s_gpu.real - s_cpu) / abs(s_cpu)
     per_signal = tcpu * len(fA) / nst
     cpu = {"value": 1.0 / per_signal, "unit": "signals/s", "cores": O.num_threads(), "kind": "port",
@@ -297,14 +297,14 @@ def run_forward(args, name):
         if nsig <= S_group:
             return ctx.solve_batched(fA, fB, k, qs, gd, rtol=args.rtol, profile=profile, lanes=args.lanes,
                                      guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond,
-                                     graphs=args.graphs, krylov=args.krylov)
+                                     graphs=args.graphs, krylov=args.krylov, pre_fp32=args.pre_fp32)
         sigs, stats = [], []
         t0 = time.perf_counter()
         for g0 in range(0, nsig, S_group):
             sl = slice(g0, min(nsig, g0 + S_group))
             s_, st_ = ctx.solve_batched(fA, fB, k, qs[sl], gd[sl], rtol=args.rtol, profile=profile, lanes=args.lanes,
                                         guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond,
-                                        graphs=args.graphs, krylov=args.krylov)
+                                        graphs=args.graphs, krylov=args.krylov, pre_fp32=args.pre_fp32)
             sigs.append(s_)
             stats.append(st_)
             print(f"  signals {sl.stop}/{nsig} done, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -342,7 +342,7 @@ def run_forward(args, name):
     if not args.no_plain:
         idx = list(range(nsig)) if nsig <= 8 else [w["cpu_signal"], nsig - 1]
         tp0 = time.perf_counter()
-        ps = [ctx.solve(fA, fB, k, qs[j], gd[j], rtol=args.rtol, F_explicit=FB, krylov=args.krylov)[0] for j in idx]
+        ps = [ctx.solve(fA, fB, k, qs[j], gd[j], rtol=args.rtol, F_explicit=FB, krylov=args.krylov, pre_fp32=args.pre_fp32)[0] for j in idx]
         tp = time.perf_counter() - tp0
         plain = {"signals_per_s": len(idx) / tp, "sample": f"{len(idx)} of {nsig} signals, one launch group each",
                  "batched_over_plain": (njob / elapsed) / (len(idx) / tp),
@@ -355,7 +355,7 @@ def run_forward(args, name):
 
     cfg = {"workload": w["desc"], "nv": info["nv"], "nc": info["nc"], "ndof": info["ndof"], "nnz": info["nnz"],
            "n_iface": info["n_iface"], "n_periodic": info["n_periodic"], "theta_steps": len(fA),
-           "signals": nsig, "launch_group": S_group, "rtol": args.rtol, "spmv_mapping_lanes": tuned, "graphs": args.graphs, "krylov": args.krylov}
+           "signals": nsig, "launch_group": S_group, "rtol": args.rtol, "spmv_mapping_lanes": tuned, "graphs": args.graphs, "krylov": args.krylov, "pre_fp32": args.pre_fp32}
     cfg.update(w["extra"])
     if "full_steps" in w:
         cfg["theta_steps_of_full_sequence"] = f"first {len(fA)} of {w['full_steps']}"
@@ -421,7 +421,7 @@ def run_stepwise(args, name):
     tb = time.perf_counter()
     batch = ctx.batch(fA, fB, k, qs, gd, rtol=args.rtol, profile=args.event_iters, lanes=args.lanes,
                       guess=args.guess, mapping=args.mapping, F_explicit=FB, precond=args.precond, group=S_group,
-                      graphs=args.graphs, krylov=args.krylov)
+                      graphs=args.graphs, krylov=args.krylov, pre_fp32=args.pre_fp32)
     t_begin = time.perf_counter() - tb
     for i in range(args.warmup):
         batch.advance(1)
@@ -453,7 +453,7 @@ def run_stepwise(args, name):
     if not args.no_plain:
         j = w["cpu_signal"]
         pb = ctx.batch(fA, fB, k, qs[j:j + 1], gd[j:j + 1], rtol=args.rtol, lanes=0, guess=args.guess,
-                       F_explicit=FB, precond=args.precond, group=1, krylov=args.krylov)
+                       F_explicit=FB, precond=args.precond, group=1, krylov=args.krylov, pre_fp32=args.pre_fp32)
         pb.advance(args.warmup)
         tp0 = time.perf_counter()
         pst = pb.advance(args.steps)
@@ -477,7 +477,7 @@ def run_stepwise(args, name):
            "theta_steps_full_sequence": nfull, "signals": nsig, "launch_group": S_group, "rtol": args.rtol,
            "timed_window": f"theta steps {args.warmup}..{args.warmup + args.steps - 1} of {nfull} (after "
                            f"{args.warmup} warm-up steps from u0 = 1), every signal",
-           "spmv_mapping_lanes": tuned, "graphs": args.graphs, "krylov": args.krylov}
+           "spmv_mapping_lanes": tuned, "graphs": args.graphs, "krylov": args.krylov, "pre_fp32": args.pre_fp32}
     cfg.update(w["extra"])
     out = {
         "metric": METRIC,
@@ -541,7 +541,7 @@ def cpu_baseline_steps(args, w, ctx, info):
         one = {"value": float(info["ndof"]) / t1, "unit": "DOF-steps/s", "cores": 1,
                "sample": f"theta step 1 of signal {j}, {int(st1['iters'])} BiCGStab iterations, {t1:.2f} s"}
     s_gpu, _ = ctx.solve(fA[:n], fB[:n], w["k"], q, g, rtol=args.rtol, F_explicit=None if FB is None else FB[:n],
-                         krylov=args.krylov)
+                         krylov=args.krylov, pre_fp32=args.pre_fp32)
     cpu = {"value": float(info["ndof"]) * (n - 1) / tcpu, "unit": "DOF-steps/s", "cores": cores, "kind": "port",
            "sample": f"FP64 CPU oracle (C/OpenMP BiCGStab+Jacobi, same mesh, rtol {args.rtol:g}): theta steps "
                      f"1..{n - 1} of signal {j} (q={q:.4g}), {int(st['iters'])} BiCGStab iterations, {tcpu:.2f} s",
@@ -587,19 +587,19 @@ def run_adjoint(args):
     ctx.assemble()
     sqrt_stride = int(math.ceil(math.sqrt(len(fA))))
     for _ in range(args.warmup):
-        ctx.solve(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov)
-        ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov)
-        ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, checkpoint_stride=sqrt_stride, krylov=args.krylov)
+        ctx.solve(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov, pre_fp32=args.pre_fp32)
+        ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov, pre_fp32=args.pre_fp32)
+        ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, checkpoint_stride=sqrt_stride, krylov=args.krylov, pre_fp32=args.pre_fp32)
     tf = ta = tq = 0.0
     grads = gq = None
     for _ in range(args.steps):
         a0 = time.perf_counter()
-        s_f, _ = ctx.solve(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov)
+        s_f, _ = ctx.solve(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov, pre_fp32=args.pre_fp32)
         a1 = time.perf_counter()
-        s_a, grads, _ = ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov)
+        s_a, grads, _ = ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, krylov=args.krylov, pre_fp32=args.pre_fp32)
         a2 = time.perf_counter()
         _, gq, _ = ctx.adjoint(fA, fB, k, q, g, rtol=args.rtol, checkpoint_stride=sqrt_stride,
-                                krylov=args.krylov)
+                                krylov=args.krylov, pre_fp32=args.pre_fp32)
         a3 = time.perf_counter()
         tf += a1 - a0
         ta += a2 - a1
@@ -671,6 +671,9 @@ def main():
     ap.add_argument("--krylov", default="cocg", choices=["bicgstab", "cocg", "gmres"],
                     help="Krylov method of every theta step (cocg, the default here: the workloads' operators are "
                          "complex symmetric, one preconditioned product per iteration against BiCGStab's two)")
+    ap.add_argument("--pre-fp32", type=int, default=0, choices=[0, 1],
+                    help="1 (COCG): the preconditioner's SpMV reads r and the operator values in FP32, summing in "
+                         "FP64 (the residual test and every other vector stay FP64)")
     ap.add_argument("--graphs", type=int, default=0, choices=[0, 1],
                     help="1: replay the Krylov iteration runs as captured hipGraphs")
     ap.add_argument("--event-iters", type=int, default=1,

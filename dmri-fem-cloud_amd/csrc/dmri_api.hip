@@ -39,6 +39,7 @@ struct SolverWork {
     DevArray<double> afix, bfix, mgg;  // strong periodic form: fixed real parts and the q^2 F^2 mass term
     DevArray<double> coefcur;          // graph mode: the current step's coefficients (A then B)
     DevArray<double2> xk;              // graph mode: the projection guess's Krylov iterate
+    DevArray<float> r32, pvals32;      // COCG's FP32 preconditioner input: r and the polynomial's values
     // restarted GMRES: basis, new vector, partials, coefficients, Hessenberg, rotations, totals
     DevArray<double2> gV, gw, gH, gcs, gg, gy;
     DevArray<double> gpart, gh, ghsum, gcc, ginv, gtot, gwn;
@@ -848,6 +849,7 @@ struct Stepper {
     bool strong = false;                     // the strong periodic form (per-step real parts)
     bool gmres_on = false;                   // restarted GMRES instead of BiCGStab
     bool cocg_on = false;                    // COCG instead of BiCGStab (complex-symmetric operators)
+    bool pre32 = false;                      // COCG's preconditioner SpMV reads FP32 r and values
     GmArgs gm{};
     std::vector<double> s2A, s2B;            // strong form: theta q^2 F^2 per step, implicit and explicit
     bool timed_first[kEventPool / 7] = {};   // timed iteration k of a chunk is the solve's first
@@ -1095,6 +1097,20 @@ struct Stepper {
         require(!(cocg_on && strong), DMRI_ERR_ARG,
                 "COCG needs a complex-symmetric operator: not available with the strong periodic form");
         if (gmres_on) setup_gmres();
+        // COCG's reduced-precision preconditioner input: FP32 copies of the polynomial's values (the
+        // real part alone or the whole operator; the complex-diagonal form keeps FP64 inputs) and of
+        // r, which the update kernel writes beside r
+        pre32 = cocg_on && poly && opts->pre_fp32 != 0 && !a.pdiag;
+        a.r32 = nullptr;
+        a.pvals32 = nullptr;
+        if (pre32) {
+            const size_t nv32 = (size_t)c->nnz * (real_pre ? 1 : (per_gdir ? 4 : 2));
+            if (w.pvals32.n != nv32) w.pvals32.alloc(nv32);
+            launch_to_f32(real_pre ? w.areal.p : w.vals.p, (int64_t)nv32, w.pvals32.p, s);
+            if (w.r32.n != (size_t)2 * n * S) w.r32.alloc((size_t)2 * n * S);
+            a.r32 = reinterpret_cast<float2 *>(w.r32.p);
+            a.pvals32 = w.pvals32.p;
+        }
         // algorithmic bytes per launch of each iteration kernel (dmri_stats.kern_bytes): the operator
         // SpMVs v = A ph and t = A sh read their matrix entries and 48 B per row and signal (one vector
         // gathered, one read at the own row, one written)
@@ -1118,6 +1134,12 @@ struct Stepper {
                 kbytes[2] = E(vw) + 32.0 * nS;
                 kbytes[3] = kbytes[4] = kbytes[5] = 0.0;
                 kbytes[6] = 48.0 * nS;
+                if (pre32) {
+                    // FP32 values and r (8 B per row and signal, gathered once); z written in FP64;
+                    // the update also writes r in FP32
+                    kbytes[1] = (double)c->nnz * (4.0 + 4.0 * npre) + 4.0 * (double)(n + 1) + 24.0 * nS;
+                    kbytes[6] = 56.0 * nS;
+                }
             }
         }
         st->spmv_bytes = cocg_on ? kbytes[2] : 0.5 * (kbytes[2] + kbytes[5]);
@@ -1529,9 +1551,17 @@ struct Stepper {
     }
 
     // BiCGStab iterations on a.x with the residual already formed; restarts on breakdown.
+    // r in FP32 for the reduced-precision preconditioner (before a solve's first iteration; the
+    // update kernel keeps it current after that)
+    void round_r32() {
+        if (a.r32) launch_to_f32(reinterpret_cast<const double *>(a.r), 2 * (int64_t)n * S,
+                                 reinterpret_cast<float *>(a.r32), s);
+    }
+
     void krylov(int step) {
         SolverWork &w = *wk;
         int it = 0, it_before = 0, restarts = 0;
+        round_r32();
         constexpr int kK = 7;  // timed kernels per iteration (dmri_stats.kern_*)
         for (;;) {
             const int nl = chunk;
@@ -1611,6 +1641,7 @@ struct Stepper {
             DMRI_HIP(hipMemcpyAsync(w.mask.p, mask.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
             DMRI_HIP(hipMemsetAsync(w.all_done.p, 0, sizeof(int), s));
             launch_rhs_residual(geo, a, 1, w.mask.p, s);
+            round_r32();
             DMRI_HIP(hipStreamSynchronize(s));
         }
         int step_max = 0;
@@ -2217,6 +2248,9 @@ void validate_seq(const dmri_seq *seq, const dmri_solver_opts *opts) {
     require(opts->krylov >= 0 && opts->krylov <= 2, DMRI_ERR_ARG, "krylov must be 0 (BiCGStab), 1 (GMRES) or 2 (COCG)");
     require(opts->krylov != 2 || opts->preconditioner != 1, DMRI_ERR_ARG,
             "COCG runs with the polynomial preconditioner (preconditioner 0 or 2)");
+    require(opts->pre_fp32 == 0 || opts->pre_fp32 == 1, DMRI_ERR_ARG, "pre_fp32 must be 0 or 1");
+    require(opts->pre_fp32 == 0 || opts->krylov == 2, DMRI_ERR_ARG,
+            "pre_fp32 (FP32 preconditioner input) is a COCG option (krylov 2)");
     for (int i = 0; i < seq->nsteps; ++i)
         require(std::isfinite(seq->f_implicit[i]) && std::isfinite(seq->f_explicit[i]), DMRI_ERR_ARG,
                 "time profile has a non-finite value");

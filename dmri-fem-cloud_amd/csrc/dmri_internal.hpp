@@ -100,6 +100,9 @@ constexpr int kOpPreCD = 3;
 // plain or with the complex diagonal) with the partials of p^T z; unconjugated products, since
 // COCG works with the complex-symmetric bilinear form
 constexpr int kOpVDot = 4, kOpPreDot = 5, kOpPreCDDot = 6;
+// kOpPreDot from FP32 inputs (SolveArgs::r32, SolveArgs::pvals32), accumulated in FP64: COCG's
+// reduced-precision preconditioner (the real-part and whole-operator polynomials)
+constexpr int kOpPreDot32 = 7;
 // the preconditioner polynomial takes the real part alone when the imaginary part's bound is at
 // most this fraction of the real part's (measured: 0.002 -> same iterations, 3.3 -> twice as many)
 constexpr double kRealPreconditioner = 0.05;
@@ -322,6 +325,11 @@ struct SolveArgs {
     const double *gsig;             // [S*3] unit gradient directions (per-direction values only)
     double *partU, *partV, *partT;  // [grid][S][kNV] each
     double *totU, *totV, *totT;     // [S][kNV] finalize outputs, or null: consumers reduce the partials
+    // reduced-precision preconditioner input (COCG, kOpPreDot32), or null: r rounded to FP32 pairs
+    // (written beside r by the update kernel; launch_to_f32 converts it at a solve's start) and
+    // the preconditioner's operator values in FP32 (one per entry with areal set, else 2 or 4)
+    float2 *r32;
+    const float *pvals32;
     KrylovState *st;
     int *all_done;
     double rtol, atol;
@@ -383,6 +391,8 @@ void launch_cocg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
                    hipEvent_t e1 = nullptr);
 void launch_cocg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);
+// dst[i] = (float)src[i], i < n (round to nearest)
+void launch_to_f32(const double *src, int64_t n, float *dst, hipStream_t s);
 // alpha = rho / <r^, v> per signal and s = r - alpha v into a.s (the polynomial preconditioner's t-kernel
 // then gathers one vector instead of forming s from two)
 void launch_spass(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,

@@ -5,6 +5,7 @@
 // waves of a block in index order, then the blocks' partials in index order, summed by the
 // consuming kernel or a small finalize kernel), so results are bitwise reproducible run to run.
 #include <algorithm>
+#include <type_traits>
 
 #include <hipcub/hipcub.hpp>
 
@@ -1358,10 +1359,42 @@ __device__ __forceinline__ void acc_entry(const double *vals, int e, double2 xv,
     acc.a.y = fma(v.x, xv.y, fma(w, xv.x, acc.a.y));
 }
 
+// FP32 values (the reduced-precision preconditioner input), converted and accumulated in FP64
+template <bool PG>
+__device__ __forceinline__ void acc_entry(const float *vals, int e, double2 xv, Acc<PG> &acc) {
+    if (PG) {
+        const float4 v = *reinterpret_cast<const float4 *>(vals + 4 * (int64_t)e);
+        const double a = v.x, j0 = v.y, j1 = v.z, j2 = v.w;
+        acc.a.x += a * xv.x;
+        acc.a.y += a * xv.y;
+        acc.j[0].x += j0 * xv.x;
+        acc.j[0].y += j0 * xv.y;
+        acc.j[1].x += j1 * xv.x;
+        acc.j[1].y += j1 * xv.y;
+        acc.j[2].x += j2 * xv.x;
+        acc.j[2].y += j2 * xv.y;
+    } else {
+        const float2 v = *reinterpret_cast<const float2 *>(vals + 2 * (int64_t)e);
+        const double a = v.x, j0 = v.y;
+        acc.a.x += a * xv.x;
+        acc.a.y += a * xv.y;
+        acc.j[0].x += j0 * xv.x;
+        acc.j[0].y += j0 * xv.y;
+    }
+}
+template <bool PG>
+__device__ __forceinline__ void acc_entry(const float *vals, int e, double2 xv, FoldAcc &acc) {
+    const float4 v = *reinterpret_cast<const float4 *>(vals + 4 * (int64_t)e);
+    const double a = v.x;
+    const double w = fma(acc.cz, (double)v.w, fma(acc.cy, (double)v.z, acc.cx * (double)v.y));
+    acc.a.x = fma(a, xv.x, fma(-w, xv.y, acc.a.x));
+    acc.a.y = fma(a, xv.y, fma(w, xv.x, acc.a.y));
+}
+
 // Accumulate row entries g, g+G, ... of x = X(col) (X(c) returns the gathered value).
 // Two entries per step, so each lane keeps two independent gathers in flight.
-template <int G, bool PG, class Gather, class AccT>
-__device__ __forceinline__ void row_accumulate(const int *rowptr, const int *col, const double *vals, int64_t row,
+template <int G, bool PG, class Gather, class AccT, class V>
+__device__ __forceinline__ void row_accumulate(const int *rowptr, const int *col, const V *vals, int64_t row,
                                                int g, Gather X, AccT &acc) {
     acc_zero(acc);
     const int e1 = rowptr[row + 1];
@@ -1379,8 +1412,8 @@ __device__ __forceinline__ void row_accumulate(const int *rowptr, const int *col
 // kRowStep entries per step keep that many independent gathers in flight per lane, and the entries
 // left go in one masked step (their gathers in flight together, not one after another).
 constexpr int kRowStep = 4;
-template <int G, bool PG, class Gather, class AccT>
-__device__ __forceinline__ void lds_row_accumulate(const int *srp, const int *scol, const double *svals, int lr,
+template <int G, bool PG, class Gather, class AccT, class V>
+__device__ __forceinline__ void lds_row_accumulate(const int *srp, const int *scol, const V *svals, int lr,
                                                    int g, Gather X, AccT &acc) {
     acc_zero(acc);
     const int e1 = srp[lr + 1];
@@ -1447,8 +1480,8 @@ __device__ __forceinline__ Lane spmv_lane(int S) {
 // emit(row, acc) receives the group-reduced accumulators (acted on by lane g == 0).
 // Real part only (RO): y = sum a_e x_col(e) from the separate array a.areal, one value per entry;
 // acc.a receives it and acc.j stays zero.
-template <int G, bool PG, class Gather, class AccT>
-__device__ __forceinline__ void real_row_accumulate(const int *col, const double *av, int e0, int e1, int g, Gather X,
+template <int G, bool PG, class Gather, class AccT, class V>
+__device__ __forceinline__ void real_row_accumulate(const int *col, const V *av, int e0, int e1, int g, Gather X,
                                                     AccT &acc) {
     acc_zero(acc);
     int e = e0 + g;
@@ -1470,33 +1503,35 @@ __device__ __forceinline__ void real_row_accumulate(const int *col, const double
 #pragma unroll
         for (int k = 0; k < 3; ++k)
             if (c[k] >= 0) {
-                acc.a.x += av[e + k * G] * x[k].x;
-                acc.a.y += av[e + k * G] * x[k].y;
+                const double v = av[e + k * G];
+                acc.a.x += v * x[k].x;
+                acc.a.y += v * x[k].y;
             }
     }
 }
 
 // Copy rows t0 .. t0+nr-1 of the pattern into LDS (the caller holds a barrier before and after):
-// their ne entries' column indices (scol) and values (svals: VW per entry, or RO: one per entry
-// from a.areal), and the row offsets relative to the tile's first entry (srp).  Loads go through
-// registers in batches of kStageBatch independent loads per thread, so a thread waits on memory
-// once per batch rather than once per element.
-template <bool RO, int VW>
-__device__ __forceinline__ void stage_tile(const SolveArgs &a, int64_t t0, int nr, int E0, int ne, double *svals,
-                                           int *scol, int *srp) {
+// their ne entries' column indices (scol) and values (svals: VW per entry from vals, which is
+// a.vals, a.areal (RO: one per entry) or their FP32 copy a.pvals32), and the row offsets relative
+// to the tile's first entry (srp).  Loads go through registers in batches of kStageBatch
+// independent loads per thread, so a thread waits on memory once per batch rather than once per
+// element.
+template <bool RO, int VW, class V>
+__device__ __forceinline__ void stage_tile(const SolveArgs &a, const V *vals, int64_t t0, int nr, int E0, int ne,
+                                           V *svals, int *scol, int *srp) {
     const int nt = blockDim.x;
     const int rp = threadIdx.x <= nr ? a.rowptr[t0 + threadIdx.x] : 0;
     if (RO) {
-        const double *gv = a.areal + E0;
+        const V *gv = vals + E0;
         for (int base = 0; base < ne; base += kStageBatch * nt) {
             int cv[kStageBatch];
-            double vv[kStageBatch];
+            V vv[kStageBatch];
 #pragma unroll
             for (int u = 0; u < kStageBatch; ++u) {
                 const int i = base + u * nt + threadIdx.x;
                 const bool ok = i < ne;
                 cv[u] = ok ? a.col[E0 + i] : 0;
-                vv[u] = ok ? gv[i] : 0.0;
+                vv[u] = ok ? gv[i] : V(0);
             }
 #pragma unroll
             for (int u = 0; u < kStageBatch; ++u) {
@@ -1508,19 +1543,21 @@ __device__ __forceinline__ void stage_tile(const SolveArgs &a, int64_t t0, int n
             }
         }
     } else {
-        const double2 *gv = reinterpret_cast<const double2 *>(a.vals + (int64_t)E0 * VW);
-        double2 *lv = reinterpret_cast<double2 *>(svals);
+        // an entry's VW values move as NC chunks of 16 bytes (8 for two FP32 values)
+        using Chunk = std::conditional_t<(VW * sizeof(V)) % 16 == 0, uint4, uint2>;
+        constexpr int NC = (int)(VW * sizeof(V) / sizeof(Chunk));
+        const Chunk *gv = reinterpret_cast<const Chunk *>(vals + (int64_t)E0 * VW);
+        Chunk *lv = reinterpret_cast<Chunk *>(svals);
         for (int base = 0; base < ne; base += kStageBatch * nt) {
             int cv[kStageBatch];
-            double2 vv[kStageBatch][VW / 2 > 0 ? VW / 2 : 1];
+            Chunk vv[kStageBatch][NC];
 #pragma unroll
             for (int u = 0; u < kStageBatch; ++u) {
                 const int i = base + u * nt + threadIdx.x;
                 const bool ok = i < ne;
                 cv[u] = ok ? a.col[E0 + i] : 0;
 #pragma unroll
-                for (int h = 0; h < VW / 2; ++h)
-                    vv[u][h] = ok ? gv[(int64_t)i * (VW / 2) + h] : make_double2(0.0, 0.0);
+                for (int h = 0; h < NC; ++h) vv[u][h] = ok ? gv[(int64_t)i * NC + h] : Chunk{};
             }
 #pragma unroll
             for (int u = 0; u < kStageBatch; ++u) {
@@ -1528,7 +1565,7 @@ __device__ __forceinline__ void stage_tile(const SolveArgs &a, int64_t t0, int n
                 if (i < ne) {
                     scol[i] = cv[u];
 #pragma unroll
-                    for (int h = 0; h < VW / 2; ++h) lv[i * (VW / 2) + h] = vv[u][h];
+                    for (int h = 0; h < NC; ++h) lv[i * NC + h] = vv[u][h];
                 }
             }
         }
@@ -1537,10 +1574,19 @@ __device__ __forceinline__ void stage_tile(const SolveArgs &a, int64_t t0, int n
     for (int i = threadIdx.x + nt; i <= nr; i += nt) srp[i] = a.rowptr[t0 + i] - E0;
 }
 
-template <int G, bool PG, bool RO = false, class Gather, class Emit, class AccT = Acc<PG>>
+// The operator values a staged SpMV reads: FP64 (a.vals, or a.areal for the real part alone) or
+// their FP32 copy (a.pvals32, laid out the same way)
+template <class V, bool RO>
+__device__ __forceinline__ const V *spmv_values(const SolveArgs &a) {
+    if constexpr (std::is_same_v<V, float>) return a.pvals32;
+    else return RO ? a.areal : a.vals;
+}
+
+template <int G, bool PG, bool RO = false, class V = double, class Gather, class Emit, class AccT = Acc<PG>>
 __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR, const Lane &L, bool act, Gather X,
                                             Emit emit, AccT proto = AccT{}) {
     constexpr int VW = RO ? 1 : (PG ? 4 : 2);
+    const V *vals = spmv_values<V, RO>(a);
     if (TR <= 0) {  // direct: entries straight from global memory
         const TileWalk W = xcd_tiles(a.n, rpb);
         if (!act) return;
@@ -1549,9 +1595,9 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
             if (row >= W.end) continue;
             AccT acc = proto;
             if (RO)
-                real_row_accumulate<G, PG>(a.col, a.areal, a.rowptr[row], a.rowptr[row + 1], L.g, X, acc);
+                real_row_accumulate<G, PG>(a.col, vals, a.rowptr[row], a.rowptr[row + 1], L.g, X, acc);
             else
-                row_accumulate<G, PG>(a.rowptr, a.col, a.vals, row, L.g, X, acc);
+                row_accumulate<G, PG>(a.rowptr, a.col, vals, row, L.g, X, acc);
             group_reduce_acc<G, PG>(acc);
             emit(row, acc);
         }
@@ -1559,7 +1605,7 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
     }
     const int ECAP = TR * kStageNnzPerRow;
     extern __shared__ double4 dmri_stage_lds[];
-    double *svals = reinterpret_cast<double *>(dmri_stage_lds);
+    V *svals = reinterpret_cast<V *>(dmri_stage_lds);
     int *scol = reinterpret_cast<int *>(svals + (size_t)ECAP * VW);
     int *srp = scol + ECAP;
     const TileWalk W = xcd_tiles(a.n, TR);
@@ -1569,17 +1615,17 @@ __device__ __forceinline__ void staged_spmv(const SolveArgs &a, int rpb, int TR,
         const int ne = a.rowptr[t0 + nr] - E0;
         const bool fits = ne <= ECAP;
         __syncthreads();
-        if (fits) stage_tile<RO, VW>(a, t0, nr, E0, ne, svals, scol, srp);
+        if (fits) stage_tile<RO, VW>(a, vals, t0, nr, E0, ne, svals, scol, srp);
         __syncthreads();
         if (!act) continue;
         for (int lr = L.lrow; lr < nr; lr += rpb) {
             AccT acc = proto;
             if (RO) {
                 if (fits) real_row_accumulate<G, PG>(scol, svals, srp[lr], srp[lr + 1], L.g, X, acc);
-                else real_row_accumulate<G, PG>(a.col, a.areal, a.rowptr[t0 + lr], a.rowptr[t0 + lr + 1], L.g, X, acc);
+                else real_row_accumulate<G, PG>(a.col, vals, a.rowptr[t0 + lr], a.rowptr[t0 + lr + 1], L.g, X, acc);
             } else {
                 if (fits) lds_row_accumulate<G, PG>(srp, scol, svals, lr, L.g, X, acc);
-                else row_accumulate<G, PG>(a.rowptr, a.col, a.vals, t0 + lr, L.g, X, acc);
+                else row_accumulate<G, PG>(a.rowptr, a.col, vals, t0 + lr, L.g, X, acc);
             }
             group_reduce_acc<G, PG>(acc);
             emit(t0 + lr, acc);
@@ -1937,11 +1983,21 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
     const double2 *p = a.p;
     // kOpPreCD, with the complex diagonal d: ph = c D^-1 (2 p - c A D^-1 p), A D^-1 p gathering p / d
     constexpr bool cd = OP == kOpPreCD || OP == kOpPreCDDot;
-    constexpr bool pre = OP == kOpPre || OP == kOpPreCD || OP == kOpPreDot || OP == kOpPreCDDot;
+    // kOpPreDot32: the input as FP32 pairs (a.r32) and FP32 operator values, summed in FP64
+    constexpr bool f32 = OP == kOpPreDot32;
+    constexpr bool pre = OP == kOpPre || OP == kOpPreCD || OP == kOpPreDot || OP == kOpPreCDDot || f32;
     // COCG's operations: the unconjugated product of the input's own row with the output
-    constexpr bool pdot = OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot;
+    constexpr bool pdot = OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot || f32;
+    auto input = [&](int64_t i) {
+        if constexpr (f32) {
+            const float2 z = a.r32[i];
+            return make_double2(z.x, z.y);
+        } else {
+            return p[i];
+        }
+    };
     auto gather = [&](int c) {
-        const double2 z = p[(int64_t)c * S_ + s];
+        const double2 z = input((int64_t)c * S_ + s);
         if constexpr (cd) return c_mul(z, cdiag_inv<PG>(a.pdiag, a.coefA, c, s));
         return z;
     };
@@ -1950,7 +2006,7 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
         const int64_t idx = row * S_ + s;
         const double2 y = imag_coef_apply<G, PG>(acc, a.coefA, s);
         if (pre) {
-            const double2 z = p[idx];
+            const double2 z = input(idx);
             const double2 w = make_double2(a.pc * (2.0 * z.x - a.pc * y.x), a.pc * (2.0 * z.y - a.pc * y.y));
             double2 out = w;
             if constexpr (cd) out = c_mul(w, cdiag_inv<PG>(a.pdiag, a.coefA, row, s));
@@ -1975,13 +2031,14 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
     };
     // the preconditioner takes the real part alone when a.areal is set; with per-signal directions
     // the signal's coefficients are folded into every entry
-    if ((OP == kOpPre || OP == kOpPreDot) && a.areal)
-        staged_spmv<G, PG, true>(a, rpb, TR, L, act, gather, emit);
+    using V = std::conditional_t<f32, float, double>;
+    if ((OP == kOpPre || OP == kOpPreDot || f32) && a.areal)
+        staged_spmv<G, PG, true, V>(a, rpb, TR, L, act, gather, emit);
     else if constexpr (PG)
-        staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit, fold_acc(a.coefA, s));
+        staged_spmv<G, PG, false, V>(a, rpb, TR, L, act, gather, emit, fold_acc(a.coefA, s));
     else
-        staged_spmv<G, PG, false>(a, rpb, TR, L, act, gather, emit);
-    if (OP == kOpV || OP == kOpPreDot || OP == kOpPreCDDot)
+        staged_spmv<G, PG, false, V>(a, rpb, TR, L, act, gather, emit);
+    if (OP == kOpV || OP == kOpPreDot || OP == kOpPreCDDot || f32)
         block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
     if (OP == kOpVDot) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
 }
@@ -2227,7 +2284,7 @@ __device__ __forceinline__ void pair_spmv(const SolveArgs &a, int SP, int TR, co
         const int ne = a.rowptr[t0 + nr] - E0;
         const bool fits = ne <= ECAP;
         __syncthreads();
-        if (fits) stage_tile<RO, VW>(a, t0, nr, E0, ne, svals, scol, srp);
+        if (fits) stage_tile<RO, VW>(a, gvals, t0, nr, E0, ne, svals, scol, srp);
         __syncthreads();
         if (!act) continue;
         for (int lr = L.lrow; lr < nr; lr += rpb) {
@@ -3148,9 +3205,13 @@ __global__ void __launch_bounds__(kThreads) cocg_p_kernel(SolveArgs a, int S, in
     if ((int)threadIdx.x < S) {
         const int s = threadIdx.x;
         KrylovState &st = a.st[s];
-        // done and stop change only in other kernels, so every block of this one reads the same
-        // values and agrees on which signals still owe their pending update of x
-        int act = !st.done && !st.stop;
+        // The decision rests on done (changed only by the update kernel) and on the totals, which
+        // every block reads alike: all blocks agree on which signals stop here and still owe their
+        // pending update of x.  stop is written below by block 0 and must not be read here (a
+        // block that saw it would skip that update on its rows); it is set only by this kernel
+        // and turned into done by the update kernel of the same iteration, so at this kernel's
+        // start it is 0 for every signal not done.
+        int act = !st.done;
         const int local = it - st.it0;
         sfirst[s] = local == 0;
         sx[s] = 0;
@@ -3261,6 +3322,7 @@ __global__ void __launch_bounds__(kThreads) cocg_update_kernel(SolveArgs a, int 
              i += (int64_t)gridDim.x * vrpb * S) {
             const double2 rn = c_sub(ld_once(a.r + i), c_mul(alpha, ld_once(a.v + i)));
             st_once(a.r + i, rn);
+            if (a.r32) a.r32[i] = make_float2((float)rn.x, (float)rn.y);
             red[2] += rn.x * rn.x + rn.y * rn.y;
         }
     }
@@ -4000,6 +4062,11 @@ static size_t stage_lds_bytes_ro(int TR) {
     const size_t ecap = (size_t)TR * kStageNnzPerRow;
     return ecap * (8 + 4) + 4 * (size_t)(TR + 1) + 32;
 }
+// staged tile of FP32 values, vw per entry
+static size_t stage_lds_bytes_f32(int TR, int vw) {
+    const size_t ecap = (size_t)TR * kStageNnzPerRow;
+    return ecap * (4 * (size_t)vw + 4) + 4 * (size_t)(TR + 1) + 32;
+}
 static size_t stage_lds_bytes_bre(int TR, bool per_gdir) {
     return stage_lds_bytes(TR, per_gdir) + (size_t)TR * kStageNnzPerRow * 8;
 }
@@ -4441,12 +4508,14 @@ static void tile_lds(const SolveGeom &g, int &ucap, uint32_t &bytes) {
 template <int OP>
 static void spmv_v_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                             hipEvent_t e1) {
-    if constexpr (OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot) {
+    if constexpr (OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot || OP == kOpPreDot32) {
         if (g.mapping != kMapSL && g.mapping != kMapStaged) throw Error(-1, "COCG needs SpMV mapping 1 or 2");
         const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
-        const uint32_t lds =
-            TR > 0 ? (uint32_t)(OP == kOpPreDot && a.areal ? stage_lds_bytes_ro(TR) : stage_lds_bytes(TR, a.per_gdir))
-                   : 0u;
+        uint32_t lds = 0;
+        if (TR > 0 && OP == kOpPreDot32)
+            lds = (uint32_t)stage_lds_bytes_f32(TR, a.areal ? 1 : (a.per_gdir ? 4 : 2));
+        else if (TR > 0)
+            lds = (uint32_t)(OP == kOpPreDot && a.areal ? stage_lds_bytes_ro(TR) : stage_lds_bytes(TR, a.per_gdir));
         DMRI_DISPATCH_G(g.G, a.per_gdir,
                         hipExtLaunchKernelGGL((spmv_v_kernel<GG, PP, OP>), dim3(g.grid), dim3(g.spmv_threads), lds, s,
                                               e0, e1, 0, a, g.S, g.rpb, TR, it));
@@ -4585,6 +4654,7 @@ void launch_cocg_spmv(const SolveGeom &g, const SolveArgs &a, int it, hipStream_
                       bool pre) {
     if (pre) {
         if (a.pdiag) spmv_v_dispatch<kOpPreCDDot>(g, a, it, s, e0, e1);
+        else if (a.r32) spmv_v_dispatch<kOpPreDot32>(g, a, it, s, e0, e1);
         else spmv_v_dispatch<kOpPreDot>(g, a, it, s, e0, e1);
     } else {
         spmv_v_dispatch<kOpVDot>(g, a, it, s, e0, e1);
@@ -4615,6 +4685,17 @@ void launch_cocg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStrea
 __global__ void __launch_bounds__(256) extract_real_kernel(int64_t nnz, const double *vals, int vw, double *areal) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
         areal[e] = vals[vw * e];
+}
+__global__ void __launch_bounds__(256) to_f32_kernel(int64_t n, const double *src, float *dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+void launch_to_f32(const double *src, int64_t n, float *dst, hipStream_t s) {
+    if (n <= 0) return;
+    unsigned g = grid_for(n, 256);
+    if (g > 4096u) g = 4096u;
+    to_f32_kernel<<<g, 256, 0, s>>>(n, src, dst);
+    check_launch();
 }
 void launch_extract_real(int64_t nnz, const double *vals, int vw, double *areal, hipStream_t s) {
     unsigned g = grid_for(nnz, 256);

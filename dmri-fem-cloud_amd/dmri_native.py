@@ -50,7 +50,7 @@ class _Opts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxit", ctypes.c_int32),
                 ("profile", ctypes.c_int32), ("lanes_per_row", ctypes.c_int32), ("initial_guess", ctypes.c_int32),
                 ("mapping", ctypes.c_int32), ("preconditioner", ctypes.c_int32), ("krylov", ctypes.c_int32),
-                ("restart", ctypes.c_int32), ("graphs", ctypes.c_int32)]
+                ("restart", ctypes.c_int32), ("graphs", ctypes.c_int32), ("pre_fp32", ctypes.c_int32)]
 
 
 class _Stats(ctypes.Structure):
@@ -256,7 +256,7 @@ class Context:
     @staticmethod
     def _seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes, guess=0,
                   mapping=0, F_explicit=None, precond=0, u0=None, nv=None, F_implicit=None, krylov=0, restart=0,
-                  graphs=0):
+                  graphs=0, pre_fp32=0):
         fA = np.ascontiguousarray(f_implicit, dtype=np.float64)
         fB = np.ascontiguousarray(f_explicit, dtype=np.float64)
         if fA.shape != fB.shape:
@@ -289,6 +289,7 @@ class Context:
         o.krylov = Context.KRYLOV[krylov] if isinstance(krylov, str) else int(krylov)
         o.restart = int(restart)
         o.graphs = int(graphs)
+        o.pre_fp32 = int(pre_fp32)
         return s, o, (fA, fB, FB, U0, FA)
 
     def _nv(self, u0):
@@ -310,7 +311,7 @@ class Context:
 
     def solve(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
               profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None, snap_every=0,
-              F_implicit=None, krylov=0, restart=0, graphs=0):
+              F_implicit=None, krylov=0, restart=0, graphs=0, pre_fp32=0):
         """precond: 0 = Jacobi + degree-1 polynomial (default), 1 = Jacobi only.  guess: Krylov start
         of each step, 0 = residual-minimising projection on the last states (default), 1 = previous
         state, 2 = linear, 3 = quadratic extrapolation.  u0: initial state per mesh vertex (real part
@@ -319,7 +320,7 @@ class Context:
         (nsnap, ndof)."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, profile, lanes,
                                     guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov, restart,
-                                    graphs)
+                                    graphs, pre_fp32)
         snaps = self._snapshots(s, snap_every, 1)
         sig = np.zeros(2)
         st = _Stats()
@@ -332,7 +333,7 @@ class Context:
 
     def solve_batched(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
                       profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None,
-                      snap_every=0, F_implicit=None, krylov=0, restart=0, graphs=0):
+                      snap_every=0, F_implicit=None, krylov=0, restart=0, graphs=0, pre_fp32=0):
         """Signals for every (q[s], gdir[s]) on the shared pattern; returns complex array (nsig,).
         snap_every > 0: stats["snapshots"] has shape (nsnap, nsig, ndof), as in solve."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
@@ -340,7 +341,7 @@ class Context:
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
                                     lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov,
-                                    restart, graphs)
+                                    restart, graphs, pre_fp32)
         snaps = self._snapshots(s, snap_every, qs.shape[0])
         sig = np.zeros(2 * qs.shape[0])
         st = _Stats()
@@ -354,7 +355,7 @@ class Context:
 
     def batch(self, f_implicit, f_explicit, dt, qs, gdirs, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
               profile=False, lanes=0, guess=0, mapping=0, F_explicit=None, precond=0, u0=None, group=0,
-              F_implicit=None, krylov=0, restart=0, graphs=0):
+              F_implicit=None, krylov=0, restart=0, graphs=0, pre_fp32=0):
         """Resumable batched solve (dmri_batch_begin): returns a Batch whose advance(k) runs k more
         theta steps of every signal and whose signals() gives the signals of the current states."""
         qs = np.ascontiguousarray(np.atleast_1d(qs), dtype=np.float64)
@@ -362,7 +363,7 @@ class Context:
                                                  (qs.shape[0], 3)))
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, 0.0, (1, 0, 0), theta, rtol, atol, maxit, profile,
                                     lanes, guess, mapping, F_explicit, precond, u0, self._nv(u0), F_implicit, krylov,
-                                    restart, graphs)
+                                    restart, graphs, pre_fp32)
         h = ctypes.c_void_p()
         _check(LIB.dmri_batch_begin(self._h, qs.shape[0], _ptr(qs, ctypes.c_double), _ptr(g, ctypes.c_double),
                                     ctypes.byref(s), ctypes.byref(o), int(group), ctypes.byref(h)))
@@ -392,10 +393,11 @@ class Context:
 
     def adjoint(self, f_implicit, f_explicit, dt, q, gdir, theta=0.5, rtol=1e-13, atol=0.0, maxit=10000,
                 checkpoint_stride=0, lanes=0, guess=0, mapping=0, precond=0, u0=None, krylov=0, restart=0,
-                graphs=0):
+                graphs=0, pre_fp32=0):
         """Signal and its adjoint gradients (see dmri_adjoint): returns (signal, grads dict, stats)."""
         s, o, keep = self._seq_opts(f_implicit, f_explicit, dt, q, gdir, theta, rtol, atol, maxit, False, lanes,
-                                    guess, mapping, None, precond, u0, self._nv(u0), None, krylov, restart, graphs)
+                                    guess, mapping, None, precond, u0, self._nv(u0), None, krylov, restart, graphs,
+                                    pre_fp32)
         ao = _AdjOpts()
         ao.checkpoint_stride = int(checkpoint_stride)
         n = s.nsteps

@@ -262,6 +262,12 @@ typedef struct {
                                coefficients go to a fixed device slot and the projection guess iterates
                                in a fixed buffer so that the arguments repeat); results are identical
                                to 0 (plain launches); timed iterations (profile) go out one by one */
+    int32_t pre_fp32;       /* 1 (COCG only): the preconditioner's SpMV reads r and the polynomial's
+                               operator values rounded to FP32 and sums in FP64; z, p, q, x and r stay
+                               FP64 and the stopping test is on the FP64 residual, so the solution meets
+                               the same tolerance (a few % more iterations, half the gathered bytes in
+                               the preconditioner's product).  The complex-diagonal form keeps FP64
+                               inputs.  0 = FP64 throughout (the default). */
 } dmri_solver_opts;
 
 typedef struct {

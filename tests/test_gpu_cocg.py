@@ -2,8 +2,9 @@
 1e-9 of the oracle at rtol 1e-13 on one- and two-compartment meshes, with the real-part, whole and
 complex-diagonal polynomial, every initial guess, batched with per-signal directions (the folded
 per-direction kernels), with the weak periodic term, in the resumable batch, under hipGraph replay
-(bit for bit against plain launches), and in the adjoint (gradients against BiCGStab's).  The
-strong periodic form, whose operator is not complex symmetric, is refused."""
+(bit for bit against plain launches), and in the adjoint (gradients against BiCGStab's); repeated
+solves are bit for bit identical.  The strong periodic form, whose operator is not complex
+symmetric, is refused."""
 import numpy as np
 import pytest
 
@@ -118,6 +119,21 @@ def test_cocg_adjoint_gradients_match_bicgstab():
         assert np.abs(g_c[key] - g_b[key]).max() <= 1e-8 * np.abs(g_b[key]).max(), key
     for key in ("kappa", "q"):
         assert abs(g_c[key] - g_b[key]) <= 1e-8 * abs(g_b[key]), key
+
+
+@pytest.mark.parametrize("precond", [0, 2])
+def test_cocg_repeated_solves_are_bit_identical(precond):
+    """One signal on a small mesh: the p-kernel's blocks reduce the partial sums themselves, and every
+    block must agree on the signals that stop (and apply their pending update of x) whatever the
+    order in which the blocks run; three identical solves give identical states."""
+    print(f"[cocg] repeated solves, precond {precond}", flush=True)
+    c = _ctx("cyl2")
+    q = O.q_from_b_pgse(4000.0, DELTA, BIGDELTA)
+    runs = [c.solve(FA, FB, K, q, (0.3, 1.0, 0.2), rtol=1e-13, krylov="cocg", precond=precond, snap_every=9)
+            for _ in range(3)]
+    for s_k, st_k in runs[1:]:
+        assert s_k == runs[0][0] and st_k["iters_total"] == runs[0][1]["iters_total"]
+        assert np.array_equal(st_k["snapshots"], runs[0][1]["snapshots"])
 
 
 def test_cocg_refused_where_the_operator_is_not_complex_symmetric():
