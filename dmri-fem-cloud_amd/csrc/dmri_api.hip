@@ -1800,7 +1800,13 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     const bool proj = opts->initial_guess == 0;
     run.setup(c, 1, &seq->q, g, false, seq, opts, &st);
     const int nck = (N + K - 1) / K;
-    const int klast = keep_all ? 0 : (nck - 1) * K;  // first step of the last segment
+    // Segment j covers steps kbeg(j) .. kend(j) - 1.  The first segment takes the remainder
+    // (1 .. K steps) and every later one K steps, so the last segment, whose states the forward
+    // sweep keeps and the backward sweep does not recompute, is a full one.
+    const int k_first = N - (nck - 1) * K;
+    auto kbeg = [&](int j) { return j == 0 ? 0 : k_first + (j - 1) * K; };
+    auto kend = [&](int j) { return j == nck - 1 ? N : kbeg(j + 1); };
+    const int klast = keep_all ? 0 : kbeg(nck - 1);  // first step of the last segment
     const size_t vb = sizeof(double2) * (size_t)n;
     int ckm = 3;  // states per checkpoint
     if (!keep_all && run.fh.m > 0) {
@@ -1808,8 +1814,8 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
         DMRI_HIP(hipMemGetInfo(&fr, &tot));
         if ((double)nck * run.fh.m * (double)vb <= 0.25 * (double)fr) ckm = run.fh.m;
     }
-    // checkpoints at k = 0, K, 2K, ...: the history newest first (x^k, x^(k-1), ...); a segment
-    // holds x^k0 .. x^k1
+    // checkpoints at the segments' first steps: the history newest first (x^k, x^(k-1), ...); a
+    // segment holds x^k0 .. x^k1
     DevArray<double2> ck, seg, mu, lam, lam2;
     std::vector<int> ck_count(std::max(nck, 1), 0);
     ck.alloc((size_t)std::max(nck, 1) * ckm * n);
@@ -1820,9 +1826,9 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     auto segp = [&](int j) { return seg.p + (size_t)j * n; };
     auto ckp = [&](int j, int b) { return ck.p + ((size_t)j * ckm + b) * n; };
     // forward sweep
-    for (int k = 0; k < N; ++k) {
-        if (k % K == 0 && !keep_all) {
-            const int j = k / K;
+    for (int k = 0, jn = 0; k < N; ++k) {
+        if (!keep_all && jn < nck && k == kbeg(jn)) {
+            const int j = jn++;
             if (run.fh.m > 0) {
                 ck_count[j] = std::min(ckm, (int)run.fh.ys.size());
                 for (int b = 0; b < ck_count[j]; ++b)
@@ -1902,7 +1908,7 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     // Segment j's states x^k0 .. x^k1 into dst, recomputed by stepper r from checkpoint j; returns
     // the Krylov iterations taken.
     auto recompute = [&](Stepper &r, int j, double2 *dst) -> int64_t {
-        const int k0 = j * K, k1 = std::min(N, k0 + K);
+        const int k0 = kbeg(j), k1 = kend(j);
         const hipStream_t rs = r.s;
         if (r.fh.m > 0) {
             const double2 *states[kMaxHist];
@@ -1923,7 +1929,7 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
         return r.st->iters_total - before;
     };
     for (int j = nck - 1; j >= 0; --j) {
-        const int k0 = j * K, k1 = std::min(N, k0 + K);
+        const int k0 = kbeg(j), k1 = kend(j);
         if (!keep_all && j != nck - 1) {
             it_rec += recompute(run, j, seg.p);
             recomputed += k1 - k0;
@@ -1950,7 +1956,7 @@ void adjoint_impl(dmri_ctx *c, const dmri_seq *seq, const dmri_solver_opts *opts
     DMRI_HIP(hipMemcpyAsync(hp.data(), parts.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
     DMRI_HIP(hipStreamSynchronize(s));
     for (int j = nck - 1; j >= 0; --j) {
-        const int k0 = j * K, k1 = std::min(N, k0 + K);
+        const int k0 = kbeg(j), k1 = kend(j);
         for (int k = k1 - 1; k >= k0; --k) {
             const double *pk = hp.data() + (size_t)k * kMaxGrid * 14;
             double t[14] = {0};

@@ -376,12 +376,15 @@ int dmri_copy_bandwidth(dmri_ctx *ctx, int64_t bytes, int32_t reps, double *gbps
  * With A_n, B_n complex symmetric the discrete adjoint of the theta scheme is
  *   lambda^N = w/|Omega|,  A_n mu_n = lambda^(n+1),  lambda^n = B_n mu_n,
  *   dS/dp = Re sum_n mu_n^T (dB_n/dp u^n - dA_n/dp u^(n+1)),
- * so the backward sweep runs the forward kernels.  States are kept at checkpoints every
- * checkpoint_stride steps as (u^k, u^(k-1), u^(k-2)) and recomputed per segment (the last segment's
- * states are kept from the forward sweep itself).  With the extrapolated initial guesses the
- * recomputed states equal the forward ones bit for bit; with the projection guess (the default),
- * whose history restarts at each checkpoint, they agree to the solver tolerance.
- * checkpoint_stride >= N keeps every state instead.  0 = automatic:
+ * so the backward sweep runs the forward kernels.  The N steps form segments of checkpoint_stride
+ * steps, the first taking the remainder (so the last is a full one).  A checkpoint at each segment's
+ * first step holds the solver's state history (the projection guess's whole history, the default,
+ * or (u^k, u^(k-1), u^(k-2)) with the extrapolated guesses), so a recomputed segment repeats the
+ * forward sweep bit for bit; when the whole histories would take more than a quarter of the free
+ * device memory, a checkpoint keeps three states and the recomputed states agree with the forward
+ * ones to the solver tolerance.  Every segment but the last is recomputed; the last one's states
+ * are kept from the forward sweep itself.  checkpoint_stride >= N keeps every state instead.
+ * 0 = automatic:
  * every state when the N+1 states fit in a quarter of the free device memory (then no step is
  * recomputed), else checkpoints every ceil(sqrt(N)) steps.
  * A D_cell field must hold symmetric tensors (DMRI_ERR_ARG otherwise: the operator is then not
