@@ -95,6 +95,8 @@ KERNEL_DESC = {"bicg_p": "p = r + beta (p - omega v)", "spmv_v_pre": "ph = c (2 
                "spmv_t": "t = A sh", "bicg_update": "x += alpha ph + omega sh, r = s - omega t",
                "cocg_p": "x += alpha p (previous iteration), p = z + beta p", "spmv_pre": "z = c (2 r - c A r), r^T z",
                "spmv_q": "q = A p, p^T q", "cocg_update": "alpha = rho / p^T q, r -= alpha q"}
+# With one gradient direction the library folds COCG's update of r into the preconditioner's SpMV
+KERNEL_DESC_FOLDED = {"spmv_pre": "r -= alpha q (gathered), z = c (2 r - c A r), r^T z, |r|^2"}
 
 
 def merge_stats(stats):
@@ -116,13 +118,15 @@ def kernel_table(st, krylov="bicgstab"):
     """Per-kernel device time and algorithmic bandwidth of the event-timed Krylov iterations."""
     total = sum(st["kern_ms"]) or 1.0
     out = {}
+    folded = krylov == "cocg" and st["kern_launches"][6] == 0 and st["kern_launches"][1] > 0
     for j, name in enumerate(COCG_NAMES if krylov == "cocg" else KERNEL_NAMES):
         nl = st["kern_launches"][j]
         if nl == 0 or name is None:
             continue
         us = 1e3 * st["kern_ms"][j] / nl
         b = st["kern_bytes"][j] / nl
-        out[name] = {"what": KERNEL_DESC[name], "launches": nl, "avg_us": us, "bytes_per_launch": b,
+        what = KERNEL_DESC_FOLDED.get(name, KERNEL_DESC[name]) if folded else KERNEL_DESC[name]
+        out[name] = {"what": what, "launches": nl, "avg_us": us, "bytes_per_launch": b,
                      "achieved_gbs": b / (us * 1e-6) / 1e9, "frac_of_peak": b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
                      "share_of_iteration_time": st["kern_ms"][j] / total}
     return out
@@ -141,7 +145,7 @@ def roofline_block(st, copy_gbs, traffic, S_group, event_iters, krylov="bicgstab
     return {"bound": "hbm", "achieved": d["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": d["achieved_gbs"] / HBM_PEAK_GBS, "traffic": t["bytes"] if t else None,
             "traffic_source": t["source"] if t else None,
-            "kernel": f"{dom}: {KERNEL_DESC[dom]} (batched complex CSR, {S_group} signals per launch)",
+            "kernel": f"{dom}: {d['what']} (batched complex CSR, {S_group} signals per launch)",
             "avg_launch_us": d["avg_us"], "bytes_per_launch": d["bytes_per_launch"], "launches": d["launches"],
             "events": f"every kernel of the first {event_iters} Krylov iteration(s) of every theta step, "
                       "kernel-bound HIP events on the solver's stream",

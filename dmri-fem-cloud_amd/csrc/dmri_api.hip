@@ -850,6 +850,7 @@ struct Stepper {
     bool gmres_on = false;                   // restarted GMRES instead of BiCGStab
     bool cocg_on = false;                    // COCG instead of BiCGStab (complex-symmetric operators)
     bool pre32 = false;                      // COCG's preconditioner SpMV reads FP32 r and values
+    bool cocg_fold = false;                  // COCG's update of r folded into the preconditioner SpMV
     GmArgs gm{};
     std::vector<double> s2A, s2B;            // strong form: theta q^2 F^2 per step, implicit and explicit
     bool timed_first[kEventPool / 7] = {};   // timed iteration k of a chunk is the solve's first
@@ -1111,6 +1112,20 @@ struct Stepper {
             a.r32 = reinterpret_cast<float2 *>(w.r32.p);
             a.pvals32 = w.pvals32.p;
         }
+        // A single signal's COCG folds its update of r into the next iteration's preconditioner
+        // SpMV (three kernels and two reductions per iteration instead of four and three) when the
+        // iteration's working set (the matrix and six vectors) fits in the 256 MB MALL: its
+        // launches are then short and the second gathered vector costs little (C1 one signal at a
+        // time: 3.70 -> 4.49 signals/s; C3 forward 414 -> 356 ms).  Beyond the MALL the second
+        // gather costs more than the update kernel (C4, one signal: forward 22.7 -> 24.4 s), and
+        // with several signals about as much (C1, 5 signals: 10.33 -> 10.19 signals/s): the update
+        // kernel stays.  r alternates between a.r and a.s (the BiCGStab s buffer, unused here).
+        {
+            const int vw = per_gdir ? 4 : 2;
+            const double ws = (double)c->nnz * (4.0 + 8.0 * vw) + 4.0 * (double)(n + 1) + 96.0 * (double)n * S;
+            cocg_fold = cocg_on && S == 1 && !per_gdir && !pre32 && !a.pdiag && a.s != nullptr &&
+                        ws <= kMallBytes;
+        }
         // algorithmic bytes per launch of each iteration kernel (dmri_stats.kern_bytes): the operator
         // SpMVs v = A ph and t = A sh read their matrix entries and 48 B per row and signal (one vector
         // gathered, one read at the own row, one written)
@@ -1139,6 +1154,11 @@ struct Stepper {
                     // the update also writes r in FP32
                     kbytes[1] = (double)c->nnz * (4.0 + 4.0 * npre) + 4.0 * (double)(n + 1) + 24.0 * nS;
                     kbytes[6] = 56.0 * nS;
+                }
+                if (cocg_fold) {
+                    // r_(k-1) and q gathered, r_k and z written; no update kernel
+                    kbytes[1] = E(npre) + 64.0 * nS;
+                    kbytes[6] = 0.0;
                 }
             }
         }
@@ -1461,6 +1481,12 @@ struct Stepper {
     // The seven kernels of BiCGStab iteration it (e0(j), e1(j): the events of kernel j, or null)
     template <class E0, class E1>
     void launch_iteration(int it, E0 e0, E1 e1) {
+        if (cocg_fold) {
+            launch_cocg_pre_update(geo, a, it, s, e0(1), e1(1));   // r -= alpha q, z = M r, r^T z, |r|^2
+            launch_cocg_p(geo, a, it, s, e0(0), e1(0), true);     // p = z + beta p
+            launch_cocg_spmv(geo, a, it, s, e0(2), e1(2), false);  // q = A p (into a.v) with p^T q
+            return;
+        }
         if (cocg_on) {
             SolveArgs ar = a;  // z = M r (into a.ph) with r^T z
             ar.p = a.r;
@@ -1595,6 +1621,7 @@ struct Stepper {
                     for (int j = 0; j < kK; ++j) {
                         if (!poly && (j == 1 || j == 3 || j == 4)) continue;
                         if (cocg_on && (j == 3 || j == 4 || j == 5)) continue;
+                        if (cocg_fold && j == 6) continue;
                         float ms = 0;
                         DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[kK * k + j], c->ev1[kK * k + j]));
                         st->kern_ms[j] += ms;

@@ -69,7 +69,8 @@ struct KrylovState {
     int done, conv, breakdown, iters;
     int it0, restarts;
     int stop;  // COCG: the p-kernel stopped the signal (converged, breakdown or maxit); the update
-               // kernel of the same iteration then marks it done (see cocg_p_kernel)
+               // kernel of the same iteration (with the folded update: the next iteration's first
+               // kernel) then marks it done (see cocg_p_kernel)
     int pad2;
 };
 
@@ -103,6 +104,14 @@ constexpr int kOpVDot = 4, kOpPreDot = 5, kOpPreCDDot = 6;
 // kOpPreDot from FP32 inputs (SolveArgs::r32, SolveArgs::pvals32), accumulated in FP64: COCG's
 // reduced-precision preconditioner (the real-part and whole-operator polynomials)
 constexpr int kOpPreDot32 = 7;
+// kOpPreDot with COCG's update of r folded in (one gradient direction, mappings 1 and 2; the solver
+// takes it for single-signal solves): alpha from
+// the previous q = A p, r_k = r_(k-1) - alpha q formed per gathered column and written for the own
+// rows to the other of the two residual buffers (a.r, a.s: r_k lives in a.s for odd k), then
+// z = M r_k with partials [r^T z, |r|^2, 0]
+constexpr int kOpPreDotR = 8;
+// MI355X's last-level cache (MALL, AMD Infinity Cache) in bytes
+constexpr double kMallBytes = 256.0 * 1024 * 1024;
 // the preconditioner polynomial takes the real part alone when the imaginary part's bound is at
 // most this fraction of the real part's (measured: 0.002 -> same iterations, 3.3 -> twice as many)
 constexpr double kRealPreconditioner = 0.05;
@@ -387,8 +396,16 @@ void launch_bicg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStrea
 // a.ph holds z, a.v holds q.
 void launch_cocg_spmv(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
                       bool pre);
+// rfold: the iteration's |r|^2 comes with rho from launch_cocg_pre_update (partials V) instead of
+// from the update kernel
 void launch_cocg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
-                   hipEvent_t e1 = nullptr);
+                   hipEvent_t e1 = nullptr, bool rfold = false);
+// The folded iteration's first kernel (kOpPreDotR): update of r from the previous iteration's q,
+// z = M r, rho = r^T z and |r|^2; it also turns the previous p-kernel's stop into done and raises
+// the all-done flag.  The iteration is then launch_cocg_p(rfold) and launch_cocg_spmv(q): no update
+// kernel.
+void launch_cocg_pre_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
+                            hipEvent_t e1 = nullptr);
 void launch_cocg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);
 // dst[i] = (float)src[i], i < n (round to nearest)

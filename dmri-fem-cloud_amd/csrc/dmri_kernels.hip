@@ -1976,7 +1976,54 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P
 spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
     if (*a.all_done) return;
     const Lane L = spmv_lane<G>(S);
-    const bool act = !a.st[L.s].done;
+    // kOpPreDotR: COCG's update of r folded in.  The head turns the previous p-kernel's stop into
+    // done, forms alpha = rho_prev / p^T q from the previous q = A p (partials T, one per block of
+    // that launch, which had this grid) and raises the all-done flag when no signal is left; every
+    // block reads the same values, so all agree.  State bits per signal: 1 active, 2 r update
+    // pending (every iteration but a solve's first), 4 r_(k-1) in a.s (k - 1 odd).
+    constexpr bool fold_r = OP == kOpPreDotR;
+    __shared__ double2 fr_alpha[fold_r ? kMaxSignals : 1];
+    __shared__ int fr_state[fold_r ? kMaxSignals : 1];
+    if constexpr (fold_r) {
+        __shared__ double ftot[kMaxSignals][kNV];
+        signal_totals(a.partT, a.totT, (int)gridDim.x, S, ftot);
+        if ((int)threadIdx.x < S) {
+            const int sg = threadIdx.x;
+            KrylovState &st = a.st[sg];
+            if (st.stop && blockIdx.x == 0) st.done = 1;  // stopped by the previous p-kernel
+            int act = !st.done && !st.stop;
+            const int local = it - st.it0;
+            double2 al = make_double2(0.0, 0.0);
+            if (act && local > 0) {
+                const double2 pq = make_double2(ftot[sg][0], ftot[sg][1]);
+                if (pq.x == 0.0 && pq.y == 0.0) {
+                    if (blockIdx.x == 0) {
+                        st.done = 1;
+                        st.breakdown = 1;
+                        st.conv = 0;
+                    }
+                    act = 0;
+                } else {
+                    al = c_div(st.rho[(local - 1) & 1], pq);
+                    if (blockIdx.x == 0) st.alpha[(local - 1) & 1] = al;
+                }
+            }
+            fr_alpha[sg] = al;
+            fr_state[sg] = (act ? 1 : 0) | (act && local > 0 ? 2 : 0) | (local > 0 && ((local - 1) & 1) ? 4 : 0);
+        }
+        __syncthreads();
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            int any = 0;
+            for (int sg = 0; sg < S; ++sg) any |= fr_state[sg] & 1;
+            if (!any) *a.all_done = 1;
+        }
+    }
+    const int fst = fold_r ? fr_state[L.s] : 0;
+    const bool act = fold_r ? (fst & 1) != 0 : !a.st[L.s].done;
+    const bool rupd = (fst & 2) != 0;
+    const double2 ralpha = fold_r ? fr_alpha[L.s] : make_double2(0.0, 0.0);
+    const double2 *rin = (fst & 4) ? a.s : a.r;
+    double2 *rout = (fst & 4) ? a.r : a.s;
     const int S_ = S;
     const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
@@ -1985,13 +2032,16 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
     constexpr bool cd = OP == kOpPreCD || OP == kOpPreCDDot;
     // kOpPreDot32: the input as FP32 pairs (a.r32) and FP32 operator values, summed in FP64
     constexpr bool f32 = OP == kOpPreDot32;
-    constexpr bool pre = OP == kOpPre || OP == kOpPreCD || OP == kOpPreDot || OP == kOpPreCDDot || f32;
+    constexpr bool pre = OP == kOpPre || OP == kOpPreCD || OP == kOpPreDot || OP == kOpPreCDDot || f32 || fold_r;
     // COCG's operations: the unconjugated product of the input's own row with the output
-    constexpr bool pdot = OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot || f32;
+    constexpr bool pdot = OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot || f32 || fold_r;
     auto input = [&](int64_t i) {
         if constexpr (f32) {
             const float2 z = a.r32[i];
             return make_double2(z.x, z.y);
+        } else if constexpr (fold_r) {
+            const double2 r0 = rin[i];
+            return rupd ? c_sub(r0, c_mul(ralpha, a.v[i])) : r0;
         } else {
             return p[i];
         }
@@ -2016,6 +2066,10 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
                 red[0] += zo.x;
                 red[1] += zo.y;
             }
+            if constexpr (fold_r) {
+                if (rupd) rout[idx] = z;
+                red[2] += z.x * z.x + z.y * z.y;
+            }
             return;
         }
         a.v[idx] = y;
@@ -2032,13 +2086,13 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
     // the preconditioner takes the real part alone when a.areal is set; with per-signal directions
     // the signal's coefficients are folded into every entry
     using V = std::conditional_t<f32, float, double>;
-    if ((OP == kOpPre || OP == kOpPreDot || f32) && a.areal)
+    if ((OP == kOpPre || OP == kOpPreDot || f32 || fold_r) && a.areal)
         staged_spmv<G, PG, true, V>(a, rpb, TR, L, act, gather, emit);
     else if constexpr (PG)
         staged_spmv<G, PG, false, V>(a, rpb, TR, L, act, gather, emit, fold_acc(a.coefA, s));
     else
         staged_spmv<G, PG, false, V>(a, rpb, TR, L, act, gather, emit);
-    if (OP == kOpV || OP == kOpPreDot || OP == kOpPreCDDot || f32)
+    if (OP == kOpV || OP == kOpPreDot || OP == kOpPreCDDot || f32 || fold_r)
         block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
     if (OP == kOpVDot) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
 }
@@ -3195,7 +3249,7 @@ __global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int 
 // iteration), beta = rho / rho_prev; body: x += alpha_prev p (every iteration but a solve's first,
 // also for a signal that stops here), then p = z (first iteration) or z + beta p.
 __global__ void __launch_bounds__(kThreads) cocg_p_kernel(SolveArgs a, int S, int vrpb, int nparts_rhs,
-                                                          int nparts_upd, int nparts_spmv, int it) {
+                                                          int nparts_upd, int nparts_spmv, int it, int rfold) {
     if (*a.all_done) return;
     __shared__ double totU[kMaxSignals][kNV], totV[kMaxSignals][kNV];
     __shared__ double2 sbeta[kMaxSignals], salpha[kMaxSignals];
@@ -3216,7 +3270,10 @@ __global__ void __launch_bounds__(kThreads) cocg_p_kernel(SolveArgs a, int S, in
         sfirst[s] = local == 0;
         sx[s] = 0;
         if (act) {
-            const double rn2 = totU[s][2];
+            // |r|^2 of this iteration's r: from the update kernel (partials U; the residual kernel's
+            // at a solve's first iteration), or with the folded update from the preconditioner SpMV
+            // (partials V); |b|^2 from the residual kernel
+            const double rn2 = rfold ? totV[s][2] : totU[s][2];
             double tol2;
             if (local == 0) {
                 const double t1 = a.rtol * a.rtol * totU[s][3], t2 = a.atol * a.atol;
@@ -4508,14 +4565,15 @@ static void tile_lds(const SolveGeom &g, int &ucap, uint32_t &bytes) {
 template <int OP>
 static void spmv_v_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                             hipEvent_t e1) {
-    if constexpr (OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot || OP == kOpPreDot32) {
+    if constexpr (OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot || OP == kOpPreDot32 || OP == kOpPreDotR) {
         if (g.mapping != kMapSL && g.mapping != kMapStaged) throw Error(-1, "COCG needs SpMV mapping 1 or 2");
         const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
         uint32_t lds = 0;
         if (TR > 0 && OP == kOpPreDot32)
             lds = (uint32_t)stage_lds_bytes_f32(TR, a.areal ? 1 : (a.per_gdir ? 4 : 2));
         else if (TR > 0)
-            lds = (uint32_t)(OP == kOpPreDot && a.areal ? stage_lds_bytes_ro(TR) : stage_lds_bytes(TR, a.per_gdir));
+            lds = (uint32_t)((OP == kOpPreDot || OP == kOpPreDotR) && a.areal ? stage_lds_bytes_ro(TR)
+                                                                                : stage_lds_bytes(TR, a.per_gdir));
         DMRI_DISPATCH_G(g.G, a.per_gdir,
                         hipExtLaunchKernelGGL((spmv_v_kernel<GG, PP, OP>), dim3(g.grid), dim3(g.spmv_threads), lds, s,
                                               e0, e1, 0, a, g.S, g.rpb, TR, it));
@@ -4666,10 +4724,21 @@ void launch_cocg_spmv(const SolveGeom &g, const SolveArgs &a, int it, hipStream_
         check_launch();
     }
 }
-void launch_cocg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+void launch_cocg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                   bool rfold) {
     hipExtLaunchKernelGGL(cocg_p_kernel, dim3(g.vgrid), dim3(g.vec_threads), 0, s, e0, e1, 0, a, g.S, g.vec_rpb,
-                          g.grid, g.vgrid, g.grid, it);
+                          g.grid, g.vgrid, g.grid, it, rfold ? 1 : 0);
     check_launch();
+}
+void launch_cocg_pre_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
+                            hipEvent_t e1) {
+    if (a.per_gdir || a.pdiag || a.r32) throw Error(-1, "the folded COCG update takes one gradient direction");
+    spmv_v_dispatch<kOpPreDotR>(g, a, it, s, e0, e1);
+    check_launch();
+    if (a.totV) {
+        finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partV, g.grid, g.S, a.totV);
+        check_launch();
+    }
 }
 void launch_cocg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                         hipEvent_t e1) {

@@ -123,14 +123,17 @@ def test_cocg_adjoint_gradients_match_bicgstab():
 
 @pytest.mark.parametrize("precond", [0, 2])
 def test_cocg_repeated_solves_are_bit_identical(precond):
-    """One signal on a small mesh: the p-kernel's blocks reduce the partial sums themselves, and every
-    block must agree on the signals that stop (and apply their pending update of x) whatever the
-    order in which the blocks run; three identical solves give identical states."""
+    """One signal on a small mesh (the update of r folded into the preconditioner's SpMV; with the
+    complex diagonal, the update kernel): the p-kernel's blocks reduce the partial sums themselves,
+    and every block must agree on the signals that stop (and apply their pending update of x)
+    whatever the order in which the blocks run; three identical solves and a graph-replayed one
+    give identical states."""
     print(f"[cocg] repeated solves, precond {precond}", flush=True)
     c = _ctx("cyl2")
     q = O.q_from_b_pgse(4000.0, DELTA, BIGDELTA)
-    runs = [c.solve(FA, FB, K, q, (0.3, 1.0, 0.2), rtol=1e-13, krylov="cocg", precond=precond, snap_every=9)
-            for _ in range(3)]
+    runs = [c.solve(FA, FB, K, q, (0.3, 1.0, 0.2), rtol=1e-13, krylov="cocg", precond=precond, snap_every=9,
+                    graphs=graphs) for graphs in (0, 0, 0, 1)]
+    assert runs[3][1]["graph_launches"] > 0  # the same iterations replayed from captured graphs
     for s_k, st_k in runs[1:]:
         assert s_k == runs[0][0] and st_k["iters_total"] == runs[0][1]["iters_total"]
         assert np.array_equal(st_k["snapshots"], runs[0][1]["snapshots"])
