@@ -253,7 +253,8 @@ typedef struct {
                                r^T z, p^T A p.  All stop at ||r|| <= max(rtol ||b||, atol).
                                dmri_stats.iters_* count Arnoldi steps for GMRES; for COCG the timed
                                kernels are [0] p = z + beta p, [1] z = M r, [2] q = A p, [6] the
-                               update of x and r. */
+                               update of x and r ([6] stays 0 when a single-signal solve whose matrix
+                               and vectors fit in the 256 MB MALL folds the update of r into [1]). */
     int32_t restart;        /* GMRES restart length, 0 = 30, at most 64 (m + 1 basis vectors per signal
                                are kept in HBM) */
     int32_t graphs;         /* 1: BiCGStab and COCG launch each run of iterations between two convergence
@@ -293,8 +294,8 @@ typedef struct {
      * [0] 64 n S (32 n S in a solve's first iteration), [1] E(npre) + 32 n S, [2] E(nval) + 48 n S,
      * [3] 48 n S, [4] E(npre) + 32 n S, [5] E(nval) + 48 n S, [6] 128 n S, with E(k) = nnz (4 + 8k) +
      * 4 (n+1), nval = 2 (4 with per-signal directions) and npre = 1 for the real-part polynomial, else
-     * nval; with COCG: [0] 80 n S (32 n S in a solve's first iteration), [1] E(npre) + 32 n S,
-     * [2] E(nval) + 32 n S, [6] 48 n S). */
+     * nval; with COCG: [0] 80 n S (32 n S in a solve's first iteration), [1] E(npre) + 32 n S
+     * (E(npre) + 64 n S with the update of r folded in), [2] E(nval) + 32 n S, [6] 48 n S). */
     double kern_ms[7];
     int64_t kern_launches[7];
     double kern_bytes[7];
