@@ -97,6 +97,8 @@ KERNEL_DESC = {"bicg_p": "p = r + beta (p - omega v)", "spmv_v_pre": "ph = c (2 
                "spmv_q": "q = A p, p^T q", "cocg_update": "alpha = rho / p^T q, r -= alpha q"}
 # With one gradient direction the library folds COCG's update of r into the preconditioner's SpMV
 KERNEL_DESC_FOLDED = {"spmv_pre": "r -= alpha q (gathered), z = c (2 r - c A r), r^T z, |r|^2"}
+# With per-signal directions it folds the p-step into the q = A p SpMV
+KERNEL_DESC_QFOLDED = {"spmv_q": "x += alpha p, p = z + beta p, q = A z + beta q, p^T q"}
 
 
 def merge_stats(stats):
@@ -119,13 +121,18 @@ def kernel_table(st, krylov="bicgstab"):
     total = sum(st["kern_ms"]) or 1.0
     out = {}
     folded = krylov == "cocg" and st["kern_launches"][6] == 0 and st["kern_launches"][1] > 0
+    qfolded = krylov == "cocg" and st["kern_launches"][0] == 0 and st["kern_launches"][2] > 0
     for j, name in enumerate(COCG_NAMES if krylov == "cocg" else KERNEL_NAMES):
         nl = st["kern_launches"][j]
         if nl == 0 or name is None:
             continue
         us = 1e3 * st["kern_ms"][j] / nl
         b = st["kern_bytes"][j] / nl
-        what = KERNEL_DESC_FOLDED.get(name, KERNEL_DESC[name]) if folded else KERNEL_DESC[name]
+        what = KERNEL_DESC[name]
+        if folded:
+            what = KERNEL_DESC_FOLDED.get(name, what)
+        if qfolded:
+            what = KERNEL_DESC_QFOLDED.get(name, what)
         out[name] = {"what": what, "launches": nl, "avg_us": us, "bytes_per_launch": b,
                      "achieved_gbs": b / (us * 1e-6) / 1e9, "frac_of_peak": b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
                      "share_of_iteration_time": st["kern_ms"][j] / total}

@@ -851,6 +851,7 @@ struct Stepper {
     bool cocg_on = false;                    // COCG instead of BiCGStab (complex-symmetric operators)
     bool pre32 = false;                      // COCG's preconditioner SpMV reads FP32 r and values
     bool cocg_fold = false;                  // COCG's update of r folded into the preconditioner SpMV
+    bool cocg_qfold = false;                 // COCG's p-step folded into the q = A p SpMV
     GmArgs gm{};
     std::vector<double> s2A, s2B;            // strong form: theta q^2 F^2 per step, implicit and explicit
     bool timed_first[kEventPool / 7] = {};   // timed iteration k of a chunk is the solve's first
@@ -1126,6 +1127,10 @@ struct Stepper {
             cocg_fold = cocg_on && S == 1 && !per_gdir && !pre32 && !a.pdiag && a.s != nullptr &&
                         ws <= kMallBytes;
         }
+        // With per-signal directions the p-step (x += alpha p, p = z + beta p) runs in the q = A p
+        // SpMV's own rows, with q = A z + beta q_prev: three kernels per iteration instead of four.
+        cocg_qfold = cocg_on && per_gdir && !pre32 && !a.pdiag;
+        a.nparts_upd = geo.vgrid;
         // algorithmic bytes per launch of each iteration kernel (dmri_stats.kern_bytes): the operator
         // SpMVs v = A ph and t = A sh read their matrix entries and 48 B per row and signal (one vector
         // gathered, one read at the own row, one written)
@@ -1159,6 +1164,11 @@ struct Stepper {
                     // r_(k-1) and q gathered, r_k and z written; no update kernel
                     kbytes[1] = E(npre) + 64.0 * nS;
                     kbytes[6] = 0.0;
+                }
+                if (cocg_qfold) {
+                    // z gathered; p, q, x read and written at the own rows; no p-kernel
+                    kbytes[0] = 0.0;
+                    kbytes[2] = E(vw) + 112.0 * nS;
                 }
             }
         }
@@ -1487,6 +1497,14 @@ struct Stepper {
             launch_cocg_spmv(geo, a, it, s, e0(2), e1(2), false);  // q = A p (into a.v) with p^T q
             return;
         }
+        if (cocg_qfold) {
+            SolveArgs ar = a;  // z = M r (into a.ph) with r^T z
+            ar.p = a.r;
+            launch_cocg_spmv(geo, ar, it, s, e0(1), e1(1), true);
+            launch_cocg_qfold(geo, a, it, s, e0(2), e1(2));  // p-step and q = A p, p^T q
+            launch_cocg_update(geo, a, it, s, e0(6), e1(6));
+            return;
+        }
         if (cocg_on) {
             SolveArgs ar = a;  // z = M r (into a.ph) with r^T z
             ar.p = a.r;
@@ -1622,13 +1640,17 @@ struct Stepper {
                         if (!poly && (j == 1 || j == 3 || j == 4)) continue;
                         if (cocg_on && (j == 3 || j == 4 || j == 5)) continue;
                         if (cocg_fold && j == 6) continue;
+                        if (cocg_qfold && j == 0) continue;
                         float ms = 0;
                         DMRI_HIP(hipEventElapsedTime(&ms, c->ev0[kK * k + j], c->ev1[kK * k + j]));
                         st->kern_ms[j] += ms;
                         st->kern_launches[j] += 1;
-                        // the first iteration's p = r (COCG: p = z) reads one vector and writes one
-                        const double first_less = cocg_on ? 48.0 : 32.0;
-                        st->kern_bytes[j] += kbytes[j] - (j == 0 && timed_first[k] ? first_less * (double)n * S : 0.0);
+                        // the first iteration's p = r (COCG: p = z) reads one vector and writes one;
+                        // with the folded p-step, the q-kernel's first iteration reads no p, q or x
+                        const double first_less = cocg_qfold ? 64.0 : (cocg_on ? 48.0 : 32.0);
+                        const int first_j = cocg_qfold ? 2 : 0;
+                        st->kern_bytes[j] +=
+                            kbytes[j] - (j == first_j && timed_first[k] ? first_less * (double)n * S : 0.0);
                         if (j == 2 || j == 5) {
                             st->spmv_ms += ms;
                             st->spmv_launches += 1;

@@ -110,6 +110,11 @@ constexpr int kOpPreDot32 = 7;
 // rows to the other of the two residual buffers (a.r, a.s: r_k lives in a.s for odd k), then
 // z = M r_k with partials [r^T z, |r|^2, 0]
 constexpr int kOpPreDotR = 8;
+// COCG's q = A p with the p-step folded in (per-signal directions, mappings 1 and 2): the head is
+// cocg_p_kernel's (convergence test, beta, the pending x update), the body gathers z and forms
+// w = A z, then for its own rows x += alpha_prev p_prev, p = z + beta p_prev and q = w + beta q_prev
+// (= A p), with partials of p^T q
+constexpr int kOpQFold = 9;
 // MI355X's last-level cache (MALL, AMD Infinity Cache) in bytes
 constexpr double kMallBytes = 256.0 * 1024 * 1024;
 // the preconditioner polynomial takes the real part alone when the imaginary part's bound is at
@@ -339,6 +344,7 @@ struct SolveArgs {
     // the preconditioner's operator values in FP32 (one per entry with areal set, else 2 or 4)
     float2 *r32;
     const float *pvals32;
+    int nparts_upd;                 // blocks of the vector kernels (partials U of the update kernel)
     KrylovState *st;
     int *all_done;
     double rtol, atol;
@@ -406,6 +412,10 @@ void launch_cocg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
 // kernel.
 void launch_cocg_pre_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
                             hipEvent_t e1 = nullptr);
+// The iteration's second kernel with the p-step folded in (kOpQFold; per-signal directions): the
+// iteration is then launch_cocg_spmv(pre), launch_cocg_qfold and launch_cocg_update.
+void launch_cocg_qfold(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
+                       hipEvent_t e1 = nullptr);
 void launch_cocg_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);
 // dst[i] = (float)src[i], i < n (round to nearest)

@@ -1964,6 +1964,10 @@ __global__ void __launch_bounds__(kThreads) bicg_p_kernel(SolveArgs a, int S, in
     }
 }
 
+__device__ __forceinline__ void cocg_head(const SolveArgs &a, int S, int nparts_rhs, int nparts_upd, int nparts_spmv,
+                                          int it, int rfold, double2 *sbeta, double2 *salpha, int *sact, int *sfirst,
+                                          int *sx);
+
 // OP kOpV:  v = A p ; partials V: [<r^, v>.re, <r^, v>.im, 0, 0]
 // OP kOpPre: ph = c (2 p - c Are p) (a degree-1 polynomial preconditioner of the scaled system,
 //            whose diagonal is 1; c from a bound of its spectrum), no partials
@@ -2018,8 +2022,18 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
             if (!any) *a.all_done = 1;
         }
     }
+    // kOpQFold: cocg_p_kernel's head; a lane also runs for a signal that stops here, to apply its
+    // pending x update
+    constexpr bool qfold = OP == kOpQFold;
+    __shared__ double2 qf_beta[qfold ? kMaxSignals : 1], qf_alpha[qfold ? kMaxSignals : 1];
+    __shared__ int qf_act[qfold ? kMaxSignals : 1], qf_first[qfold ? kMaxSignals : 1], qf_x[qfold ? kMaxSignals : 1];
+    if constexpr (qfold)
+        cocg_head(a, S, (int)gridDim.x, a.nparts_upd, (int)gridDim.x, it, 0, qf_beta, qf_alpha, qf_act, qf_first,
+                  qf_x);
     const int fst = fold_r ? fr_state[L.s] : 0;
-    const bool act = fold_r ? (fst & 1) != 0 : !a.st[L.s].done;
+    const bool act = fold_r ? (fst & 1) != 0
+                   : qfold  ? (qf_act[L.s] | qf_x[L.s]) != 0
+                            : !a.st[L.s].done;
     const bool rupd = (fst & 2) != 0;
     const double2 ralpha = fold_r ? fr_alpha[L.s] : make_double2(0.0, 0.0);
     const double2 *rin = (fst & 4) ? a.s : a.r;
@@ -2027,7 +2041,7 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
     const int S_ = S;
     const int s = L.s;
     double red[kNV] = {0.0, 0.0, 0.0, 0.0};
-    const double2 *p = a.p;
+    const double2 *p = qfold ? a.ph : a.p;  // the gathered vector (z for the folded p-step)
     // kOpPreCD, with the complex diagonal d: ph = c D^-1 (2 p - c A D^-1 p), A D^-1 p gathering p / d
     constexpr bool cd = OP == kOpPreCD || OP == kOpPreCDDot;
     // kOpPreDot32: the input as FP32 pairs (a.r32) and FP32 operator values, summed in FP64
@@ -2072,6 +2086,33 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
             }
             return;
         }
+        if constexpr (qfold) {
+            const int sg = L.s;
+            const double2 z = a.ph[idx];
+            if (qf_first[sg]) {  // a solve's first iteration: p = z, q = A z
+                if (qf_act[sg]) {
+                    a.p[idx] = z;
+                    a.v[idx] = y;
+                    const double2 pq = c_mul(z, y);
+                    red[0] += pq.x;
+                    red[1] += pq.y;
+                }
+                return;
+            }
+            const double2 pp = a.p[idx];
+            if (qf_x[sg]) a.x[idx] = c_add(a.x[idx], c_mul(qf_alpha[sg], pp));
+            if (qf_act[sg]) {
+                const double2 b = qf_beta[sg];
+                const double2 pn = c_add(z, c_mul(b, pp));
+                const double2 qn = c_add(y, c_mul(b, a.v[idx]));
+                a.p[idx] = pn;
+                a.v[idx] = qn;
+                const double2 pq = c_mul(pn, qn);
+                red[0] += pq.x;
+                red[1] += pq.y;
+            }
+            return;
+        }
         a.v[idx] = y;
         if constexpr (pdot) {
             const double2 py = c_mul(p[idx], y);
@@ -2094,7 +2135,7 @@ spmv_v_kernel(SolveArgs a, int S, int rpb, int TR, int it) {
         staged_spmv<G, PG, false, V>(a, rpb, TR, L, act, gather, emit);
     if (OP == kOpV || OP == kOpPreDot || OP == kOpPreCDDot || f32 || fold_r)
         block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partV);
-    if (OP == kOpVDot) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
+    if (OP == kOpVDot || qfold) block_signal_partials(red, L.g == 0, L.lrow * S + L.s, rpb * S, S, a.partT);
 }
 
 // r = b - (Are + i cA.J) x with the stored b, r^ = r, in the SpMV kernels' staged row mapping
@@ -3248,12 +3289,13 @@ __global__ void __launch_bounds__(kThreads) bicg_update_kernel(SolveArgs a, int 
 // iteration, else from the update), rho = r^T z (partials V, from the preconditioner SpMV of this
 // iteration), beta = rho / rho_prev; body: x += alpha_prev p (every iteration but a solve's first,
 // also for a signal that stops here), then p = z (first iteration) or z + beta p.
-__global__ void __launch_bounds__(kThreads) cocg_p_kernel(SolveArgs a, int S, int vrpb, int nparts_rhs,
-                                                          int nparts_upd, int nparts_spmv, int it, int rfold) {
-    if (*a.all_done) return;
+// Head of COCG's p-step (cocg_p_kernel, or the q-kernel that folds it in): the convergence test on
+// |r|^2, beta = rho / rho_prev, and which signals still owe the previous iteration's x += alpha p;
+// per-signal results in the shared arrays, followed by a barrier.
+__device__ __forceinline__ void cocg_head(const SolveArgs &a, int S, int nparts_rhs, int nparts_upd, int nparts_spmv,
+                                          int it, int rfold, double2 *sbeta, double2 *salpha, int *sact, int *sfirst,
+                                          int *sx) {
     __shared__ double totU[kMaxSignals][kNV], totV[kMaxSignals][kNV];
-    __shared__ double2 sbeta[kMaxSignals], salpha[kMaxSignals];
-    __shared__ int sact[kMaxSignals], sfirst[kMaxSignals], sx[kMaxSignals];
     signal_totals(a.partU, a.totU, (it - a.st[0].it0) == 0 ? nparts_rhs : nparts_upd, S, totU);
     signal_totals(a.partV, a.totV, nparts_spmv, S, totV);
     if ((int)threadIdx.x < S) {
@@ -3312,6 +3354,14 @@ __global__ void __launch_bounds__(kThreads) cocg_p_kernel(SolveArgs a, int S, in
     // (the all-done flag is raised by the update kernel: raised here, it would stop this kernel's
     // later blocks before they apply the pending updates of the signals that stop now)
     __syncthreads();
+}
+
+__global__ void __launch_bounds__(kThreads) cocg_p_kernel(SolveArgs a, int S, int vrpb, int nparts_rhs,
+                                                          int nparts_upd, int nparts_spmv, int it, int rfold) {
+    if (*a.all_done) return;
+    __shared__ double2 sbeta[kMaxSignals], salpha[kMaxSignals];
+    __shared__ int sact[kMaxSignals], sfirst[kMaxSignals], sx[kMaxSignals];
+    cocg_head(a, S, nparts_rhs, nparts_upd, nparts_spmv, it, rfold, sbeta, salpha, sact, sfirst, sx);
     const int64_t total = a.n * S;
     const int s = threadIdx.x % S;
     if ((int)threadIdx.x >= vrpb * S) return;
@@ -4565,7 +4615,8 @@ static void tile_lds(const SolveGeom &g, int &ucap, uint32_t &bytes) {
 template <int OP>
 static void spmv_v_dispatch(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                             hipEvent_t e1) {
-    if constexpr (OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot || OP == kOpPreDot32 || OP == kOpPreDotR) {
+    if constexpr (OP == kOpVDot || OP == kOpPreDot || OP == kOpPreCDDot || OP == kOpPreDot32 || OP == kOpPreDotR ||
+                  OP == kOpQFold) {
         if (g.mapping != kMapSL && g.mapping != kMapStaged) throw Error(-1, "COCG needs SpMV mapping 1 or 2");
         const int TR = g.mapping == kMapStaged ? stage_rows(g, a.per_gdir) : 0;
         uint32_t lds = 0;
@@ -4729,6 +4780,15 @@ void launch_cocg_p(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s
     hipExtLaunchKernelGGL(cocg_p_kernel, dim3(g.vgrid), dim3(g.vec_threads), 0, s, e0, e1, 0, a, g.S, g.vec_rpb,
                           g.grid, g.vgrid, g.grid, it, rfold ? 1 : 0);
     check_launch();
+}
+void launch_cocg_qfold(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    if (!a.per_gdir || a.pdiag) throw Error(-1, "the folded COCG p-step takes per-signal directions");
+    spmv_v_dispatch<kOpQFold>(g, a, it, s, e0, e1);
+    check_launch();
+    if (a.totT) {
+        finalize_kernel<<<g.S, kThreads, 0, s>>>(a.partT, g.grid, g.S, a.totT);
+        check_launch();
+    }
 }
 void launch_cocg_pre_update(const SolveGeom &g, const SolveArgs &a, int it, hipStream_t s, hipEvent_t e0,
                             hipEvent_t e1) {

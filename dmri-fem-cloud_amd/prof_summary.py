@@ -175,7 +175,7 @@ def iteration_kernel(name: str, mapping: int):
     mapping only), or None.  The last template argument of the SpMV kernels is the operation: 0 v =
     A ph (or t = A s), 1 the polynomial preconditioner, 2 t = A sh, 3 the preconditioner with the
     complex diagonal; COCG's 4 q = A p (spmv_q) and 5, 6, 7, 8 z = M r (spmv_pre: FP64, complex
-    diagonal, FP32 input, with the update of r folded in)."""
+    diagonal, FP32 input, with the update of r folded in); 9 q = A p with the p-step folded in."""
     b = base(name)
     if b in ("bicg_p_kernel", "spass_kernel", "bicg_update_kernel", "cocg_p_kernel", "cocg_update_kernel"):
         return b[:-len("_kernel")]
@@ -184,7 +184,7 @@ def iteration_kernel(name: str, mapping: int):
     side = "spmv_v" if b.startswith("spmv_v") else "spmv_t"
     args = short(name).split("<")[1].rstrip(">").split(",") if "<" in short(name) else []
     op = int(args[-1]) if args and args[-1].strip().lstrip("-").isdigit() else 0
-    if op == 4:  # COCG: q = A p
+    if op in (4, 9):  # COCG: q = A p (9: with the p-step folded in)
         return "spmv_q"
     if op in (5, 6, 7, 8):  # COCG: z = M r
         return "spmv_pre"

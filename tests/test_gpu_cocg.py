@@ -107,6 +107,26 @@ def test_cocg_resumable_batch_and_graphs_bit_for_bit():
     assert done == len(FA) and np.array_equal(sb, s0)
 
 
+def test_cocg_per_direction_resumable_and_graphs_bit_for_bit():
+    """Per-signal directions (the p-step folded into the q = A p SpMV): graph replay and the
+    resumable batch give the one-call solve's states bit for bit."""
+    print("[cocg] per-direction batch: graph replay, resumable batch", flush=True)
+    c = _ctx("cyl2")
+    qs = np.array([O.q_from_b_pgse(b, DELTA, BIGDELTA) for b in (1000.0, 2000.0, 4000.0, 3000.0)])
+    gd = np.array([[1, 0, 0], [0, 1, 0], [0.2, 0.5, 1.0], [-0.4, 0.1, 0.7]], dtype=float)
+    s0, st0 = c.solve_batched(FA, FB, K, qs, gd, rtol=1e-13, krylov="cocg", snap_every=3)
+    s1, st1 = c.solve_batched(FA, FB, K, qs, gd, rtol=1e-13, krylov="cocg", snap_every=3, graphs=1)
+    assert st1["graph_launches"] > 0 and st1["iters_total"] == st0["iters_total"]
+    assert np.array_equal(st1["snapshots"], st0["snapshots"]) and np.array_equal(s1, s0)
+    with c.batch(FA, FB, K, qs, gd, rtol=1e-13, krylov="cocg") as b:
+        b.advance(4)
+        b.advance(len(FA) - 4)
+        sb, done = b.signals()
+    assert done == len(FA) and np.array_equal(sb, s0)
+    ref, _ = c.solve_batched(FA, FB, K, qs, gd, rtol=1e-13)  # BiCGStab
+    assert np.abs(s0 - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
 def test_cocg_adjoint_gradients_match_bicgstab():
     print("[cocg] adjoint", flush=True)
     c = _ctx("cyl2")
