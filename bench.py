@@ -149,9 +149,17 @@ def roofline_block(st, copy_gbs, traffic, S_group, event_iters, krylov="bicgstab
     dom = max(table, key=lambda k: table[k]["avg_us"] * table[k]["launches"])
     d = table[dom]
     t = traffic.get(dom) if traffic else None
+    # the events time each step's first iteration(s); with one, the counters' mean over the steps'
+    # first iterations is the traffic of the same launches
+    tb, tsrc = None, None
+    if t:
+        if event_iters == 1 and t.get("bytes_first"):
+            tb, tsrc = t["bytes_first"], t["source"] + ", each step's first iteration"
+        else:
+            tb, tsrc = t["bytes"], t["source"] + ", every iteration"
     return {"bound": "hbm", "achieved": d["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": d["achieved_gbs"] / HBM_PEAK_GBS, "traffic": t["bytes"] if t else None,
-            "traffic_source": t["source"] if t else None,
+            "frac": d["achieved_gbs"] / HBM_PEAK_GBS, "traffic": tb,
+            "traffic_source": tsrc,
             "kernel": f"{dom}: {d['what']} (batched complex CSR, {S_group} signals per launch)",
             "avg_launch_us": d["avg_us"], "bytes_per_launch": d["bytes_per_launch"], "launches": d["launches"],
             "events": f"every kernel of the first {event_iters} Krylov iteration(s) of every theta step, "
@@ -181,7 +189,9 @@ def measured_traffic(info, mapping_lanes, name="c1"):
             list(d.get("mapping_lanes", [])) != list(mapping_lanes or []):
         return None
     src = f"profiles/{fname} ({d.get('calibration', 'rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-kernel calibrated')})"
-    return {k: {"bytes": float(v), "source": src} for k, v in d.get("traffic_per_launch_by_kernel", {}).items()}
+    first = d.get("traffic_per_launch_first_iteration_by_kernel", {})
+    return {k: {"bytes": float(v), "bytes_first": first.get(k), "source": src}
+            for k, v in d.get("traffic_per_launch_by_kernel", {}).items()}
 
 
 # ------------------------------------------------------------------------------------------

@@ -212,15 +212,23 @@ def traffic(fetch_csv, write_csv, copy_bytes, out, meta, mapping):
         cp = [v for r, v in zip(disp, vals) if short(r["Kernel_Name"]) == "copy_kernel"]
         scale = (copy_bytes / 1024.0) / statistics.mean(cp)
         scales[which] = scale
-        acc = collections.defaultdict(list)
-        for r, v in zip(disp, vals):
+        acc, acc1 = collections.defaultdict(list), collections.defaultdict(list)
+        first = sampled(disp, 1)  # each kernel's first dispatch after a step's residual
+        for i, (r, v) in enumerate(zip(disp, vals)):
             nm = iteration_kernel(r["Kernel_Name"], mapping)
             if nm:
                 acc[nm].append(v * scale * 1024.0)
+                if i in first:
+                    acc1[nm].append(v * scale * 1024.0)
         for nm, v in acc.items():
             full = [x for x in v if x > 0.5 * max(v)]
             res.setdefault(nm, {})[which] = statistics.mean(full)
             res[nm][which + "_launches"] = len(full)
+        # first iterations of the steps that iterate (a step solved by its initial state stops at once)
+        for nm, v in acc1.items():
+            full = [x for x in v if x > 0.5 * max(v)]
+            res.setdefault(nm, {})[which + "_first"] = statistics.mean(full)
+            res[nm][which + "_first_launches"] = len(full)
     doc = dict(meta)
     doc["fetch_scale"] = scales["fetch"]
     doc["write_scale"] = scales["write"]
@@ -228,12 +236,16 @@ def traffic(fetch_csv, write_csv, copy_bytes, out, meta, mapping):
     doc["write_bytes_per_launch_by_kernel"] = {nm: v.get("write", 0.0) for nm, v in res.items()}
     doc["traffic_per_launch_by_kernel"] = {nm: v.get("fetch", 0.0) + v.get("write", 0.0) for nm, v in res.items()}
     doc["launches_by_kernel"] = {nm: v.get("fetch_launches", 0) for nm, v in res.items()}
+    doc["traffic_per_launch_first_iteration_by_kernel"] = {
+        nm: v.get("fetch_first", 0.0) + v.get("write_first", 0.0) for nm, v in res.items() if "fetch_first" in v}
+    doc["first_iteration_launches_by_kernel"] = {nm: v.get("fetch_first_launches", 0) for nm, v in res.items()}
     doc["calibration"] = (f"rocprofv3 FETCH_SIZE x {scales['fetch']:.4f} + WRITE_SIZE x {scales['write']:.4f} "
                           f"(copy kernel of {copy_bytes:.0f} B), mean over each kernel's full dispatches")
     with open(out, "w") as fh:
         json.dump(doc, fh, indent=1)
     for nm, v in sorted(doc["traffic_per_launch_by_kernel"].items()):
-        print(f"{nm:12s} {v / 1e9:8.3f} GB per launch")
+        v1 = doc["traffic_per_launch_first_iteration_by_kernel"].get(nm, 0.0)
+        print(f"{nm:12s} {v / 1e9:8.3f} GB per launch ({v1 / 1e9:.3f} GB in a step's first iteration)")
 
 
 def main():
