@@ -163,6 +163,32 @@ def test_profile_summary_samples_first_iterations_of_every_step(tmp_path):
     assert [ps.iteration_kernel(f"void dmri::{k}(dmri::SolveArgs, int)", 2) for k in
             ("cocg_p_kernel", "spmv_v_kernel<1, true, 5>", "spmv_v_kernel<1, false, 6>", "spmv_v_kernel<1, true, 4>",
              "cocg_update_kernel")] == ["cocg_p", "spmv_pre", "spmv_pre", "spmv_q", "cocg_update"]
+    # the FP32-input and folded-update preconditioner SpMVs, and the q-kernel with the p-step folded in
+    assert [ps.iteration_kernel(f"void dmri::spmv_v_kernel<1, {pg}, {op}>(dmri::SolveArgs, int)", 2) for pg, op in
+            (("true", 7), ("false", 8), ("true", 9))] == ["spmv_pre", "spmv_pre", "spmv_q"]
+
+
+def test_bench_roofline_names_folded_kernels_and_matches_first_iteration_traffic():
+    """bench.py's kernel table describes the folded COCG kernels (no p-kernel launches: the p-step is
+    in the q-kernel; no update launches: the update of r is in the preconditioner's SpMV), and the
+    roofline's traffic is the counters' first-iteration mean when the events time one iteration."""
+    import bench
+
+    def stats(launches):  # the q-kernel (slot 2) takes the most time
+        return {"kern_ms": [(2.0 if j == 2 else 1.0) * (n > 0) for j, n in enumerate(launches)],
+                "kern_launches": list(launches), "kern_bytes": [4e9 * n for n in launches]}
+
+    traffic = {"spmv_q": {"bytes": 9.0e9, "bytes_first": 5.0e9, "source": "x"}}
+    qf = stats([0, 4, 4, 0, 0, 0, 4])  # p-step folded into the q-kernel
+    t = bench.kernel_table(qf, "cocg")
+    assert "cocg_p" not in t and t["spmv_q"]["what"].startswith("x += alpha p")
+    r1 = bench.roofline_block(qf, 6000.0, traffic, 32, 1, "cocg")
+    r2 = bench.roofline_block(qf, 6000.0, traffic, 32, 2, "cocg")
+    assert r1["traffic"] == 5.0e9 and r2["traffic"] == 9.0e9
+    fr = bench.kernel_table(stats([4, 4, 4, 0, 0, 0, 0]), "cocg")  # update of r folded into z = M r
+    assert "cocg_update" not in fr and fr["spmv_pre"]["what"].startswith("r -= alpha q")
+    plain = bench.kernel_table(stats([4, 4, 4, 0, 0, 0, 4]), "cocg")
+    assert plain["spmv_q"]["what"] == bench.KERNEL_DESC["spmv_q"]
 
 
 def _layered_cylinder():
